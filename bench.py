@@ -1,0 +1,142 @@
+"""Benchmark: particle-steps/s of the explicit MPM step on one MI355X.
+
+Workload: a 10 x 10 tiling of the re-entrant polygon cell
+at h = 0.1 mm, about 1.0M deformable material points, J2 hardening
+(E = 70 GPa, nu = 0.33, sigma_y = 250 MPa, H = 700 MPa), top platen driven to
+30% strain over 5000 steps, bottom node plane fixed.  One "step" is one MPM
+time step (grid update + fused G2P/constitutive/P2G) over all points, with the
+state resident in HBM.
+
+Prints ONE JSON line.  ``value`` is the FP32-state (FP64 return map / det F)
+rate; the FP64 "plain" rate of the same scene on the same GPU is reported as
+``fp64_value`` and their ratio as ``fp32_over_fp64``.  ``roofline`` covers the
+dominant kernel (the fused particle kernel) from HIP-event timing of every
+launch in a profiled pass; ``cpu_baseline`` times the FP64 CPU oracle (a
+restatement of the reference step, 1 thread as the reference default) on a
+bounded sample of the same scene.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1M MPs (FP32 vs FP64 plain, same GPU); adjoint/forward ratio"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes (read once + write once per step; sort / arena / scratch
+# traffic excluded): per material point and per active grid node
+MP_BYTES = {("j2", 4): 232, ("j2", 8): 464, ("elastic", 4): 176, ("elastic", 8): 352}
+# the fused particle kernel owns P2G node writes (7 words) and G2P node reads (7)
+NODE_BYTES_PARTICLE_KERNEL = {4: 56, 8: 112}
+
+
+def build_scene(scale):
+    from code_mpm_auxetics_amd import scene
+    return scene.config2_world(material="j2", scale=scale, steps=5000, platen="strain")
+
+
+def run_gpu(sc, precision, steps, warmup, profile_steps):
+    import code_mpm_auxetics_amd as m
+    eng = m.PhysicsEngine(sc, precision=precision, track_strain=False)
+    if warmup:
+        eng.run_steps(warmup)
+    t0 = time.perf_counter()
+    eng.run_steps(steps)
+    t1 = time.perf_counter()
+    c = eng.checks()
+    dev_ms = eng.getTime_Runtime() * 1e3
+    eng.close()
+    res = dict(wall_s=t1 - t0, device_ms=dev_ms, checks=c)
+    if profile_steps:
+        eng = m.PhysicsEngine(sc, precision=precision, track_strain=False, profile=True)
+        eng.run_steps(warmup + steps)      # same trajectory position as the timed window
+        eng.run_steps(profile_steps)
+        res["kt"] = eng.kernel_times()
+        res["checks_prof"] = eng.checks()
+        eng.close()
+    return res
+
+
+def cpu_baseline(sc, seconds=15.0):
+    from oracle.oracle import Oracle
+    orc = Oracle(sc, nthreads=1, strain_form=0)
+    n = sc.n_deformable
+    orc.run_steps(1)                       # warm-up (page in, first grid sweep)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        orc.run_steps(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 50:
+            break
+    return dict(value=n * steps / el, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{steps} steps of the {n}-point config-2 scene, FP64 oracle "
+                       f"(restatement of the reference step), 1 thread as the reference default")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--scale", type=float, default=0.2062)
+    ap.add_argument("--fp64-steps", type=int, default=200)
+    ap.add_argument("--profile-steps", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
+                                   "(see DESIGN.md), not a measured path", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import code_mpm_auxetics_amd as m
+
+    sc = build_scene(args.scale)
+    n = sc.n_deformable
+    r32 = run_gpu(sc, m.FP32, args.steps, args.warmup, args.profile_steps)
+    r64 = run_gpu(sc, m.FP64, args.fp64_steps, args.warmup, 0)
+    v32 = n * args.steps / r32["wall_s"]
+    v64 = n * args.fp64_steps / r64["wall_s"]
+
+    kt = r32["kt"]
+    avg_ms = kt["particle_ms"] / max(kt["particle_launches"], 1)
+    avg_grid_ms = kt["grid_ms"] / max(kt["grid_launches"], 1)
+    nodes = r32["checks_prof"]["grid_nodes"]
+    bytes_launch = n * MP_BYTES[("j2", 4)] + nodes * NODE_BYTES_PARTICLE_KERNEL[4]
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_particle<float,true>", "avg_launch_ms": round(avg_ms, 5),
+            "algorithmic_bytes_per_launch": int(bytes_launch)}
+    step_bytes = n * MP_BYTES[("j2", 4)] + nodes * 108
+    out = {
+        "metric": METRIC, "value": round(v32, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "config2_polygon_mesh_10x10_j2", "n_points": n, "h_mm": 0.1,
+                   "grid_cells": list(sc.cells), "precision": "fp32 state, fp64 return map and det F",
+                   "platen_speed_m_s": round(sc.info["platen_speed"], 3), "dt_s": sc.dt_max,
+                   "parallelism": "single GPU"},
+        "fp64_value": round(v64, 1), "fp64_ms_per_step": round(r64["wall_s"] * 1e3 / args.fp64_steps, 5),
+        "fp32_over_fp64": round(v32 / v64, 3),
+        "step_roofline_frac": round(step_bytes * args.steps / r32["wall_s"] / 1e9 / HBM_PEAK_GBS, 4),
+        "grid_kernel_avg_ms": round(avg_grid_ms, 5),
+        "rebin_ms_total_profiled": round(kt["rebin_ms"], 3), "rebins_profiled": kt["rebins"],
+        "active_grid_nodes": int(nodes),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,340 @@
+// mpm_bin_kernels.hpp -- deterministic binning of points into bricks.
+//
+// A rebin runs after upload and whenever a point drifts more than half a cell
+// outside its brick.  Keys are (dense brick id << cbits) | cell-in-brick; a
+// stable LSD radix sort (8-bit digits) orders them, ties kept in input order,
+// so the permutation equals a CPU stable sort of the same keys bit for bit.
+// Stability inside a block comes from wave-level match masks (ballot over the
+// distinct digits of a wave) and a fixed wave order; no atomics decide order.
+#pragma once
+#include "mpm_device.hpp"
+
+namespace mpm {
+
+constexpr int kSortItems = 8;                    // keys per thread per tile
+constexpr int kSortTile = kBlock * kSortItems;   // 2048 keys per block
+
+// ---------------------------------------------------------------- scans
+// exclusive scan of n u32, tile of 1024 per block (256 threads x 4)
+__global__ void __launch_bounds__(256) k_scan_tiles(const uint32_t *in, uint32_t *out,
+                                                   uint32_t *tile_sums, uint32_t n)
+{
+    __shared__ uint32_t s[256];
+    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t v[4], t = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { v[k] = (base + k < n) ? in[base + k] : 0u; t += v[k]; }
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        uint32_t x = (threadIdx.x >= (unsigned)o) ? s[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - t;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) tile_sums[blockIdx.x] = s[255];
+}
+
+__global__ void __launch_bounds__(256) k_scan_add(uint32_t *out, const uint32_t *tile_off, uint32_t n)
+{
+    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    const uint32_t add = tile_off[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+// ----------------------------------------------------------- radix sort
+__global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t *keys, uint32_t n, int shift,
+                                                      uint32_t *hist, uint32_t nblocks)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (uint32_t)kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t i = base + r * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];   // digit-major
+}
+
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t *keys, const uint32_t *vals,
+                                                         uint32_t *keys_out, uint32_t *vals_out,
+                                                         uint32_t n, int shift,
+                                                         const uint32_t *offsets, uint32_t nblocks)
+{
+    __shared__ uint32_t run_off[256];
+    __shared__ uint32_t wcnt[kBlock / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    run_off[tid] = offsets[tid * nblocks + blockIdx.x];
+    for (int w = 0; w < kBlock / 64; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (uint32_t)kSortTile;
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t i = base + r * kBlock + tid;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys[i] : 0u;
+        const uint32_t val = valid ? vals[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        // match mask of lanes with the same digit
+        unsigned long long rem = __ballot(valid), mine = 0ull;
+        while (rem) {
+            const int leader = __ffsll((long long)rem) - 1;
+            const uint32_t ld = __shfl(d, leader, 64);
+            const unsigned long long m = __ballot(valid && d == ld);
+            if (valid && d == ld) mine = m;
+            rem &= ~m;
+        }
+        const uint32_t rank = __popcll(mine & lt);
+        if (valid && rank == 0) wcnt[wv][d] = __popcll(mine);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run_off[d] + rank;
+            for (int w = 0; w < wv; ++w) pos += wcnt[w][d];
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        {
+            uint32_t tot = 0;
+            for (int w = 0; w < kBlock / 64; ++w) { tot += wcnt[w][tid]; wcnt[w][tid] = 0; }
+            run_off[tid] += tot;
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- rebin
+// Key of every deformable point from its current position.  FP64 state:
+// cell = trunc(x/h) (GridPoint_Mediator.h:35).  FP32 state: the brick origin
+// of the point plus floor(x_rel/h); from_abs uses the FP64 staging positions.
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_rebin_keys(GridParams g, PArrays<real> p,
+                                                      const double *xabs0, const double *xabs1,
+                                                      const double *xabs2, uint32_t n,
+                                                      uint32_t *keys, uint32_t *vals, int32_t *flags)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int cell[3];
+    bool bad = false;
+    const double *xa[3] = {xabs0, xabs1, xabs2};
+    for (int a = 0; a < 3; ++a) {
+        int c;
+        if (xabs0) {
+            const double x = xa[a][i];
+            c = (int)(x / g.h[a]);
+            if (!(x >= 0.0)) bad = true;
+        } else if (sizeof(real) == 8) {
+            const double x = (double)p.x[a][i];
+            c = (int)(x / g.h[a]);
+            if (!(x >= 0.0)) bad = true;
+        } else {
+            const uint32_t pb = p.pbrick[i];
+            const int bc = (a == 0) ? (int)(pb % g.nb[0]) : (a == 1) ? (int)((pb / g.nb[0]) % g.nb[1])
+                                                                   : (int)(pb / (g.nb[0] * g.nb[1]));
+            const float u = (float)p.x[a][i] / g.hf[a];
+            if (!(u == u)) bad = true;
+            c = bc * g.S[a] + (int)floorf(u);
+        }
+        if (c < 0 || c >= g.cells[a]) { bad = true; c = c < 0 ? 0 : g.cells[a] - 1; }
+        cell[a] = c;
+    }
+    if (bad) atomicOr(flags, FLAG_OUT_OF_GRID);
+    const uint32_t b = (uint32_t)(cell[0] / g.S[0]) +
+                       (uint32_t)g.nb[0] * ((uint32_t)(cell[1] / g.S[1]) + (uint32_t)g.nb[1] * (uint32_t)(cell[2] / g.S[2]));
+    const uint32_t lc = (uint32_t)(cell[0] % g.S[0]) +
+                        (uint32_t)g.S[0] * ((uint32_t)(cell[1] % g.S[1]) + (uint32_t)g.S[1] * (uint32_t)(cell[2] % g.S[2]));
+    keys[i] = (b << g.cbits) | lc;
+    vals[i] = i;
+}
+
+struct GatherDesc {
+    const void *src;
+    void *dst;
+    int esize;
+};
+
+// gather every state array through the permutation (one array per blockIdx.y)
+__global__ void __launch_bounds__(kBlock) k_gather(const GatherDesc *d, const uint32_t *perm, uint32_t n)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const GatherDesc gd = d[blockIdx.y];
+    const uint32_t i = perm[j];
+    if (gd.esize == 8) ((uint64_t *)gd.dst)[j] = ((const uint64_t *)gd.src)[i];
+    else if (gd.esize == 4) ((uint32_t *)gd.dst)[j] = ((const uint32_t *)gd.src)[i];
+    else ((uint8_t *)gd.dst)[j] = ((const uint8_t *)gd.src)[i];
+}
+
+// positions into the new order; FP32 re-bases to the new brick origin
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_rebin_pos(GridParams g, PArrays<real> src, PArrays<real> dst,
+                                                     const double *xabs0, const double *xabs1,
+                                                     const double *xabs2, const uint32_t *perm,
+                                                     const uint32_t *skeys, uint32_t n)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t i = perm[j];
+    const uint32_t nbk = skeys[j] >> g.cbits;
+    dst.pbrick[j] = nbk;
+    const int nbc[3] = {(int)(nbk % g.nb[0]), (int)((nbk / g.nb[0]) % g.nb[1]), (int)(nbk / (g.nb[0] * g.nb[1]))};
+    const double *xa[3] = {xabs0, xabs1, xabs2};
+    for (int a = 0; a < 3; ++a) {
+        if (sizeof(real) == 8) {
+            dst.x[a][j] = xabs0 ? (real)xa[a][i] : src.x[a][i];
+        } else {
+            double xabs;
+            if (xabs0) {
+                xabs = xa[a][i];
+            } else {
+                const uint32_t pb = src.pbrick[i];
+                const int bc = (a == 0) ? (int)(pb % g.nb[0]) : (a == 1) ? (int)((pb / g.nb[0]) % g.nb[1])
+                                                                       : (int)(pb / (g.nb[0] * g.nb[1]));
+                xabs = (double)(bc * g.S[a]) * g.h[a] + (double)src.x[a][i];
+            }
+            dst.x[a][j] = (real)(xabs - (double)(nbc[a] * g.S[a]) * g.h[a]);
+        }
+    }
+}
+
+__global__ void k_heads(const uint32_t *skeys, uint32_t n, int cbits, uint32_t *head)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    head[j] = (j == 0 || (skeys[j] >> cbits) != (skeys[j - 1] >> cbits)) ? 1u : 0u;
+}
+
+__global__ void k_occ(const uint32_t *skeys, const uint32_t *head, const uint32_t *pos, uint32_t n,
+                      int cbits, uint32_t *occ_list, uint32_t *occ_start)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || !head[j]) return;
+    occ_list[pos[j]] = skeys[j] >> cbits;
+    occ_start[pos[j]] = j;
+}
+
+__global__ void k_occ_count(const uint32_t *occ_list, const uint32_t *occ_start, uint32_t n_occ,
+                            uint32_t n, uint32_t *occ_count, int32_t *brick_slot)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_occ) return;
+    occ_count[s] = ((s + 1 < n_occ) ? occ_start[s + 1] : n) - occ_start[s];
+    brick_slot[occ_list[s]] = (int32_t)s;
+}
+
+__global__ void k_dilate(GridParams g, const uint32_t *occ_list, uint32_t n_occ, uint32_t *flag)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_occ) return;
+    const uint32_t b = occ_list[s];
+    const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1], bz = b / (g.nb[0] * g.nb[1]);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = bx + dx, y = by + dy, z = bz + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= g.nb[0] || y >= g.nb[1] || z >= g.nb[2]) continue;
+                flag[x + g.nb[0] * (y + g.nb[1] * z)] = 1u;
+            }
+}
+
+__global__ void k_compact(const uint32_t *flag, const uint32_t *pos, uint32_t n, uint32_t *list)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || !flag[j]) return;
+    list[pos[j]] = j;
+}
+
+// initial platen override marks for the first step after upload
+__global__ void k_dc_marks(GridParams g, const double *dc_x, int n_dc, uint32_t epoch,
+                           unsigned long long *marks)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_dc) return;
+    int i0[3];
+    for (int q = 0; q < 3; ++q) i0[q] = (int)(dc_x[3 * j + q] / g.h[q]);
+    const unsigned long long tag = ((unsigned long long)(epoch + 1u) << 32) | (unsigned)j;
+    for (int k = 0; k < 8; ++k) {
+        const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
+        if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= g.nodes[0] || n1 >= g.nodes[1] || n2 >= g.nodes[2]) continue;
+        atomicMax(&marks[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
+    }
+}
+
+// ------------------------------------------------------------ readback
+struct ReadbackOut {
+    double *x, *v, *F, *s, *eps, *ep, *alpha, *vol;
+};
+
+template <typename real>
+__global__ void __launch_bounds__(kBlock) k_readback(GridParams g, PArrays<real> p, uint32_t n,
+                                                    ReadbackOut o, uint32_t mask)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t id = p.id[j];
+    if (mask & 1u) {
+        for (int a = 0; a < 3; ++a) {
+            double xa = (double)p.x[a][j];
+            if (sizeof(real) == 4) {
+                const uint32_t pb = p.pbrick[j];
+                const int bc = (a == 0) ? (int)(pb % g.nb[0]) : (a == 1) ? (int)((pb / g.nb[0]) % g.nb[1])
+                                                                       : (int)(pb / (g.nb[0] * g.nb[1]));
+                xa += (double)(bc * g.S[a]) * g.h[a];
+            }
+            o.x[3 * (size_t)id + a] = xa;
+        }
+    }
+    if (mask & 2u) for (int a = 0; a < 3; ++a) o.v[3 * (size_t)id + a] = (double)p.v[a][j];
+    double Fd[9];
+    for (int a = 0; a < 9; ++a) Fd[a] = (double)p.F[a][j];
+    if (mask & 4u) for (int a = 0; a < 9; ++a) o.F[9 * (size_t)id + a] = Fd[a];
+    if (mask & 8u) for (int a = 0; a < 6; ++a) o.s[6 * (size_t)id + a] = (double)p.s[a][j];
+    if ((mask & 16u) && g.track_strain) for (int a = 0; a < 6; ++a) o.eps[6 * (size_t)id + a] = (double)p.eps[a][j];
+    if (mask & 32u) for (int a = 0; a < 6; ++a) o.ep[6 * (size_t)id + a] = (double)p.ep[a][j];
+    if (mask & 64u) o.alpha[id] = (double)p.alpha[j];
+    if (mask & 128u) o.vol[id] = det3(Fd) * (double)p.vol0[j];
+}
+
+// FP64 sums over the per-block partials, fixed order (mpm_checks)
+__global__ void k_finalize(const double *kp_part, const int32_t *n_occ, const double *k2_part,
+                           const int32_t *n_k2, const double *dc_v, const double *dc_m, int n_dc,
+                           double *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int no = *n_occ, nk = *n_k2;
+    for (int s = 0; s < nk; ++s) {
+        r[0] += k2_part[3 * s];
+        r[1] += k2_part[3 * s + 1];
+        r[7] += k2_part[3 * s + 2];
+    }
+    for (int s = 0; s < no; ++s) {
+        r[2] += kp_part[5 * s];
+        r[3] += kp_part[5 * s + 1];
+        r[4] += kp_part[5 * s + 2];
+        r[5] += kp_part[5 * s + 3];
+        r[6] += kp_part[5 * s + 4];
+    }
+    for (int j = 0; j < n_dc; ++j) {
+        const double *v = &dc_v[3 * j];
+        const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        r[2] += dc_m[j] * (lv * lv);
+        r[3] += dc_m[j] * v[0];
+        r[4] += dc_m[j] * v[1];
+        r[5] += dc_m[j] * v[2];
+    }
+    for (int k = 0; k < 8; ++k) out[k] = r[k];
+}
+
+}  // namespace mpm

@@ -1,0 +1,117 @@
+// mpm_device.hpp -- shared device-side types and helpers for the MI355X MPM step.
+//
+// Layout in HBM (one handle):
+//   deformable points: structure-of-arrays, one array per scalar field, kept in
+//     brick-binned order (bricks of S = (Sx,Sy,Sz) grid cells); FP64 mode stores
+//     absolute positions, FP32 mode stores positions relative to the origin of
+//     the point's brick (keeps FP32 resolution at ~1e-7 of a brick, not of the
+//     whole grid).
+//   displacement-controlled points (platens): FP64 SoA, unsorted.
+//   grid nodes: dense lattice (cells+1 per axis); per node an 8-word record
+//     {v.xyz, a.x | a.y, a.z, -, -} written by the grid kernel, read by G2P.
+//   brick arenas: per occupied brick, 7 node fields (m, p.xyz, f.xyz) on the
+//     (S+3)^3 node window the brick's points can reach; written by the fused
+//     particle kernel (P2G), merged in a fixed order by the grid kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpm {
+
+constexpr int kBlock = 256;          // threads per particle workgroup
+constexpr int kQ = 7;                // node fields: m, p.xyz, f.xyz
+constexpr uint32_t kNoBrick = 0xFFFFFFFFu;
+
+enum : int {
+    FLAG_REBIN = 1,        // a point drifted past half a cell outside its brick
+    FLAG_ARENA = 2,        // a point left its brick's arena window (fatal)
+    FLAG_OUT_OF_GRID = 4,  // a point left the grid box (fatal)
+    FLAG_NAN = 8           // non-finite state (fatal)
+};
+
+struct GridParams {
+    double L[3];
+    double h[3];
+    float hf[3];
+    int cells[3];
+    int nodes[3];
+    int S[3];      // brick size in cells
+    int nb[3];     // brick lattice size
+    int A[3];      // arena window size in nodes = S + 3
+    int arenaN;    // A[0]*A[1]*A[2]
+    int cbits;     // bits for the cell-in-brick index in sort keys
+    double m_min, damping;
+    int strain_form, neohookean, track_strain, has_fext, has_pointE;
+};
+
+// Per-step record filled on the host (time, dt, platen velocity, epoch).
+struct StepRec {
+    double dt;
+    double t;
+    double vtl[3];
+    int32_t has_tl;
+    uint32_t epoch;
+};
+
+// Material table entry (reference per-point fields E, nu, sigma_y, C0, C1 and
+// type; MaterialPoint.h:28-56), deduplicated on upload.
+struct Material {
+    int32_t type;
+    int32_t pad;
+    double E, nu, sy, C0, C1;
+};
+
+template <typename real>
+struct PArrays {
+    real *x[3], *v[3], *F[9], *s[6], *ep[6], *eps[6];
+    real *alpha, *mass, *vol0, *E, *fext[3];
+    uint8_t *mat;
+    uint32_t *id, *pbrick;
+};
+
+template <typename real>
+struct NodeRec {
+    real d[8];
+};
+
+// ---------------------------------------------------------------------------
+// DPP helpers (row = 16 lanes).  bound_ctrl=true: lanes without a source read 0.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    int2 p = *reinterpret_cast<int2 *>(&v);
+    p.x = dpp_i<CTRL>(p.x);
+    p.y = dpp_i<CTRL>(p.y);
+    return *reinterpret_cast<double *>(&p);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_r(float v) { return dpp_f<CTRL>(v); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_r(double v) { return dpp_d<CTRL>(v); }
+
+// row_shr:d = 0x110+d, row_shl:d = 0x100+d
+#define MPM_ROW_SHR(d) (0x110 + (d))
+#define MPM_ROW_SHL(d) (0x100 + (d))
+
+__device__ __forceinline__ double sgn(double a) { return (double)((0.0 < a) - (a < 0.0)); }
+__device__ __forceinline__ float sgn(float a) { return (float)((0.0f < a) - (a < 0.0f)); }
+
+// glm determinant of a column-major 3x3 (…_MP.cpp:301), always in FP64.
+__device__ __forceinline__ double det3(const double m[9])
+{
+    return +m[0] * (m[4] * m[8] - m[7] * m[5])
+           - m[3] * (m[1] * m[8] - m[7] * m[2])
+           + m[6] * (m[1] * m[5] - m[4] * m[2]);
+}
+
+}  // namespace mpm

@@ -1,0 +1,1051 @@
+// mpm_engine.hip -- host orchestration and C ABI of the MI355X MPM step.
+//
+// One handle = one device, one HIP stream, calls serialised by the caller.
+// Step pipeline per time step n (2 kernels):
+//   k_grid(n):      merge brick arenas(n) -> node momentum, damping, BC,
+//                   reaction, platen override (…_MP.cpp:209-265)
+//   k_particle(n):  G2P(n) + F/strain/constitutive (…_MP.cpp:271-354), P2G(n+1)
+//                   into arenas(n+1) (…_MP.cpp:106-202), platen motion and
+//                   override marks for n+1 (…_MP.cpp:360-368)
+// The host checks drift flags every cfg.drift_check steps and rebins when a
+// point moved past half a cell outside its brick.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mpm_auxetics.h"
+#include "mpm_bin_kernels.hpp"
+#include "mpm_step_kernels.hpp"
+
+using namespace mpm;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct mpm_handle {
+    mpm_config cfg{};
+    std::vector<double> tl_t, tl_v;
+    std::vector<uint8_t> fixed_host;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    GridParams g{};
+    bool fp32 = false;
+    size_t rsize = 8;
+    long long nnodes = 0;
+    int nbricks = 0;
+
+    // points
+    int n_total = 0, n_def = 0, n_dc = 0;
+    bool uploaded = false;
+    std::vector<Buf> pfields[2];   // per buffer: every field allocation
+    PArrays<float> pf[2]{};
+    PArrays<double> pd[2]{};
+    int cur = 0;
+    double *xstage[3] = {nullptr, nullptr, nullptr};
+    double *dc_x = nullptr, *dc_v = nullptr, *dc_m = nullptr;
+    std::vector<int32_t> dc_id_host, def_id_host;
+    mpm_points dc_saved{};                  // host copy of static platen fields
+    std::vector<double> dc_saved_store;
+
+    // grid
+    void *nodes = nullptr;
+    uint8_t *fixed = nullptr;
+    unsigned long long *marks = nullptr;
+
+    // bricks
+    uint32_t *occ_list = nullptr, *occ_start = nullptr, *occ_count = nullptr;
+    int32_t *n_occ_d = nullptr, *n_k2_d = nullptr, *brick_slot = nullptr;
+    uint32_t *k2_list = nullptr, *k2flag = nullptr, *k2pos = nullptr;
+    int n_occ = 0, n_k2 = 0, occ_cap = 0, k2_cap = 0;
+    void *arena[2] = {nullptr, nullptr};
+    size_t arena_cap_slots = 0;
+    int parity = 0;
+
+    Material *mats = nullptr;
+    int n_mats = 0;
+    int has_pointE = 0, has_fext = 0;
+
+    StepRec *steps_d = nullptr, *steps_h = nullptr;
+    int steps_cap = 0;
+    int32_t *flags_d = nullptr, *flags_h = nullptr;
+    double *kp_part = nullptr, *k2_part = nullptr;
+    size_t kp_cap = 0, k2p_cap = 0;
+    double *hist_d = nullptr, *hist_h = nullptr;
+    std::vector<double> hist;
+    double *checks_d = nullptr;
+
+    // rebin scratch
+    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+    uint32_t *radix_hist = nullptr, *radix_off = nullptr;
+    std::vector<uint32_t *> scan_scratch;
+    uint32_t *heads = nullptr, *headpos = nullptr;
+    GatherDesc *gdesc = nullptr;
+    size_t gdesc_cap = 0;
+
+    double time = 0.0;
+    long long cycle = 0;
+    double console_last = -1.0e12;
+    long long rebins = 0;
+    double last_ms = 0.0;
+    // profiling (cfg.profile): event pairs around every launch
+    std::vector<hipEvent_t> pev;
+    double kt[6] = {0, 0, 0, 0, 0, 0};
+};
+
+#define HIPCHK(h, call)                                                                     \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);                   \
+            return MPM_E_HIP;                                                               \
+        }                                                                                   \
+    } while (0)
+
+#define RCHK(call)                \
+    do {                          \
+        int r_ = (call);          \
+        if (r_ != MPM_OK) return r_; \
+    } while (0)
+
+static int dev_alloc(mpm_handle *h, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        h->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return MPM_E_NOMEM;
+    }
+    return MPM_OK;
+}
+
+static void dev_free(void *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T>
+static int dalloc(mpm_handle *h, T **p, size_t count)
+{
+    return dev_alloc(h, (void **)p, count * sizeof(T));
+}
+
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ scans
+static int scan_u32(mpm_handle *h, const uint32_t *in, uint32_t *out, uint32_t n, int level,
+                    uint32_t *total)
+{
+    if (n == 0) { if (total) *total = 0; return MPM_OK; }
+    const unsigned tiles = cdiv(n, 1024);
+    while ((int)h->scan_scratch.size() < 2 * (level + 1)) {
+        uint32_t *p = nullptr;
+        size_t cap = (size_t)cdiv(std::max<size_t>((size_t)h->n_def, (size_t)h->nbricks) + 1024, 1024) + 64;
+        RCHK(dalloc(h, &p, cap));
+        h->scan_scratch.push_back(p);
+    }
+    uint32_t *tsum = h->scan_scratch[2 * level], *toff = h->scan_scratch[2 * level + 1];
+    k_scan_tiles<<<tiles, 256, 0, h->stream>>>(in, out, tsum, n);
+    HIPCHK(h, hipGetLastError());
+    if (tiles > 1) {
+        RCHK(scan_u32(h, tsum, toff, tiles, level + 1, nullptr));
+        k_scan_add<<<tiles, 256, 0, h->stream>>>(out, toff, n);
+        HIPCHK(h, hipGetLastError());
+    }
+    if (total) {
+        uint32_t a = 0, b = 0;
+        HIPCHK(h, hipMemcpyAsync(&a, out + n - 1, 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(&b, in + n - 1, 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        *total = a + b;
+    }
+    return MPM_OK;
+}
+
+// stable LSD radix sort of (keys[0], vals[0]) -> result left in keys[r], vals[r]
+static int radix_sort(mpm_handle *h, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
+                      uint32_t n, uint32_t max_key, uint32_t **kout, uint32_t **vout)
+{
+    int bits = 0;
+    while (bits < 32 && (max_key >> bits) != 0) ++bits;
+    int passes = std::max(1, (bits + 7) / 8);
+    const unsigned nb = std::max(1u, cdiv(n, kSortTile));
+    uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
+    for (int ps = 0; ps < passes; ++ps) {
+        const int shift = 8 * ps;
+        k_radix_hist<<<nb, kBlock, 0, h->stream>>>(ka, n, shift, h->radix_hist, nb);
+        HIPCHK(h, hipGetLastError());
+        RCHK(scan_u32(h, h->radix_hist, h->radix_off, 256u * nb, 0, nullptr));
+        k_radix_scatter<<<nb, kBlock, 0, h->stream>>>(ka, va, kb, vb, n, shift, h->radix_off, nb);
+        HIPCHK(h, hipGetLastError());
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    *kout = ka;
+    *vout = va;
+    return MPM_OK;
+}
+
+static int alloc_sort_scratch(mpm_handle *h, uint32_t n)
+{
+    for (int b = 0; b < 2; ++b) {
+        RCHK(dalloc(h, &h->keys[b], (size_t)n + 1));
+        RCHK(dalloc(h, &h->vals[b], (size_t)n + 1));
+    }
+    const size_t nb = std::max(1u, cdiv(n, kSortTile));
+    RCHK(dalloc(h, &h->radix_hist, 256 * nb));
+    RCHK(dalloc(h, &h->radix_off, 256 * nb));
+    return MPM_OK;
+}
+
+// ---------------------------------------------------------------- fields
+template <typename real>
+static PArrays<real> &parr(mpm_handle *h, int b);
+template <>
+PArrays<float> &parr<float>(mpm_handle *h, int b) { return h->pf[b]; }
+template <>
+PArrays<double> &parr<double>(mpm_handle *h, int b) { return h->pd[b]; }
+
+// every particle array with its element size, in a fixed order
+template <typename real>
+static void field_list(mpm_handle *h, PArrays<real> &p, std::vector<std::pair<void **, int>> &out)
+{
+    const int rs = (int)sizeof(real);
+    for (int a = 0; a < 3; ++a) out.push_back({(void **)&p.v[a], rs});
+    for (int a = 0; a < 9; ++a) out.push_back({(void **)&p.F[a], rs});
+    for (int a = 0; a < 6; ++a) out.push_back({(void **)&p.s[a], rs});
+    for (int a = 0; a < 6; ++a) out.push_back({(void **)&p.ep[a], rs});
+    if (h->cfg.track_strain)
+        for (int a = 0; a < 6; ++a) out.push_back({(void **)&p.eps[a], rs});
+    out.push_back({(void **)&p.alpha, rs});
+    out.push_back({(void **)&p.mass, rs});
+    out.push_back({(void **)&p.vol0, rs});
+    if (h->has_pointE) out.push_back({(void **)&p.E, rs});
+    if (h->has_fext)
+        for (int a = 0; a < 3; ++a) out.push_back({(void **)&p.fext[a], rs});
+    out.push_back({(void **)&p.mat, 1});
+    out.push_back({(void **)&p.id, 4});
+}
+
+template <typename real>
+static int alloc_points(mpm_handle *h, int n)
+{
+    for (int b = 0; b < 2; ++b) {
+        PArrays<real> &p = parr<real>(h, b);
+        std::memset(&p, 0, sizeof(p));
+        std::vector<std::pair<void **, int>> fl;
+        field_list<real>(h, p, fl);
+        for (int a = 0; a < 3; ++a) fl.push_back({(void **)&p.x[a], (int)sizeof(real)});
+        fl.push_back({(void **)&p.pbrick, 4});
+        for (auto &f : fl) {
+            RCHK(dev_alloc(h, f.first, (size_t)std::max(n, 1) * f.second));
+            h->pfields[b].push_back(Buf{*f.first, (size_t)n * f.second});
+        }
+    }
+    return MPM_OK;
+}
+
+static int ensure_slots(mpm_handle *h, int n_occ, int n_k2)
+{
+    const int want_occ = std::max(1, n_occ + n_occ / 4 + 16);
+    const int want_k2 = std::max(1, n_k2 + n_k2 / 4 + 16);
+    if ((size_t)n_occ > h->arena_cap_slots || h->arena[0] == nullptr) {
+        for (int b = 0; b < 2; ++b) dev_free(h->arena[b]);
+        dev_free((void *&)h->kp_part);
+        h->arena_cap_slots = (size_t)want_occ;
+        for (int b = 0; b < 2; ++b)
+            RCHK(dev_alloc(h, &h->arena[b], h->arena_cap_slots * kQ * h->g.arenaN * h->rsize));
+        RCHK(dalloc(h, &h->kp_part, h->arena_cap_slots * 5));
+        HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * 8, h->stream));
+        h->occ_cap = want_occ;
+    }
+    if ((size_t)n_k2 > h->k2p_cap || h->k2_part == nullptr) {
+        dev_free((void *&)h->k2_part);
+        h->k2p_cap = (size_t)want_k2;
+        RCHK(dalloc(h, &h->k2_part, h->k2p_cap * 3));
+        HIPCHK(h, hipMemsetAsync(h->k2_part, 0, h->k2p_cap * 3 * 8, h->stream));
+        h->k2_cap = want_k2;
+    }
+    return MPM_OK;
+}
+
+template <typename real>
+static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index);
+
+// profiling: events recorded around launches, summed after each sync
+struct ProfMark {
+    int kind;   // 0 particle, 1 grid, 2 rebin
+    int ev;     // index of the first event of the pair
+};
+
+static int prof_begin(mpm_handle *h, std::vector<ProfMark> &marks, int kind)
+{
+    if (!h->cfg.profile) return -1;
+    const int idx = (int)marks.size() * 2;
+    while ((int)h->pev.size() < idx + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        h->pev.push_back(e);
+    }
+    (void)hipEventRecord(h->pev[idx], h->stream);
+    marks.push_back(ProfMark{kind, idx});
+    return idx;
+}
+
+static void prof_end(mpm_handle *h, int idx)
+{
+    if (idx >= 0) (void)hipEventRecord(h->pev[idx + 1], h->stream);
+}
+
+static void prof_collect(mpm_handle *h, std::vector<ProfMark> &marks)
+{
+    for (const ProfMark &m : marks) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->pev[m.ev], h->pev[m.ev + 1]) == hipSuccess) {
+            h->kt[2 * m.kind] += ms;
+            h->kt[2 * m.kind + 1] += 1.0;
+        }
+    }
+    marks.clear();
+}
+
+// ------------------------------------------------------------------ rebin
+template <typename real>
+static int rebin_t(mpm_handle *h, bool from_abs)
+{
+    const uint32_t n = (uint32_t)h->n_def;
+    if (n == 0) {
+        h->n_occ = h->n_k2 = 0;
+        RCHK(ensure_slots(h, 0, 0));
+        HIPCHK(h, hipMemsetAsync(h->n_occ_d, 0, 4, h->stream));
+        HIPCHK(h, hipMemsetAsync(h->n_k2_d, 0, 4, h->stream));
+        return MPM_OK;
+    }
+    PArrays<real> &src = parr<real>(h, h->cur), &dst = parr<real>(h, 1 - h->cur);
+    const unsigned nbk = cdiv(n, kBlock);
+    k_rebin_keys<real><<<nbk, kBlock, 0, h->stream>>>(h->g, src, from_abs ? h->xstage[0] : nullptr,
+                                                      from_abs ? h->xstage[1] : nullptr,
+                                                      from_abs ? h->xstage[2] : nullptr, n, h->keys[0],
+                                                      h->vals[0], h->flags_d);
+    HIPCHK(h, hipGetLastError());
+    const uint32_t max_key = ((uint32_t)(h->nbricks - 1) << h->g.cbits) | ((1u << h->g.cbits) - 1u);
+    uint32_t *sk, *perm;
+    RCHK(radix_sort(h, h->keys[0], h->vals[0], h->keys[1], h->vals[1], n, max_key, &sk, &perm));
+    // gather all fields
+    std::vector<std::pair<void **, int>> fs, fd;
+    field_list<real>(h, src, fs);
+    field_list<real>(h, dst, fd);
+    std::vector<GatherDesc> desc;
+    for (size_t k = 0; k < fs.size(); ++k) desc.push_back(GatherDesc{*fs[k].first, *fd[k].first, fs[k].second});
+    if (desc.size() > h->gdesc_cap) {
+        dev_free((void *&)h->gdesc);
+        h->gdesc_cap = desc.size() + 8;
+        RCHK(dalloc(h, &h->gdesc, h->gdesc_cap));
+    }
+    HIPCHK(h, hipMemcpyAsync(h->gdesc, desc.data(), desc.size() * sizeof(GatherDesc), hipMemcpyHostToDevice, h->stream));
+    k_gather<<<dim3(nbk, (unsigned)desc.size()), kBlock, 0, h->stream>>>(h->gdesc, perm, n);
+    HIPCHK(h, hipGetLastError());
+    k_rebin_pos<real><<<nbk, kBlock, 0, h->stream>>>(h->g, src, dst, from_abs ? h->xstage[0] : nullptr,
+                                                     from_abs ? h->xstage[1] : nullptr,
+                                                     from_abs ? h->xstage[2] : nullptr, perm, sk, n);
+    HIPCHK(h, hipGetLastError());
+    h->cur = 1 - h->cur;
+    // brick tables
+    k_heads<<<nbk, kBlock, 0, h->stream>>>(sk, n, h->g.cbits, h->heads);
+    HIPCHK(h, hipGetLastError());
+    uint32_t n_occ = 0;
+    RCHK(scan_u32(h, h->heads, h->headpos, n, 0, &n_occ));
+    RCHK(ensure_slots(h, (int)n_occ, h->nbricks));
+    k_occ<<<nbk, kBlock, 0, h->stream>>>(sk, h->heads, h->headpos, n, h->g.cbits, h->occ_list, h->occ_start);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemsetAsync(h->brick_slot, 0xFF, (size_t)h->nbricks * 4, h->stream));
+    k_occ_count<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->occ_list, h->occ_start, n_occ, n,
+                                                              h->occ_count, h->brick_slot);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemsetAsync(h->k2flag, 0, (size_t)h->nbricks * 4, h->stream));
+    k_dilate<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->g, h->occ_list, n_occ, h->k2flag);
+    HIPCHK(h, hipGetLastError());
+    uint32_t n_k2 = 0;
+    RCHK(scan_u32(h, h->k2flag, h->k2pos, (uint32_t)h->nbricks, 0, &n_k2));
+    k_compact<<<cdiv(h->nbricks, kBlock), kBlock, 0, h->stream>>>(h->k2flag, h->k2pos, (uint32_t)h->nbricks, h->k2_list);
+    HIPCHK(h, hipGetLastError());
+    RCHK(ensure_slots(h, (int)n_occ, (int)n_k2));
+    h->n_occ = (int)n_occ;
+    h->n_k2 = (int)n_k2;
+    int32_t cnt[2] = {(int32_t)n_occ, (int32_t)n_k2};
+    HIPCHK(h, hipMemcpyAsync(h->n_occ_d, &cnt[0], 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->n_k2_d, &cnt[1], 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    // rebuild the arena the next grid update reads
+    RCHK(launch_particle<real>(h, false, 0, 0));
+    h->rebins++;
+    return MPM_OK;
+}
+
+static int rebin(mpm_handle *h, bool from_abs)
+{
+    return h->fp32 ? rebin_t<float>(h, from_abs) : rebin_t<double>(h, from_abs);
+}
+
+// ---------------------------------------------------------------- launches
+template <typename real>
+static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index)
+{
+    if (h->n_def == 0 && !(g2p && h->n_dc > 0)) return MPM_OK;
+    KpArgs<real> a{};
+    a.g = h->g;
+    a.p = parr<real>(h, h->cur);
+    a.nodes = (const NodeRec<real> *)h->nodes;
+    // G2P(n)+P2G(n+1) writes the arena of step n+1; a rebuild writes step n's
+    a.arena_out = (real *)h->arena[g2p ? (h->parity ^ 1) : h->parity];
+    a.occ_list = h->occ_list;
+    a.occ_start = h->occ_start;
+    a.occ_count = h->occ_count;
+    a.n_occ = h->n_occ_d;
+    a.occ_cap = h->occ_cap;
+    a.mats = h->mats;
+    a.steps = h->steps_d;
+    a.flags = h->flags_d;
+    a.kp_part = h->kp_part;
+    a.n_dc = h->n_dc;
+    a.dc_x = h->dc_x;
+    a.dc_v = h->dc_v;
+    a.marks = h->marks;
+    a.k2_part = h->k2_part;
+    a.n_k2 = h->n_k2_d;
+    a.hist = h->hist_d;
+    a.hist_index = hist_index;
+    const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long);
+    unsigned grid = (unsigned)h->occ_cap;
+    if (g2p) grid += cdiv(h->n_dc, kBlock);
+    if (grid == 0) return MPM_OK;
+    if (g2p)
+        k_particle<real, true><<<grid, kBlock, lds, h->stream>>>(a, step);
+    else
+        k_particle<real, false><<<grid, kBlock, lds, h->stream>>>(a, step);
+    HIPCHK(h, hipGetLastError());
+    return MPM_OK;
+}
+
+template <typename real>
+static int launch_grid(mpm_handle *h, int step)
+{
+    if (h->n_k2 == 0) return MPM_OK;
+    K2Args<real> a{};
+    a.g = h->g;
+    a.arena_in = (const real *)h->arena[h->parity];
+    a.brick_slot = h->brick_slot;
+    a.k2_list = h->k2_list;
+    a.n_k2 = h->n_k2_d;
+    a.k2_cap = h->k2_cap;
+    a.nodes = (NodeRec<real> *)h->nodes;
+    a.fixed = h->fixed;
+    a.marks = h->marks;
+    a.dc_v = h->dc_v;
+    a.steps = h->steps_d;
+    a.k2_part = h->k2_part;
+    k_grid<real><<<(unsigned)h->k2_cap, 64, 0, h->stream>>>(a, step);
+    HIPCHK(h, hipGetLastError());
+    return MPM_OK;
+}
+
+// TimeLine::getVelocity (TimeLine.h:33-46), evaluated on the host per step
+static void timeline_velocity(const mpm_handle *h, double t, double out[3])
+{
+    out[0] = out[1] = out[2] = 0.0;
+    const int n = (int)h->tl_t.size();
+    for (int i = 0; i < n - 1; ++i) {
+        if (h->tl_t[i] <= t && t < h->tl_t[i + 1]) {
+            const double sc = (t - h->tl_t[i]) / (h->tl_t[i + 1] - h->tl_t[i]);
+            for (int a = 0; a < 3; ++a) out[a] = (1.0 - sc) * h->tl_v[3 * i + a] + sc * h->tl_v[3 * (i + 1) + a];
+        }
+    }
+}
+
+static int check_flags(mpm_handle *h, bool *need_rebin)
+{
+    HIPCHK(h, hipMemcpyAsync(h->flags_h, h->flags_d, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int f = *h->flags_h;
+    *need_rebin = (f & FLAG_REBIN) != 0;
+    if (f & (FLAG_OUT_OF_GRID | FLAG_ARENA | FLAG_NAN)) {
+        if (f & FLAG_NAN) { h->err = "non-finite point state"; return MPM_E_NAN; }
+        if (f & FLAG_OUT_OF_GRID) { h->err = "a material point left the grid box"; return MPM_E_OUT_OF_GRID; }
+        h->err = "a point moved more than one cell beyond its brick between drift checks "
+                 "(lower mpm_config.drift_check)";
+        return MPM_E_ARENA;
+    }
+    if (*need_rebin) HIPCHK(h, hipMemsetAsync(h->flags_d, 0, 4, h->stream));
+    return MPM_OK;
+}
+
+static int checks_impl(mpm_handle *h, double out[12]);
+
+static void report_console(mpm_handle *h)
+{
+    double c[12];
+    if (checks_impl(h, c) != MPM_OK) return;
+    std::printf("\ttime: %+.6e\tRuntime: %+.3e\tForce_y: %+.6e\tKinetic Energy: %+.6e\n", h->time,
+                h->last_ms * 1e-3, c[1], c[2]);
+}
+
+template <typename real>
+static int step_impl(mpm_handle *h, double dt_total, int32_t *finished)
+{
+    h->hist.clear();
+    for (double &v : h->kt) v = 0.0;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    double acc = 0.0;
+    int in_batch = 0;
+    auto flush = [&](bool force_check) -> int {
+        if (in_batch == 0) return MPM_OK;
+        // launch the batch: steps records already in steps_h[0..in_batch)
+        HIPCHK(h, hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
+        std::vector<ProfMark> marks;
+        for (int k = 0; k < in_batch; ++k) {
+            int pe = prof_begin(h, marks, 1);
+            RCHK(launch_grid<real>(h, k));
+            prof_end(h, pe);
+            pe = prof_begin(h, marks, 0);
+            RCHK(launch_particle<real>(h, true, k, k));
+            prof_end(h, pe);
+            h->parity ^= 1;
+        }
+        HIPCHK(h, hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * in_batch, hipMemcpyDeviceToHost, h->stream));
+        bool need = false;
+        RCHK(check_flags(h, &need));   // synchronises
+        for (int k = 0; k < in_batch; ++k) h->hist.push_back(h->n_def ? h->hist_h[k] : 0.0);
+        if (h->cfg.profile) prof_collect(h, marks);
+        in_batch = 0;
+        if (need) {
+            int pe = prof_begin(h, marks, 2);
+            RCHK(rebin(h, false));
+            prof_end(h, pe);
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            if (h->cfg.profile) prof_collect(h, marks);
+        }
+        (void)force_check;
+        return MPM_OK;
+    };
+    const int K = std::max(1, std::min(h->cfg.drift_check > 0 ? h->cfg.drift_check : 8, h->steps_cap));
+    while (acc < dt_total) {
+        double dt = h->cfg.dt_max;
+        if (h->cfg.dt_max > (dt_total - acc)) dt = dt_total - acc;
+        acc += dt;
+        StepRec &r = h->steps_h[in_batch];
+        r.dt = dt;
+        r.t = h->time;
+        r.has_tl = h->tl_t.empty() ? 0 : 1;
+        timeline_velocity(h, h->time, r.vtl);
+        r.epoch = (uint32_t)h->cycle;
+        ++in_batch;
+        bool report = h->cfg.console_interval > 0.0 && (h->time - h->console_last > h->cfg.console_interval);
+        if (in_batch == K || report) RCHK(flush(true));
+        if (report) {
+            h->console_last = h->time;
+            if (h->cfg.verbose) report_console(h);
+        }
+        h->time += dt;
+        h->cycle++;
+    }
+    RCHK(flush(true));
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    if (finished) *finished = (h->time < h->cfg.time_end) ? 0 : 1;
+    if (finished && *finished && h->cfg.verbose) {
+        h->console_last = h->time;
+        report_console(h);
+        std::printf("Analysis complete ...\n");
+    }
+    return MPM_OK;
+}
+
+static int checks_impl(mpm_handle *h, double out[12])
+{
+    k_finalize<<<1, 64, 0, h->stream>>>(h->kp_part, h->n_occ_d, h->k2_part, h->n_k2_d, h->dc_v, h->dc_m,
+                                       h->n_dc, h->checks_d);
+    HIPCHK(h, hipGetLastError());
+    double r[8] = {0};
+    HIPCHK(h, hipMemcpyAsync(r, h->checks_d, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 7; ++k) out[k] = r[k];
+    out[7] = h->time;
+    out[8] = (double)h->cycle;
+    out[9] = (double)h->rebins;
+    out[10] = (double)h->n_occ;
+    out[11] = r[7];
+    return MPM_OK;
+}
+
+// ================================================================== C ABI
+extern "C" {
+
+int mpm_abi_version(void) { return MPM_ABI_VERSION; }
+
+const char *mpm_last_error(const mpm_handle *h)
+{
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+static void destroy_all(mpm_handle *h)
+{
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int b = 0; b < 2; ++b) {
+        for (auto &f : h->pfields[b]) dev_free(f.p);
+        h->pfields[b].clear();
+        dev_free(h->arena[b]);
+        dev_free((void *&)h->keys[b]);
+        dev_free((void *&)h->vals[b]);
+    }
+    for (int a = 0; a < 3; ++a) dev_free((void *&)h->xstage[a]);
+    void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v, (void **)&h->dc_m, &h->nodes, (void **)&h->fixed,
+                     (void **)&h->marks, (void **)&h->occ_list, (void **)&h->occ_start, (void **)&h->occ_count,
+                     (void **)&h->n_occ_d, (void **)&h->n_k2_d, (void **)&h->brick_slot, (void **)&h->k2_list,
+                     (void **)&h->k2flag, (void **)&h->k2pos, (void **)&h->mats, (void **)&h->steps_d,
+                     (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->k2_part, (void **)&h->hist_d,
+                     (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
+                     (void **)&h->headpos, (void **)&h->gdesc};
+    for (void **p : ptrs) dev_free(*p);
+    for (auto *p : h->scan_scratch) { void *q = p; dev_free(q); }
+    h->scan_scratch.clear();
+    if (h->steps_h) (void)hipHostFree(h->steps_h);
+    if (h->flags_h) (void)hipHostFree(h->flags_h);
+    if (h->hist_h) (void)hipHostFree(h->hist_h);
+    h->steps_h = nullptr;
+    h->flags_h = nullptr;
+    h->hist_h = nullptr;
+    for (hipEvent_t e : h->pev) (void)hipEventDestroy(e);
+    h->pev.clear();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+void mpm_destroy(mpm_handle *h)
+{
+    if (!h) return;
+    destroy_all(h);
+    delete h;
+}
+
+int mpm_create(const mpm_config *cfg, mpm_handle **out)
+{
+    if (!cfg || !out) { g_create_error = "null argument"; return MPM_E_ARG; }
+    *out = nullptr;
+    for (int a = 0; a < 3; ++a)
+        if (!(cfg->length[a] > 0.0) || cfg->cells[a] < 1) { g_create_error = "grid length and cells must be positive"; return MPM_E_ARG; }
+    if (!(cfg->dt_max > 0.0)) { g_create_error = "dt_max must be positive"; return MPM_E_ARG; }
+    if (cfg->precision != MPM_FP64 && cfg->precision != MPM_FP32) { g_create_error = "unknown precision"; return MPM_E_ARG; }
+    if (cfg->n_timeline < 0 || (cfg->n_timeline > 0 && (!cfg->timeline_t || !cfg->timeline_v))) {
+        g_create_error = "bad timeline";
+        return MPM_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return MPM_E_NO_DEVICE; }
+    if (cfg->device < 0 || cfg->device >= ndev) { g_create_error = "bad device ordinal"; return MPM_E_ARG; }
+    mpm_handle *h = new mpm_handle();
+    h->cfg = *cfg;
+    h->device = cfg->device;
+    h->fp32 = cfg->precision == MPM_FP32;
+    h->rsize = h->fp32 ? 4 : 8;
+    h->tl_t.assign(cfg->timeline_t, cfg->timeline_t + cfg->n_timeline);
+    h->tl_v.assign(cfg->timeline_v, cfg->timeline_v + 3 * cfg->n_timeline);
+    h->cfg.timeline_t = nullptr;
+    h->cfg.timeline_v = nullptr;
+    h->cfg.node_fixed = nullptr;
+    h->time = cfg->time_start;
+    h->console_last = -1.0e12;
+    auto fail = [&](int code) {
+        g_create_error = h->err;
+        destroy_all(h);
+        delete h;
+        return code;
+    };
+    if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return fail(MPM_E_HIP); }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { h->err = "stream"; return fail(MPM_E_HIP); }
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) { h->err = "event"; return fail(MPM_E_HIP); }
+
+    GridParams &g = h->g;
+    for (int a = 0; a < 3; ++a) {
+        g.L[a] = cfg->length[a];
+        g.cells[a] = cfg->cells[a];
+        g.nodes[a] = cfg->cells[a] + 1;
+        g.h[a] = cfg->length[a] / (double)cfg->cells[a];
+        g.hf[a] = (float)g.h[a];
+        int S = cfg->brick[a] > 0 ? cfg->brick[a] : (a == 0 ? 8 : a == 1 ? 4 : 2);
+        S = std::min(S, std::max(1, cfg->cells[a]));
+        g.S[a] = S;
+        g.nb[a] = g.nodes[a] / S + ((g.nodes[a] % S) ? 1 : 0);
+        g.A[a] = S + 3;
+    }
+    g.arenaN = g.A[0] * g.A[1] * g.A[2];
+    if ((size_t)kQ * g.arenaN * 8 > 64 * 1024) { h->err = "brick too large for the LDS arena"; return fail(MPM_E_ARG); }
+    const int ncb = g.S[0] * g.S[1] * g.S[2];
+    g.cbits = 0;
+    while ((1 << g.cbits) < ncb) ++g.cbits;
+    h->nbricks = g.nb[0] * g.nb[1] * g.nb[2];
+    if (((unsigned long long)h->nbricks << g.cbits) >= (1ull << 32)) { h->err = "grid too large for 32-bit brick keys"; return fail(MPM_E_ARG); }
+    g.m_min = cfg->mass_min;
+    g.damping = cfg->damping;
+    g.strain_form = h->fp32 ? 1 : cfg->strain_form;
+    g.neohookean = cfg->neohookean;
+    g.track_strain = cfg->track_strain;
+    h->nnodes = (long long)g.nodes[0] * g.nodes[1] * g.nodes[2];
+
+    int rc;
+    if ((rc = dev_alloc(h, &h->nodes, (size_t)h->nnodes * 8 * h->rsize))) return fail(rc);
+    if ((rc = dalloc(h, &h->fixed, (size_t)h->nnodes))) return fail(rc);
+    if ((rc = dalloc(h, &h->marks, (size_t)h->nnodes))) return fail(rc);
+    (void)hipMemsetAsync(h->nodes, 0, (size_t)h->nnodes * 8 * h->rsize, h->stream);
+    (void)hipMemsetAsync(h->marks, 0, (size_t)h->nnodes * 8, h->stream);
+    if (cfg->node_fixed) {
+        if (hipMemcpyAsync(h->fixed, cfg->node_fixed, (size_t)h->nnodes, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+            h->err = "copy node_fixed";
+            return fail(MPM_E_HIP);
+        }
+    } else {
+        (void)hipMemsetAsync(h->fixed, 0, (size_t)h->nnodes, h->stream);
+    }
+    const size_t nb = (size_t)h->nbricks;
+    if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
+        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->n_k2_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) ||
+        (rc = dalloc(h, &h->k2_list, nb)) || (rc = dalloc(h, &h->k2flag, nb)) || (rc = dalloc(h, &h->k2pos, nb)) ||
+        (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)))
+        return fail(rc);
+    (void)hipMemsetAsync(h->n_occ_d, 0, 4, h->stream);
+    (void)hipMemsetAsync(h->n_k2_d, 0, 4, h->stream);
+    (void)hipMemsetAsync(h->flags_d, 0, 4, h->stream);
+    h->steps_cap = 64;
+    if ((rc = dalloc(h, &h->steps_d, h->steps_cap)) || (rc = dalloc(h, &h->hist_d, h->steps_cap))) return fail(rc);
+    if (hipHostMalloc((void **)&h->steps_h, sizeof(StepRec) * h->steps_cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&h->flags_h, 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&h->hist_h, sizeof(double) * h->steps_cap, hipHostMallocDefault) != hipSuccess) {
+        h->err = "hipHostMalloc";
+        return fail(MPM_E_NOMEM);
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) { h->err = "init sync"; return fail(MPM_E_HIP); }
+    *out = h;
+    return MPM_OK;
+}
+
+}  // extern "C"
+
+template <typename real>
+static int upload_t(mpm_handle *h, const mpm_points *pts)
+{
+    const int n = pts->n;
+    std::vector<int> def, dc;
+    for (int i = 0; i < n; ++i) ((pts->dc && pts->dc[i]) ? dc : def).push_back(i);
+    h->n_total = n;
+    h->n_def = (int)def.size();
+    h->n_dc = (int)dc.size();
+    h->def_id_host.assign(def.begin(), def.end());
+    h->dc_id_host.assign(dc.begin(), dc.end());
+    // materials: dedupe (type, nu, sy, C0, C1[, E])
+    auto mkey = [&](int i, bool withE) {
+        return std::make_tuple(pts->type ? pts->type[i] : 0, pts->nu ? pts->nu[i] : 0.0, pts->sy ? pts->sy[i] : 0.0,
+                               pts->C0 ? pts->C0[i] : 0.0, pts->C1 ? pts->C1[i] : 0.0,
+                               withE && pts->E ? pts->E[i] : 0.0);
+    };
+    std::map<std::tuple<int, double, double, double, double, double>, int> mm;
+    bool withE = true;
+    for (int i : def) { mm.emplace(mkey(i, true), (int)mm.size()); if (mm.size() > 255) break; }
+    if (mm.size() > 255) {
+        withE = false;
+        mm.clear();
+        for (int i : def) { mm.emplace(mkey(i, false), (int)mm.size()); if (mm.size() > 255) break; }
+        if (mm.size() > 255) { h->err = "more than 255 distinct materials"; return MPM_E_ARG; }
+    }
+    h->has_pointE = withE ? 0 : 1;
+    h->has_fext = 0;
+    if (pts->fext)
+        for (int i : def) if (pts->fext[3 * i] != 0.0 || pts->fext[3 * i + 1] != 0.0 || pts->fext[3 * i + 2] != 0.0) { h->has_fext = 1; break; }
+    h->g.has_pointE = h->has_pointE;
+    h->g.has_fext = h->has_fext;
+    std::vector<Material> mats(std::max<size_t>(mm.size(), 1));
+    for (auto &kv : mm) {
+        Material m{};
+        m.type = std::get<0>(kv.first);
+        m.nu = std::get<1>(kv.first);
+        m.sy = std::get<2>(kv.first);
+        m.C0 = std::get<3>(kv.first);
+        m.C1 = std::get<4>(kv.first);
+        m.E = std::get<5>(kv.first);
+        mats[kv.second] = m;
+    }
+    RCHK(dalloc(h, &h->mats, mats.size()));
+    HIPCHK(h, hipMemcpyAsync(h->mats, mats.data(), mats.size() * sizeof(Material), hipMemcpyHostToDevice, h->stream));
+    h->n_mats = (int)mats.size();
+
+    RCHK(alloc_points<real>(h, h->n_def));
+    RCHK(alloc_sort_scratch(h, (uint32_t)std::max(h->n_def, 1)));
+    RCHK(dalloc(h, &h->heads, (size_t)std::max(h->n_def, 1)));
+    RCHK(dalloc(h, &h->headpos, (size_t)std::max(h->n_def, 1)));
+    for (int a = 0; a < 3; ++a) RCHK(dalloc(h, &h->xstage[a], (size_t)std::max(h->n_def, 1)));
+
+    // host staging in input order, converted to the device precision
+    const int nd = h->n_def;
+    PArrays<real> &P = parr<real>(h, h->cur);
+    std::vector<real> tmp(std::max(nd, 1));
+    std::vector<double> tmpd(std::max(nd, 1));
+    auto put = [&](real *dst, auto getter) -> int {
+        for (int k = 0; k < nd; ++k) tmp[k] = (real)getter(def[k]);
+        HIPCHK(h, hipMemcpy(dst, tmp.data(), sizeof(real) * nd, hipMemcpyHostToDevice));
+        return MPM_OK;
+    };
+    for (int a = 0; a < 3; ++a) {
+        for (int k = 0; k < nd; ++k) tmpd[k] = pts->x[3 * def[k] + a];
+        HIPCHK(h, hipMemcpy(h->xstage[a], tmpd.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+        RCHK(put(P.v[a], [&](int i) { return pts->v ? pts->v[3 * i + a] : 0.0; }));
+        if (h->has_fext) RCHK(put(P.fext[a], [&](int i) { return pts->fext[3 * i + a]; }));
+    }
+    for (int a = 0; a < 9; ++a)
+        RCHK(put(P.F[a], [&](int i) { return pts->F ? pts->F[9 * i + a] : ((a % 4) == 0 ? 1.0 : 0.0); }));
+    for (int a = 0; a < 6; ++a) {
+        RCHK(put(P.s[a], [&](int i) { return pts->stress ? pts->stress[6 * i + a] : 0.0; }));
+        RCHK(put(P.ep[a], [&](int i) { return pts->eps_p ? pts->eps_p[6 * i + a] : 0.0; }));
+        if (h->cfg.track_strain) RCHK(put(P.eps[a], [&](int i) { return pts->strain ? pts->strain[6 * i + a] : 0.0; }));
+    }
+    RCHK(put(P.alpha, [&](int i) { return pts->alpha ? pts->alpha[i] : 0.0; }));
+    RCHK(put(P.mass, [&](int i) { return pts->mass[i]; }));
+    RCHK(put(P.vol0, [&](int i) { return pts->vol0 ? pts->vol0[i] : pts->vol[i]; }));
+    if (h->has_pointE) RCHK(put(P.E, [&](int i) { return pts->E[i]; }));
+    {
+        std::vector<uint8_t> mt(std::max(nd, 1));
+        std::vector<uint32_t> id(std::max(nd, 1));
+        for (int k = 0; k < nd; ++k) {
+            mt[k] = (uint8_t)mm.at(mkey(def[k], withE));
+            id[k] = (uint32_t)def[k];
+        }
+        HIPCHK(h, hipMemcpy(P.mat, mt.data(), nd, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(P.id, id.data(), 4 * (size_t)nd, hipMemcpyHostToDevice));
+    }
+    // platens
+    RCHK(dalloc(h, &h->dc_x, 3 * (size_t)std::max(h->n_dc, 1)));
+    RCHK(dalloc(h, &h->dc_v, 3 * (size_t)std::max(h->n_dc, 1)));
+    RCHK(dalloc(h, &h->dc_m, (size_t)std::max(h->n_dc, 1)));
+    {
+        std::vector<double> x(3 * h->n_dc + 1), v(3 * h->n_dc + 1), m(h->n_dc + 1);
+        for (int j = 0; j < h->n_dc; ++j) {
+            const int i = dc[j];
+            for (int a = 0; a < 3; ++a) {
+                x[3 * j + a] = pts->x[3 * i + a];
+                v[3 * j + a] = pts->v ? pts->v[3 * i + a] : 0.0;
+            }
+            m[j] = pts->mass[i];
+        }
+        HIPCHK(h, hipMemcpy(h->dc_x, x.data(), sizeof(double) * 3 * h->n_dc, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dc_v, v.data(), sizeof(double) * 3 * h->n_dc, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dc_m, m.data(), sizeof(double) * h->n_dc, hipMemcpyHostToDevice));
+        // static platen fields for readback (F, stress, ... never change)
+        h->dc_saved_store.assign((size_t)h->n_dc * 36, 0.0);
+        for (int j = 0; j < h->n_dc; ++j) {
+            const int i = dc[j];
+            double *o = &h->dc_saved_store[(size_t)36 * j];
+            for (int a = 0; a < 9; ++a) o[a] = pts->F ? pts->F[9 * i + a] : ((a % 4) == 0 ? 1.0 : 0.0);
+            for (int a = 0; a < 6; ++a) o[9 + a] = pts->stress ? pts->stress[6 * i + a] : 0.0;
+            for (int a = 0; a < 6; ++a) o[15 + a] = pts->strain ? pts->strain[6 * i + a] : 0.0;
+            for (int a = 0; a < 6; ++a) o[21 + a] = pts->eps_p ? pts->eps_p[6 * i + a] : 0.0;
+            o[27] = pts->alpha ? pts->alpha[i] : 0.0;
+            o[28] = pts->vol ? pts->vol[i] : (pts->vol0 ? pts->vol0[i] : 0.0);
+        }
+    }
+    HIPCHK(h, hipMemsetAsync(h->marks, 0, (size_t)h->nnodes * 8, h->stream));
+    if (h->n_dc > 0) {
+        k_dc_marks<<<cdiv(h->n_dc, 256), 256, 0, h->stream>>>(h->g, h->dc_x, h->n_dc, (uint32_t)h->cycle, h->marks);
+        HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipMemsetAsync(h->flags_d, 0, 4, h->stream));
+    h->parity = 0;
+    RCHK(rebin(h, true));
+    bool need = false;
+    RCHK(check_flags(h, &need));
+    HIPCHK(h, hipMemsetAsync(h->flags_d, 0, 4, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->rebins = 0;
+    return MPM_OK;
+}
+
+extern "C" {
+
+int mpm_upload_points(mpm_handle *h, const mpm_points *pts)
+{
+    if (!h || !pts || pts->n < 0 || !pts->x || !pts->mass) { if (h) h->err = "null points"; return MPM_E_ARG; }
+    if (h->uploaded) { h->err = "points already uploaded"; return MPM_E_STATE; }
+    if (!pts->vol0 && !pts->vol) { h->err = "vol0 or vol required"; return MPM_E_ARG; }
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc = h->fp32 ? upload_t<float>(h, pts) : upload_t<double>(h, pts);
+    if (rc == MPM_OK) h->uploaded = true;
+    return rc;
+}
+
+int mpm_step(mpm_handle *h, double dt_total, int32_t path, int32_t *finished)
+{
+    if (!h) return MPM_E_ARG;
+    if (!h->uploaded) { h->err = "mpm_step before mpm_upload_points"; return MPM_E_STATE; }
+    if (path != MPM_PATH_PLAIN) { h->err = "contact path not available in this build"; return MPM_E_ARG; }
+    if (!(dt_total >= 0.0)) { h->err = "dt_total must be >= 0"; return MPM_E_ARG; }
+    HIPCHK(h, hipSetDevice(h->device));
+    return h->fp32 ? step_impl<float>(h, dt_total, finished) : step_impl<double>(h, dt_total, finished);
+}
+
+}  // extern "C"
+
+template <typename real>
+static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
+{
+    const int n = h->n_total;
+    double *st[8] = {nullptr};
+    const int widths[8] = {3, 3, 9, 6, 6, 6, 1, 1};
+    double *dst[8] = {out->x, out->v, out->F, out->stress, out->strain, out->eps_p, out->alpha, out->vol};
+    for (int k = 0; k < 8; ++k) {
+        if (!(mask & (1u << k)) || !dst[k]) continue;
+        RCHK(dalloc(h, &st[k], (size_t)widths[k] * std::max(n, 1)));
+    }
+    uint32_t m = 0;
+    for (int k = 0; k < 8; ++k) if (st[k]) m |= (1u << k);
+    ReadbackOut o{st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7]};
+    if (h->n_def > 0) {
+        k_readback<real><<<cdiv(h->n_def, kBlock), kBlock, 0, h->stream>>>(h->g, parr<real>(h, h->cur), (uint32_t)h->n_def, o, m);
+        HIPCHK(h, hipGetLastError());
+    }
+    std::vector<double> hx(3 * (size_t)h->n_dc + 1), hv(3 * (size_t)h->n_dc + 1);
+    if (h->n_dc > 0) {
+        HIPCHK(h, hipMemcpyAsync(hx.data(), h->dc_x, sizeof(double) * 3 * h->n_dc, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(hv.data(), h->dc_v, sizeof(double) * 3 * h->n_dc, hipMemcpyDeviceToHost, h->stream));
+    }
+    for (int k = 0; k < 8; ++k)
+        if (st[k] && h->n_def > 0) {
+            // copy whole array; platen entries are overwritten below
+            std::vector<double> buf((size_t)widths[k] * n);
+            HIPCHK(h, hipMemcpyAsync(buf.data(), st[k], sizeof(double) * widths[k] * n, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            for (int i : h->def_id_host)
+                for (int w = 0; w < widths[k]; ++w) dst[k][(size_t)widths[k] * i + w] = buf[(size_t)widths[k] * i + w];
+        }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int j = 0; j < h->n_dc; ++j) {
+        const int i = h->dc_id_host[j];
+        const double *sv = &h->dc_saved_store[(size_t)36 * j];
+        if (st[0]) for (int a = 0; a < 3; ++a) out->x[3 * i + a] = hx[3 * j + a];
+        if (st[1]) for (int a = 0; a < 3; ++a) out->v[3 * i + a] = hv[3 * j + a];
+        if (st[2]) for (int a = 0; a < 9; ++a) out->F[9 * i + a] = sv[a];
+        if (st[3]) for (int a = 0; a < 6; ++a) out->stress[6 * i + a] = sv[9 + a];
+        if (st[4]) for (int a = 0; a < 6; ++a) out->strain[6 * i + a] = sv[15 + a];
+        if (st[5]) for (int a = 0; a < 6; ++a) out->eps_p[6 * i + a] = sv[21 + a];
+        if (st[6]) out->alpha[i] = sv[27];
+        if (st[7]) out->vol[i] = sv[28];
+    }
+    for (int k = 0; k < 8; ++k) dev_free((void *&)st[k]);
+    return MPM_OK;
+}
+
+extern "C" {
+
+int mpm_readback(mpm_handle *h, uint32_t field_mask, mpm_points *out)
+{
+    if (!h || !out) return MPM_E_ARG;
+    if (!h->uploaded) { h->err = "readback before upload"; return MPM_E_STATE; }
+    if (out->n != h->n_total) { h->err = "readback point count mismatch"; return MPM_E_ARG; }
+    if ((field_mask & MPM_F_STRAIN) && !h->cfg.track_strain) { h->err = "strain not tracked (mpm_config.track_strain=0)"; return MPM_E_ARG; }
+    HIPCHK(h, hipSetDevice(h->device));
+    return h->fp32 ? readback_t<float>(h, field_mask, out) : readback_t<double>(h, field_mask, out);
+}
+
+int mpm_checks(mpm_handle *h, double out[12])
+{
+    if (!h || !out) return MPM_E_ARG;
+    if (!h->uploaded) { h->err = "checks before upload"; return MPM_E_STATE; }
+    HIPCHK(h, hipSetDevice(h->device));
+    return checks_impl(h, out);
+}
+
+int mpm_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t *n)
+{
+    if (!h || (!out && cap > 0) || !n) return MPM_E_ARG;
+    const int m = std::min<int>((int)h->hist.size(), cap);
+    for (int k = 0; k < m; ++k) out[k] = h->hist[k];
+    *n = m;
+    return MPM_OK;
+}
+
+int mpm_kernel_times(mpm_handle *h, double out[6])
+{
+    if (!h || !out) return MPM_E_ARG;
+    for (int k = 0; k < 6; ++k) out[k] = h->kt[k];
+    return MPM_OK;
+}
+
+int mpm_get_time(mpm_handle *h, double *time, double *time_end, double *dt_max)
+{
+    if (!h) return MPM_E_ARG;
+    if (time) *time = h->time;
+    if (time_end) *time_end = h->cfg.time_end;
+    if (dt_max) *dt_max = h->cfg.dt_max;
+    return MPM_OK;
+}
+
+int mpm_last_step_ms(mpm_handle *h, double *ms)
+{
+    if (!h || !ms) return MPM_E_ARG;
+    *ms = h->last_ms;
+    return MPM_OK;
+}
+
+int mpm_sort_keys(const uint32_t *keys_host, int32_t n, uint32_t *sorted_host, uint32_t *perm_host, int32_t device)
+{
+    if (n < 0 || (n > 0 && (!keys_host || !sorted_host || !perm_host))) { g_create_error = "bad sort args"; return MPM_E_ARG; }
+    if (n == 0) return MPM_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return MPM_E_NO_DEVICE; }
+    mpm_handle tmp;
+    mpm_handle *h = &tmp;
+    h->device = device;
+    h->n_def = n;
+    h->nbricks = 1;
+    int rc = MPM_OK;
+    auto run = [&]() -> int {
+        HIPCHK(h, hipSetDevice(device));
+        HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        RCHK(alloc_sort_scratch(h, (uint32_t)n));
+        uint32_t mx = 0;
+        for (int i = 0; i < n; ++i) mx = std::max(mx, keys_host[i]);
+        std::vector<uint32_t> iota(n);
+        for (int i = 0; i < n; ++i) iota[i] = (uint32_t)i;
+        HIPCHK(h, hipMemcpy(h->keys[0], keys_host, 4 * (size_t)n, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->vals[0], iota.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
+        uint32_t *ko, *vo;
+        RCHK(radix_sort(h, h->keys[0], h->vals[0], h->keys[1], h->vals[1], (uint32_t)n, mx, &ko, &vo));
+        HIPCHK(h, hipMemcpyAsync(sorted_host, ko, 4 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(perm_host, vo, 4 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return MPM_OK;
+    };
+    rc = run();
+    if (rc != MPM_OK) g_create_error = h->err;
+    destroy_all(h);
+    return rc;
+}
+
+}  // extern "C"

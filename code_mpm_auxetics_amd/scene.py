@@ -1,0 +1,373 @@
+"""Host-side scene construction: material-point domains, grids and worlds.
+
+Domain creation stays on the host (it is not part of the GPU step).  The
+functions here restate the reference factories so that the points they emit
+have the same positions and the same order:
+
+* ``create_domain_auxetic_cell_polygon`` --
+  MaterialPoint_Factory_createDomain_AuxeticCell_Polygon.cpp:4-112
+* ``create_domain_cuboid`` -- MaterialPoint_Factory_createDomain_Cuboid.cpp:4-57
+* ``create_domain_auxetic_cell_swisscheese`` --
+  MaterialPoint_Factory_createDomain_AuxeticCell_Swisscheese.cpp:4-101
+* ``is_inside_polygon`` / ``is_inside_ellipse`` -- MaterialPoint_Factory_isInside.cpp:4-35
+* ``reference_polygon_world`` -- PhysicsEngine_initializeWorld_AuxeticPolygonCell.cpp:4-313
+
+Loop coordinates are accumulated sequentially (``dx += dOffset``) exactly as
+the reference loops do, so the FP64 positions match bit for bit.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+ELASTIC, NEOHOOKEAN, VISCOELASTIC, PLASTIC, VONMISESHARDENING, GASS = range(6)
+FIX_X, FIX_Y, FIX_Z = 1, 2, 4
+
+
+# ---------------------------------------------------------------- factories
+def _accumulate(start, step, stop):
+    vals = []
+    d = start
+    while d <= stop:
+        vals.append(d)
+        d += step
+    return np.array(vals, dtype=np.float64)
+
+
+def is_inside_polygon(coords, verts):
+    """Vectorised MaterialPoint_Factory::isInside(coord, vertices) (isInside.cpp:4-21).
+
+    coords: (n, 2) x/y; verts: list of (x, y, z).  True when the z component of
+    cross(v[i+1]-v[i], p-v[i]) is >= 0 for every edge.
+    """
+    inside = np.ones(len(coords), dtype=bool)
+    for i in range(len(verts) - 1):
+        v1x = verts[i + 1][0] - verts[i][0]
+        v1y = verts[i + 1][1] - verts[i][1]
+        v2x = coords[:, 0] - verts[i][0]
+        v2y = coords[:, 1] - verts[i][1]
+        cz = v1x * v2y - v2x * v1y          # glm::cross(a, b).z = a.x*b.y - b.x*a.y
+        inside &= ~(cz < 0.0)
+    return inside
+
+
+def is_inside_ellipse(coords, center, radii):
+    """MaterialPoint_Factory::isInside(coord, center, radii) (isInside.cpp:23-35)."""
+    lx = coords[:, 0] - center[0]
+    ly = coords[:, 1] - center[1]
+    a = lx / radii[0]
+    b = ly / radii[1]
+    return ~((a * a + b * b) > 1.0)
+
+
+_MIRROR_POLY = [(+1, +1, +1), (-1, +1, +1), (+1, -1, +1), (-1, -1, +1),
+                (+1, +1, -1), (-1, +1, -1), (+1, -1, -1), (-1, -1, -1)]
+_MIRROR_CUBOID = [(+1, +1, +1), (-1, +1, +1), (-1, -1, +1), (+1, -1, +1),
+                  (+1, +1, -1), (-1, +1, -1), (-1, -1, -1), (+1, -1, -1)]
+
+
+def _emit(center, q, mirrors):
+    """For each quarter coordinate (in loop order) emit the mirrored points in
+    the reference order: result shape (len(q)*len(mirrors), 3)."""
+    out = np.empty((len(q), len(mirrors), 3), dtype=np.float64)
+    for k, (sx, sy, sz) in enumerate(mirrors):
+        out[:, k, 0] = center[0] + sx * q[:, 0]
+        out[:, k, 1] = center[1] + sy * q[:, 1]
+        out[:, k, 2] = center[2] + sz * q[:, 2]
+    return out.reshape(-1, 3)
+
+
+def _quarter_grid(xs, ys, zs):
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")   # x outer, z inner
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+
+def polygon_cell_vertices(dims, dent, thickness):
+    """Quarter-cell polygons of createDomain_AuxeticCell_Polygon (…Polygon.cpp:8-39)."""
+    Dx, Dy = dims[0], dims[1]
+    l_inc = math.sqrt((0.5 * Dx - 2.0 * dent) ** 2 + (0.5 * Dy) ** 2)
+    s = 0.5 * Dy / l_inc
+    c = math.sqrt(1.0 - s ** 2)
+    t = s / c
+    total = [(0.0, 0.0, 0.0), (0.5 * Dx, 0.0, 0.0), (0.5 * Dx, 0.5 * Dy, 0.0), (0.0, 0.5 * Dy, 0.0), (0.0, 0.0, 0.0)]
+    right = [(0.5 * Dx, 0.5 * thickness, 0.0), (0.5 * Dx, 0.5 * Dy, 0.0),
+             (0.5 * Dx - dent + 0.5 * thickness / s, 0.5 * Dy, 0.0),
+             (dent + 0.5 * thickness / s + 0.5 * thickness / t, 0.5 * thickness, 0.0),
+             (0.5 * Dx, 0.5 * thickness, 0.0)]
+    left = [(0.0, 0.0, 0.0), (dent - 0.5 * thickness / s, 0.0, 0.0),
+            (0.5 * Dx - dent - 0.5 * thickness / s - 0.5 * thickness / t, 0.5 * Dy - 0.5 * thickness, 0.0),
+            (0.0, 0.5 * Dy - 0.5 * thickness, 0.0), (0.0, 0.0, 0.0)]
+    return total, right, left
+
+
+def create_domain_auxetic_cell_polygon(center, dims, dent, thickness, offset):
+    """MaterialPoint_Factory::createDomain_AuxeticCell_Polygon (…Polygon.cpp:4-112)."""
+    total, right, left = polygon_cell_vertices(dims, dent, thickness)
+    xs = _accumulate(0.5 * offset, offset, 0.5 * dims[0])
+    ys = _accumulate(0.5 * offset, offset, 0.5 * dims[1])
+    zs = _accumulate(0.5 * offset, offset, 0.5 * dims[2])
+    q = _quarter_grid(xs, ys, zs)
+    keep = is_inside_polygon(q, total) & ~is_inside_polygon(q, right) & ~is_inside_polygon(q, left)
+    return _emit(center, q[keep], _MIRROR_POLY)
+
+
+def create_domain_cuboid(center, dims, offset):
+    """MaterialPoint_Factory::createDomain_Cuboid (…Cuboid.cpp:4-57)."""
+    single = dims[2] <= offset
+    xs = _accumulate(0.5 * offset, offset, 0.5 * dims[0])
+    ys = _accumulate(0.5 * offset, offset, 0.5 * dims[1])
+    zs = _accumulate(0.0 if single else 0.5 * offset, offset, 0.5 * dims[2])
+    q = _quarter_grid(xs, ys, zs)
+    return _emit(center, q, _MIRROR_CUBOID[:4] if single else _MIRROR_CUBOID)
+
+
+def create_domain_auxetic_cell_swisscheese(center, dims, spacing, radii, offset):
+    """MaterialPoint_Factory::createDomain_AuxeticCell_Swisscheese (…Swisscheese.cpp:4-101)."""
+    centers, rad = [], []
+    ix = 0
+    while ix < dims[0] / spacing[0]:
+        iy = 0
+        while iy < dims[1] / spacing[1] - 2:
+            centers.append((ix * spacing[0], iy * spacing[1]))
+            rad.append(tuple(radii) if (ix + iy) % 2 == 0 else (radii[1], radii[0]))
+            iy += 1
+        ix += 1
+    single = dims[2] <= offset
+    xs = _accumulate(0.5 * offset, offset, 0.5 * dims[0])
+    ys = _accumulate(0.5 * offset, offset, 0.5 * dims[1])
+    zs = _accumulate(0.0 if single else 0.5 * offset, offset, 0.5 * dims[2])
+    q = _quarter_grid(xs, ys, zs)
+    keep = np.ones(len(q), dtype=bool)
+    for c, r in zip(centers, rad):
+        keep &= ~is_inside_ellipse(q, c, r)
+    return _emit(center, q[keep], _MIRROR_POLY[:4] if single else _MIRROR_POLY)
+
+
+# -------------------------------------------------------------------- scene
+@dataclass
+class Scene:
+    """Everything an engine needs: grid, node BCs, points, solver parameters."""
+    length: tuple
+    cells: tuple
+    fixed: np.ndarray                      # uint8 per node, FIX_* bits
+    x: np.ndarray                          # (n, 3)
+    mass: np.ndarray
+    vol0: np.ndarray
+    E: np.ndarray
+    nu: np.ndarray
+    sy: np.ndarray
+    C0: np.ndarray
+    C1: np.ndarray
+    type: np.ndarray                       # int32
+    dc: np.ndarray                         # uint8
+    v: np.ndarray = None
+    fext: np.ndarray = None
+    surface: np.ndarray = None
+    dt_max: float = 1e-9
+    time_end: float = 1.0
+    damping: float = 0.0
+    mass_min: float = 0.0
+    console_interval: float = 0.0
+    timeline_t: np.ndarray = None
+    timeline_v: np.ndarray = None
+    neohookean: int = 0
+    info: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        n = len(self.x)
+        if self.v is None:
+            self.v = np.zeros((n, 3))
+        if self.fext is None:
+            self.fext = np.zeros((n, 3))
+        if self.surface is None:
+            self.surface = np.zeros(n, dtype=np.uint8)
+        if self.timeline_t is None:
+            self.timeline_t = np.zeros(0)
+            self.timeline_v = np.zeros((0, 3))
+
+    @property
+    def n(self):
+        return len(self.x)
+
+    @property
+    def n_deformable(self):
+        return int(np.count_nonzero(self.dc == 0))
+
+    @property
+    def nodes(self):
+        return tuple(c + 1 for c in self.cells)
+
+    @property
+    def h(self):
+        return tuple(self.length[a] / self.cells[a] for a in range(3))
+
+    def copy(self):
+        import copy
+        return copy.deepcopy(self)
+
+
+def node_positions(length, cells):
+    """GridPoint_Factory::createGrid node positions (GridPoint_Factory.h:33-67),
+    x fastest: returns (nx*ny*nz, 3)."""
+    nn = [c + 1 for c in cells]
+    ax = [np.arange(nn[a], dtype=np.float64) * length[a] / cells[a] for a in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+
+def reference_bcs(length, cells, tol_frac=0.1):
+    """Node BCs of initializeWorld_AuxeticPolygonCell (…PolygonCell.cpp:42-79):
+    x=0 / x=L fixed in x, y=0 fixed in x,y,z, z=0 / z=L fixed in z."""
+    p = node_positions(length, cells)
+    h = length[0] / cells[0]
+    tol = tol_frac * h
+    f = np.zeros(len(p), dtype=np.uint8)
+    f[np.abs(p[:, 0] - 0.0) < tol] |= FIX_X
+    f[np.abs(p[:, 0] - length[0]) < tol] |= FIX_X
+    f[np.abs(p[:, 1] - 0.0) < tol] |= FIX_X | FIX_Y | FIX_Z
+    f[np.abs(p[:, 2] - 0.0) < tol] |= FIX_Z
+    f[np.abs(p[:, 2] - length[2]) < tol] |= FIX_Z
+    return f
+
+
+def _material_arrays(n, E, nu, sy, C0, C1, mtype, rho, vol):
+    return dict(
+        mass=np.full(n, rho * vol), vol0=np.full(n, vol), E=np.full(n, E), nu=np.full(n, nu),
+        sy=np.full(n, sy), C0=np.full(n, C0), C1=np.full(n, C1), type=np.full(n, mtype, dtype=np.int32))
+
+
+def _assemble(length, cells, fixed, bodies, **params):
+    xs, cols, dcs = [], {k: [] for k in ("mass", "vol0", "E", "nu", "sy", "C0", "C1", "type")}, []
+    for pts, mat, dc in bodies:
+        xs.append(pts)
+        for k in cols:
+            cols[k].append(mat[k])
+        dcs.append(np.full(len(pts), 1 if dc else 0, dtype=np.uint8))
+    return Scene(length=tuple(length), cells=tuple(cells), fixed=fixed, x=np.concatenate(xs),
+                 dc=np.concatenate(dcs), **{k: np.concatenate(v) for k, v in cols.items()}, **params)
+
+
+def p_wave_speed(E, nu, rho):
+    """c = sqrt((lambda + 2 mu) / rho)."""
+    return math.sqrt(E * (1.0 - nu) / ((1.0 + nu) * (1.0 - 2.0 * nu)) / rho)
+
+
+# ------------------------------------------------------------------- worlds
+def reference_polygon_world():
+    """PhysicsEngine::initializeWorld_AuxeticPolygonCell (…PolygonCell.cpp:4-313)."""
+    L = (0.06, 0.03, 0.002)
+    cells = (120, 60, 2)
+    h = tuple(L[a] / cells[a] for a in range(3))
+    fixed = reference_bcs(L, cells)
+    off = 1.0 / 2.0 * h[0]
+    thick = 2.0 * off
+    dims_cell = (0.050346, 0.02, thick)
+    center = [0.5 * L[0], 0.5 * L[1], 0.5 * L[2]]
+    center[1] = 0.5 * dims_cell[1] + 1.0 * h[1]
+    center[2] = 0.5 * thick
+    dent = 0.166666666 * dims_cell[0]
+    cell = create_domain_auxetic_cell_polygon(center, dims_cell, dent, 1.0 * 0.0006, off)
+    vol = off * off * off
+    al = _material_arrays(len(cell), 70.0e9, 0.33, 1.0 * 150.0e6, 4.0e1, 50.0e6, PLASTIC, 2700.0, vol)
+    top_c = [center[0], center[1] + 0.5 * dims_cell[1] + 0.5 * h[1], center[2]]
+    top = create_domain_cuboid(top_c, (dims_cell[0], 2.0 * off, dims_cell[2]), off)
+    steel_t = _material_arrays(len(top), 210.0e9, 0.33, 200.0e6, 0.0, 0.0, ELASTIC, 7800.0, vol)
+    bot_c = [center[0], 0.5 * h[1], center[2]]
+    bot = create_domain_cuboid(bot_c, (dims_cell[0], 2.0 * off, dims_cell[2]), off)
+    steel_b = _material_arrays(len(bot), 210.0e9, 0.33, 200.0e6, 0.0, 0.0, ELASTIC, 7800.0, vol)
+    sc = _assemble(L, cells, fixed, [(cell, al, False), (top, steel_t, True), (bot, steel_b, False)],
+                   dt_max=1.0e-9, time_end=0.010, damping=0.001, mass_min=0.001 * 2700.0 * vol,
+                   console_interval=1.0e-6,
+                   timeline_t=np.array([0.0, 0.1]), timeline_v=np.array([[0.0, -1.0, 0.0], [0.0, -1.0, 0.0]]))
+    sc.info = dict(name="reference_polygon_world", offset=off)
+    return sc
+
+
+def _grid_for(extent, h):
+    cells = tuple(int(math.ceil(extent[a] / h[a] - 1e-9)) for a in range(3))
+    return tuple(cells[a] * h[a] for a in range(3)), cells
+
+
+def polygon_mesh_world(nx=1, ny=1, cell_w=0.010, cell_h=0.010, wall=0.0008, thickness=0.001, h=0.000125,
+                       material="elastic", E=1.0e9, nu=0.3, rho=1200.0, sy=250.0e6, H=700.0e6,
+                       platen_speed=0.1, strain_target=None, steps=2000, cfl=0.3, margin_cells=2,
+                       damping=0.001, dent_frac=1.0 / 6.0):
+    """A tiled mesh of reference polygon cells compressed by a displacement-
+    controlled top platen; bottom node plane fixed.
+
+    Each unit cell is createDomain_AuxeticCell_Polygon with dent = W/6 (as
+    …PolygonCell.cpp:103), points at 2x2x2 per grid cell (offset h/2,
+    …PolygonCell.cpp:89).  The platen is a createDomain_Cuboid slab of two
+    point layers on top of the mesh, moving at -platen_speed in y; with
+    ``strain_target`` the speed is set so the platen travels
+    strain_target * mesh height in ``steps`` steps of dt = cfl*h/c.
+    """
+    hx = h
+    off = 0.5 * hx
+    mesh_w, mesh_h = nx * cell_w, ny * cell_h
+    extent = (mesh_w + 2 * margin_cells * hx, mesh_h + (2 + margin_cells) * hx, thickness)
+    L, cells = _grid_for(extent, (hx, hx, hx))
+    L = (L[0], L[1], thickness)
+    cells = (cells[0], cells[1], max(1, int(round(thickness / hx))))
+    x0 = 0.5 * (L[0] - mesh_w)
+    pts = []
+    dims = (cell_w, cell_h, thickness)
+    for j in range(ny):
+        for i in range(nx):
+            c = (x0 + (i + 0.5) * cell_w, (j + 0.5) * cell_h, 0.5 * thickness)
+            pts.append(create_domain_auxetic_cell_polygon(c, dims, dent_frac * cell_w, wall, off))
+    body = np.concatenate(pts)
+    vol = off ** 3
+    if material == "elastic":
+        mat = _material_arrays(len(body), E, nu, 0.0, 0.0, 0.0, ELASTIC, rho, vol)
+    elif material == "j2":
+        C1 = 1.0e15
+        mat = _material_arrays(len(body), E, nu, sy, H / C1, C1, VONMISESHARDENING, rho, vol)
+    elif material == "plastic":
+        mat = _material_arrays(len(body), E, nu, sy, 0.0, 0.0, PLASTIC, rho, vol)
+    elif material == "neohookean":
+        mat = _material_arrays(len(body), E, nu, 0.0, 0.0, 0.0, NEOHOOKEAN, rho, vol)
+    else:
+        raise ValueError(material)
+    top_c = (0.5 * L[0], mesh_h + 0.5 * hx, 0.5 * thickness)
+    top = create_domain_cuboid(top_c, (mesh_w, 2.0 * off, thickness), off)
+    plat = _material_arrays(len(top), 210.0e9, 0.33, 200.0e6, 0.0, 0.0, ELASTIC, 7800.0, vol)
+    fixed = reference_bcs(L, cells)
+    c = p_wave_speed(E, nu, rho)
+    dt = cfl * hx / c
+    speed = platen_speed
+    if strain_target is not None:
+        speed = strain_target * mesh_h / (steps * dt)
+    sc = _assemble(L, cells, fixed, [(body, mat, False), (top, plat, True)],
+                   dt_max=dt, time_end=steps * dt, damping=damping, mass_min=0.001 * rho * vol,
+                   timeline_t=np.array([0.0, 1.0e3]),
+                   timeline_v=np.array([[0.0, -speed, 0.0], [0.0, -speed, 0.0]]),
+                   neohookean=1 if material == "neohookean" else 0)
+    sc.info = dict(name="polygon_mesh", nx=nx, ny=ny, h=hx, dt=dt, wave_speed=c, platen_speed=speed,
+                   steps=steps, material=material, cell_w=cell_w, cell_h=cell_h, wall=wall)
+    return sc
+
+
+def config1_world(platen="literal", steps=2000):
+    """one 10 x 10 x 1 mm re-entrant polygon cell, wall
+    0.8 mm, h = 0.125 mm, linear elastic E = 1 GPa, nu = 0.3, rho = 1200."""
+    return polygon_mesh_world(1, 1, 0.010, 0.010, 0.0008, 0.001, 0.000125, "elastic", 1.0e9, 0.3, 1200.0,
+                              platen_speed=0.1, strain_target=(0.2 if platen == "strain" else None),
+                              steps=steps)
+
+
+def config2_world(material="j2", scale=0.2, steps=5000, platen="strain"):
+    """10 x 10 tiling of the polygon cell at h = 0.1 mm,
+    J2 hardening E = 70 GPa, nu = 0.33, sigma_y = 250 MPa, H = 700 MPa.
+    The config-1 cell (10 mm, wall 0.8 mm) is scaled by ``scale`` in-plane
+    (thickness 1 mm kept) so the mesh holds about 1.0M points."""
+    return polygon_mesh_world(10, 10, 0.010 * scale, 0.010 * scale, 0.0008 * scale, 0.001, 0.0001, material,
+                              70.0e9, 0.33, 2700.0, sy=250.0e6, H=700.0e6, platen_speed=0.1,
+                              strain_target=(0.3 if platen == "strain" else None), steps=steps)
+
+
+def small_world(nx=1, ny=1, material="elastic", h=0.0005, cell=0.004, wall=0.0008, thickness=0.001,
+                steps=100, speed=2.0):
+    """A few-thousand-point polygon world for quick parity tests."""
+    return polygon_mesh_world(nx, ny, cell, cell, wall, thickness, h, material, 1.0e9 if material != "j2" else 70e9,
+                              0.3, 1200.0, sy=5.0e6, H=50.0e6, platen_speed=speed, steps=steps)

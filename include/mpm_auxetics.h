@@ -1,0 +1,189 @@
+/*
+ * mpm_auxetics.h -- C ABI of the MI355X explicit MPM step.
+ *
+ * Plain C, plain pointers and sizes.  The host C++ PhysicsEngine shim
+ * (code_mpm_auxetics_amd/host/PhysicsEngine.h), the ctypes test harness and
+ * any other FFI bind exactly these symbols.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the reference tree Sina-mon/code_MPM_Auxetics).
+ */
+#ifndef MPM_AUXETICS_H
+#define MPM_AUXETICS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+
+/* ---- status codes (the reference has none; it exits via fatalError,
+ *      Errors.cpp:5-12) ---------------------------------------------------- */
+enum mpm_status {
+    MPM_OK = 0,
+    MPM_E_ARG = -1,          /* bad argument / config                         */
+    MPM_E_HIP = -2,          /* HIP runtime error (see mpm_last_error)        */
+    MPM_E_NAN = -3,          /* non-finite particle state detected            */
+    MPM_E_STATE = -4,        /* call order violated (e.g. step before upload) */
+    MPM_E_OUT_OF_GRID = -5,  /* a point left the background grid box          */
+    MPM_E_ARENA = -6,        /* a point moved further than one cell beyond its
+                                brick between two drift checks                */
+    MPM_E_NOMEM = -7,
+    MPM_E_NO_DEVICE = -8
+};
+
+/* precision of the device state */
+enum mpm_precision {
+    MPM_FP64 = 0,            /* FP64 "plain": every field and operation FP64   */
+    MPM_FP32 = 1             /* FP32 state; FP64 det(F)/log J, J2 return map,
+                                global checks                                 */
+};
+
+/* step paths: runSimulation_Classic_SinglePass_MP / _MP_Contact
+ * (PhysicsEngine.h:45-46) */
+enum mpm_path { MPM_PATH_PLAIN = 0, MPM_PATH_CONTACT = 1 };
+
+/* material types: same numbering as the reference enum MaterialType
+ * (MaterialPoint.h:8-16) */
+enum mpm_material {
+    MPM_ELASTIC = 0, MPM_NEOHOOKEAN = 1, MPM_VISCOELASTIC = 2,
+    MPM_PLASTIC = 3, MPM_VONMISESHARDENING = 4, MPM_GASS = 5
+};
+
+/* node fixity bits: GridPoint::b3_Fixed (GridPoint.h:18) */
+enum { MPM_FIX_X = 1, MPM_FIX_Y = 2, MPM_FIX_Z = 4 };
+
+/* readback field mask bits */
+enum {
+    MPM_F_X = 1u << 0, MPM_F_V = 1u << 1, MPM_F_F = 1u << 2, MPM_F_STRESS = 1u << 3,
+    MPM_F_STRAIN = 1u << 4, MPM_F_EPSP = 1u << 5, MPM_F_ALPHA = 1u << 6,
+    MPM_F_VOL = 1u << 7, MPM_F_KERNEL = 1u << 8, MPM_F_ALL = 0x1FFu
+};
+
+/* Solver configuration.  Replaces the protected members that
+ * PhysicsEngine::initializeWorld_* sets (PhysicsEngine.h:68-93,
+ * PhysicsEngine_initializeWorld_AuxeticPolygonCell.cpp:10-40,269-274). */
+typedef struct mpm_config {
+    double length[3];          /* d3_Length_Grid                                */
+    int32_t cells[3];          /* i3_Cells; nodes = cells + 1 per axis          */
+    double dt_max;             /* d_TimeIncrement_Maximum                       */
+    double time_start;         /* d_Time at creation                            */
+    double time_end;           /* d_TimeEnd                                     */
+    double damping;            /* d_DampingCoefficient                          */
+    double mass_min;           /* d_Mass_Minimum                                */
+    double console_interval;   /* d_TimeConsole_Interval (<=0: no reports)      */
+    int32_t precision;         /* enum mpm_precision                            */
+    int32_t strain_form;       /* 0: (I+L dt)-I as the reference (…_MP.cpp:304);
+                                  1: L dt (always used in MPM_FP32)             */
+    int32_t neohookean;        /* 0: _NEOHOOKEAN falls to PerfectlyPlastic as in
+                                  the reference dispatch (…_MP.cpp:332-339);
+                                  1: dispatch calculateIncrement_NeoHookean_6D  */
+    int32_t track_strain;      /* keep the total strain d6_Strain (…_MP.cpp:318) */
+    int32_t n_timeline;        /* TimeLine knots (TimeLine.h:27-31), may be 0   */
+    const double *timeline_t;  /* n_timeline times                              */
+    const double *timeline_v;  /* 3*n_timeline velocities                       */
+    const uint8_t *node_fixed; /* (cells+1) product entries of MPM_FIX_* bits, or
+                                  NULL; copied at mpm_create                    */
+    int32_t brick[3];          /* cells per brick; 0 = default (8,4,2)          */
+    int32_t drift_check;       /* steps between drift checks; 0 = default (8)   */
+    double friction;           /* contact path: Coulomb coefficient (extension;
+                                  0 reproduces the frictionless reference)     */
+    int32_t device;            /* HIP device ordinal                            */
+    int32_t verbose;           /* 1: print reportConsole lines to stdout        */
+    int32_t profile;           /* 1: HIP events around every kernel launch, read
+                                  with mpm_kernel_times                         */
+} mpm_config;
+
+/* Host point arrays, FP64, caller-allocated, original point order.  Mirrors
+ * the MaterialPoint fields (MaterialPoint.h:24-56). Vectors are 3 per point,
+ * F is 9 per point column-major (glm m[col][row]), Voigt 6-vectors are
+ * [xx, yy, zz, xy, yz, zx].  Pointers a call does not need may be NULL. */
+typedef struct mpm_points {
+    int32_t n;
+    double *x, *v, *fext, *F, *strain, *stress, *eps_p, *alpha;
+    double *mass, *vol, *vol0;
+    double *E, *nu, *sy, *C0, *C1;
+    int32_t *type;
+    uint8_t *dc;               /* b_DisplacementControl                         */
+    uint8_t *surface;          /* b_Surface (contact path)                      */
+    double *kernel, *kernel_grad;  /* d_Kernel, d3_Kernel_Gradient (readback)   */
+} mpm_points;
+
+typedef struct mpm_handle mpm_handle;
+
+/* ABI version (for the FFI binding to check) */
+int mpm_abi_version(void);
+
+/* Create a solver on one device.  Replaces PhysicsEngine::PhysicsEngine()
+ * plus the grid part of initializeWorld_* (GridPoint_Factory::createGrid,
+ * GridPoint_Factory.h:33-67; node BCs …PolygonCell.cpp:42-79). */
+int mpm_create(const mpm_config *cfg, mpm_handle **out);
+
+/* Upload the material points.  Replaces the allMaterialPoint /
+ * v_MarkedMaterialPoints_Displacement_Control vectors that initializeWorld_*
+ * fills (PhysicsEngine.h:78-80, …PolygonCell.cpp:137-191).  Points with dc!=0
+ * are displacement-controlled (platens), in index order. */
+int mpm_upload_points(mpm_handle *h, const mpm_points *pts);
+
+/* Advance by dt_total in steps of at most dt_max.  Replaces
+ * int PhysicsEngine::runSimulation_Classic_SinglePass_MP(double) and
+ * …_MP_Contact(double) (PhysicsEngine.h:45-46,
+ * PhysicsEngine_runSimulation_Classic_SinglePass_MP.cpp:4-394).
+ * *finished receives the reference's return value: 1 if d_Time >= d_TimeEnd,
+ * else 0 (…_MP.cpp:385-393).  Blocks until the steps completed. */
+int mpm_step(mpm_handle *h, double dt_total, int32_t path, int32_t *finished);
+
+/* Copy point fields (mask of MPM_F_*) back to host arrays in the original
+ * point order.  Replaces PhysicsEngine::getMaterialPoints (PhysicsEngine.h:64),
+ * whose callers read the live MaterialPoint objects. */
+int mpm_readback(mpm_handle *h, uint32_t field_mask, mpm_points *out);
+
+/* FP64 global checks, computed on the device.  Replaces the sums of
+ * PhysicsEngine::reportConsole (PhysicsEngine.cpp:33-97):
+ * out[0] grid mass, out[1] reaction (sum of node Force_Temp.y), out[2]
+ * sum m|v|^2 (no 1/2, as the reference), out[3..5] momentum sum m v,
+ * out[6] non-finite point count, out[7] current time, out[8] step count,
+ * out[9] number of rebins, out[10] occupied bricks, out[11] grid nodes with
+ * mass > 0 after the last grid update. */
+int mpm_checks(mpm_handle *h, double out[12]);
+
+/* Per-step reaction force (sum of Force_Temp.y after each step; the
+ * "Force_y" of reportConsole, PhysicsEngine.cpp:65-71,195) of the steps of the
+ * last mpm_step call; *n receives the count (<= cap). */
+int mpm_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t *n);
+
+/* getTime_Current / getTime_End / getTime_Increment (PhysicsEngine.h:57-60) */
+int mpm_get_time(mpm_handle *h, double *time, double *time_end, double *dt_max);
+
+/* Device-time of the last mpm_step call in milliseconds (HIP events on the
+ * handle's stream).  Replaces getTime_Runtime (PhysicsEngine.h:57). */
+int mpm_last_step_ms(mpm_handle *h, double *ms);
+
+/* Per-kernel device time of the last mpm_step call (needs cfg.profile=1):
+ * out[0] fused particle kernel total ms, out[1] its launches, out[2] grid
+ * kernel total ms, out[3] its launches, out[4] rebin total ms, out[5] rebins.
+ * Replaces the per-phase a_Runtime timers (PhysicsEngine.h:73,
+ * …_MP.cpp:39-369). */
+int mpm_kernel_times(mpm_handle *h, double out[6]);
+
+/* Human-readable last error of the handle (or of mpm_create when h is NULL). */
+const char *mpm_last_error(const mpm_handle *h);
+
+/* Release device memory.  Replaces PhysicsEngine::~PhysicsEngine
+ * (PhysicsEngine.cpp:8-31). */
+void mpm_destroy(mpm_handle *h);
+
+/* ---- stable radix sort of particle keys (exposed for parity tests of the
+ *      deterministic binning).  Sorts n 32-bit keys ascending, ties by input
+ *      position, on the device; writes the sorted keys and the permutation
+ *      (perm[j] = input index of the j-th output).  No reference counterpart:
+ *      the reference keeps points unsorted (PhysicsEngine.h:78). ----------- */
+int mpm_sort_keys(const uint32_t *keys_host, int32_t n, uint32_t *sorted_host,
+                  uint32_t *perm_host, int32_t device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_AUXETICS_H */
