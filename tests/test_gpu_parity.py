@@ -58,7 +58,7 @@ def test_sort_bitexact_wide_keys(mpm):
 
 @pytest.mark.parametrize("material", ["elastic", "j2", "plastic"])
 def test_fp64_parity_small(mpm, oracle_mod, material):
-    sc = mpm.scene.small_world(material=material, speed=5.0)
+    sc = mpm.scene.small_world(material=material, speed=(5.0 if material == "elastic" else 2.0))
     x0 = sc.x.copy()
     eng = mpm.PhysicsEngine(sc, precision=mpm.FP64)
     orc = oracle_mod.Oracle(sc)
@@ -122,30 +122,41 @@ def test_fp32_deterministic(mpm):
         assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
-@pytest.mark.parametrize("material", ["elastic", "j2"])
-def test_fp32_error_vs_fp64_oracle(mpm, oracle_mod, material):
-    sc = mpm.scene.small_world(material=material, speed=5.0, nx=2, ny=1)
+@pytest.mark.parametrize("material,speed,steps", [("elastic", 5.0, 100), ("j2", 2.0, 400)])
+def test_fp32_error_vs_fp64_oracle(mpm, oracle_mod, material, speed, steps):
+    """FP32 state + FP64 islands against the FP64 oracle (L dt strain form),
+    within the window where the reference update stays bounded."""
+    sc = mpm.scene.small_world(material=material, speed=speed, nx=2, ny=1)
     x0 = sc.x.copy()
     eng = mpm.PhysicsEngine(sc, precision=mpm.FP32)
     orc = oracle_mod.Oracle(sc, strain_form=1)
-    eng.run_steps(200)
-    orc.run_steps(200)
-    _compare(eng, orc, x0, 1e-4, 1e-3, 1e-4)
+    eng.run_steps(steps)
+    orc.run_steps(steps)
+    _compare(eng, orc, x0, 1e-4, 1e-4, 1e-4)
     eng.close()
 
 
-def test_rebin_keeps_parity(mpm, oracle_mod):
-    """A fast platen forces many rebins; results still match the oracle."""
-    sc = mpm.scene.small_world(material="elastic", speed=40.0, nx=1, ny=1, h=0.0005)
-    x0 = sc.x.copy()
-    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64, drift_check=4)
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_rebin_translation(mpm, oracle_mod, prec):
+    """A block in free flight crosses bricks (many rebins); positions follow
+    x0 + v t and match the oracle."""
+    sc = mpm.scene.translation_world()
+    p = mpm.FP64 if prec == "fp64" else mpm.FP32
+    eng = mpm.PhysicsEngine(sc, precision=p, drift_check=4)
     orc = oracle_mod.Oracle(sc)
-    eng.run_steps(400)
-    orc.run_steps(400)
+    eng.run_steps(300)
+    orc.run_steps(300)
     c = eng.checks()
     print("rebins", c["rebins"], flush=True)
-    assert c["rebins"] >= 1
-    _compare(eng, orc, x0, 1e-8, 1e-8, 1e-8)
+    assert c["rebins"] >= 3
+    g = eng.getMaterialPoints()
+    o = orc.points()
+    t = eng.getTime_Current()
+    exact = sc.x + sc.v * t
+    tol = 1e-12 if prec == "fp64" else 2e-9
+    assert np.abs(g["x"] - o["x"]).max() <= tol
+    assert np.abs(g["x"] - exact).max() <= tol + 1e-12
+    assert np.abs(g["stress"]).max() <= (1e-3 if prec == "fp64" else 1.0)
     eng.close()
 
 
