@@ -116,7 +116,8 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t *keys, 
 // --------------------------------------------------------------- rebin
 // Key of every deformable point from its current position.  FP64 state:
 // cell = trunc(x/h) (GridPoint_Mediator.h:35).  FP32 state: the brick origin
-// of the point plus floor(x_rel/h); from_abs uses the FP64 staging positions.
+// of the point plus the integer part of its fixed-point position; from_abs
+// uses the FP64 staging positions.
 template <typename real>
 __global__ void __launch_bounds__(kBlock) k_rebin_keys(GridParams g, PArrays<real> p,
                                                       const double *xabs0, const double *xabs1,
@@ -142,9 +143,7 @@ __global__ void __launch_bounds__(kBlock) k_rebin_keys(GridParams g, PArrays<rea
             const uint32_t pb = p.pbrick[i];
             const int bc = (a == 0) ? (int)(pb % g.nb[0]) : (a == 1) ? (int)((pb / g.nb[0]) % g.nb[1])
                                                                    : (int)(pb / (g.nb[0] * g.nb[1]));
-            const float u = (float)p.x[a][i] / g.hf[a];
-            if (!(u == u)) bad = true;
-            c = bc * g.S[a] + (int)floorf(u);
+            c = bc * g.S[a] + (p.xq[a][i] >> kQFrac);   // exact floor of the fixed-point cell
         }
         if (c < 0 || c >= g.cells[a]) { bad = true; c = c < 0 ? 0 : g.cells[a] - 1; }
         cell[a] = c;
@@ -193,17 +192,15 @@ __global__ void __launch_bounds__(kBlock) k_rebin_pos(GridParams g, PArrays<real
     for (int a = 0; a < 3; ++a) {
         if (sizeof(real) == 8) {
             dst.x[a][j] = xabs0 ? (real)xa[a][i] : src.x[a][i];
+        } else if (xabs0) {
+            const double u = xa[a][i] / g.h[a] - (double)(nbc[a] * g.S[a]);
+            dst.xq[a][j] = (int32_t)rint(u * kQOne);
         } else {
-            double xabs;
-            if (xabs0) {
-                xabs = xa[a][i];
-            } else {
-                const uint32_t pb = src.pbrick[i];
-                const int bc = (a == 0) ? (int)(pb % g.nb[0]) : (a == 1) ? (int)((pb / g.nb[0]) % g.nb[1])
-                                                                       : (int)(pb / (g.nb[0] * g.nb[1]));
-                xabs = (double)(bc * g.S[a]) * g.h[a] + (double)src.x[a][i];
-            }
-            dst.x[a][j] = (real)(xabs - (double)(nbc[a] * g.S[a]) * g.h[a]);
+            const uint32_t pb = src.pbrick[i];
+            const int bc = (a == 0) ? (int)(pb % g.nb[0]) : (a == 1) ? (int)((pb / g.nb[0]) % g.nb[1])
+                                                                   : (int)(pb / (g.nb[0] * g.nb[1]));
+            // exact re-basing: shift by whole cells
+            dst.xq[a][j] = src.xq[a][i] + (int32_t)((unsigned)((bc - nbc[a]) * g.S[a]) << kQFrac);
         }
     }
 }
@@ -255,6 +252,23 @@ __global__ void k_compact(const uint32_t *flag, const uint32_t *pos, uint32_t n,
     list[pos[j]] = j;
 }
 
+// 27 neighbour slots of every grid-update brick, offsets (dz, dy, dx) ascending
+__global__ void k_k2_neighbors(GridParams g, const uint32_t *k2_list, uint32_t n_k2, const int32_t *brick_slot,
+                               int32_t *k2_nb)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 27u * n_k2) return;
+    const uint32_t kb = t / 27u;
+    const int d = (int)(t % 27u);
+    const uint32_t b = k2_list[kb];
+    const int x = (int)(b % g.nb[0]) + d % 3 - 1, y = (int)((b / g.nb[0]) % g.nb[1]) + (d / 3) % 3 - 1,
+              z = (int)(b / (g.nb[0] * g.nb[1])) + d / 9 - 1;
+    int sl = -1;
+    if (x >= 0 && y >= 0 && z >= 0 && x < g.nb[0] && y < g.nb[1] && z < g.nb[2])
+        sl = brick_slot[x + g.nb[0] * (y + g.nb[1] * z)];
+    k2_nb[t] = sl;
+}
+
 // initial platen override marks for the first step after upload
 __global__ void k_dc_marks(GridParams g, const double *dc_x, int n_dc, uint32_t epoch,
                            unsigned long long *marks)
@@ -285,12 +299,14 @@ __global__ void __launch_bounds__(kBlock) k_readback(GridParams g, PArrays<real>
     const uint32_t id = p.id[j];
     if (mask & 1u) {
         for (int a = 0; a < 3; ++a) {
-            double xa = (double)p.x[a][j];
+            double xa;
             if (sizeof(real) == 4) {
                 const uint32_t pb = p.pbrick[j];
                 const int bc = (a == 0) ? (int)(pb % g.nb[0]) : (a == 1) ? (int)((pb / g.nb[0]) % g.nb[1])
                                                                        : (int)(pb / (g.nb[0] * g.nb[1]));
-                xa += (double)(bc * g.S[a]) * g.h[a];
+                xa = ((double)(bc * g.S[a]) + (double)p.xq[a][j] / kQOne) * g.h[a];
+            } else {
+                xa = (double)p.x[a][j];
             }
             o.x[3 * (size_t)id + a] = xa;
         }
@@ -306,27 +322,24 @@ __global__ void __launch_bounds__(kBlock) k_readback(GridParams g, PArrays<real>
     if (mask & 128u) o.vol[id] = det3(Fd) * (double)p.vol0[j];
 }
 
-// FP64 sums over the per-block partials, fixed order (mpm_checks)
-__global__ void k_finalize(const double *kp_part, const int32_t *n_occ, const double *k2_part,
-                           const int32_t *n_k2, const double *dc_v, const double *dc_m, int n_dc,
-                           double *out)
+// FP64 sums over the per-block partials (mpm_checks): one block of 256,
+// strided partial sums then a fixed tree -- deterministic.
+__global__ void __launch_bounds__(256) k_finalize(const double *kp_part, const int32_t *n_occ, const double *k2_part,
+                                                 const int32_t *n_k2, const double *dc_v, const double *dc_m,
+                                                 int n_dc, double *out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double s[8][256];
+    const int t = threadIdx.x;
     double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int no = *n_occ, nk = *n_k2;
-    for (int s = 0; s < nk; ++s) {
-        r[0] += k2_part[3 * s];
-        r[1] += k2_part[3 * s + 1];
-        r[7] += k2_part[3 * s + 2];
+    for (int i = t; i < nk; i += 256) {
+        r[0] += k2_part[3 * i];
+        r[1] += k2_part[3 * i + 1];
+        r[7] += k2_part[3 * i + 2];
     }
-    for (int s = 0; s < no; ++s) {
-        r[2] += kp_part[5 * s];
-        r[3] += kp_part[5 * s + 1];
-        r[4] += kp_part[5 * s + 2];
-        r[5] += kp_part[5 * s + 3];
-        r[6] += kp_part[5 * s + 4];
-    }
-    for (int j = 0; j < n_dc; ++j) {
+    for (int i = t; i < no; i += 256)
+        for (int k = 0; k < 5; ++k) r[2 + k] += kp_part[5 * i + k];
+    for (int j = t; j < n_dc; j += 256) {
         const double *v = &dc_v[3 * j];
         const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         r[2] += dc_m[j] * (lv * lv);
@@ -334,7 +347,14 @@ __global__ void k_finalize(const double *kp_part, const int32_t *n_occ, const do
         r[4] += dc_m[j] * v[1];
         r[5] += dc_m[j] * v[2];
     }
-    for (int k = 0; k < 8; ++k) out[k] = r[k];
+    for (int k = 0; k < 8; ++k) s[k][t] = r[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 8; ++k) s[k][t] += s[k][t + w];
+        __syncthreads();
+    }
+    if (t < 8) out[t] = s[t][0];
 }
 
 }  // namespace mpm

@@ -3,9 +3,9 @@
 // Layout in HBM (one handle):
 //   deformable points: structure-of-arrays, one array per scalar field, kept in
 //     brick-binned order (bricks of S = (Sx,Sy,Sz) grid cells); FP64 mode stores
-//     absolute positions, FP32 mode stores positions relative to the origin of
-//     the point's brick (keeps FP32 resolution at ~1e-7 of a brick, not of the
-//     whole grid).
+//     absolute positions, FP32 mode stores positions as Q6.26 fixed point in
+//     cell units relative to the origin of the point's brick (uniform 1.5e-8
+//     cell resolution; re-basing at a rebin is an exact integer shift).
 //   displacement-controlled points (platens): FP64 SoA, unsorted.
 //   grid nodes: dense lattice (cells+1 per axis); per node an 8-word record
 //     {v.xyz, a.x | a.y, a.z, -, -} written by the grid kernel, read by G2P.
@@ -21,6 +21,8 @@ namespace mpm {
 constexpr int kBlock = 256;          // threads per particle workgroup
 constexpr int kQ = 7;                // node fields: m, p.xyz, f.xyz
 constexpr uint32_t kNoBrick = 0xFFFFFFFFu;
+constexpr int kQFrac = 26;                       // fractional bits of FP32 positions
+constexpr double kQOne = 67108864.0;             // 2^26
 
 enum : int {
     FLAG_REBIN = 1,        // a point drifted past half a cell outside its brick
@@ -33,6 +35,7 @@ struct GridParams {
     double L[3];
     double h[3];
     float hf[3];
+    float ihf[3];  // 1/h in FP32
     int cells[3];
     int nodes[3];
     int S[3];      // brick size in cells
@@ -59,11 +62,18 @@ struct Material {
     int32_t type;
     int32_t pad;
     double E, nu, sy, C0, C1;
+    // derived on the host with the reference's own expressions
+    double C;      // E/(1+nu)/(1-2nu)   (…Elastic.cpp:6)
+    double G;      // 0.5*E/(1+nu)       (…PerfectlyPlastic.cpp:54)
+    double k0;     // sy/sqrt(3)         (…PerfectlyPlastic.cpp:39)
+    double invE, invG, hs;   // 1/E, 1/G, sqrt(2/3)*C1
 };
 
 template <typename real>
 struct PArrays {
-    real *x[3], *v[3], *F[9], *s[6], *ep[6], *eps[6];
+    real *x[3];        // FP64 state: absolute position
+    int32_t *xq[3];    // FP32 state: Q6.26 cells relative to the brick origin
+    real *v[3], *F[9], *s[6], *ep[6], *eps[6];
     real *alpha, *mass, *vol0, *E, *fext[3];
     uint8_t *mat;
     uint32_t *id, *pbrick;

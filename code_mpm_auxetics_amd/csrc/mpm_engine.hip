@@ -73,8 +73,10 @@ struct mpm_handle {
     uint32_t *occ_list = nullptr, *occ_start = nullptr, *occ_count = nullptr;
     int32_t *n_occ_d = nullptr, *n_k2_d = nullptr, *brick_slot = nullptr;
     uint32_t *k2_list = nullptr, *k2flag = nullptr, *k2pos = nullptr;
+    int32_t *k2_nb = nullptr;               // per grid brick: 27 neighbour slots
     int n_occ = 0, n_k2 = 0, occ_cap = 0, k2_cap = 0;
     void *arena[2] = {nullptr, nullptr};
+    int32_t *box = nullptr;                 // per slot touched node box (6 ints)
     size_t arena_cap_slots = 0;
     int parity = 0;
 
@@ -252,7 +254,10 @@ static int alloc_points(mpm_handle *h, int n)
         std::memset(&p, 0, sizeof(p));
         std::vector<std::pair<void **, int>> fl;
         field_list<real>(h, p, fl);
-        for (int a = 0; a < 3; ++a) fl.push_back({(void **)&p.x[a], (int)sizeof(real)});
+        for (int a = 0; a < 3; ++a) {
+            if (sizeof(real) == 4) fl.push_back({(void **)&p.xq[a], 4});
+            else fl.push_back({(void **)&p.x[a], 8});
+        }
         fl.push_back({(void **)&p.pbrick, 4});
         for (auto &f : fl) {
             RCHK(dev_alloc(h, f.first, (size_t)std::max(n, 1) * f.second));
@@ -269,9 +274,11 @@ static int ensure_slots(mpm_handle *h, int n_occ, int n_k2)
     if ((size_t)n_occ > h->arena_cap_slots || h->arena[0] == nullptr) {
         for (int b = 0; b < 2; ++b) dev_free(h->arena[b]);
         dev_free((void *&)h->kp_part);
+        dev_free((void *&)h->box);
         h->arena_cap_slots = (size_t)want_occ;
         for (int b = 0; b < 2; ++b)
             RCHK(dev_alloc(h, &h->arena[b], h->arena_cap_slots * kQ * h->g.arenaN * h->rsize));
+        RCHK(dalloc(h, &h->box, h->arena_cap_slots * 6));
         RCHK(dalloc(h, &h->kp_part, h->arena_cap_slots * 5));
         HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * 8, h->stream));
         h->occ_cap = want_occ;
@@ -387,6 +394,8 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     k_compact<<<cdiv(h->nbricks, kBlock), kBlock, 0, h->stream>>>(h->k2flag, h->k2pos, (uint32_t)h->nbricks, h->k2_list);
     HIPCHK(h, hipGetLastError());
     RCHK(ensure_slots(h, (int)n_occ, (int)n_k2));
+    k_k2_neighbors<<<cdiv((size_t)27 * n_k2, kBlock), kBlock, 0, h->stream>>>(h->g, h->k2_list, n_k2, h->brick_slot, h->k2_nb);
+    HIPCHK(h, hipGetLastError());
     h->n_occ = (int)n_occ;
     h->n_k2 = (int)n_k2;
     int32_t cnt[2] = {(int32_t)n_occ, (int32_t)n_k2};
@@ -415,6 +424,7 @@ static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index)
     a.nodes = (const NodeRec<real> *)h->nodes;
     // G2P(n)+P2G(n+1) writes the arena of step n+1; a rebuild writes step n's
     a.arena_out = (real *)h->arena[g2p ? (h->parity ^ 1) : h->parity];
+    a.box_out = h->box;
     a.occ_list = h->occ_list;
     a.occ_start = h->occ_start;
     a.occ_count = h->occ_count;
@@ -451,7 +461,9 @@ static int launch_grid(mpm_handle *h, int step)
     K2Args<real> a{};
     a.g = h->g;
     a.arena_in = (const real *)h->arena[h->parity];
+    a.box_in = h->box;
     a.brick_slot = h->brick_slot;
+    a.k2_nb = h->k2_nb;
     a.k2_list = h->k2_list;
     a.n_k2 = h->n_k2_d;
     a.k2_cap = h->k2_cap;
@@ -461,7 +473,7 @@ static int launch_grid(mpm_handle *h, int step)
     a.dc_v = h->dc_v;
     a.steps = h->steps_d;
     a.k2_part = h->k2_part;
-    k_grid<real><<<(unsigned)h->k2_cap, 64, 0, h->stream>>>(a, step);
+    k_grid<real><<<(unsigned)h->k2_cap, 256, 0, h->stream>>>(a, step);
     HIPCHK(h, hipGetLastError());
     return MPM_OK;
 }
@@ -582,7 +594,7 @@ static int step_impl(mpm_handle *h, double dt_total, int32_t *finished)
 
 static int checks_impl(mpm_handle *h, double out[12])
 {
-    k_finalize<<<1, 64, 0, h->stream>>>(h->kp_part, h->n_occ_d, h->k2_part, h->n_k2_d, h->dc_v, h->dc_m,
+    k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, h->k2_part, h->n_k2_d, h->dc_v, h->dc_m,
                                        h->n_dc, h->checks_d);
     HIPCHK(h, hipGetLastError());
     double r[8] = {0};
@@ -625,7 +637,7 @@ static void destroy_all(mpm_handle *h)
                      (void **)&h->k2flag, (void **)&h->k2pos, (void **)&h->mats, (void **)&h->steps_d,
                      (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->k2_part, (void **)&h->hist_d,
                      (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
-                     (void **)&h->headpos, (void **)&h->gdesc};
+                     (void **)&h->headpos, (void **)&h->gdesc, (void **)&h->box, (void **)&h->k2_nb};
     for (void **p : ptrs) dev_free(*p);
     for (auto *p : h->scan_scratch) { void *q = p; dev_free(q); }
     h->scan_scratch.clear();
@@ -693,7 +705,8 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         g.nodes[a] = cfg->cells[a] + 1;
         g.h[a] = cfg->length[a] / (double)cfg->cells[a];
         g.hf[a] = (float)g.h[a];
-        int S = cfg->brick[a] > 0 ? cfg->brick[a] : (a == 0 ? 8 : a == 1 ? 4 : 2);
+        g.ihf[a] = (float)(1.0 / g.h[a]);
+        int S = cfg->brick[a] > 0 ? cfg->brick[a] : (a == 2 ? 10 : 4);
         S = std::min(S, std::max(1, cfg->cells[a]));
         g.S[a] = S;
         g.nb[a] = g.nodes[a] / S + ((g.nodes[a] % S) ? 1 : 0);
@@ -730,7 +743,7 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     const size_t nb = (size_t)h->nbricks;
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
         (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->n_k2_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) ||
-        (rc = dalloc(h, &h->k2_list, nb)) || (rc = dalloc(h, &h->k2flag, nb)) || (rc = dalloc(h, &h->k2pos, nb)) ||
+        (rc = dalloc(h, &h->k2_list, nb)) || (rc = dalloc(h, &h->k2_nb, 27 * nb)) || (rc = dalloc(h, &h->k2flag, nb)) || (rc = dalloc(h, &h->k2pos, nb)) ||
         (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)))
         return fail(rc);
     (void)hipMemsetAsync(h->n_occ_d, 0, 4, h->stream);
@@ -792,6 +805,12 @@ static int upload_t(mpm_handle *h, const mpm_points *pts)
         m.C0 = std::get<3>(kv.first);
         m.C1 = std::get<4>(kv.first);
         m.E = std::get<5>(kv.first);
+        m.C = m.E / (1.0 + m.nu) / (1.0 - 2.0 * m.nu);
+        m.G = 0.5 * m.E / (1.0 + m.nu);
+        m.k0 = m.sy / std::sqrt(3.0);
+        m.invE = 1.0 / m.E;
+        m.invG = 1.0 / m.G;
+        m.hs = std::sqrt(2.0 / 3.0) * m.C1;
         mats[kv.second] = m;
     }
     RCHK(dalloc(h, &h->mats, mats.size()));

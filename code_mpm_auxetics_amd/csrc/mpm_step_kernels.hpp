@@ -19,6 +19,15 @@
 #pragma once
 #include "mpm_device.hpp"
 
+// register budget of the fused particle kernel: waves per SIMD (3 -> 168
+// VGPRs, 2 -> 256); FP64 state needs the larger budget to avoid spills
+#ifndef MPM_KP_WAVES32
+#define MPM_KP_WAVES32 3
+#endif
+#ifndef MPM_KP_WAVES64
+#define MPM_KP_WAVES64 2
+#endif
+
 namespace mpm {
 
 // ------------------------------------------------------------- constitutive
@@ -85,6 +94,45 @@ __device__ __forceinline__ bool radial_return(double E, double nu, double radius
     return true;
 }
 
+// The same radial return for FP32 state: FP64 arithmetic, material constants
+// precomputed per material (reciprocals instead of the reference's divisions).
+// Returns true on the plastic branch; s is updated in place (rounded once).
+__device__ __forceinline__ bool radial_return_fp32(const Material &mt, double C, double radius, float s[6],
+                                                   const float de[6], double dep[6])
+{
+    const double nu = mt.nu;
+    double st[6];
+    st[0] = (double)s[0] + C * ((1.0 - nu) * de[0] + nu * de[1] + nu * de[2]);
+    st[1] = (double)s[1] + C * ((1.0 - nu) * de[1] + nu * de[0] + nu * de[2]);
+    st[2] = (double)s[2] + C * ((1.0 - nu) * de[2] + nu * de[0] + nu * de[1]);
+    st[3] = (double)s[3] + C * ((0.5 - nu) * de[3]);
+    st[4] = (double)s[4] + C * ((0.5 - nu) * de[4]);
+    st[5] = (double)s[5] + C * ((0.5 - nu) * de[5]);
+    const double j2 = j2_of(st);
+    if (j2 <= radius * radius) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = (float)st[i];
+        return false;
+    }
+    const double p = 1.0 / 3.0 * (st[0] + st[1] + st[2]);
+    const double sc = radius / sqrt(j2);
+    double ds[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double snew = (i < 3) ? (st[i] - p) * sc + p : st[i] * sc;
+        ds[i] = snew - (double)s[i];
+        s[i] = (float)snew;
+    }
+    const double ie = mt.invE, ig = mt.invG;
+    dep[0] = de[0] - (ds[0] - nu * (ds[1] + ds[2])) * ie;
+    dep[1] = de[1] - (ds[1] - nu * (ds[0] + ds[2])) * ie;
+    dep[2] = de[2] - (ds[2] - nu * (ds[0] + ds[1])) * ie;
+    dep[3] = de[3] - ds[3] * ig;
+    dep[4] = de[4] - ds[4] * ig;
+    dep[5] = de[5] - ds[5] * ig;
+    return true;
+}
+
 // ConstitutiveRelation_calculateIncrement_NeoHookean.cpp:4-22, FP64.
 __device__ __forceinline__ void neohookean_inc(double E, double nu, const double s[6],
                                                const double F[9], double ds[6])
@@ -112,76 +160,91 @@ __device__ __forceinline__ void neohookean_inc(double E, double nu, const double
 }
 
 // ------------------------------------------------------------------ locate
-// FP64: absolute position, neighbour search and node positions exactly as
-// GridPoint_Mediator.h:35 (trunc of x/h) and GridPoint_Factory.h:40-42 (i*L/n).
-__device__ __forceinline__ void locate_axis(double x, int a, const GridParams &g, int /*O*/,
-                                            int &cell, double &d0, double &d1, double &u)
-{
-    int i0 = (int)(x / g.h[a]);
-    cell = i0;
-    double xi0 = ((double)i0 * g.L[a]) / (double)g.cells[a];
-    double xi1 = ((double)(i0 + 1) * g.L[a]) / (double)g.cells[a];
-    d0 = x - xi0;
-    d1 = x - xi1;
-    u = x / g.h[a];
-    if (!(x >= 0.0)) cell = -1;   // out of grid (or NaN)
-}
-// FP32: position relative to the brick origin O (in cells).
-__device__ __forceinline__ void locate_axis(float xr, int a, const GridParams &g, int O,
-                                            int &cell, float &d0, float &d1, float &u)
-{
-    float h = g.hf[a];
-    u = xr / h;
-    int ci = (int)floorf(u);
-    cell = O + ci;
-    d0 = xr - (float)ci * h;
-    d1 = xr - (float)(ci + 1) * h;
-    u = u + (float)O;
-    if (!(u == u)) cell = -1;
-}
-
+// Per-axis linear-hat data of one point: shape values of the lower/upper node
+// (s0, s1), their gradient factors (dN/dx of that axis' hat) and the cell.
 template <typename real>
-struct Bases {
-    real s[3][2];   // per-axis shape values for node offset 0/1
-    real sg[3][2];  // sign(d)
-    real h[3];      // cell size per axis (divided, as the reference)
-    int cell[3];
-    real u[3];      // continuous cell coordinate
+struct Axis {
+    real s0, s1, sg0, sg1, h;   // shape values, sign(d) of lower/upper node, cell size
+    real ih;                     // 1/h (FP32 state multiplies instead of dividing)
+    int cell;                    // global cell index
+    real u;                      // continuous cell coordinate relative to the brick origin
 };
 
-template <typename real>
-__device__ __forceinline__ void make_bases(const real x[3], const GridParams &g,
-                                           const int O[3], Bases<real> &b)
+// FP64: absolute position, neighbour search and node positions exactly as
+// GridPoint_Mediator.h:35 (trunc of x/h), GridPoint_Factory.h:40-42 (i*L/n) and
+// the bases of GridPoint_Mediator.h:64-88.
+__device__ __forceinline__ void axis_of(double x, int a, const GridParams &g, int O, Axis<double> &r)
 {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        real d0, d1, u;
-        locate_axis(x[a], a, g, O[a], b.cell[a], d0, d1, u);
-        b.u[a] = u;
-        real h = (sizeof(real) == 8) ? (real)g.h[a] : (real)g.hf[a];
-        real s0 = real(1) - fabs(d0) / h;
-        real s1 = real(1) - fabs(d1) / h;
-        if (s0 < real(0) || s0 > real(1)) s0 = real(0);
-        if (s1 < real(0) || s1 > real(1)) s1 = real(0);
-        b.s[a][0] = s0;
-        b.s[a][1] = s1;
-        b.sg[a][0] = sgn(d0);
-        b.sg[a][1] = sgn(d1);
-        b.h[a] = h;
+    const double h = g.h[a];
+    const int i0 = (int)(x / h);
+    const double xi0 = ((double)i0 * g.L[a]) / (double)g.cells[a];
+    const double xi1 = ((double)(i0 + 1) * g.L[a]) / (double)g.cells[a];
+    const double d0 = x - xi0, d1 = x - xi1;
+    double s0 = 1.0 - fabs(d0) / h, s1 = 1.0 - fabs(d1) / h;
+    if (s0 < 0.0 || s0 > 1.0) s0 = 0.0;
+    if (s1 < 0.0 || s1 > 1.0) s1 = 0.0;
+    r.s0 = s0;
+    r.s1 = s1;
+    r.sg0 = sgn(d0);
+    r.sg1 = sgn(d1);
+    r.h = h;
+    r.ih = 1.0 / h;
+    r.cell = (x >= 0.0) ? i0 : -1;   // out of grid (or NaN) -> -1
+    r.u = x / h - (double)O;
+}
+
+// FP32: Q6.26 fixed-point position in cells relative to the brick origin O.
+// Lower node at distance frac*h (s = 1 - frac), upper at (frac-1)*h (s = frac).
+__device__ __forceinline__ void axis_of(int32_t q, int a, const GridParams &g, int O, Axis<float> &r)
+{
+    const int ci = q >> kQFrac;   // floor
+    const float frac = (float)(q & ((1 << kQFrac) - 1)) * (float)(1.0 / kQOne);
+    r.s0 = 1.0f - frac;
+    r.s1 = frac;
+    r.sg0 = frac > 0.0f ? 1.0f : 0.0f;
+    r.sg1 = -1.0f;
+    r.h = g.hf[a];
+    r.ih = g.ihf[a];
+    r.cell = O + ci;
+    r.u = (float)ci + frac;
+}
+
+// N and grad N of corner k (x-outer, z-inner as GridPoint_Mediator.h:39-45),
+// evaluation order of GridPoint_Mediator.h:83-87.
+// (Axes are passed by value: an indexed array of them would be put on the
+// stack once the corner index is a run-time value.)
+template <typename real>
+__device__ __forceinline__ void corner(const Axis<real> bx, const Axis<real> by, const Axis<real> bz, int k,
+                                       real &N, real G[3])
+{
+    const bool ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
+    const real sx = ax ? bx.s1 : bx.s0, sy = ay ? by.s1 : by.s0, sz = az ? bz.s1 : bz.s0;
+    const real gx = ax ? bx.sg1 : bx.sg0, gy = ay ? by.sg1 : by.sg0, gz = az ? bz.sg1 : bz.sg0;
+    N = sx * sy * sz;
+    if constexpr (sizeof(real) == 4) {
+        G[0] = -sy * sz * gx * bx.ih;
+        G[1] = -sx * sz * gy * by.ih;
+        G[2] = -sx * sy * gz * bz.ih;
+    } else {   // FP64 state: the reference's division (GridPoint_Mediator.h:85-87)
+        G[0] = -sy * sz * gx / bx.h;
+        G[1] = -sx * sz * gy / by.h;
+        G[2] = -sx * sy * gz / bz.h;
     }
 }
 
-// N and grad N of corner k (x-outer, z-inner as GridPoint_Mediator.h:39-45);
-// GridPoint_Mediator.h:83-87 evaluation order.
 template <typename real>
-__device__ __forceinline__ void corner(const Bases<real> &b, int k, real &N, real G[3])
+__device__ __forceinline__ void axes_of(const real x[3], const int32_t xq[3], const GridParams &g, const int O[3],
+                                        Axis<real> &bx, Axis<real> &by, Axis<real> &bz)
 {
-    int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
-    real sx = b.s[0][ax], sy = b.s[1][ay], sz = b.s[2][az];
-    N = sx * sy * sz;
-    G[0] = -sy * sz * b.sg[0][ax] / b.h[0];
-    G[1] = -sx * sz * b.sg[1][ay] / b.h[1];
-    G[2] = -sx * sy * b.sg[2][az] / b.h[2];
+    if constexpr (sizeof(real) == 4) {
+        axis_of(xq[0], 0, g, O[0], bx);
+        axis_of(xq[1], 1, g, O[1], by);
+        axis_of(xq[2], 2, g, O[2], bz);
+    } else {
+        axis_of(x[0], 0, g, O[0], bx);
+        axis_of(x[1], 1, g, O[1], by);
+        axis_of(x[2], 2, g, O[2], bz);
+    }
 }
 
 // ------------------------------------------------------- block reductions
@@ -192,11 +255,22 @@ __device__ __forceinline__ T wave_sum(T v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-template <typename T>
-__device__ __forceinline__ T wave_max(T v)
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v)
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_imin(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_imax(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
 
@@ -208,6 +282,7 @@ struct KpArgs {
     PArrays<real> p;
     const NodeRec<real> *nodes;
     real *arena_out;
+    int32_t *box_out;     // per slot: touched local node box x0,x1,y0,y1,z0,z1
     const uint32_t *occ_list, *occ_start, *occ_count;
     const int32_t *n_occ;
     int32_t occ_cap;
@@ -224,26 +299,35 @@ struct KpArgs {
     int32_t hist_index;
 };
 
+// One step of a row-segmented inclusive scan: lane l adds lane l-D when l-D
+// lies in the same run (run start sst, row lane rl).
 template <int D, typename real>
-__device__ __forceinline__ void seg_step(real (&c)[8 * kQ], int rl, int st)
+__device__ __forceinline__ void seg_step(real (&c)[kQ], int rl, int sst)
 {
-    const bool take = (rl - D) >= st;
+    const real take = ((rl - D) >= sst) ? real(1) : real(0);
 #pragma unroll
-    for (int i = 0; i < 8 * kQ; ++i) {
-        real t = dpp_r<MPM_ROW_SHR(D)>(c[i]);
-        c[i] += take ? t : real(0);
-    }
+    for (int i = 0; i < kQ; ++i) c[i] = fma(dpp_r<MPM_ROW_SHR(D)>(c[i]), take, c[i]);
 }
+
+// Fixed-point fraction bits.  FP32 state: run sums are converted to 32-bit
+// integers scaled so that the chunk's largest contribution bound is < 2^26
+// (run sums of <= 16 lanes stay < 2^30), then added to the 64-bit arena.
+// FP64 state: 64-bit conversion with FRAC = 61 - headroom bits.
+constexpr int kFrac32 = 26;
+
+__device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
 
 // One brick per workgroup.  Dynamic LDS: long long arena[7 * arenaN].
 template <typename real, bool G2P>
-__global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
+__global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : MPM_KP_WAVES64) k_particle(KpArgs<real> a, int step)
 {
     extern __shared__ long long s_arena[];
-    __shared__ double s_wmax[kBlock / 64][kQ];
-    __shared__ int s_e[kQ], s_shift[kQ], s_any;
+    __shared__ uint32_t s_wmax[kBlock / 64][kQ];
+    __shared__ int s_e[kQ], s_shift[kQ];
+    __shared__ int s_box[kBlock / 64][6];
     __shared__ double s_red[kBlock / 64][5];
 
+    constexpr bool FP32 = sizeof(real) == 4;
     const GridParams &g = a.g;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, rl = lane & 15;
     const StepRec st = a.steps[step];
@@ -253,7 +337,7 @@ __global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
     // ---- platen (displacement-controlled) blocks: phase 8 + override marks
     if ((int)blockIdx.x >= a.occ_cap) {
         if (!G2P) return;
-        int j = ((int)blockIdx.x - a.occ_cap) * kBlock + tid;
+        const int j = ((int)blockIdx.x - a.occ_cap) * kBlock + tid;
         if (j >= a.n_dc) return;
         double *x = &a.dc_x[3 * j], *v = &a.dc_v[3 * j];
         // …_MP.cpp:360-368: velocity from the TimeLine at the step's start time
@@ -261,15 +345,14 @@ __global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
         for (int q = 0; q < 3; ++q) x[q] += v[q] * dtd;
         // override marks for step n+1 (…_MP.cpp:245-265): the highest platen
         // index touching a node wins, as in the reference's sequential loop.
+        // The tag carries epoch (n+1)+1 so that 0 means "no mark".
         int i0[3];
         for (int q = 0; q < 3; ++q) i0[q] = (int)(x[q] / g.h[q]);
-        // mark for step n+1 carries epoch tag (n+1)+1; 0 means "no mark"
-        unsigned long long tag = ((unsigned long long)(st.epoch + 2u) << 32) | (unsigned)j;
+        const unsigned long long tag = ((unsigned long long)(st.epoch + 2u) << 32) | (unsigned)j;
         for (int k = 0; k < 8; ++k) {
-            int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
+            const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
             if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= g.nodes[0] || n1 >= g.nodes[1] || n2 >= g.nodes[2]) continue;
-            long long nid = n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2);
-            atomicMax(&a.marks[nid], tag);
+            atomicMax(&a.marks[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
         }
         return;
     }
@@ -277,15 +360,15 @@ __global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
     const int slot = blockIdx.x;
     if (slot >= *a.n_occ) return;
     const uint32_t brick = a.occ_list[slot];
-    const int bx = brick % g.nb[0], by = (brick / g.nb[0]) % g.nb[1], bz = brick / (g.nb[0] * g.nb[1]);
-    const int O[3] = {bx * g.S[0], by * g.S[1], bz * g.S[2]};
+    const int O[3] = {(int)(brick % g.nb[0]) * g.S[0], (int)((brick / g.nb[0]) % g.nb[1]) * g.S[1],
+                      (int)(brick / (g.nb[0] * g.nb[1])) * g.S[2]};
     const uint32_t start = a.occ_start[slot], count = a.occ_count[slot];
     const int A0 = g.A[0], A01 = g.A[0] * g.A[1], AN = g.arenaN;
 
-    // reaction history of the previous grid update (…PhysicsEngine.cpp:65-71)
+    // reaction history of the preceding grid update (PhysicsEngine.cpp:65-71)
     if (G2P && slot == 0 && wv == 0) {
         double r = 0.0;
-        int nk = *a.n_k2;
+        const int nk = *a.n_k2;
         for (int i = lane; i < nk; i += 64) r += a.k2_part[3 * i + 1];
         r = wave_sum(r);
         if (lane == 0) a.hist[a.hist_index] = r;
@@ -293,64 +376,82 @@ __global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
 
     for (int i = tid; i < kQ * AN; i += kBlock) s_arena[i] = 0;
     if (tid < kQ) s_e[tid] = kUnsetExp;
-    int H = 5;
-    while ((1u << H) <= count) ++H;
-    const int FRAC = 61 - H;
-    double red[5] = {0, 0, 0, 0, 0};
+    int H = 6;
+    while ((1u << (H - 1)) <= count) ++H;   // >= log2(count) + 1 bits of headroom
+    const int FRAC = FP32 ? kFrac32 : 61 - H;
+    double wred[5] = {0, 0, 0, 0, 0};   // lane 0: this wave's running check sums
     int flags = 0;
+    int bmin[3] = {1 << 20, 1 << 20, 1 << 20}, bmax[3] = {-1, -1, -1};
+    const PArrays<real> &P = a.p;
+    // FP32 position increment scale: dt / h * 2^26 per axis
+    const float qscale[3] = {(float)(dtd * (double)g.ihf[0] * kQOne), (float)(dtd * (double)g.ihf[1] * kQOne),
+                             (float)(dtd * (double)g.ihf[2] * kQOne)};
     __syncthreads();
 
-    const PArrays<real> &P = a.p;
     for (uint32_t c0 = 0; c0 < count; c0 += kBlock) {
         const bool active = (c0 + tid) < count;
         const uint32_t i = start + c0 + tid;
-        real c[8 * kQ];
-#pragma unroll
-        for (int k = 0; k < 8 * kQ; ++k) c[k] = real(0);
         int key = -1;
+        double red[5] = {0, 0, 0, 0, 0};
+        Axis<real> bx, by, bz;
+        real m = real(0), mv[3] = {real(0), real(0), real(0)}, vs[6], fe[3] = {real(0), real(0), real(0)};
+        uint32_t bq[kQ] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) vs[q] = real(0);
 
         if (active) {
             real x[3], v[3], F[9], s[6];
+            int32_t xq[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) { x[q] = P.x[q][i]; v[q] = P.v[q][i]; }
+            for (int q = 0; q < 3; ++q) {
+                if (FP32) xq[q] = P.xq[q][i];
+                else x[q] = P.x[q][i];
+                v[q] = P.v[q][i];
+            }
 #pragma unroll
             for (int q = 0; q < 9; ++q) F[q] = P.F[q][i];
 #pragma unroll
             for (int q = 0; q < 6; ++q) s[q] = P.s[q][i];
-            const real m = P.mass[i], V0 = P.vol0[i];
+            m = P.mass[i];
+            const real V0 = P.vol0[i];
+            real V = real(0);
 
             if (G2P) {
                 // ---- G2P of step n (…_MP.cpp:279-297)
-                Bases<real> b;
-                make_bases(x, g, O, b);
-                bool oob = false;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) oob |= (b.cell[q] < 0) || (b.cell[q] >= g.cells[q]);
+                axes_of(x, xq, g, O, bx, by, bz);
+                const bool oob = bx.cell < 0 || bx.cell >= g.cells[0] || by.cell < 0 || by.cell >= g.cells[1] ||
+                                 bz.cell < 0 || bz.cell >= g.cells[2];
                 if (oob) {
                     flags |= FLAG_OUT_OF_GRID;
                 } else {
-                    const long long nb0 = b.cell[0] + (long long)g.nodes[0] * (b.cell[1] + (long long)g.nodes[1] * b.cell[2]);
+                    const long long nb0 = bx.cell + (long long)g.nodes[0] * (by.cell + (long long)g.nodes[1] * bz.cell);
                     real L[9];
 #pragma unroll
                     for (int q = 0; q < 9; ++q) L[q] = real(0);
-                    real xn[3] = {x[0], x[1], x[2]};
-#pragma unroll
+                    real dx[3] = {real(0), real(0), real(0)};
+#pragma unroll 2
                     for (int k = 0; k < 8; ++k) {
                         real N, G[3];
-                        corner(b, k, N, G);
+                        corner(bx, by, bz, k, N, G);
                         const long long nid = nb0 + ((k >> 2) & 1) + (long long)g.nodes[0] * (((k >> 1) & 1) + (long long)g.nodes[1] * (k & 1));
                         const NodeRec<real> nr = a.nodes[nid];
 #pragma unroll
                         for (int q = 0; q < 3; ++q) v[q] += N * nr.d[3 + q] * dt;
+                        if (FP32) {
 #pragma unroll
-                        for (int q = 0; q < 3; ++q) xn[q] += N * nr.d[q] * dt;
+                            for (int q = 0; q < 3; ++q) dx[q] += N * nr.d[q];
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) x[q] += N * nr.d[q] * dt;
+                        }
 #pragma unroll
                         for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
                             for (int r = 0; r < 3; ++r) L[3 * cc + r] += nr.d[r] * G[cc];
                     }
+                    if (FP32)
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) x[q] = xn[q];
+                        for (int q = 0; q < 3; ++q) xq[q] += __float2int_rn((float)dx[q] * qscale[q]);
                     // F += (L F) dt  (…_MP.cpp:299)
                     real T[9];
 #pragma unroll
@@ -360,9 +461,25 @@ __global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
                             T[3 * cc + r] = L[r] * F[3 * cc] + L[3 + r] * F[3 * cc + 1] + L[6 + r] * F[3 * cc + 2];
 #pragma unroll
                     for (int q = 0; q < 9; ++q) F[q] += T[q] * dt;
-                    // strain increment (…_MP.cpp:304-312)
+                    // volume from the new F (…_MP.cpp:301-302), FP64
+                    {
+                        double Fd[9];
+#pragma unroll
+                        for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
+                        V = (real)(det3(Fd) * (double)V0);
+                    }
+                    // write back the kinematic state before the material update
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        if (FP32) P.xq[q][i] = xq[q];
+                        else P.x[q][i] = x[q];
+                        P.v[q][i] = v[q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) P.F[q][i] = F[q];
+                    // strain increment (…_MP.cpp:304-312); FP32 state uses L dt
                     real de[6];
-                    if (sizeof(real) == 8 && g.strain_form == 0) {
+                    if (!FP32 && g.strain_form == 0) {
                         de[0] = (real(1) + L[0] * dt) - real(1);
                         de[1] = (real(1) + L[4] * dt) - real(1);
                         de[2] = (real(1) + L[8] * dt) - real(1);
@@ -378,127 +495,138 @@ __global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
 #pragma unroll
                         for (int q = 0; q < 6; ++q) P.eps[q][i] += de[q];
                     // constitutive dispatch (…_MP.cpp:322-353)
-                    const Material mt = a.mats[P.mat[i]];
-                    const double E = g.has_pointE ? (double)P.E[i] : mt.E;
-                    const double nu = mt.nu;
+                    const Material &mt = a.mats[P.mat[i]];
                     int type = mt.type;
                     if (type == 1 && !g.neohookean) type = 3;
                     if (type == 2 || type == 5) type = 3;
+                    const double E = g.has_pointE ? (double)P.E[i] : mt.E;
                     if (type == 0) {
                         real ds[6];
-                        elastic_inc<real>((real)E, (real)nu, de, ds);
+                        elastic_inc<real>((real)E, (real)mt.nu, de, ds);
 #pragma unroll
                         for (int q = 0; q < 6; ++q) s[q] += ds[q];
                     } else if (type == 1) {
                         double Fd[9], sd[6], ds[6];
                         for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
                         for (int q = 0; q < 6; ++q) sd[q] = (double)s[q];
-                        neohookean_inc(E, nu, sd, Fd, ds);
+                        neohookean_inc(E, mt.nu, sd, Fd, ds);
                         for (int q = 0; q < 6; ++q) s[q] = (real)(sd[q] + ds[q]);
+                    } else if constexpr (FP32) {
+                        // FP64 island for FP32 state
+                        double dep[6];
+                        const double C = g.has_pointE ? E / (1.0 + mt.nu) / (1.0 - 2.0 * mt.nu) : mt.C;
+                        const double al = (type == 4) ? (double)P.alpha[i] : 0.0;
+                        const double radius = mt.k0 + al / sqrt(2.0);
+                        if (radial_return_fp32(mt, C, radius, s, de, dep)) {
+                            if (type == 4) {
+                                double eff = 0.0;
+#pragma unroll
+                                for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
+                                eff = sqrt(eff * (2.0 / 3.0));
+                                const double da = ((mt.hs - al) > 0.0) ? mt.C0 * (mt.hs - al) * eff : 0.0;
+                                P.alpha[i] = (real)(al + da);
+                            }
+#pragma unroll
+                            for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
+                        }
                     } else {
+                        // FP64 state: reference evaluation order
                         double sd[6], ded[6], ds[6], dep[6], da = 0.0;
                         for (int q = 0; q < 6; ++q) { sd[q] = (double)s[q]; ded[q] = (double)de[q]; }
                         const double k0 = mt.sy / sqrt(3.0);
                         if (type == 4) {
                             const double al = (double)P.alpha[i];
-                            const double R = al / sqrt(2.0);
-                            if (radial_return(E, nu, k0 + R, sd, ded, ds, dep)) {
+                            if (radial_return(E, mt.nu, k0 + al / sqrt(2.0), sd, ded, ds, dep)) {
                                 double eff = 0.0;
                                 for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
                                 eff *= 2.0 / 3.0;
                                 eff = sqrt(eff);
                                 if ((sqrt(2.0 / 3.0) * mt.C1 - al) > 0.0)
                                     da = mt.C0 * (sqrt(2.0 / 3.0) * mt.C1 - al) * eff;
+                                P.alpha[i] = (real)(al + da);
+                                for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
                             }
-                            P.alpha[i] = (real)(al + da);
-                        } else {
-                            radial_return(E, nu, k0, sd, ded, ds, dep);
+                        } else if (radial_return(E, mt.nu, k0, sd, ded, ds, dep)) {
+                            for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
                         }
-                        for (int q = 0; q < 6; ++q) {
-                            s[q] = (real)(sd[q] + ds[q]);
-                            P.ep[q][i] += (real)dep[q];
-                        }
+                        for (int q = 0; q < 6; ++q) s[q] = (real)(sd[q] + ds[q]);
                     }
-                    // write back
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) { P.x[q][i] = x[q]; P.v[q][i] = v[q]; }
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) P.F[q][i] = F[q];
 #pragma unroll
                     for (int q = 0; q < 6; ++q) P.s[q][i] = s[q];
                 }
+            } else {
+                double Fd[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
+                V = (real)(det3(Fd) * (double)V0);
             }
             // ---- FP64 checks of the new state (PhysicsEngine.cpp:73-79)
             {
-                double vv = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                double lv = sqrt(vv);
+                const double vv = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+                const double lv = sqrt(vv);
                 red[0] += (double)m * (lv * lv);
                 red[1] += (double)m * v[0];
                 red[2] += (double)m * v[1];
                 red[3] += (double)m * v[2];
-                bool fin = isfinite((double)x[0] + x[1] + x[2]) && isfinite(vv) &&
-                           isfinite((double)s[0] + s[1] + s[2] + s[3] + s[4] + s[5]);
+                const bool fin = isfinite(vv) && isfinite((double)s[0] + s[1] + s[2] + s[3] + s[4] + s[5]) &&
+                                 (FP32 || isfinite((double)x[0] + x[1] + x[2]));
                 if (!fin) { red[4] += 1.0; flags |= FLAG_NAN; }
             }
-            // ---- P2G contributions of step n+1 (…_MP.cpp:106-185)
-            double Fd[9];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
-            const real V = (real)(det3(Fd) * (double)V0);
-            Bases<real> b;
-            make_bases(x, g, O, b);
+            // ---- P2G inputs of step n+1 (…_MP.cpp:106-185)
+            axes_of(x, xq, g, O, bx, by, bz);
+            const int cl[3] = {bx.cell, by.cell, bz.cell};
+            const real cu[3] = {bx.u, by.u, bz.u};
             int lc[3];
             bool ok = true;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                lc[q] = b.cell[q] - (O[q] - 1);
-                if (b.cell[q] < 0 || b.cell[q] >= g.cells[q]) { flags |= FLAG_OUT_OF_GRID; ok = false; }
+                lc[q] = cl[q] - (O[q] - 1);
+                if (cl[q] < 0 || cl[q] >= g.cells[q]) { flags |= FLAG_OUT_OF_GRID; ok = false; }
                 if (lc[q] < 0 || lc[q] > g.S[q] + 1) { flags |= FLAG_ARENA; ok = false; }
-                const real uu = b.u[q] - (real)O[q];
-                if (uu < real(-0.5) || uu >= (real)g.S[q] + real(0.5)) flags |= FLAG_REBIN;
+                if (cu[q] < real(-0.5) || cu[q] >= (real)g.S[q] + real(0.5)) flags |= FLAG_REBIN;
             }
             if (ok) {
                 key = lc[0] + A0 * lc[1] + A01 * lc[2];
-                real fe[3] = {real(0), real(0), real(0)};
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    bmin[q] = min(bmin[q], lc[q]);
+                    bmax[q] = max(bmax[q], lc[q] + 1);
+                }
                 if (g.has_fext)
 #pragma unroll
                     for (int q = 0; q < 3; ++q) fe[q] = P.fext[q][i];
-                const real mv[3] = {m * v[0], m * v[1], m * v[2]};
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    real N, G[3];
-                    corner(b, k, N, G);
-                    real *ck = &c[kQ * k];
-                    ck[0] = N * m;
-                    ck[1] = N * mv[0];
-                    ck[2] = N * mv[1];
-                    ck[3] = N * mv[2];
-                    ck[4] = -V * (G[0] * s[0] + G[1] * s[3] + G[2] * s[5]) + N * fe[0];
-                    ck[5] = -V * (G[1] * s[1] + G[0] * s[3] + G[2] * s[4]) + N * fe[1];
-                    ck[6] = -V * (G[2] * s[2] + G[0] * s[5] + G[1] * s[4]) + N * fe[2];
-                }
+                for (int q = 0; q < 3; ++q) mv[q] = m * v[q];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) vs[q] = V * s[q];
+                // bounds of |contribution| per field (N <= 1, |dN/dx_j| <= 1/h_j)
+                const float ih0 = (float)bx.ih, ih1 = (float)by.ih, ih2 = (float)bz.ih;
+                bq[0] = fbits(fabsf((float)m) * 1.0001f);
+                bq[1] = fbits(fabsf((float)mv[0]) * 1.0001f);
+                bq[2] = fbits(fabsf((float)mv[1]) * 1.0001f);
+                bq[3] = fbits(fabsf((float)mv[2]) * 1.0001f);
+                bq[4] = fbits((fabsf((float)vs[0]) * ih0 + fabsf((float)vs[3]) * ih1 + fabsf((float)vs[5]) * ih2 + fabsf((float)fe[0])) * 1.0001f);
+                bq[5] = fbits((fabsf((float)vs[3]) * ih0 + fabsf((float)vs[1]) * ih1 + fabsf((float)vs[4]) * ih2 + fabsf((float)fe[1])) * 1.0001f);
+                bq[6] = fbits((fabsf((float)vs[5]) * ih0 + fabsf((float)vs[4]) * ih1 + fabsf((float)vs[2]) * ih2 + fabsf((float)fe[2])) * 1.0001f);
             }
         }
 
-        // ---- per-chunk fixed-point exponent (max |contribution| per field)
-        {
-            double mx[kQ];
+        // ---- FP64 check sums: fixed-order wave reduction per chunk
 #pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                real t = real(0);
+        for (int k = 0; k < 5; ++k) wred[k] += wave_sum(red[k]);
+
+        // ---- per-chunk fixed-point exponent of each field
 #pragma unroll
-                for (int k = 0; k < 8; ++k) t = fmax(t, fabs(c[kQ * k + q]));
-                mx[q] = wave_max((double)t);
-            }
-            if (lane == 0)
-#pragma unroll
-                for (int q = 0; q < kQ; ++q) s_wmax[wv][q] = mx[q];
+        for (int q = 0; q < kQ; ++q) {
+            const uint32_t w = wave_umax(bq[q]);
+            if (lane == 0) s_wmax[wv][q] = w;
         }
         __syncthreads();
         if (tid < kQ) {
-            double m = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) m = fmax(m, s_wmax[w][tid]);
-            int e = (m > 0.0) ? ilogb(m) + 1 : kUnsetExp;
+            uint32_t mb = 0;
+            for (int w = 0; w < kBlock / 64; ++w) mb = max(mb, s_wmax[w][tid]);
+            // positive float bits: biased exponent + 1 bounds the value
+            const int e = (mb == 0u) ? kUnsetExp : (int)(mb >> 23) - 126;
             int sh = 0;
             if (e > s_e[tid]) {
                 if (s_e[tid] != kUnsetExp) sh = e - s_e[tid];
@@ -513,14 +641,18 @@ __global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
             for (int q = 0; q < kQ; ++q) any |= s_shift[q] != 0;
             if (any) {
                 for (int idx = tid; idx < kQ * AN; idx += kBlock) {
-                    int sh = s_shift[idx / AN];
+                    const int sh = s_shift[idx / AN];
                     if (sh) s_arena[idx] = (sh >= 63) ? (s_arena[idx] >> 63) : (s_arena[idx] >> sh);
                 }
                 __syncthreads();
             }
         }
+        // block-uniform scale exponents (scalar registers)
+        int sexp[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) sexp[q] = __builtin_amdgcn_readfirstlane(FRAC - s_e[q]);
 
-        // ---- run-segmented sums within each 16-lane row (lane order)
+        // ---- runs of equal cell inside each 16-lane row (lane order)
         const int kprev = dpp_i<MPM_ROW_SHR(1)>(key);
         const int knext = dpp_i<MPM_ROW_SHL(1)>(key);
         const bool head = (rl == 0) || (kprev != key);
@@ -531,40 +663,80 @@ __global__ void __launch_bounds__(kBlock) k_particle(KpArgs<real> a, int step)
         sst = max(sst, dpp_i<MPM_ROW_SHR(4)>(sst));
         sst = max(sst, dpp_i<MPM_ROW_SHR(8)>(sst));
         const int len = rl - sst + 1;
-        if (__ballot(len > 1)) seg_step<1>(c, rl, sst);
-        if (__ballot(len > 2)) seg_step<2>(c, rl, sst);
-        if (__ballot(len > 4)) seg_step<4>(c, rl, sst);
-        if (__ballot(len > 8)) seg_step<8>(c, rl, sst);
+        const bool st1 = __ballot(len > 1) != 0ull, st2 = __ballot(len > 2) != 0ull;
+        const bool st4 = __ballot(len > 4) != 0ull, st8 = __ballot(len > 8) != 0ull;
+        const bool emit = tail && key >= 0;
 
-        // ---- run totals -> LDS arena (64-bit fixed point, order independent)
-        if (tail && key >= 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
+        // ---- corner by corner: contributions, run sums, fixed-point LDS adds
+#pragma unroll 1
+        for (int k = 0; k < 8; ++k) {
+            real c[kQ];
+            {
+                real N, G[3];
+                corner(bx, by, bz, k, N, G);
+                if (key < 0) { N = real(0); G[0] = G[1] = G[2] = real(0); }
+                c[0] = N * m;
+                c[1] = N * mv[0];
+                c[2] = N * mv[1];
+                c[3] = N * mv[2];
+                c[4] = -(G[0] * vs[0] + G[1] * vs[3] + G[2] * vs[5]) + N * fe[0];
+                c[5] = -(G[1] * vs[1] + G[0] * vs[3] + G[2] * vs[4]) + N * fe[1];
+                c[6] = -(G[2] * vs[2] + G[0] * vs[5] + G[1] * vs[4]) + N * fe[2];
+            }
+            if (st1) seg_step<1>(c, rl, sst);
+            if (st2) seg_step<2>(c, rl, sst);
+            if (st4) seg_step<4>(c, rl, sst);
+            if (st8) seg_step<8>(c, rl, sst);
+            if (emit) {
                 const int node = key + ((k >> 2) & 1) + A0 * ((k >> 1) & 1) + A01 * (k & 1);
 #pragma unroll
                 for (int q = 0; q < kQ; ++q) {
-                    const real val = c[kQ * k + q];
-                    if (val != real(0)) {
-                        long long iv = __double2ll_rn(ldexp((double)val, FRAC - s_e[q]));
-                        atomicAdd((unsigned long long *)&s_arena[q * AN + node], (unsigned long long)iv);
-                    }
+                    long long iv;
+                    if constexpr (FP32) iv = (long long)__float2int_rn(ldexpf((float)c[q], sexp[q]));
+                    else iv = __double2ll_rn(ldexp((double)c[q], sexp[q]));
+                    atomicAdd((unsigned long long *)&s_arena[q * AN + node], (unsigned long long)iv);
                 }
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
 
-    // ---- arena out (fixed point -> real), fixed-order block sums of checks
-    real *out = a.arena_out + (size_t)slot * kQ * AN;
-    for (int idx = tid; idx < kQ * AN; idx += kBlock) {
-        const int e = s_e[idx / AN];
-        out[idx] = (e == kUnsetExp) ? real(0) : (real)ldexp((double)s_arena[idx], e - FRAC);
-    }
+    // ---- touched node box of the brick
 #pragma unroll
-    for (int k = 0; k < 5; ++k) red[k] = wave_sum(red[k]);
+    for (int q = 0; q < 3; ++q) {
+        const int lo = wave_imin(bmin[q]), hi = wave_imax(bmax[q]);
+        if (lane == 0) { s_box[wv][2 * q] = lo; s_box[wv][2 * q + 1] = hi; }
+    }
+    __syncthreads();
+    int box[6];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        int lo = 1 << 20, hi = -1;
+        for (int w = 0; w < kBlock / 64; ++w) { lo = min(lo, s_box[w][2 * q]); hi = max(hi, s_box[w][2 * q + 1]); }
+        box[2 * q] = lo;
+        box[2 * q + 1] = hi;
+    }
+    if (tid == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a.box_out[6 * slot + q] = box[q];
+
+    // ---- arena out (fixed point -> real) over the touched box only
+    if (box[1] >= box[0]) {
+        const int bx = box[1] - box[0] + 1, by = box[3] - box[2] + 1, bz = box[5] - box[4] + 1;
+        const int nbox = bx * by * bz;
+        real *out = a.arena_out + (size_t)slot * kQ * AN;
+        for (int t = tid; t < kQ * nbox; t += kBlock) {
+            const int q = t / nbox, r = t - q * nbox;
+            const int lx = box[0] + r % bx, ly = box[2] + (r / bx) % by, lz = box[4] + r / (bx * by);
+            const int idx = q * AN + lx + A0 * ly + A01 * lz;
+            const int e = s_e[q];
+            out[idx] = (e == kUnsetExp) ? real(0) : (real)ldexp((double)s_arena[idx], e - FRAC);
+        }
+    }
+    // ---- fixed-order block sums of the checks, flags
     if (lane == 0)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) s_red[wv][k] = red[k];
+        for (int k = 0; k < 5; ++k) s_red[wv][k] = wred[k];
     int fl = flags;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
@@ -582,7 +754,9 @@ template <typename real>
 struct K2Args {
     GridParams g;
     const real *arena_in;
+    const int32_t *box_in;       // per slot: touched node box
     const int32_t *brick_slot;   // dense brick -> occupied slot or -1
+    const int32_t *k2_nb;        // per grid brick: 27 neighbour slots (dz,dy,dx order) or -1
     const uint32_t *k2_list;
     const int32_t *n_k2;
     int32_t k2_cap;
@@ -595,14 +769,29 @@ struct K2Args {
 };
 
 // Grid momentum, damping, BC, reaction, platen override (…_MP.cpp:209-265).
+// Arenas covering a node are summed in a fixed brick order (z, y, x offsets
+// ascending), so the grid state is independent of scheduling.  Nodes that no
+// point touches are left alone: no G2P reads them.  The 27 neighbour slots of
+// the brick (built at rebin) and their touched boxes are staged in LDS.
 template <typename real>
-__global__ void __launch_bounds__(64) k_grid(K2Args<real> a, int step)
+__global__ void __launch_bounds__(256) k_grid(K2Args<real> a, int step)
 {
+    __shared__ double s_r[4][3];
+    __shared__ int s_nb[27];
+    __shared__ int s_bx[27][6];
     const GridParams &g = a.g;
     const int kb = blockIdx.x;
     if (kb >= *a.n_k2) return;
     const uint32_t brick = a.k2_list[kb];
     const int b[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
+    if (threadIdx.x < 27) {
+        const int sl = a.k2_nb[27 * kb + threadIdx.x];
+        s_nb[threadIdx.x] = sl;
+        if (sl >= 0)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s_bx[threadIdx.x][q] = a.box_in[6 * sl + q];
+    }
+    __syncthreads();
     const StepRec st = a.steps[step];
     const real dt = (real)st.dt;
     const real m_min = (real)g.m_min, damp = (real)g.damping;
@@ -619,29 +808,22 @@ __global__ void __launch_bounds__(64) k_grid(K2Args<real> a, int step)
         real sum[kQ];
 #pragma unroll
         for (int q = 0; q < kQ; ++q) sum[q] = real(0);
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int bz = b[2] + dz;
-            if (bz < 0 || bz >= g.nb[2]) continue;
-            const int lz = n[2] - (bz * g.S[2] - 1);
-            if (lz < 0 || lz >= g.A[2]) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int by = b[1] + dy;
-                if (by < 0 || by >= g.nb[1]) continue;
-                const int ly = n[1] - (by * g.S[1] - 1);
-                if (ly < 0 || ly >= g.A[1]) continue;
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int bx = b[0] + dx;
-                    if (bx < 0 || bx >= g.nb[0]) continue;
-                    const int lx = n[0] - (bx * g.S[0] - 1);
-                    if (lx < 0 || lx >= g.A[0]) continue;
-                    const int sl = a.brick_slot[bx + g.nb[0] * (by + g.nb[1] * bz)];
-                    if (sl < 0) continue;
-                    const real *ar = a.arena_in + (size_t)sl * kQ * AN + lx + g.A[0] * (ly + g.A[1] * lz);
+        bool covered = false;
+        for (int d = 0; d < 27; ++d) {
+            const int sl = s_nb[d];
+            if (sl < 0) continue;
+            // local node of the neighbour's arena: n - (origin' - 1)
+            const int lx = o[0] + 1 - (d % 3 - 1) * g.S[0];
+            const int ly = o[1] + 1 - ((d / 3) % 3 - 1) * g.S[1];
+            const int lz = o[2] + 1 - (d / 9 - 1) * g.S[2];
+            const int *bb = s_bx[d];
+            if (lx < bb[0] || lx > bb[1] || ly < bb[2] || ly > bb[3] || lz < bb[4] || lz > bb[5]) continue;
+            covered = true;
+            const real *ar = a.arena_in + (size_t)sl * kQ * AN + lx + g.A[0] * (ly + g.A[1] * lz);
 #pragma unroll
-                    for (int q = 0; q < kQ; ++q) sum[q] += ar[(size_t)q * AN];
-                }
-            }
+            for (int q = 0; q < kQ; ++q) sum[q] += ar[(size_t)q * AN];
         }
+        if (!covered) continue;
         const long long nid = n[0] + (long long)g.nodes[0] * (n[1] + (long long)g.nodes[1] * n[2]);
         const real m = sum[0];
         real v[3], f[3] = {sum[4], sum[5], sum[6]};
@@ -683,10 +865,13 @@ __global__ void __launch_bounds__(64) k_grid(K2Args<real> a, int step)
     rm = wave_sum(rm);
     rr = wave_sum(rr);
     ra = wave_sum(ra);
-    if (threadIdx.x == 0) {
-        a.k2_part[3 * kb] = rm;
-        a.k2_part[3 * kb + 1] = rr;
-        a.k2_part[3 * kb + 2] = ra;
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { s_r[wv][0] = rm; s_r[wv][1] = rr; s_r[wv][2] = ra; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double r = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_r[w][threadIdx.x];
+        a.k2_part[3 * kb + threadIdx.x] = r;
     }
 }
 

@@ -86,7 +86,8 @@ typedef struct mpm_config {
     const double *timeline_v;  /* 3*n_timeline velocities                       */
     const uint8_t *node_fixed; /* (cells+1) product entries of MPM_FIX_* bits, or
                                   NULL; copied at mpm_create                    */
-    int32_t brick[3];          /* cells per brick; 0 = default (8,4,2)          */
+    int32_t brick[3];          /* cells per brick; 0 = default (4,4,10), clamped
+                                  to the grid                                  */
     int32_t drift_check;       /* steps between drift checks; 0 = default (8)   */
     double friction;           /* contact path: Coulomb coefficient (extension;
                                   0 reproduces the frictionless reference)     */
