@@ -230,6 +230,24 @@ __global__ void k_occ_count(const uint32_t *occ_list, const uint32_t *occ_start,
     brick_slot[occ_list[s]] = (int32_t)s;
 }
 
+// Spread the points of each brick over the lanes of a wave.  Sorted position
+// t = q*P + r of a brick holding n points moves to sum_{r'<r} n_r' + q, where
+// n_r' = #{t < n : t mod P = r'}: neighbouring lanes then hold points P apart
+// in the cell order, so the P2G adds of one instruction rarely share a node.
+constexpr uint32_t kSpread = 4;
+__global__ void k_spread(const uint32_t *perm, const uint32_t *head, const uint32_t *pos, const uint32_t *occ_start,
+                         const uint32_t *occ_count, uint32_t n, uint32_t *perm_out)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t s = pos[j] + head[j] - 1u;   // occupied slot of position j
+    const uint32_t st = occ_start[s], cnt = occ_count[s], t = j - st;
+    const uint32_t r = t % kSpread;
+    uint32_t p = t / kSpread;
+    for (uint32_t rr = 0; rr < r; ++rr) p += (cnt > rr) ? (cnt - rr - 1u) / kSpread + 1u : 0u;
+    perm_out[st + p] = perm[j];
+}
+
 __global__ void k_dilate(GridParams g, const uint32_t *occ_list, uint32_t n_occ, uint32_t *flag)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;

@@ -355,6 +355,23 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     const uint32_t max_key = ((uint32_t)(h->nbricks - 1) << h->g.cbits) | ((1u << h->g.cbits) - 1u);
     uint32_t *sk, *perm;
     RCHK(radix_sort(h, h->keys[0], h->vals[0], h->keys[1], h->vals[1], n, max_key, &sk, &perm));
+    // brick tables
+    k_heads<<<nbk, kBlock, 0, h->stream>>>(sk, n, h->g.cbits, h->heads);
+    HIPCHK(h, hipGetLastError());
+    uint32_t n_occ = 0;
+    RCHK(scan_u32(h, h->heads, h->headpos, n, 0, &n_occ));
+    RCHK(ensure_slots(h, (int)n_occ, h->nbricks));
+    k_occ<<<nbk, kBlock, 0, h->stream>>>(sk, h->heads, h->headpos, n, h->g.cbits, h->occ_list, h->occ_start);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemsetAsync(h->brick_slot, 0xFF, (size_t)h->nbricks * 4, h->stream));
+    k_occ_count<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->occ_list, h->occ_start, n_occ, n,
+                                                              h->occ_count, h->brick_slot);
+    HIPCHK(h, hipGetLastError());
+    // spread each brick's points over the lanes (permutation inside bricks)
+    uint32_t *perm2 = (perm == h->vals[0]) ? h->vals[1] : h->vals[0];
+    k_spread<<<nbk, kBlock, 0, h->stream>>>(perm, h->heads, h->headpos, h->occ_start, h->occ_count, n, perm2);
+    HIPCHK(h, hipGetLastError());
+    perm = perm2;
     // gather all fields
     std::vector<std::pair<void **, int>> fs, fd;
     field_list<real>(h, src, fs);
@@ -374,18 +391,6 @@ static int rebin_t(mpm_handle *h, bool from_abs)
                                                      from_abs ? h->xstage[2] : nullptr, perm, sk, n);
     HIPCHK(h, hipGetLastError());
     h->cur = 1 - h->cur;
-    // brick tables
-    k_heads<<<nbk, kBlock, 0, h->stream>>>(sk, n, h->g.cbits, h->heads);
-    HIPCHK(h, hipGetLastError());
-    uint32_t n_occ = 0;
-    RCHK(scan_u32(h, h->heads, h->headpos, n, 0, &n_occ));
-    RCHK(ensure_slots(h, (int)n_occ, h->nbricks));
-    k_occ<<<nbk, kBlock, 0, h->stream>>>(sk, h->heads, h->headpos, n, h->g.cbits, h->occ_list, h->occ_start);
-    HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipMemsetAsync(h->brick_slot, 0xFF, (size_t)h->nbricks * 4, h->stream));
-    k_occ_count<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->occ_list, h->occ_start, n_occ, n,
-                                                              h->occ_count, h->brick_slot);
-    HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemsetAsync(h->k2flag, 0, (size_t)h->nbricks * 4, h->stream));
     k_dilate<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->g, h->occ_list, n_occ, h->k2flag);
     HIPCHK(h, hipGetLastError());

@@ -4,10 +4,9 @@
 //     the brick: G2P of step n (gather 8 nodes), F / volume / strain update,
 //     constitutive update, state write-back, then the P2G contributions of
 //     step n+1 (m, m v, -V sigma.grad N + N f_ext to the 8 corner nodes).
-//     Contributions are summed per run of points sharing a cell inside each
-//     16-lane DPP row (segmented shuffle scan, lane order), and the run totals
-//     are added to the brick's LDS arena as 64-bit fixed point with a
-//     per-brick exponent.  Integer addition is associative, so the arena is
+//     Every contribution is converted to fixed point with a per-brick,
+//     per-field exponent and added to the brick's LDS arena with a 64-bit
+//     integer add.  Integer addition is associative, so the arena is
 //     independent of wave scheduling: the step is bit-reproducible with no
 //     float atomics anywhere.  With G2P=false the kernel only rebuilds the
 //     arena (after upload or a rebin).
@@ -247,6 +246,36 @@ __device__ __forceinline__ void axes_of(const real x[3], const int32_t xq[3], co
     }
 }
 
+// FP32 linear-hat weights of one point arranged for the 8-corner loops:
+// N_k = wxy[ax,ay] wz[az], dN_k/dx = gx[ax] wyz[ay,az], dN_k/dy = gy[ay]
+// wxz[ax,az], dN_k/dz = gz[az] wxy[ax,ay]; g*[0] = -sign(d0)/h, g*[1] = 1/h
+// (GridPoint_Mediator.h:83-87 with d1 = (frac - 1) h < 0).
+struct Hat32 {
+    float wxy[4], wyz[4], wxz[4], wz[2], gx[2], gy[2], gz[2];
+};
+__device__ __forceinline__ Hat32 hat32(const Axis<float> &bx, const Axis<float> &by, const Axis<float> &bz)
+{
+    Hat32 w;
+    const float sx[2] = {bx.s0, bx.s1}, sy[2] = {by.s0, by.s1}, sz[2] = {bz.s0, bz.s1};
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            w.wxy[2 * i + j] = sx[i] * sy[j];
+            w.wyz[2 * i + j] = sy[i] * sz[j];
+            w.wxz[2 * i + j] = sx[i] * sz[j];
+        }
+    w.wz[0] = sz[0];
+    w.wz[1] = sz[1];
+    w.gx[0] = -bx.sg0 * bx.ih;
+    w.gx[1] = bx.ih;
+    w.gy[0] = -by.sg0 * by.ih;
+    w.gy[1] = by.ih;
+    w.gz[0] = -bz.sg0 * bz.ih;
+    w.gz[1] = bz.ih;
+    return w;
+}
+
 // ------------------------------------------------------- block reductions
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -273,8 +302,24 @@ __device__ __forceinline__ int wave_imax(int v)
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
+// Max of an unsigned value over the wave without the LDS crossbar: inclusive
+// DPP row scans (lanes without a source read 0, the identity), then the four
+// row maxima.  The result is wave-uniform.
+__device__ __forceinline__ uint32_t wave_umax_dpp(uint32_t v)
+{
+    v = max(v, (uint32_t)dpp_i<MPM_ROW_SHR(1)>((int)v));
+    v = max(v, (uint32_t)dpp_i<MPM_ROW_SHR(2)>((int)v));
+    v = max(v, (uint32_t)dpp_i<MPM_ROW_SHR(4)>((int)v));
+    v = max(v, (uint32_t)dpp_i<MPM_ROW_SHR(8)>((int)v));
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+    const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+    return max(max(r0, r1), max(r2, r3));
+}
 
 constexpr int kUnsetExp = -100000;
+constexpr int kMinExp = -90;   // contributions below 2^-90 of a unit are dropped
 
 template <typename real>
 struct KpArgs {
@@ -299,20 +344,10 @@ struct KpArgs {
     int32_t hist_index;
 };
 
-// One step of a row-segmented inclusive scan: lane l adds lane l-D when l-D
-// lies in the same run (run start sst, row lane rl).
-template <int D, typename real>
-__device__ __forceinline__ void seg_step(real (&c)[kQ], int rl, int sst)
-{
-    const real take = ((rl - D) >= sst) ? real(1) : real(0);
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) c[i] = fma(dpp_r<MPM_ROW_SHR(D)>(c[i]), take, c[i]);
-}
-
-// Fixed-point fraction bits.  FP32 state: run sums are converted to 32-bit
-// integers scaled so that the chunk's largest contribution bound is < 2^26
-// (run sums of <= 16 lanes stay < 2^30), then added to the 64-bit arena.
-// FP64 state: 64-bit conversion with FRAC = 61 - headroom bits.
+// Fixed-point fraction bits.  FP32 state: each contribution is converted to a
+// 32-bit integer scaled so that the brick's largest contribution bound so far
+// is < 2^26, then added to the 64-bit arena.  FP64 state: 64-bit conversion
+// with FRAC = 61 - H, count < 2^(H-1), so every node sum stays below 2^60.
 constexpr int kFrac32 = 26;
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
@@ -321,9 +356,8 @@ __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); 
 template <typename real, bool G2P>
 __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : MPM_KP_WAVES64) k_particle(KpArgs<real> a, int step)
 {
-    extern __shared__ long long s_arena[];
-    __shared__ uint32_t s_wmax[kBlock / 64][kQ];
-    __shared__ int s_e[kQ], s_shift[kQ];
+    extern __shared__ long long s_arena[];   // node-major: kQ int64 per arena node
+    __shared__ uint32_t s_max[8];            // running max bound bits per field
     __shared__ int s_box[kBlock / 64][6];
     __shared__ double s_red[kBlock / 64][5];
 
@@ -375,11 +409,14 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
     }
 
     for (int i = tid; i < kQ * AN; i += kBlock) s_arena[i] = 0;
-    if (tid < kQ) s_e[tid] = kUnsetExp;
-    int H = 6;
-    while ((1u << (H - 1)) <= count) ++H;   // >= log2(count) + 1 bits of headroom
+    if (tid < 8) s_max[tid] = 0u;
+    int H = 6;   // a node takes at most one contribution per point: count < 2^(H-1)
+    while ((1u << (H - 1)) <= count) ++H;
     const int FRAC = FP32 ? kFrac32 : 61 - H;
-    double wred[5] = {0, 0, 0, 0, 0};   // lane 0: this wave's running check sums
+    int ecur[kQ];                        // block-uniform field exponents (value < 2^e)
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) ecur[q] = kUnsetExp;
+    double red[5] = {0, 0, 0, 0, 0};     // this lane's running check sums (chunk order)
     int flags = 0;
     int bmin[3] = {1 << 20, 1 << 20, 1 << 20}, bmax[3] = {-1, -1, -1};
     const PArrays<real> &P = a.p;
@@ -392,7 +429,6 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         const bool active = (c0 + tid) < count;
         const uint32_t i = start + c0 + tid;
         int key = -1;
-        double red[5] = {0, 0, 0, 0, 0};
         Axis<real> bx, by, bz;
         real m = real(0), mv[3] = {real(0), real(0), real(0)}, vs[6], fe[3] = {real(0), real(0), real(0)};
         uint32_t bq[kQ] = {0, 0, 0, 0, 0, 0, 0};
@@ -429,29 +465,53 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
 #pragma unroll
                     for (int q = 0; q < 9; ++q) L[q] = real(0);
                     real dx[3] = {real(0), real(0), real(0)};
-#pragma unroll 2
-                    for (int k = 0; k < 8; ++k) {
-                        real N, G[3];
-                        corner(bx, by, bz, k, N, G);
-                        const long long nid = nb0 + ((k >> 2) & 1) + (long long)g.nodes[0] * (((k >> 1) & 1) + (long long)g.nodes[1] * (k & 1));
-                        const NodeRec<real> nr = a.nodes[nid];
+                    if constexpr (FP32) {
+                        // 8 corners fully unrolled; node record {v.xyz a.x | a.y a.z m -}
+                        const Hat32 w = hat32(bx, by, bz);
+                        const NodeRec<float> *nb = a.nodes + nb0;
+                        const long long sy_ = g.nodes[0], sz_ = (long long)g.nodes[0] * g.nodes[1];
+                        float av[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-                        for (int q = 0; q < 3; ++q) v[q] += N * nr.d[3 + q] * dt;
-                        if (FP32) {
+                        for (int k = 0; k < 8; ++k) {
+                            const int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
+                            const float N = w.wxy[2 * ax + ay] * w.wz[az];
+                            const float G[3] = {w.gx[ax] * w.wyz[2 * ay + az], w.gy[ay] * w.wxz[2 * ax + az],
+                                                w.gz[az] * w.wxy[2 * ax + ay]};
+                            const float4 *np = reinterpret_cast<const float4 *>(nb + ax + sy_ * ay + sz_ * az);
+                            const float4 p0 = np[0], p1 = np[1];
+                            const float vn[3] = {p0.x, p0.y, p0.z}, an[3] = {p0.w, p1.x, p1.y};
 #pragma unroll
-                            for (int q = 0; q < 3; ++q) dx[q] += N * nr.d[q];
-                        } else {
+                            for (int q = 0; q < 3; ++q) {
+                                av[q] += N * an[q];
+                                dx[q] += N * vn[q];
+                            }
 #pragma unroll
-                            for (int q = 0; q < 3; ++q) x[q] += N * nr.d[q] * dt;
+                            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                                for (int r = 0; r < 3; ++r) L[3 * cc + r] += vn[r] * G[cc];
                         }
 #pragma unroll
-                        for (int cc = 0; cc < 3; ++cc)
+                        for (int q = 0; q < 3; ++q) {
+                            v[q] += av[q] * dt;
+                            xq[q] += __float2int_rn((float)dx[q] * qscale[q]);
+                        }
+                    } else {
+#pragma unroll 2
+                        for (int k = 0; k < 8; ++k) {
+                            real N, G[3];
+                            corner(bx, by, bz, k, N, G);
+                            const long long nid = nb0 + ((k >> 2) & 1) + (long long)g.nodes[0] * (((k >> 1) & 1) + (long long)g.nodes[1] * (k & 1));
+                            const NodeRec<real> nr = a.nodes[nid];
 #pragma unroll
-                            for (int r = 0; r < 3; ++r) L[3 * cc + r] += nr.d[r] * G[cc];
+                            for (int q = 0; q < 3; ++q) v[q] += N * nr.d[3 + q] * dt;
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) x[q] += N * nr.d[q] * dt;
+#pragma unroll
+                            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                                for (int r = 0; r < 3; ++r) L[3 * cc + r] += nr.d[r] * G[cc];
+                        }
                     }
-                    if (FP32)
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) xq[q] += __float2int_rn((float)dx[q] * qscale[q]);
                     // F += (L F) dt  (…_MP.cpp:299)
                     real T[9];
 #pragma unroll
@@ -562,14 +622,14 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
             }
             // ---- FP64 checks of the new state (PhysicsEngine.cpp:73-79)
             {
-                const double vv = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-                const double lv = sqrt(vv);
-                red[0] += (double)m * (lv * lv);
-                red[1] += (double)m * v[0];
-                red[2] += (double)m * v[1];
-                red[3] += (double)m * v[2];
-                const bool fin = isfinite(vv) && isfinite((double)s[0] + s[1] + s[2] + s[3] + s[4] + s[5]) &&
-                                 (FP32 || isfinite((double)x[0] + x[1] + x[2]));
+                const double md = (double)m, vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
+                const double vv = vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2];
+                red[0] += md * vv;
+                red[1] += md * vd[0];
+                red[2] += md * vd[1];
+                red[3] += md * vd[2];
+                const real ssum = s[0] + s[1] + s[2] + s[3] + s[4] + s[5];
+                const bool fin = isfinite(vv) && isfinite(ssum) && (FP32 || isfinite(x[0] + x[1] + x[2]));
                 if (!fin) { red[4] += 1.0; flags |= FLAG_NAN; }
             }
             // ---- P2G inputs of step n+1 (…_MP.cpp:106-185)
@@ -611,90 +671,84 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
             }
         }
 
-        // ---- FP64 check sums: fixed-order wave reduction per chunk
-#pragma unroll
-        for (int k = 0; k < 5; ++k) wred[k] += wave_sum(red[k]);
-
-        // ---- per-chunk fixed-point exponent of each field
+        // ---- block-wide fixed-point exponents: running max of the bounds
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
-            const uint32_t w = wave_umax(bq[q]);
-            if (lane == 0) s_wmax[wv][q] = w;
-        }
-        __syncthreads();
-        if (tid < kQ) {
-            uint32_t mb = 0;
-            for (int w = 0; w < kBlock / 64; ++w) mb = max(mb, s_wmax[w][tid]);
-            // positive float bits: biased exponent + 1 bounds the value
-            const int e = (mb == 0u) ? kUnsetExp : (int)(mb >> 23) - 126;
-            int sh = 0;
-            if (e > s_e[tid]) {
-                if (s_e[tid] != kUnsetExp) sh = e - s_e[tid];
-                s_e[tid] = e;
-            }
-            s_shift[tid] = sh;
+            const uint32_t w = wave_umax_dpp(bq[q]);
+            if (lane == 0 && w) atomicMax(&s_max[q], w);
         }
         __syncthreads();
         {
             bool any = false;
+            int sh[kQ];
 #pragma unroll
-            for (int q = 0; q < kQ; ++q) any |= s_shift[q] != 0;
-            if (any) {
+            for (int q = 0; q < kQ; ++q) {
+                const uint32_t mb = s_max[q];
+                // positive float bits: biased exponent + 1 bounds the value
+                const int e = (mb == 0u) ? kUnsetExp : max((int)(mb >> 23) - 126, kMinExp);
+                const int en = __builtin_amdgcn_readfirstlane(e);
+                sh[q] = (ecur[q] != kUnsetExp && en > ecur[q]) ? en - ecur[q] : 0;
+                any |= sh[q] != 0;
+                ecur[q] = en;
+            }
+            if (any) {   // the exponent grew: rescale what the arena holds
                 for (int idx = tid; idx < kQ * AN; idx += kBlock) {
-                    const int sh = s_shift[idx / AN];
-                    if (sh) s_arena[idx] = (sh >= 63) ? (s_arena[idx] >> 63) : (s_arena[idx] >> sh);
+                    int shq = 0;
+#pragma unroll
+                    for (int q = 0; q < kQ; ++q) shq = (idx % kQ == q) ? sh[q] : shq;
+                    if (shq) s_arena[idx] = (shq >= 63) ? (s_arena[idx] >> 63) : (s_arena[idx] >> shq);
                 }
                 __syncthreads();
             }
         }
-        // block-uniform scale exponents (scalar registers)
-        int sexp[kQ];
+
+        // ---- P2G: every lane adds its 8 x 7 contributions (fixed point)
+        if (key >= 0) {
+            real sc[kQ];
 #pragma unroll
-        for (int q = 0; q < kQ; ++q) sexp[q] = __builtin_amdgcn_readfirstlane(FRAC - s_e[q]);
-
-        // ---- runs of equal cell inside each 16-lane row (lane order)
-        const int kprev = dpp_i<MPM_ROW_SHR(1)>(key);
-        const int knext = dpp_i<MPM_ROW_SHL(1)>(key);
-        const bool head = (rl == 0) || (kprev != key);
-        const bool tail = (rl == 15) || (knext != key);
-        int sst = head ? rl : 0;
-        sst = max(sst, dpp_i<MPM_ROW_SHR(1)>(sst));
-        sst = max(sst, dpp_i<MPM_ROW_SHR(2)>(sst));
-        sst = max(sst, dpp_i<MPM_ROW_SHR(4)>(sst));
-        sst = max(sst, dpp_i<MPM_ROW_SHR(8)>(sst));
-        const int len = rl - sst + 1;
-        const bool st1 = __ballot(len > 1) != 0ull, st2 = __ballot(len > 2) != 0ull;
-        const bool st4 = __ballot(len > 4) != 0ull, st8 = __ballot(len > 8) != 0ull;
-        const bool emit = tail && key >= 0;
-
-        // ---- corner by corner: contributions, run sums, fixed-point LDS adds
-#pragma unroll 1
-        for (int k = 0; k < 8; ++k) {
-            real c[kQ];
-            {
-                real N, G[3];
-                corner(bx, by, bz, k, N, G);
-                if (key < 0) { N = real(0); G[0] = G[1] = G[2] = real(0); }
-                c[0] = N * m;
-                c[1] = N * mv[0];
-                c[2] = N * mv[1];
-                c[3] = N * mv[2];
-                c[4] = -(G[0] * vs[0] + G[1] * vs[3] + G[2] * vs[5]) + N * fe[0];
-                c[5] = -(G[1] * vs[1] + G[0] * vs[3] + G[2] * vs[4]) + N * fe[1];
-                c[6] = -(G[2] * vs[2] + G[0] * vs[5] + G[1] * vs[4]) + N * fe[2];
+            for (int q = 0; q < kQ; ++q) {
+                if constexpr (FP32)
+                    sc[q] = (ecur[q] == kUnsetExp) ? 0.f : __int_as_float((127 + FRAC - ecur[q]) << 23);
+                else
+                    sc[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, FRAC - ecur[q]);
             }
-            if (st1) seg_step<1>(c, rl, sst);
-            if (st2) seg_step<2>(c, rl, sst);
-            if (st4) seg_step<4>(c, rl, sst);
-            if (st8) seg_step<8>(c, rl, sst);
-            if (emit) {
-                const int node = key + ((k >> 2) & 1) + A0 * ((k >> 1) & 1) + A01 * (k & 1);
+            // scaled per-point factors (power-of-two scaling is exact)
+            const real ms = m * sc[0], mvs[3] = {mv[0] * sc[1], mv[1] * sc[2], mv[2] * sc[3]};
+            const real f4[4] = {-vs[0] * sc[4], -vs[3] * sc[4], -vs[5] * sc[4], fe[0] * sc[4]};
+            const real f5[4] = {-vs[3] * sc[5], -vs[1] * sc[5], -vs[4] * sc[5], fe[1] * sc[5]};
+            const real f6[4] = {-vs[5] * sc[6], -vs[4] * sc[6], -vs[2] * sc[6], fe[2] * sc[6]};
+            long long *rowp[4];   // arena records of the (ay, az) corner rows
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rowp[r] = s_arena + kQ * (key + A0 * (r >> 1) + A01 * (r & 1));
+            Hat32 w;
+            if constexpr (FP32) w = hat32(bx, by, bz);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
+                real N, G[3];
+                if constexpr (FP32) {
+                    N = w.wxy[2 * ax + ay] * w.wz[az];
+                    G[0] = w.gx[ax] * w.wyz[2 * ay + az];
+                    G[1] = w.gy[ay] * w.wxz[2 * ax + az];
+                    G[2] = w.gz[az] * w.wxy[2 * ax + ay];
+                } else {
+                    corner(bx, by, bz, k, N, G);
+                }
+                real c[kQ];
+                c[0] = N * ms;
+                c[1] = N * mvs[0];
+                c[2] = N * mvs[1];
+                c[3] = N * mvs[2];
+                c[4] = (G[0] * f4[0] + G[1] * f4[1] + G[2] * f4[2]) + N * f4[3];
+                c[5] = (G[1] * f5[1] + G[0] * f5[0] + G[2] * f5[2]) + N * f5[3];
+                c[6] = (G[2] * f6[2] + G[0] * f6[0] + G[1] * f6[1]) + N * f6[3];
+                unsigned long long *rec = reinterpret_cast<unsigned long long *>(rowp[2 * ay + az] + kQ * ax);
 #pragma unroll
                 for (int q = 0; q < kQ; ++q) {
                     long long iv;
-                    if constexpr (FP32) iv = (long long)__float2int_rn(ldexpf((float)c[q], sexp[q]));
-                    else iv = __double2ll_rn(ldexp((double)c[q], sexp[q]));
-                    atomicAdd((unsigned long long *)&s_arena[q * AN + node], (unsigned long long)iv);
+                    if constexpr (FP32) iv = (long long)__float2int_rn((float)c[q]);
+                    else iv = __double2ll_rn((double)c[q]);
+                    atomicAdd(rec + q, (unsigned long long)iv);
                 }
             }
         }
@@ -728,15 +782,19 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         for (int t = tid; t < kQ * nbox; t += kBlock) {
             const int q = t / nbox, r = t - q * nbox;
             const int lx = box[0] + r % bx, ly = box[2] + (r / bx) % by, lz = box[4] + r / (bx * by);
-            const int idx = q * AN + lx + A0 * ly + A01 * lz;
-            const int e = s_e[q];
-            out[idx] = (e == kUnsetExp) ? real(0) : (real)ldexp((double)s_arena[idx], e - FRAC);
+            const int nd = lx + A0 * ly + A01 * lz;
+            int e = kUnsetExp;
+#pragma unroll
+            for (int f = 0; f < kQ; ++f) e = (q == f) ? ecur[f] : e;
+            out[q * AN + nd] = (e == kUnsetExp) ? real(0) : (real)ldexp((double)s_arena[kQ * nd + q], e - FRAC);
         }
     }
     // ---- fixed-order block sums of the checks, flags
-    if (lane == 0)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) s_red[wv][k] = wred[k];
+    for (int k = 0; k < 5; ++k) {
+        const double wsum = wave_sum(red[k]);
+        if (lane == 0) s_red[wv][k] = wsum;
+    }
     int fl = flags;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
