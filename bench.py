@@ -82,6 +82,23 @@ def cpu_baseline(sc, seconds=15.0):
                        f"(restatement of the reference step), 1 thread as the reference default")
 
 
+def counters_run(m, sc, warmup, steps):
+    """FP32 steps of the bench scene, then one plain device-to-device copy of
+    256 MiB (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+    eng = m.PhysicsEngine(sc, precision=m.FP32, track_strain=False)
+    eng.run_steps(warmup + steps)
+    c = eng.checks()
+    eng.close()
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"counters_run": True, "steps": warmup + steps, "n_points": sc.n_deformable,
+                      "grid_nodes": int(c["grid_nodes"]), "copy_bytes": 2 * a.numel() * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +110,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="short FP32 run plus a 256 MiB device copy, for rocprofv3 --pmc passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
@@ -102,6 +121,9 @@ def main():
 
     sc = build_scene(args.scale, args.speed)
     n = sc.n_deformable
+    if args.counters:
+        counters_run(m, sc, args.warmup, args.steps)
+        return
     r32 = run_gpu(sc, m.FP32, args.steps, args.warmup, args.profile_steps)
     r64 = run_gpu(sc, m.FP64, args.fp64_steps, args.warmup, 0)
     v32 = n * args.steps / r32["wall_s"]

@@ -234,7 +234,7 @@ __global__ void k_occ_count(const uint32_t *occ_list, const uint32_t *occ_start,
 // t = q*P + r of a brick holding n points moves to sum_{r'<r} n_r' + q, where
 // n_r' = #{t < n : t mod P = r'}: neighbouring lanes then hold points P apart
 // in the cell order, so the P2G adds of one instruction rarely share a node.
-constexpr uint32_t kSpread = 4;
+constexpr uint32_t kSpread = 8;   // >= typical points per cell
 __global__ void k_spread(const uint32_t *perm, const uint32_t *head, const uint32_t *pos, const uint32_t *occ_start,
                          const uint32_t *occ_count, uint32_t n, uint32_t *perm_out)
 {
@@ -268,23 +268,6 @@ __global__ void k_compact(const uint32_t *flag, const uint32_t *pos, uint32_t n,
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n || !flag[j]) return;
     list[pos[j]] = j;
-}
-
-// 27 neighbour slots of every grid-update brick, offsets (dz, dy, dx) ascending
-__global__ void k_k2_neighbors(GridParams g, const uint32_t *k2_list, uint32_t n_k2, const int32_t *brick_slot,
-                               int32_t *k2_nb)
-{
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 27u * n_k2) return;
-    const uint32_t kb = t / 27u;
-    const int d = (int)(t % 27u);
-    const uint32_t b = k2_list[kb];
-    const int x = (int)(b % g.nb[0]) + d % 3 - 1, y = (int)((b / g.nb[0]) % g.nb[1]) + (d / 3) % 3 - 1,
-              z = (int)(b / (g.nb[0] * g.nb[1])) + d / 9 - 1;
-    int sl = -1;
-    if (x >= 0 && y >= 0 && z >= 0 && x < g.nb[0] && y < g.nb[1] && z < g.nb[2])
-        sl = brick_slot[x + g.nb[0] * (y + g.nb[1] * z)];
-    k2_nb[t] = sl;
 }
 
 // initial platen override marks for the first step after upload

@@ -9,9 +9,10 @@
 //   displacement-controlled points (platens): FP64 SoA, unsorted.
 //   grid nodes: dense lattice (cells+1 per axis); per node an 8-word record
 //     {v.xyz, a.x | a.y, a.z, -, -} written by the grid kernel, read by G2P.
-//   brick arenas: per occupied brick, 7 node fields (m, p.xyz, f.xyz) on the
-//     (S+3)^3 node window the brick's points can reach; written by the fused
-//     particle kernel (P2G), merged in a fixed order by the grid kernel.
+//   brick arenas: per occupied brick, the P2G sums (m, p.xyz, f.xyz) of the
+//     nodes its points touched, as 8-real node records over the touched box
+//     of the (S+3)^3 node window (box-compact, x fastest); written by the
+//     fused particle kernel, merged in a fixed order by the grid kernel.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -67,6 +68,7 @@ struct Material {
     double G;      // 0.5*E/(1+nu)       (…PerfectlyPlastic.cpp:54)
     double k0;     // sy/sqrt(3)         (…PerfectlyPlastic.cpp:39)
     double invE, invG, hs;   // 1/E, 1/G, sqrt(2/3)*C1
+    double Cn;               // 1/((1+nu)(1-2nu)): C = E*Cn for per-point E
 };
 
 template <typename real>

@@ -66,6 +66,7 @@ struct mpm_handle {
 
     // grid
     void *nodes = nullptr;
+    void *zero_rec = nullptr;               // 8 zero reals (uncovered arena candidates)
     uint8_t *fixed = nullptr;
     unsigned long long *marks = nullptr;
 
@@ -73,10 +74,10 @@ struct mpm_handle {
     uint32_t *occ_list = nullptr, *occ_start = nullptr, *occ_count = nullptr;
     int32_t *n_occ_d = nullptr, *n_k2_d = nullptr, *brick_slot = nullptr;
     uint32_t *k2_list = nullptr, *k2flag = nullptr, *k2pos = nullptr;
-    int32_t *k2_nb = nullptr;               // per grid brick: 27 neighbour slots
     int n_occ = 0, n_k2 = 0, occ_cap = 0, k2_cap = 0;
+    int n_k2b = 0;                          // grid-kernel blocks (= per-block check partials)
     void *arena[2] = {nullptr, nullptr};
-    int32_t *box = nullptr;                 // per slot touched node box (6 ints)
+    int4 *binfo = nullptr;                  // per brick (dense): {slot, packed touched node box}
     size_t arena_cap_slots = 0;
     int parity = 0;
 
@@ -274,11 +275,9 @@ static int ensure_slots(mpm_handle *h, int n_occ, int n_k2)
     if ((size_t)n_occ > h->arena_cap_slots || h->arena[0] == nullptr) {
         for (int b = 0; b < 2; ++b) dev_free(h->arena[b]);
         dev_free((void *&)h->kp_part);
-        dev_free((void *&)h->box);
         h->arena_cap_slots = (size_t)want_occ;
         for (int b = 0; b < 2; ++b)
-            RCHK(dev_alloc(h, &h->arena[b], h->arena_cap_slots * kQ * h->g.arenaN * h->rsize));
-        RCHK(dalloc(h, &h->box, h->arena_cap_slots * 6));
+            RCHK(dev_alloc(h, &h->arena[b], h->arena_cap_slots * 8 * h->g.arenaN * h->rsize));
         RCHK(dalloc(h, &h->kp_part, h->arena_cap_slots * 5));
         HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * 8, h->stream));
         h->occ_cap = want_occ;
@@ -339,7 +338,7 @@ static int rebin_t(mpm_handle *h, bool from_abs)
 {
     const uint32_t n = (uint32_t)h->n_def;
     if (n == 0) {
-        h->n_occ = h->n_k2 = 0;
+        h->n_occ = h->n_k2 = h->n_k2b = 0;
         RCHK(ensure_slots(h, 0, 0));
         HIPCHK(h, hipMemsetAsync(h->n_occ_d, 0, 4, h->stream));
         HIPCHK(h, hipMemsetAsync(h->n_k2_d, 0, 4, h->stream));
@@ -398,12 +397,15 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     RCHK(scan_u32(h, h->k2flag, h->k2pos, (uint32_t)h->nbricks, 0, &n_k2));
     k_compact<<<cdiv(h->nbricks, kBlock), kBlock, 0, h->stream>>>(h->k2flag, h->k2pos, (uint32_t)h->nbricks, h->k2_list);
     HIPCHK(h, hipGetLastError());
-    RCHK(ensure_slots(h, (int)n_occ, (int)n_k2));
-    k_k2_neighbors<<<cdiv((size_t)27 * n_k2, kBlock), kBlock, 0, h->stream>>>(h->g, h->k2_list, n_k2, h->brick_slot, h->k2_nb);
-    HIPCHK(h, hipGetLastError());
+    const size_t nS = (size_t)h->g.S[0] * h->g.S[1] * h->g.S[2];
+    const int n_k2b = (int)cdiv((size_t)n_k2 * nS, 256);
+    RCHK(ensure_slots(h, (int)n_occ, n_k2b));
     h->n_occ = (int)n_occ;
     h->n_k2 = (int)n_k2;
-    int32_t cnt[2] = {(int32_t)n_occ, (int32_t)n_k2};
+    h->n_k2b = n_k2b;
+    // brick records are rewritten by the rebuild below; stale slots must read -1
+    HIPCHK(h, hipMemsetAsync(h->binfo, 0xFF, (size_t)h->nbricks * sizeof(int4), h->stream));
+    int32_t cnt[2] = {(int32_t)n_occ, (int32_t)n_k2b};
     HIPCHK(h, hipMemcpyAsync(h->n_occ_d, &cnt[0], 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->n_k2_d, &cnt[1], 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -429,12 +431,12 @@ static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index)
     a.nodes = (const NodeRec<real> *)h->nodes;
     // G2P(n)+P2G(n+1) writes the arena of step n+1; a rebuild writes step n's
     a.arena_out = (real *)h->arena[g2p ? (h->parity ^ 1) : h->parity];
-    a.box_out = h->box;
+    a.binfo = h->binfo;
     a.occ_list = h->occ_list;
     a.occ_start = h->occ_start;
     a.occ_count = h->occ_count;
     a.n_occ = h->n_occ_d;
-    a.occ_cap = h->occ_cap;
+    a.occ_cap = h->n_occ;    // brick blocks first, platen blocks after them
     a.mats = h->mats;
     a.steps = h->steps_d;
     a.flags = h->flags_d;
@@ -448,7 +450,7 @@ static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index)
     a.hist = h->hist_d;
     a.hist_index = hist_index;
     const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long);
-    unsigned grid = (unsigned)h->occ_cap;
+    unsigned grid = (unsigned)h->n_occ;
     if (g2p) grid += cdiv(h->n_dc, kBlock);
     if (grid == 0) return MPM_OK;
     if (g2p)
@@ -466,19 +468,17 @@ static int launch_grid(mpm_handle *h, int step)
     K2Args<real> a{};
     a.g = h->g;
     a.arena_in = (const real *)h->arena[h->parity];
-    a.box_in = h->box;
-    a.brick_slot = h->brick_slot;
-    a.k2_nb = h->k2_nb;
+    a.binfo = h->binfo;
+    a.zero_rec = (const real *)h->zero_rec;
     a.k2_list = h->k2_list;
-    a.n_k2 = h->n_k2_d;
-    a.k2_cap = h->k2_cap;
+    a.n_k2 = h->n_k2;
     a.nodes = (NodeRec<real> *)h->nodes;
     a.fixed = h->fixed;
     a.marks = h->marks;
     a.dc_v = h->dc_v;
     a.steps = h->steps_d;
     a.k2_part = h->k2_part;
-    k_grid<real><<<(unsigned)h->k2_cap, 256, 0, h->stream>>>(a, step);
+    k_grid<real><<<(unsigned)h->n_k2b, 256, 0, h->stream>>>(a, step);
     HIPCHK(h, hipGetLastError());
     return MPM_OK;
 }
@@ -636,13 +636,13 @@ static void destroy_all(mpm_handle *h)
         dev_free((void *&)h->vals[b]);
     }
     for (int a = 0; a < 3; ++a) dev_free((void *&)h->xstage[a]);
-    void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v, (void **)&h->dc_m, &h->nodes, (void **)&h->fixed,
+    void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v, (void **)&h->dc_m, &h->nodes, &h->zero_rec, (void **)&h->fixed,
                      (void **)&h->marks, (void **)&h->occ_list, (void **)&h->occ_start, (void **)&h->occ_count,
                      (void **)&h->n_occ_d, (void **)&h->n_k2_d, (void **)&h->brick_slot, (void **)&h->k2_list,
                      (void **)&h->k2flag, (void **)&h->k2pos, (void **)&h->mats, (void **)&h->steps_d,
                      (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->k2_part, (void **)&h->hist_d,
                      (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
-                     (void **)&h->headpos, (void **)&h->gdesc, (void **)&h->box, (void **)&h->k2_nb};
+                     (void **)&h->headpos, (void **)&h->gdesc, (void **)&h->binfo};
     for (void **p : ptrs) dev_free(*p);
     for (auto *p : h->scan_scratch) { void *q = p; dev_free(q); }
     h->scan_scratch.clear();
@@ -713,6 +713,8 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         g.ihf[a] = (float)(1.0 / g.h[a]);
         int S = cfg->brick[a] > 0 ? cfg->brick[a] : (a == 2 ? 10 : 4);
         S = std::min(S, std::max(1, cfg->cells[a]));
+        // an arena window reaches one neighbour per side only when S >= 3
+        if (S < 3 && S < cfg->cells[a]) { h->err = "brick size must be >= 3 cells (or span the axis)"; return fail(MPM_E_ARG); }
         g.S[a] = S;
         g.nb[a] = g.nodes[a] / S + ((g.nodes[a] % S) ? 1 : 0);
         g.A[a] = S + 3;
@@ -733,6 +735,8 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
 
     int rc;
     if ((rc = dev_alloc(h, &h->nodes, (size_t)h->nnodes * 8 * h->rsize))) return fail(rc);
+    if ((rc = dev_alloc(h, &h->zero_rec, 64))) return fail(rc);
+    if (hipMemset(h->zero_rec, 0, 64) != hipSuccess) { h->err = "hipMemset failed"; return fail(MPM_E_HIP); }
     if ((rc = dalloc(h, &h->fixed, (size_t)h->nnodes))) return fail(rc);
     if ((rc = dalloc(h, &h->marks, (size_t)h->nnodes))) return fail(rc);
     (void)hipMemsetAsync(h->nodes, 0, (size_t)h->nnodes * 8 * h->rsize, h->stream);
@@ -748,7 +752,7 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     const size_t nb = (size_t)h->nbricks;
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
         (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->n_k2_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) ||
-        (rc = dalloc(h, &h->k2_list, nb)) || (rc = dalloc(h, &h->k2_nb, 27 * nb)) || (rc = dalloc(h, &h->k2flag, nb)) || (rc = dalloc(h, &h->k2pos, nb)) ||
+        (rc = dalloc(h, &h->k2_list, nb)) || (rc = dalloc(h, &h->binfo, nb)) || (rc = dalloc(h, &h->k2flag, nb)) || (rc = dalloc(h, &h->k2pos, nb)) ||
         (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)))
         return fail(rc);
     (void)hipMemsetAsync(h->n_occ_d, 0, 4, h->stream);
@@ -815,6 +819,7 @@ static int upload_t(mpm_handle *h, const mpm_points *pts)
         m.k0 = m.sy / std::sqrt(3.0);
         m.invE = 1.0 / m.E;
         m.invG = 1.0 / m.G;
+        m.Cn = 1.0 / ((1.0 + m.nu) * (1.0 - 2.0 * m.nu));
         m.hs = std::sqrt(2.0 / 3.0) * m.C1;
         mats[kv.second] = m;
     }

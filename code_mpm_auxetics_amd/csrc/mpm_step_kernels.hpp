@@ -93,13 +93,13 @@ __device__ __forceinline__ bool radial_return(double E, double nu, double radius
     return true;
 }
 
-// The same radial return for FP32 state: FP64 arithmetic, material constants
-// precomputed per material (reciprocals instead of the reference's divisions).
-// Returns true on the plastic branch; s is updated in place (rounded once).
-__device__ __forceinline__ bool radial_return_fp32(const Material &mt, double C, double radius, float s[6],
-                                                   const float de[6], double dep[6])
+// The same radial return for FP32 state: FP64 arithmetic, elastic constants
+// precomputed (C = E/(1+nu)/(1-2nu), 1/E, 1/G: reciprocals instead of the
+// reference's divisions).  Returns true on the plastic branch; s is updated in
+// place (rounded once).
+__device__ __forceinline__ bool radial_return_fp32(double nu, double C, double ie, double ig, double radius,
+                                                   float s[6], const float de[6], double dep[6])
 {
-    const double nu = mt.nu;
     double st[6];
     st[0] = (double)s[0] + C * ((1.0 - nu) * de[0] + nu * de[1] + nu * de[2]);
     st[1] = (double)s[1] + C * ((1.0 - nu) * de[1] + nu * de[0] + nu * de[2]);
@@ -122,7 +122,6 @@ __device__ __forceinline__ bool radial_return_fp32(const Material &mt, double C,
         ds[i] = snew - (double)s[i];
         s[i] = (float)snew;
     }
-    const double ie = mt.invE, ig = mt.invG;
     dep[0] = de[0] - (ds[0] - nu * (ds[1] + ds[2])) * ie;
     dep[1] = de[1] - (ds[1] - nu * (ds[0] + ds[2])) * ie;
     dep[2] = de[2] - (ds[2] - nu * (ds[0] + ds[1])) * ie;
@@ -276,6 +275,33 @@ __device__ __forceinline__ Hat32 hat32(const Axis<float> &bx, const Axis<float> 
     return w;
 }
 
+// 8-real node records, 32 B (FP32) or 64 B (FP64) aligned
+__device__ __forceinline__ void store8(float *p, const float r[8])
+{
+    reinterpret_cast<float4 *>(p)[0] = make_float4(r[0], r[1], r[2], r[3]);
+    reinterpret_cast<float4 *>(p)[1] = make_float4(r[4], r[5], r[6], r[7]);
+}
+__device__ __forceinline__ void store8(double *p, const double r[8])
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) reinterpret_cast<double2 *>(p)[k] = make_double2(r[2 * k], r[2 * k + 1]);
+}
+__device__ __forceinline__ void load8(const float *p, float r[8])
+{
+    const float4 a = reinterpret_cast<const float4 *>(p)[0], b = reinterpret_cast<const float4 *>(p)[1];
+    r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+    r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+}
+__device__ __forceinline__ void load8(const double *p, double r[8])
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double2 v = reinterpret_cast<const double2 *>(p)[k];
+        r[2 * k] = v.x;
+        r[2 * k + 1] = v.y;
+    }
+}
+
 // ------------------------------------------------------- block reductions
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -327,7 +353,7 @@ struct KpArgs {
     PArrays<real> p;
     const NodeRec<real> *nodes;
     real *arena_out;
-    int32_t *box_out;     // per slot: touched local node box x0,x1,y0,y1,z0,z1
+    int4 *binfo;          // per brick (dense): {slot, touched node box x, y, z as lo | hi << 16}
     const uint32_t *occ_list, *occ_start, *occ_count;
     const int32_t *n_occ;
     int32_t occ_cap;
@@ -560,11 +586,30 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                     if (type == 1 && !g.neohookean) type = 3;
                     if (type == 2 || type == 5) type = 3;
                     const double E = g.has_pointE ? (double)P.E[i] : mt.E;
+                    // FP32 state: elastic constants of this point without divisions
+                    // on the common path (per-point E only when materials overflow
+                    // the table)
+                    double C = mt.C, ie = mt.invE, ig = mt.invG;
+                    if (FP32 && g.has_pointE) {
+                        C = E * mt.Cn;
+                        ie = 1.0 / E;
+                        ig = (2.0 * (1.0 + mt.nu)) * ie;
+                    }
                     if (type == 0) {
-                        real ds[6];
-                        elastic_inc<real>((real)E, (real)mt.nu, de, ds);
+                        if constexpr (FP32) {
+                            const float Cf = (float)C, nu = (float)mt.nu, a0 = 1.0f - nu, a1 = 0.5f - nu;
+                            s[0] += Cf * (a0 * de[0] + nu * de[1] + nu * de[2]);
+                            s[1] += Cf * (a0 * de[1] + nu * de[0] + nu * de[2]);
+                            s[2] += Cf * (a0 * de[2] + nu * de[0] + nu * de[1]);
+                            s[3] += Cf * (a1 * de[3]);
+                            s[4] += Cf * (a1 * de[4]);
+                            s[5] += Cf * (a1 * de[5]);
+                        } else {
+                            real ds[6];
+                            elastic_inc<real>((real)E, (real)mt.nu, de, ds);
 #pragma unroll
-                        for (int q = 0; q < 6; ++q) s[q] += ds[q];
+                            for (int q = 0; q < 6; ++q) s[q] += ds[q];
+                        }
                     } else if (type == 1) {
                         double Fd[9], sd[6], ds[6];
                         for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
@@ -574,10 +619,9 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                     } else if constexpr (FP32) {
                         // FP64 island for FP32 state
                         double dep[6];
-                        const double C = g.has_pointE ? E / (1.0 + mt.nu) / (1.0 - 2.0 * mt.nu) : mt.C;
                         const double al = (type == 4) ? (double)P.alpha[i] : 0.0;
-                        const double radius = mt.k0 + al / sqrt(2.0);
-                        if (radial_return_fp32(mt, C, radius, s, de, dep)) {
+                        const double radius = mt.k0 + al * 0.70710678118654752440;   // al / sqrt(2)
+                        if (radial_return_fp32(mt.nu, C, ie, ig, radius, s, de, dep)) {
                             if (type == 4) {
                                 double eff = 0.0;
 #pragma unroll
@@ -770,23 +814,29 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         box[2 * q] = lo;
         box[2 * q + 1] = hi;
     }
-    if (tid == 0)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) a.box_out[6 * slot + q] = box[q];
+    if (tid == 0) {
+        const bool empty = box[1] < box[0] || box[3] < box[2] || box[5] < box[4];
+        a.binfo[brick] = make_int4(empty ? -1 : slot, box[0] | (box[1] << 16), box[2] | (box[3] << 16),
+                                   box[4] | (box[5] << 16));
+    }
 
-    // ---- arena out (fixed point -> real) over the touched box only
+    // ---- arena out (fixed point -> real) over the touched box only, as
+    // box-compact node records of 8 reals {m, p.xyz, f.xyz, 0}
     if (box[1] >= box[0]) {
         const int bx = box[1] - box[0] + 1, by = box[3] - box[2] + 1, bz = box[5] - box[4] + 1;
         const int nbox = bx * by * bz;
-        real *out = a.arena_out + (size_t)slot * kQ * AN;
-        for (int t = tid; t < kQ * nbox; t += kBlock) {
-            const int q = t / nbox, r = t - q * nbox;
-            const int lx = box[0] + r % bx, ly = box[2] + (r / bx) % by, lz = box[4] + r / (bx * by);
-            const int nd = lx + A0 * ly + A01 * lz;
-            int e = kUnsetExp;
+        real *out = a.arena_out + (size_t)slot * 8 * AN;
+        double scl[kQ];
 #pragma unroll
-            for (int f = 0; f < kQ; ++f) e = (q == f) ? ecur[f] : e;
-            out[q * AN + nd] = (e == kUnsetExp) ? real(0) : (real)ldexp((double)s_arena[kQ * nd + q], e - FRAC);
+        for (int q = 0; q < kQ; ++q) scl[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, ecur[q] - FRAC);
+        for (int r = tid; r < nbox; r += kBlock) {
+            const int lx = box[0] + r % bx, ly = box[2] + (r / bx) % by, lz = box[4] + r / (bx * by);
+            const long long *src = s_arena + kQ * (lx + A0 * ly + A01 * lz);
+            real rec[8];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) rec[q] = (real)((double)src[q] * scl[q]);
+            rec[7] = real(0);
+            store8(out + 8 * (size_t)r, rec);
         }
     }
     // ---- fixed-order block sums of the checks, flags
@@ -812,12 +862,10 @@ template <typename real>
 struct K2Args {
     GridParams g;
     const real *arena_in;
-    const int32_t *box_in;       // per slot: touched node box
-    const int32_t *brick_slot;   // dense brick -> occupied slot or -1
-    const int32_t *k2_nb;        // per grid brick: 27 neighbour slots (dz,dy,dx order) or -1
-    const uint32_t *k2_list;
-    const int32_t *n_k2;
-    int32_t k2_cap;
+    const int4 *binfo;           // per brick (dense): {slot or -1, packed touched box x, y, z}
+    const real *zero_rec;        // 8 zero reals: the record of an uncovered candidate
+    const uint32_t *k2_list;     // grid-update bricks
+    int32_t n_k2;                // number of grid-update bricks
     NodeRec<real> *nodes;
     const uint8_t *fixed;
     const unsigned long long *marks;
@@ -826,99 +874,130 @@ struct K2Args {
     double *k2_part;             // per block: grid mass, reaction, active nodes
 };
 
-// Grid momentum, damping, BC, reaction, platen override (…_MP.cpp:209-265).
-// Arenas covering a node are summed in a fixed brick order (z, y, x offsets
-// ascending), so the grid state is independent of scheduling.  Nodes that no
-// point touches are left alone: no G2P reads them.  The 27 neighbour slots of
-// the brick (built at rebin) and their touched boxes are staged in LDS.
+// Grid momentum, damping, BC, reaction, platen override (…_MP.cpp:209-265),
+// one thread per node of the grid-update bricks.  A brick's arena window
+// spans nodes [O-1, O+S+1] per axis, so with S >= 3 a node at offset o of its
+// own brick is covered by at most two arenas per axis: its own and the
+// neighbour on the side o <= 1 (below) or o >= S-1 (above).  The (up to) 8
+// candidate records are loaded unconditionally -- uncovered candidates read a
+// shared zero record -- and summed in a fixed candidate order, so the grid
+// state is independent of scheduling.  Nodes that no point touches are left
+// alone: no G2P reads them.
 template <typename real>
 __global__ void __launch_bounds__(256) k_grid(K2Args<real> a, int step)
 {
     __shared__ double s_r[4][3];
-    __shared__ int s_nb[27];
-    __shared__ int s_bx[27][6];
     const GridParams &g = a.g;
-    const int kb = blockIdx.x;
-    if (kb >= *a.n_k2) return;
-    const uint32_t brick = a.k2_list[kb];
-    const int b[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
-    if (threadIdx.x < 27) {
-        const int sl = a.k2_nb[27 * kb + threadIdx.x];
-        s_nb[threadIdx.x] = sl;
-        if (sl >= 0)
-#pragma unroll
-            for (int q = 0; q < 6; ++q) s_bx[threadIdx.x][q] = a.box_in[6 * sl + q];
-    }
-    __syncthreads();
+    const int nS = g.S[0] * g.S[1] * g.S[2];
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
     const StepRec st = a.steps[step];
     const real dt = (real)st.dt;
     const real m_min = (real)g.m_min, damp = (real)g.damping;
-    const int nS = g.S[0] * g.S[1] * g.S[2];
-    const int AN = g.arenaN;
+    const int AN = g.arenaN;   // arena record capacity per slot
     double rm = 0.0, rr = 0.0, ra = 0.0;
-    for (int t = threadIdx.x; t < nS; t += blockDim.x) {
-        const int o[3] = {t % g.S[0], (t / g.S[0]) % g.S[1], t / (g.S[0] * g.S[1])};
-        int n[3];
-        bool in = true;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) { n[q] = b[q] * g.S[q] + o[q]; in &= n[q] < g.nodes[q]; }
-        if (!in) continue;
-        real sum[kQ];
-#pragma unroll
-        for (int q = 0; q < kQ; ++q) sum[q] = real(0);
-        bool covered = false;
-        for (int d = 0; d < 27; ++d) {
-            const int sl = s_nb[d];
-            if (sl < 0) continue;
-            // local node of the neighbour's arena: n - (origin' - 1)
-            const int lx = o[0] + 1 - (d % 3 - 1) * g.S[0];
-            const int ly = o[1] + 1 - ((d / 3) % 3 - 1) * g.S[1];
-            const int lz = o[2] + 1 - (d / 9 - 1) * g.S[2];
-            const int *bb = s_bx[d];
-            if (lx < bb[0] || lx > bb[1] || ly < bb[2] || ly > bb[3] || lz < bb[4] || lz > bb[5]) continue;
-            covered = true;
-            const real *ar = a.arena_in + (size_t)sl * kQ * AN + lx + g.A[0] * (ly + g.A[1] * lz);
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) sum[q] += ar[(size_t)q * AN];
-        }
-        if (!covered) continue;
-        const long long nid = n[0] + (long long)g.nodes[0] * (n[1] + (long long)g.nodes[1] * n[2]);
-        const real m = sum[0];
-        real v[3], f[3] = {sum[4], sum[5], sum[6]};
-#pragma unroll
-        for (int q = 0; q < 3; ++q) v[q] = (m > m_min) ? sum[1 + q] / m : real(0);
-        const real vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        if (sqrt(vv) > real(1.0e-9)) {
-            const real lf = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-            const real inv = real(1) / sqrt(vv);
-            const real sc = damp * lf;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) f[q] -= sc * (v[q] * inv);
-        }
-        if (m > m_min)
-#pragma unroll
-            for (int q = 0; q < 3; ++q) v[q] += f[q] / m * dt;
-        const uint8_t fx = a.fixed[nid];
-        if (fx & 1) { v[0] = real(0); f[0] = real(0); }
-        if (fx & 2) { v[1] = real(0); rr += (double)f[1]; f[1] = real(0); }
-        if (fx & 4) { v[2] = real(0); f[2] = real(0); }
-        const unsigned long long mk = a.marks[nid];
-        if ((uint32_t)(mk >> 32) == st.epoch + 1u) {
-            const int j = (int)(mk & 0xFFFFFFFFull);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) { v[q] = (real)a.dc_v[3 * j + q]; f[q] = real(0); }
-        }
-        NodeRec<real> rec;
+    int o[3], n[3], b[3];
+    bool in = gt < (uint32_t)a.n_k2 * (uint32_t)nS;
+    if (in) {
+        const uint32_t kb = gt / nS, t = gt - kb * nS;
+        const uint32_t brick = a.k2_list[kb];
+        b[0] = (int)(brick % g.nb[0]);
+        b[1] = (int)((brick / g.nb[0]) % g.nb[1]);
+        b[2] = (int)(brick / (g.nb[0] * g.nb[1]));
+        o[0] = (int)(t % g.S[0]);
+        o[1] = (int)((t / g.S[0]) % g.S[1]);
+        o[2] = (int)(t / (g.S[0] * g.S[1]));
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            rec.d[q] = v[q];
-            rec.d[3 + q] = (fabs(m) > m_min) ? f[q] / m : real(0);
+            n[q] = b[q] * g.S[q] + o[q];
+            in &= n[q] < g.nodes[q];
         }
-        rec.d[6] = m;
-        rec.d[7] = real(0);
-        a.nodes[nid] = rec;
-        rm += (double)m;
-        ra += (m > real(0)) ? 1.0 : 0.0;
+    }
+    if (in) {
+        const long long nid = n[0] + (long long)g.nodes[0] * (n[1] + (long long)g.nodes[1] * n[2]);
+        const uint8_t fx = a.fixed[nid];
+        const unsigned long long mk = a.marks[nid];
+        int side[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            side[q] = (o[q] <= 1) ? -1 : ((o[q] >= g.S[q] - 1) ? 1 : 0);
+            if (b[q] + side[q] < 0 || b[q] + side[q] >= g.nb[q]) side[q] = 0;
+        }
+        const real *src[8];
+        bool covered = false;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int e[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};   // 0: own brick, 1: neighbour side
+            int dd[3];
+            bool ok = true;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                dd[q] = e[q] ? side[q] : 0;
+                ok &= !(e[q] && side[q] == 0);
+            }
+            const size_t nbk = (b[0] + dd[0]) + (size_t)g.nb[0] * ((b[1] + dd[1]) + (size_t)g.nb[1] * (b[2] + dd[2]));
+            const int4 bi = a.binfo[nbk];
+            // local node of that arena: n - (origin' - 1)
+            const int l[3] = {o[0] + 1 - dd[0] * g.S[0], o[1] + 1 - dd[1] * g.S[1], o[2] + 1 - dd[2] * g.S[2]};
+            const int pk[3] = {bi.y, bi.z, bi.w};
+            int lo[3], ext[3];
+            ok &= bi.x >= 0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                lo[q] = pk[q] & 0xFFFF;
+                const int hi = pk[q] >> 16;
+                ext[q] = hi - lo[q] + 1;
+                ok &= l[q] >= lo[q] && l[q] <= hi;
+            }
+            const int r = (l[0] - lo[0]) + ext[0] * ((l[1] - lo[1]) + ext[1] * (l[2] - lo[2]));
+            src[c] = ok ? a.arena_in + (size_t)bi.x * 8 * AN + 8 * (size_t)r : a.zero_rec;
+            covered |= ok;
+        }
+        if (covered) {
+            real sum[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) sum[q] = real(0);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                real rec[8];
+                load8(src[c], rec);
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
+            }
+            const real m = sum[0];
+            real v[3], f[3] = {sum[4], sum[5], sum[6]};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[q] = (m > m_min) ? sum[1 + q] / m : real(0);
+            const real vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (sqrt(vv) > real(1.0e-9)) {
+                const real lf = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+                const real inv = real(1) / sqrt(vv);
+                const real sc = damp * lf;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) f[q] -= sc * (v[q] * inv);
+            }
+            if (m > m_min)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) v[q] += f[q] / m * dt;
+            if (fx & 1) { v[0] = real(0); f[0] = real(0); }
+            if (fx & 2) { v[1] = real(0); rr += (double)f[1]; f[1] = real(0); }
+            if (fx & 4) { v[2] = real(0); f[2] = real(0); }
+            if ((uint32_t)(mk >> 32) == st.epoch + 1u) {
+                const int j = (int)(mk & 0xFFFFFFFFull);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) { v[q] = (real)a.dc_v[3 * j + q]; f[q] = real(0); }
+            }
+            NodeRec<real> rec;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                rec.d[q] = v[q];
+                rec.d[3 + q] = (fabs(m) > m_min) ? f[q] / m : real(0);
+            }
+            rec.d[6] = m;
+            rec.d[7] = real(0);
+            a.nodes[nid] = rec;
+            rm += (double)m;
+            ra += (m > real(0)) ? 1.0 : 0.0;
+        }
     }
     rm = wave_sum(rm);
     rr = wave_sum(rr);
@@ -929,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_grid(K2Args<real> a, int step)
     if (threadIdx.x < 3) {
         double r = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_r[w][threadIdx.x];
-        a.k2_part[3 * kb + threadIdx.x] = r;
+        a.k2_part[3 * (size_t)blockIdx.x + threadIdx.x] = r;
     }
 }
 
