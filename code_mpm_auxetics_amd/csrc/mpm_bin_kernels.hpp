@@ -248,27 +248,7 @@ __global__ void k_spread(const uint32_t *perm, const uint32_t *head, const uint3
     perm_out[st + p] = perm[j];
 }
 
-__global__ void k_dilate(GridParams g, const uint32_t *occ_list, uint32_t n_occ, uint32_t *flag)
-{
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_occ) return;
-    const uint32_t b = occ_list[s];
-    const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1], bz = b / (g.nb[0] * g.nb[1]);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int x = bx + dx, y = by + dy, z = bz + dz;
-                if (x < 0 || y < 0 || z < 0 || x >= g.nb[0] || y >= g.nb[1] || z >= g.nb[2]) continue;
-                flag[x + g.nb[0] * (y + g.nb[1] * z)] = 1u;
-            }
-}
 
-__global__ void k_compact(const uint32_t *flag, const uint32_t *pos, uint32_t n, uint32_t *list)
-{
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n || !flag[j]) return;
-    list[pos[j]] = j;
-}
 
 // initial platen override marks for the first step after upload
 __global__ void k_dc_marks(GridParams g, const double *dc_x, int n_dc, uint32_t epoch,
@@ -323,27 +303,47 @@ __global__ void __launch_bounds__(kBlock) k_readback(GridParams g, PArrays<real>
     if (mask & 128u) o.vol[id] = det3(Fd) * (double)p.vol0[j];
 }
 
+// Reaction of each step of a batch (PhysicsEngine.cpp:65-71): block k sums the
+// per-slot grid partials of step k -- strided lane sums, then a fixed tree.
+__global__ void __launch_bounds__(256) k_step_sums(const double *gpart, int occ_cap, const int32_t *n_occ,
+                                                  double *hist)
+{
+    __shared__ double s[256];
+    const int t = threadIdx.x, no = *n_occ;
+    const double *gp = gpart + (size_t)blockIdx.x * occ_cap * 3;
+    double r = 0.0;
+    for (int i = t; i < no; i += 256) r += gp[3 * i + 1];
+    s[t] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) hist[blockIdx.x] = s[0];
+}
+
 // FP64 sums over the per-block partials (mpm_checks): one block of 256,
-// strided partial sums then a fixed tree -- deterministic.
-__global__ void __launch_bounds__(256) k_finalize(const double *kp_part, const int32_t *n_occ, const double *k2_part,
-                                                 const int32_t *n_k2, const double *dc_v, const double *dc_m,
-                                                 int n_dc, double *out)
+// strided partial sums then a fixed tree -- deterministic.  gpart: the last
+// step's grid partials (mass, reaction, active nodes per slot) or null.
+__global__ void __launch_bounds__(256) k_finalize(const double *kp_part, const int32_t *n_occ, const double *gpart,
+                                                 const double *dc_v, const double *dc_m, int n_dc, double *out)
 {
     __shared__ double s[8][256];
     const int t = threadIdx.x;
     double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int no = *n_occ, nk = *n_k2;
-    for (int i = t; i < nk; i += 256) {
-        r[0] += k2_part[3 * i];
-        r[1] += k2_part[3 * i + 1];
-        r[7] += k2_part[3 * i + 2];
-    }
-    for (int i = t; i < no; i += 256)
+    const int no = *n_occ;
+    for (int i = t; i < no; i += 256) {
+        if (gpart) {
+            r[0] += gpart[3 * i];
+            r[1] += gpart[3 * i + 1];
+            r[7] += gpart[3 * i + 2];
+        }
         for (int k = 0; k < 5; ++k) r[2 + k] += kp_part[5 * i + k];
+    }
     for (int j = t; j < n_dc; j += 256) {
         const double *v = &dc_v[3 * j];
-        const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        r[2] += dc_m[j] * (lv * lv);
+        const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        r[2] += dc_m[j] * vv;
         r[3] += dc_m[j] * v[0];
         r[4] += dc_m[j] * v[1];
         r[5] += dc_m[j] * v[2];

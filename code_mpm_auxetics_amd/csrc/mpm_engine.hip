@@ -59,25 +59,23 @@ struct mpm_handle {
     PArrays<double> pd[2]{};
     int cur = 0;
     double *xstage[3] = {nullptr, nullptr, nullptr};
-    double *dc_x = nullptr, *dc_v = nullptr, *dc_m = nullptr;
+    double *dc_x = nullptr, *dc_m = nullptr;
+    double *dc_v[2] = {nullptr, nullptr};   // platen velocities, ping-pong by step parity
     std::vector<int32_t> dc_id_host, def_id_host;
     mpm_points dc_saved{};                  // host copy of static platen fields
     std::vector<double> dc_saved_store;
 
-    // grid
-    void *nodes = nullptr;
+    // grid: no node arrays -- each step's grid update lives in LDS
     void *zero_rec = nullptr;               // 8 zero reals (uncovered arena candidates)
     uint8_t *fixed = nullptr;
-    unsigned long long *marks = nullptr;
+    unsigned long long *marks[2] = {nullptr, nullptr};   // platen override tags, ping-pong
 
     // bricks
     uint32_t *occ_list = nullptr, *occ_start = nullptr, *occ_count = nullptr;
-    int32_t *n_occ_d = nullptr, *n_k2_d = nullptr, *brick_slot = nullptr;
-    uint32_t *k2_list = nullptr, *k2flag = nullptr, *k2pos = nullptr;
-    int n_occ = 0, n_k2 = 0, occ_cap = 0, k2_cap = 0;
-    int n_k2b = 0;                          // grid-kernel blocks (= per-block check partials)
+    int32_t *n_occ_d = nullptr, *brick_slot = nullptr;
+    int n_occ = 0, occ_cap = 0;
     void *arena[2] = {nullptr, nullptr};
-    int4 *binfo = nullptr;                  // per brick (dense): {slot, packed touched node box}
+    int4 *binfo[2] = {nullptr, nullptr};    // per brick (dense): {slot, packed touched node box}
     size_t arena_cap_slots = 0;
     int parity = 0;
 
@@ -88,8 +86,10 @@ struct mpm_handle {
     StepRec *steps_d = nullptr, *steps_h = nullptr;
     int steps_cap = 0;
     int32_t *flags_d = nullptr, *flags_h = nullptr;
-    double *kp_part = nullptr, *k2_part = nullptr;
-    size_t kp_cap = 0, k2p_cap = 0;
+    double *kp_part = nullptr;
+    double *gpart = nullptr;                // [steps_cap][occ_cap][3] grid-update partials
+    int last_gk = -1;                       // batch index of the last grid update in gpart
+    size_t kp_cap = 0;
     double *hist_d = nullptr, *hist_h = nullptr;
     std::vector<double> hist;
     double *checks_d = nullptr;
@@ -268,32 +268,27 @@ static int alloc_points(mpm_handle *h, int n)
     return MPM_OK;
 }
 
-static int ensure_slots(mpm_handle *h, int n_occ, int n_k2)
+static int ensure_slots(mpm_handle *h, int n_occ)
 {
     const int want_occ = std::max(1, n_occ + n_occ / 4 + 16);
-    const int want_k2 = std::max(1, n_k2 + n_k2 / 4 + 16);
     if ((size_t)n_occ > h->arena_cap_slots || h->arena[0] == nullptr) {
         for (int b = 0; b < 2; ++b) dev_free(h->arena[b]);
         dev_free((void *&)h->kp_part);
+        dev_free((void *&)h->gpart);
         h->arena_cap_slots = (size_t)want_occ;
         for (int b = 0; b < 2; ++b)
             RCHK(dev_alloc(h, &h->arena[b], h->arena_cap_slots * 8 * h->g.arenaN * h->rsize));
         RCHK(dalloc(h, &h->kp_part, h->arena_cap_slots * 5));
         HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * 8, h->stream));
+        RCHK(dalloc(h, &h->gpart, (size_t)h->steps_cap * h->arena_cap_slots * 3));
         h->occ_cap = want_occ;
-    }
-    if ((size_t)n_k2 > h->k2p_cap || h->k2_part == nullptr) {
-        dev_free((void *&)h->k2_part);
-        h->k2p_cap = (size_t)want_k2;
-        RCHK(dalloc(h, &h->k2_part, h->k2p_cap * 3));
-        HIPCHK(h, hipMemsetAsync(h->k2_part, 0, h->k2p_cap * 3 * 8, h->stream));
-        h->k2_cap = want_k2;
+        h->last_gk = -1;
     }
     return MPM_OK;
 }
 
 template <typename real>
-static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index);
+static int launch_particle(mpm_handle *h, bool g2p, int step);
 
 // profiling: events recorded around launches, summed after each sync
 struct ProfMark {
@@ -338,10 +333,9 @@ static int rebin_t(mpm_handle *h, bool from_abs)
 {
     const uint32_t n = (uint32_t)h->n_def;
     if (n == 0) {
-        h->n_occ = h->n_k2 = h->n_k2b = 0;
-        RCHK(ensure_slots(h, 0, 0));
+        h->n_occ = 0;
+        RCHK(ensure_slots(h, 0));
         HIPCHK(h, hipMemsetAsync(h->n_occ_d, 0, 4, h->stream));
-        HIPCHK(h, hipMemsetAsync(h->n_k2_d, 0, 4, h->stream));
         return MPM_OK;
     }
     PArrays<real> &src = parr<real>(h, h->cur), &dst = parr<real>(h, 1 - h->cur);
@@ -359,7 +353,7 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     HIPCHK(h, hipGetLastError());
     uint32_t n_occ = 0;
     RCHK(scan_u32(h, h->heads, h->headpos, n, 0, &n_occ));
-    RCHK(ensure_slots(h, (int)n_occ, h->nbricks));
+    RCHK(ensure_slots(h, (int)n_occ));
     k_occ<<<nbk, kBlock, 0, h->stream>>>(sk, h->heads, h->headpos, n, h->g.cbits, h->occ_list, h->occ_start);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemsetAsync(h->brick_slot, 0xFF, (size_t)h->nbricks * 4, h->stream));
@@ -390,27 +384,16 @@ static int rebin_t(mpm_handle *h, bool from_abs)
                                                      from_abs ? h->xstage[2] : nullptr, perm, sk, n);
     HIPCHK(h, hipGetLastError());
     h->cur = 1 - h->cur;
-    HIPCHK(h, hipMemsetAsync(h->k2flag, 0, (size_t)h->nbricks * 4, h->stream));
-    k_dilate<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->g, h->occ_list, n_occ, h->k2flag);
-    HIPCHK(h, hipGetLastError());
-    uint32_t n_k2 = 0;
-    RCHK(scan_u32(h, h->k2flag, h->k2pos, (uint32_t)h->nbricks, 0, &n_k2));
-    k_compact<<<cdiv(h->nbricks, kBlock), kBlock, 0, h->stream>>>(h->k2flag, h->k2pos, (uint32_t)h->nbricks, h->k2_list);
-    HIPCHK(h, hipGetLastError());
-    const size_t nS = (size_t)h->g.S[0] * h->g.S[1] * h->g.S[2];
-    const int n_k2b = (int)cdiv((size_t)n_k2 * nS, 256);
-    RCHK(ensure_slots(h, (int)n_occ, n_k2b));
     h->n_occ = (int)n_occ;
-    h->n_k2 = (int)n_k2;
-    h->n_k2b = n_k2b;
-    // brick records are rewritten by the rebuild below; stale slots must read -1
-    HIPCHK(h, hipMemsetAsync(h->binfo, 0xFF, (size_t)h->nbricks * sizeof(int4), h->stream));
-    int32_t cnt[2] = {(int32_t)n_occ, (int32_t)n_k2b};
-    HIPCHK(h, hipMemcpyAsync(h->n_occ_d, &cnt[0], 4, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->n_k2_d, &cnt[1], 4, hipMemcpyHostToDevice, h->stream));
+    // brick records of both parities: stale slots must read -1; the rebuild
+    // below writes this parity's records of the occupied bricks
+    for (int b = 0; b < 2; ++b)
+        HIPCHK(h, hipMemsetAsync(h->binfo[b], 0xFF, (size_t)h->nbricks * sizeof(int4), h->stream));
+    int32_t cnt = (int32_t)n_occ;
+    HIPCHK(h, hipMemcpyAsync(h->n_occ_d, &cnt, 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     // rebuild the arena the next grid update reads
-    RCHK(launch_particle<real>(h, false, 0, 0));
+    RCHK(launch_particle<real>(h, false, 0));
     h->rebins++;
     return MPM_OK;
 }
@@ -422,16 +405,21 @@ static int rebin(mpm_handle *h, bool from_abs)
 
 // ---------------------------------------------------------------- launches
 template <typename real>
-static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index)
+static int launch_particle(mpm_handle *h, bool g2p, int step)
 {
-    if (h->n_def == 0 && !(g2p && h->n_dc > 0)) return MPM_OK;
     KpArgs<real> a{};
     a.g = h->g;
     a.p = parr<real>(h, h->cur);
-    a.nodes = (const NodeRec<real> *)h->nodes;
-    // G2P(n)+P2G(n+1) writes the arena of step n+1; a rebuild writes step n's
-    a.arena_out = (real *)h->arena[g2p ? (h->parity ^ 1) : h->parity];
-    a.binfo = h->binfo;
+    const int p = h->parity;
+    // a step reads parity p and writes p^1; a rebuild (P2G only) writes p
+    a.arena_in = (const real *)h->arena[p];
+    a.binfo_in = h->binfo[p];
+    a.arena_out = (real *)h->arena[g2p ? (p ^ 1) : p];
+    a.binfo_out = h->binfo[g2p ? (p ^ 1) : p];
+    a.zero_rec = (const real *)h->zero_rec;
+    a.fixed = h->fixed;
+    a.marks_in = h->marks[p];
+    a.marks_out = h->marks[p ^ 1];
     a.occ_list = h->occ_list;
     a.occ_start = h->occ_start;
     a.occ_count = h->occ_count;
@@ -441,15 +429,12 @@ static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index)
     a.steps = h->steps_d;
     a.flags = h->flags_d;
     a.kp_part = h->kp_part;
+    a.gpart = h->gpart + (size_t)step * h->occ_cap * 3;
     a.n_dc = h->n_dc;
     a.dc_x = h->dc_x;
-    a.dc_v = h->dc_v;
-    a.marks = h->marks;
-    a.k2_part = h->k2_part;
-    a.n_k2 = h->n_k2_d;
-    a.hist = h->hist_d;
-    a.hist_index = hist_index;
-    const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long);
+    a.dc_v_in = h->dc_v[p];
+    a.dc_v_out = h->dc_v[p ^ 1];
+    const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long) + (g2p ? (size_t)6 * h->g.arenaN * h->rsize : 0);
     unsigned grid = (unsigned)h->n_occ;
     if (g2p) grid += cdiv(h->n_dc, kBlock);
     if (grid == 0) return MPM_OK;
@@ -457,28 +442,6 @@ static int launch_particle(mpm_handle *h, bool g2p, int step, int hist_index)
         k_particle<real, true><<<grid, kBlock, lds, h->stream>>>(a, step);
     else
         k_particle<real, false><<<grid, kBlock, lds, h->stream>>>(a, step);
-    HIPCHK(h, hipGetLastError());
-    return MPM_OK;
-}
-
-template <typename real>
-static int launch_grid(mpm_handle *h, int step)
-{
-    if (h->n_k2 == 0) return MPM_OK;
-    K2Args<real> a{};
-    a.g = h->g;
-    a.arena_in = (const real *)h->arena[h->parity];
-    a.binfo = h->binfo;
-    a.zero_rec = (const real *)h->zero_rec;
-    a.k2_list = h->k2_list;
-    a.n_k2 = h->n_k2;
-    a.nodes = (NodeRec<real> *)h->nodes;
-    a.fixed = h->fixed;
-    a.marks = h->marks;
-    a.dc_v = h->dc_v;
-    a.steps = h->steps_d;
-    a.k2_part = h->k2_part;
-    k_grid<real><<<(unsigned)h->n_k2b, 256, 0, h->stream>>>(a, step);
     HIPCHK(h, hipGetLastError());
     return MPM_OK;
 }
@@ -537,13 +500,17 @@ static int step_impl(mpm_handle *h, double dt_total, int32_t *finished)
         HIPCHK(h, hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
         std::vector<ProfMark> marks;
         for (int k = 0; k < in_batch; ++k) {
-            int pe = prof_begin(h, marks, 1);
-            RCHK(launch_grid<real>(h, k));
-            prof_end(h, pe);
-            pe = prof_begin(h, marks, 0);
-            RCHK(launch_particle<real>(h, true, k, k));
+            const int pe = prof_begin(h, marks, 0);
+            RCHK(launch_particle<real>(h, true, k));
             prof_end(h, pe);
             h->parity ^= 1;
+        }
+        h->last_gk = in_batch - 1;
+        if (h->n_occ > 0) {
+            k_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, h->hist_d);
+            HIPCHK(h, hipGetLastError());
+        } else {
+            HIPCHK(h, hipMemsetAsync(h->hist_d, 0, sizeof(double) * in_batch, h->stream));
         }
         HIPCHK(h, hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * in_batch, hipMemcpyDeviceToHost, h->stream));
         bool need = false;
@@ -599,8 +566,9 @@ static int step_impl(mpm_handle *h, double dt_total, int32_t *finished)
 
 static int checks_impl(mpm_handle *h, double out[12])
 {
-    k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, h->k2_part, h->n_k2_d, h->dc_v, h->dc_m,
-                                       h->n_dc, h->checks_d);
+    const double *gp = (h->last_gk >= 0) ? h->gpart + (size_t)h->last_gk * h->occ_cap * 3 : nullptr;
+    k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, gp, h->dc_v[h->parity], h->dc_m, h->n_dc,
+                                       h->checks_d);
     HIPCHK(h, hipGetLastError());
     double r[8] = {0};
     HIPCHK(h, hipMemcpyAsync(r, h->checks_d, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -636,13 +604,13 @@ static void destroy_all(mpm_handle *h)
         dev_free((void *&)h->vals[b]);
     }
     for (int a = 0; a < 3; ++a) dev_free((void *&)h->xstage[a]);
-    void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v, (void **)&h->dc_m, &h->nodes, &h->zero_rec, (void **)&h->fixed,
-                     (void **)&h->marks, (void **)&h->occ_list, (void **)&h->occ_start, (void **)&h->occ_count,
-                     (void **)&h->n_occ_d, (void **)&h->n_k2_d, (void **)&h->brick_slot, (void **)&h->k2_list,
-                     (void **)&h->k2flag, (void **)&h->k2pos, (void **)&h->mats, (void **)&h->steps_d,
-                     (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->k2_part, (void **)&h->hist_d,
+    void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v[0], (void **)&h->dc_v[1], (void **)&h->dc_m, &h->zero_rec,
+                     (void **)&h->fixed, (void **)&h->marks[0], (void **)&h->marks[1], (void **)&h->occ_list,
+                     (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d, (void **)&h->brick_slot,
+                     (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats, (void **)&h->steps_d,
+                     (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->hist_d,
                      (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
-                     (void **)&h->headpos, (void **)&h->gdesc, (void **)&h->binfo};
+                     (void **)&h->headpos, (void **)&h->gdesc};
     for (void **p : ptrs) dev_free(*p);
     for (auto *p : h->scan_scratch) { void *q = p; dev_free(q); }
     h->scan_scratch.clear();
@@ -720,7 +688,24 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         g.A[a] = S + 3;
     }
     g.arenaN = g.A[0] * g.A[1] * g.A[2];
-    if ((size_t)kQ * g.arenaN * 8 > 64 * 1024) { h->err = "brick too large for the LDS arena"; return fail(MPM_E_ARG); }
+    {
+        // dynamic LDS of the step kernel: fixed-point arena + node window
+        // (plus about 1 KB of static LDS)
+        const size_t need = (size_t)kQ * g.arenaN * 8 + (size_t)6 * g.arenaN * h->rsize;
+        int maxlds = 0;
+        if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess)
+            maxlds = 64 * 1024;
+        if (need + 1024 > (size_t)maxlds) { h->err = "brick too large for the LDS arena and node window"; return fail(MPM_E_ARG); }
+        hipError_t e1, e2;
+        if (h->fp32) {
+            e1 = hipFuncSetAttribute((const void *)k_particle<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
+            e2 = hipFuncSetAttribute((const void *)k_particle<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
+        } else {
+            e1 = hipFuncSetAttribute((const void *)k_particle<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
+            e2 = hipFuncSetAttribute((const void *)k_particle<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
+        }
+        if (e1 != hipSuccess || e2 != hipSuccess) { h->err = "hipFuncSetAttribute (dynamic LDS) failed"; return fail(MPM_E_HIP); }
+    }
     const int ncb = g.S[0] * g.S[1] * g.S[2];
     g.cbits = 0;
     while ((1 << g.cbits) < ncb) ++g.cbits;
@@ -734,13 +719,13 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     h->nnodes = (long long)g.nodes[0] * g.nodes[1] * g.nodes[2];
 
     int rc;
-    if ((rc = dev_alloc(h, &h->nodes, (size_t)h->nnodes * 8 * h->rsize))) return fail(rc);
     if ((rc = dev_alloc(h, &h->zero_rec, 64))) return fail(rc);
     if (hipMemset(h->zero_rec, 0, 64) != hipSuccess) { h->err = "hipMemset failed"; return fail(MPM_E_HIP); }
     if ((rc = dalloc(h, &h->fixed, (size_t)h->nnodes))) return fail(rc);
-    if ((rc = dalloc(h, &h->marks, (size_t)h->nnodes))) return fail(rc);
-    (void)hipMemsetAsync(h->nodes, 0, (size_t)h->nnodes * 8 * h->rsize, h->stream);
-    (void)hipMemsetAsync(h->marks, 0, (size_t)h->nnodes * 8, h->stream);
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = dalloc(h, &h->marks[b], (size_t)h->nnodes))) return fail(rc);
+        (void)hipMemsetAsync(h->marks[b], 0, (size_t)h->nnodes * 8, h->stream);
+    }
     if (cfg->node_fixed) {
         if (hipMemcpyAsync(h->fixed, cfg->node_fixed, (size_t)h->nnodes, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
             h->err = "copy node_fixed";
@@ -751,12 +736,11 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     }
     const size_t nb = (size_t)h->nbricks;
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
-        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->n_k2_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) ||
-        (rc = dalloc(h, &h->k2_list, nb)) || (rc = dalloc(h, &h->binfo, nb)) || (rc = dalloc(h, &h->k2flag, nb)) || (rc = dalloc(h, &h->k2pos, nb)) ||
+        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) ||
+        (rc = dalloc(h, &h->binfo[0], nb)) || (rc = dalloc(h, &h->binfo[1], nb)) ||
         (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)))
         return fail(rc);
     (void)hipMemsetAsync(h->n_occ_d, 0, 4, h->stream);
-    (void)hipMemsetAsync(h->n_k2_d, 0, 4, h->stream);
     (void)hipMemsetAsync(h->flags_d, 0, 4, h->stream);
     h->steps_cap = 64;
     if ((rc = dalloc(h, &h->steps_d, h->steps_cap)) || (rc = dalloc(h, &h->hist_d, h->steps_cap))) return fail(rc);
@@ -872,7 +856,7 @@ static int upload_t(mpm_handle *h, const mpm_points *pts)
     }
     // platens
     RCHK(dalloc(h, &h->dc_x, 3 * (size_t)std::max(h->n_dc, 1)));
-    RCHK(dalloc(h, &h->dc_v, 3 * (size_t)std::max(h->n_dc, 1)));
+    for (int b = 0; b < 2; ++b) RCHK(dalloc(h, &h->dc_v[b], 3 * (size_t)std::max(h->n_dc, 1)));
     RCHK(dalloc(h, &h->dc_m, (size_t)std::max(h->n_dc, 1)));
     {
         std::vector<double> x(3 * h->n_dc + 1), v(3 * h->n_dc + 1), m(h->n_dc + 1);
@@ -885,7 +869,8 @@ static int upload_t(mpm_handle *h, const mpm_points *pts)
             m[j] = pts->mass[i];
         }
         HIPCHK(h, hipMemcpy(h->dc_x, x.data(), sizeof(double) * 3 * h->n_dc, hipMemcpyHostToDevice));
-        HIPCHK(h, hipMemcpy(h->dc_v, v.data(), sizeof(double) * 3 * h->n_dc, hipMemcpyHostToDevice));
+        for (int b = 0; b < 2; ++b)
+            HIPCHK(h, hipMemcpy(h->dc_v[b], v.data(), sizeof(double) * 3 * h->n_dc, hipMemcpyHostToDevice));
         HIPCHK(h, hipMemcpy(h->dc_m, m.data(), sizeof(double) * h->n_dc, hipMemcpyHostToDevice));
         // static platen fields for readback (F, stress, ... never change)
         h->dc_saved_store.assign((size_t)h->n_dc * 36, 0.0);
@@ -900,13 +885,14 @@ static int upload_t(mpm_handle *h, const mpm_points *pts)
             o[28] = pts->vol ? pts->vol[i] : (pts->vol0 ? pts->vol0[i] : 0.0);
         }
     }
-    HIPCHK(h, hipMemsetAsync(h->marks, 0, (size_t)h->nnodes * 8, h->stream));
+    for (int b = 0; b < 2; ++b) HIPCHK(h, hipMemsetAsync(h->marks[b], 0, (size_t)h->nnodes * 8, h->stream));
+    h->parity = 0;
     if (h->n_dc > 0) {
-        k_dc_marks<<<cdiv(h->n_dc, 256), 256, 0, h->stream>>>(h->g, h->dc_x, h->n_dc, (uint32_t)h->cycle, h->marks);
+        k_dc_marks<<<cdiv(h->n_dc, 256), 256, 0, h->stream>>>(h->g, h->dc_x, h->n_dc, (uint32_t)h->cycle,
+                                                             h->marks[h->parity]);
         HIPCHK(h, hipGetLastError());
     }
     HIPCHK(h, hipMemsetAsync(h->flags_d, 0, 4, h->stream));
-    h->parity = 0;
     RCHK(rebin(h, true));
     bool need = false;
     RCHK(check_flags(h, &need));
@@ -962,7 +948,7 @@ static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
     std::vector<double> hx(3 * (size_t)h->n_dc + 1), hv(3 * (size_t)h->n_dc + 1);
     if (h->n_dc > 0) {
         HIPCHK(h, hipMemcpyAsync(hx.data(), h->dc_x, sizeof(double) * 3 * h->n_dc, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipMemcpyAsync(hv.data(), h->dc_v, sizeof(double) * 3 * h->n_dc, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(hv.data(), h->dc_v[h->parity], sizeof(double) * 3 * h->n_dc, hipMemcpyDeviceToHost, h->stream));
     }
     for (int k = 0; k < 8; ++k)
         if (st[k] && h->n_def > 0) {

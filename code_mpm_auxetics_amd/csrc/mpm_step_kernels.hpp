@@ -1,18 +1,22 @@
 // mpm_step_kernels.hpp -- the per-step kernels of the explicit MPM step.
 //
-//   k_particle<real, G2P>: one workgroup per occupied brick.  For each point of
-//     the brick: G2P of step n (gather 8 nodes), F / volume / strain update,
-//     constitutive update, state write-back, then the P2G contributions of
-//     step n+1 (m, m v, -V sigma.grad N + N f_ext to the 8 corner nodes).
+//   k_particle<real, G2P>: one launch per time step, one workgroup per
+//     occupied brick.  Phase A: the grid update of step n for the nodes of
+//     the brick's touched box -- the P2G sums of the (at most 8) brick arenas
+//     covering each node, in a canonical brick order, then momentum, damping,
+//     boundary conditions, reaction and the platen override -- into an LDS
+//     node window.  Phase B, for each point of the brick: G2P of step n
+//     (gather 8 window nodes), F / volume / strain update, constitutive
+//     update, state write-back, then the P2G contributions of step n+1
+//     (m, m v, -V sigma.grad N + N f_ext to the 8 corner nodes).
 //     Every contribution is converted to fixed point with a per-brick,
 //     per-field exponent and added to the brick's LDS arena with a 64-bit
 //     integer add.  Integer addition is associative, so the arena is
 //     independent of wave scheduling: the step is bit-reproducible with no
 //     float atomics anywhere.  With G2P=false the kernel only rebuilds the
 //     arena (after upload or a rebin).
-//   k_grid<real>: one workgroup per brick of the dilated active set: merges the
-//     arenas covering each owned node in a fixed order, then momentum,
-//     damping, boundary conditions, reaction and the platen override.
+//   Blocks of one launch read step n's arenas, brick records, platen marks and
+//   platen velocities and write step n+1's: all four are ping-pong buffers.
 //
 // Reference: PhysicsEngine_runSimulation_Classic_SinglePass_MP.cpp:41-368.
 #pragma once
@@ -351,9 +355,14 @@ template <typename real>
 struct KpArgs {
     GridParams g;
     PArrays<real> p;
-    const NodeRec<real> *nodes;
-    real *arena_out;
-    int4 *binfo;          // per brick (dense): {slot, touched node box x, y, z as lo | hi << 16}
+    const real *arena_in;       // step n: P2G sums per slot (box-compact 8-real records)
+    const int4 *binfo_in;       // step n: per brick {slot or -1, packed touched box}
+    real *arena_out;            // step n+1
+    int4 *binfo_out;            // step n+1
+    const real *zero_rec;       // 8 zero reals: the record of an uncovered candidate
+    const uint8_t *fixed;       // node fixity bits
+    const unsigned long long *marks_in;   // platen override tags of step n
+    unsigned long long *marks_out;        // ... of step n+1
     const uint32_t *occ_list, *occ_start, *occ_count;
     const int32_t *n_occ;
     int32_t occ_cap;
@@ -361,13 +370,11 @@ struct KpArgs {
     const StepRec *steps;
     int32_t *flags;
     double *kp_part;      // per slot: 5 doubles (sum m|v|^2, m v.xyz, nonfinite)
+    double *gpart;        // per slot: 3 doubles of this step's grid update (mass, reaction, active nodes)
     int32_t n_dc;
-    double *dc_x, *dc_v;  // 3 per platen point, FP64
-    unsigned long long *marks;
-    const double *k2_part;  // per grid block: 3 doubles (mass, reaction, active nodes)
-    const int32_t *n_k2;
-    double *hist;           // per step reaction history
-    int32_t hist_index;
+    double *dc_x;                       // 3 per platen point, FP64
+    const double *dc_v_in;              // platen velocities of step n
+    double *dc_v_out;                   // ... after this step
 };
 
 // Fixed-point fraction bits.  FP32 state: each contribution is converted to a
@@ -382,8 +389,10 @@ __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); 
 template <typename real, bool G2P>
 __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : MPM_KP_WAVES64) k_particle(KpArgs<real> a, int step)
 {
-    extern __shared__ long long s_arena[];   // node-major: kQ int64 per arena node
+    extern __shared__ long long s_arena[];   // node-major: kQ int64 per arena node, then
+                                             // (G2P) the node window: 6 reals {v, a} per node
     __shared__ uint32_t s_max[8];            // running max bound bits per field
+    __shared__ int4 s_bi[27];                // neighbour brick records (dz, dy, dx ascending)
     __shared__ int s_box[kBlock / 64][6];
     __shared__ double s_red[kBlock / 64][5];
 
@@ -399,10 +408,13 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         if (!G2P) return;
         const int j = ((int)blockIdx.x - a.occ_cap) * kBlock + tid;
         if (j >= a.n_dc) return;
-        double *x = &a.dc_x[3 * j], *v = &a.dc_v[3 * j];
+        double *x = &a.dc_x[3 * j], v[3];
         // …_MP.cpp:360-368: velocity from the TimeLine at the step's start time
-        if (st.has_tl) { v[0] = st.vtl[0]; v[1] = st.vtl[1]; v[2] = st.vtl[2]; }
-        for (int q = 0; q < 3; ++q) x[q] += v[q] * dtd;
+        for (int q = 0; q < 3; ++q) {
+            v[q] = st.has_tl ? st.vtl[q] : a.dc_v_in[3 * j + q];
+            a.dc_v_out[3 * j + q] = v[q];
+            x[q] += v[q] * dtd;
+        }
         // override marks for step n+1 (…_MP.cpp:245-265): the highest platen
         // index touching a node wins, as in the reference's sequential loop.
         // The tag carries epoch (n+1)+1 so that 0 means "no mark".
@@ -412,7 +424,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         for (int k = 0; k < 8; ++k) {
             const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
             if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= g.nodes[0] || n1 >= g.nodes[1] || n2 >= g.nodes[2]) continue;
-            atomicMax(&a.marks[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
+            atomicMax(&a.marks_out[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
         }
         return;
     }
@@ -420,18 +432,135 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
     const int slot = blockIdx.x;
     if (slot >= *a.n_occ) return;
     const uint32_t brick = a.occ_list[slot];
-    const int O[3] = {(int)(brick % g.nb[0]) * g.S[0], (int)((brick / g.nb[0]) % g.nb[1]) * g.S[1],
-                      (int)(brick / (g.nb[0] * g.nb[1])) * g.S[2]};
+    const int bc[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
+    const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
     const uint32_t start = a.occ_start[slot], count = a.occ_count[slot];
     const int A0 = g.A[0], A01 = g.A[0] * g.A[1], AN = g.arenaN;
 
-    // reaction history of the preceding grid update (PhysicsEngine.cpp:65-71)
-    if (G2P && slot == 0 && wv == 0) {
-        double r = 0.0;
-        const int nk = *a.n_k2;
-        for (int i = lane; i < nk; i += 64) r += a.k2_part[3 * i + 1];
-        r = wave_sum(r);
-        if (lane == 0) a.hist[a.hist_index] = r;
+    // ---- phase A: grid update of step n over the node window = this brick's
+    // touched box of the last P2G (…_MP.cpp:209-265)
+    real *s_node = reinterpret_cast<real *>(s_arena + kQ * AN);
+    int wlo[3] = {0, 0, 0}, wext[3] = {0, 0, 0};
+    if (G2P) {
+        if (tid < 27) {
+            const int x = bc[0] + tid % 3 - 1, y = bc[1] + (tid / 3) % 3 - 1, z = bc[2] + tid / 9 - 1;
+            int4 bi = make_int4(-1, 0, 0, 0);
+            if (x >= 0 && y >= 0 && z >= 0 && x < g.nb[0] && y < g.nb[1] && z < g.nb[2])
+                bi = a.binfo_in[x + (size_t)g.nb[0] * (y + (size_t)g.nb[1] * z)];
+            s_bi[tid] = bi;
+        }
+        __syncthreads();
+        const int4 own = s_bi[13];
+        const int opk[3] = {own.y, own.z, own.w};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            wlo[q] = opk[q] & 0xFFFF;
+            wext[q] = (own.x >= 0) ? (opk[q] >> 16) - wlo[q] + 1 : 0;
+        }
+        const int nwin = wext[0] * wext[1] * wext[2];
+        const real dtg = (real)st.dt, m_min = (real)g.m_min, damp = (real)g.damping;
+        double gs[3] = {0.0, 0.0, 0.0};   // owned nodes: mass, reaction, active count
+        for (int r = tid; r < nwin; r += kBlock) {
+            const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
+            const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
+            const uint8_t fx = a.fixed[nid];
+            const unsigned long long mk = a.marks_in[nid];
+            // per axis the arenas covering this node: own and the neighbour below
+            // (l <= 2) or above (l >= S); candidates in ascending brick order
+            int lower[3];
+            bool two[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const bool lo_n = l[q] <= 2 && bc[q] > 0, hi_n = l[q] >= g.S[q] && bc[q] + 1 < g.nb[q];
+                lower[q] = lo_n ? -1 : 0;
+                two[q] = lo_n || hi_n;
+            }
+            const real *src[8];
+            int first = -1;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int e[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+                int dd[3];
+                bool ok = true;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    dd[q] = lower[q] + e[q];
+                    ok &= !(e[q] && !two[q]);
+                }
+                const int4 bi = s_bi[(dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1)];
+                const int pk[3] = {bi.y, bi.z, bi.w};
+                int lp[3], lo[3], ext[3];
+                ok &= bi.x >= 0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    lp[q] = l[q] - dd[q] * g.S[q];   // node in that arena's window
+                    lo[q] = pk[q] & 0xFFFF;
+                    const int hi = pk[q] >> 16;
+                    ext[q] = hi - lo[q] + 1;
+                    ok &= lp[q] >= lo[q] && lp[q] <= hi;
+                }
+                const int rr = (lp[0] - lo[0]) + ext[0] * ((lp[1] - lo[1]) + ext[1] * (lp[2] - lo[2]));
+                src[c] = ok ? a.arena_in + (size_t)bi.x * 8 * AN + 8 * (size_t)rr : a.zero_rec;
+                if (ok && first < 0) first = c;
+            }
+            const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
+            real sum[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) sum[q] = real(0);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                real rec[8];
+                load8(src[c], rec);
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
+            }
+            const real mn = sum[0];
+            real vn[3], fn[3] = {sum[4], sum[5], sum[6]};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) vn[q] = (mn > m_min) ? sum[1 + q] / mn : real(0);
+            const real vv = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+            if (sqrt(vv) > real(1.0e-9)) {   // damping (…_MP.cpp:209-222)
+                const real lf = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+                const real inv = real(1) / sqrt(vv);
+                const real sc = damp * lf;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) fn[q] -= sc * (vn[q] * inv);
+            }
+            if (mn > m_min)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) vn[q] += fn[q] / mn * dtg;
+            double react = 0.0;
+            if (fx & 1) { vn[0] = real(0); fn[0] = real(0); }
+            if (fx & 2) { vn[1] = real(0); react = (double)fn[1]; fn[1] = real(0); }
+            if (fx & 4) { vn[2] = real(0); fn[2] = real(0); }
+            if ((uint32_t)(mk >> 32) == st.epoch + 1u) {   // platen override (…_MP.cpp:245-265)
+                const int j = (int)(mk & 0xFFFFFFFFull);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) { vn[q] = (real)a.dc_v_in[3 * j + q]; fn[q] = real(0); }
+            }
+            real *wn = s_node + 6 * r;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                wn[q] = vn[q];
+                wn[3 + q] = (fabs(mn) > m_min) ? fn[q] / mn : real(0);
+            }
+            if (first == cown) {   // counted once: by the lowest covering brick
+                gs[0] += (double)mn;
+                gs[1] += react;
+                gs[2] += (mn > real(0)) ? 1.0 : 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double w = wave_sum(gs[k]);
+            if (lane == 0) s_red[wv][k] = w;
+        }
+        __syncthreads();
+        if (tid < 3) {
+            double t = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) t += s_red[w][tid];
+            a.gpart[3 * (size_t)slot + tid] = t;
+        }
     }
 
     for (int i = tid; i < kQ * AN; i += kBlock) s_arena[i] = 0;
@@ -483,10 +612,15 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                 axes_of(x, xq, g, O, bx, by, bz);
                 const bool oob = bx.cell < 0 || bx.cell >= g.cells[0] || by.cell < 0 || by.cell >= g.cells[1] ||
                                  bz.cell < 0 || bz.cell >= g.cells[2];
-                if (oob) {
-                    flags |= FLAG_OUT_OF_GRID;
+                // the 8 corner nodes in the window (the P2G of step n put them there)
+                const int wl[3] = {bx.cell - (O[0] - 1) - wlo[0], by.cell - (O[1] - 1) - wlo[1], bz.cell - (O[2] - 1) - wlo[2]};
+                const bool outw = wl[0] < 0 || wl[0] + 1 >= wext[0] || wl[1] < 0 || wl[1] + 1 >= wext[1] ||
+                                  wl[2] < 0 || wl[2] + 1 >= wext[2];
+                if (oob || outw) {
+                    flags |= oob ? FLAG_OUT_OF_GRID : FLAG_ARENA;
                 } else {
-                    const long long nb0 = bx.cell + (long long)g.nodes[0] * (by.cell + (long long)g.nodes[1] * bz.cell);
+                    const real *w0 = s_node + 6 * (wl[0] + wext[0] * (wl[1] + wext[1] * wl[2]));
+                    const int sy_ = wext[0], sz_ = wext[0] * wext[1];
                     real L[9];
 #pragma unroll
                     for (int q = 0; q < 9; ++q) L[q] = real(0);
@@ -494,8 +628,6 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                     if constexpr (FP32) {
                         // 8 corners fully unrolled; node record {v.xyz a.x | a.y a.z m -}
                         const Hat32 w = hat32(bx, by, bz);
-                        const NodeRec<float> *nb = a.nodes + nb0;
-                        const long long sy_ = g.nodes[0], sz_ = (long long)g.nodes[0] * g.nodes[1];
                         float av[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
@@ -503,9 +635,8 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                             const float N = w.wxy[2 * ax + ay] * w.wz[az];
                             const float G[3] = {w.gx[ax] * w.wyz[2 * ay + az], w.gy[ay] * w.wxz[2 * ax + az],
                                                 w.gz[az] * w.wxy[2 * ax + ay]};
-                            const float4 *np = reinterpret_cast<const float4 *>(nb + ax + sy_ * ay + sz_ * az);
-                            const float4 p0 = np[0], p1 = np[1];
-                            const float vn[3] = {p0.x, p0.y, p0.z}, an[3] = {p0.w, p1.x, p1.y};
+                            const real *wp = w0 + 6 * (ax + sy_ * ay + sz_ * az);
+                            const float vn[3] = {wp[0], wp[1], wp[2]}, an[3] = {wp[3], wp[4], wp[5]};
 #pragma unroll
                             for (int q = 0; q < 3; ++q) {
                                 av[q] += N * an[q];
@@ -526,16 +657,15 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                         for (int k = 0; k < 8; ++k) {
                             real N, G[3];
                             corner(bx, by, bz, k, N, G);
-                            const long long nid = nb0 + ((k >> 2) & 1) + (long long)g.nodes[0] * (((k >> 1) & 1) + (long long)g.nodes[1] * (k & 1));
-                            const NodeRec<real> nr = a.nodes[nid];
+                            const real *wp = w0 + 6 * (((k >> 2) & 1) + sy_ * ((k >> 1) & 1) + sz_ * (k & 1));
 #pragma unroll
-                            for (int q = 0; q < 3; ++q) v[q] += N * nr.d[3 + q] * dt;
+                            for (int q = 0; q < 3; ++q) v[q] += N * wp[3 + q] * dt;
 #pragma unroll
-                            for (int q = 0; q < 3; ++q) x[q] += N * nr.d[q] * dt;
+                            for (int q = 0; q < 3; ++q) x[q] += N * wp[q] * dt;
 #pragma unroll
                             for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-                                for (int r = 0; r < 3; ++r) L[3 * cc + r] += nr.d[r] * G[cc];
+                                for (int r = 0; r < 3; ++r) L[3 * cc + r] += wp[r] * G[cc];
                         }
                     }
                     // F += (L F) dt  (…_MP.cpp:299)
@@ -816,7 +946,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
     }
     if (tid == 0) {
         const bool empty = box[1] < box[0] || box[3] < box[2] || box[5] < box[4];
-        a.binfo[brick] = make_int4(empty ? -1 : slot, box[0] | (box[1] << 16), box[2] | (box[3] << 16),
+        a.binfo_out[brick] = make_int4(empty ? -1 : slot, box[0] | (box[1] << 16), box[2] | (box[3] << 16),
                                    box[4] | (box[5] << 16));
     }
 
@@ -854,161 +984,6 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         double r = 0.0;
         for (int w = 0; w < kBlock / 64; ++w) r += s_red[w][tid];
         a.kp_part[5 * slot + tid] = r;
-    }
-}
-
-// ------------------------------------------------------------------ grid
-template <typename real>
-struct K2Args {
-    GridParams g;
-    const real *arena_in;
-    const int4 *binfo;           // per brick (dense): {slot or -1, packed touched box x, y, z}
-    const real *zero_rec;        // 8 zero reals: the record of an uncovered candidate
-    const uint32_t *k2_list;     // grid-update bricks
-    int32_t n_k2;                // number of grid-update bricks
-    NodeRec<real> *nodes;
-    const uint8_t *fixed;
-    const unsigned long long *marks;
-    const double *dc_v;
-    const StepRec *steps;
-    double *k2_part;             // per block: grid mass, reaction, active nodes
-};
-
-// Grid momentum, damping, BC, reaction, platen override (…_MP.cpp:209-265),
-// one thread per node of the grid-update bricks.  A brick's arena window
-// spans nodes [O-1, O+S+1] per axis, so with S >= 3 a node at offset o of its
-// own brick is covered by at most two arenas per axis: its own and the
-// neighbour on the side o <= 1 (below) or o >= S-1 (above).  The (up to) 8
-// candidate records are loaded unconditionally -- uncovered candidates read a
-// shared zero record -- and summed in a fixed candidate order, so the grid
-// state is independent of scheduling.  Nodes that no point touches are left
-// alone: no G2P reads them.
-template <typename real>
-__global__ void __launch_bounds__(256) k_grid(K2Args<real> a, int step)
-{
-    __shared__ double s_r[4][3];
-    const GridParams &g = a.g;
-    const int nS = g.S[0] * g.S[1] * g.S[2];
-    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const StepRec st = a.steps[step];
-    const real dt = (real)st.dt;
-    const real m_min = (real)g.m_min, damp = (real)g.damping;
-    const int AN = g.arenaN;   // arena record capacity per slot
-    double rm = 0.0, rr = 0.0, ra = 0.0;
-    int o[3], n[3], b[3];
-    bool in = gt < (uint32_t)a.n_k2 * (uint32_t)nS;
-    if (in) {
-        const uint32_t kb = gt / nS, t = gt - kb * nS;
-        const uint32_t brick = a.k2_list[kb];
-        b[0] = (int)(brick % g.nb[0]);
-        b[1] = (int)((brick / g.nb[0]) % g.nb[1]);
-        b[2] = (int)(brick / (g.nb[0] * g.nb[1]));
-        o[0] = (int)(t % g.S[0]);
-        o[1] = (int)((t / g.S[0]) % g.S[1]);
-        o[2] = (int)(t / (g.S[0] * g.S[1]));
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            n[q] = b[q] * g.S[q] + o[q];
-            in &= n[q] < g.nodes[q];
-        }
-    }
-    if (in) {
-        const long long nid = n[0] + (long long)g.nodes[0] * (n[1] + (long long)g.nodes[1] * n[2]);
-        const uint8_t fx = a.fixed[nid];
-        const unsigned long long mk = a.marks[nid];
-        int side[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            side[q] = (o[q] <= 1) ? -1 : ((o[q] >= g.S[q] - 1) ? 1 : 0);
-            if (b[q] + side[q] < 0 || b[q] + side[q] >= g.nb[q]) side[q] = 0;
-        }
-        const real *src[8];
-        bool covered = false;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int e[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};   // 0: own brick, 1: neighbour side
-            int dd[3];
-            bool ok = true;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                dd[q] = e[q] ? side[q] : 0;
-                ok &= !(e[q] && side[q] == 0);
-            }
-            const size_t nbk = (b[0] + dd[0]) + (size_t)g.nb[0] * ((b[1] + dd[1]) + (size_t)g.nb[1] * (b[2] + dd[2]));
-            const int4 bi = a.binfo[nbk];
-            // local node of that arena: n - (origin' - 1)
-            const int l[3] = {o[0] + 1 - dd[0] * g.S[0], o[1] + 1 - dd[1] * g.S[1], o[2] + 1 - dd[2] * g.S[2]};
-            const int pk[3] = {bi.y, bi.z, bi.w};
-            int lo[3], ext[3];
-            ok &= bi.x >= 0;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                lo[q] = pk[q] & 0xFFFF;
-                const int hi = pk[q] >> 16;
-                ext[q] = hi - lo[q] + 1;
-                ok &= l[q] >= lo[q] && l[q] <= hi;
-            }
-            const int r = (l[0] - lo[0]) + ext[0] * ((l[1] - lo[1]) + ext[1] * (l[2] - lo[2]));
-            src[c] = ok ? a.arena_in + (size_t)bi.x * 8 * AN + 8 * (size_t)r : a.zero_rec;
-            covered |= ok;
-        }
-        if (covered) {
-            real sum[kQ];
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) sum[q] = real(0);
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                real rec[8];
-                load8(src[c], rec);
-#pragma unroll
-                for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
-            }
-            const real m = sum[0];
-            real v[3], f[3] = {sum[4], sum[5], sum[6]};
-#pragma unroll
-            for (int q = 0; q < 3; ++q) v[q] = (m > m_min) ? sum[1 + q] / m : real(0);
-            const real vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            if (sqrt(vv) > real(1.0e-9)) {
-                const real lf = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-                const real inv = real(1) / sqrt(vv);
-                const real sc = damp * lf;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) f[q] -= sc * (v[q] * inv);
-            }
-            if (m > m_min)
-#pragma unroll
-                for (int q = 0; q < 3; ++q) v[q] += f[q] / m * dt;
-            if (fx & 1) { v[0] = real(0); f[0] = real(0); }
-            if (fx & 2) { v[1] = real(0); rr += (double)f[1]; f[1] = real(0); }
-            if (fx & 4) { v[2] = real(0); f[2] = real(0); }
-            if ((uint32_t)(mk >> 32) == st.epoch + 1u) {
-                const int j = (int)(mk & 0xFFFFFFFFull);
-#pragma unroll
-                for (int q = 0; q < 3; ++q) { v[q] = (real)a.dc_v[3 * j + q]; f[q] = real(0); }
-            }
-            NodeRec<real> rec;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                rec.d[q] = v[q];
-                rec.d[3 + q] = (fabs(m) > m_min) ? f[q] / m : real(0);
-            }
-            rec.d[6] = m;
-            rec.d[7] = real(0);
-            a.nodes[nid] = rec;
-            rm += (double)m;
-            ra += (m > real(0)) ? 1.0 : 0.0;
-        }
-    }
-    rm = wave_sum(rm);
-    rr = wave_sum(rr);
-    ra = wave_sum(ra);
-    const int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { s_r[wv][0] = rm; s_r[wv][1] = rr; s_r[wv][2] = ra; }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        double r = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_r[w][threadIdx.x];
-        a.k2_part[3 * (size_t)blockIdx.x + threadIdx.x] = r;
     }
 }
 
