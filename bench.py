@@ -64,6 +64,17 @@ def run_gpu(sc, precision, steps, warmup, profile_steps):
     return res
 
 
+def measured_traffic():
+    """HBM bytes per launch of the particle kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by a 256 MiB copy), or None."""
+    f = os.path.join(ROOT, "profiles", "k_particle_fp32_config2_1M.traffic.json")
+    if not os.path.exists(f):
+        return None, None
+    with open(f) as fh:
+        d = json.load(fh)
+    return int(d["traffic_bytes"]), d.get("source")
+
+
 def cpu_baseline(sc, seconds=15.0):
     from oracle.oracle import Oracle
     orc = Oracle(sc, nthreads=1, strain_form=0)
@@ -135,8 +146,9 @@ def main():
     nodes = r32["checks_prof"]["grid_nodes"]
     bytes_launch = n * MP_BYTES[("j2", 4)] + nodes * NODE_BYTES_PARTICLE_KERNEL[4]
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic()
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
             "kernel": "k_particle<float,true>", "avg_launch_ms": round(avg_ms, 5),
             "algorithmic_bytes_per_launch": int(bytes_launch)}
     step_bytes = n * MP_BYTES[("j2", 4)] + nodes * 108

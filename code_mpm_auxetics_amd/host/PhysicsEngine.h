@@ -1,0 +1,195 @@
+// PhysicsEngine.h -- C++ host mirror of the reference PhysicsEngine's step
+// interface, on top of the C ABI in include/mpm_auxetics.h.
+//
+// Reference: PhysicsEngine.h:34-93 (class PhysicsEngine), MaterialPoint.h:18-60
+// (class MaterialPoint).  Method names, argument meaning and return values are
+// the reference's; the state lives on the GPU and the MaterialPoint objects
+// returned by getMaterialPoints() are host mirrors refreshed by a readback.
+// glm is not used: dvec3 is std::array<double, 3>, dmat3 is a column-major
+// std::array<double, 9> (m[col][row] at [3*col+row], as glm stores it).
+//
+// Errors: every C ABI status other than MPM_OK throws mpm_host::Error with
+// the library's message (the reference exits through fatalError instead).
+#pragma once
+
+#include <array>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_auxetics.h"
+
+namespace mpm_host {
+
+enum MaterialType { _ELASTIC, _NEOHOOKEAN, _VISCOELASTIC, _PLASTIC, _VONMISESHARDENING, _GASS };
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error("mpm error " + std::to_string(c) + ": " + m), code(c) {}
+};
+
+// MaterialPoint.h:18-60 fields (monitor flags and viscosity are carried but unused by the step)
+class MaterialPoint {
+  public:
+    bool b_DisplacementControl = false;
+    bool b_Mark_Force = false;
+    bool b_Mark_Stress = false;
+    unsigned int i_ID = 0;
+    unsigned int i_MaterialType = _ELASTIC;
+    double d_Mass = 0.0, d_Volume = 0.0, d_Volume_Initial = 0.0;
+    double d_ElasticModulus = 0.0, d_PoissonRatio = 0.0, d_YieldStress = 0.0, d_Viscosity = 0.0;
+    bool b_Surface = false;
+    double d_Kernel = 0.0;
+    std::array<double, 3> d3_Kernel_Gradient{};
+    std::array<double, 3> d3_Position{}, d3_Velocity{}, d3_Force_External{};
+    std::array<double, 9> d33_DeformationGradient{1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double d6_Strain[6] = {0, 0, 0, 0, 0, 0};
+    double d6_Stress[6] = {0, 0, 0, 0, 0, 0};
+    double d6_Strain_Plastic[6] = {0, 0, 0, 0, 0, 0};
+    double d_BackStress_Isotropic = 0.0;
+    double d_Hardening_Isotropic_C0 = 0.0;
+    double d_Hardening_Isotropic_C1 = 0.0;
+};
+
+class PhysicsEngine {
+  public:
+    // The protected members initializeWorld_* sets (PhysicsEngine.h:68-93)
+    // come in through mpm_config; node fixity is cfg.node_fixed.
+    explicit PhysicsEngine(const mpm_config &cfg)
+    {
+        mpm_handle *h = nullptr;
+        check(mpm_create(&cfg, &h), nullptr);
+        h_.reset(h);
+        t_end_ = cfg.time_end;
+    }
+
+    // The point part of initializeWorld_* (…PolygonCell.cpp:137-191): every
+    // point, displacement-controlled ones flagged by b_DisplacementControl.
+    void setMaterialPoints(const std::vector<MaterialPoint> &pts)
+    {
+        const size_t n = pts.size();
+        std::vector<double> x(3 * n), v(3 * n), fe(3 * n), F(9 * n), eps(6 * n), sig(6 * n), ep(6 * n);
+        std::vector<double> al(n), m(n), vol(n), vol0(n), E(n), nu(n), sy(n), c0(n), c1(n);
+        std::vector<int32_t> ty(n);
+        std::vector<uint8_t> dc(n), surf(n);
+        for (size_t i = 0; i < n; ++i) {
+            const MaterialPoint &p = pts[i];
+            for (int a = 0; a < 3; ++a) {
+                x[3 * i + a] = p.d3_Position[a];
+                v[3 * i + a] = p.d3_Velocity[a];
+                fe[3 * i + a] = p.d3_Force_External[a];
+            }
+            for (int a = 0; a < 9; ++a) F[9 * i + a] = p.d33_DeformationGradient[a];
+            for (int a = 0; a < 6; ++a) {
+                eps[6 * i + a] = p.d6_Strain[a];
+                sig[6 * i + a] = p.d6_Stress[a];
+                ep[6 * i + a] = p.d6_Strain_Plastic[a];
+            }
+            al[i] = p.d_BackStress_Isotropic;
+            m[i] = p.d_Mass;
+            vol[i] = p.d_Volume;
+            vol0[i] = p.d_Volume_Initial;
+            E[i] = p.d_ElasticModulus;
+            nu[i] = p.d_PoissonRatio;
+            sy[i] = p.d_YieldStress;
+            c0[i] = p.d_Hardening_Isotropic_C0;
+            c1[i] = p.d_Hardening_Isotropic_C1;
+            ty[i] = (int32_t)p.i_MaterialType;
+            dc[i] = p.b_DisplacementControl ? 1 : 0;
+            surf[i] = p.b_Surface ? 1 : 0;
+        }
+        mpm_points P{};
+        P.n = (int32_t)n;
+        P.x = x.data(); P.v = v.data(); P.fext = fe.data(); P.F = F.data();
+        P.strain = eps.data(); P.stress = sig.data(); P.eps_p = ep.data(); P.alpha = al.data();
+        P.mass = m.data(); P.vol = vol.data(); P.vol0 = vol0.data();
+        P.E = E.data(); P.nu = nu.data(); P.sy = sy.data(); P.C0 = c0.data(); P.C1 = c1.data();
+        P.type = ty.data(); P.dc = dc.data(); P.surface = surf.data();
+        check(mpm_upload_points(h_.get(), &P), h_.get());
+        mirror_ = pts;
+        stale_ = false;
+    }
+
+    // PhysicsEngine.h:45-46: advance dTimeIncrement_Total in steps of at most
+    // the maximum increment; 1 once d_Time >= d_TimeEnd, else 0.
+    int runSimulation_Classic_SinglePass_MP(double dTimeIncrement_Total) { return step(dTimeIncrement_Total, MPM_PATH_PLAIN); }
+    int runSimulation_Classic_SinglePass_MP_Contact(double dTimeIncrement_Total) { return step(dTimeIncrement_Total, MPM_PATH_CONTACT); }
+
+    // PhysicsEngine.h:57-60
+    double getTime_Runtime() const { double ms = 0; mpm_last_step_ms(h_.get(), &ms); return ms * 1e-3; }
+    double getTime_Current() const { double t, te, dt; mpm_get_time(h_.get(), &t, &te, &dt); return t; }
+    double getTime_End() const { return t_end_; }
+    double getTime_Increment() const { double t, te, dt; mpm_get_time(h_.get(), &t, &te, &dt); return dt; }
+
+    // PhysicsEngine.h:62-64: the caller reads the points' current state
+    unsigned int getCount_MaterialPoint() const { return (unsigned int)mirror_.size(); }
+    std::vector<MaterialPoint *> getMaterialPoints()
+    {
+        refresh();
+        std::vector<MaterialPoint *> out;
+        out.reserve(mirror_.size());
+        for (MaterialPoint &p : mirror_) out.push_back(&p);
+        return out;
+    }
+
+    // reportConsole sums (PhysicsEngine.cpp:33-97): grid mass, reaction
+    // (Force_y), sum m|v|^2, momentum, non-finite count, time, cycle, ...
+    std::array<double, 12> checks() const
+    {
+        std::array<double, 12> c{};
+        check(mpm_checks(h_.get(), c.data()), h_.get());
+        return c;
+    }
+
+  private:
+    struct Del {
+        void operator()(mpm_handle *h) const { mpm_destroy(h); }
+    };
+    std::unique_ptr<mpm_handle, Del> h_;
+    std::vector<MaterialPoint> mirror_;
+    bool stale_ = true;
+    double t_end_ = 0.0;
+
+    static void check(int rc, const mpm_handle *h)
+    {
+        if (rc != MPM_OK) throw Error(rc, mpm_last_error(h));
+    }
+    int step(double dt_total, int path)
+    {
+        int32_t fin = 0;
+        check(mpm_step(h_.get(), dt_total, path, &fin), h_.get());
+        stale_ = true;
+        return fin;
+    }
+    void refresh()
+    {
+        if (!stale_) return;
+        const size_t n = mirror_.size();
+        std::vector<double> x(3 * n), v(3 * n), F(9 * n), sig(6 * n), eps(6 * n), ep(6 * n), al(n), vol(n);
+        mpm_points P{};
+        P.n = (int32_t)n;
+        P.x = x.data(); P.v = v.data(); P.F = F.data(); P.stress = sig.data(); P.strain = eps.data();
+        P.eps_p = ep.data(); P.alpha = al.data(); P.vol = vol.data();
+        const uint32_t mask = MPM_F_X | MPM_F_V | MPM_F_F | MPM_F_STRESS | MPM_F_STRAIN | MPM_F_EPSP | MPM_F_ALPHA | MPM_F_VOL;
+        check(mpm_readback(h_.get(), mask, &P), h_.get());
+        for (size_t i = 0; i < n; ++i) {
+            MaterialPoint &p = mirror_[i];
+            for (int a = 0; a < 3; ++a) {
+                p.d3_Position[a] = x[3 * i + a];
+                p.d3_Velocity[a] = v[3 * i + a];
+            }
+            for (int a = 0; a < 9; ++a) p.d33_DeformationGradient[a] = F[9 * i + a];
+            for (int a = 0; a < 6; ++a) {
+                p.d6_Stress[a] = sig[6 * i + a];
+                p.d6_Strain[a] = eps[6 * i + a];
+                p.d6_Strain_Plastic[a] = ep[6 * i + a];
+            }
+            p.d_BackStress_Isotropic = al[i];
+            p.d_Volume = vol[i];
+        }
+        stale_ = false;
+    }
+};
+
+}  // namespace mpm_host

@@ -204,3 +204,33 @@ def test_out_of_grid_is_an_error(mpm):
         eng.run_steps(200)
     assert ei.value.code in (-5, -6)
     eng.close()
+
+
+def test_cpp_host_mirror_free_flight(mpm, tmp_path):
+    """The C++ PhysicsEngine mirror (game-loop style calls of
+    runSimulation_Classic_SinglePass_MP until it returns 1) moves a stress-free
+    block in free flight: mean position x0 + v t_end, velocity unchanged."""
+    import os
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    import code_mpm_auxetics_amd._abi as A
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.dirname(A.LIB_PATH)
+    exe = tmp_path / "demo"
+    subprocess.run([gxx, "-std=c++17", "-O1", os.path.join(root, "tests", "host_engine_demo.cpp"), "-o", str(exe),
+                    "-L", libdir, "-lmpm_auxetics", f"-Wl,-rpath,{libdir}"], check=True)
+    print("running host demo", flush=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = r.stdout.split()
+    n, t = int(f[0]), float(f[1])
+    mx, mv = np.array(list(map(float, f[2:5]))), np.array(list(map(float, f[5:8])))
+    h = 0.001
+    x0 = np.array([(6.0 + 0.5 * 3.5 + 0.25) * h + 1.5e-6, (6.0 + 0.5 * 3.5 + 0.25) * h, (6.0 + 0.5 * 1.5 + 0.25) * h])
+    v = np.array([1.5, -0.5, 0.25])
+    assert n == 256 and t == pytest.approx(40e-6, rel=1e-12)
+    assert np.allclose(mv, v, rtol=0, atol=1e-12)
+    assert np.allclose(mx, x0 + v * t, rtol=0, atol=1e-12)

@@ -257,3 +257,101 @@ def test_openmp_privatized_grid_matches_serial(O):
     for k in ("x", "v", "stress"):
         ref = a.points()[k]
         assert np.allclose(b.points()[k], ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_one_point_step_by_hand(O):
+    """One stressed point, worked through the step by hand (…_MP.cpp:106-312):
+    m_I = N_I m, v_I = v, f_I = -V sigma.grad N_I, v_I += f_I/m_I dt, then
+    G2P.  For an interior point sum_I grad N_I = 0, so v and x move as in
+    free flight while L = -(V dt / m) sum_I (sigma grad N_I) (x) grad N_I / N_I,
+    F = I + L dt and V = det(F) V0.  Pins the force sign, the Voigt shear
+    mapping, the per-contribution /m_I, the outer-product orientation
+    (L[j][i] = v_i dN/dx_j) and the F update."""
+    h = 0.001
+    cells = (8, 8, 8)
+    x = np.array([[3.37 * h, 4.71 * h, 2.23 * h]])
+    v = np.array([[0.4, -0.3, 0.2]])
+    m, V0, dt = 2.0e-6, 1.0e-9, 1.0e-6
+    sig = np.array([3.0e6, -1.0e6, 2.0e6, 4.0e5, -7.0e5, 9.0e5])     # xx yy zz xy yz zx
+    sc = scene.Scene(length=tuple(c * h for c in cells), cells=cells,
+                     fixed=np.zeros(9 ** 3, np.uint8), x=x, mass=np.array([m]), vol0=np.array([V0]),
+                     E=np.array([1e7]), nu=np.array([0.3]), sy=np.zeros(1), C0=np.zeros(1), C1=np.zeros(1),
+                     type=np.zeros(1, np.int32), dc=np.zeros(1, np.uint8), v=v.copy(), dt_max=dt,
+                     time_end=1.0, damping=0.0, mass_min=0.0)
+    orc = O.Oracle(sc)
+    orc.a["stress"][0] = sig
+    orc.step(dt)
+    p = orc.points()
+    S = np.array([[sig[0], sig[3], sig[5]], [sig[3], sig[1], sig[4]], [sig[5], sig[4], sig[2]]])
+    fr = x[0] / h - np.floor(x[0] / h)
+    L = np.zeros((3, 3))
+    for c in np.ndindex(2, 2, 2):
+        w = [fr[a] if c[a] else 1.0 - fr[a] for a in range(3)]
+        g = [(1.0 if c[a] else -1.0) / h for a in range(3)]
+        N = w[0] * w[1] * w[2]
+        G = np.array([g[0] * w[1] * w[2], w[0] * g[1] * w[2], w[0] * w[1] * g[2]])
+        fI = -V0 * (S @ G)
+        vI = v[0] + fI / (N * m) * dt
+        L += np.outer(vI, G)
+    F = np.eye(3) + L * dt
+    assert np.allclose(p["v"][0], v[0], rtol=0, atol=1e-12)
+    assert np.allclose(p["x"][0], x[0] + v[0] * dt, rtol=0, atol=1e-18)
+    Fo = p["F"][0].reshape(3, 3).T            # column-major storage -> F[r, c]
+    assert np.allclose(Fo, F, rtol=0, atol=1e-13)
+    assert np.abs(L).max() * dt > 1e-6            # the stress term is visible
+    assert p["vol"][0] == pytest.approx(np.linalg.det(F) * V0, rel=1e-13)
+
+
+def test_contact_correction_conserves_momentum_and_resists_approach(O):
+    """Two blocks closing on each other across half a cell, facing layers
+    flagged as surface points.  The two-field step finds contact nodes
+    (b_Contact_Negative), and the normal correction (…_MP_Contact.cpp:445-480)
+    must leave no approaching normal velocity at any of them
+    ((v+ - v-).n >= 0 with n = normalize(-(grad m+ - grad m-)/2)), keep total
+    momentum (equal and opposite node forces) and slow the closing speed of
+    the facing layers below the single-field step's."""
+    h = 0.001
+    cells = (16, 8, 6)
+    off = h / 2
+
+    def block(x0, nx):
+        xs = [(x0 + i) * h + (k + 0.5) * off for i in range(nx) for k in range(2)]
+        ys = [(2 + i) * h + (k + 0.5) * off for i in range(4) for k in range(2)]
+        zs = [(2 + i) * h + (k + 0.5) * off for i in range(2) for k in range(2)]
+        return np.array([(a, b, c) for a in xs for b in ys for c in zs])
+    A, B = block(3, 4), block(7.5, 4)
+    X = np.vstack([A, B])
+    n = len(X)
+    vel = np.zeros((n, 3))
+    vel[: len(A), 0] = 2.0
+    vel[len(A):, 0] = -2.0
+    surf = np.zeros(n, np.uint8)
+    surf[: len(A)][A[:, 0] > A[:, 0].max() - 0.6 * off] = 1
+    surf[len(A):][B[:, 0] < B[:, 0].min() + 0.6 * off] = 1
+    vol = off ** 3
+    sc = scene.Scene(length=tuple(c * h for c in cells), cells=cells,
+                     fixed=np.zeros(np.prod([c + 1 for c in cells]), np.uint8), x=X,
+                     mass=np.full(n, 1000 * vol), vol0=np.full(n, vol), E=np.full(n, 1e7),
+                     nu=np.full(n, 0.3), sy=np.zeros(n), C0=np.zeros(n), C1=np.zeros(n),
+                     type=np.zeros(n, np.int32), dc=np.zeros(n, np.uint8), v=vel, surface=surf,
+                     dt_max=1e-7, time_end=1.0, damping=0.0, mass_min=0.0)
+    c, pl = O.Oracle(sc), O.Oracle(sc)
+    p0 = (sc.mass[:, None] * sc.v).sum(axis=0)
+    c.step(sc.dt_max, 1)
+    pl.step(sc.dt_max, 0)
+    a = c.a
+    idx = np.nonzero(a["gcneg"])[0]
+    assert len(idx) > 0
+    both = idx[(a["gm"][idx] > 0) & (a["gmn"][idx] > 0)]
+    assert len(both) > 0
+    nrm = -0.5 * (a["gmg"][both] - a["gmgn"][both])
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    rel = ((a["gv"][both] - a["gvn"][both]) * nrm).sum(axis=1)
+    assert rel.min() >= -1e-12 * 2.0
+    vc, vp = c.points()["v"], pl.points()["v"]
+    p1 = (sc.mass[:, None] * vc).sum(axis=0)
+    assert np.allclose(p1, p0, rtol=0, atol=1e-12 * np.abs(sc.mass[:, None] * sc.v).sum())
+    fa, fb = (surf == 1) & (np.arange(n) < len(A)), (surf == 1) & (np.arange(n) >= len(A))
+    closing_c = vc[fa, 0].mean() - vc[fb, 0].mean()
+    closing_p = vp[fa, 0].mean() - vp[fb, 0].mean()
+    assert closing_c < closing_p
