@@ -62,6 +62,7 @@ class PhysicsEngine {
         check(mpm_create(&cfg, &h), nullptr);
         h_.reset(h);
         t_end_ = cfg.time_end;
+        track_strain_ = cfg.track_strain != 0;
     }
 
     // The point part of initializeWorld_* (…PolygonCell.cpp:137-191): every
@@ -150,6 +151,7 @@ class PhysicsEngine {
     std::vector<MaterialPoint> mirror_;
     bool stale_ = true;
     double t_end_ = 0.0;
+    bool track_strain_ = false;
 
     static void check(int rc, const mpm_handle *h)
     {
@@ -171,7 +173,8 @@ class PhysicsEngine {
         P.n = (int32_t)n;
         P.x = x.data(); P.v = v.data(); P.F = F.data(); P.stress = sig.data(); P.strain = eps.data();
         P.eps_p = ep.data(); P.alpha = al.data(); P.vol = vol.data();
-        const uint32_t mask = MPM_F_X | MPM_F_V | MPM_F_F | MPM_F_STRESS | MPM_F_STRAIN | MPM_F_EPSP | MPM_F_ALPHA | MPM_F_VOL;
+        const uint32_t mask = MPM_F_X | MPM_F_V | MPM_F_F | MPM_F_STRESS | MPM_F_EPSP | MPM_F_ALPHA | MPM_F_VOL |
+                              (track_strain_ ? MPM_F_STRAIN : 0u);
         check(mpm_readback(h_.get(), mask, &P), h_.get());
         for (size_t i = 0; i < n; ++i) {
             MaterialPoint &p = mirror_[i];
@@ -182,7 +185,7 @@ class PhysicsEngine {
             for (int a = 0; a < 9; ++a) p.d33_DeformationGradient[a] = F[9 * i + a];
             for (int a = 0; a < 6; ++a) {
                 p.d6_Stress[a] = sig[6 * i + a];
-                p.d6_Strain[a] = eps[6 * i + a];
+                if (track_strain_) p.d6_Strain[a] = eps[6 * i + a];
                 p.d6_Strain_Plastic[a] = ep[6 * i + a];
             }
             p.d_BackStress_Isotropic = al[i];

@@ -43,5 +43,9 @@ def test_fp64_gpu_matches_fixture(name):
     es = rel_l2(g["stress"], out["stress"])
     print(f"{name}: rel L2 disp {ex:.3e} vel {ev:.3e} stress {es:.3e}", flush=True)
     assert ex <= 1e-9 and ev <= 1e-9 and es <= 1e-9
-    assert rel_l2(eng.reaction_history(), out["reaction"]) <= 1e-8
+    # reaction (sum of fixed-y node forces): relative to its scale, with an
+    # absolute floor for worlds where it is round-off only (|R| < 1e-20 N)
+    hg, ho = eng.reaction_history(), out["reaction"]
+    assert len(hg) == len(ho)
+    assert np.abs(hg - ho).max() <= max(1e-8 * np.abs(ho).max(), 1e-20)
     eng.close()
