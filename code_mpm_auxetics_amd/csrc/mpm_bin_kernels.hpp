@@ -266,6 +266,13 @@ __global__ void k_dc_marks(GridParams g, const double *dc_x, int n_dc, uint32_t 
     }
 }
 
+// position of the point with original index id (contact path: point 0)
+__global__ void k_find_id(const uint32_t *ids, uint32_t n, uint32_t id, uint32_t *out)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n && ids[j] == id) out[1] = j;
+}
+
 // ------------------------------------------------------------ readback
 struct ReadbackOut {
     double *x, *v, *F, *s, *eps, *ep, *alpha, *vol;
@@ -325,21 +332,22 @@ __global__ void __launch_bounds__(256) k_step_sums(const double *gpart, int occ_
 // FP64 sums over the per-block partials (mpm_checks): one block of 256,
 // strided partial sums then a fixed tree -- deterministic.  gpart: the last
 // step's grid partials (mass, reaction, active nodes per slot) or null.
-__global__ void __launch_bounds__(256) k_finalize(const double *kp_part, const int32_t *n_occ, const double *gpart,
+// Partial counts: n_pp / n_gp, or *n_pp_d / *n_gp_d when those are given.
+__global__ void __launch_bounds__(256) k_finalize(const double *kp_part, const int32_t *n_pp_d, int n_pp,
+                                                 const double *gpart, const int32_t *n_gp_d, int n_gp,
                                                  const double *dc_v, const double *dc_m, int n_dc, double *out)
 {
     __shared__ double s[8][256];
     const int t = threadIdx.x;
     double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int no = *n_occ;
-    for (int i = t; i < no; i += 256) {
-        if (gpart) {
-            r[0] += gpart[3 * i];
-            r[1] += gpart[3 * i + 1];
-            r[7] += gpart[3 * i + 2];
-        }
-        for (int k = 0; k < 5; ++k) r[2 + k] += kp_part[5 * i + k];
+    const int npp = n_pp_d ? *n_pp_d : n_pp, ngp = gpart ? (n_gp_d ? *n_gp_d : n_gp) : 0;
+    for (int i = t; i < ngp; i += 256) {
+        r[0] += gpart[3 * i];
+        r[1] += gpart[3 * i + 1];
+        r[7] += gpart[3 * i + 2];
     }
+    for (int i = t; i < npp; i += 256)
+        for (int k = 0; k < 5; ++k) r[2 + k] += kp_part[5 * i + k];
     for (int j = t; j < n_dc; j += 256) {
         const double *v = &dc_v[3 * j];
         const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];

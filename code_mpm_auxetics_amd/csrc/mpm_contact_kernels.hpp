@@ -1,0 +1,698 @@
+// mpm_contact_kernels.hpp -- the two-field contact step
+// (PhysicsEngine_runSimulation_Classic_SinglePass_MP_Contact.cpp:30-646).
+//
+// One launch per phase over points or grid nodes.  Every node sum is a 64-bit
+// fixed-point integer add (global atomics), with per-field scales fixed for
+// the step by bounds taken over the points first, so the result is
+// independent of scheduling.  Order-dependent choices of the reference's
+// sequential loops are restated as order-free rules:
+//   nearest point of a node   = smallest (distance, point index), starting
+//                               from point 0 (…_MP_Contact.cpp:228-233)
+//   surface kernel gradient   = largest |grad k| of a surface point, ties to
+//                               the smallest index (…:246-249)
+//   platen override           = the highest platen index wins, the node's
+//                               force is moved to the reaction once (…:494-518)
+// Node values are FP64 for both state precisions; point state is read and
+// written in its own precision.
+#pragma once
+#include "mpm_step_kernels.hpp"
+
+namespace mpm {
+
+constexpr uint32_t CF_ACTIVE = 1, CF_KACTIVE = 2, CF_CPOS = 4, CF_CNEG = 8;
+constexpr int kCB = 6;   // bounds: m, m/h, |v|, |force|, 1 (kernel), spare
+
+template <typename real>
+struct CArgs {
+    GridParams g;
+    PArrays<real> p;
+    int32_t n_def, n_dc, n_all;
+    const uint8_t *surf;          // per deformable point (current order)
+    const double *dc_x, *dc_v, *dc_m, *dc_vol, *dc_s, *dc_fext;
+    double *dc_x_out;             // platen positions after the step (may alias dc_x)
+    const uint8_t *dc_surf;
+    const uint32_t *dc_id;
+    const uint32_t *def_id0;      // (is platen, index) of original point 0: [0] flag, [1] index
+    uint8_t surf0;                // surface flag of original point 0
+    long long nn;                 // node count
+    const uint8_t *fixed;         // node fixity bits
+    // node data
+    long long *acc;               // 21 x nn int64: k | m[2] | mg[2][3] | mv[2][3] | f[2][3]
+    double *gkgrad, *gkg;         // 3 x nn each
+    double *gm, *gv, *gf;         // 2 x nn, 6 x nn, 6 x nn (field-major: field, component)
+    uint32_t *flags;              // CF_* per node
+    unsigned long long *near_d, *sg_d;
+    uint32_t *near_i, *sg_i;
+    uint8_t *near_surf;
+    unsigned long long *cmarks;   // platen tags of this step
+    unsigned long long *bounds;   // kCB positive doubles as bits
+    double *kp, *kg;              // per point (current order, platens after deformable): k, grad k (3 x n_all)
+    const Material *mats;
+    const StepRec *steps;
+    int32_t *gflags;
+    double *cpart;                // per node block: mass, reaction, active nodes
+    double *ppart;                // per point block: sum m|v|^2, m v.xyz, nonfinite
+    int32_t H;                    // headroom bits of the fixed-point sums
+};
+
+// ---- point access ---------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ void c_pos(const CArgs<real> &a, int j, double x[3])
+{
+    if (j >= a.n_def) {
+        const int d = j - a.n_def;
+        x[0] = a.dc_x[3 * d];
+        x[1] = a.dc_x[3 * d + 1];
+        x[2] = a.dc_x[3 * d + 2];
+        return;
+    }
+    if constexpr (sizeof(real) == 8) {
+        x[0] = a.p.x[0][j];
+        x[1] = a.p.x[1][j];
+        x[2] = a.p.x[2][j];
+    } else {
+        const uint32_t b = a.p.pbrick[j];
+        const int bc[3] = {(int)(b % a.g.nb[0]), (int)((b / a.g.nb[0]) % a.g.nb[1]), (int)(b / (a.g.nb[0] * a.g.nb[1]))};
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            x[q] = ((double)(bc[q] * a.g.S[q]) + (double)a.p.xq[q][j] * (1.0 / kQOne)) * a.g.h[q];
+    }
+}
+template <typename real>
+__device__ __forceinline__ uint32_t c_id(const CArgs<real> &a, int j)
+{
+    return j >= a.n_def ? a.dc_id[j - a.n_def] : a.p.id[j];
+}
+template <typename real>
+__device__ __forceinline__ uint8_t c_surf(const CArgs<real> &a, int j)
+{
+    return j >= a.n_def ? a.dc_surf[j - a.n_def] : a.surf[j];
+}
+
+__device__ __forceinline__ double c_len3(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+__device__ __forceinline__ double c_dot3(const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
+__device__ __forceinline__ double c_lenm3(double m) { return sqrt(m * m + m * m + m * m); }
+
+// GridPoint_Factory.h:40-42 node position (i*L)/n
+__device__ __forceinline__ void c_node_pos(const GridParams &g, int ix, int iy, int iz, double xi[3])
+{
+    xi[0] = (double)ix * g.L[0] / (double)g.cells[0];
+    xi[1] = (double)iy * g.L[1] / (double)g.cells[1];
+    xi[2] = (double)iz * g.L[2] / (double)g.cells[2];
+}
+
+// linear-hat N and grad N of point x at node xi (GridPoint_Mediator.h:64-88)
+__device__ __forceinline__ void c_bases(const GridParams &g, const double x[3], const double xi[3], double &N, double G[3])
+{
+    double s[3], sg[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double d = x[q] - xi[q];
+        double v = 1.0 - fabs(d) / g.h[q];
+        if (v < 0.0 || v > 1.0) v = 0.0;
+        s[q] = v;
+        sg[q] = sgn(d);
+    }
+    N = s[0] * s[1] * s[2];
+    G[0] = -s[1] * s[2] * sg[0] / g.h[0];
+    G[1] = -s[0] * s[2] * sg[1] / g.h[1];
+    G[2] = -s[0] * s[1] * sg[2] / g.h[2];
+}
+
+// the 8 adjacent nodes of x in reference order (x-outer, z-inner); false if out of grid
+__device__ __forceinline__ bool c_adjacent(const GridParams &g, const double x[3], long long idx[8], int ijk[8][3])
+{
+    int i0[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (!(x[q] >= 0.0)) return false;
+        i0[q] = (int)(x[q] / g.h[q]);
+        if (!(i0[q] < g.cells[q])) return false;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        ijk[k][0] = i0[0] + ((k >> 2) & 1);
+        ijk[k][1] = i0[1] + ((k >> 1) & 1);
+        ijk[k][2] = i0[2] + (k & 1);
+        idx[k] = ijk[k][0] + (long long)g.nodes[0] * (ijk[k][1] + (long long)g.nodes[1] * ijk[k][2]);
+    }
+    return true;
+}
+
+// fixed-point scale of bound slot q: contributions |c| <= bound < 2^e are
+// stored as c * 2^(FRAC - e) with FRAC = 62 - H, H headroom bits
+template <typename real>
+__device__ __forceinline__ double c_scale(const CArgs<real> &a, int q)
+{
+    const double b = __longlong_as_double((long long)a.bounds[q]);
+    if (!(b > 0.0)) return 0.0;
+    int e;
+    frexp(b * 1.0001, &e);   // b*1.0001 < 2^e
+    return ldexp(1.0, 62 - a.H - e);
+}
+template <typename real>
+__device__ __forceinline__ double c_unscale(const CArgs<real> &a, int q)
+{
+    const double b = __longlong_as_double((long long)a.bounds[q]);
+    if (!(b > 0.0)) return 0.0;
+    int e;
+    frexp(b * 1.0001, &e);
+    return ldexp(1.0, e - 62 + a.H);
+}
+__device__ __forceinline__ void c_add(long long *p, double v, double sc)
+{
+    atomicAdd((unsigned long long *)p, (unsigned long long)__double2ll_rn(v * sc));
+}
+__device__ __forceinline__ void c_umax(unsigned long long *p, double v)
+{
+    if (v > 0.0) atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// accumulator offsets inside CArgs::acc (in units of nn)
+__device__ __forceinline__ long long off_k() { return 0; }
+__device__ __forceinline__ long long off_m(int f) { return 1 + f; }
+__device__ __forceinline__ long long off_mg(int f, int c) { return 3 + 3 * f + c; }
+__device__ __forceinline__ long long off_mv(int f, int c) { return 9 + 3 * f + c; }
+__device__ __forceinline__ long long off_f(int f, int c) { return 15 + 3 * f + c; }
+
+// ---- C1: reset ---------------------------------------------------------------
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_clear(CArgs<real> a)
+{
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I < a.nn) {
+        for (int q = 0; q < 21; ++q) a.acc[q * a.nn + I] = 0;
+        for (int q = 0; q < 3; ++q) a.gkgrad[q * a.nn + I] = a.gkg[q * a.nn + I] = 0.0;
+        a.flags[I] = 0u;
+        a.near_d[I] = 0x7FF0000000000000ull;   // +inf
+        a.near_i[I] = 0xFFFFFFFFu;
+        a.sg_d[I] = 0ull;
+        a.sg_i[I] = 0xFFFFFFFFu;
+        a.near_surf[I] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kCB) a.bounds[threadIdx.x] = 0ull;
+}
+
+// ---- bounds of the contributions of this step ----------------------------------
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_bounds(CArgs<real> a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_all) return;
+    const GridParams &g = a.g;
+    const double ihm = fmax(1.0 / g.h[0], fmax(1.0 / g.h[1], 1.0 / g.h[2]));
+    double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
+    if (j >= a.n_def) {
+        const int d = j - a.n_def;
+        m = a.dc_m[d];
+        vol = a.dc_vol[d];
+        for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
+        for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
+    } else {
+        m = (double)a.p.mass[j];
+        double Fd[9];
+        for (int q = 0; q < 9; ++q) Fd[q] = (double)a.p.F[q][j];
+        vol = det3(Fd) * (double)a.p.vol0[j];
+        for (int q = 0; q < 3; ++q) {
+            v[q] = (double)a.p.v[q][j];
+            if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
+        }
+        for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
+    }
+    double smax = 0.0;
+    for (int q = 0; q < 6; ++q) smax = fmax(smax, fabs(s[q]));
+    c_umax(&a.bounds[0], fabs(m));
+    c_umax(&a.bounds[1], fabs(m) * ihm);
+    c_umax(&a.bounds[2], fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2]))));
+    c_umax(&a.bounds[3], fabs(vol) * 3.0 * smax * ihm + fmax(fabs(fe[0]), fmax(fabs(fe[1]), fabs(fe[2]))));
+    if (j == 0) a.bounds[4] = (unsigned long long)__double_as_longlong(1.0);
+}
+
+// ---- C2 + C3: active nodes, kernel on the grid (…_MP_Contact.cpp:94-153) ----
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_all) return;
+    const GridParams &g = a.g;
+    double x[3];
+    c_pos(a, j, x);
+    long long idx[8];
+    int ijk[8][3];
+    if (!c_adjacent(g, x, idx, ijk)) { atomicOr(a.gflags, FLAG_OUT_OF_GRID); return; }
+    for (int k = 0; k < 8; ++k) atomicOr(&a.flags[idx[k]], CF_ACTIVE);
+    const double R = fabs(1.0 * g.h[0]);
+    const double sck = c_scale(a, 4);
+    int i0[3];
+    for (int q = 0; q < 3; ++q) i0[q] = (int)(x[q] / g.h[q]);
+    for (int ix = -1; ix < 3; ++ix)
+        for (int iy = -1; iy < 3; ++iy)
+            for (int iz = -1; iz < 3; ++iz) {
+                const int n0 = i0[0] + ix, n1 = i0[1] + iy, n2 = i0[2] + iz;
+                if (n0 < 0 || n1 < 0 || n2 < 0 || n0 > g.nodes[0] - 1 || n1 > g.nodes[1] - 1 || n2 > g.nodes[2] - 1) continue;
+                double xi[3];
+                c_node_pos(g, n0, n1, n2, xi);
+                const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
+                const double dist = c_len3(d);
+                if (dist < R) {
+                    const long long I = n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2);
+                    atomicOr(&a.flags[I], CF_KACTIVE);
+                    const double q = dist / R;
+                    c_add(&a.acc[off_k() * a.nn + I], 1.0 - 3.0 * q * q + 2.0 * q * q * q, sck);
+                }
+            }
+}
+
+// ---- C4: kernel gradient by central differences (…_MP_Contact.cpp:159-191).
+// Neighbours are found by linear index, range-checked on the linear index only
+// (GridPoint_Factory.h:27-31): x-1 at ix = 0 wraps to the previous row.
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_kgrad(CArgs<real> a)
+{
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I >= a.nn) return;
+    const double usk = c_unscale(a, 4);
+    const double k = (double)a.acc[I] * usk;
+    if (k < 1.0 || !(a.flags[I] & CF_KACTIVE)) return;
+    const GridParams &g = a.g;
+    const long long sx = 1, sy = g.nodes[0], sz = (long long)g.nodes[0] * g.nodes[1];
+    const long long nb[6] = {I - sx, I + sx, I - sy, I + sy, I - sz, I + sz};
+    double kn[6];
+    for (int q = 0; q < 6; ++q) kn[q] = (nb[q] >= 0 && nb[q] < a.nn) ? (double)a.acc[nb[q]] * usk : 0.0;
+    double gr[3] = {0.0, 0.0, 0.0};
+    if (nb[0] >= 0 && nb[0] < a.nn) gr[0] += +0.5 / g.h[0] * (k - kn[0]);
+    if (nb[1] >= 0 && nb[1] < a.nn) gr[0] += -0.5 / g.h[0] * (k - kn[1]);
+    if (nb[2] >= 0 && nb[2] < a.nn) gr[1] += +0.5 / g.h[1] * (k - kn[2]);
+    if (nb[3] >= 0 && nb[3] < a.nn) gr[1] += -0.5 / g.h[1] * (k - kn[3]);
+    if (nb[4] >= 0 && nb[4] < a.nn) gr[2] += +0.5 / g.h[2] * (k - kn[4]);
+    if (nb[5] >= 0 && nb[5] < a.nn) gr[2] += -0.5 / g.h[2] * (k - kn[5]);
+    if (c_len3(gr) < 1.0 / g.h[0]) gr[0] = gr[1] = gr[2] = 0.0;
+    for (int q = 0; q < 3; ++q) a.gkgrad[q * a.nn + I] = gr[q];
+}
+
+// ---- C5: point kernel and its gradient; nearest point and surface gradient,
+// pass 1 (minimum distance, maximum |grad k|) (…_MP_Contact.cpp:199-252)
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_all) return;
+    const GridParams &g = a.g;
+    double x[3];
+    c_pos(a, j, x);
+    long long idx[8];
+    int ijk[8][3];
+    if (!c_adjacent(g, x, idx, ijk)) return;
+    const double usk = c_unscale(a, 4);
+    double kp = 0.0, kg[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 8; ++k) {
+        double xi[3], N, G[3];
+        c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
+        c_bases(g, x, xi, N, G);
+        kp += N * ((double)a.acc[idx[k]] * usk);
+        for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + idx[k]];
+        const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
+        atomicMin(&a.near_d[idx[k]], (unsigned long long)__double_as_longlong(c_len3(d)));
+    }
+    a.kp[j] = kp;
+    for (int q = 0; q < 3; ++q) a.kg[q * a.n_all + j] = kg[q];
+    if (c_surf(a, j)) {
+        const double lk = c_len3(kg);
+        for (int k = 0; k < 8; ++k) c_umax(&a.sg_d[idx[k]], lk);
+    }
+}
+
+// pass 2: the smallest index among the points at the minimum
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_all) return;
+    const GridParams &g = a.g;
+    double x[3];
+    c_pos(a, j, x);
+    long long idx[8];
+    int ijk[8][3];
+    if (!c_adjacent(g, x, idx, ijk)) return;
+    const uint32_t id = c_id(a, j);
+    const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
+    const bool surf = c_surf(a, j) != 0;
+    const unsigned long long lk = (unsigned long long)__double_as_longlong(c_len3(kg));
+    for (int k = 0; k < 8; ++k) {
+        double xi[3];
+        c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
+        const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
+        if ((unsigned long long)__double_as_longlong(c_len3(d)) == a.near_d[idx[k]]) atomicMin(&a.near_i[idx[k]], id);
+        if (surf && lk != 0ull && lk == a.sg_d[idx[k]]) atomicMin(&a.sg_i[idx[k]], id);
+    }
+}
+
+// pass 3: the winners publish their surface flag / kernel gradient.  The
+// reference starts every node's nearest point at point 0, so point 0 wins
+// where it is at least as close as the nearest touching point.
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_point3(CArgs<real> a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_all) return;
+    const GridParams &g = a.g;
+    double x[3];
+    c_pos(a, j, x);
+    long long idx[8];
+    int ijk[8][3];
+    if (!c_adjacent(g, x, idx, ijk)) return;
+    const uint32_t id = c_id(a, j);
+    double x0[3];
+    c_pos(a, a.def_id0[0] ? a.n_def + (int)a.def_id0[1] : (int)a.def_id0[1], x0);
+    for (int k = 0; k < 8; ++k) {
+        const long long I = idx[k];
+        if (id == a.sg_i[I])
+            for (int q = 0; q < 3; ++q) a.gkg[q * a.nn + I] = a.kg[q * a.n_all + j];
+        if (id == a.near_i[I]) {
+            double xi[3];
+            c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
+            const double d0[3] = {x0[0] - xi[0], x0[1] - xi[1], x0[2] - xi[2]};
+            const bool zero_wins = (unsigned long long)__double_as_longlong(c_len3(d0)) <= a.near_d[I];
+            a.near_surf[I] = zero_wins ? a.surf0 : c_surf(a, j);
+        }
+    }
+}
+
+// ---- C6: contact detection (…_MP_Contact.cpp:258-282) ---------------------------
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_detect(CArgs<real> a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_all || !c_surf(a, j)) return;
+    double x[3];
+    c_pos(a, j, x);
+    long long idx[8];
+    int ijk[8][3];
+    if (!c_adjacent(a.g, x, idx, ijk)) return;
+    const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
+    for (int k = 0; k < 8; ++k) {
+        const long long I = idx[k];
+        if (!a.near_surf[I]) continue;
+        const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
+        atomicOr(&a.flags[I], c_dot3(gk, kg) >= 0.0 ? CF_CPOS : CF_CNEG);
+    }
+}
+
+// ---- C6: split P2G (…_MP_Contact.cpp:288-385); MASS=true: mass and grad m,
+// MASS=false: momentum (/ field mass, sqrt(3) gate) and force
+template <typename real, bool MASS>
+__global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_all) return;
+    const GridParams &g = a.g;
+    double x[3];
+    c_pos(a, j, x);
+    long long idx[8];
+    int ijk[8][3];
+    if (!c_adjacent(g, x, idx, ijk)) return;
+    const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
+    double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
+    if (j >= a.n_def) {
+        const int d = j - a.n_def;
+        m = a.dc_m[d];
+        vol = a.dc_vol[d];
+        for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
+        for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
+    } else {
+        m = (double)a.p.mass[j];
+        double Fd[9];
+        for (int q = 0; q < 9; ++q) Fd[q] = (double)a.p.F[q][j];
+        vol = (double)(real)(det3(Fd) * (double)a.p.vol0[j]);
+        for (int q = 0; q < 3; ++q) {
+            v[q] = (double)a.p.v[q][j];
+            if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
+        }
+        for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
+    }
+    const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
+    const double um = c_unscale(a, 0);
+    for (int k = 0; k < 8; ++k) {
+        const long long I = idx[k];
+        double xi[3], N, G[3];
+        c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
+        c_bases(g, x, xi, N, G);
+        const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
+        const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
+        if (MASS) {
+            c_add(&a.acc[off_m(f) * a.nn + I], N * m, sc0);
+            for (int q = 0; q < 3; ++q) c_add(&a.acc[off_mg(f, q) * a.nn + I], G[q] * m, sc1);
+        } else {
+            const double mI = (double)a.acc[off_m(f) * a.nn + I] * um;
+            if (c_lenm3(mI) > g.m_min)
+                for (int q = 0; q < 3; ++q) c_add(&a.acc[off_mv(f, q) * a.nn + I], N * (m * v[q]) / mI, sc0);
+            const double fr[3] = {-vol * (G[0] * s[0] + G[1] * s[3] + G[2] * s[5]) + N * fe[0],
+                                  -vol * (G[1] * s[1] + G[0] * s[3] + G[2] * s[4]) + N * fe[1],
+                                  -vol * (G[2] * s[2] + G[0] * s[5] + G[1] * s[4]) + N * fe[2]};
+            for (int q = 0; q < 3; ++q) c_add(&a.acc[off_f(f, q) * a.nn + I], fr[q], sc1);
+        }
+    }
+}
+
+// platen tags of this step: highest platen index per node (…_MP_Contact.cpp:494-518)
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_dcmarks(CArgs<real> a, int step)
+{
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= a.n_dc) return;
+    double x[3] = {a.dc_x[3 * d], a.dc_x[3 * d + 1], a.dc_x[3 * d + 2]};
+    long long idx[8];
+    int ijk[8][3];
+    if (!c_adjacent(a.g, x, idx, ijk)) return;
+    const unsigned long long tag = ((unsigned long long)(a.steps[step].epoch + 1u) << 32) | (unsigned)d;
+    for (int k = 0; k < 8; ++k) atomicMax(&a.cmarks[idx[k]], tag);
+}
+
+// ---- C7 + C8 (nodes): two-field update, contact correction, platen override
+// with reaction (…_MP_Contact.cpp:392-518)
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_grid(CArgs<real> a, int step)
+{
+    __shared__ double s_r[4][3];
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    double rm = 0.0, rr = 0.0, ra = 0.0;
+    if (I < a.nn && (a.flags[I] & CF_ACTIVE)) {
+        const GridParams &g = a.g;
+        const StepRec st = a.steps[step];
+        const double dt = st.dt;
+        const double um = c_unscale(a, 0), umg = c_unscale(a, 1), umv = c_unscale(a, 2), uf = c_unscale(a, 3);
+        double mm[2], v[2][3], f[2][3], mg[2][3];
+        for (int fl = 0; fl < 2; ++fl) {
+            mm[fl] = (double)a.acc[off_m(fl) * a.nn + I] * um;
+            for (int q = 0; q < 3; ++q) {
+                v[fl][q] = (double)a.acc[off_mv(fl, q) * a.nn + I] * umv;
+                f[fl][q] = (double)a.acc[off_f(fl, q) * a.nn + I] * uf;
+                mg[fl][q] = (double)a.acc[off_mg(fl, q) * a.nn + I] * umg;
+            }
+        }
+        const uint8_t fx = a.fixed[I];
+        for (int s = 0; s < 2; ++s) {
+            double *vv = v[s], *ff = f[s];
+            if (c_len3(vv) > (s == 0 ? 1.0e-9 : 1.0e-16)) {
+                const double lf = c_len3(ff), inv = 1.0 / sqrt(c_dot3(vv, vv)), sc = g.damping * lf;
+                for (int q = 0; q < 3; ++q) ff[q] -= sc * (vv[q] * inv);
+            }
+            if (c_lenm3(mm[s]) > g.m_min)
+                for (int q = 0; q < 3; ++q) vv[q] += ff[q] / mm[s] * dt;
+        }
+        a.gm[I] = mm[0];
+        a.gm[a.nn + I] = mm[1];
+        // boundary conditions on both fields (no reaction here: the contact
+        // step's reaction is the platen's, …_MP_Contact.cpp:510-511)
+        for (int s = 0; s < 2; ++s)
+            for (int q = 0; q < 3; ++q)
+                if (fx & (1u << q)) { v[s][q] = 0.0; f[s][q] = 0.0; }
+        if (a.flags[I] & CF_CNEG) {
+            const double m = mm[0], mn = mm[1];
+            double vcm[3], nrm[3], fc[3] = {0, 0, 0}, fcn[3] = {0, 0, 0};
+            for (int q = 0; q < 3; ++q) vcm[q] = (m * v[0][q] + mn * v[1][q]) / (m + mn);
+            for (int q = 0; q < 3; ++q) nrm[q] = -0.5 * (mg[0][q] - mg[1][q]);
+            const double inv = 1.0 / sqrt(c_dot3(nrm, nrm));
+            for (int q = 0; q < 3; ++q) nrm[q] = nrm[q] * inv;
+            const double nneg[3] = {-nrm[0], -nrm[1], -nrm[2]};
+            if (c_dot3(v[0], nrm) + c_dot3(v[1], nneg) < 0.0) {
+                double d1[3], d2[3];
+                for (int q = 0; q < 3; ++q) { d1[q] = vcm[q] - v[0][q]; d2[q] = vcm[q] - v[1][q]; }
+                const double s1 = m / dt * c_dot3(d1, nrm), s2 = mn / dt * c_dot3(d2, nneg);
+                for (int q = 0; q < 3; ++q) { fc[q] = s1 * nrm[q]; fcn[q] = s2 * nneg[q]; }
+            }
+            if (c_lenm3(m) > g.m_min)
+                for (int q = 0; q < 3; ++q) { f[0][q] += fc[q]; v[0][q] += fc[q] / m * dt; }
+            if (c_lenm3(mn) > g.m_min)
+                for (int q = 0; q < 3; ++q) { f[1][q] += fcn[q]; v[1][q] += fcn[q] / mn * dt; }
+        }
+        const unsigned long long mk = a.cmarks[I];
+        if ((uint32_t)(mk >> 32) == st.epoch + 1u) {
+            const int d = (int)(mk & 0xFFFFFFFFull);
+            rr += f[0][1] + f[1][1];
+            for (int q = 0; q < 3; ++q) {
+                v[0][q] = v[1][q] = a.dc_v[3 * d + q];
+                f[0][q] = f[1][q] = 0.0;
+            }
+        }
+        for (int s = 0; s < 2; ++s)
+            for (int q = 0; q < 3; ++q) {
+                a.gv[(3 * s + q) * a.nn + I] = v[s][q];
+                a.gf[(3 * s + q) * a.nn + I] = f[s][q];
+            }
+        rm += mm[0];
+        ra += 1.0;
+    }
+    rm = wave_sum(rm);
+    rr = wave_sum(rr);
+    ra = wave_sum(ra);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { s_r[wv][0] = rm; s_r[wv][1] = rr; s_r[wv][2] = ra; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double r = 0.0;
+        for (int w = 0; w < 4; ++w) r += s_r[w][threadIdx.x];
+        a.cpart[3 * (size_t)blockIdx.x + threadIdx.x] = r;
+    }
+}
+
+// ---- C8: split G2P, kinematics, constitutive update (…_MP_Contact.cpp:524-632)
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
+{
+    __shared__ double s_r[4][5];
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double red[5] = {0, 0, 0, 0, 0};
+    int flags = 0;
+    if (j < a.n_def) {
+        constexpr bool FP32 = sizeof(real) == 4;
+        const GridParams &g = a.g;
+        const PArrays<real> &P = a.p;
+        const double dt = a.steps[step].dt;
+        double x[3];
+        c_pos(a, j, x);
+        long long idx[8];
+        int ijk[8][3];
+        if (!c_adjacent(g, x, idx, ijk)) {
+            flags |= FLAG_OUT_OF_GRID;
+        } else {
+            const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
+            double v[3], xn[3] = {x[0], x[1], x[2]}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int q = 0; q < 3; ++q) v[q] = (double)P.v[q][j];
+            for (int k = 0; k < 8; ++k) {
+                const long long I = idx[k];
+                double xi[3], N, G[3];
+                c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
+                c_bases(g, x, xi, N, G);
+                const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
+                const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
+                const double m = a.gm[f * a.nn + I];
+                double vI[3], fI[3];
+                for (int q = 0; q < 3; ++q) {
+                    vI[q] = a.gv[(3 * f + q) * a.nn + I];
+                    fI[q] = a.gf[(3 * f + q) * a.nn + I];
+                }
+                if (c_lenm3(m) > g.m_min)
+                    for (int q = 0; q < 3; ++q) v[q] += N * (fI[q] / m) * dt;
+                for (int q = 0; q < 3; ++q) xn[q] += N * vI[q] * dt;
+                for (int c = 0; c < 3; ++c)
+                    for (int r = 0; r < 3; ++r) L[3 * c + r] += vI[r] * G[c];
+            }
+            // F += L F dt, V = det(F) V0 (…_MP.cpp:299-302), then the material update
+            real F[9], Lr[9], s[6];
+            for (int q = 0; q < 9; ++q) { F[q] = P.F[q][j]; Lr[q] = (real)L[q]; }
+            real T[9];
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r)
+                    T[3 * c + r] = Lr[r] * F[3 * c] + Lr[3 + r] * F[3 * c + 1] + Lr[6 + r] * F[3 * c + 2];
+            for (int q = 0; q < 9; ++q) F[q] += T[q] * (real)dt;
+            for (int q = 0; q < 9; ++q) P.F[q][j] = F[q];
+            for (int q = 0; q < 6; ++q) s[q] = P.s[q][j];
+            for (int q = 0; q < 3; ++q) P.v[q][j] = (real)v[q];
+            if constexpr (FP32) {
+                const uint32_t b = P.pbrick[j];
+                const int bc[3] = {(int)(b % g.nb[0]), (int)((b / g.nb[0]) % g.nb[1]), (int)(b / (g.nb[0] * g.nb[1]))};
+                for (int q = 0; q < 3; ++q) {
+                    const double u = xn[q] / g.h[q] - (double)(bc[q] * g.S[q]);
+                    P.xq[q][j] = (int32_t)rint(u * kQOne);
+                    if (u < -0.5 || u >= (double)g.S[q] + 0.5) flags |= FLAG_REBIN;
+                }
+            } else {
+                for (int q = 0; q < 3; ++q) P.x[q][j] = (real)xn[q];
+                const uint32_t b = P.pbrick[j];
+                const int bc[3] = {(int)(b % g.nb[0]), (int)((b / g.nb[0]) % g.nb[1]), (int)(b / (g.nb[0] * g.nb[1]))};
+                for (int q = 0; q < 3; ++q) {
+                    const double u = xn[q] / g.h[q] - (double)(bc[q] * g.S[q]);
+                    if (u < -0.5 || u >= (double)g.S[q] + 0.5) flags |= FLAG_REBIN;
+                }
+            }
+            material_update<real>(g, P, a.mats, (uint32_t)j, Lr, (real)dt, F, s);
+            // checks of the new state (PhysicsEngine.cpp:73-79)
+            const double md = (double)P.mass[j];
+            const double vd[3] = {(double)P.v[0][j], (double)P.v[1][j], (double)P.v[2][j]};
+            const double vv = vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2];
+            red[0] = md * vv;
+            for (int q = 0; q < 3; ++q) red[1 + q] = md * vd[q];
+            const real ssum = s[0] + s[1] + s[2] + s[3] + s[4] + s[5];
+            if (!(isfinite(vv) && isfinite((double)ssum) && isfinite(xn[0] + xn[1] + xn[2]))) {
+                red[4] = 1.0;
+                flags |= FLAG_NAN;
+            }
+        }
+    }
+    for (int k = 0; k < 5; ++k) {
+        const double w = wave_sum(red[k]);
+        if (lane == 0) s_r[wv][k] = w;
+    }
+    int fl = flags;
+    for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
+    if (lane == 0 && fl) atomicOr(a.gflags, fl);
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double r = 0.0;
+        for (int w = 0; w < 4; ++w) r += s_r[w][threadIdx.x];
+        a.ppart[5 * (size_t)blockIdx.x + threadIdx.x] = r;
+    }
+}
+
+// the platen moves with its stored velocity, no TimeLine (…_MP_Contact.cpp:638-644)
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_dcmove(CArgs<real> a, int step)
+{
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= a.n_dc) return;
+    const double dt = a.steps[step].dt;
+    for (int q = 0; q < 3; ++q) a.dc_x_out[3 * d + q] = a.dc_x[3 * d + q] + a.dc_v[3 * d + q] * dt;
+}
+
+// point kernel and its gradient by original point index (for readback)
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_kernel_out(CArgs<real> a, double *k_id, double *g_id, int n_total)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_all) return;
+    const uint32_t id = c_id(a, j);
+    k_id[id] = a.kp[j];
+    for (int q = 0; q < 3; ++q) g_id[3 * (size_t)id + q] = a.kg[q * a.n_all + j];
+    (void)n_total;
+}
+
+// per-step reaction of a batch of contact steps: block k sums its step's node-block partials
+__global__ void __launch_bounds__(256) k_c_step_sums(const double *cpart, int nblk, double *hist)
+{
+    __shared__ double s[256];
+    const int t = threadIdx.x;
+    const double *cp = cpart + (size_t)blockIdx.x * nblk * 3;
+    double r = 0.0;
+    for (int i = t; i < nblk; i += 256) r += cp[3 * i + 1];
+    s[t] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) hist[blockIdx.x] = s[0];
+}
+
+}  // namespace mpm

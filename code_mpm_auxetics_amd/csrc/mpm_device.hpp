@@ -78,6 +78,7 @@ struct PArrays {
     real *v[3], *F[9], *s[6], *ep[6], *eps[6];
     real *alpha, *mass, *vol0, *E, *fext[3];
     uint8_t *mat;
+    uint8_t *surf;     // b_Surface (contact path)
     uint32_t *id, *pbrick;
 };
 

@@ -23,6 +23,7 @@
 #include "../../include/mpm_auxetics.h"
 #include "mpm_bin_kernels.hpp"
 #include "mpm_step_kernels.hpp"
+#include "mpm_contact_kernels.hpp"
 
 using namespace mpm;
 
@@ -61,6 +62,13 @@ struct mpm_handle {
     double *xstage[3] = {nullptr, nullptr, nullptr};
     double *dc_x = nullptr, *dc_m = nullptr;
     double *dc_v[2] = {nullptr, nullptr};   // platen velocities, ping-pong by step parity
+    // platen fields only the contact step reads (P2G includes platens there)
+    double *dc_vol = nullptr, *dc_s = nullptr, *dc_fext = nullptr;
+    uint8_t *dc_surf = nullptr;
+    uint32_t *dc_id = nullptr;
+    uint8_t surf0 = 0;                      // surface flag of original point 0
+    bool id0_dc = false;                    // original point 0 is a platen point
+    int id0_dc_index = 0;
     std::vector<int32_t> dc_id_host, def_id_host;
     mpm_points dc_saved{};                  // host copy of static platen fields
     std::vector<double> dc_saved_store;
@@ -93,6 +101,22 @@ struct mpm_handle {
     double *hist_d = nullptr, *hist_h = nullptr;
     std::vector<double> hist;
     double *checks_d = nullptr;
+
+    // contact step (allocated at its first use)
+    struct Contact {
+        bool ready = false;
+        long long *acc = nullptr;
+        double *gkgrad = nullptr, *gkg = nullptr, *gm = nullptr, *gv = nullptr, *gf = nullptr;
+        uint32_t *flags = nullptr, *near_i = nullptr, *sg_i = nullptr, *id0 = nullptr;
+        unsigned long long *near_d = nullptr, *sg_d = nullptr, *cmarks = nullptr, *bounds = nullptr;
+        uint8_t *near_surf = nullptr;
+        double *kp = nullptr, *kg = nullptr, *cpart = nullptr, *ppart = nullptr;
+        double *kp_id = nullptr, *kg_id = nullptr;   // by original point index
+        int nblk_n = 0, nblk_p = 0;
+        bool kernel_valid = false;          // kp/kg hold the last contact step's point kernel
+    } ct;
+    int last_path = 0;                      // path of the last step (checks read its partials)
+    bool plain_stale = false;               // a contact step ran: plain-path arenas must be rebuilt
 
     // rebin scratch
     uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
@@ -244,6 +268,7 @@ static void field_list(mpm_handle *h, PArrays<real> &p, std::vector<std::pair<vo
     if (h->has_fext)
         for (int a = 0; a < 3; ++a) out.push_back({(void **)&p.fext[a], rs});
     out.push_back({(void **)&p.mat, 1});
+    out.push_back({(void **)&p.surf, 1});
     out.push_back({(void **)&p.id, 4});
 }
 
@@ -446,6 +471,137 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     return MPM_OK;
 }
 
+// ---------------------------------------------------------------- contact
+static int ensure_contact(mpm_handle *h)
+{
+    auto &c = h->ct;
+    if (c.ready) return MPM_OK;
+    const size_t nn = (size_t)h->nnodes;
+    const size_t n_all = (size_t)std::max(h->n_def + h->n_dc, 1);
+    c.nblk_n = (int)cdiv(nn, 256);
+    c.nblk_p = (int)cdiv((size_t)std::max(h->n_def, 1), 256);
+    RCHK(dalloc(h, &c.acc, 21 * nn));
+    RCHK(dalloc(h, &c.gkgrad, 3 * nn));
+    RCHK(dalloc(h, &c.gkg, 3 * nn));
+    RCHK(dalloc(h, &c.gm, 2 * nn));
+    RCHK(dalloc(h, &c.gv, 6 * nn));
+    RCHK(dalloc(h, &c.gf, 6 * nn));
+    RCHK(dalloc(h, &c.flags, nn));
+    RCHK(dalloc(h, &c.near_i, nn));
+    RCHK(dalloc(h, &c.sg_i, nn));
+    RCHK(dalloc(h, &c.id0, 2));
+    RCHK(dalloc(h, &c.near_d, nn));
+    RCHK(dalloc(h, &c.sg_d, nn));
+    RCHK(dalloc(h, &c.cmarks, nn));
+    RCHK(dalloc(h, &c.bounds, (size_t)kCB));
+    RCHK(dalloc(h, &c.near_surf, nn));
+    RCHK(dalloc(h, &c.kp, n_all));
+    RCHK(dalloc(h, &c.kg, 3 * n_all));
+    RCHK(dalloc(h, &c.cpart, (size_t)h->steps_cap * c.nblk_n * 3));
+    RCHK(dalloc(h, &c.ppart, (size_t)c.nblk_p * 5));
+    RCHK(dalloc(h, &c.kp_id, (size_t)std::max(h->n_total, 1)));
+    RCHK(dalloc(h, &c.kg_id, 3 * (size_t)std::max(h->n_total, 1)));
+    HIPCHK(h, hipMemsetAsync(c.cmarks, 0, nn * 8, h->stream));
+    HIPCHK(h, hipMemsetAsync(c.ppart, 0, (size_t)c.nblk_p * 5 * 8, h->stream));
+    c.ready = true;
+    return MPM_OK;
+}
+
+template <typename real>
+static CArgs<real> contact_args(mpm_handle *h)
+{
+    auto &c = h->ct;
+    CArgs<real> a{};
+    a.g = h->g;
+    a.p = parr<real>(h, h->cur);
+    a.n_def = h->n_def;
+    a.n_dc = h->n_dc;
+    a.n_all = h->n_def + h->n_dc;
+    a.surf = a.p.surf;
+    a.dc_x = h->dc_x;
+    a.dc_x_out = h->dc_x;
+    a.dc_v = h->dc_v[h->parity];
+    a.dc_m = h->dc_m;
+    a.dc_vol = h->dc_vol;
+    a.dc_s = h->dc_s;
+    a.dc_fext = h->dc_fext;
+    a.dc_surf = h->dc_surf;
+    a.dc_id = h->dc_id;
+    a.def_id0 = c.id0;
+    a.surf0 = h->surf0;
+    a.nn = h->nnodes;
+    a.fixed = h->fixed;
+    a.acc = c.acc;
+    a.gkgrad = c.gkgrad;
+    a.gkg = c.gkg;
+    a.gm = c.gm;
+    a.gv = c.gv;
+    a.gf = c.gf;
+    a.flags = c.flags;
+    a.near_d = c.near_d;
+    a.sg_d = c.sg_d;
+    a.near_i = c.near_i;
+    a.sg_i = c.sg_i;
+    a.near_surf = c.near_surf;
+    a.cmarks = c.cmarks;
+    a.bounds = c.bounds;
+    a.kp = c.kp;
+    a.kg = c.kg;
+    a.mats = h->mats;
+    a.steps = h->steps_d;
+    a.gflags = h->flags_d;
+    a.ppart = c.ppart;
+    int H = 2;
+    while ((1ll << (H - 2)) <= (long long)a.n_all) ++H;   // 2^(H-2) > points touching a node
+    a.H = H;
+    return a;
+}
+
+// one contact step (…_MP_Contact.cpp:30-646): a fixed sequence of launches
+template <typename real>
+static int launch_contact(mpm_handle *h, int k)
+{
+    auto &c = h->ct;
+    CArgs<real> a = contact_args<real>(h);
+    a.cpart = c.cpart + (size_t)k * c.nblk_n * 3;
+    const unsigned gp = cdiv((size_t)std::max(a.n_all, 1), 256), gn = (unsigned)c.nblk_n;
+    hipStream_t st = h->stream;
+    k_c_clear<real><<<gn, 256, 0, st>>>(a);
+    if (a.n_all > 0) {
+        k_c_bounds<real><<<gp, 256, 0, st>>>(a);
+        k_c_kernel<real><<<gp, 256, 0, st>>>(a);
+    }
+    k_c_kgrad<real><<<gn, 256, 0, st>>>(a);
+    if (a.n_all > 0) {
+        k_c_point1<real><<<gp, 256, 0, st>>>(a);
+        k_c_point2<real><<<gp, 256, 0, st>>>(a);
+        k_c_point3<real><<<gp, 256, 0, st>>>(a);
+        k_c_detect<real><<<gp, 256, 0, st>>>(a);
+        k_c_p2g<real, true><<<gp, 256, 0, st>>>(a);
+        k_c_p2g<real, false><<<gp, 256, 0, st>>>(a);
+    }
+    if (a.n_dc > 0) k_c_dcmarks<real><<<cdiv(a.n_dc, 256), 256, 0, st>>>(a, k);
+    k_c_grid<real><<<gn, 256, 0, st>>>(a, k);
+    if (a.n_def > 0) k_c_g2p<real><<<(unsigned)c.nblk_p, 256, 0, st>>>(a, k);
+    if (a.n_dc > 0) k_c_dcmove<real><<<cdiv(a.n_dc, 256), 256, 0, st>>>(a, k);
+    HIPCHK(h, hipGetLastError());
+    c.kernel_valid = true;
+    return MPM_OK;
+}
+
+// where original point 0 lives now (platen index, or its sorted position)
+template <typename real>
+static int locate_point0(mpm_handle *h)
+{
+    uint32_t v[2] = {h->id0_dc ? 1u : 0u, h->id0_dc ? (uint32_t)h->id0_dc_index : 0u};
+    HIPCHK(h, hipMemcpyAsync(h->ct.id0, v, sizeof(v), hipMemcpyHostToDevice, h->stream));
+    if (!h->id0_dc && h->n_def > 0)
+        k_find_id<<<cdiv(h->n_def, 256), 256, 0, h->stream>>>(parr<real>(h, h->cur).id, (uint32_t)h->n_def, 0u, h->ct.id0);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
 // TimeLine::getVelocity (TimeLine.h:33-46), evaluated on the host per step
 static void timeline_velocity(const mpm_handle *h, double t, double out[3])
 {
@@ -487,8 +643,22 @@ static void report_console(mpm_handle *h)
 }
 
 template <typename real>
-static int step_impl(mpm_handle *h, double dt_total, int32_t *finished)
+static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished)
 {
+    if (path == MPM_PATH_CONTACT) {
+        RCHK(ensure_contact(h));
+        RCHK(locate_point0<real>(h));
+    } else if (h->plain_stale) {
+        // contact steps moved the points: re-bin, rebuild the arenas and the
+        // platen tags of the current step
+        RCHK(rebin(h, false));
+        if (h->n_dc > 0) {
+            k_dc_marks<<<cdiv(h->n_dc, 256), 256, 0, h->stream>>>(h->g, h->dc_x, h->n_dc, (uint32_t)h->cycle,
+                                                                 h->marks[h->parity]);
+            HIPCHK(h, hipGetLastError());
+        }
+        h->plain_stale = false;
+    }
     h->hist.clear();
     for (double &v : h->kt) v = 0.0;
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
@@ -499,23 +669,38 @@ static int step_impl(mpm_handle *h, double dt_total, int32_t *finished)
         // launch the batch: steps records already in steps_h[0..in_batch)
         HIPCHK(h, hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
         std::vector<ProfMark> marks;
-        for (int k = 0; k < in_batch; ++k) {
-            const int pe = prof_begin(h, marks, 0);
-            RCHK(launch_particle<real>(h, true, k));
-            prof_end(h, pe);
-            h->parity ^= 1;
+        if (path == MPM_PATH_CONTACT) {
+            for (int k = 0; k < in_batch; ++k) RCHK(launch_contact<real>(h, k));
+            {
+                CArgs<real> ca = contact_args<real>(h);
+                if (ca.n_all > 0)
+                    k_c_kernel_out<real><<<cdiv(ca.n_all, 256), 256, 0, h->stream>>>(ca, h->ct.kp_id, h->ct.kg_id, h->n_total);
+            }
+            // per-step platen reaction of the batch
+            k_c_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->ct.cpart, h->ct.nblk_n, h->hist_d);
+            HIPCHK(h, hipGetLastError());
+            h->plain_stale = true;
+            h->last_path = 1;
+        } else {
+            for (int k = 0; k < in_batch; ++k) {
+                const int pe = prof_begin(h, marks, 0);
+                RCHK(launch_particle<real>(h, true, k));
+                prof_end(h, pe);
+                h->parity ^= 1;
+            }
+            h->last_path = 0;
+            if (h->n_occ > 0) {   // per-step y reaction of the batch
+                k_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, h->hist_d);
+                HIPCHK(h, hipGetLastError());
+            } else {
+                HIPCHK(h, hipMemsetAsync(h->hist_d, 0, sizeof(double) * in_batch, h->stream));
+            }
         }
         h->last_gk = in_batch - 1;
-        if (h->n_occ > 0) {
-            k_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, h->hist_d);
-            HIPCHK(h, hipGetLastError());
-        } else {
-            HIPCHK(h, hipMemsetAsync(h->hist_d, 0, sizeof(double) * in_batch, h->stream));
-        }
         HIPCHK(h, hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * in_batch, hipMemcpyDeviceToHost, h->stream));
         bool need = false;
         RCHK(check_flags(h, &need));   // synchronises
-        for (int k = 0; k < in_batch; ++k) h->hist.push_back(h->n_def ? h->hist_h[k] : 0.0);
+        for (int k = 0; k < in_batch; ++k) h->hist.push_back(h->hist_h[k]);
         if (h->cfg.profile) prof_collect(h, marks);
         in_batch = 0;
         if (need) {
@@ -524,6 +709,7 @@ static int step_impl(mpm_handle *h, double dt_total, int32_t *finished)
             prof_end(h, pe);
             HIPCHK(h, hipStreamSynchronize(h->stream));
             if (h->cfg.profile) prof_collect(h, marks);
+            if (path == MPM_PATH_CONTACT) RCHK(locate_point0<real>(h));
         }
         (void)force_check;
         return MPM_OK;
@@ -566,9 +752,15 @@ static int step_impl(mpm_handle *h, double dt_total, int32_t *finished)
 
 static int checks_impl(mpm_handle *h, double out[12])
 {
-    const double *gp = (h->last_gk >= 0) ? h->gpart + (size_t)h->last_gk * h->occ_cap * 3 : nullptr;
-    k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, gp, h->dc_v[h->parity], h->dc_m, h->n_dc,
-                                       h->checks_d);
+    if (h->last_path == 1) {
+        const double *gp = (h->last_gk >= 0) ? h->ct.cpart + (size_t)h->last_gk * h->ct.nblk_n * 3 : nullptr;
+        k_finalize<<<1, 256, 0, h->stream>>>(h->ct.ppart, nullptr, h->ct.nblk_p, gp, nullptr, h->ct.nblk_n,
+                                           h->dc_v[h->parity], h->dc_m, h->n_dc, h->checks_d);
+    } else {
+        const double *gp = (h->last_gk >= 0) ? h->gpart + (size_t)h->last_gk * h->occ_cap * 3 : nullptr;
+        k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, 0, gp, h->n_occ_d, 0, h->dc_v[h->parity],
+                                           h->dc_m, h->n_dc, h->checks_d);
+    }
     HIPCHK(h, hipGetLastError());
     double r[8] = {0};
     HIPCHK(h, hipMemcpyAsync(r, h->checks_d, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -604,6 +796,17 @@ static void destroy_all(mpm_handle *h)
         dev_free((void *&)h->vals[b]);
     }
     for (int a = 0; a < 3; ++a) dev_free((void *&)h->xstage[a]);
+    {
+        auto &c = h->ct;
+        void **cp[] = {(void **)&c.acc, (void **)&c.gkgrad, (void **)&c.gkg, (void **)&c.gm, (void **)&c.gv,
+                       (void **)&c.gf, (void **)&c.flags, (void **)&c.near_i, (void **)&c.sg_i, (void **)&c.id0,
+                       (void **)&c.near_d, (void **)&c.sg_d, (void **)&c.cmarks, (void **)&c.bounds,
+                       (void **)&c.near_surf, (void **)&c.kp, (void **)&c.kg, (void **)&c.cpart, (void **)&c.ppart,
+                       (void **)&c.kp_id, (void **)&c.kg_id,
+                       (void **)&h->dc_vol, (void **)&h->dc_s, (void **)&h->dc_fext, (void **)&h->dc_surf,
+                       (void **)&h->dc_id};
+        for (void **q : cp) dev_free(*q);
+    }
     void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v[0], (void **)&h->dc_v[1], (void **)&h->dc_m, &h->zero_rec,
                      (void **)&h->fixed, (void **)&h->marks[0], (void **)&h->marks[1], (void **)&h->occ_list,
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d, (void **)&h->brick_slot,
@@ -852,12 +1055,41 @@ static int upload_t(mpm_handle *h, const mpm_points *pts)
             id[k] = (uint32_t)def[k];
         }
         HIPCHK(h, hipMemcpy(P.mat, mt.data(), nd, hipMemcpyHostToDevice));
+        for (int k = 0; k < nd; ++k) mt[k] = (pts->surface && pts->surface[def[k]]) ? 1 : 0;
+        HIPCHK(h, hipMemcpy(P.surf, mt.data(), nd, hipMemcpyHostToDevice));
         HIPCHK(h, hipMemcpy(P.id, id.data(), 4 * (size_t)nd, hipMemcpyHostToDevice));
     }
     // platens
     RCHK(dalloc(h, &h->dc_x, 3 * (size_t)std::max(h->n_dc, 1)));
     for (int b = 0; b < 2; ++b) RCHK(dalloc(h, &h->dc_v[b], 3 * (size_t)std::max(h->n_dc, 1)));
     RCHK(dalloc(h, &h->dc_m, (size_t)std::max(h->n_dc, 1)));
+    RCHK(dalloc(h, &h->dc_vol, (size_t)std::max(h->n_dc, 1)));
+    RCHK(dalloc(h, &h->dc_s, 6 * (size_t)std::max(h->n_dc, 1)));
+    RCHK(dalloc(h, &h->dc_fext, 3 * (size_t)std::max(h->n_dc, 1)));
+    RCHK(dalloc(h, &h->dc_surf, (size_t)std::max(h->n_dc, 1)));
+    RCHK(dalloc(h, &h->dc_id, (size_t)std::max(h->n_dc, 1)));
+    h->surf0 = (n > 0 && pts->surface && pts->surface[0]) ? 1 : 0;
+    h->id0_dc = false;
+    for (int j = 0; j < h->n_dc; ++j)
+        if (dc[j] == 0) { h->id0_dc = true; h->id0_dc_index = j; }
+    if (h->n_dc > 0) {
+        std::vector<double> vol(h->n_dc), st(6 * (size_t)h->n_dc, 0.0), fe(3 * (size_t)h->n_dc, 0.0);
+        std::vector<uint8_t> sf(h->n_dc);
+        std::vector<uint32_t> id(h->n_dc);
+        for (int j = 0; j < h->n_dc; ++j) {
+            const int i = dc[j];
+            vol[j] = pts->vol ? pts->vol[i] : pts->vol0[i];
+            for (int a = 0; a < 6; ++a) st[6 * j + a] = pts->stress ? pts->stress[6 * i + a] : 0.0;
+            for (int a = 0; a < 3; ++a) fe[3 * j + a] = pts->fext ? pts->fext[3 * i + a] : 0.0;
+            sf[j] = (pts->surface && pts->surface[i]) ? 1 : 0;
+            id[j] = (uint32_t)i;
+        }
+        HIPCHK(h, hipMemcpy(h->dc_vol, vol.data(), sizeof(double) * h->n_dc, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dc_s, st.data(), sizeof(double) * 6 * h->n_dc, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dc_fext, fe.data(), sizeof(double) * 3 * h->n_dc, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dc_surf, sf.data(), h->n_dc, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dc_id, id.data(), sizeof(uint32_t) * h->n_dc, hipMemcpyHostToDevice));
+    }
     {
         std::vector<double> x(3 * h->n_dc + 1), v(3 * h->n_dc + 1), m(h->n_dc + 1);
         for (int j = 0; j < h->n_dc; ++j) {
@@ -919,10 +1151,10 @@ int mpm_step(mpm_handle *h, double dt_total, int32_t path, int32_t *finished)
 {
     if (!h) return MPM_E_ARG;
     if (!h->uploaded) { h->err = "mpm_step before mpm_upload_points"; return MPM_E_STATE; }
-    if (path != MPM_PATH_PLAIN) { h->err = "contact path not available in this build"; return MPM_E_ARG; }
+    if (path != MPM_PATH_PLAIN && path != MPM_PATH_CONTACT) { h->err = "unknown step path"; return MPM_E_ARG; }
     if (!(dt_total >= 0.0)) { h->err = "dt_total must be >= 0"; return MPM_E_ARG; }
     HIPCHK(h, hipSetDevice(h->device));
-    return h->fp32 ? step_impl<float>(h, dt_total, finished) : step_impl<double>(h, dt_total, finished);
+    return h->fp32 ? step_impl<float>(h, dt_total, path, finished) : step_impl<double>(h, dt_total, path, finished);
 }
 
 }  // extern "C"
@@ -973,6 +1205,17 @@ static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
         if (st[7]) out->vol[i] = sv[28];
     }
     for (int k = 0; k < 8; ++k) dev_free((void *&)st[k]);
+    // d_Kernel / d3_Kernel_Gradient of the last contact step (zero before one)
+    if (mask & MPM_F_KERNEL) {
+        if (out->kernel) {
+            if (h->ct.kernel_valid) HIPCHK(h, hipMemcpy(out->kernel, h->ct.kp_id, sizeof(double) * n, hipMemcpyDeviceToHost));
+            else std::fill(out->kernel, out->kernel + n, 0.0);
+        }
+        if (out->kernel_grad) {
+            if (h->ct.kernel_valid) HIPCHK(h, hipMemcpy(out->kernel_grad, h->ct.kg_id, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+            else std::fill(out->kernel_grad, out->kernel_grad + 3 * (size_t)n, 0.0);
+        }
+    }
     return MPM_OK;
 }
 

@@ -377,6 +377,109 @@ struct KpArgs {
     double *dc_v_out;                   // ... after this step
 };
 
+// Strain increment, total strain, constitutive update and stress write of
+// point i (…_MP.cpp:304-353), shared by the plain and the contact step.
+template <typename real>
+__device__ __forceinline__ void material_update(const GridParams &g, const PArrays<real> &P, const Material *mats,
+                                                uint32_t i, const real L[9], real dt, const real F[9], real s[6])
+{
+    constexpr bool FP32 = sizeof(real) == 4;
+    // strain increment (…_MP.cpp:304-312); FP32 state uses L dt
+    real de[6];
+    if (!FP32 && g.strain_form == 0) {
+        de[0] = (real(1) + L[0] * dt) - real(1);
+        de[1] = (real(1) + L[4] * dt) - real(1);
+        de[2] = (real(1) + L[8] * dt) - real(1);
+    } else {
+        de[0] = L[0] * dt;
+        de[1] = L[4] * dt;
+        de[2] = L[8] * dt;
+    }
+    de[3] = L[1] * dt + L[3] * dt;
+    de[4] = L[5] * dt + L[7] * dt;
+    de[5] = L[6] * dt + L[2] * dt;
+    if (g.track_strain)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) P.eps[q][i] += de[q];
+    // constitutive dispatch (…_MP.cpp:322-353)
+    const Material &mt = mats[P.mat[i]];
+    int type = mt.type;
+    if (type == 1 && !g.neohookean) type = 3;
+    if (type == 2 || type == 5) type = 3;
+    const double E = g.has_pointE ? (double)P.E[i] : mt.E;
+    // FP32 state: elastic constants of this point without divisions
+    // on the common path (per-point E only when materials overflow
+    // the table)
+    double C = mt.C, ie = mt.invE, ig = mt.invG;
+    if (FP32 && g.has_pointE) {
+        C = E * mt.Cn;
+        ie = 1.0 / E;
+        ig = (2.0 * (1.0 + mt.nu)) * ie;
+    }
+    if (type == 0) {
+        if constexpr (FP32) {
+            const float Cf = (float)C, nu = (float)mt.nu, a0 = 1.0f - nu, a1 = 0.5f - nu;
+            s[0] += Cf * (a0 * de[0] + nu * de[1] + nu * de[2]);
+            s[1] += Cf * (a0 * de[1] + nu * de[0] + nu * de[2]);
+            s[2] += Cf * (a0 * de[2] + nu * de[0] + nu * de[1]);
+            s[3] += Cf * (a1 * de[3]);
+            s[4] += Cf * (a1 * de[4]);
+            s[5] += Cf * (a1 * de[5]);
+        } else {
+            real ds[6];
+            elastic_inc<real>((real)E, (real)mt.nu, de, ds);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s[q] += ds[q];
+        }
+    } else if (type == 1) {
+        double Fd[9], sd[6], ds[6];
+        for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
+        for (int q = 0; q < 6; ++q) sd[q] = (double)s[q];
+        neohookean_inc(E, mt.nu, sd, Fd, ds);
+        for (int q = 0; q < 6; ++q) s[q] = (real)(sd[q] + ds[q]);
+    } else if constexpr (FP32) {
+        // FP64 island for FP32 state
+        double dep[6];
+        const double al = (type == 4) ? (double)P.alpha[i] : 0.0;
+        const double radius = mt.k0 + al * 0.70710678118654752440;   // al / sqrt(2)
+        if (radial_return_fp32(mt.nu, C, ie, ig, radius, s, de, dep)) {
+            if (type == 4) {
+                double eff = 0.0;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
+                eff = sqrt(eff * (2.0 / 3.0));
+                const double da = ((mt.hs - al) > 0.0) ? mt.C0 * (mt.hs - al) * eff : 0.0;
+                P.alpha[i] = (real)(al + da);
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
+        }
+    } else {
+        // FP64 state: reference evaluation order
+        double sd[6], ded[6], ds[6], dep[6], da = 0.0;
+        for (int q = 0; q < 6; ++q) { sd[q] = (double)s[q]; ded[q] = (double)de[q]; }
+        const double k0 = mt.sy / sqrt(3.0);
+        if (type == 4) {
+            const double al = (double)P.alpha[i];
+            if (radial_return(E, mt.nu, k0 + al / sqrt(2.0), sd, ded, ds, dep)) {
+                double eff = 0.0;
+                for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
+                eff *= 2.0 / 3.0;
+                eff = sqrt(eff);
+                if ((sqrt(2.0 / 3.0) * mt.C1 - al) > 0.0)
+                    da = mt.C0 * (sqrt(2.0 / 3.0) * mt.C1 - al) * eff;
+                P.alpha[i] = (real)(al + da);
+                for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
+            }
+        } else if (radial_return(E, mt.nu, k0, sd, ded, ds, dep)) {
+            for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
+        }
+        for (int q = 0; q < 6; ++q) s[q] = (real)(sd[q] + ds[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) P.s[q][i] = s[q];
+}
+
 // Fixed-point fraction bits.  FP32 state: each contribution is converted to a
 // 32-bit integer scaled so that the brick's largest contribution bound so far
 // is < 2^26, then added to the 64-bit arena.  FP64 state: 64-bit conversion
@@ -693,100 +796,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                     }
 #pragma unroll
                     for (int q = 0; q < 9; ++q) P.F[q][i] = F[q];
-                    // strain increment (…_MP.cpp:304-312); FP32 state uses L dt
-                    real de[6];
-                    if (!FP32 && g.strain_form == 0) {
-                        de[0] = (real(1) + L[0] * dt) - real(1);
-                        de[1] = (real(1) + L[4] * dt) - real(1);
-                        de[2] = (real(1) + L[8] * dt) - real(1);
-                    } else {
-                        de[0] = L[0] * dt;
-                        de[1] = L[4] * dt;
-                        de[2] = L[8] * dt;
-                    }
-                    de[3] = L[1] * dt + L[3] * dt;
-                    de[4] = L[5] * dt + L[7] * dt;
-                    de[5] = L[6] * dt + L[2] * dt;
-                    if (g.track_strain)
-#pragma unroll
-                        for (int q = 0; q < 6; ++q) P.eps[q][i] += de[q];
-                    // constitutive dispatch (…_MP.cpp:322-353)
-                    const Material &mt = a.mats[P.mat[i]];
-                    int type = mt.type;
-                    if (type == 1 && !g.neohookean) type = 3;
-                    if (type == 2 || type == 5) type = 3;
-                    const double E = g.has_pointE ? (double)P.E[i] : mt.E;
-                    // FP32 state: elastic constants of this point without divisions
-                    // on the common path (per-point E only when materials overflow
-                    // the table)
-                    double C = mt.C, ie = mt.invE, ig = mt.invG;
-                    if (FP32 && g.has_pointE) {
-                        C = E * mt.Cn;
-                        ie = 1.0 / E;
-                        ig = (2.0 * (1.0 + mt.nu)) * ie;
-                    }
-                    if (type == 0) {
-                        if constexpr (FP32) {
-                            const float Cf = (float)C, nu = (float)mt.nu, a0 = 1.0f - nu, a1 = 0.5f - nu;
-                            s[0] += Cf * (a0 * de[0] + nu * de[1] + nu * de[2]);
-                            s[1] += Cf * (a0 * de[1] + nu * de[0] + nu * de[2]);
-                            s[2] += Cf * (a0 * de[2] + nu * de[0] + nu * de[1]);
-                            s[3] += Cf * (a1 * de[3]);
-                            s[4] += Cf * (a1 * de[4]);
-                            s[5] += Cf * (a1 * de[5]);
-                        } else {
-                            real ds[6];
-                            elastic_inc<real>((real)E, (real)mt.nu, de, ds);
-#pragma unroll
-                            for (int q = 0; q < 6; ++q) s[q] += ds[q];
-                        }
-                    } else if (type == 1) {
-                        double Fd[9], sd[6], ds[6];
-                        for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
-                        for (int q = 0; q < 6; ++q) sd[q] = (double)s[q];
-                        neohookean_inc(E, mt.nu, sd, Fd, ds);
-                        for (int q = 0; q < 6; ++q) s[q] = (real)(sd[q] + ds[q]);
-                    } else if constexpr (FP32) {
-                        // FP64 island for FP32 state
-                        double dep[6];
-                        const double al = (type == 4) ? (double)P.alpha[i] : 0.0;
-                        const double radius = mt.k0 + al * 0.70710678118654752440;   // al / sqrt(2)
-                        if (radial_return_fp32(mt.nu, C, ie, ig, radius, s, de, dep)) {
-                            if (type == 4) {
-                                double eff = 0.0;
-#pragma unroll
-                                for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
-                                eff = sqrt(eff * (2.0 / 3.0));
-                                const double da = ((mt.hs - al) > 0.0) ? mt.C0 * (mt.hs - al) * eff : 0.0;
-                                P.alpha[i] = (real)(al + da);
-                            }
-#pragma unroll
-                            for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
-                        }
-                    } else {
-                        // FP64 state: reference evaluation order
-                        double sd[6], ded[6], ds[6], dep[6], da = 0.0;
-                        for (int q = 0; q < 6; ++q) { sd[q] = (double)s[q]; ded[q] = (double)de[q]; }
-                        const double k0 = mt.sy / sqrt(3.0);
-                        if (type == 4) {
-                            const double al = (double)P.alpha[i];
-                            if (radial_return(E, mt.nu, k0 + al / sqrt(2.0), sd, ded, ds, dep)) {
-                                double eff = 0.0;
-                                for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
-                                eff *= 2.0 / 3.0;
-                                eff = sqrt(eff);
-                                if ((sqrt(2.0 / 3.0) * mt.C1 - al) > 0.0)
-                                    da = mt.C0 * (sqrt(2.0 / 3.0) * mt.C1 - al) * eff;
-                                P.alpha[i] = (real)(al + da);
-                                for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
-                            }
-                        } else if (radial_return(E, mt.nu, k0, sd, ded, ds, dep)) {
-                            for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
-                        }
-                        for (int q = 0; q < 6; ++q) s[q] = (real)(sd[q] + ds[q]);
-                    }
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) P.s[q][i] = s[q];
+                    material_update<real>(g, P, a.mats, i, L, dt, F, s);
                 }
             } else {
                 double Fd[9];

@@ -139,11 +139,14 @@ class PhysicsEngine:
             mask &= ~_abi.MPM_F_STRAIN
         out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), F=np.zeros((n, 9)), stress=np.zeros((n, 6)),
                    strain=np.zeros((n, 6)), eps_p=np.zeros((n, 6)), alpha=np.zeros(n), vol=np.zeros(n))
+        if mask & _abi.MPM_F_KERNEL:
+            out["kernel"] = np.zeros(n)
+            out["kernel_grad"] = np.zeros((n, 3))
         pts = MpmPoints()
         pts.n = n
         for k, a in out.items():
             setattr(pts, k, _ptr(a))
-        check(lib().mpm_readback(self._h, mask & 0xFF, C.byref(pts)), self._h)
+        check(lib().mpm_readback(self._h, mask & _abi.MPM_F_ALL, C.byref(pts)), self._h)
         return out
 
     def checks(self):

@@ -1,0 +1,152 @@
+"""GPU contact path (runSimulation_Classic_SinglePass_MP_Contact) against the
+FP64 oracle's two-field step: FP64 to the FP64 parity bound, FP32 to the
+FP32 bound, plus determinism, the point kernel readback and switching paths
+within one run."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_golden  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def mpm():
+    import code_mpm_auxetics_amd as m
+    m.lib()
+    return m
+
+
+def _cmp(g, o, x0, tol, sel=slice(None)):
+    ex = rel_l2((g["x"] - x0)[sel], (o["x"] - x0)[sel])
+    ev = rel_l2(g["v"][sel], o["v"][sel])
+    es = rel_l2(g["stress"][sel], o["stress"][sel])
+    print(f"rel L2 disp {ex:.3e} vel {ev:.3e} stress {es:.3e}", flush=True)
+    assert ex <= tol and ev <= tol and es <= tol, (ex, ev, es)
+
+
+def test_contact_fixture_fp64(mpm):
+    sc, path, steps, sf, out = make_golden.load("contact_blocks")
+    x0 = sc.x.copy()
+    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64)
+    eng.run_steps(steps, path=mpm.PATH_CONTACT)
+    _cmp(eng.getMaterialPoints(), out, x0, 1e-9)
+    eng.close()
+
+
+def test_contact_blocks_first_steps_fp64(mpm, oracle_mod):
+    """The steps in which contact engages match the oracle to the FP64 bound,
+    including the point kernel (d_Kernel, d3_Kernel_Gradient).  Once a node's
+    two fields have been corrected, their normal velocities are equal and the
+    reference's approach test (…_MP_Contact.cpp:455) compares round-off, so
+    later steps are checked by invariants (next test)."""
+    sc = make_golden.contact_blocks()
+    x0 = sc.x.copy()
+    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64)
+    orc = oracle_mod.Oracle(sc)
+    eng.run_steps(2, path=mpm.PATH_CONTACT)
+    orc.run_steps(2, 1)
+    g = eng.getMaterialPoints(mpm.F_ALL)
+    o = orc.points()
+    _cmp(g, o, x0, 1e-9)
+    assert rel_l2(g["kernel"], orc.a["kernel"]) <= 1e-12
+    assert rel_l2(g["kernel_grad"], orc.a["kernel_grad"]) <= 1e-9
+    eng.close()
+
+
+def test_contact_blocks_invariants(mpm, oracle_mod):
+    """40 contact steps of the closing blocks: total momentum is conserved
+    (equal and opposite contact forces, no boundary conditions), the state
+    stays finite, and the aggregate motion of the facing layers (their
+    closing speed) follows the oracle's to 1e-5 although individual contact
+    decisions may follow round-off differently."""
+    sc = make_golden.contact_blocks()
+    p0 = (sc.mass[:, None] * sc.v).sum(axis=0)
+    scale = np.abs(sc.mass[:, None] * sc.v).sum()
+    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64)
+    eng.run_steps(40, path=mpm.PATH_CONTACT)
+    vc = eng.getMaterialPoints()["v"]
+    eng.close()
+    orc = oracle_mod.Oracle(sc)
+    orc.run_steps(40, 1)
+    vo = orc.points()["v"]
+    assert np.isfinite(vc).all()
+    p1 = (sc.mass[:, None] * vc).sum(axis=0)
+    assert np.allclose(p1, p0, rtol=0, atol=1e-10 * scale)
+    nA = len(sc.x) // 2
+    fa = (sc.surface == 1) & (np.arange(len(sc.x)) < nA)
+    fb = (sc.surface == 1) & (np.arange(len(sc.x)) >= nA)
+    closing_g = vc[fa, 0].mean() - vc[fb, 0].mean()
+    closing_o = vo[fa, 0].mean() - vo[fb, 0].mean()
+    print(f"closing speed gpu {closing_g:.9f} oracle {closing_o:.9f}", flush=True)
+    assert abs(closing_g - closing_o) <= 1e-5 * abs(closing_o)
+    assert rel_l2(vc, vo) <= 1e-3
+
+
+@pytest.mark.parametrize("material", ["elastic", "j2"])
+def test_contact_world_with_platens_fp64(mpm, oracle_mod, material):
+    """Platen points take part in the contact P2G and override both fields;
+    the reaction history is the platen's (…_MP_Contact.cpp:494-518)."""
+    sc = mpm.scene.small_world(material=material, speed=(5.0 if material == "elastic" else 2.0))
+    # the contact step moves platens with their stored velocity (no TimeLine,
+    # …_MP_Contact.cpp:638-644): the upper platen closes at 2 m/s
+    top = (sc.dc == 1) & (sc.x[:, 1] > sc.x[:, 1].mean())
+    sc.v[top] = (0.0, -2.0, 0.0)
+    x0 = sc.x.copy()
+    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64)
+    orc = oracle_mod.Oracle(sc)
+    eng.run_steps(60, path=mpm.PATH_CONTACT)
+    orc.run_steps(60, 1)
+    _cmp(eng.getMaterialPoints(), orc.points(), x0, 1e-9)
+    hg, ho = eng.reaction_history(), orc.reaction
+    assert len(hg) == len(ho)
+    assert np.abs(hg - ho).max() <= 1e-8 * np.abs(ho).max() + 1e-20
+    assert np.abs(ho).max() > 0
+    eng.close()
+
+
+def test_contact_fp32_bound(mpm, oracle_mod):
+    """FP32 state on the contact path (platens in the P2G, no surface
+    points) against the FP64 oracle's contact step, L dt strain form."""
+    sc = mpm.scene.small_world(material="j2", speed=2.0)
+    top = (sc.dc == 1) & (sc.x[:, 1] > sc.x[:, 1].mean())
+    sc.v[top] = (0.0, -2.0, 0.0)
+    x0 = sc.x.copy()
+    eng = mpm.PhysicsEngine(sc, precision=mpm.FP32)
+    orc = oracle_mod.Oracle(sc, strain_form=1)
+    eng.run_steps(100, path=mpm.PATH_CONTACT)
+    orc.run_steps(100, 1)
+    _cmp(eng.getMaterialPoints(), orc.points(), x0, 1e-4, sel=sc.dc == 0)
+    eng.close()
+
+
+def test_contact_deterministic(mpm):
+    sc = make_golden.contact_blocks()
+    outs = []
+    for _ in range(2):
+        eng = mpm.PhysicsEngine(sc, precision=mpm.FP64)
+        eng.run_steps(20, path=mpm.PATH_CONTACT)
+        outs.append(eng.getMaterialPoints())
+        eng.close()
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_switching_paths(mpm, oracle_mod):
+    """plain, contact, plain within one run matches the oracle doing the same."""
+    sc = mpm.scene.small_world(material="j2", speed=2.0)
+    x0 = sc.x.copy()
+    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64)
+    orc = oracle_mod.Oracle(sc)
+    for path, n in ((0, 20), (1, 20), (0, 20)):
+        eng.run_steps(n, path=path)
+        orc.run_steps(n, path)
+    _cmp(eng.getMaterialPoints(), orc.points(), x0, 1e-9)
+    eng.close()
