@@ -133,12 +133,15 @@ int mpm_upload_points(mpm_handle *h, const mpm_points *pts);
  * …_MP_Contact(double) (PhysicsEngine.h:45-46,
  * PhysicsEngine_runSimulation_Classic_SinglePass_MP.cpp:4-394).
  * *finished receives the reference's return value: 1 if d_Time >= d_TimeEnd,
- * else 0 (…_MP.cpp:385-393).  Blocks until the steps completed. */
+ * else 0 (…_MP.cpp:385-393).  Blocks until the steps completed.  The path may
+ * change from call to call (…_MP_Contact.cpp:4-676 for MPM_PATH_CONTACT). */
 int mpm_step(mpm_handle *h, double dt_total, int32_t path, int32_t *finished);
 
 /* Copy point fields (mask of MPM_F_*) back to host arrays in the original
  * point order.  Replaces PhysicsEngine::getMaterialPoints (PhysicsEngine.h:64),
- * whose callers read the live MaterialPoint objects. */
+ * whose callers read the live MaterialPoint objects.  MPM_F_KERNEL fills
+ * kernel / kernel_grad with d_Kernel / d3_Kernel_Gradient of the last contact
+ * step (…_MP_Contact.cpp:199-219), zeros before one. */
 int mpm_readback(mpm_handle *h, uint32_t field_mask, mpm_points *out);
 
 /* FP64 global checks, computed on the device.  Replaces the sums of
