@@ -24,6 +24,8 @@ MPM_FP64 = 0
 MPM_FP32 = 1
 MPM_PATH_PLAIN = 0
 MPM_PATH_CONTACT = 1
+MPM_LAUNCH_GRAPH = 0
+MPM_LAUNCH_DIRECT = 1
 
 MPM_ELASTIC, MPM_NEOHOOKEAN, MPM_VISCOELASTIC, MPM_PLASTIC, MPM_VONMISESHARDENING, MPM_GASS = range(6)
 MPM_FIX_X, MPM_FIX_Y, MPM_FIX_Z = 1, 2, 4
@@ -69,6 +71,7 @@ class MpmConfig(C.Structure):
         ("device", C.c_int32),
         ("verbose", C.c_int32),
         ("profile", C.c_int32),
+        ("launch", C.c_int32),
     ]
 
 

@@ -329,41 +329,59 @@ __global__ void __launch_bounds__(256) k_step_sums(const double *gpart, int occ_
     if (t == 0) hist[blockIdx.x] = s[0];
 }
 
-// FP64 sums over the per-block partials (mpm_checks): one block of 256,
-// strided partial sums then a fixed tree -- deterministic.  gpart: the last
-// step's grid partials (mass, reaction, active nodes per slot) or null.
-// Partial counts: n_pp / n_gp, or *n_pp_d / *n_gp_d when those are given.
-__global__ void __launch_bounds__(256) k_finalize(const double *kp_part, const int32_t *n_pp_d, int n_pp,
-                                                 const double *gpart, const int32_t *n_gp_d, int n_gp,
-                                                 const double *dc_v, const double *dc_m, int n_dc, double *out)
+// Sum of 3-double partials (grid mass, reaction, active nodes of one step):
+// one block of 256, strided lane sums then a fixed tree.  Count: *n_d or n.
+__global__ void __launch_bounds__(256) k_sum3(const double *part, const int32_t *n_d, int n, double *out3)
 {
-    __shared__ double s[8][256];
-    const int t = threadIdx.x;
-    double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int npp = n_pp_d ? *n_pp_d : n_pp, ngp = gpart ? (n_gp_d ? *n_gp_d : n_gp) : 0;
-    for (int i = t; i < ngp; i += 256) {
-        r[0] += gpart[3 * i];
-        r[1] += gpart[3 * i + 1];
-        r[7] += gpart[3 * i + 2];
-    }
-    for (int i = t; i < npp; i += 256)
-        for (int k = 0; k < 5; ++k) r[2 + k] += kp_part[5 * i + k];
-    for (int j = t; j < n_dc; j += 256) {
-        const double *v = &dc_v[3 * j];
-        const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        r[2] += dc_m[j] * vv;
-        r[3] += dc_m[j] * v[0];
-        r[4] += dc_m[j] * v[1];
-        r[5] += dc_m[j] * v[2];
-    }
-    for (int k = 0; k < 8; ++k) s[k][t] = r[k];
+    __shared__ double s[3][256];
+    const int t = threadIdx.x, cnt = n_d ? *n_d : n;
+    double r[3] = {0, 0, 0};
+    for (int i = t; i < cnt; i += 256)
+        for (int k = 0; k < 3; ++k) r[k] += part[3 * i + k];
+    for (int k = 0; k < 3; ++k) s[k][t] = r[k];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (t < w)
-            for (int k = 0; k < 8; ++k) s[k][t] += s[k][t + w];
+            for (int k = 0; k < 3; ++k) s[k][t] += s[k][t + w];
         __syncthreads();
     }
-    if (t < 8) out[t] = s[t][0];
+    if (t < 3) out3[t] = s[t][0];
+}
+
+// FP64 check sums (mpm_checks): point partials (count *n_pp_d or n_pp) in a
+// fixed tree, the last grid update's totals (grid3: mass, reaction, active
+// nodes) and the platen points.
+__global__ void __launch_bounds__(256) k_finalize(const double *kp_part, const int32_t *n_pp_d, int n_pp,
+                                                 const double *grid3, const double *dc_v, const double *dc_m,
+                                                 int n_dc, double *out)
+{
+    __shared__ double s[5][256];
+    const int t = threadIdx.x;
+    double r[5] = {0, 0, 0, 0, 0};
+    const int npp = n_pp_d ? *n_pp_d : n_pp;
+    for (int i = t; i < npp; i += 256)
+        for (int k = 0; k < 5; ++k) r[k] += kp_part[5 * i + k];
+    for (int j = t; j < n_dc; j += 256) {
+        const double *v = &dc_v[3 * j];
+        const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        r[0] += dc_m[j] * vv;
+        r[1] += dc_m[j] * v[0];
+        r[2] += dc_m[j] * v[1];
+        r[3] += dc_m[j] * v[2];
+    }
+    for (int k = 0; k < 5; ++k) s[k][t] = r[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 5; ++k) s[k][t] += s[k][t + w];
+        __syncthreads();
+    }
+    if (t < 5) out[2 + t] = s[t][0];
+    if (t == 0) {
+        out[0] = grid3[0];
+        out[1] = grid3[1];
+        out[7] = grid3[2];
+    }
 }
 
 }  // namespace mpm

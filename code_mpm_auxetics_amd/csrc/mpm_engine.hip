@@ -96,11 +96,11 @@ struct mpm_handle {
     int32_t *flags_d = nullptr, *flags_h = nullptr;
     double *kp_part = nullptr;
     double *gpart = nullptr;                // [steps_cap][occ_cap][3] grid-update partials
-    int last_gk = -1;                       // batch index of the last grid update in gpart
     size_t kp_cap = 0;
     double *hist_d = nullptr, *hist_h = nullptr;
     std::vector<double> hist;
     double *checks_d = nullptr;
+    double *grid3_d = nullptr;              // last grid update: mass, reaction, active nodes
 
     // contact step (allocated at its first use)
     struct Contact {
@@ -116,6 +116,15 @@ struct mpm_handle {
         bool kernel_valid = false;          // kp/kg hold the last contact step's point kernel
     } ct;
     int last_path = 0;                      // path of the last step (checks read its partials)
+    // hipGraphs of batches of plain steps, keyed by (steps, starting parity)
+    // and valid for one brick layout (bumped at every rebin / reallocation)
+    struct GraphEntry {
+        int steps, parity;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    long long graph_layout = -1, layout = 0;
     bool plain_stale = false;               // a contact step ran: plain-path arenas must be rebuilt
 
     // rebin scratch
@@ -307,7 +316,7 @@ static int ensure_slots(mpm_handle *h, int n_occ)
         HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * 8, h->stream));
         RCHK(dalloc(h, &h->gpart, (size_t)h->steps_cap * h->arena_cap_slots * 3));
         h->occ_cap = want_occ;
-        h->last_gk = -1;
+        h->layout++;
     }
     return MPM_OK;
 }
@@ -420,6 +429,7 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     // rebuild the arena the next grid update reads
     RCHK(launch_particle<real>(h, false, 0));
     h->rebins++;
+    h->layout++;
     return MPM_OK;
 }
 
@@ -615,9 +625,9 @@ static void timeline_velocity(const mpm_handle *h, double t, double out[3])
     }
 }
 
-static int check_flags(mpm_handle *h, bool *need_rebin)
+static int check_flags(mpm_handle *h, bool *need_rebin, bool copied = false)
 {
-    HIPCHK(h, hipMemcpyAsync(h->flags_h, h->flags_d, 4, hipMemcpyDeviceToHost, h->stream));
+    if (!copied) HIPCHK(h, hipMemcpyAsync(h->flags_h, h->flags_d, 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     const int f = *h->flags_h;
     *need_rebin = (f & FLAG_REBIN) != 0;
@@ -640,6 +650,67 @@ static void report_console(mpm_handle *h)
     if (checks_impl(h, c) != MPM_OK) return;
     std::printf("\ttime: %+.6e\tRuntime: %+.3e\tForce_y: %+.6e\tKinetic Energy: %+.6e\n", h->time,
                 h->last_ms * 1e-3, c[1], c[2]);
+}
+
+static void drop_graphs(mpm_handle *h)
+{
+    for (auto &e : h->graphs) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipGraphDestroy(e.graph);
+    }
+    h->graphs.clear();
+}
+
+// A batch of plain steps as one hipGraph: step records in, the fused step
+// kernels, the per-step reaction sums, history and flags back.  Captured
+// for a (steps, starting parity, brick layout) and replayed while those hold;
+// the host fills steps_h (pinned) before every replay.  No memset nodes.
+template <typename real>
+static int graph_batch(mpm_handle *h, int nsteps)
+{
+    if (h->graph_layout != h->layout) {
+        drop_graphs(h);
+        h->graph_layout = h->layout;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (const auto &e : h->graphs)
+        if (e.steps == nsteps && e.parity == h->parity) exec = e.exec;
+    if (!exec) {
+        const int p0 = h->parity;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int rc = MPM_OK;
+        if (hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * nsteps, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+            rc = MPM_E_HIP;
+        for (int k = 0; k < nsteps && rc == MPM_OK; ++k) {
+            rc = launch_particle<real>(h, true, k);
+            h->parity ^= 1;
+        }
+        if (rc == MPM_OK) {
+            k_step_sums<<<(unsigned)nsteps, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, h->hist_d);
+            k_sum3<<<1, 256, 0, h->stream>>>(h->gpart + (size_t)(nsteps - 1) * h->occ_cap * 3, h->n_occ_d, 0, h->grid3_d);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * nsteps, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                hipMemcpyAsync(h->flags_h, h->flags_d, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+                rc = MPM_E_HIP;
+        }
+        h->parity = p0;
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+        if (rc != MPM_OK || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            h->err = "hipGraph capture failed";
+            return MPM_E_HIP;
+        }
+        if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            h->err = "hipGraphInstantiate failed";
+            return MPM_E_HIP;
+        }
+        if (h->graphs.size() >= 4) drop_graphs(h);
+        h->graphs.push_back({nsteps, p0, g, exec});
+    }
+    HIPCHK(h, hipGraphLaunch(exec, h->stream));
+    return MPM_OK;
 }
 
 template <typename real>
@@ -667,8 +738,10 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
     auto flush = [&](bool force_check) -> int {
         if (in_batch == 0) return MPM_OK;
         // launch the batch: steps records already in steps_h[0..in_batch)
-        HIPCHK(h, hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
         std::vector<ProfMark> marks;
+        bool graphed = false;   // the graph already copied history and flags back
+        if (path == MPM_PATH_CONTACT || !(h->cfg.launch == MPM_LAUNCH_GRAPH && !h->cfg.profile && h->n_occ > 0))
+            HIPCHK(h, hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
         if (path == MPM_PATH_CONTACT) {
             for (int k = 0; k < in_batch; ++k) RCHK(launch_contact<real>(h, k));
             {
@@ -678,9 +751,16 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
             }
             // per-step platen reaction of the batch
             k_c_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->ct.cpart, h->ct.nblk_n, h->hist_d);
+            k_sum3<<<1, 256, 0, h->stream>>>(h->ct.cpart + (size_t)(in_batch - 1) * h->ct.nblk_n * 3, nullptr,
+                                            h->ct.nblk_n, h->grid3_d);
             HIPCHK(h, hipGetLastError());
             h->plain_stale = true;
             h->last_path = 1;
+        } else if (h->cfg.launch == MPM_LAUNCH_GRAPH && !h->cfg.profile && h->n_occ > 0) {
+            RCHK(graph_batch<real>(h, in_batch));
+            h->parity ^= (in_batch & 1);
+            h->last_path = 0;
+            graphed = true;
         } else {
             for (int k = 0; k < in_batch; ++k) {
                 const int pe = prof_begin(h, marks, 0);
@@ -689,17 +769,20 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
                 h->parity ^= 1;
             }
             h->last_path = 0;
-            if (h->n_occ > 0) {   // per-step y reaction of the batch
+            if (h->n_occ > 0) {   // per-step y reaction of the batch, last step's grid totals
                 k_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, h->hist_d);
+                k_sum3<<<1, 256, 0, h->stream>>>(h->gpart + (size_t)(in_batch - 1) * h->occ_cap * 3, h->n_occ_d, 0,
+                                                h->grid3_d);
                 HIPCHK(h, hipGetLastError());
             } else {
                 HIPCHK(h, hipMemsetAsync(h->hist_d, 0, sizeof(double) * in_batch, h->stream));
+                HIPCHK(h, hipMemsetAsync(h->grid3_d, 0, 24, h->stream));
             }
         }
-        h->last_gk = in_batch - 1;
-        HIPCHK(h, hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * in_batch, hipMemcpyDeviceToHost, h->stream));
+        if (!graphed)
+            HIPCHK(h, hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * in_batch, hipMemcpyDeviceToHost, h->stream));
         bool need = false;
-        RCHK(check_flags(h, &need));   // synchronises
+        RCHK(check_flags(h, &need, graphed));   // synchronises
         for (int k = 0; k < in_batch; ++k) h->hist.push_back(h->hist_h[k]);
         if (h->cfg.profile) prof_collect(h, marks);
         in_batch = 0;
@@ -752,15 +835,12 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
 
 static int checks_impl(mpm_handle *h, double out[12])
 {
-    if (h->last_path == 1) {
-        const double *gp = (h->last_gk >= 0) ? h->ct.cpart + (size_t)h->last_gk * h->ct.nblk_n * 3 : nullptr;
-        k_finalize<<<1, 256, 0, h->stream>>>(h->ct.ppart, nullptr, h->ct.nblk_p, gp, nullptr, h->ct.nblk_n,
-                                           h->dc_v[h->parity], h->dc_m, h->n_dc, h->checks_d);
-    } else {
-        const double *gp = (h->last_gk >= 0) ? h->gpart + (size_t)h->last_gk * h->occ_cap * 3 : nullptr;
-        k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, 0, gp, h->n_occ_d, 0, h->dc_v[h->parity],
+    if (h->last_path == 1)
+        k_finalize<<<1, 256, 0, h->stream>>>(h->ct.ppart, nullptr, h->ct.nblk_p, h->grid3_d, h->dc_v[h->parity],
                                            h->dc_m, h->n_dc, h->checks_d);
-    }
+    else
+        k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, 0, h->grid3_d, h->dc_v[h->parity], h->dc_m,
+                                           h->n_dc, h->checks_d);
     HIPCHK(h, hipGetLastError());
     double r[8] = {0};
     HIPCHK(h, hipMemcpyAsync(r, h->checks_d, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -788,6 +868,7 @@ static void destroy_all(mpm_handle *h)
 {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graphs(h);
     for (int b = 0; b < 2; ++b) {
         for (auto &f : h->pfields[b]) dev_free(f.p);
         h->pfields[b].clear();
@@ -811,7 +892,7 @@ static void destroy_all(mpm_handle *h)
                      (void **)&h->fixed, (void **)&h->marks[0], (void **)&h->marks[1], (void **)&h->occ_list,
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d, (void **)&h->brick_slot,
                      (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats, (void **)&h->steps_d,
-                     (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->hist_d,
+                     (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->hist_d, (void **)&h->grid3_d,
                      (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
                      (void **)&h->headpos, (void **)&h->gdesc};
     for (void **p : ptrs) dev_free(*p);
@@ -941,10 +1022,11 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
         (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) ||
         (rc = dalloc(h, &h->binfo[0], nb)) || (rc = dalloc(h, &h->binfo[1], nb)) ||
-        (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)))
+        (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)) || (rc = dalloc(h, &h->grid3_d, 3)))
         return fail(rc);
     (void)hipMemsetAsync(h->n_occ_d, 0, 4, h->stream);
     (void)hipMemsetAsync(h->flags_d, 0, 4, h->stream);
+    (void)hipMemsetAsync(h->grid3_d, 0, 24, h->stream);
     h->steps_cap = 64;
     if ((rc = dalloc(h, &h->steps_d, h->steps_cap)) || (rc = dalloc(h, &h->hist_d, h->steps_cap))) return fail(rc);
     if (hipHostMalloc((void **)&h->steps_h, sizeof(StepRec) * h->steps_cap, hipHostMallocDefault) != hipSuccess ||

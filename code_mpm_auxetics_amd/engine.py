@@ -34,7 +34,8 @@ class PhysicsEngine:
     """One solver handle on one GPU (one HIP stream)."""
 
     def __init__(self, scene, precision=FP64, device=0, brick=(0, 0, 0), drift_check=8,
-                 track_strain=True, strain_form=0, verbose=False, friction=0.0, profile=False):
+                 track_strain=True, strain_form=0, verbose=False, friction=0.0, profile=False,
+                 graph=True):
         self.scene = scene
         L = lib()
         cfg = MpmConfig()
@@ -64,6 +65,7 @@ class PhysicsEngine:
         cfg.device = int(device)
         cfg.verbose = 1 if verbose else 0
         cfg.profile = 1 if profile else 0
+        cfg.launch = _abi.MPM_LAUNCH_GRAPH if graph else _abi.MPM_LAUNCH_DIRECT
         self.precision = precision
         self.track_strain = track_strain
         h = C.c_void_p()

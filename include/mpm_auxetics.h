@@ -17,7 +17,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 1
+#define MPM_ABI_VERSION 2
 
 /* ---- status codes (the reference has none; it exits via fatalError,
  *      Errors.cpp:5-12) ---------------------------------------------------- */
@@ -40,6 +40,12 @@ enum mpm_precision {
     MPM_FP32 = 1             /* FP32 state; FP64 det(F)/log J, J2 return map,
                                 global checks                                 */
 };
+
+/* how a batch of plain steps reaches the GPU: captured once into a hipGraph
+ * and replayed while the brick layout is unchanged (default), or launched
+ * kernel by kernel.  Results are identical.  Profiling (cfg.profile) and the
+ * contact path always launch directly. */
+enum mpm_launch { MPM_LAUNCH_GRAPH = 0, MPM_LAUNCH_DIRECT = 1 };
 
 /* step paths: runSimulation_Classic_SinglePass_MP / _MP_Contact
  * (PhysicsEngine.h:45-46) */
@@ -95,6 +101,7 @@ typedef struct mpm_config {
     int32_t verbose;           /* 1: print reportConsole lines to stdout        */
     int32_t profile;           /* 1: HIP events around every kernel launch, read
                                   with mpm_kernel_times                         */
+    int32_t launch;            /* enum mpm_launch                               */
 } mpm_config;
 
 /* Host point arrays, FP64, caller-allocated, original point order.  Mirrors
