@@ -42,9 +42,10 @@ def build_scene(scale, speed):
     return scene.config2_world(material="j2", scale=scale, steps=5000, platen="ramp", speed=speed)
 
 
-def run_gpu(sc, precision, steps, warmup, profile_steps):
+def run_gpu(sc, precision, steps, warmup, profile_steps, shape=None):
     import code_mpm_auxetics_amd as m
-    eng = m.PhysicsEngine(sc, precision=precision, track_strain=False)
+    kw = dict(brick=shape[0], drift_check=shape[1]) if shape else {}
+    eng = m.PhysicsEngine(sc, precision=precision, track_strain=False, **kw)
     if warmup:
         eng.run_steps(warmup)
     t0 = time.perf_counter()
@@ -55,7 +56,7 @@ def run_gpu(sc, precision, steps, warmup, profile_steps):
     eng.close()
     res = dict(wall_s=t1 - t0, device_ms=dev_ms, checks=c)
     if profile_steps:
-        eng = m.PhysicsEngine(sc, precision=precision, track_strain=False, profile=True)
+        eng = m.PhysicsEngine(sc, precision=precision, track_strain=False, profile=True, **kw)
         eng.run_steps(warmup + steps)      # same trajectory position as the timed window
         eng.run_steps(profile_steps)
         res["kt"] = eng.kernel_times()
@@ -121,6 +122,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="pick brick size and batch length with the autotuner before timing")
     ap.add_argument("--counters", action="store_true",
                     help="short FP32 run plus a 256 MiB device copy, for rocprofv3 --pmc passes")
     args = ap.parse_args()
@@ -135,8 +138,14 @@ def main():
     if args.counters:
         counters_run(m, sc, args.warmup, args.steps)
         return
-    r32 = run_gpu(sc, m.FP32, args.steps, args.warmup, args.profile_steps)
-    r64 = run_gpu(sc, m.FP64, args.fp64_steps, args.warmup, 0)
+    shape32 = shape64 = None
+    if args.tune:
+        from code_mpm_auxetics_amd import autotune
+        t32 = autotune.tune(sc, precision=m.FP32)
+        t64 = autotune.tune(sc, precision=m.FP64)
+        shape32, shape64 = (t32["brick"], t32["drift_check"]), (t64["brick"], t64["drift_check"])
+    r32 = run_gpu(sc, m.FP32, args.steps, args.warmup, args.profile_steps, shape32)
+    r64 = run_gpu(sc, m.FP64, args.fp64_steps, args.warmup, 0, shape64)
     v32 = n * args.steps / r32["wall_s"]
     v64 = n * args.fp64_steps / r64["wall_s"]
 
@@ -161,7 +170,10 @@ def main():
                    "grid_cells": list(sc.cells), "precision": "fp32 state, fp64 return map and det F",
                    "platen_speed_m_s": round(sc.info["platen_speed"], 3), "platen_ramp_steps": 100,
                    "dt_s": sc.dt_max,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "launch": {"fp32": list(shape32) if shape32 else "default (bricks 4x4x10, batch 8)",
+                              "fp64": list(shape64) if shape64 else "default (bricks 4x4x10, batch 8)",
+                              "tuned": bool(args.tune)}},
         "fp64_value": round(v64, 1), "fp64_ms_per_step": round(r64["wall_s"] * 1e3 / args.fp64_steps, 5),
         "fp32_over_fp64": round(v32 / v64, 3),
         "step_roofline_frac": round(step_bytes * args.steps / r32["wall_s"] / 1e9 / HBM_PEAK_GBS, 4),

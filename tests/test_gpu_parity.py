@@ -260,3 +260,24 @@ def test_graph_replay_matches_direct_launch(mpm, prec):
         print("rebins", eg.checks()["rebins"], flush=True)
         eg.close()
         ed.close()
+
+
+def test_autotune_picks_a_working_launch_shape(mpm, oracle_mod):
+    """The tuner times candidates on the scene, skips shapes the library
+    rejects, and its choice runs the step to the FP64 parity bound."""
+    from code_mpm_auxetics_amd import autotune
+    sc = mpm.scene.small_world(material="j2", speed=2.0, nx=2, ny=1)
+    res = autotune.tune(sc, precision=mpm.FP64, bricks=[(4, 4, 10), (3, 3, 2), (2, 2, 2), (40, 40, 40)],
+                        batches=[4, 16], steps=8, warmup=2, repeats=1)
+    errs = [e for e in res["table"] if "error" in e]
+    assert any(e["brick"] == [2, 2, 2] for e in errs)          # below the 3-cell minimum
+    assert any(e["brick"] == [40, 40, 40] for e in errs)       # arena does not fit the LDS
+    assert res["brick"] in ((4, 4, 10), (3, 3, 2)) and res["ms_per_step"] > 0
+    x0 = sc.x.copy()
+    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64, brick=res["brick"], drift_check=res["drift_check"])
+    orc = oracle_mod.Oracle(sc)
+    eng.run_steps(50)
+    orc.run_steps(50)
+    g, o = eng.getMaterialPoints(), orc.points()
+    assert rel_l2(g["x"] - x0, o["x"] - x0) <= 1e-9
+    eng.close()
