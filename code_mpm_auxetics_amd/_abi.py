@@ -38,7 +38,7 @@ MPM_F_ALL = 0x1FF
 EXPORTED = [
     "mpm_abi_version", "mpm_create", "mpm_upload_points", "mpm_step", "mpm_readback",
     "mpm_checks", "mpm_reaction_history", "mpm_get_time", "mpm_last_step_ms",
-    "mpm_last_error", "mpm_destroy", "mpm_sort_keys", "mpm_kernel_times",
+    "mpm_last_error", "mpm_destroy", "mpm_sort_keys", "mpm_kernel_times", "mpm_group_reaction_history",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -72,6 +72,8 @@ class MpmConfig(C.Structure):
         ("verbose", C.c_int32),
         ("profile", C.c_int32),
         ("launch", C.c_int32),
+        ("node_group", C.POINTER(C.c_uint16)),
+        ("n_groups", C.c_int32),
     ]
 
 
@@ -106,6 +108,7 @@ def lib():
     L.mpm_readback.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(MpmPoints)]
     L.mpm_checks.argtypes = [C.c_void_p, _dp]
     L.mpm_reaction_history.argtypes = [C.c_void_p, _dp, C.c_int32, _i32p]
+    L.mpm_group_reaction_history.argtypes = [C.c_void_p, _dp, C.c_int32, _i32p]
     L.mpm_get_time.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.mpm_last_step_ms.argtypes = [C.c_void_p, _dp]
     L.mpm_last_error.argtypes = [C.c_void_p]

@@ -329,6 +329,32 @@ __global__ void __launch_bounds__(256) k_step_sums(const double *gpart, int occ_
     if (t == 0) hist[blockIdx.x] = s[0];
 }
 
+// Per-group reaction of a batch: block (step k, group g) sums the group's
+// y-fixed slots [off[g], off[g+1]) of step k in a fixed tree.
+__global__ void __launch_bounds__(256) k_group_sums(const double *fy_val, int n_fy, const int32_t *off, int n_groups,
+                                                   double *ghist)
+{
+    __shared__ double s[256];
+    const int k = blockIdx.x / n_groups, gr = blockIdx.x % n_groups, t = threadIdx.x;
+    const double *v = fy_val + (size_t)k * n_fy;
+    double r = 0.0;
+    for (int i = off[gr] + t; i < off[gr + 1]; i += 256) r += v[i];
+    s[t] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) ghist[(size_t)k * n_groups + gr] = s[0];
+}
+
+// fill a double array (graph-safe replacement of a memset)
+__global__ void k_fill_f64(double *p, size_t n, double v)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // Sum of 3-double partials (grid mass, reaction, active nodes of one step):
 // one block of 256, strided lane sums then a fixed tree.  Count: *n_d or n.
 __global__ void __launch_bounds__(256) k_sum3(const double *part, const int32_t *n_d, int n, double *out3)

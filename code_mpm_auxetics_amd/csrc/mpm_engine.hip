@@ -101,6 +101,11 @@ struct mpm_handle {
     std::vector<double> hist;
     double *checks_d = nullptr;
     double *grid3_d = nullptr;              // last grid update: mass, reaction, active nodes
+    // batched worlds: reaction per group (world)
+    int n_groups = 0, n_fy = 0;
+    int32_t *fy_slot = nullptr, *fy_off = nullptr;   // node -> y-fixed slot; CSR offsets per group
+    double *fy_val = nullptr, *ghist_d = nullptr, *ghist_h = nullptr;
+    std::vector<double> ghist;              // [step][group] of the last mpm_step call
 
     // contact step (allocated at its first use)
     struct Contact {
@@ -469,6 +474,8 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.dc_x = h->dc_x;
     a.dc_v_in = h->dc_v[p];
     a.dc_v_out = h->dc_v[p ^ 1];
+    a.fy_slot = (g2p && h->n_fy > 0) ? h->fy_slot : nullptr;
+    a.fy_val = h->fy_val ? h->fy_val + (size_t)step * h->n_fy : nullptr;
     const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long) + (g2p ? (size_t)6 * h->g.arenaN * h->rsize : 0);
     unsigned grid = (unsigned)h->n_occ;
     if (g2p) grid += cdiv(h->n_dc, kBlock);
@@ -652,6 +659,32 @@ static void report_console(mpm_handle *h)
                 h->last_ms * 1e-3, c[1], c[2]);
 }
 
+// batched worlds: clear this batch's y-fixed slots (uncovered nodes stay 0)
+static int group_pre(mpm_handle *h, int nsteps)
+{
+    if (h->n_fy == 0) return MPM_OK;
+    const size_t n = (size_t)nsteps * h->n_fy;
+    k_fill_f64<<<cdiv(n, 256), 256, 0, h->stream>>>(h->fy_val, n, 0.0);
+    HIPCHK(h, hipGetLastError());
+    return MPM_OK;
+}
+// ... and reduce them per (step, group) in a fixed order, then copy back
+static int group_post(mpm_handle *h, int nsteps)
+{
+    if (h->n_groups == 0) return MPM_OK;
+    if (h->n_fy > 0) {
+        k_group_sums<<<(unsigned)(nsteps * h->n_groups), 256, 0, h->stream>>>(h->fy_val, h->n_fy, h->fy_off, h->n_groups,
+                                                                            h->ghist_d);
+        HIPCHK(h, hipGetLastError());
+    } else {
+        const size_t n = (size_t)nsteps * h->n_groups;
+        k_fill_f64<<<cdiv(n, 256), 256, 0, h->stream>>>(h->ghist_d, n, 0.0);
+    }
+    HIPCHK(h, hipMemcpyAsync(h->ghist_h, h->ghist_d, sizeof(double) * nsteps * h->n_groups, hipMemcpyDeviceToHost,
+                             h->stream));
+    return MPM_OK;
+}
+
 static void drop_graphs(mpm_handle *h)
 {
     for (auto &e : h->graphs) {
@@ -681,6 +714,7 @@ static int graph_batch(mpm_handle *h, int nsteps)
         int rc = MPM_OK;
         if (hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * nsteps, hipMemcpyHostToDevice, h->stream) != hipSuccess)
             rc = MPM_E_HIP;
+        if (rc == MPM_OK && group_pre(h, nsteps) != MPM_OK) rc = MPM_E_HIP;
         for (int k = 0; k < nsteps && rc == MPM_OK; ++k) {
             rc = launch_particle<real>(h, true, k);
             h->parity ^= 1;
@@ -688,6 +722,7 @@ static int graph_batch(mpm_handle *h, int nsteps)
         if (rc == MPM_OK) {
             k_step_sums<<<(unsigned)nsteps, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, h->hist_d);
             k_sum3<<<1, 256, 0, h->stream>>>(h->gpart + (size_t)(nsteps - 1) * h->occ_cap * 3, h->n_occ_d, 0, h->grid3_d);
+            if (group_post(h, nsteps) != MPM_OK) rc = MPM_E_HIP;
             if (hipGetLastError() != hipSuccess ||
                 hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * nsteps, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
                 hipMemcpyAsync(h->flags_h, h->flags_d, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
@@ -731,6 +766,7 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
         h->plain_stale = false;
     }
     h->hist.clear();
+    h->ghist.clear();
     for (double &v : h->kt) v = 0.0;
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     double acc = 0.0;
@@ -762,6 +798,7 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
             h->last_path = 0;
             graphed = true;
         } else {
+            RCHK(group_pre(h, in_batch));
             for (int k = 0; k < in_batch; ++k) {
                 const int pe = prof_begin(h, marks, 0);
                 RCHK(launch_particle<real>(h, true, k));
@@ -774,6 +811,7 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
                 k_sum3<<<1, 256, 0, h->stream>>>(h->gpart + (size_t)(in_batch - 1) * h->occ_cap * 3, h->n_occ_d, 0,
                                                 h->grid3_d);
                 HIPCHK(h, hipGetLastError());
+                RCHK(group_post(h, in_batch));
             } else {
                 HIPCHK(h, hipMemsetAsync(h->hist_d, 0, sizeof(double) * in_batch, h->stream));
                 HIPCHK(h, hipMemsetAsync(h->grid3_d, 0, 24, h->stream));
@@ -784,6 +822,12 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
         bool need = false;
         RCHK(check_flags(h, &need, graphed));   // synchronises
         for (int k = 0; k < in_batch; ++k) h->hist.push_back(h->hist_h[k]);
+        if (h->n_groups > 0) {
+            const bool have = path != MPM_PATH_CONTACT && h->n_occ > 0;
+            for (int k = 0; k < in_batch; ++k)
+                for (int gr = 0; gr < h->n_groups; ++gr)
+                    h->ghist.push_back(have ? h->ghist_h[(size_t)k * h->n_groups + gr] : 0.0);
+        }
         if (h->cfg.profile) prof_collect(h, marks);
         in_batch = 0;
         if (need) {
@@ -893,6 +937,7 @@ static void destroy_all(mpm_handle *h)
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d, (void **)&h->brick_slot,
                      (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats, (void **)&h->steps_d,
                      (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->hist_d, (void **)&h->grid3_d,
+                     (void **)&h->fy_slot, (void **)&h->fy_off, (void **)&h->fy_val, (void **)&h->ghist_d,
                      (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
                      (void **)&h->headpos, (void **)&h->gdesc};
     for (void **p : ptrs) dev_free(*p);
@@ -901,6 +946,8 @@ static void destroy_all(mpm_handle *h)
     if (h->steps_h) (void)hipHostFree(h->steps_h);
     if (h->flags_h) (void)hipHostFree(h->flags_h);
     if (h->hist_h) (void)hipHostFree(h->hist_h);
+    if (h->ghist_h) (void)hipHostFree(h->ghist_h);
+    h->ghist_h = nullptr;
     h->steps_h = nullptr;
     h->flags_h = nullptr;
     h->hist_h = nullptr;
@@ -1034,6 +1081,31 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         hipHostMalloc((void **)&h->hist_h, sizeof(double) * h->steps_cap, hipHostMallocDefault) != hipSuccess) {
         h->err = "hipHostMalloc";
         return fail(MPM_E_NOMEM);
+    }
+    // batched worlds: y-fixed nodes of every group, in group order
+    if (cfg->node_group && cfg->n_groups > 0) {
+        const int ng = cfg->n_groups;
+        std::vector<int32_t> cnt(ng + 1, 0), slot((size_t)h->nnodes, -1);
+        for (long long I = 0; I < h->nnodes; ++I) {
+            if (cfg->node_group[I] >= ng) { h->err = "node_group id >= n_groups"; return fail(MPM_E_ARG); }
+            if (cfg->node_fixed && (cfg->node_fixed[I] & MPM_FIX_Y)) cnt[cfg->node_group[I] + 1]++;
+        }
+        for (int gr = 0; gr < ng; ++gr) cnt[gr + 1] += cnt[gr];
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (long long I = 0; I < h->nnodes; ++I)
+            if (cfg->node_fixed && (cfg->node_fixed[I] & MPM_FIX_Y)) slot[I] = fill[cfg->node_group[I]]++;
+        h->n_groups = ng;
+        h->n_fy = cnt[ng];
+        if ((rc = dalloc(h, &h->fy_slot, (size_t)h->nnodes)) || (rc = dalloc(h, &h->fy_off, (size_t)ng + 1)) ||
+            (rc = dalloc(h, &h->fy_val, (size_t)h->steps_cap * std::max(h->n_fy, 1))) ||
+            (rc = dalloc(h, &h->ghist_d, (size_t)h->steps_cap * ng)))
+            return fail(rc);
+        if (hipMemcpy(h->fy_slot, slot.data(), sizeof(int32_t) * h->nnodes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->fy_off, cnt.data(), sizeof(int32_t) * (ng + 1), hipMemcpyHostToDevice) != hipSuccess ||
+            hipHostMalloc((void **)&h->ghist_h, sizeof(double) * h->steps_cap * ng, hipHostMallocDefault) != hipSuccess) {
+            h->err = "group tables";
+            return fail(MPM_E_HIP);
+        }
     }
     if (hipStreamSynchronize(h->stream) != hipSuccess) { h->err = "init sync"; return fail(MPM_E_HIP); }
     *out = h;
@@ -1334,6 +1406,17 @@ int mpm_kernel_times(mpm_handle *h, double out[6])
 {
     if (!h || !out) return MPM_E_ARG;
     for (int k = 0; k < 6; ++k) out[k] = h->kt[k];
+    return MPM_OK;
+}
+
+int mpm_group_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t *n)
+{
+    if (!h || (!out && cap > 0) || !n) return MPM_E_ARG;
+    if (h->n_groups == 0) { *n = 0; return MPM_OK; }
+    const int steps = (int)(h->ghist.size() / h->n_groups);
+    const int m = std::min<int>(steps, cap);
+    for (size_t k = 0; k < (size_t)m * h->n_groups; ++k) out[k] = h->ghist[k];
+    *n = m;
     return MPM_OK;
 }
 

@@ -372,6 +372,8 @@ struct KpArgs {
     double *kp_part;      // per slot: 5 doubles (sum m|v|^2, m v.xyz, nonfinite)
     double *gpart;        // per slot: 3 doubles of this step's grid update (mass, reaction, active nodes)
     int32_t n_dc;
+    const int32_t *fy_slot;             // batched worlds: node -> y-fixed slot or -1 (null: off)
+    double *fy_val;                     // this step's reaction per y-fixed slot
     double *dc_x;                       // 3 per platen point, FP64
     const double *dc_v_in;              // platen velocities of step n
     double *dc_v_out;                   // ... after this step
@@ -651,6 +653,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                 gs[0] += (double)mn;
                 gs[1] += react;
                 gs[2] += (mn > real(0)) ? 1.0 : 0.0;
+                if (a.fy_slot && (fx & 2)) a.fy_val[a.fy_slot[nid]] = react;
             }
         }
 #pragma unroll

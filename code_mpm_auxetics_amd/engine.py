@@ -66,6 +66,10 @@ class PhysicsEngine:
         cfg.verbose = 1 if verbose else 0
         cfg.profile = 1 if profile else 0
         cfg.launch = _abi.MPM_LAUNCH_GRAPH if graph else _abi.MPM_LAUNCH_DIRECT
+        if scene.node_group is not None and scene.n_groups > 0:
+            self._group = np.ascontiguousarray(scene.node_group, np.uint16)
+            cfg.node_group = self._group.ctypes.data_as(C.POINTER(C.c_uint16))
+            cfg.n_groups = int(scene.n_groups)
         self.precision = precision
         self.track_strain = track_strain
         h = C.c_void_p()
@@ -163,6 +167,16 @@ class PhysicsEngine:
         n = C.c_int32(0)
         check(lib().mpm_reaction_history(self._h, _ptr(buf), cap, C.byref(n)), self._h)
         return buf[: n.value].copy()
+
+    def group_reaction_history(self, cap=1 << 20):
+        """Per-step reaction of every batched world, shape (steps, n_groups)."""
+        ng = int(self.scene.n_groups or 0)
+        if ng == 0:
+            return np.zeros((0, 0))
+        buf = np.zeros(cap * ng)
+        n = C.c_int32(0)
+        check(lib().mpm_group_reaction_history(self._h, _ptr(buf), cap, C.byref(n)), self._h)
+        return buf[: n.value * ng].reshape(n.value, ng).copy()
 
     def kernel_times(self):
         out = np.zeros(6)

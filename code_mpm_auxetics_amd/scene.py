@@ -171,6 +171,8 @@ class Scene:
     timeline_t: np.ndarray = None
     timeline_v: np.ndarray = None
     neohookean: int = 0
+    node_group: np.ndarray = None          # uint16 per node (batched worlds), or None
+    n_groups: int = 0
     info: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -413,3 +415,167 @@ def translation_world(v=(31.7, -19.3, 10.9), h=0.001, block=(4, 4, 2), cells=(40
                damping=0.0, mass_min=0.0, **mat)
     sc.info = dict(name="translation")
     return sc
+
+
+# ------------------------------------------------------------ batched worlds
+def splitmix64(seed):
+    """SplitMix64 (Steele, Lea and Flood, 2014): an infinite generator of
+    uniform doubles in [0, 1) from a 64-bit seed (the same recurrence in C++
+    and Python, so a seed fixes a batch everywhere)."""
+    m = (1 << 64) - 1
+    state = seed & m
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & m
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+        z ^= z >> 31
+        yield (z >> 11) * (1.0 / (1 << 53))
+
+
+def dent_for_angle(width, height, angle_deg):
+    """Dent of the polygon cell whose inclined walls make ``angle_deg`` with
+    the horizontal: tan(angle) = (H/2) / (W/2 - 2 dent)."""
+    return 0.5 * (0.5 * width - 0.5 * height / math.tan(math.radians(angle_deg)))
+
+
+def swisscheese_cell_world(width, wall, thickness, h, material="elastic", E=1.0e9, nu=0.3, rho=1200.0,
+                           sy=250.0e6, H=700.0e6, platen_speed=0.1, steps=2000, cfl=0.3, margin_cells=2,
+                           damping=0.001):
+    """One square swiss-cheese cell (createDomain_AuxeticCell_Swisscheese with
+    elliptic holes on a W/4 pitch, ligaments ``wall`` wide) between a fixed
+    bottom node plane and a top platen, as polygon_mesh_world builds it."""
+    off = 0.5 * h
+    extent = (width + 2 * margin_cells * h, width + (2 + margin_cells) * h, thickness)
+    L, cells = _grid_for(extent, (h, h, h))
+    L = (L[0], L[1], thickness)
+    cells = (cells[0], cells[1], max(1, int(round(thickness / h))))
+    pitch = 0.25 * width
+    r = 0.5 * (pitch - wall)
+    body = create_domain_auxetic_cell_swisscheese((0.5 * L[0], 0.5 * width, 0.5 * thickness),
+                                                  (width, width, thickness), (pitch, pitch), (1.3 * r, 0.7 * r), off)
+    return _cell_world(L, cells, body, width, width, thickness, h, material, E, nu, rho, sy, H, platen_speed, steps,
+                       cfl, damping, "swisscheese")
+
+
+def _cell_world(L, cells, body, mesh_w, mesh_h, thickness, h, material, E, nu, rho, sy, H, platen_speed, steps,
+                cfl, damping, name):
+    off = 0.5 * h
+    vol = off ** 3
+    if material == "elastic":
+        mat = _material_arrays(len(body), E, nu, 0.0, 0.0, 0.0, ELASTIC, rho, vol)
+    else:
+        C1 = 1.0e15
+        mat = _material_arrays(len(body), E, nu, sy, H / C1, C1, VONMISESHARDENING, rho, vol)
+    top = create_domain_cuboid((0.5 * L[0], mesh_h + 0.5 * h, 0.5 * thickness), (mesh_w, 2.0 * off, thickness), off)
+    plat = _material_arrays(len(top), 210.0e9, 0.33, 200.0e6, 0.0, 0.0, ELASTIC, 7800.0, vol)
+    c = p_wave_speed(E, nu, rho)
+    dt = cfl * h / c
+    sc = _assemble(L, cells, reference_bcs(L, cells), [(body, mat, False), (top, plat, True)], dt_max=dt,
+                   time_end=steps * dt, damping=damping, mass_min=0.001 * rho * vol,
+                   timeline_t=np.array([0.0, 1.0e3]),
+                   timeline_v=np.array([[0.0, -platen_speed, 0.0], [0.0, -platen_speed, 0.0]]))
+    sc.info = dict(name=name, h=h, dt=dt, wave_speed=c, platen_speed=platen_speed, steps=steps, material=material,
+                   cell_w=mesh_w, cell_h=mesh_h)
+    return sc
+
+
+def config4_worlds(n=256, seed=20240607, h=0.0002, thickness=0.001, platen_speed=0.1, steps=2000,
+                   width_range=(0.004, 0.0105)):
+    """``n`` independent single cells drawn from ``seed``
+    with SplitMix64: re-entrant angle uniform in [55, 80] degrees, wall
+    thickness uniform in [0.5, 1.2] mm, polygon or swiss-cheese topology,
+    elastic (E = 1 GPa) or J2 (aluminium) material, cell width uniform in
+    ``width_range`` (sizes 5k-80k points at h = 0.2 mm)."""
+    rnd = splitmix64(seed)
+    worlds = []
+    for _ in range(n):
+        angle = 55.0 + 25.0 * next(rnd)
+        wall = 0.0005 + 0.0007 * next(rnd)
+        poly = next(rnd) < 0.5
+        elastic = next(rnd) < 0.5
+        width = width_range[0] + (width_range[1] - width_range[0]) * next(rnd)
+        mat = dict(material="elastic", E=1.0e9, nu=0.3, rho=1200.0) if elastic else \
+            dict(material="j2", E=70.0e9, nu=0.33, rho=2700.0, sy=250.0e6, H=700.0e6)
+        if poly:
+            sc = polygon_mesh_world(1, 1, width, width, wall, thickness, h, platen_speed=platen_speed, steps=steps,
+                                    dent_frac=dent_for_angle(width, width, angle) / width, **mat)
+        else:
+            sc = swisscheese_cell_world(width, wall, thickness, h, platen_speed=platen_speed, steps=steps, **mat)
+        sc.info.update(angle=angle, wall=wall, topology="polygon" if poly else "swisscheese")
+        worlds.append(sc)
+    return worlds
+
+
+def batch_worlds(worlds, gap_cells=2, row_cells=None):
+    """Pack independent worlds into one grid for a single launch per step.
+
+    All worlds share the cell size, time step rule and TimeLine; each keeps its
+    own node BCs, and worlds sit ``gap_cells`` (>= 2) empty cells apart so no
+    node is touched by two of them.  Shelf packing in x-y rows of about
+    ``row_cells`` cells.  Returns the combined Scene with ``node_group`` (world
+    id per node), ``info["world"]`` (world id per point) and the offsets."""
+    if gap_cells < 2:
+        raise ValueError("worlds need >= 2 empty cells between them")
+    h = [worlds[0].length[a] / worlds[0].cells[a] for a in range(3)]
+    for w in worlds:
+        hw = [w.length[a] / w.cells[a] for a in range(3)]
+        if not np.allclose(hw, h, rtol=1e-12, atol=0.0):
+            raise ValueError("worlds must share the cell size")
+        if not (np.array_equal(w.timeline_t, worlds[0].timeline_t) and np.array_equal(w.timeline_v, worlds[0].timeline_v)):
+            raise ValueError("worlds must share the TimeLine")
+        if w.neohookean != worlds[0].neohookean or w.damping != worlds[0].damping:
+            raise ValueError("worlds must share damping and the NeoHookean dispatch flag")
+    if row_cells is None:
+        row_cells = int(math.ceil(math.sqrt(sum((w.cells[0] + gap_cells) * (w.cells[1] + gap_cells) for w in worlds))))
+    offs, ox, oy, row_h, width = [], 0, 0, 0, 0
+    for w in worlds:
+        if ox > 0 and ox + w.cells[0] > row_cells:
+            ox, oy, row_h = 0, oy + row_h + gap_cells, 0
+        offs.append((ox, oy))
+        ox += w.cells[0] + gap_cells
+        row_h = max(row_h, w.cells[1])
+        width = max(width, ox - gap_cells)
+    cells = (width, oy + row_h, max(w.cells[2] for w in worlds))
+    L = tuple(cells[a] * h[a] for a in range(3))
+    nn = [c + 1 for c in cells]
+    fixed = np.zeros(nn[0] * nn[1] * nn[2], np.uint8)
+    group = np.zeros(len(fixed), np.uint16)
+    cols = {k: [] for k in ("x", "mass", "vol0", "E", "nu", "sy", "C0", "C1", "type", "dc", "v", "fext", "surface")}
+    world_of = []
+    for i, (w, (ox, oy)) in enumerate(zip(worlds, offs)):
+        wn = [c + 1 for c in w.cells]
+        iz, iy, ix = np.meshgrid(np.arange(wn[2]), np.arange(wn[1]), np.arange(wn[0]), indexing="ij")
+        dst = (ox + ix) + nn[0] * ((oy + iy) + nn[1] * iz)
+        fixed[dst.ravel()] = w.fixed
+        group[dst.ravel()] = i
+        shift = np.array([ox * h[0], oy * h[1], 0.0])
+        for k in cols:
+            a = getattr(w, k)
+            cols[k].append(a + shift if k == "x" else a)
+        world_of.append(np.full(w.n, i, np.int32))
+    w0 = worlds[0]
+    sc = Scene(length=L, cells=cells, fixed=fixed, **{k: np.concatenate(v) for k, v in cols.items()},
+               dt_max=min(w.dt_max for w in worlds), time_end=max(w.time_end for w in worlds), damping=w0.damping,
+               mass_min=min(w.mass_min for w in worlds), timeline_t=w0.timeline_t, timeline_v=w0.timeline_v,
+               neohookean=w0.neohookean, node_group=group, n_groups=len(worlds))
+    sc.info = dict(name="batch", world=np.concatenate(world_of), offsets=offs, n_worlds=len(worlds))
+    return sc
+
+
+def effective_poisson(x0, x1, keep):
+    """Effective Poisson's ratio of one cell from point positions before and
+    after loading: -(lateral strain) / (axial strain), with the lateral
+    extent measured on the points of the middle half of the height and the
+    axial extent on all points, both as 2%-98% quantile spans."""
+    y0 = x0[keep, 1]
+    lo, hi = np.quantile(y0, [0.25, 0.75])
+    mid = keep.copy()
+    mid[keep] = (y0 >= lo) & (y0 <= hi)
+
+    def span(x, sel, a):
+        q = np.quantile(x[sel, a], [0.02, 0.98])
+        return q[1] - q[0]
+    eps_ax = span(x1, keep, 1) / span(x0, keep, 1) - 1.0
+    eps_lat = span(x1, mid, 0) / span(x0, mid, 0) - 1.0
+    return -eps_lat / eps_ax

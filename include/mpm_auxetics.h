@@ -102,6 +102,10 @@ typedef struct mpm_config {
     int32_t profile;           /* 1: HIP events around every kernel launch, read
                                   with mpm_kernel_times                         */
     int32_t launch;            /* enum mpm_launch                               */
+    const uint16_t *node_group;/* batched worlds: group (world) id per node, or
+                                  NULL; copied at mpm_create.  Groups must not
+                                  share nodes (keep 2 empty cells between them) */
+    int32_t n_groups;          /* number of groups when node_group != NULL      */
 } mpm_config;
 
 /* Host point arrays, FP64, caller-allocated, original point order.  Mirrors
@@ -164,6 +168,13 @@ int mpm_checks(mpm_handle *h, double out[12]);
  * "Force_y" of reportConsole, PhysicsEngine.cpp:65-71,195) of the steps of the
  * last mpm_step call; *n receives the count (<= cap). */
 int mpm_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t *n);
+
+/* Per-group reaction (the Force_y sum of each world's y-fixed nodes) of every
+ * step of the last mpm_step call on the plain path, out[step * n_groups + g];
+ * *n receives the step count (<= cap).  Zeros for contact-path steps.  No
+ * reference counterpart: the reference runs one world per PhysicsEngine
+ * (PhysicsEngine.cpp:65-71 is the single-world sum). */
+int mpm_group_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t *n);
 
 /* getTime_Current / getTime_End / getTime_Increment (PhysicsEngine.h:57-60) */
 int mpm_get_time(mpm_handle *h, double *time, double *time_end, double *dt_max);
