@@ -486,7 +486,8 @@ def config4_worlds(n=256, seed=20240607, h=0.0002, thickness=0.001, platen_speed
     with SplitMix64: re-entrant angle uniform in [55, 80] degrees, wall
     thickness uniform in [0.5, 1.2] mm, polygon or swiss-cheese topology,
     elastic (E = 1 GPa) or J2 (aluminium) material, cell width uniform in
-    ``width_range`` (sizes 5k-80k points at h = 0.2 mm)."""
+    ``width_range`` rounded to a multiple of 2h (sizes 5k-80k points at
+    h = 0.2 mm)."""
     rnd = splitmix64(seed)
     worlds = []
     for _ in range(n):
@@ -495,6 +496,10 @@ def config4_worlds(n=256, seed=20240607, h=0.0002, thickness=0.001, platen_speed
         poly = next(rnd) < 0.5
         elastic = next(rnd) < 0.5
         width = width_range[0] + (width_range[1] - width_range[0]) * next(rnd)
+        # on the grid (a multiple of 2h) as in the reference worlds: the point
+        # lattice then sits h/4 off the node planes instead of arbitrarily
+        # close to them (where the linear-hat update loses a node's mass)
+        width = 2 * h * max(2, round(width / (2 * h)))
         mat = dict(material="elastic", E=1.0e9, nu=0.3, rho=1200.0) if elastic else \
             dict(material="j2", E=70.0e9, nu=0.33, rho=2700.0, sy=250.0e6, H=700.0e6)
         if poly:
