@@ -39,6 +39,7 @@ EXPORTED = [
     "mpm_abi_version", "mpm_create", "mpm_upload_points", "mpm_step", "mpm_readback",
     "mpm_checks", "mpm_reaction_history", "mpm_get_time", "mpm_last_step_ms",
     "mpm_last_error", "mpm_destroy", "mpm_sort_keys", "mpm_kernel_times", "mpm_group_reaction_history",
+    "mpm_diff_create", "mpm_diff_forward", "mpm_diff_backward", "mpm_diff_last_error", "mpm_diff_destroy",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -117,8 +118,17 @@ def lib():
     L.mpm_destroy.restype = None
     L.mpm_sort_keys.argtypes = [_u32p, C.c_int32, _u32p, _u32p, C.c_int32]
     L.mpm_kernel_times.argtypes = [C.c_void_p, _dp]
+    L.mpm_diff_create.argtypes = [C.POINTER(MpmConfig), C.POINTER(MpmPoints), C.c_int32, C.c_int32,
+                                  C.POINTER(C.c_void_p)]
+    L.mpm_diff_forward.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.mpm_diff_backward.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.mpm_diff_last_error.argtypes = [C.c_void_p]
+    L.mpm_diff_last_error.restype = C.c_char_p
+    L.mpm_diff_destroy.argtypes = [C.c_void_p]
+    L.mpm_diff_destroy.restype = None
     for name in EXPORTED:
-        if name not in ("mpm_last_error", "mpm_destroy", "mpm_abi_version"):
+        if name not in ("mpm_last_error", "mpm_destroy", "mpm_abi_version", "mpm_diff_last_error",
+                        "mpm_diff_destroy"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -25,7 +25,7 @@ HIP_FLAGS = [
 
 
 def sources():
-    return [os.path.join(CSRC, "mpm_engine.hip")]
+    return [os.path.join(CSRC, "mpm_engine.hip"), os.path.join(CSRC, "mpm_diff.hip")]
 
 
 def build(verbose=False):

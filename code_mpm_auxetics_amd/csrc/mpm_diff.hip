@@ -1,0 +1,970 @@
+// mpm_diff.hip -- differentiable FP64 run of the plain step and its adjoint.
+//
+// The forward is the plain step of PhysicsEngine_runSimulation_Classic_-
+// SinglePass_MP.cpp:41-368 for elastic points (…Elastic.cpp:4-15) with the
+// strain increment L dt, on a dense node grid: P2G (…:106-202), grid update
+// with damping, boundary conditions and the platen override (…:209-265), G2P
+// with F, volume and stress update (…:271-319).  The backward pass is the
+// hand-written adjoint of each phase in reverse order; the forward keeps
+// checkpoints every C steps and the backward recomputes one segment at a time.
+//
+// Node sums are 64-bit fixed point with a per-node, per-field exponent: a
+// first pass over the points takes each node's largest term and term count,
+// the second adds the terms scaled so that the count of them fits in 62 bits.
+// The sums are exact, so a replay is bit-identical to the forward, and every
+// node keeps ~54 bits relative to its own largest term (small far-field nodes
+// are not quantised against the loaded region's forces).  Each point also
+// carries its displacement u = x - x0 as state, so that displacements of
+// 1e-9 m on positions of 1e-2 m keep full FP64 precision; the run's output
+// is u.
+//
+// Adjoint of one step (state s = x, u, v, F, sigma; parameters E; N and grad N
+// evaluated at the step's starting positions; piecewise-constant choices --
+// cells, mass gates, boundary masks, platen nodes -- have zero derivative):
+//   G2P      x' = x + dt sum_I N vt_I (u' likewise), v' = v + dt sum_I N a_I,
+//            L = sum_I vt_I (x) grad N,  F' = F + dt L F,
+//            sigma' = sigma + C(E) eps(L dt)
+//   grid     vh = P/m, vt = mask(vh + f/m dt) (or the platen velocity),
+//            a = mask(f)/m (0 on platen nodes)
+//   P2G      m_I = sum N m, P_I = sum N m v, f_I = -sum V sigma.grad N,
+//            V = det(F) V0
+// Position adjoints use grad N and the second derivatives of the linear hats
+// (d2N/dxa dxb = ga gb s_c for a != b, 0 for a == b).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_auxetics.h"
+#include "mpm_device.hpp"
+
+using namespace mpm;
+
+namespace {
+
+struct DGrid {
+    double L[3], h[3];
+    int cells[3], nodes[3];
+    long long nn;
+    double m_min;
+};
+
+struct DArgs {
+    DGrid g;
+    int n, n_dc, clear_adj;
+    double dt, damping;
+    // point state (SoA, component-major: x[a * n + p])
+    double *x, *u, *v, *F, *s;
+    const double *m, *V0, *E, *nu, *fext;
+    const uint8_t *fixed;
+    double *xdc, *vdc;             // platen positions / velocities (3 * n_dc)
+    double vtl[3];                 // TimeLine velocity of this step (platen move)
+    int has_tl;
+    unsigned long long *marks;     // per node: 1 + highest platen index, 0 = none
+    long long *acc;                // 7 x nn: m | P xyz | f xyz
+    double *nm, *nP, *nf, *nvt, *na;
+    uint8_t *nflag;                // 1 gate (m > m_min), 2 platen node
+    unsigned long long *nb;        // 5 x nn node bounds (positive doubles as bits):
+                                   // m | P | f | vt-bar | a-bar
+    unsigned *cnt;                 // per node: number of point terms
+    int32_t *err;
+    // adjoint
+    double *xb, *vb, *Fb, *sb, *Eb;
+    double *scratch;               // 12 per point: L-bar (9), dt x-bar' (3)
+    long long *aacc;               // 6 x nn: vt-bar xyz | a-bar xyz
+    double *nmb, *nPb, *nfb;
+};
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ void hat(const DGrid &g, const double x[3], int k, const int i0[3], double &N, double G[3],
+                                    double w[3], double gr[3])
+{
+    // corner k (x-outer, z-inner), node position (i L)/n, reference bases
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int ia = i0[a] + ((k >> (2 - a)) & 1);
+        const double xi = (double)ia * g.L[a] / (double)g.cells[a];
+        const double d = x[a] - xi;
+        double sv = 1.0 - fabs(d) / g.h[a];
+        if (sv < 0.0 || sv > 1.0) sv = 0.0;
+        w[a] = sv;
+        gr[a] = -sgn(d) / g.h[a];   // d s_a / d x_a
+    }
+    N = w[0] * w[1] * w[2];
+    G[0] = gr[0] * w[1] * w[2];
+    G[1] = w[0] * gr[1] * w[2];
+    G[2] = w[0] * w[1] * gr[2];
+}
+// d2N / dx_a dx_b
+__device__ __forceinline__ double hess(const double w[3], const double gr[3], int a, int b)
+{
+    if (a == b) return 0.0;
+    const int c = 3 - a - b;
+    return gr[a] * gr[b] * w[c];
+}
+__device__ __forceinline__ bool cell_of(const DGrid &g, const double x[3], int i0[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!(x[a] >= 0.0)) return false;
+        i0[a] = (int)(x[a] / g.h[a]);
+        if (!(i0[a] < g.cells[a])) return false;
+    }
+    return true;
+}
+__device__ __forceinline__ long long nidx(const DGrid &g, const int i0[3], int k)
+{
+    return (i0[0] + ((k >> 2) & 1)) + (long long)g.nodes[0] * ((i0[1] + ((k >> 1) & 1)) + (long long)g.nodes[1] * (i0[2] + (k & 1)));
+}
+__device__ __forceinline__ void umax(unsigned long long *p, double v)
+{
+    if (v > 0.0) atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+// Fixed-point scale of node field q: 2^(62 - H - e) with b < 2^e the node's
+// largest term and 2^H > 2 (count + 1), so that count terms fit in 62 bits.
+__device__ __forceinline__ double node_scale(const DArgs &a, int q, long long I, bool inverse)
+{
+    const double b = __longlong_as_double((long long)a.nb[q * a.g.nn + I]);
+    if (!(b > 0.0)) return 0.0;
+    int e;
+    frexp(b, &e);
+    const unsigned c = a.cnt[I];
+    int H = 1;
+    while ((1ull << H) <= 2ull * (c + 1ull)) ++H;
+    const int sh = 62 - H - e;
+    return inverse ? ldexp(1.0, -sh) : ldexp(1.0, sh);
+}
+__device__ __forceinline__ void fadd(long long *p, double v, double sc)
+{
+    atomicAdd((unsigned long long *)p, (unsigned long long)__double2ll_rn(v * sc));
+}
+// column-major 3x3 (m[3c + r]) products
+__device__ __forceinline__ void matmul(const double A[9], const double B[9], double C[9])
+{
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) C[3 * c + r] = A[r] * B[3 * c] + A[3 + r] * B[3 * c + 1] + A[6 + r] * B[3 * c + 2];
+}
+// elastic stress increment of eps (engineering shear): Hooke (…Elastic.cpp:4-15)
+__device__ __forceinline__ void hooke(double E, double nu, const double de[6], double ds[6])
+{
+    const double C = E / (1.0 + nu) / (1.0 - 2.0 * nu);
+    ds[0] = C * ((1.0 - nu) * de[0] + nu * de[1] + nu * de[2]);
+    ds[1] = C * ((1.0 - nu) * de[1] + nu * de[0] + nu * de[2]);
+    ds[2] = C * ((1.0 - nu) * de[2] + nu * de[0] + nu * de[1]);
+    ds[3] = C * ((0.5 - nu) * de[3]);
+    ds[4] = C * ((0.5 - nu) * de[4]);
+    ds[5] = C * ((0.5 - nu) * de[5]);
+}
+// symmetric Voigt eps of L dt (…_MP.cpp:304-312 with L dt)
+__device__ __forceinline__ void eps_of(const double L[9], double dt, double de[6])
+{
+    de[0] = L[0] * dt;
+    de[1] = L[4] * dt;
+    de[2] = L[8] * dt;
+    de[3] = L[1] * dt + L[3] * dt;
+    de[4] = L[5] * dt + L[7] * dt;
+    de[5] = L[6] * dt + L[2] * dt;
+}
+// f contribution of stress s (Voigt) and gradient G: -V (sigma . G)
+__device__ __forceinline__ void sigma_dot(const double s[6], const double G[3], double o[3])
+{
+    o[0] = G[0] * s[0] + G[1] * s[3] + G[2] * s[5];
+    o[1] = G[1] * s[1] + G[0] * s[3] + G[2] * s[4];
+    o[2] = G[2] * s[2] + G[0] * s[5] + G[1] * s[4];
+}
+
+// --------------------------------------------------------------- forward
+__global__ void kd_clear(DArgs a)
+{
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I < a.g.nn) {
+        for (int q = 0; q < 7; ++q) a.acc[q * a.g.nn + I] = 0;
+        for (int q = 0; q < 3; ++q) a.nb[q * a.g.nn + I] = 0ull;
+        if (a.clear_adj) {
+            for (int q = 0; q < 6; ++q) a.aacc[q * a.g.nn + I] = 0;
+            for (int q = 3; q < 5; ++q) a.nb[q * a.g.nn + I] = 0ull;
+        }
+        a.cnt[I] = 0u;
+        a.marks[I] = 0ull;
+    }
+}
+
+// P2G of mass, momentum and force (…_MP.cpp:106-202).  BOUNDS: count the
+// terms of every node and take each field's largest term; else add them.
+template <bool BOUNDS>
+__global__ void kd_p2g(DArgs a)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.n) return;
+    const int n = a.n;
+    const long long nn = a.g.nn;
+    const double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
+    int i0[3];
+    if (!cell_of(a.g, x, i0)) {
+        if (BOUNDS) atomicOr(a.err, 1);
+        return;
+    }
+    double F[9], s[6], v[3], fe[3];
+    for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
+    for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
+    for (int q = 0; q < 3; ++q) {
+        v[q] = a.v[q * n + p];
+        fe[q] = a.fext[q * n + p];
+    }
+    const double m = a.m[p], V = det3(F) * a.V0[p];
+    for (int k = 0; k < 8; ++k) {
+        double N, G[3], w[3], gr[3], sg[3], tP[3], tf[3];
+        hat(a.g, x, k, i0, N, G, w, gr);
+        const long long I = nidx(a.g, i0, k);
+        sigma_dot(s, G, sg);
+        const double tm = N * m;
+        for (int q = 0; q < 3; ++q) {
+            tP[q] = N * m * v[q];
+            tf[q] = -V * sg[q] + N * fe[q];
+        }
+        if (BOUNDS) {
+            atomicAdd(&a.cnt[I], 1u);
+            umax(&a.nb[I], fabs(tm));
+            umax(&a.nb[nn + I], fmax(fabs(tP[0]), fmax(fabs(tP[1]), fabs(tP[2]))));
+            umax(&a.nb[2 * nn + I], fmax(fabs(tf[0]), fmax(fabs(tf[1]), fabs(tf[2]))));
+        } else {
+            fadd(&a.acc[I], tm, node_scale(a, 0, I, false));
+            const double sp = node_scale(a, 1, I, false), sf = node_scale(a, 2, I, false);
+            for (int q = 0; q < 3; ++q) {
+                fadd(&a.acc[(1 + q) * nn + I], tP[q], sp);
+                fadd(&a.acc[(4 + q) * nn + I], tf[q], sf);
+            }
+        }
+    }
+}
+
+__global__ void kd_marks(DArgs a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_dc) return;
+    int i0[3];
+    for (int q = 0; q < 3; ++q) i0[q] = (int)(a.xdc[3 * j + q] / a.g.h[q]);
+    for (int k = 0; k < 8; ++k) {
+        const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
+        if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= a.g.nodes[0] || n1 >= a.g.nodes[1] || n2 >= a.g.nodes[2]) continue;
+        atomicMax(&a.marks[n0 + (long long)a.g.nodes[0] * (n1 + (long long)a.g.nodes[1] * n2)], (unsigned long long)j + 1ull);
+    }
+}
+
+// grid momentum, damping, boundary conditions and platen override
+// (…_MP.cpp:209-265)
+__global__ void kd_grid(DArgs a)
+{
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I >= a.g.nn) return;
+    const long long nn = a.g.nn;
+    const double um = node_scale(a, 0, I, true), up = node_scale(a, 1, I, true), uf = node_scale(a, 2, I, true);
+    const double m = (double)a.acc[I] * um;
+    double P[3], f[3], vh[3], fd[3], vt[3], ac[3];
+    for (int q = 0; q < 3; ++q) {
+        P[q] = (double)a.acc[(1 + q) * nn + I] * up;
+        f[q] = (double)a.acc[(4 + q) * nn + I] * uf;
+    }
+    const bool gate = m > a.g.m_min;
+    for (int q = 0; q < 3; ++q) {
+        vh[q] = gate ? P[q] / m : 0.0;
+        fd[q] = f[q];
+    }
+    const double vv = vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2];
+    if (sqrt(vv) > 1.0e-9) {
+        const double lf = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        const double inv = 1.0 / sqrt(vv), sc = a.damping * lf;
+        for (int q = 0; q < 3; ++q) fd[q] -= sc * (vh[q] * inv);
+    }
+    for (int q = 0; q < 3; ++q) vt[q] = gate ? vh[q] + fd[q] / m * a.dt : vh[q];
+    const uint8_t fx = a.fixed[I];
+    for (int q = 0; q < 3; ++q)
+        if (fx & (1u << q)) { vt[q] = 0.0; fd[q] = 0.0; }
+    uint8_t flag = gate ? 1 : 0;
+    const unsigned long long mk = a.marks[I];
+    if (mk) {
+        const int j = (int)(mk - 1ull);
+        for (int q = 0; q < 3; ++q) { vt[q] = a.vdc[3 * j + q]; fd[q] = 0.0; }
+        flag |= 2;
+    }
+    for (int q = 0; q < 3; ++q) ac[q] = (fabs(m) > a.g.m_min) ? fd[q] / m : 0.0;
+    a.nm[I] = m;
+    for (int q = 0; q < 3; ++q) {
+        a.nP[q * nn + I] = P[q];
+        a.nf[q * nn + I] = f[q];
+        a.nvt[q * nn + I] = vt[q];
+        a.na[q * nn + I] = ac[q];
+    }
+    a.nflag[I] = flag;
+}
+
+__global__ void kd_g2p(DArgs a)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.n) return;
+    const int n = a.n;
+    const long long nn = a.g.nn;
+    double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
+    int i0[3];
+    if (!cell_of(a.g, x, i0)) return;
+    double dx[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 8; ++k) {
+        double N, G[3], w[3], gr[3];
+        hat(a.g, x, k, i0, N, G, w, gr);
+        const long long I = nidx(a.g, i0, k);
+        for (int q = 0; q < 3; ++q) {
+            const double vt = a.nvt[q * nn + I];
+            dx[q] += N * vt;
+            dv[q] += N * a.na[q * nn + I];
+            for (int c = 0; c < 3; ++c) L[3 * c + q] += vt * G[c];
+        }
+    }
+    double F[9], T[9], s[6], de[6], ds[6];
+    for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
+    matmul(L, F, T);
+    for (int q = 0; q < 9; ++q) a.F[q * n + p] = F[q] + T[q] * a.dt;
+    for (int q = 0; q < 3; ++q) {
+        a.x[q * n + p] = x[q] + dx[q] * a.dt;
+        a.u[q * n + p] += dx[q] * a.dt;
+        a.v[q * n + p] += dv[q] * a.dt;
+    }
+    eps_of(L, a.dt, de);
+    hooke(a.E[p], a.nu[p], de, ds);
+    for (int q = 0; q < 6; ++q) {
+        s[q] = a.s[q * n + p];
+        a.s[q * n + p] = s[q] + ds[q];
+    }
+}
+
+// platen motion at the end of the step (…_MP.cpp:360-368)
+__global__ void kd_dc(DArgs a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n_dc) return;
+    for (int q = 0; q < 3; ++q) {
+        if (a.has_tl) a.vdc[3 * j + q] = a.vtl[q];
+        a.xdc[3 * j + q] += a.vdc[3 * j + q] * a.dt;
+    }
+}
+
+// --------------------------------------------------------------- adjoint
+// G2P and the material update, per point: in xb, vb, Fb, sb = adjoints of
+// the state after the step; out = adjoints of the state before it (partial
+// in x: the P2G terms are added by kd_a_p2g).  Writes the node scatter
+// inputs.
+__global__ void kd_a_g2p(DArgs a)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.n) return;
+    const int n = a.n;
+    const long long nn = a.g.nn;
+    const double dt = a.dt;
+    double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
+    int i0[3];
+    if (!cell_of(a.g, x, i0)) return;
+    double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 8; ++k) {
+        double N, G[3], w[3], gr[3];
+        hat(a.g, x, k, i0, N, G, w, gr);
+        const long long I = nidx(a.g, i0, k);
+        for (int q = 0; q < 3; ++q)
+            for (int c = 0; c < 3; ++c) L[3 * c + q] += a.nvt[q * nn + I] * G[c];
+    }
+    double F[9], Fb[9], sbv[6], xb[3], vb[3];
+    for (int q = 0; q < 9; ++q) { F[q] = a.F[q * n + p]; Fb[q] = a.Fb[q * n + p]; }
+    for (int q = 0; q < 6; ++q) sbv[q] = a.sb[q * n + p];
+    for (int q = 0; q < 3; ++q) { xb[q] = a.xb[q * n + p]; vb[q] = a.vb[q * n + p]; }
+    // sigma' = sigma + C(E) eps(L dt): E-bar, eps-bar = C^T sigma-bar'
+    const double E = a.E[p], nu = a.nu[p];
+    double de[6], ds[6], eb[6];
+    eps_of(L, dt, de);
+    hooke(E, nu, de, ds);
+    double gE = 0.0;
+    for (int q = 0; q < 6; ++q) gE += sbv[q] * ds[q] / E;
+    a.Eb[p] += gE;
+    hooke(E, nu, sbv, eb);   // the normal block is symmetric; shear: C (1/2 - nu)
+    // L-bar from eps (engineering shear) and from F' = F + dt L F
+    double Lb[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) Lb[3 * c + r] = dt * (Fb[r] * F[c] + Fb[3 + r] * F[3 + c] + Fb[6 + r] * F[6 + c]);
+    Lb[0] += dt * eb[0];
+    Lb[4] += dt * eb[1];
+    Lb[8] += dt * eb[2];
+    Lb[1] += dt * eb[3];
+    Lb[3] += dt * eb[3];
+    Lb[5] += dt * eb[4];
+    Lb[7] += dt * eb[4];
+    Lb[6] += dt * eb[5];
+    Lb[2] += dt * eb[5];
+    // F-bar = F-bar' + dt L^T F-bar'
+    double Fn[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) Fn[3 * c + r] = Fb[3 * c + r] + dt * (L[3 * r] * Fb[3 * c] + L[3 * r + 1] * Fb[3 * c + 1] + L[3 * r + 2] * Fb[3 * c + 2]);
+    // x-bar through the bases of x', v' and L
+    double xn[3] = {xb[0], xb[1], xb[2]};
+    for (int k = 0; k < 8; ++k) {
+        double N, G[3], w[3], gr[3];
+        hat(a.g, x, k, i0, N, G, w, gr);
+        const long long I = nidx(a.g, i0, k);
+        double vt[3], ac[3];
+        for (int q = 0; q < 3; ++q) { vt[q] = a.nvt[q * nn + I]; ac[q] = a.na[q * nn + I]; }
+        const double cx = dt * (vt[0] * xb[0] + vt[1] * xb[1] + vt[2] * xb[2] + ac[0] * vb[0] + ac[1] * vb[1] + ac[2] * vb[2]);
+        for (int q = 0; q < 3; ++q) xn[q] += cx * G[q];
+        // L_rc = sum vt_r dN/dx_c: x_a-bar += sum_rc Lb_rc vt_r d2N/dx_a dx_c
+        for (int aa = 0; aa < 3; ++aa) {
+            double t = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double hc = hess(w, gr, aa, c);
+                if (hc != 0.0) t += hc * (Lb[3 * c] * vt[0] + Lb[3 * c + 1] * vt[1] + Lb[3 * c + 2] * vt[2]);
+            }
+            xn[aa] += t;
+        }
+    }
+    for (int q = 0; q < 9; ++q) a.Fb[q * n + p] = Fn[q];
+    for (int q = 0; q < 3; ++q) a.xb[q * n + p] = xn[q];
+    // scatter inputs: L-bar and dt x-bar'; v-bar is unchanged (v-bar = v-bar')
+    double *sc = a.scratch + 12 * (size_t)p;
+    for (int q = 0; q < 9; ++q) sc[q] = Lb[q];
+    for (int q = 0; q < 3; ++q) sc[9 + q] = dt * xb[q];
+}
+
+// scatter: vt-bar_I += N dt x-bar' + L-bar . grad N,  a-bar_I += N dt v-bar
+// (BOUNDS: each node's largest term, as in kd_p2g; the term counts are the
+// forward's, same points and cells)
+template <bool BOUNDS>
+__global__ void kd_a_scatter(DArgs a)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.n) return;
+    const int n = a.n;
+    const long long nn = a.g.nn;
+    double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
+    int i0[3];
+    if (!cell_of(a.g, x, i0)) return;
+    const double *sc = a.scratch + 12 * (size_t)p;
+    double vb[3];
+    for (int q = 0; q < 3; ++q) vb[q] = a.vb[q * n + p] * a.dt;
+    for (int k = 0; k < 8; ++k) {
+        double N, G[3], w[3], gr[3], t[3], ta[3];
+        hat(a.g, x, k, i0, N, G, w, gr);
+        const long long I = nidx(a.g, i0, k);
+        for (int r = 0; r < 3; ++r) {
+            t[r] = N * sc[9 + r] + sc[r] * G[0] + sc[3 + r] * G[1] + sc[6 + r] * G[2];
+            ta[r] = N * vb[r];
+        }
+        if (BOUNDS) {
+            umax(&a.nb[3 * nn + I], fmax(fabs(t[0]), fmax(fabs(t[1]), fabs(t[2]))));
+            umax(&a.nb[4 * nn + I], fmax(fabs(ta[0]), fmax(fabs(ta[1]), fabs(ta[2]))));
+        } else {
+            const double s3 = node_scale(a, 3, I, false), s4 = node_scale(a, 4, I, false);
+            for (int r = 0; r < 3; ++r) {
+                fadd(&a.aacc[r * nn + I], t[r], s3);
+                fadd(&a.aacc[(3 + r) * nn + I], ta[r], s4);
+            }
+        }
+    }
+}
+
+// grid update reversed, per node: (vt-bar, a-bar) -> (m-bar, P-bar, f-bar)
+__global__ void kd_a_grid(DArgs a)
+{
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I >= a.g.nn) return;
+    const long long nn = a.g.nn;
+    const double u3 = node_scale(a, 3, I, true), u4 = node_scale(a, 4, I, true);
+    double vtb[3], ab[3], P[3], f[3];
+    for (int q = 0; q < 3; ++q) {
+        vtb[q] = (double)a.aacc[q * nn + I] * u3;
+        ab[q] = (double)a.aacc[(3 + q) * nn + I] * u4;
+        P[q] = a.nP[q * nn + I];
+        f[q] = a.nf[q * nn + I];
+    }
+    const double m = a.nm[I];
+    double mb = 0.0, Pb[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
+    const uint8_t flag = a.nflag[I];
+    if (!(flag & 2)) {   // platen nodes take the platen velocity: no dependence
+        const bool gate = flag & 1;
+        const uint8_t fx = a.fixed[I];
+        for (int q = 0; q < 3; ++q)
+            if (fx & (1u << q)) { vtb[q] = 0.0; ab[q] = 0.0; }
+        // forward recomputed: vh = P/m, fd = f - c |f| vh/|vh|
+        double vh[3], fd[3], hat_v[3] = {0, 0, 0};
+        for (int q = 0; q < 3; ++q) {
+            vh[q] = gate ? P[q] / m : 0.0;
+            fd[q] = f[q];
+        }
+        const double vv = vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2];
+        const bool damped = sqrt(vv) > 1.0e-9;
+        const double lf = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        const double inv = damped ? 1.0 / sqrt(vv) : 0.0;
+        if (damped)
+            for (int q = 0; q < 3; ++q) {
+                hat_v[q] = vh[q] * inv;
+                fd[q] -= a.damping * lf * hat_v[q];
+            }
+        double fdb[3] = {0, 0, 0}, vhb[3] = {0, 0, 0};
+        // a = fd / m
+        if (fabs(m) > a.g.m_min)
+            for (int q = 0; q < 3; ++q) {
+                fdb[q] += ab[q] / m;
+                mb -= ab[q] * fd[q] / (m * m);
+            }
+        // vt = vh + fd / m dt
+        if (gate)
+            for (int q = 0; q < 3; ++q) {
+                vhb[q] += vtb[q];
+                fdb[q] += vtb[q] / m * a.dt;
+                mb -= vtb[q] * fd[q] * a.dt / (m * m);
+            }
+        for (int q = 0; q < 3; ++q) fb[q] = fdb[q];
+        if (damped) {
+            const double fdv = fdb[0] * hat_v[0] + fdb[1] * hat_v[1] + fdb[2] * hat_v[2];
+            if (lf > 0.0)
+                for (int q = 0; q < 3; ++q) fb[q] -= a.damping * fdv * f[q] / lf;
+            // hat-bar = -c |f| fd-bar; vh-bar += (I - hat hat^T) hat-bar / |vh|
+            for (int q = 0; q < 3; ++q) vhb[q] += -a.damping * lf * (fdb[q] - fdv * hat_v[q]) * inv;
+        }
+        if (gate)
+            for (int q = 0; q < 3; ++q) {
+                Pb[q] = vhb[q] / m;
+                mb -= vhb[q] * P[q] / (m * m);
+            }
+    }
+    a.nmb[I] = mb;
+    for (int q = 0; q < 3; ++q) {
+        a.nPb[q * nn + I] = Pb[q];
+        a.nfb[q * nn + I] = fb[q];
+    }
+}
+
+// P2G reversed, per point (gather): v-bar, sigma-bar, F-bar (through V) and
+// the P2G part of x-bar
+__global__ void kd_a_p2g(DArgs a)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.n) return;
+    const int n = a.n;
+    const long long nn = a.g.nn;
+    double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
+    int i0[3];
+    if (!cell_of(a.g, x, i0)) return;
+    double F[9], s[6], v[3], fe[3];
+    for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
+    for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
+    for (int q = 0; q < 3; ++q) {
+        v[q] = a.v[q * n + p];
+        fe[q] = a.fext[q * n + p];
+    }
+    const double m = a.m[p], det = det3(F), V = det * a.V0[p];
+    double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, sb[6] = {0, 0, 0, 0, 0, 0}, Vb = 0.0;
+    for (int k = 0; k < 8; ++k) {
+        double N, G[3], w[3], gr[3];
+        hat(a.g, x, k, i0, N, G, w, gr);
+        const long long I = nidx(a.g, i0, k);
+        const double mb = a.nmb[I];
+        double Pb[3], fb[3];
+        for (int q = 0; q < 3; ++q) { Pb[q] = a.nPb[q * nn + I]; fb[q] = a.nfb[q * nn + I]; }
+        for (int q = 0; q < 3; ++q) vb[q] += N * m * Pb[q];
+        const double cs = mb * m + m * (v[0] * Pb[0] + v[1] * Pb[1] + v[2] * Pb[2]) +
+                          (fb[0] * fe[0] + fb[1] * fe[1] + fb[2] * fe[2]);
+        for (int q = 0; q < 3; ++q) xb[q] += cs * G[q];
+        // f_I = -V sigma.G
+        double sg[3];
+        sigma_dot(s, G, sg);
+        Vb -= fb[0] * sg[0] + fb[1] * sg[1] + fb[2] * sg[2];
+        sb[0] -= V * fb[0] * G[0];
+        sb[1] -= V * fb[1] * G[1];
+        sb[2] -= V * fb[2] * G[2];
+        sb[3] -= V * (fb[0] * G[1] + fb[1] * G[0]);
+        sb[4] -= V * (fb[1] * G[2] + fb[2] * G[1]);
+        sb[5] -= V * (fb[0] * G[2] + fb[2] * G[0]);
+        // x-bar from d(sigma.G)/dx: f_i = -V sum_j sigma_ij dN/dx_j
+        const double S[9] = {s[0], s[3], s[5], s[3], s[1], s[4], s[5], s[4], s[2]};   // S[3i + j]
+        for (int aa = 0; aa < 3; ++aa) {
+            double t = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                const double hj = hess(w, gr, aa, j);
+                if (hj != 0.0) t += hj * (fb[0] * S[j] + fb[1] * S[3 + j] + fb[2] * S[6 + j]);
+            }
+            xb[aa] -= V * t;
+        }
+    }
+    for (int q = 0; q < 3; ++q) {
+        a.xb[q * n + p] += xb[q];
+        a.vb[q * n + p] += vb[q];
+    }
+    for (int q = 0; q < 6; ++q) a.sb[q * n + p] += sb[q];
+    // V = det(F) V0: F-bar += V-bar V0 det(F) F^-T (cofactor matrix)
+    const double cof[9] = {F[4] * F[8] - F[5] * F[7], F[5] * F[6] - F[3] * F[8], F[3] * F[7] - F[4] * F[6],
+                           F[2] * F[7] - F[1] * F[8], F[0] * F[8] - F[2] * F[6], F[1] * F[6] - F[0] * F[7],
+                           F[1] * F[5] - F[2] * F[4], F[2] * F[3] - F[0] * F[5], F[0] * F[4] - F[1] * F[3]};
+    // cof[3c + r] = d det / d F[3c + r]
+    for (int q = 0; q < 9; ++q) a.Fb[q * n + p] += Vb * a.V0[p] * cof[q];
+}
+
+}  // namespace
+
+// ================================================================ handle
+struct mpm_diff {
+    hipStream_t stream = nullptr;
+    std::string err;
+    DGrid g{};
+    int device = 0, n = 0, n_dc = 0, n_total = 0, steps = 0, ckpt = 1, n_ckpt = 0;
+    double dt = 0.0, damping = 0.0;
+    std::vector<double> tl_t, tl_v, times;   // times[k]: start time of step k
+    std::vector<int> def_ids, dc_ids;
+    // device
+    double *x = nullptr, *u = nullptr, *v = nullptr, *F = nullptr, *s = nullptr, *m = nullptr, *V0 = nullptr,
+           *E = nullptr, *nu = nullptr, *fext = nullptr;
+    uint8_t *fixed = nullptr;
+    double *xdc = nullptr, *vdc = nullptr;
+    double *x0dc = nullptr, *v0dc = nullptr;   // platen state at step 0
+    double *v0 = nullptr, *F0 = nullptr;       // initial point velocities, identity F
+    unsigned long long *marks = nullptr, *nb = nullptr;
+    unsigned *cnt = nullptr;
+    long long *acc = nullptr, *aacc = nullptr;
+    double *nm = nullptr, *nP = nullptr, *nf = nullptr, *nvt = nullptr, *na = nullptr;
+    double *nmb = nullptr, *nPb = nullptr, *nfb = nullptr;
+    uint8_t *nflag = nullptr;
+    int32_t *errd = nullptr;
+    double *xb = nullptr, *vb = nullptr, *Fb = nullptr, *sb = nullptr, *Eb = nullptr, *scratch = nullptr;
+    double *ck = nullptr;      // checkpoints: n_ckpt x state
+    double *seg = nullptr;     // one segment: ckpt x state
+    bool have_forward = false;
+    // doubles of one saved state: x, u, v (3 n each), F (9 n), sigma (6 n),
+    // platen x and v (3 n_dc each)
+    size_t state() const { return 24 * (size_t)n + 6 * (size_t)n_dc; }
+};
+
+namespace {
+
+int dfail(mpm_diff *d, const char *msg, int code)
+{
+    d->err = msg;
+    return code;
+}
+#define DCHK(x)                                                        \
+    do {                                                               \
+        hipError_t e_ = (x);                                           \
+        if (e_ != hipSuccess) return dfail(d, hipGetErrorString(e_), MPM_E_HIP); \
+    } while (0)
+template <typename T>
+int dalloc_(mpm_diff *d, T **p, size_t n)
+{
+    if (hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return dfail(d, "hipMalloc", MPM_E_NOMEM);
+    return MPM_OK;
+}
+inline unsigned gsz(size_t n) { return (unsigned)((n + 255) / 256); }
+
+DArgs args_of(mpm_diff *d, int step, bool adjoint)
+{
+    DArgs a{};
+    a.g = d->g;
+    a.n = d->n;
+    a.n_dc = d->n_dc;
+    a.clear_adj = adjoint ? 1 : 0;
+    a.dt = d->dt;
+    a.damping = d->damping;
+    a.fext = d->fext;
+    a.x = d->x; a.u = d->u; a.v = d->v; a.F = d->F; a.s = d->s;
+    a.m = d->m; a.V0 = d->V0; a.E = d->E; a.nu = d->nu;
+    a.fixed = d->fixed;
+    a.xdc = d->xdc; a.vdc = d->vdc;
+    // TimeLine::getVelocity at the step's start time (TimeLine.h:33-46)
+    const double t = d->times[step];
+    a.vtl[0] = a.vtl[1] = a.vtl[2] = 0.0;
+    const int nt = (int)d->tl_t.size();
+    for (int i = 0; i < nt - 1; ++i)
+        if (d->tl_t[i] <= t && t < d->tl_t[i + 1]) {
+            const double sc = (t - d->tl_t[i]) / (d->tl_t[i + 1] - d->tl_t[i]);
+            for (int q = 0; q < 3; ++q) a.vtl[q] = (1.0 - sc) * d->tl_v[3 * i + q] + sc * d->tl_v[3 * (i + 1) + q];
+        }
+    a.has_tl = nt > 0;
+    a.marks = d->marks; a.acc = d->acc;
+    a.nm = d->nm; a.nP = d->nP; a.nf = d->nf; a.nvt = d->nvt; a.na = d->na; a.nflag = d->nflag;
+    a.nb = d->nb; a.cnt = d->cnt; a.err = d->errd;
+    a.xb = d->xb; a.vb = d->vb; a.Fb = d->Fb; a.sb = d->sb; a.Eb = d->Eb; a.scratch = d->scratch;
+    a.aacc = d->aacc; a.nmb = d->nmb; a.nPb = d->nPb; a.nfb = d->nfb;
+    return a;
+}
+
+// node quantities of the step from the current point state (P2G + grid)
+int forward_nodes(mpm_diff *d, const DArgs &a)
+{
+    const long long nn = d->g.nn;
+    kd_clear<<<gsz(nn), 256, 0, d->stream>>>(a);
+    if (d->n > 0) {
+        kd_p2g<true><<<gsz(d->n), 256, 0, d->stream>>>(a);
+        kd_p2g<false><<<gsz(d->n), 256, 0, d->stream>>>(a);
+    }
+    if (d->n_dc > 0) kd_marks<<<gsz(d->n_dc), 256, 0, d->stream>>>(a);
+    kd_grid<<<gsz(nn), 256, 0, d->stream>>>(a);
+    DCHK(hipGetLastError());
+    return MPM_OK;
+}
+
+int forward_step(mpm_diff *d, int step)
+{
+    DArgs a = args_of(d, step, false);
+    int rc = forward_nodes(d, a);
+    if (rc) return rc;
+    if (d->n > 0) kd_g2p<<<gsz(d->n), 256, 0, d->stream>>>(a);
+    if (d->n_dc > 0) kd_dc<<<gsz(d->n_dc), 256, 0, d->stream>>>(a);
+    DCHK(hipGetLastError());
+    return MPM_OK;
+}
+
+// copy the state (x, u, v, F, sigma, platen x and v) to dst (save) or from
+// src (load), stream-ordered
+int move_state(mpm_diff *d, double *buf, bool save)
+{
+    const size_t n = d->n, nd = d->n_dc;
+    double *parts[7] = {d->x, d->u, d->v, d->F, d->s, d->xdc, d->vdc};
+    const size_t len[7] = {3 * n, 3 * n, 3 * n, 9 * n, 6 * n, 3 * nd, 3 * nd};
+    size_t off = 0;
+    for (int q = 0; q < 7; ++q) {
+        if (len[q]) {
+            double *dst = save ? buf + off : parts[q];
+            const double *src = save ? parts[q] : buf + off;
+            DCHK(hipMemcpyAsync(dst, src, sizeof(double) * len[q], hipMemcpyDeviceToDevice, d->stream));
+        }
+        off += len[q];
+    }
+    return MPM_OK;
+}
+
+void free_all(mpm_diff *d)
+{
+    void *ptrs[] = {d->x, d->u, d->v, d->F, d->s, d->m, d->V0, d->E, d->nu, d->fext, d->fixed, d->xdc, d->vdc,
+                    d->x0dc, d->v0dc, d->v0, d->F0, d->marks, d->nb, d->cnt, d->acc, d->aacc, d->nm, d->nP, d->nf,
+                    d->nvt, d->na, d->nmb, d->nPb, d->nfb, d->nflag, d->errd, d->xb, d->vb, d->Fb, d->sb, d->Eb,
+                    d->scratch, d->ck, d->seg};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_steps, int32_t checkpoint, mpm_diff **out)
+{
+    if (!cfg || !pts || !out || n_steps < 1 || checkpoint < 1 || pts->n < 1 || !pts->x || !pts->mass || !pts->nu ||
+        (!pts->vol0 && !pts->vol) || (cfg->n_timeline > 0 && (!cfg->timeline_t || !cfg->timeline_v)))
+        return MPM_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MPM_E_NO_DEVICE;
+    if (cfg->device < 0 || cfg->device >= ndev || hipSetDevice(cfg->device) != hipSuccess) return MPM_E_ARG;
+    mpm_diff *d = new mpm_diff();
+    d->device = cfg->device;
+    auto fail = [&](int code) {
+        free_all(d);
+        delete d;
+        return code;
+    };
+    for (int a = 0; a < 3; ++a) {
+        if (!(cfg->length[a] > 0.0) || cfg->cells[a] < 1) return fail(MPM_E_ARG);
+        d->g.L[a] = cfg->length[a];
+        d->g.cells[a] = cfg->cells[a];
+        d->g.nodes[a] = cfg->cells[a] + 1;
+        d->g.h[a] = cfg->length[a] / (double)cfg->cells[a];
+    }
+    if (!(cfg->dt_max > 0.0)) return fail(MPM_E_ARG);
+    d->g.nn = (long long)d->g.nodes[0] * d->g.nodes[1] * d->g.nodes[2];
+    d->g.m_min = cfg->mass_min;
+    d->dt = cfg->dt_max;
+    d->damping = cfg->damping;
+    // start time of each step, accumulated as the reference's d_Time (…_MP.cpp:381-393)
+    d->times.resize(n_steps);
+    double t = cfg->time_start;
+    for (int k = 0; k < n_steps; ++k) {
+        d->times[k] = t;
+        t += cfg->dt_max;
+    }
+    d->steps = n_steps;
+    d->ckpt = std::min(checkpoint, n_steps);
+    d->n_ckpt = (n_steps + d->ckpt - 1) / d->ckpt;
+    d->tl_t.assign(cfg->timeline_t, cfg->timeline_t + cfg->n_timeline);
+    d->tl_v.assign(cfg->timeline_v, cfg->timeline_v + 3 * cfg->n_timeline);
+    d->n_total = pts->n;
+    for (int i = 0; i < pts->n; ++i) ((pts->dc && pts->dc[i]) ? d->dc_ids : d->def_ids).push_back(i);
+    d->n = (int)d->def_ids.size();
+    d->n_dc = (int)d->dc_ids.size();
+    if (d->n < 1) {
+        d->err = "differentiable run: no deformable points";
+        return fail(MPM_E_ARG);
+    }
+    for (int i : d->def_ids)
+        if (pts->type && pts->type[i] != MPM_ELASTIC) {
+            d->err = "differentiable run: elastic points only";
+            return fail(MPM_E_ARG);
+        }
+    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) return fail(MPM_E_HIP);
+    const size_t n = d->n, nd = d->n_dc, nn = d->g.nn, state = d->state();
+    int rc;
+    if ((rc = dalloc_(d, &d->x, 3 * n)) || (rc = dalloc_(d, &d->u, 3 * n)) || (rc = dalloc_(d, &d->v, 3 * n)) ||
+        (rc = dalloc_(d, &d->F, 9 * n)) || (rc = dalloc_(d, &d->s, 6 * n)) || (rc = dalloc_(d, &d->m, n)) ||
+        (rc = dalloc_(d, &d->V0, n)) || (rc = dalloc_(d, &d->E, n)) || (rc = dalloc_(d, &d->nu, n)) ||
+        (rc = dalloc_(d, &d->fext, 3 * n)) || (rc = dalloc_(d, &d->fixed, nn)) || (rc = dalloc_(d, &d->xdc, 3 * nd)) ||
+        (rc = dalloc_(d, &d->vdc, 3 * nd)) || (rc = dalloc_(d, &d->x0dc, 3 * nd)) || (rc = dalloc_(d, &d->v0dc, 3 * nd)) ||
+        (rc = dalloc_(d, &d->v0, 3 * n)) || (rc = dalloc_(d, &d->F0, 9 * n)) || (rc = dalloc_(d, &d->marks, nn)) ||
+        (rc = dalloc_(d, &d->nb, 5 * nn)) || (rc = dalloc_(d, &d->cnt, nn)) || (rc = dalloc_(d, &d->acc, 7 * nn)) ||
+        (rc = dalloc_(d, &d->aacc, 6 * nn)) || (rc = dalloc_(d, &d->nm, nn)) || (rc = dalloc_(d, &d->nP, 3 * nn)) ||
+        (rc = dalloc_(d, &d->nf, 3 * nn)) || (rc = dalloc_(d, &d->nvt, 3 * nn)) || (rc = dalloc_(d, &d->na, 3 * nn)) ||
+        (rc = dalloc_(d, &d->nmb, nn)) || (rc = dalloc_(d, &d->nPb, 3 * nn)) || (rc = dalloc_(d, &d->nfb, 3 * nn)) ||
+        (rc = dalloc_(d, &d->nflag, nn)) || (rc = dalloc_(d, &d->errd, 1)) || (rc = dalloc_(d, &d->xb, 3 * n)) ||
+        (rc = dalloc_(d, &d->vb, 3 * n)) || (rc = dalloc_(d, &d->Fb, 9 * n)) || (rc = dalloc_(d, &d->sb, 6 * n)) ||
+        (rc = dalloc_(d, &d->Eb, n)) || (rc = dalloc_(d, &d->scratch, 12 * n)) ||
+        (rc = dalloc_(d, &d->ck, (size_t)d->n_ckpt * state)) || (rc = dalloc_(d, &d->seg, (size_t)d->ckpt * state)))
+        return fail(rc);
+    // static point data, component-major (blocking copies from pageable memory)
+    std::vector<double> tmp(9 * n);
+    auto up = [&](double *dst, size_t count) { return hipMemcpy(dst, tmp.data(), sizeof(double) * count, hipMemcpyHostToDevice); };
+    for (size_t k = 0; k < n; ++k) tmp[k] = pts->mass[d->def_ids[k]];
+    if (up(d->m, n) != hipSuccess) return fail(MPM_E_HIP);
+    for (size_t k = 0; k < n; ++k) tmp[k] = pts->vol0 ? pts->vol0[d->def_ids[k]] : pts->vol[d->def_ids[k]];
+    if (up(d->V0, n) != hipSuccess) return fail(MPM_E_HIP);
+    for (size_t k = 0; k < n; ++k) tmp[k] = pts->nu[d->def_ids[k]];
+    if (up(d->nu, n) != hipSuccess) return fail(MPM_E_HIP);
+    for (size_t k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) tmp[a * n + k] = pts->v ? pts->v[3 * d->def_ids[k] + a] : 0.0;
+    if (up(d->v0, 3 * n) != hipSuccess) return fail(MPM_E_HIP);
+    for (size_t k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) tmp[a * n + k] = pts->fext ? pts->fext[3 * d->def_ids[k] + a] : 0.0;
+    if (up(d->fext, 3 * n) != hipSuccess) return fail(MPM_E_HIP);
+    std::fill(tmp.begin(), tmp.end(), 0.0);
+    for (size_t k = 0; k < n; ++k) tmp[k] = tmp[4 * n + k] = tmp[8 * n + k] = 1.0;
+    if (up(d->F0, 9 * n) != hipSuccess) return fail(MPM_E_HIP);
+    std::vector<double> dx(3 * nd), dv(3 * nd);
+    for (size_t j = 0; j < nd; ++j)
+        for (int a = 0; a < 3; ++a) {
+            dx[3 * j + a] = pts->x[3 * d->dc_ids[j] + a];
+            dv[3 * j + a] = pts->v ? pts->v[3 * d->dc_ids[j] + a] : 0.0;
+        }
+    if (nd && (hipMemcpy(d->x0dc, dx.data(), sizeof(double) * 3 * nd, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(d->v0dc, dv.data(), sizeof(double) * 3 * nd, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(MPM_E_HIP);
+    std::vector<uint8_t> fx(nn, 0);
+    if (cfg->node_fixed) std::memcpy(fx.data(), cfg->node_fixed, nn);
+    if (hipMemcpy(d->fixed, fx.data(), nn, hipMemcpyHostToDevice) != hipSuccess) return fail(MPM_E_HIP);
+    *out = d;
+    return MPM_OK;
+}
+
+int mpm_diff_forward(mpm_diff *d, const double *E, const double *x0, double *u_final)
+{
+    if (!d || !E || !x0 || !u_final) return MPM_E_ARG;
+    const size_t n = d->n, nd = d->n_dc, state = d->state();
+    d->have_forward = false;
+    DCHK(hipSetDevice(d->device));
+    DCHK(hipStreamSynchronize(d->stream));
+    // parameters and initial state: blocking uploads, then stream-ordered
+    // device copies and clears
+    std::vector<double> hE(n), hx(3 * n);
+    for (size_t k = 0; k < n; ++k) {
+        hE[k] = E[d->def_ids[k]];
+        for (int a = 0; a < 3; ++a) hx[a * n + k] = x0[3 * d->def_ids[k] + a];
+    }
+    DCHK(hipMemcpy(d->E, hE.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    DCHK(hipMemcpy(d->x, hx.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    DCHK(hipMemsetAsync(d->u, 0, sizeof(double) * 3 * n, d->stream));
+    DCHK(hipMemcpyAsync(d->v, d->v0, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, d->stream));
+    DCHK(hipMemcpyAsync(d->F, d->F0, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, d->stream));
+    DCHK(hipMemsetAsync(d->s, 0, sizeof(double) * 6 * n, d->stream));
+    if (nd) {
+        DCHK(hipMemcpyAsync(d->xdc, d->x0dc, sizeof(double) * 3 * nd, hipMemcpyDeviceToDevice, d->stream));
+        DCHK(hipMemcpyAsync(d->vdc, d->v0dc, sizeof(double) * 3 * nd, hipMemcpyDeviceToDevice, d->stream));
+    }
+    DCHK(hipMemsetAsync(d->errd, 0, 4, d->stream));
+    for (int k = 0; k < d->steps; ++k) {
+        int rc;
+        if (k % d->ckpt == 0 && (rc = move_state(d, d->ck + (size_t)(k / d->ckpt) * state, true))) return rc;
+        if ((rc = forward_step(d, k))) return rc;
+    }
+    int32_t err = 0;
+    std::vector<double> pdc(3 * nd), p0(3 * nd);
+    DCHK(hipMemcpyAsync(&err, d->errd, 4, hipMemcpyDeviceToHost, d->stream));
+    DCHK(hipMemcpyAsync(hx.data(), d->u, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, d->stream));
+    if (nd) {
+        DCHK(hipMemcpyAsync(pdc.data(), d->xdc, sizeof(double) * 3 * nd, hipMemcpyDeviceToHost, d->stream));
+        DCHK(hipMemcpyAsync(p0.data(), d->x0dc, sizeof(double) * 3 * nd, hipMemcpyDeviceToHost, d->stream));
+    }
+    DCHK(hipStreamSynchronize(d->stream));
+    if (err) return dfail(d, "a material point left the grid box", MPM_E_OUT_OF_GRID);
+    for (size_t k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) u_final[3 * d->def_ids[k] + a] = hx[a * n + k];
+    for (size_t j = 0; j < nd; ++j)   // platen displacements (no parameters)
+        for (int a = 0; a < 3; ++a) u_final[3 * d->dc_ids[j] + a] = pdc[3 * j + a] - p0[3 * j + a];
+    d->have_forward = true;
+    return MPM_OK;
+}
+
+int mpm_diff_backward(mpm_diff *d, const double *g_ufinal, double *g_E, double *g_x0)
+{
+    if (!d || !g_ufinal || !g_E || !g_x0) return MPM_E_ARG;
+    if (!d->have_forward) return dfail(d, "backward before forward", MPM_E_STATE);
+    const size_t n = d->n, state = d->state();
+    // The running position adjoint starts at dL/du_final: positions and
+    // displacements take the same increments, so with y = x-bar + u-bar the
+    // step adjoint is that of x alone, and dL/dx0 = y_0 - dL/du_final.
+    std::vector<double> g(3 * n);
+    for (size_t k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) g[a * n + k] = g_ufinal[3 * d->def_ids[k] + a];
+    DCHK(hipSetDevice(d->device));
+    DCHK(hipStreamSynchronize(d->stream));
+    DCHK(hipMemcpy(d->xb, g.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    DCHK(hipMemsetAsync(d->vb, 0, sizeof(double) * 3 * n, d->stream));
+    DCHK(hipMemsetAsync(d->Fb, 0, sizeof(double) * 9 * n, d->stream));
+    DCHK(hipMemsetAsync(d->sb, 0, sizeof(double) * 6 * n, d->stream));
+    DCHK(hipMemsetAsync(d->Eb, 0, sizeof(double) * n, d->stream));
+    for (int c = d->n_ckpt - 1; c >= 0; --c) {
+        const int k0 = c * d->ckpt, k1 = std::min(d->steps, k0 + d->ckpt);
+        // replay the segment from its checkpoint, keeping every step's state
+        int rc = move_state(d, d->ck + (size_t)c * state, false);
+        if (rc) return rc;
+        for (int k = k0; k < k1; ++k) {
+            if ((rc = move_state(d, d->seg + (size_t)(k - k0) * state, true))) return rc;
+            if (k + 1 < k1 && (rc = forward_step(d, k))) return rc;
+        }
+        // adjoint steps, last first
+        for (int k = k1 - 1; k >= k0; --k) {
+            if ((rc = move_state(d, d->seg + (size_t)(k - k0) * state, false))) return rc;
+            DArgs a = args_of(d, k, true);
+            if ((rc = forward_nodes(d, a))) return rc;
+            kd_a_g2p<<<gsz(n), 256, 0, d->stream>>>(a);
+            kd_a_scatter<true><<<gsz(n), 256, 0, d->stream>>>(a);
+            kd_a_scatter<false><<<gsz(n), 256, 0, d->stream>>>(a);
+            kd_a_grid<<<gsz(d->g.nn), 256, 0, d->stream>>>(a);
+            kd_a_p2g<<<gsz(n), 256, 0, d->stream>>>(a);
+            DCHK(hipGetLastError());
+        }
+    }
+    std::vector<double> ge(n), gx(3 * n);
+    DCHK(hipMemcpyAsync(ge.data(), d->Eb, sizeof(double) * n, hipMemcpyDeviceToHost, d->stream));
+    DCHK(hipMemcpyAsync(gx.data(), d->xb, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, d->stream));
+    DCHK(hipStreamSynchronize(d->stream));
+    for (int i = 0; i < d->n_total; ++i) g_E[i] = 0.0;
+    for (int i = 0; i < 3 * d->n_total; ++i) g_x0[i] = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+        g_E[d->def_ids[k]] = ge[k];
+        for (int a = 0; a < 3; ++a) g_x0[3 * d->def_ids[k] + a] = gx[a * n + k] - g[a * n + k];
+    }
+    return MPM_OK;
+}
+
+const char *mpm_diff_last_error(const mpm_diff *d) { return d ? d->err.c_str() : "null handle"; }
+
+void mpm_diff_destroy(mpm_diff *d)
+{
+    if (!d) return;
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    free_all(d);
+    delete d;
+}
+
+}  // extern "C"

@@ -512,6 +512,29 @@ def config4_worlds(n=256, seed=20240607, h=0.0002, thickness=0.001, platen_speed
     return worlds
 
 
+def log_uniform_moduli(n, seed, lo=0.5e9, hi=2.0e9):
+    """``n`` Young's moduli log-uniform in [lo, hi] from SplitMix64(seed)."""
+    rnd = splitmix64(seed)
+    u = np.fromiter((next(rnd) for _ in range(n)), dtype=np.float64, count=n)
+    return np.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))
+
+
+def config5_world(seed=20240611, nx=4, ny=4, h=0.00025, thickness=0.00075, steps=400, platen_speed=1.0,
+                  ramp_steps=100, E_range=(0.5e9, 2.0e9)):
+    """an ``nx`` x ``ny`` re-entrant polygon mesh (about
+    160k points at h = 0.25 mm, 0.75 mm thick), elastic, with a per-point
+    Young's modulus log-uniform in ``E_range`` drawn from ``seed``, the time
+    step set by the stiffest modulus, and a platen ramped to ``platen_speed``
+    over ``ramp_steps`` steps."""
+    sc = polygon_mesh_world(nx, ny, h=h, thickness=thickness, material="elastic", E=E_range[1],
+                            platen_speed=platen_speed, steps=steps, ramp_steps=ramp_steps)
+    keep = sc.dc == 0
+    sc.E = sc.E.copy()
+    sc.E[keep] = log_uniform_moduli(int(keep.sum()), seed, *E_range)
+    sc.info.update(name="config5", seed=seed, E_range=E_range)
+    return sc
+
+
 def batch_worlds(worlds, gap_cells=2, row_cells=None):
     """Pack independent worlds into one grid for a single launch per step.
 

@@ -205,6 +205,42 @@ void mpm_destroy(mpm_handle *h);
 int mpm_sort_keys(const uint32_t *keys_host, int32_t n, uint32_t *sorted_host,
                   uint32_t *perm_host, int32_t device);
 
+/* ---- differentiable run (no reference counterpart: the reference has no
+ *      gradients).  n_steps plain steps of cfg->dt_max in FP64 for elastic
+ *      points (…_MP.cpp:106-319 with …Elastic.cpp:4-15, strain increment
+ *      L dt, damping and TimeLine platen as the reference), on a dense node
+ *      grid with deterministic fixed-point node sums; the backward pass is the
+ *      adjoint of every phase, replayed from checkpoints every `checkpoint`
+ *      steps.  Parameters are the per-point Young's modulus and the initial
+ *      point positions; the output is the final displacement u = x - x0,
+ *      carried as its own state so that it keeps full FP64 precision.  Host
+ *      arrays are in the original point order of `pts` (platen points move
+ *      as in the reference and get zero gradients). ------------------------ */
+typedef struct mpm_diff mpm_diff;
+
+/* Copies the scene: masses, vol0 (or vol), nu, initial v, fext, type (must be
+ * MPM_ELASTIC for non-platen points), dc flags, platen positions; node
+ * fixity, damping and TimeLine from cfg.  MPM_E_ARG for other materials. */
+int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_steps,
+                    int32_t checkpoint, mpm_diff **out);
+
+/* Runs the forward from F = I, stress 0, the scene's initial velocities and
+ * x0 (3 per point) with Young's moduli E (1 per point); writes the final
+ * displacements u_final (3 per point) and keeps the checkpoints.
+ * MPM_E_OUT_OF_GRID if a point leaves the grid box. */
+int mpm_diff_forward(mpm_diff *d, const double *E, const double *x0, double *u_final);
+
+/* Given dLoss/du_final (3 per point), writes dLoss/dE (1 per point) and
+ * dLoss/dx0 (3 per point) of the last forward.  MPM_E_STATE before a
+ * forward. */
+int mpm_diff_backward(mpm_diff *d, const double *g_ufinal, double *g_E, double *g_x0);
+
+/* Last error message of the run. */
+const char *mpm_diff_last_error(const mpm_diff *d);
+
+/* Releases the run's device memory. */
+void mpm_diff_destroy(mpm_diff *d);
+
 #ifdef __cplusplus
 }
 #endif
