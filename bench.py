@@ -125,6 +125,38 @@ def batched_run(m, steps, warmup):
                       "simulated_time_s": steps * sc.dt_max + warmup * sc.dt_max}), flush=True)
 
 
+def config5_run(reps=3):
+    """Config 5: the differentiable run on the 4x4 polygon mesh (about 160k
+    points, per-point E log-uniform in [0.5, 2] GPa), 400 FP64 steps with
+    checkpoints every 20.  Times the forward alone and forward + adjoint
+    through the torch.autograd.Function with the effective-Poisson loss
+    (best of ``reps``, wall clock around each call, state on the GPU)."""
+    import torch
+    from code_mpm_auxetics_amd import diff, scene
+    sc = scene.config5_world()
+    sets = diff.boundary_sets(sc)
+    with diff.DiffRun(sc, 400, checkpoint=20) as run:
+        run.forward(sc.E, sc.x)
+        t_f, t_fb = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            u = run.forward(sc.E, sc.x)
+            t_f.append(time.perf_counter() - t0)
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            E = torch.tensor(sc.E, requires_grad=True)
+            x0 = torch.tensor(sc.x, requires_grad=True)
+            loss = diff.poisson_loss(x0, diff.rollout(run, E, x0), sets)
+            loss.backward()
+            t_fb.append(time.perf_counter() - t0)
+    f, fb = min(t_f), min(t_fb)
+    return {"workload": "config5_polygon_mesh_4x4_elastic_adjoint", "n_points": sc.n_deformable, "steps": 400,
+            "checkpoint": 20, "dtype": "f64", "forward_ms": round(f * 1e3, 2),
+            "forward_adjoint_ms": round(fb * 1e3, 2), "adjoint_over_forward": round(fb / f, 3),
+            "forward_particle_steps_per_s": round(sc.n_deformable * 400 / f, 1),
+            "loss": float(loss.detach()), "nu_eff": float(diff.effective_poisson(sc.x, u, sets))}
+
+
 def counters_run(m, sc, warmup, steps):
     """FP32 steps of the bench scene, then one plain device-to-device copy of
     256 MiB (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
@@ -153,8 +185,10 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="config2", choices=["config2", "config4"],
-                    help="config4: the 256-cell batch (~8M points) in one launch per step; prints its own line")
+    ap.add_argument("--workload", default="config2", choices=["config2", "config4", "config5"],
+                    help="config4: the 256-cell batch (~8M points) in one launch per step; config5: the "
+                         "differentiable run alone; each prints its own line")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the config-5 adjoint timing in the main line")
     ap.add_argument("--tune", action="store_true",
                     help="pick brick size and batch length with the autotuner before timing")
     ap.add_argument("--counters", action="store_true",
@@ -168,6 +202,9 @@ def main():
 
     if args.workload == "config4":
         batched_run(m, args.steps, args.warmup)
+        return
+    if args.workload == "config5":
+        print(json.dumps(config5_run()), flush=True)
         return
     sc = build_scene(args.scale, args.speed)
     n = sc.n_deformable
@@ -218,6 +255,10 @@ def main():
         "active_grid_nodes": int(nodes),
         "roofline": roof,
     }
+    if not args.no_adjoint:
+        c5 = config5_run()
+        out["adjoint_over_forward"] = c5["adjoint_over_forward"]
+        out["adjoint"] = c5
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -54,7 +55,7 @@ struct DGrid {
 
 struct DArgs {
     DGrid g;
-    int n, n_dc, clear_adj;
+    int n, n_dc;
     double dt, damping;
     // point state (SoA, component-major: x[a * n + p])
     double *x, *u, *v, *F, *s;
@@ -70,6 +71,8 @@ struct DArgs {
     unsigned long long *nb;        // 5 x nn node bounds (positive doubles as bits):
                                    // m | P | f | vt-bar | a-bar
     unsigned *cnt;                 // per node: number of point terms
+    const int2 *items;             // scatter work items: (first point, count <= 256)
+    double *nsc;                   // 5 x nn fixed-point scales of the node fields
     int32_t *err;
     // adjoint
     double *xb, *vb, *Fb, *sb, *Eb;
@@ -79,19 +82,35 @@ struct DArgs {
 };
 
 // ---------------------------------------------------------------- helpers
-__device__ __forceinline__ void hat(const DGrid &g, const double x[3], int k, const int i0[3], double &N, double G[3],
-                                    double w[3], double gr[3])
+// Linear-hat bases of a point in cell i0 (GridPoint_Mediator.h:64-88), per
+// axis for the two node planes of the cell: s = 1 - |x - x_i| / h (0 outside
+// [0, 1]) and ds/dx = -sign(x - x_i) / h, node position x_i = (i L) / cells.
+// Corner k (x-outer, z-inner) takes N = s_x s_y s_z and G_a = ds_a * the other
+// two s.
+struct Bases {
+    double w[3][2], gr[3][2];
+};
+__device__ __forceinline__ void bases(const DGrid &g, const double x[3], const int i0[3], Bases &B)
 {
-    // corner k (x-outer, z-inner), node position (i L)/n, reference bases
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double xi = (double)(i0[a] + c) * g.L[a] / (double)g.cells[a];
+            const double d = x[a] - xi;
+            double sv = 1.0 - fabs(d) / g.h[a];
+            if (sv < 0.0 || sv > 1.0) sv = 0.0;
+            B.w[a][c] = sv;
+            B.gr[a][c] = -sgn(d) / g.h[a];
+        }
+}
+__device__ __forceinline__ void corner(const Bases &B, int k, double &N, double G[3], double w[3], double gr[3])
+{
+    const int c[3] = {(k >> 2) & 1, (k >> 1) & 1, k & 1};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int ia = i0[a] + ((k >> (2 - a)) & 1);
-        const double xi = (double)ia * g.L[a] / (double)g.cells[a];
-        const double d = x[a] - xi;
-        double sv = 1.0 - fabs(d) / g.h[a];
-        if (sv < 0.0 || sv > 1.0) sv = 0.0;
-        w[a] = sv;
-        gr[a] = -sgn(d) / g.h[a];   // d s_a / d x_a
+        w[a] = B.w[a][c[a]];
+        gr[a] = B.gr[a][c[a]];
     }
     N = w[0] * w[1] * w[2];
     G[0] = gr[0] * w[1] * w[2];
@@ -119,13 +138,9 @@ __device__ __forceinline__ long long nidx(const DGrid &g, const int i0[3], int k
 {
     return (i0[0] + ((k >> 2) & 1)) + (long long)g.nodes[0] * ((i0[1] + ((k >> 1) & 1)) + (long long)g.nodes[1] * (i0[2] + (k & 1)));
 }
-__device__ __forceinline__ void umax(unsigned long long *p, double v)
-{
-    if (v > 0.0) atomicMax(p, (unsigned long long)__double_as_longlong(v));
-}
 // Fixed-point scale of node field q: 2^(62 - H - e) with b < 2^e the node's
 // largest term and 2^H > 2 (count + 1), so that count terms fit in 62 bits.
-__device__ __forceinline__ double node_scale(const DArgs &a, int q, long long I, bool inverse)
+__device__ __forceinline__ double node_scale(const DArgs &a, int q, long long I)
 {
     const double b = __longlong_as_double((long long)a.nb[q * a.g.nn + I]);
     if (!(b > 0.0)) return 0.0;
@@ -134,13 +149,10 @@ __device__ __forceinline__ double node_scale(const DArgs &a, int q, long long I,
     const unsigned c = a.cnt[I];
     int H = 1;
     while ((1ull << H) <= 2ull * (c + 1ull)) ++H;
-    const int sh = 62 - H - e;
-    return inverse ? ldexp(1.0, -sh) : ldexp(1.0, sh);
+    return ldexp(1.0, 62 - H - e);
 }
-__device__ __forceinline__ void fadd(long long *p, double v, double sc)
-{
-    atomicAdd((unsigned long long *)p, (unsigned long long)__double2ll_rn(v * sc));
-}
+// 1 / a power-of-two scale (exact), 0 for an empty field
+__device__ __forceinline__ double inv_scale(double sc) { return sc > 0.0 ? 1.0 / sc : 0.0; }
 // column-major 3x3 (m[3c + r]) products
 __device__ __forceinline__ void matmul(const double A[9], const double B[9], double C[9])
 {
@@ -176,72 +188,193 @@ __device__ __forceinline__ void sigma_dot(const double s[6], const double G[3], 
     o[2] = G[2] * s[2] + G[0] * s[5] + G[1] * s[4];
 }
 
+// ------------------------------------------------------ block node windows
+// The points are kept sorted by the brick of 8 x 8 x 4 cells that holds their
+// starting cell, then in Morton order inside it (fixed at creation; the node
+// sums are exact integers, so the order does not change any result).  A
+// scatter block takes one work item: up to 256 consecutive points of one
+// brick.  They touch a small box of nodes (the brick plus one node layer,
+// more if points drift out of it): the block adds its terms there in LDS and
+// each node's total reaches global memory with one atomic.  Lane t takes
+// point (t % 8) * 32 + t / 8 of the item, so the 8 points of one cell
+// (adjacent in the order) fall on lanes 8 apart instead of hitting the same
+// LDS words in one instruction.  A block whose box exceeds the window adds
+// straight to global memory.
+constexpr int kWinNodes = 640;
+constexpr int kBrick[3] = {8, 8, 4};
+
+struct Win {
+    int lo[3], dim[3], nodes;
+    bool lds;
+};
+
+__device__ __forceinline__ int block_point(const int2 *items, bool &in)
+{
+    const int2 it = items[blockIdx.x];
+    const int l = (threadIdx.x & 7) * 32 + (threadIdx.x >> 3);
+    in = l < it.y;
+    return it.x + l;
+}
+
+__device__ __forceinline__ Win block_window(bool valid, const int i0[3], int *sbox)
+{
+    if (threadIdx.x < 3) {
+        sbox[threadIdx.x] = INT_MAX;
+        sbox[3 + threadIdx.x] = -1;
+    }
+    __syncthreads();
+    if (valid)
+        for (int q = 0; q < 3; ++q) {
+            atomicMin(&sbox[q], i0[q]);
+            atomicMax(&sbox[3 + q], i0[q]);
+        }
+    __syncthreads();
+    Win w{};
+    if (sbox[3] < 0) return w;   // no point of the block in the grid
+    long long nodes = 1;
+    for (int q = 0; q < 3; ++q) {
+        w.lo[q] = sbox[q];
+        w.dim[q] = sbox[3 + q] - sbox[q] + 2;
+        nodes *= w.dim[q];
+    }
+    w.lds = nodes <= kWinNodes;
+    w.nodes = w.lds ? (int)nodes : 0;
+    return w;
+}
+// window slot of corner k of cell i0
+__device__ __forceinline__ int win_slot(const Win &w, const int i0[3], int k)
+{
+    return (i0[0] + ((k >> 2) & 1) - w.lo[0]) +
+           w.dim[0] * ((i0[1] + ((k >> 1) & 1) - w.lo[1]) + w.dim[1] * (i0[2] + (k & 1) - w.lo[2]));
+}
+// global node of window slot j
+__device__ __forceinline__ long long win_node(const DGrid &g, const Win &w, int j)
+{
+    const int jx = j % w.dim[0], r = j / w.dim[0];
+    const int jy = r % w.dim[1], jz = r / w.dim[1];
+    return (w.lo[0] + jx) + (long long)g.nodes[0] * ((w.lo[1] + jy) + (long long)g.nodes[1] * (w.lo[2] + jz));
+}
+__device__ __forceinline__ unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
+__device__ __forceinline__ long long fixp(double v, double sc) { return __double2ll_rn(v * sc); }
+
 // --------------------------------------------------------------- forward
-__global__ void kd_clear(DArgs a)
+__global__ __launch_bounds__(256) void kd_clear(DArgs a)
 {
     const long long I = blockIdx.x * 256ll + threadIdx.x;
     if (I < a.g.nn) {
         for (int q = 0; q < 7; ++q) a.acc[q * a.g.nn + I] = 0;
         for (int q = 0; q < 3; ++q) a.nb[q * a.g.nn + I] = 0ull;
-        if (a.clear_adj) {
-            for (int q = 0; q < 6; ++q) a.aacc[q * a.g.nn + I] = 0;
-            for (int q = 3; q < 5; ++q) a.nb[q * a.g.nn + I] = 0ull;
-        }
         a.cnt[I] = 0u;
         a.marks[I] = 0ull;
+    }
+}
+__global__ __launch_bounds__(256) void kd_a_clear(DArgs a)
+{
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I < a.g.nn) {
+        for (int q = 0; q < 6; ++q) a.aacc[q * a.g.nn + I] = 0;
+        for (int q = 3; q < 5; ++q) a.nb[q * a.g.nn + I] = 0ull;
     }
 }
 
 // P2G of mass, momentum and force (…_MP.cpp:106-202).  BOUNDS: count the
 // terms of every node and take each field's largest term; else add them.
 template <bool BOUNDS>
-__global__ void kd_p2g(DArgs a)
+__global__ __launch_bounds__(256) void kd_p2g(DArgs a)
 {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= a.n) return;
+    constexpr int NF = BOUNDS ? 4 : 7;   // count | max m, P, f  or  m | P xyz | f xyz
+    __shared__ int sbox[6];
+    __shared__ unsigned long long lds[kWinNodes * NF];
     const int n = a.n;
     const long long nn = a.g.nn;
-    const double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
-    int i0[3];
-    if (!cell_of(a.g, x, i0)) {
-        if (BOUNDS) atomicOr(a.err, 1);
-        return;
+    bool valid;
+    const int p = block_point(a.items, valid);
+    double x[3] = {0.0, 0.0, 0.0};
+    int i0[3] = {0, 0, 0};
+    if (valid) {
+        for (int q = 0; q < 3; ++q) x[q] = a.x[q * n + p];
+        valid = cell_of(a.g, x, i0);
+        if (!valid && BOUNDS) atomicOr(a.err, 1);
     }
-    double F[9], s[6], v[3], fe[3];
-    for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
-    for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
-    for (int q = 0; q < 3; ++q) {
-        v[q] = a.v[q * n + p];
-        fe[q] = a.fext[q * n + p];
-    }
-    const double m = a.m[p], V = det3(F) * a.V0[p];
-    for (int k = 0; k < 8; ++k) {
-        double N, G[3], w[3], gr[3], sg[3], tP[3], tf[3];
-        hat(a.g, x, k, i0, N, G, w, gr);
-        const long long I = nidx(a.g, i0, k);
-        sigma_dot(s, G, sg);
-        const double tm = N * m;
+    const Win w = block_window(valid, i0, sbox);
+    for (int j = threadIdx.x; j < w.nodes * NF; j += 256) lds[j] = 0ull;
+    __syncthreads();
+    if (valid) {
+        Bases B;
+        bases(a.g, x, i0, B);
+        double F[9], s[6], v[3], fe[3];
+        for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
+        for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
         for (int q = 0; q < 3; ++q) {
-            tP[q] = N * m * v[q];
-            tf[q] = -V * sg[q] + N * fe[q];
+            v[q] = a.v[q * n + p];
+            fe[q] = a.fext[q * n + p];
         }
-        if (BOUNDS) {
-            atomicAdd(&a.cnt[I], 1u);
-            umax(&a.nb[I], fabs(tm));
-            umax(&a.nb[nn + I], fmax(fabs(tP[0]), fmax(fabs(tP[1]), fabs(tP[2]))));
-            umax(&a.nb[2 * nn + I], fmax(fabs(tf[0]), fmax(fabs(tf[1]), fabs(tf[2]))));
-        } else {
-            fadd(&a.acc[I], tm, node_scale(a, 0, I, false));
-            const double sp = node_scale(a, 1, I, false), sf = node_scale(a, 2, I, false);
+        const double m = a.m[p], V = det3(F) * a.V0[p];
+        for (int k = 0; k < 8; ++k) {
+            double N, G[3], wt[3], gr[3], sg[3], tP[3], tf[3];
+            corner(B, k, N, G, wt, gr);
+            const long long I = nidx(a.g, i0, k);
+            const int j = w.lds ? win_slot(w, i0, k) : 0;
+            sigma_dot(s, G, sg);
+            const double tm = N * m;
             for (int q = 0; q < 3; ++q) {
-                fadd(&a.acc[(1 + q) * nn + I], tP[q], sp);
-                fadd(&a.acc[(4 + q) * nn + I], tf[q], sf);
+                tP[q] = N * m * v[q];
+                tf[q] = -V * sg[q] + N * fe[q];
+            }
+            if (BOUNDS) {
+                const unsigned long long b[3] = {dbits(fabs(tm)),
+                                                 dbits(fmax(fabs(tP[0]), fmax(fabs(tP[1]), fabs(tP[2])))),
+                                                 dbits(fmax(fabs(tf[0]), fmax(fabs(tf[1]), fabs(tf[2]))))};
+                if (w.lds) {
+                    atomicAdd(&lds[j], 1ull);
+                    for (int q = 0; q < 3; ++q) atomicMax(&lds[(1 + q) * w.nodes + j], b[q]);
+                } else {
+                    atomicAdd(&a.cnt[I], 1u);
+                    for (int q = 0; q < 3; ++q) atomicMax(&a.nb[q * nn + I], b[q]);
+                }
+            } else {
+                const double sm = a.nsc[I], sp = a.nsc[nn + I], sf = a.nsc[2 * nn + I];
+                long long iv[7];
+                iv[0] = fixp(tm, sm);
+                for (int q = 0; q < 3; ++q) {
+                    iv[1 + q] = fixp(tP[q], sp);
+                    iv[4 + q] = fixp(tf[q], sf);
+                }
+                for (int q = 0; q < 7; ++q) {
+                    if (w.lds) atomicAdd(&lds[q * w.nodes + j], (unsigned long long)iv[q]);
+                    else atomicAdd((unsigned long long *)&a.acc[q * nn + I], (unsigned long long)iv[q]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < w.nodes; j += 256) {
+        const long long I = win_node(a.g, w, j);
+        if (BOUNDS) {
+            const unsigned long long c = lds[j];
+            if (c) atomicAdd(&a.cnt[I], (unsigned)c);
+            for (int q = 0; q < 3; ++q) {
+                const unsigned long long b = lds[(1 + q) * w.nodes + j];
+                if (b) atomicMax(&a.nb[q * nn + I], b);
+            }
+        } else {
+            for (int q = 0; q < 7; ++q) {
+                const unsigned long long v = lds[q * w.nodes + j];
+                if (v) atomicAdd((unsigned long long *)&a.acc[q * nn + I], v);
             }
         }
     }
 }
 
-__global__ void kd_marks(DArgs a)
+// per node: the scales of fields q0 .. q0 + nq - 1 from their bounds
+__global__ __launch_bounds__(256) void kd_scales(DArgs a, int q0, int nq)
+{
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I >= a.g.nn) return;
+    for (int q = q0; q < q0 + nq; ++q) a.nsc[q * a.g.nn + I] = node_scale(a, q, I);
+}
+
+__global__ __launch_bounds__(256) void kd_marks(DArgs a)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= a.n_dc) return;
@@ -256,12 +389,12 @@ __global__ void kd_marks(DArgs a)
 
 // grid momentum, damping, boundary conditions and platen override
 // (…_MP.cpp:209-265)
-__global__ void kd_grid(DArgs a)
+__global__ __launch_bounds__(256) void kd_grid(DArgs a)
 {
     const long long I = blockIdx.x * 256ll + threadIdx.x;
     if (I >= a.g.nn) return;
     const long long nn = a.g.nn;
-    const double um = node_scale(a, 0, I, true), up = node_scale(a, 1, I, true), uf = node_scale(a, 2, I, true);
+    const double um = inv_scale(a.nsc[I]), up = inv_scale(a.nsc[nn + I]), uf = inv_scale(a.nsc[2 * nn + I]);
     const double m = (double)a.acc[I] * um;
     double P[3], f[3], vh[3], fd[3], vt[3], ac[3];
     for (int q = 0; q < 3; ++q) {
@@ -301,7 +434,7 @@ __global__ void kd_grid(DArgs a)
     a.nflag[I] = flag;
 }
 
-__global__ void kd_g2p(DArgs a)
+__global__ __launch_bounds__(256) void kd_g2p(DArgs a)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= a.n) return;
@@ -310,10 +443,12 @@ __global__ void kd_g2p(DArgs a)
     double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
     int i0[3];
     if (!cell_of(a.g, x, i0)) return;
+    Bases B;
+    bases(a.g, x, i0, B);
     double dx[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = 0; k < 8; ++k) {
         double N, G[3], w[3], gr[3];
-        hat(a.g, x, k, i0, N, G, w, gr);
+        corner(B, k, N, G, w, gr);
         const long long I = nidx(a.g, i0, k);
         for (int q = 0; q < 3; ++q) {
             const double vt = a.nvt[q * nn + I];
@@ -340,7 +475,7 @@ __global__ void kd_g2p(DArgs a)
 }
 
 // platen motion at the end of the step (…_MP.cpp:360-368)
-__global__ void kd_dc(DArgs a)
+__global__ __launch_bounds__(256) void kd_dc(DArgs a)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= a.n_dc) return;
@@ -355,7 +490,7 @@ __global__ void kd_dc(DArgs a)
 // the state after the step; out = adjoints of the state before it (partial
 // in x: the P2G terms are added by kd_a_p2g).  Writes the node scatter
 // inputs.
-__global__ void kd_a_g2p(DArgs a)
+__global__ __launch_bounds__(256) void kd_a_g2p(DArgs a)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= a.n) return;
@@ -365,10 +500,12 @@ __global__ void kd_a_g2p(DArgs a)
     double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
     int i0[3];
     if (!cell_of(a.g, x, i0)) return;
+    Bases B;
+    bases(a.g, x, i0, B);
     double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = 0; k < 8; ++k) {
         double N, G[3], w[3], gr[3];
-        hat(a.g, x, k, i0, N, G, w, gr);
+        corner(B, k, N, G, w, gr);
         const long long I = nidx(a.g, i0, k);
         for (int q = 0; q < 3; ++q)
             for (int c = 0; c < 3; ++c) L[3 * c + q] += a.nvt[q * nn + I] * G[c];
@@ -407,7 +544,7 @@ __global__ void kd_a_g2p(DArgs a)
     double xn[3] = {xb[0], xb[1], xb[2]};
     for (int k = 0; k < 8; ++k) {
         double N, G[3], w[3], gr[3];
-        hat(a.g, x, k, i0, N, G, w, gr);
+        corner(B, k, N, G, w, gr);
         const long long I = nidx(a.g, i0, k);
         double vt[3], ac[3];
         for (int q = 0; q < 3; ++q) { vt[q] = a.nvt[q * nn + I]; ac[q] = a.na[q * nn + I]; }
@@ -435,46 +572,79 @@ __global__ void kd_a_g2p(DArgs a)
 // (BOUNDS: each node's largest term, as in kd_p2g; the term counts are the
 // forward's, same points and cells)
 template <bool BOUNDS>
-__global__ void kd_a_scatter(DArgs a)
+__global__ __launch_bounds__(256) void kd_a_scatter(DArgs a)
 {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= a.n) return;
+    constexpr int NF = BOUNDS ? 2 : 6;   // max vt-bar, a-bar  or  vt-bar xyz | a-bar xyz
+    __shared__ int sbox[6];
+    __shared__ unsigned long long lds[kWinNodes * NF];
     const int n = a.n;
     const long long nn = a.g.nn;
-    double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
-    int i0[3];
-    if (!cell_of(a.g, x, i0)) return;
-    const double *sc = a.scratch + 12 * (size_t)p;
-    double vb[3];
-    for (int q = 0; q < 3; ++q) vb[q] = a.vb[q * n + p] * a.dt;
-    for (int k = 0; k < 8; ++k) {
-        double N, G[3], w[3], gr[3], t[3], ta[3];
-        hat(a.g, x, k, i0, N, G, w, gr);
-        const long long I = nidx(a.g, i0, k);
-        for (int r = 0; r < 3; ++r) {
-            t[r] = N * sc[9 + r] + sc[r] * G[0] + sc[3 + r] * G[1] + sc[6 + r] * G[2];
-            ta[r] = N * vb[r];
-        }
-        if (BOUNDS) {
-            umax(&a.nb[3 * nn + I], fmax(fabs(t[0]), fmax(fabs(t[1]), fabs(t[2]))));
-            umax(&a.nb[4 * nn + I], fmax(fabs(ta[0]), fmax(fabs(ta[1]), fabs(ta[2]))));
-        } else {
-            const double s3 = node_scale(a, 3, I, false), s4 = node_scale(a, 4, I, false);
+    bool valid;
+    const int p = block_point(a.items, valid);
+    double x[3] = {0.0, 0.0, 0.0};
+    int i0[3] = {0, 0, 0};
+    if (valid) {
+        for (int q = 0; q < 3; ++q) x[q] = a.x[q * n + p];
+        valid = cell_of(a.g, x, i0);
+    }
+    const Win w = block_window(valid, i0, sbox);
+    for (int j = threadIdx.x; j < w.nodes * NF; j += 256) lds[j] = 0ull;
+    __syncthreads();
+    if (valid) {
+        Bases B;
+        bases(a.g, x, i0, B);
+        const double *sc = a.scratch + 12 * (size_t)p;
+        double vb[3];
+        for (int q = 0; q < 3; ++q) vb[q] = a.vb[q * n + p] * a.dt;
+        for (int k = 0; k < 8; ++k) {
+            double N, G[3], wt[3], gr[3], t[3], ta[3];
+            corner(B, k, N, G, wt, gr);
+            const long long I = nidx(a.g, i0, k);
+            const int j = w.lds ? win_slot(w, i0, k) : 0;
             for (int r = 0; r < 3; ++r) {
-                fadd(&a.aacc[r * nn + I], t[r], s3);
-                fadd(&a.aacc[(3 + r) * nn + I], ta[r], s4);
+                t[r] = N * sc[9 + r] + sc[r] * G[0] + sc[3 + r] * G[1] + sc[6 + r] * G[2];
+                ta[r] = N * vb[r];
             }
+            if (BOUNDS) {
+                const unsigned long long b[2] = {dbits(fmax(fabs(t[0]), fmax(fabs(t[1]), fabs(t[2])))),
+                                                 dbits(fmax(fabs(ta[0]), fmax(fabs(ta[1]), fabs(ta[2]))))};
+                for (int q = 0; q < 2; ++q) {
+                    if (w.lds) atomicMax(&lds[q * w.nodes + j], b[q]);
+                    else atomicMax(&a.nb[(3 + q) * nn + I], b[q]);
+                }
+            } else {
+                const double s3 = a.nsc[3 * nn + I], s4 = a.nsc[4 * nn + I];
+                long long iv[6];
+                for (int r = 0; r < 3; ++r) {
+                    iv[r] = fixp(t[r], s3);
+                    iv[3 + r] = fixp(ta[r], s4);
+                }
+                for (int q = 0; q < 6; ++q) {
+                    if (w.lds) atomicAdd(&lds[q * w.nodes + j], (unsigned long long)iv[q]);
+                    else atomicAdd((unsigned long long *)&a.aacc[q * nn + I], (unsigned long long)iv[q]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < w.nodes; j += 256) {
+        const long long I = win_node(a.g, w, j);
+        for (int q = 0; q < NF; ++q) {
+            const unsigned long long v = lds[q * w.nodes + j];
+            if (!v) continue;
+            if (BOUNDS) atomicMax(&a.nb[(3 + q) * nn + I], v);
+            else atomicAdd((unsigned long long *)&a.aacc[q * nn + I], v);
         }
     }
 }
 
 // grid update reversed, per node: (vt-bar, a-bar) -> (m-bar, P-bar, f-bar)
-__global__ void kd_a_grid(DArgs a)
+__global__ __launch_bounds__(256) void kd_a_grid(DArgs a)
 {
     const long long I = blockIdx.x * 256ll + threadIdx.x;
     if (I >= a.g.nn) return;
     const long long nn = a.g.nn;
-    const double u3 = node_scale(a, 3, I, true), u4 = node_scale(a, 4, I, true);
+    const double u3 = inv_scale(a.nsc[3 * nn + I]), u4 = inv_scale(a.nsc[4 * nn + I]);
     double vtb[3], ab[3], P[3], f[3];
     for (int q = 0; q < 3; ++q) {
         vtb[q] = (double)a.aacc[q * nn + I] * u3;
@@ -542,7 +712,7 @@ __global__ void kd_a_grid(DArgs a)
 
 // P2G reversed, per point (gather): v-bar, sigma-bar, F-bar (through V) and
 // the P2G part of x-bar
-__global__ void kd_a_p2g(DArgs a)
+__global__ __launch_bounds__(256) void kd_a_p2g(DArgs a)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= a.n) return;
@@ -551,6 +721,8 @@ __global__ void kd_a_p2g(DArgs a)
     double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
     int i0[3];
     if (!cell_of(a.g, x, i0)) return;
+    Bases B;
+    bases(a.g, x, i0, B);
     double F[9], s[6], v[3], fe[3];
     for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
     for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
@@ -562,7 +734,7 @@ __global__ void kd_a_p2g(DArgs a)
     double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, sb[6] = {0, 0, 0, 0, 0, 0}, Vb = 0.0;
     for (int k = 0; k < 8; ++k) {
         double N, G[3], w[3], gr[3];
-        hat(a.g, x, k, i0, N, G, w, gr);
+        corner(B, k, N, G, w, gr);
         const long long I = nidx(a.g, i0, k);
         const double mb = a.nmb[I];
         double Pb[3], fb[3];
@@ -616,19 +788,24 @@ struct mpm_diff {
     double dt = 0.0, damping = 0.0;
     std::vector<double> tl_t, tl_v, times;   // times[k]: start time of step k
     std::vector<int> def_ids, dc_ids;
-    // device
-    double *x = nullptr, *u = nullptr, *v = nullptr, *F = nullptr, *s = nullptr, *m = nullptr, *V0 = nullptr,
-           *E = nullptr, *nu = nullptr, *fext = nullptr;
+    // device; the state arrays x, u, v, F, s, xdc, vdc are slices of st
+    double *st = nullptr, *x = nullptr, *u = nullptr, *v = nullptr, *F = nullptr, *s = nullptr;
+    double *m = nullptr, *V0 = nullptr, *E = nullptr, *nu = nullptr, *fext = nullptr;
     uint8_t *fixed = nullptr;
     double *xdc = nullptr, *vdc = nullptr;
     double *x0dc = nullptr, *v0dc = nullptr;   // platen state at step 0
     double *v0 = nullptr, *F0 = nullptr;       // initial point velocities, identity F
     unsigned long long *marks = nullptr, *nb = nullptr;
-    unsigned *cnt = nullptr;
+    double *nsc = nullptr;
     long long *acc = nullptr, *aacc = nullptr;
-    double *nm = nullptr, *nP = nullptr, *nf = nullptr, *nvt = nullptr, *na = nullptr;
     double *nmb = nullptr, *nPb = nullptr, *nfb = nullptr;
-    uint8_t *nflag = nullptr;
+    // node quantities of a step (m, P, f, vt, a, term counts, flags) in slots
+    // of node_bytes: slot 0 in the forward, slot k - k0 for step k of the
+    // segment being replayed, which the adjoint of step k then reads
+    uint8_t *nodebuf = nullptr;
+    size_t node_bytes = 0;
+    int2 *items = nullptr;
+    int n_items = 0;
     int32_t *errd = nullptr;
     double *xb = nullptr, *vb = nullptr, *Fb = nullptr, *sb = nullptr, *Eb = nullptr, *scratch = nullptr;
     double *ck = nullptr;      // checkpoints: n_ckpt x state
@@ -659,13 +836,13 @@ int dalloc_(mpm_diff *d, T **p, size_t n)
 }
 inline unsigned gsz(size_t n) { return (unsigned)((n + 255) / 256); }
 
-DArgs args_of(mpm_diff *d, int step, bool adjoint)
+DArgs args_of(mpm_diff *d, int step, int slot)
 {
     DArgs a{};
     a.g = d->g;
     a.n = d->n;
     a.n_dc = d->n_dc;
-    a.clear_adj = adjoint ? 1 : 0;
+    a.items = d->items;
     a.dt = d->dt;
     a.damping = d->damping;
     a.fext = d->fext;
@@ -684,8 +861,12 @@ DArgs args_of(mpm_diff *d, int step, bool adjoint)
         }
     a.has_tl = nt > 0;
     a.marks = d->marks; a.acc = d->acc;
-    a.nm = d->nm; a.nP = d->nP; a.nf = d->nf; a.nvt = d->nvt; a.na = d->na; a.nflag = d->nflag;
-    a.nb = d->nb; a.cnt = d->cnt; a.err = d->errd;
+    const size_t nn = d->g.nn;
+    double *nd = (double *)(d->nodebuf + (size_t)slot * d->node_bytes);
+    a.nm = nd; a.nP = nd + nn; a.nf = nd + 4 * nn; a.nvt = nd + 7 * nn; a.na = nd + 10 * nn;
+    a.cnt = (unsigned *)(nd + 13 * nn);
+    a.nflag = (uint8_t *)(a.cnt + nn);
+    a.nb = d->nb; a.nsc = d->nsc; a.err = d->errd;
     a.xb = d->xb; a.vb = d->vb; a.Fb = d->Fb; a.sb = d->sb; a.Eb = d->Eb; a.scratch = d->scratch;
     a.aacc = d->aacc; a.nmb = d->nmb; a.nPb = d->nPb; a.nfb = d->nfb;
     return a;
@@ -697,8 +878,9 @@ int forward_nodes(mpm_diff *d, const DArgs &a)
     const long long nn = d->g.nn;
     kd_clear<<<gsz(nn), 256, 0, d->stream>>>(a);
     if (d->n > 0) {
-        kd_p2g<true><<<gsz(d->n), 256, 0, d->stream>>>(a);
-        kd_p2g<false><<<gsz(d->n), 256, 0, d->stream>>>(a);
+        kd_p2g<true><<<d->n_items, 256, 0, d->stream>>>(a);
+        kd_scales<<<gsz(nn), 256, 0, d->stream>>>(a, 0, 3);
+        kd_p2g<false><<<d->n_items, 256, 0, d->stream>>>(a);
     }
     if (d->n_dc > 0) kd_marks<<<gsz(d->n_dc), 256, 0, d->stream>>>(a);
     kd_grid<<<gsz(nn), 256, 0, d->stream>>>(a);
@@ -706,41 +888,47 @@ int forward_nodes(mpm_diff *d, const DArgs &a)
     return MPM_OK;
 }
 
-int forward_step(mpm_diff *d, int step)
+// one step: node quantities into node slot `slot`, then (advance) G2P and
+// the platen motion
+int forward_step(mpm_diff *d, int step, int slot, bool advance)
 {
-    DArgs a = args_of(d, step, false);
+    DArgs a = args_of(d, step, slot);
     int rc = forward_nodes(d, a);
-    if (rc) return rc;
-    if (d->n > 0) kd_g2p<<<gsz(d->n), 256, 0, d->stream>>>(a);
+    if (rc || !advance) return rc;
+    kd_g2p<<<gsz(d->n), 256, 0, d->stream>>>(a);
     if (d->n_dc > 0) kd_dc<<<gsz(d->n_dc), 256, 0, d->stream>>>(a);
     DCHK(hipGetLastError());
     return MPM_OK;
 }
 
-// copy the state (x, u, v, F, sigma, platen x and v) to dst (save) or from
-// src (load), stream-ordered
+// copy the state (x, u, v, F, sigma, platen x and v: one contiguous block)
+// to buf (save) or from it (load), stream-ordered
 int move_state(mpm_diff *d, double *buf, bool save)
 {
-    const size_t n = d->n, nd = d->n_dc;
-    double *parts[7] = {d->x, d->u, d->v, d->F, d->s, d->xdc, d->vdc};
-    const size_t len[7] = {3 * n, 3 * n, 3 * n, 9 * n, 6 * n, 3 * nd, 3 * nd};
-    size_t off = 0;
-    for (int q = 0; q < 7; ++q) {
-        if (len[q]) {
-            double *dst = save ? buf + off : parts[q];
-            const double *src = save ? parts[q] : buf + off;
-            DCHK(hipMemcpyAsync(dst, src, sizeof(double) * len[q], hipMemcpyDeviceToDevice, d->stream));
-        }
-        off += len[q];
-    }
+    DCHK(hipMemcpyAsync(save ? buf : d->st, save ? d->st : buf, sizeof(double) * d->state(), hipMemcpyDeviceToDevice,
+                        d->stream));
     return MPM_OK;
+}
+
+// 3D Morton key of a cell (21 bits per axis)
+uint64_t morton3(uint32_t x, uint32_t y, uint32_t z)
+{
+    auto spread = [](uint64_t v) {
+        v &= 0x1FFFFFull;
+        v = (v | v << 32) & 0x1F00000000FFFFull;
+        v = (v | v << 16) & 0x1F0000FF0000FFull;
+        v = (v | v << 8) & 0x100F00F00F00F00Full;
+        v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+        v = (v | v << 2) & 0x1249249249249249ull;
+        return v;
+    };
+    return spread(x) | spread(y) << 1 | spread(z) << 2;
 }
 
 void free_all(mpm_diff *d)
 {
-    void *ptrs[] = {d->x, d->u, d->v, d->F, d->s, d->m, d->V0, d->E, d->nu, d->fext, d->fixed, d->xdc, d->vdc,
-                    d->x0dc, d->v0dc, d->v0, d->F0, d->marks, d->nb, d->cnt, d->acc, d->aacc, d->nm, d->nP, d->nf,
-                    d->nvt, d->na, d->nmb, d->nPb, d->nfb, d->nflag, d->errd, d->xb, d->vb, d->Fb, d->sb, d->Eb,
+    void *ptrs[] = {d->st, d->m, d->V0, d->E, d->nu, d->fext, d->fixed, d->x0dc, d->v0dc, d->v0, d->F0, d->marks, d->nb, d->nsc, d->acc, d->aacc, d->nodebuf, d->items,
+                    d->nmb, d->nPb, d->nfb, d->errd, d->xb, d->vb, d->Fb, d->sb, d->Eb,
                     d->scratch, d->ck, d->seg};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -804,24 +992,54 @@ int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_step
             d->err = "differentiable run: elastic points only";
             return fail(MPM_E_ARG);
         }
+    // device order: by brick of the starting cell, then Morton order of the
+    // cell inside it (ties: input order); scatter work items of at most 256
+    // points of one brick
+    std::vector<int2> items;
+    {
+        std::vector<uint64_t> bkey(pts->n, 0), ckey(pts->n, 0);
+        for (int i : d->def_ids) {
+            uint32_t c[3];
+            for (int a = 0; a < 3; ++a) {
+                const double f = pts->x[3 * i + a] / d->g.h[a];
+                c[a] = f > 0.0 ? (uint32_t)std::min<double>(f, (double)(d->g.cells[a] - 1)) : 0u;
+            }
+            bkey[i] = morton3(c[0] / kBrick[0], c[1] / kBrick[1], c[2] / kBrick[2]);
+            ckey[i] = morton3(c[0] % kBrick[0], c[1] % kBrick[1], c[2] % kBrick[2]);
+        }
+        std::stable_sort(d->def_ids.begin(), d->def_ids.end(), [&](int p, int q) {
+            return bkey[p] != bkey[q] ? bkey[p] < bkey[q] : ckey[p] < ckey[q];
+        });
+        for (int k = 0; k < d->n; ++k) {
+            const bool same = !items.empty() && bkey[d->def_ids[k]] == bkey[d->def_ids[k - 1]] && items.back().y < 256;
+            if (same) ++items.back().y;
+            else items.push_back(make_int2(k, 1));
+        }
+        d->n_items = (int)items.size();
+    }
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) return fail(MPM_E_HIP);
     const size_t n = d->n, nd = d->n_dc, nn = d->g.nn, state = d->state();
     int rc;
-    if ((rc = dalloc_(d, &d->x, 3 * n)) || (rc = dalloc_(d, &d->u, 3 * n)) || (rc = dalloc_(d, &d->v, 3 * n)) ||
-        (rc = dalloc_(d, &d->F, 9 * n)) || (rc = dalloc_(d, &d->s, 6 * n)) || (rc = dalloc_(d, &d->m, n)) ||
-        (rc = dalloc_(d, &d->V0, n)) || (rc = dalloc_(d, &d->E, n)) || (rc = dalloc_(d, &d->nu, n)) ||
-        (rc = dalloc_(d, &d->fext, 3 * n)) || (rc = dalloc_(d, &d->fixed, nn)) || (rc = dalloc_(d, &d->xdc, 3 * nd)) ||
-        (rc = dalloc_(d, &d->vdc, 3 * nd)) || (rc = dalloc_(d, &d->x0dc, 3 * nd)) || (rc = dalloc_(d, &d->v0dc, 3 * nd)) ||
+    // node slot: m, P, f, vt, a (13 doubles), term count (u32), flags (u8)
+    d->node_bytes = ((13 * 8 + 4 + 1) * nn + 255) / 256 * 256;
+    if ((rc = dalloc_(d, &d->st, state)) || (rc = dalloc_(d, &d->m, n)) || (rc = dalloc_(d, &d->V0, n)) || (rc = dalloc_(d, &d->E, n)) || (rc = dalloc_(d, &d->nu, n)) ||
+        (rc = dalloc_(d, &d->fext, 3 * n)) || (rc = dalloc_(d, &d->fixed, nn)) || (rc = dalloc_(d, &d->x0dc, 3 * nd)) || (rc = dalloc_(d, &d->v0dc, 3 * nd)) ||
         (rc = dalloc_(d, &d->v0, 3 * n)) || (rc = dalloc_(d, &d->F0, 9 * n)) || (rc = dalloc_(d, &d->marks, nn)) ||
-        (rc = dalloc_(d, &d->nb, 5 * nn)) || (rc = dalloc_(d, &d->cnt, nn)) || (rc = dalloc_(d, &d->acc, 7 * nn)) ||
-        (rc = dalloc_(d, &d->aacc, 6 * nn)) || (rc = dalloc_(d, &d->nm, nn)) || (rc = dalloc_(d, &d->nP, 3 * nn)) ||
-        (rc = dalloc_(d, &d->nf, 3 * nn)) || (rc = dalloc_(d, &d->nvt, 3 * nn)) || (rc = dalloc_(d, &d->na, 3 * nn)) ||
-        (rc = dalloc_(d, &d->nmb, nn)) || (rc = dalloc_(d, &d->nPb, 3 * nn)) || (rc = dalloc_(d, &d->nfb, 3 * nn)) ||
-        (rc = dalloc_(d, &d->nflag, nn)) || (rc = dalloc_(d, &d->errd, 1)) || (rc = dalloc_(d, &d->xb, 3 * n)) ||
+        (rc = dalloc_(d, &d->nb, 5 * nn)) || (rc = dalloc_(d, &d->nsc, 5 * nn)) || (rc = dalloc_(d, &d->acc, 7 * nn)) ||
+        (rc = dalloc_(d, &d->aacc, 6 * nn)) || (rc = dalloc_(d, &d->nodebuf, (size_t)d->ckpt * d->node_bytes)) ||
+        (rc = dalloc_(d, &d->items, items.size())) || (rc = dalloc_(d, &d->nmb, nn)) || (rc = dalloc_(d, &d->nPb, 3 * nn)) || (rc = dalloc_(d, &d->nfb, 3 * nn)) ||
+        (rc = dalloc_(d, &d->errd, 1)) || (rc = dalloc_(d, &d->xb, 3 * n)) ||
         (rc = dalloc_(d, &d->vb, 3 * n)) || (rc = dalloc_(d, &d->Fb, 9 * n)) || (rc = dalloc_(d, &d->sb, 6 * n)) ||
         (rc = dalloc_(d, &d->Eb, n)) || (rc = dalloc_(d, &d->scratch, 12 * n)) ||
         (rc = dalloc_(d, &d->ck, (size_t)d->n_ckpt * state)) || (rc = dalloc_(d, &d->seg, (size_t)d->ckpt * state)))
         return fail(rc);
+    d->x = d->st;
+    d->u = d->x + 3 * n;
+    d->v = d->u + 3 * n;
+    d->F = d->v + 3 * n;
+    d->s = d->F + 9 * n;
+    d->xdc = d->s + 6 * n;
+    d->vdc = d->xdc + 3 * nd;
     // static point data, component-major (blocking copies from pageable memory)
     std::vector<double> tmp(9 * n);
     auto up = [&](double *dst, size_t count) { return hipMemcpy(dst, tmp.data(), sizeof(double) * count, hipMemcpyHostToDevice); };
@@ -848,6 +1066,8 @@ int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_step
         }
     if (nd && (hipMemcpy(d->x0dc, dx.data(), sizeof(double) * 3 * nd, hipMemcpyHostToDevice) != hipSuccess ||
                hipMemcpy(d->v0dc, dv.data(), sizeof(double) * 3 * nd, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(MPM_E_HIP);
+    if (hipMemcpy(d->items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
         return fail(MPM_E_HIP);
     std::vector<uint8_t> fx(nn, 0);
     if (cfg->node_fixed) std::memcpy(fx.data(), cfg->node_fixed, nn);
@@ -884,7 +1104,7 @@ int mpm_diff_forward(mpm_diff *d, const double *E, const double *x0, double *u_f
     for (int k = 0; k < d->steps; ++k) {
         int rc;
         if (k % d->ckpt == 0 && (rc = move_state(d, d->ck + (size_t)(k / d->ckpt) * state, true))) return rc;
-        if ((rc = forward_step(d, k))) return rc;
+        if ((rc = forward_step(d, k, 0, true))) return rc;
     }
     int32_t err = 0;
     std::vector<double> pdc(3 * nd), p0(3 * nd);
@@ -929,16 +1149,17 @@ int mpm_diff_backward(mpm_diff *d, const double *g_ufinal, double *g_E, double *
         if (rc) return rc;
         for (int k = k0; k < k1; ++k) {
             if ((rc = move_state(d, d->seg + (size_t)(k - k0) * state, true))) return rc;
-            if (k + 1 < k1 && (rc = forward_step(d, k))) return rc;
+            if ((rc = forward_step(d, k, k - k0, k + 1 < k1))) return rc;
         }
-        // adjoint steps, last first
+        // adjoint steps, last first, on the replayed node quantities
         for (int k = k1 - 1; k >= k0; --k) {
             if ((rc = move_state(d, d->seg + (size_t)(k - k0) * state, false))) return rc;
-            DArgs a = args_of(d, k, true);
-            if ((rc = forward_nodes(d, a))) return rc;
+            DArgs a = args_of(d, k, k - k0);
+            kd_a_clear<<<gsz(d->g.nn), 256, 0, d->stream>>>(a);
             kd_a_g2p<<<gsz(n), 256, 0, d->stream>>>(a);
-            kd_a_scatter<true><<<gsz(n), 256, 0, d->stream>>>(a);
-            kd_a_scatter<false><<<gsz(n), 256, 0, d->stream>>>(a);
+            kd_a_scatter<true><<<d->n_items, 256, 0, d->stream>>>(a);
+            kd_scales<<<gsz(d->g.nn), 256, 0, d->stream>>>(a, 3, 2);
+            kd_a_scatter<false><<<d->n_items, 256, 0, d->stream>>>(a);
             kd_a_grid<<<gsz(d->g.nn), 256, 0, d->stream>>>(a);
             kd_a_p2g<<<gsz(n), 256, 0, d->stream>>>(a);
             DCHK(hipGetLastError());
