@@ -40,6 +40,7 @@ EXPORTED = [
     "mpm_checks", "mpm_reaction_history", "mpm_get_time", "mpm_last_step_ms",
     "mpm_last_error", "mpm_destroy", "mpm_sort_keys", "mpm_kernel_times", "mpm_group_reaction_history",
     "mpm_diff_create", "mpm_diff_forward", "mpm_diff_backward", "mpm_diff_last_error", "mpm_diff_destroy",
+    "mpm_report", "mpm_set_log",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -118,6 +119,8 @@ def lib():
     L.mpm_destroy.restype = None
     L.mpm_sort_keys.argtypes = [_u32p, C.c_int32, _u32p, _u32p, C.c_int32]
     L.mpm_kernel_times.argtypes = [C.c_void_p, _dp]
+    L.mpm_report.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32, _i32p]
+    L.mpm_set_log.argtypes = [C.c_void_p, C.c_char_p]
     L.mpm_diff_create.argtypes = [C.POINTER(MpmConfig), C.POINTER(MpmPoints), C.c_int32, C.c_int32,
                                   C.POINTER(C.c_void_p)]
     L.mpm_diff_forward.argtypes = [C.c_void_p, _dp, _dp, _dp]

@@ -142,9 +142,11 @@ struct mpm_handle {
 
     double time = 0.0;
     long long cycle = 0;
-    double console_last = -1.0e12;
+    double console_last = 0.0;
     long long rebins = 0;
     double last_ms = 0.0;
+    double runtime_s = 0.0;      // device time of all mpm_step calls (d_Runtime_Total)
+    std::string log_path;        // report lines are appended here when set
     // profiling (cfg.profile): event pairs around every launch
     std::vector<hipEvent_t> pev;
     double kt[6] = {0, 0, 0, 0, 0, 0};
@@ -651,12 +653,60 @@ static int check_flags(mpm_handle *h, bool *need_rebin, bool copied = false)
 
 static int checks_impl(mpm_handle *h, double out[12]);
 
-static void report_console(mpm_handle *h)
+// Script(double, p) of Definitions.h:43-56: scientific, p digits, explicit sign
+static std::string script(double v, int p)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "%+.*e", p, v);
+    return b;
+}
+
+// reportConsole (PhysicsEngine.cpp:33-120): the description, then time,
+// runtime, the displacement monitor (the first platen point, as the worlds
+// mark it: …PolygonCell.cpp:189), the reaction Force_y (the sum of the last
+// step's Force_Temp) and sum m|v|^2 over all points.  The stress monitor is
+// empty in the reference worlds (…PolygonCell.cpp:230 is commented out), so
+// its fields are not printed.
+static int report_line(mpm_handle *h, const char *desc, double runtime_s, std::string &out)
 {
     double c[12];
-    if (checks_impl(h, c) != MPM_OK) return;
-    std::printf("\ttime: %+.6e\tRuntime: %+.3e\tForce_y: %+.6e\tKinetic Energy: %+.6e\n", h->time,
-                h->last_ms * 1e-3, c[1], c[2]);
+    int rc = checks_impl(h, c);
+    if (rc) return rc;
+    out = desc ? desc : "";
+    out += "\ttime: " + script(h->time, 6);
+    out += "\tRuntime: " + script(runtime_s, 3);
+    if (h->n_dc > 0) {
+        double y = 0.0;
+        HIPCHK(h, hipMemcpyAsync(&y, h->dc_x + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        out += "\tPosition_y: " + script(y, 6);
+    }
+    out += "\tForce_y: " + script(c[1], 6);
+    out += "\tKinetic Energy: " + script(c[2], 6);
+    out += "\n";
+    return MPM_OK;
+}
+
+// the console (cfg.verbose) and the log file (_STR_LOGFILE, Definitions.h:22)
+static void report_emit(mpm_handle *h, const std::string &line)
+{
+    if (h->cfg.verbose) {
+        std::fputs(line.c_str(), stdout);
+        std::fflush(stdout);
+    }
+    if (!h->log_path.empty()) {
+        if (FILE *f = std::fopen(h->log_path.c_str(), "a")) {
+            std::fputs(line.c_str(), f);
+            std::fclose(f);
+        }
+    }
+}
+
+static void report_console(mpm_handle *h, double runtime_s)
+{
+    if (!h->cfg.verbose && h->log_path.empty()) return;
+    std::string line;
+    if (report_line(h, nullptr, runtime_s, line) == MPM_OK) report_emit(h, line);
 }
 
 // batched worlds: clear this batch's y-fixed slots (uncovered nodes stay 0)
@@ -853,11 +903,21 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
         timeline_velocity(h, h->time, r.vtl);
         r.epoch = (uint32_t)h->cycle;
         ++in_batch;
+        // reportConsole every console_interval (…_MP.cpp:374-379); the batch is
+        // cut there only when the line goes somewhere
         bool report = h->cfg.console_interval > 0.0 && (h->time - h->console_last > h->cfg.console_interval);
+        if (report && !h->cfg.verbose && h->log_path.empty()) {
+            h->console_last = h->time;
+            report = false;
+        }
         if (in_batch == K || report) RCHK(flush(true));
         if (report) {
             h->console_last = h->time;
-            if (h->cfg.verbose) report_console(h);
+            float ms_now = 0.f;
+            HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+            HIPCHK(h, hipEventSynchronize(h->ev1));
+            HIPCHK(h, hipEventElapsedTime(&ms_now, h->ev0, h->ev1));
+            report_console(h, h->runtime_s + ms_now * 1e-3);
         }
         h->time += dt;
         h->cycle++;
@@ -868,11 +928,12 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
+    h->runtime_s += ms * 1e-3;
     if (finished) *finished = (h->time < h->cfg.time_end) ? 0 : 1;
-    if (finished && *finished && h->cfg.verbose) {
+    if (finished && *finished) {
         h->console_last = h->time;
-        report_console(h);
-        std::printf("Analysis complete ...\n");
+        report_console(h, h->runtime_s);
+        if (h->cfg.verbose) std::printf("Analysis complete ...\n");
     }
     return MPM_OK;
 }
@@ -991,7 +1052,8 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     h->cfg.timeline_v = nullptr;
     h->cfg.node_fixed = nullptr;
     h->time = cfg->time_start;
-    h->console_last = -1.0e12;
+    // d_TimeConsole_Last = 0 after the world's first report (…PolygonCell.cpp:311)
+    h->console_last = cfg->time_start;
     auto fail = [&](int code) {
         g_create_error = h->err;
         destroy_all(h);
@@ -1433,6 +1495,34 @@ int mpm_last_step_ms(mpm_handle *h, double *ms)
 {
     if (!h || !ms) return MPM_E_ARG;
     *ms = h->last_ms;
+    return MPM_OK;
+}
+
+int mpm_set_log(mpm_handle *h, const char *path)
+{
+    if (!h) return MPM_E_ARG;
+    h->log_path = path ? path : "";
+    return MPM_OK;
+}
+
+int mpm_report(mpm_handle *h, const char *description, char *buf, int32_t cap, int32_t *len)
+{
+    if (!h || (buf && cap < 1)) return MPM_E_ARG;
+    if (h->n_total == 0) {
+        h->err = "report before upload";
+        return MPM_E_STATE;
+    }
+    HIPCHK(h, hipSetDevice(h->device));
+    std::string line;
+    int rc = report_line(h, description, h->runtime_s, line);
+    if (rc) return rc;
+    report_emit(h, line);
+    if (len) *len = (int32_t)line.size();
+    if (buf) {
+        const size_t k = std::min<size_t>(line.size(), (size_t)cap - 1);
+        std::memcpy(buf, line.data(), k);
+        buf[k] = '\0';
+    }
     return MPM_OK;
 }
 

@@ -8,6 +8,7 @@ reference reads the same:
 * ``getMaterialPoints()`` -> dict of numpy arrays, original order (PhysicsEngine.h:64)
 * ``getTime_Current / getTime_End / getTime_Increment / getTime_Runtime`` (:57-60)
 * ``getCount_MaterialPoint / getCount_GridPoint`` (:62-63)
+* ``reportConsole(description)`` -> the console/log line (PhysicsEngine.cpp:33-120)
 
 All compute runs in the HIP library; this module only marshals arrays.
 """
@@ -48,7 +49,7 @@ class PhysicsEngine:
         cfg.time_end = scene.time_end
         cfg.damping = scene.damping
         cfg.mass_min = scene.mass_min
-        cfg.console_interval = scene.console_interval if verbose else 0.0
+        cfg.console_interval = scene.console_interval
         cfg.precision = int(precision)
         cfg.strain_form = int(strain_form)
         cfg.neohookean = int(scene.neohookean)
@@ -194,6 +195,20 @@ class PhysicsEngine:
 
     def getTime_Increment(self):
         return self.scene.dt_max
+
+    def reportConsole(self, sDescription=""):
+        """The reference's console line for the current state (printed when
+        the engine is verbose, appended to the log file of ``set_log``)."""
+        n = C.c_int32()
+        cap = 4096 + 2 * len(sDescription)
+        buf = C.create_string_buffer(cap)
+        check(lib().mpm_report(self._h, sDescription.encode(), buf, cap, C.byref(n)), self._h)
+        return buf.value.decode()
+
+    def set_log(self, path):
+        """Append every report line to ``path`` (the reference's ./bmp/log.txt);
+        None stops writing."""
+        check(lib().mpm_set_log(self._h, None if path is None else str(path).encode()), self._h)
 
     def getTime_Runtime(self):
         ms = C.c_double()

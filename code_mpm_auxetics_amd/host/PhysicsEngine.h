@@ -143,6 +143,21 @@ class PhysicsEngine {
         return c;
     }
 
+    // PhysicsEngine::reportConsole(std::string) (PhysicsEngine.cpp:33-120):
+    // prints (when verbose) and logs the line; returns it as well
+    std::string reportConsole(const std::string &sDescription = "")
+    {
+        // the fields after the description take well under 512 bytes
+        std::vector<char> buf(sDescription.size() + 512);
+        int32_t len = 0;
+        check(mpm_report(h_.get(), sDescription.c_str(), buf.data(), (int32_t)buf.size(), &len), h_.get());
+        return std::string(buf.data());
+    }
+
+    // the log file of every report line (the reference's ./bmp/log.txt,
+    // Definitions.h:22); empty: none
+    void setLogFile(const std::string &path) { check(mpm_set_log(h_.get(), path.c_str()), h_.get()); }
+
   private:
     struct Del {
         void operator()(mpm_handle *h) const { mpm_destroy(h); }

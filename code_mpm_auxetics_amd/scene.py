@@ -11,6 +11,9 @@ have the same positions and the same order:
   MaterialPoint_Factory_createDomain_AuxeticCell_Swisscheese.cpp:4-101
 * ``is_inside_polygon`` / ``is_inside_ellipse`` -- MaterialPoint_Factory_isInside.cpp:4-35
 * ``reference_polygon_world`` -- PhysicsEngine_initializeWorld_AuxeticPolygonCell.cpp:4-313
+* ``reference_swisscheese_world`` -- PhysicsEngine_initializeWorld_AuxeticSwisscheeseCell.cpp:4-283
+* ``describe`` -- the world description both initializers report
+* ``from_config`` -- any of the worlds here by name, JSON file or dict
 
 Loop coordinates are accumulated sequentially (``dx += dOffset``) exactly as
 the reference loops do, so the FP64 positions match bit for bit.
@@ -283,6 +286,86 @@ def reference_polygon_world():
                    timeline_t=np.array([0.0, 0.1]), timeline_v=np.array([[0.0, -1.0, 0.0], [0.0, -1.0, 0.0]]))
     sc.info = dict(name="reference_polygon_world", offset=off)
     return sc
+
+
+def reference_swisscheese_world():
+    """PhysicsEngine::initializeWorld_AuxeticSwisscheeseCell
+    (…SwisscheeseCell.cpp:4-283): a 50 x 50 mm swiss-cheese cell (elliptical
+    holes 7.5 x 5.5 mm on a 14 mm pitch), perfectly plastic aluminium,
+    between a displacement-controlled top platen and a free bottom platen;
+    the top and bottom node planes fixed, the TimeLine ramping the platen to
+    1 m/s over 10 us."""
+    L = (0.06, 0.06, 0.001)
+    cells = (120, 120, 2)
+    h = tuple(L[a] / cells[a] for a in range(3))
+    p = node_positions(L, cells)
+    tol = 0.1 * h[0]
+    fixed = np.zeros(len(p), dtype=np.uint8)
+    fixed[np.abs(p[:, 0] - 0.0) < tol] |= FIX_X
+    fixed[np.abs(p[:, 0] - L[0]) < tol] |= FIX_X
+    fixed[np.abs(p[:, 1] - 0.0) < tol] |= FIX_X | FIX_Y | FIX_Z
+    fixed[np.abs(p[:, 1] - L[1]) < tol] |= FIX_X | FIX_Y | FIX_Z
+    fixed[np.abs(p[:, 2] - 0.0) < tol] |= FIX_Z
+    off = 1.0 / 2.0 * h[0]
+    thick = 2.0 * off
+    dims_cell = (0.05, 0.05, thick)
+    center = [0.5 * L[0], 0.5 * dims_cell[1] + 1.5 * h[1], 0.5 * thick]
+    cell = create_domain_auxetic_cell_swisscheese(center, dims_cell, (0.014, 0.014), (0.0075, 0.0055), off)
+    vol = off * off * off
+    al = _material_arrays(len(cell), 70.0e9, 0.33, 280.0e6, 4.0e1, 50.0e6, PLASTIC, 2700.0, vol)
+    plat_dims = (1.1 * dims_cell[0], 2.0 * off, dims_cell[2])
+    top = create_domain_cuboid([center[0], center[1] + 0.5 * dims_cell[1] + 1.0 * h[1], center[2]], plat_dims, off)
+    steel_t = _material_arrays(len(top), 210.0e9, 0.33, 200.0e6, 0.0, 0.0, ELASTIC, 7800.0, vol)
+    bot = create_domain_cuboid([center[0], center[1] - 0.5 * dims_cell[1] - 1.0 * h[1], center[2]], plat_dims, off)
+    steel_b = _material_arrays(len(bot), 210.0e9, 0.33, 200.0e6, 0.0, 0.0, ELASTIC, 7800.0, vol)
+    sc = _assemble(L, cells, fixed, [(cell, al, False), (top, steel_t, True), (bot, steel_b, False)],
+                   dt_max=2.0e-9, time_end=1.0e-3, damping=0.0, mass_min=0.001 * 2700.0 * vol,
+                   console_interval=1.0e-6, timeline_t=np.array([0.0, 1.0e-5, 1.0e6]),
+                   timeline_v=np.array([[0.0, 0.0, 0.0], [0.0, -1.0, 0.0], [0.0, -1.0, 0.0]]))
+    sc.info = dict(name="reference_swisscheese_world", offset=off, describe=dict(threads=False, timeline_probe=1.0))
+    return sc
+
+
+def _script(v, p):
+    """Script(double, p) of Definitions.h:43-56: scientific, explicit sign."""
+    return f"{v:+.{p}e}"
+
+
+def describe(sc, started=None):
+    """The world description initializeWorld_* hands to reportConsole
+    (…PolygonCell.cpp:275-309, …SwisscheeseCell.cpp:253-280); ``started`` is
+    the "Process started on" time (a ``time.struct_time``; now by default).
+    Pass it to ``PhysicsEngine.reportConsole`` for the first log line."""
+    import time as _time
+    fmt = sc.info.get("describe", dict(threads=True, timeline_probe=1.0e-4))
+    started = _time.localtime() if started is None else started
+    bar = "-" * 61 + "\n"
+    mass = 0.0
+    for m in np.asarray(sc.mass, dtype=np.float64).tolist():   # in point order, as the reference loop
+        mass += m
+    tl = np.asarray(sc.timeline_t, dtype=np.float64)
+    tv = np.asarray(sc.timeline_v, dtype=np.float64).reshape(-1, 3)
+    v_probe = 0.0
+    for i in range(len(tl) - 1):   # TimeLine::getVelocity (TimeLine.h:33-46)
+        t = fmt["timeline_probe"]
+        if tl[i] <= t < tl[i + 1]:
+            w = (t - tl[i]) / (tl[i + 1] - tl[i])
+            v_probe = (1.0 - w) * tv[i, 1] + w * tv[i + 1, 1]
+    out = bar + bar + "Process started on: " + _time.strftime("%d-%m-%Y %H:%M:%S", started) + "\n" + bar
+    if fmt["threads"]:
+        out += "Number of threads: 1\n"
+    out += "Time increment: " + _script(sc.dt_max, 6) + "\n"
+    out += "Material Point count: " + str(sc.n) + "\n"
+    out += "Mass: " + _script(mass, 6) + "\n" + bar
+    out += "Grid Resolution: (%d,%d,%d)\n" % tuple(sc.cells)
+    out += "Kernel Resolution: (%d,%d,%d)\n" % tuple(sc.cells)
+    out += "Material Point offset: " + _script(sc.info.get("offset", 0.5 * sc.h[0]), 3) + "\n"
+    out += "Timeline Speed: " + _script(v_probe, 3) + " m/s\n"
+    out += "Yield: " + _script(float(sc.sy[0]), 3) + " N/m^2\n"
+    out += "Modulus: " + _script(float(sc.E[0]), 3) + " N/m^2\n"
+    out += "Global Damping: " + _script(sc.damping, 3) + "\n"
+    out += "Non-slip contact\n"
+    return out
 
 
 def _grid_for(extent, h):
@@ -607,3 +690,52 @@ def effective_poisson(x0, x1, keep):
     eps_ax = span(x1, keep, 1) / span(x0, keep, 1) - 1.0
     eps_lat = span(x1, mid, 0) / span(x0, mid, 0) - 1.0
     return -eps_lat / eps_ax
+
+
+# ------------------------------------------------------------ config-driven
+def worlds():
+    """Name -> builder of every world this module makes (the
+    initializeWorld_* of the reference and the benchmark configurations)."""
+    return {
+        "reference_polygon": reference_polygon_world,
+        "reference_swisscheese": reference_swisscheese_world,
+        "polygon_mesh": polygon_mesh_world,
+        "swisscheese_cell": swisscheese_cell_world,
+        "small": small_world,
+        "translation": translation_world,
+        "config1": config1_world,
+        "config2": config2_world,
+        "config4": lambda **kw: batch_worlds(config4_worlds(**kw)),
+        "config5": config5_world,
+    }
+
+
+_SCENE_FIELDS = ("dt_max", "time_end", "damping", "mass_min", "console_interval", "neohookean")
+
+
+def from_config(spec):
+    """Build a scene from a spec: a world name, a path to a JSON file, or a
+    dict ``{"world": name, <builder keyword arguments>..., "set": {field:
+    value}}`` where ``set`` overrides solver fields (dt_max, time_end,
+    damping, mass_min, console_interval, neohookean) after the build."""
+    import json
+    if isinstance(spec, str):
+        if spec.endswith(".json"):
+            with open(spec) as f:
+                spec = json.load(f)
+        else:
+            spec = {"world": spec}
+    spec = dict(spec)
+    name = spec.pop("world")
+    over = spec.pop("set", {})
+    table = worlds()
+    if name not in table:
+        raise ValueError(f"unknown world {name!r}; known: {sorted(table)}")
+    sc = table[name](**spec)
+    for k, v in over.items():
+        if k not in _SCENE_FIELDS:
+            raise ValueError(f"cannot set {k!r}; settable: {_SCENE_FIELDS}")
+        setattr(sc, k, type(getattr(sc, k))(v))
+    sc.info.setdefault("spec", dict(world=name, **spec, set=over))
+    return sc
+

@@ -190,6 +190,24 @@ int mpm_last_step_ms(mpm_handle *h, double *ms);
  * …_MP.cpp:39-369). */
 int mpm_kernel_times(mpm_handle *h, double out[6]);
 
+/* reportConsole(sDescription) (PhysicsEngine.cpp:33-120): the console line of
+ * the current state, "<description>\ttime: T\tRuntime: R[\tPosition_y: Y]
+ * \tForce_y: F\tKinetic Energy: K\n", numbers in the reference's Script()
+ * format (Definitions.h:43-56: %+.6e, Runtime %+.3e).  Position_y is the first
+ * platen point's y (the displacement monitor of …PolygonCell.cpp:189), Force_y
+ * the last step's reaction, K = sum m|v|^2 over all points, R the device time
+ * of all mpm_step calls.  Printed to stdout when cfg.verbose, appended to the
+ * log file of mpm_set_log, copied to buf (NUL-terminated, at most cap bytes)
+ * when buf != NULL; *len (if not NULL) receives the full length.  mpm_step
+ * emits the same line (empty description) every console_interval and at the
+ * end, as …_MP.cpp:374-391 does. */
+int mpm_report(mpm_handle *h, const char *description, char *buf, int32_t cap, int32_t *len);
+
+/* The log file every report line is appended to: replaces the fixed
+ * _STR_LOGFILE "./bmp/log.txt" (Definitions.h:22).  NULL or "" (the default)
+ * writes no file. */
+int mpm_set_log(mpm_handle *h, const char *path);
+
 /* Human-readable last error of the handle (or of mpm_create when h is NULL). */
 const char *mpm_last_error(const mpm_handle *h);
 
