@@ -76,10 +76,20 @@ def measured_traffic():
     return int(d["traffic_bytes"]), d.get("source")
 
 
-def cpu_baseline(sc, seconds=15.0):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(sc, nthreads, seconds, max_steps):
     from oracle.oracle import Oracle
-    orc = Oracle(sc, nthreads=1, strain_form=0)
-    n = sc.n_deformable
+    orc = Oracle(sc, nthreads=nthreads, strain_form=0)
     orc.run_steps(1)                       # warm-up (page in, first grid sweep)
     steps = 0
     t0 = time.perf_counter()
@@ -87,11 +97,27 @@ def cpu_baseline(sc, seconds=15.0):
         orc.run_steps(1)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or steps >= 50:
+        if el >= seconds or steps >= max_steps:
             break
-    return dict(value=n * steps / el, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"{steps} steps of the {n}-point config-2 scene, FP64 oracle "
-                       f"(restatement of the reference step), 1 thread as the reference default")
+    return sc.n_deformable * steps / el, steps
+
+
+def cpu_baseline(sc, seconds=15.0):
+    """The FP64 oracle (a restatement of the reference step) on the host:
+    1 thread, the reference default (_MAX_N_THREADS 1, PhysicsEngine.h:23),
+    and all of this process's CPU share (OpenMP over points with per-thread
+    node grids, the reference's privatized scheme), each on a bounded sample
+    of the same scene."""
+    n = sc.n_deformable
+    v1, s1 = _oracle_rate(sc, 1, seconds, 50)
+    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)))
+    vt, st = _oracle_rate(sc, threads, 0.5 * seconds, 100)
+    return dict(value=v1, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{s1} steps of the {n}-point config-2 scene, FP64 oracle "
+                       f"(restatement of the reference step), 1 thread as the reference default",
+                all_cores=dict(value=vt, cores=threads, sample=f"{st} steps of the same scene, OpenMP, "
+                                                               f"privatized node grids"),
+                cpu_model=_cpu_model(), nproc=os.cpu_count())
 
 
 def batched_run(m, steps, warmup):
