@@ -82,6 +82,17 @@ struct PArrays {
     uint32_t *id, *pbrick;
 };
 
+// Element i of a point array through a 32-bit byte offset: the address is an
+// SGPR base plus a zero-extended 32-bit VGPR offset, which global loads and
+// stores take directly (no per-array 64-bit address arithmetic).  Point
+// counts are limited so that every offset fits 32 bits.
+constexpr uint32_t kMaxPoints = 1u << 28;
+template <typename T>
+__device__ __forceinline__ T &elt(T *base, uint32_t i)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T));
+}
+
 template <typename real>
 struct NodeRec {
     real d[8];

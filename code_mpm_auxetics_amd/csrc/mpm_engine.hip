@@ -480,12 +480,12 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.fy_val = h->fy_val ? h->fy_val + (size_t)step * h->n_fy : nullptr;
     const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long) + (g2p ? (size_t)6 * h->g.arenaN * h->rsize : 0);
     unsigned grid = (unsigned)h->n_occ;
-    if (g2p) grid += cdiv(h->n_dc, kBlock);
+    if (g2p) grid += cdiv(h->n_dc, kPBlock);
     if (grid == 0) return MPM_OK;
     if (g2p)
-        k_particle<real, true><<<grid, kBlock, lds, h->stream>>>(a, step);
+        k_particle<real, true><<<grid, kPBlock, lds, h->stream>>>(a, step);
     else
-        k_particle<real, false><<<grid, kBlock, lds, h->stream>>>(a, step);
+        k_particle<real, false><<<grid, kPBlock, lds, h->stream>>>(a, step);
     HIPCHK(h, hipGetLastError());
     return MPM_OK;
 }
@@ -1357,6 +1357,7 @@ int mpm_upload_points(mpm_handle *h, const mpm_points *pts)
     if (!h || !pts || pts->n < 0 || !pts->x || !pts->mass) { if (h) h->err = "null points"; return MPM_E_ARG; }
     if (h->uploaded) { h->err = "points already uploaded"; return MPM_E_STATE; }
     if (!pts->vol0 && !pts->vol) { h->err = "vol0 or vol required"; return MPM_E_ARG; }
+    if ((uint32_t)pts->n > kMaxPoints) { h->err = "more than 2^28 points"; return MPM_E_ARG; }
     HIPCHK(h, hipSetDevice(h->device));
     int rc = h->fp32 ? upload_t<float>(h, pts) : upload_t<double>(h, pts);
     if (rc == MPM_OK) h->uploaded = true;

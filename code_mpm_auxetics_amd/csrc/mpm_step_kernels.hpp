@@ -30,6 +30,10 @@
 #ifndef MPM_KP_WAVES64
 #define MPM_KP_WAVES64 2
 #endif
+// threads per particle workgroup (a multiple of 64)
+#ifndef MPM_KP_BLOCK
+#define MPM_KP_BLOCK 256
+#endif
 
 namespace mpm {
 
@@ -402,13 +406,13 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
     de[5] = L[6] * dt + L[2] * dt;
     if (g.track_strain)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) P.eps[q][i] += de[q];
+        for (int q = 0; q < 6; ++q) elt(P.eps[q], i) += de[q];
     // constitutive dispatch (…_MP.cpp:322-353)
-    const Material &mt = mats[P.mat[i]];
+    const Material &mt = mats[elt(P.mat, i)];
     int type = mt.type;
     if (type == 1 && !g.neohookean) type = 3;
     if (type == 2 || type == 5) type = 3;
-    const double E = g.has_pointE ? (double)P.E[i] : mt.E;
+    const double E = g.has_pointE ? (double)elt(P.E, i) : mt.E;
     // FP32 state: elastic constants of this point without divisions
     // on the common path (per-point E only when materials overflow
     // the table)
@@ -442,7 +446,7 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
     } else if constexpr (FP32) {
         // FP64 island for FP32 state
         double dep[6];
-        const double al = (type == 4) ? (double)P.alpha[i] : 0.0;
+        const double al = (type == 4) ? (double)elt(P.alpha, i) : 0.0;
         const double radius = mt.k0 + al * 0.70710678118654752440;   // al / sqrt(2)
         if (radial_return_fp32(mt.nu, C, ie, ig, radius, s, de, dep)) {
             if (type == 4) {
@@ -451,10 +455,10 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
                 for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
                 eff = sqrt(eff * (2.0 / 3.0));
                 const double da = ((mt.hs - al) > 0.0) ? mt.C0 * (mt.hs - al) * eff : 0.0;
-                P.alpha[i] = (real)(al + da);
+                elt(P.alpha, i) = (real)(al + da);
             }
 #pragma unroll
-            for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
+            for (int q = 0; q < 6; ++q) elt(P.ep[q], i) += (real)dep[q];
         }
     } else {
         // FP64 state: reference evaluation order
@@ -462,7 +466,7 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
         for (int q = 0; q < 6; ++q) { sd[q] = (double)s[q]; ded[q] = (double)de[q]; }
         const double k0 = mt.sy / sqrt(3.0);
         if (type == 4) {
-            const double al = (double)P.alpha[i];
+            const double al = (double)elt(P.alpha, i);
             if (radial_return(E, mt.nu, k0 + al / sqrt(2.0), sd, ded, ds, dep)) {
                 double eff = 0.0;
                 for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
@@ -470,16 +474,16 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
                 eff = sqrt(eff);
                 if ((sqrt(2.0 / 3.0) * mt.C1 - al) > 0.0)
                     da = mt.C0 * (sqrt(2.0 / 3.0) * mt.C1 - al) * eff;
-                P.alpha[i] = (real)(al + da);
-                for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
+                elt(P.alpha, i) = (real)(al + da);
+                for (int q = 0; q < 6; ++q) elt(P.ep[q], i) += (real)dep[q];
             }
         } else if (radial_return(E, mt.nu, k0, sd, ded, ds, dep)) {
-            for (int q = 0; q < 6; ++q) P.ep[q][i] += (real)dep[q];
+            for (int q = 0; q < 6; ++q) elt(P.ep[q], i) += (real)dep[q];
         }
         for (int q = 0; q < 6; ++q) s[q] = (real)(sd[q] + ds[q]);
     }
 #pragma unroll
-    for (int q = 0; q < 6; ++q) P.s[q][i] = s[q];
+    for (int q = 0; q < 6; ++q) elt(P.s[q], i) = s[q];
 }
 
 // Fixed-point fraction bits.  FP32 state: each contribution is converted to a
@@ -487,19 +491,20 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
 // is < 2^26, then added to the 64-bit arena.  FP64 state: 64-bit conversion
 // with FRAC = 61 - H, count < 2^(H-1), so every node sum stays below 2^60.
 constexpr int kFrac32 = 26;
+constexpr int kPBlock = MPM_KP_BLOCK;
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
 
 // One brick per workgroup.  Dynamic LDS: long long arena[7 * arenaN].
 template <typename real, bool G2P>
-__global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : MPM_KP_WAVES64) k_particle(KpArgs<real> a, int step)
+__global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : MPM_KP_WAVES64) k_particle(KpArgs<real> a, int step)
 {
     extern __shared__ long long s_arena[];   // node-major: kQ int64 per arena node, then
                                              // (G2P) the node window: 6 reals {v, a} per node
     __shared__ uint32_t s_max[8];            // running max bound bits per field
     __shared__ int4 s_bi[27];                // neighbour brick records (dz, dy, dx ascending)
-    __shared__ int s_box[kBlock / 64][6];
-    __shared__ double s_red[kBlock / 64][5];
+    __shared__ int s_box[kPBlock / 64][6];
+    __shared__ double s_red[kPBlock / 64][5];
 
     constexpr bool FP32 = sizeof(real) == 4;
     const GridParams &g = a.g;
@@ -511,7 +516,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
     // ---- platen (displacement-controlled) blocks: phase 8 + override marks
     if ((int)blockIdx.x >= a.occ_cap) {
         if (!G2P) return;
-        const int j = ((int)blockIdx.x - a.occ_cap) * kBlock + tid;
+        const int j = ((int)blockIdx.x - a.occ_cap) * kPBlock + tid;
         if (j >= a.n_dc) return;
         double *x = &a.dc_x[3 * j], v[3];
         // …_MP.cpp:360-368: velocity from the TimeLine at the step's start time
@@ -565,7 +570,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         const int nwin = wext[0] * wext[1] * wext[2];
         const real dtg = (real)st.dt, m_min = (real)g.m_min, damp = (real)g.damping;
         double gs[3] = {0.0, 0.0, 0.0};   // owned nodes: mass, reaction, active count
-        for (int r = tid; r < nwin; r += kBlock) {
+        for (int r = tid; r < nwin; r += kPBlock) {
             const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
             const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
             const uint8_t fx = a.fixed[nid];
@@ -664,12 +669,12 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         __syncthreads();
         if (tid < 3) {
             double t = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) t += s_red[w][tid];
+            for (int w = 0; w < kPBlock / 64; ++w) t += s_red[w][tid];
             a.gpart[3 * (size_t)slot + tid] = t;
         }
     }
 
-    for (int i = tid; i < kQ * AN; i += kBlock) s_arena[i] = 0;
+    for (int i = tid; i < kQ * AN; i += kPBlock) s_arena[i] = 0;
     if (tid < 8) s_max[tid] = 0u;
     int H = 6;   // a node takes at most one contribution per point: count < 2^(H-1)
     while ((1u << (H - 1)) <= count) ++H;
@@ -686,7 +691,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                              (float)(dtd * (double)g.ihf[2] * kQOne)};
     __syncthreads();
 
-    for (uint32_t c0 = 0; c0 < count; c0 += kBlock) {
+    for (uint32_t c0 = 0; c0 < count; c0 += kPBlock) {
         const bool active = (c0 + tid) < count;
         const uint32_t i = start + c0 + tid;
         int key = -1;
@@ -701,16 +706,16 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
             int32_t xq[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                if (FP32) xq[q] = P.xq[q][i];
-                else x[q] = P.x[q][i];
-                v[q] = P.v[q][i];
+                if (FP32) xq[q] = elt(P.xq[q], i);
+                else x[q] = elt(P.x[q], i);
+                v[q] = elt(P.v[q], i);
             }
 #pragma unroll
-            for (int q = 0; q < 9; ++q) F[q] = P.F[q][i];
+            for (int q = 0; q < 9; ++q) F[q] = elt(P.F[q], i);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) s[q] = P.s[q][i];
-            m = P.mass[i];
-            const real V0 = P.vol0[i];
+            for (int q = 0; q < 6; ++q) s[q] = elt(P.s[q], i);
+            m = elt(P.mass, i);
+            const real V0 = elt(P.vol0, i);
             real V = real(0);
 
             if (G2P) {
@@ -793,12 +798,12 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                     // write back the kinematic state before the material update
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
-                        if (FP32) P.xq[q][i] = xq[q];
-                        else P.x[q][i] = x[q];
-                        P.v[q][i] = v[q];
+                        if (FP32) elt(P.xq[q], i) = xq[q];
+                        else elt(P.x[q], i) = x[q];
+                        elt(P.v[q], i) = v[q];
                     }
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) P.F[q][i] = F[q];
+                    for (int q = 0; q < 9; ++q) elt(P.F[q], i) = F[q];
                     material_update<real>(g, P, a.mats, i, L, dt, F, s);
                 }
             } else {
@@ -841,7 +846,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                 }
                 if (g.has_fext)
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) fe[q] = P.fext[q][i];
+                    for (int q = 0; q < 3; ++q) fe[q] = elt(P.fext[q], i);
 #pragma unroll
                 for (int q = 0; q < 3; ++q) mv[q] = m * v[q];
 #pragma unroll
@@ -879,7 +884,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
                 ecur[q] = en;
             }
             if (any) {   // the exponent grew: rescale what the arena holds
-                for (int idx = tid; idx < kQ * AN; idx += kBlock) {
+                for (int idx = tid; idx < kQ * AN; idx += kPBlock) {
                     int shq = 0;
 #pragma unroll
                     for (int q = 0; q < kQ; ++q) shq = (idx % kQ == q) ? sh[q] : shq;
@@ -953,7 +958,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         int lo = 1 << 20, hi = -1;
-        for (int w = 0; w < kBlock / 64; ++w) { lo = min(lo, s_box[w][2 * q]); hi = max(hi, s_box[w][2 * q + 1]); }
+        for (int w = 0; w < kPBlock / 64; ++w) { lo = min(lo, s_box[w][2 * q]); hi = max(hi, s_box[w][2 * q + 1]); }
         box[2 * q] = lo;
         box[2 * q + 1] = hi;
     }
@@ -972,7 +977,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
         double scl[kQ];
 #pragma unroll
         for (int q = 0; q < kQ; ++q) scl[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, ecur[q] - FRAC);
-        for (int r = tid; r < nbox; r += kBlock) {
+        for (int r = tid; r < nbox; r += kPBlock) {
             const int lx = box[0] + r % bx, ly = box[2] + (r / bx) % by, lz = box[4] + r / (bx * by);
             const long long *src = s_arena + kQ * (lx + A0 * ly + A01 * lz);
             real rec[8];
@@ -995,7 +1000,7 @@ __global__ void __launch_bounds__(kBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : M
     __syncthreads();
     if (tid < 5) {
         double r = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) r += s_red[w][tid];
+        for (int w = 0; w < kPBlock / 64; ++w) r += s_red[w][tid];
         a.kp_part[5 * slot + tid] = r;
     }
 }
