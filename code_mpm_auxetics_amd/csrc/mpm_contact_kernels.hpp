@@ -53,6 +53,7 @@ struct CArgs {
     double *cpart;                // per node block: mass, reaction, active nodes
     double *ppart;                // per point block: sum m|v|^2, m v.xyz, nonfinite
     int32_t H;                    // headroom bits of the fixed-point sums
+    double mu;                    // Coulomb coefficient of the contact correction (0: frictionless)
 };
 
 // ---- point access ---------------------------------------------------------
@@ -520,6 +521,24 @@ __global__ void __launch_bounds__(256) k_c_grid(CArgs<real> a, int step)
                 for (int q = 0; q < 3; ++q) { d1[q] = vcm[q] - v[0][q]; d2[q] = vcm[q] - v[1][q]; }
                 const double s1 = m / dt * c_dot3(d1, nrm), s2 = mn / dt * c_dot3(d2, nneg);
                 for (int q = 0; q < 3; ++q) { fc[q] = s1 * nrm[q]; fcn[q] = s2 * nneg[q]; }
+                if (a.mu > 0.0) {
+                    // Coulomb friction (no reference counterpart: the reference
+                    // corrects the normal velocity only).  The tangential part
+                    // of field 0's stick correction d1 is capped at mu times its
+                    // normal part; field 1 takes the opposite impulse.
+                    const double dn = c_dot3(d1, nrm);
+                    double dtg[3];
+                    for (int q = 0; q < 3; ++q) dtg[q] = d1[q] - dn * nrm[q];
+                    const double lt = c_len3(dtg);
+                    if (lt > 0.0) {
+                        const double scl = fmin(1.0, a.mu * fabs(dn) / lt);
+                        for (int q = 0; q < 3; ++q) {
+                            const double ft = m / dt * scl * dtg[q];
+                            fc[q] += ft;
+                            fcn[q] -= ft;
+                        }
+                    }
+                }
             }
             if (c_lenm3(m) > g.m_min)
                 for (int q = 0; q < 3; ++q) { f[0][q] += fc[q]; v[0][q] += fc[q] / m * dt; }

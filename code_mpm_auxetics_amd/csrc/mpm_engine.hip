@@ -573,6 +573,7 @@ static CArgs<real> contact_args(mpm_handle *h)
     int H = 2;
     while ((1ll << (H - 2)) <= (long long)a.n_all) ++H;   // 2^(H-2) > points touching a node
     a.H = H;
+    a.mu = h->cfg.friction;
     return a;
 }
 

@@ -150,3 +150,53 @@ def test_switching_paths(mpm, oracle_mod):
         orc.run_steps(n, path)
     _cmp(eng.getMaterialPoints(), orc.points(), x0, 1e-9)
     eng.close()
+
+
+def _sliding_blocks():
+    """The closing blocks with a tangential relative velocity (A +y, B -y)."""
+    sc = make_golden.contact_blocks()
+    nA = len(sc.x) // 2
+    sc.v[:nA, 1] = 1.0
+    sc.v[nA:, 1] = -1.0
+    return sc, nA
+
+
+def test_friction_zero_is_the_reference_path(mpm):
+    sc, _ = _sliding_blocks()
+    out = []
+    for kw in ({}, {"friction": 0.0}):
+        eng = mpm.PhysicsEngine(sc, precision=mpm.FP64, **kw)
+        eng.run_steps(20, path=mpm.PATH_CONTACT)
+        out.append(eng.getMaterialPoints())
+        eng.close()
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_friction_conserves_momentum_and_adds_tangential_coupling(mpm):
+    """Coulomb friction (an extension; the reference's correction is normal
+    only): the capped tangential impulse is applied equal and opposite to the
+    two fields, so total momentum stays conserved (no boundary conditions,
+    no mass gate in this scene), and the tangential momentum block A gives up
+    to block B grows with the coefficient.  Most of the blocks' coupling goes
+    through nodes both blocks share in one field, which the reference treats
+    as no-slip, so the frictional share is a fraction of the total."""
+    sc, nA = _sliding_blocks()
+    m = sc.mass
+    p0 = (m[:, None] * sc.v).sum(axis=0)
+    scale = np.abs(m[:, None] * sc.v).sum()
+    pa0 = (m[:nA, None] * sc.v[:nA]).sum(axis=0)
+    dy = {}
+    for mu in (0.0, 0.2, 5.0):
+        eng = mpm.PhysicsEngine(sc, precision=mpm.FP64, friction=mu)
+        eng.run_steps(40, path=mpm.PATH_CONTACT)
+        v = eng.getMaterialPoints()["v"]
+        eng.close()
+        assert np.isfinite(v).all()
+        p1 = (m[:, None] * v).sum(axis=0)
+        assert np.allclose(p1, p0, rtol=0, atol=1e-12 * scale), (mu, p1 - p0)
+        da = (m[:nA, None] * v[:nA]).sum(axis=0) - pa0
+        assert da[0] < 0.0            # the normal impulse slows A
+        dy[mu] = -da[1]               # tangential momentum A loses (it slides in +y)
+        print(f"mu {mu}: A normal {da[0]:+.4e} tangential {da[1]:+.4e}", flush=True)
+    assert dy[0.0] > 0.0 and dy[5.0] > 1.03 * dy[0.0] and dy[0.2] >= dy[0.0]
