@@ -151,6 +151,28 @@ def batched_run(m, steps, warmup):
                       "simulated_time_s": steps * sc.dt_max + warmup * sc.dt_max}), flush=True)
 
 
+def config3_run(m, steps, warmup):
+    """Config 3: the swiss-cheese block (8 x 8 holes, Neo-Hookean, about 2.4M
+    points) under a point platen on the contact path (two-field grid, kernel
+    grid, surface normals, contact correction with Coulomb friction 0.2), FP32
+    state with FP64 checks.  Times ``steps`` steps after ``warmup``."""
+    from code_mpm_auxetics_amd import scene
+    sc = scene.config3_world()
+    n = sc.n_deformable
+    eng = m.PhysicsEngine(sc, precision=m.FP32, track_strain=False, friction=0.2)
+    eng.run_steps(warmup, path=m.PATH_CONTACT)
+    t0 = time.perf_counter()
+    eng.run_steps(steps, path=m.PATH_CONTACT)
+    t1 = time.perf_counter()
+    c = eng.checks()
+    eng.close()
+    return {"workload": "config3_swisscheese_block_contact", "n_points": n, "grid_cells": list(sc.cells),
+            "steps": steps, "warmup": warmup, "value": round(n * steps / (t1 - t0), 1), "unit": "particle-steps/s",
+            "ms_per_step": round((t1 - t0) * 1e3 / steps, 4), "dtype": "f32", "friction": 0.2,
+            "dt_s": sc.dt_max, "kinetic_2": c["kinetic_2"], "reaction_y": c["reaction_y"],
+            "nonfinite": c["nonfinite"]}
+
+
 def config5_run(reps=3):
     """Config 5: the differentiable run on the 4x4 polygon mesh (about 160k
     points, per-point E log-uniform in [0.5, 2] GPa), 400 FP64 steps with
@@ -211,7 +233,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="config2", choices=["config2", "config4", "config5"],
+    ap.add_argument("--workload", default="config2", choices=["config2", "config3", "config4", "config5"],
                     help="config4: the 256-cell batch (~8M points) in one launch per step; config5: the "
                          "differentiable run alone; each prints its own line")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the config-5 adjoint timing in the main line")
@@ -231,6 +253,9 @@ def main():
         return
     if args.workload == "config5":
         print(json.dumps(config5_run()), flush=True)
+        return
+    if args.workload == "config3":
+        print(json.dumps(config3_run(m, args.steps, args.warmup)), flush=True)
         return
     sc = build_scene(args.scale, args.speed)
     n = sc.n_deformable

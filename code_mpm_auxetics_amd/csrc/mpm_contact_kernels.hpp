@@ -140,6 +140,45 @@ __device__ __forceinline__ bool c_adjacent(const GridParams &g, const double x[3
     return true;
 }
 
+// Per axis, the two node planes of the cell (node index i0 and i0 + 1): the
+// node coordinate (i L) / n, the hat value and sign(x - x_i), each computed
+// with the same expressions as c_node_pos / c_bases, once per point instead
+// of once per corner.
+struct CAx {
+    double xi[3][2], s[3][2], sg[3][2];
+};
+__device__ __forceinline__ void c_axes(const GridParams &g, const double x[3], const int i0[3], CAx &A)
+{
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double xi = (double)(i0[q] + c) * g.L[q] / (double)g.cells[q];
+            const double d = x[q] - xi;
+            double v = 1.0 - fabs(d) / g.h[q];
+            if (v < 0.0 || v > 1.0) v = 0.0;
+            A.xi[q][c] = xi;
+            A.s[q][c] = v;
+            A.sg[q][c] = sgn(d);
+        }
+}
+// node position, N and grad N of corner k (x-outer, z-inner), as c_bases
+__device__ __forceinline__ void c_corner(const GridParams &g, const CAx &A, int k, double xi[3], double &N, double G[3])
+{
+    const int c[3] = {(k >> 2) & 1, (k >> 1) & 1, k & 1};
+    double s[3], sg[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        xi[q] = A.xi[q][c[q]];
+        s[q] = A.s[q][c[q]];
+        sg[q] = A.sg[q][c[q]];
+    }
+    N = s[0] * s[1] * s[2];
+    G[0] = -s[1] * s[2] * sg[0] / g.h[0];
+    G[1] = -s[0] * s[2] * sg[1] / g.h[1];
+    G[2] = -s[0] * s[1] * sg[2] / g.h[2];
+}
+
 // fixed-point scale of bound slot q: contributions |c| <= bound < 2^e are
 // stored as c * 2^(FRAC - e) with FRAC = 62 - H, H headroom bits
 template <typename real>
@@ -169,6 +208,61 @@ __device__ __forceinline__ void c_umax(unsigned long long *p, double v)
     if (v > 0.0) atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
+__device__ __forceinline__ double wave_dmax(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Node window of a block: the points of one launch are in the plain step's
+// brick order, so a block's 256 points touch a small box of nodes.  The block
+// adds its fixed-point terms there in LDS and flushes each node once with a
+// global atomic (the sums are exact integers, so this equals adding every
+// term globally).  A box beyond the window falls back to global atomics.
+constexpr int kCWin = 512;
+struct CWin {
+    int lo[3], dim[3], nodes;
+    bool lds;
+};
+// nodes i0 - below .. i0 + above of every valid point's cell i0, at most cap
+__device__ __forceinline__ CWin c_window(bool valid, const int i0[3], int *sbox, int below = 0, int above = 1,
+                                         int cap = kCWin)
+{
+    if (threadIdx.x < 3) {
+        sbox[threadIdx.x] = 0x7FFFFFFF;
+        sbox[3 + threadIdx.x] = -1;
+    }
+    __syncthreads();
+    if (valid)
+        for (int q = 0; q < 3; ++q) {
+            atomicMin(&sbox[q], i0[q]);
+            atomicMax(&sbox[3 + q], i0[q]);
+        }
+    __syncthreads();
+    CWin w{};
+    if (sbox[3] < 0) return w;
+    long long nodes = 1;
+    for (int q = 0; q < 3; ++q) {
+        w.lo[q] = sbox[q] - below;
+        w.dim[q] = sbox[3 + q] - sbox[q] + 1 + below + above;
+        nodes *= w.dim[q];
+    }
+    w.lds = nodes <= cap;
+    w.nodes = w.lds ? (int)nodes : 0;
+    return w;
+}
+__device__ __forceinline__ int c_wslot(const CWin &w, const int n[3])
+{
+    return (n[0] - w.lo[0]) + w.dim[0] * ((n[1] - w.lo[1]) + w.dim[1] * (n[2] - w.lo[2]));
+}
+__device__ __forceinline__ long long c_wnode(const GridParams &g, const CWin &w, int t)
+{
+    const int jx = t % w.dim[0], r = t / w.dim[0];
+    const int jy = r % w.dim[1], jz = r / w.dim[1];
+    return (w.lo[0] + jx) + (long long)g.nodes[0] * ((w.lo[1] + jy) + (long long)g.nodes[1] * (w.lo[2] + jz));
+}
+
 // accumulator offsets inside CArgs::acc (in units of nn)
 __device__ __forceinline__ long long off_k() { return 0; }
 __device__ __forceinline__ long long off_m(int f) { return 1 + f; }
@@ -194,74 +288,139 @@ __global__ void __launch_bounds__(256) k_c_clear(CArgs<real> a)
     if (blockIdx.x == 0 && threadIdx.x < kCB) a.bounds[threadIdx.x] = 0ull;
 }
 
-// ---- bounds of the contributions of this step ----------------------------------
+// ---- bounds of the contributions of this step (wave maxima, one atomic per
+// wave and bound) ----------------------------------------------------------------
 template <typename real>
 __global__ void __launch_bounds__(256) k_c_bounds(CArgs<real> a)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= a.n_all) return;
     const GridParams &g = a.g;
-    const double ihm = fmax(1.0 / g.h[0], fmax(1.0 / g.h[1], 1.0 / g.h[2]));
-    double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
-    if (j >= a.n_def) {
-        const int d = j - a.n_def;
-        m = a.dc_m[d];
-        vol = a.dc_vol[d];
-        for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
-        for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
-    } else {
-        m = (double)a.p.mass[j];
-        double Fd[9];
-        for (int q = 0; q < 9; ++q) Fd[q] = (double)a.p.F[q][j];
-        vol = det3(Fd) * (double)a.p.vol0[j];
-        for (int q = 0; q < 3; ++q) {
-            v[q] = (double)a.p.v[q][j];
-            if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
+    double b[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < a.n_all) {
+        const double ihm = fmax(1.0 / g.h[0], fmax(1.0 / g.h[1], 1.0 / g.h[2]));
+        double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
+        if (j >= a.n_def) {
+            const int d = j - a.n_def;
+            m = a.dc_m[d];
+            vol = a.dc_vol[d];
+            for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
+            for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
+        } else {
+            m = (double)a.p.mass[j];
+            double Fd[9];
+            for (int q = 0; q < 9; ++q) Fd[q] = (double)a.p.F[q][j];
+            vol = det3(Fd) * (double)a.p.vol0[j];
+            for (int q = 0; q < 3; ++q) {
+                v[q] = (double)a.p.v[q][j];
+                if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
+            }
+            for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
         }
-        for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
+        double smax = 0.0;
+        for (int q = 0; q < 6; ++q) smax = fmax(smax, fabs(s[q]));
+        b[0] = fabs(m);
+        b[1] = fabs(m) * ihm;
+        b[2] = fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2])));
+        b[3] = fabs(vol) * 3.0 * smax * ihm + fmax(fabs(fe[0]), fmax(fabs(fe[1]), fabs(fe[2])));
     }
-    double smax = 0.0;
-    for (int q = 0; q < 6; ++q) smax = fmax(smax, fabs(s[q]));
-    c_umax(&a.bounds[0], fabs(m));
-    c_umax(&a.bounds[1], fabs(m) * ihm);
-    c_umax(&a.bounds[2], fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2]))));
-    c_umax(&a.bounds[3], fabs(vol) * 3.0 * smax * ihm + fmax(fabs(fe[0]), fmax(fabs(fe[1]), fabs(fe[2]))));
+    // block maxima; the global slot is only written when the block beats it
+    __shared__ double s_b[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double w = wave_dmax(b[q]);
+        if ((threadIdx.x & 63) == 0) s_b[threadIdx.x >> 6][q] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int q = threadIdx.x;
+        const double w = fmax(fmax(s_b[0][q], s_b[1][q]), fmax(s_b[2][q], s_b[3][q]));
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(w);
+        if (w > 0.0 && wb > __hip_atomic_load(&a.bounds[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&a.bounds[q], wb);
+    }
     if (j == 0) a.bounds[4] = (unsigned long long)__double_as_longlong(1.0);
 }
 
-// ---- C2 + C3: active nodes, kernel on the grid (…_MP_Contact.cpp:94-153) ----
+// ---- C2 + C3: active nodes, kernel on the grid (…_MP_Contact.cpp:94-153).
+// The block's kernel sums and node flags go through an LDS window of the
+// nodes i0 - 1 .. i0 + 2 of its points' cells.
+constexpr int kCWinK = 1024;
 template <typename real>
 __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
 {
+    __shared__ int sbox[6];
+    __shared__ unsigned long long s_k[kCWinK];
+    __shared__ uint32_t s_f[kCWinK];
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= a.n_all) return;
     const GridParams &g = a.g;
-    double x[3];
-    c_pos(a, j, x);
+    double x[3] = {0.0, 0.0, 0.0};
     long long idx[8];
     int ijk[8][3];
-    if (!c_adjacent(g, x, idx, ijk)) { atomicOr(a.gflags, FLAG_OUT_OF_GRID); return; }
-    for (int k = 0; k < 8; ++k) atomicOr(&a.flags[idx[k]], CF_ACTIVE);
-    const double R = fabs(1.0 * g.h[0]);
-    const double sck = c_scale(a, 4);
-    int i0[3];
-    for (int q = 0; q < 3; ++q) i0[q] = (int)(x[q] / g.h[q]);
-    for (int ix = -1; ix < 3; ++ix)
-        for (int iy = -1; iy < 3; ++iy)
-            for (int iz = -1; iz < 3; ++iz) {
-                const int n0 = i0[0] + ix, n1 = i0[1] + iy, n2 = i0[2] + iz;
-                if (n0 < 0 || n1 < 0 || n2 < 0 || n0 > g.nodes[0] - 1 || n1 > g.nodes[1] - 1 || n2 > g.nodes[2] - 1) continue;
-                double xi[3];
-                c_node_pos(g, n0, n1, n2, xi);
-                const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
-                const double dist = c_len3(d);
-                if (dist < R) {
-                    const long long I = n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2);
-                    atomicOr(&a.flags[I], CF_KACTIVE);
-                    const double q = dist / R;
-                    c_add(&a.acc[off_k() * a.nn + I], 1.0 - 3.0 * q * q + 2.0 * q * q * q, sck);
+    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
+    bool valid = j < a.n_all;
+    if (valid) {
+        c_pos(a, j, x);
+        valid = c_adjacent(g, x, idx, ijk);
+        if (!valid) atomicOr(a.gflags, FLAG_OUT_OF_GRID);
+    }
+    const CWin w = c_window(valid, ijk[0], sbox, 1, 2, kCWinK);
+    for (int t = threadIdx.x; t < w.nodes; t += 256) {
+        s_k[t] = 0ull;
+        s_f[t] = 0u;
+    }
+    __syncthreads();
+    if (valid) {
+        for (int k = 0; k < 8; ++k) {
+            if (w.lds) atomicOr(&s_f[c_wslot(w, ijk[k])], CF_ACTIVE);
+            else atomicOr(&a.flags[idx[k]], CF_ACTIVE);
+        }
+        const double R = fabs(1.0 * g.h[0]);
+        const double sck = c_scale(a, 4);
+        const int *i0 = ijk[0];
+        // node coordinates (i L) / n of the 4 candidate planes per axis
+        double xc[3][4];
+        for (int q = 0; q < 3; ++q)
+            for (int t = 0; t < 4; ++t) xc[q][t] = (double)(i0[q] + t - 1) * g.L[q] / (double)g.cells[q];
+        // a node can only be within R if it is within R on every axis (a
+        // slightly inclusive prefilter; the exact test below decides)
+        bool near_ax[3][4];
+        for (int q = 0; q < 3; ++q)
+            for (int t = 0; t < 4; ++t) near_ax[q][t] = fabs(x[q] - xc[q][t]) < R * (1.0 + 1.0e-12);
+        for (int ix = -1; ix < 3; ++ix)
+            for (int iy = -1; iy < 3; ++iy)
+                for (int iz = -1; iz < 3; ++iz) {
+                    if (!(near_ax[0][ix + 1] && near_ax[1][iy + 1] && near_ax[2][iz + 1])) continue;
+                    const int n[3] = {i0[0] + ix, i0[1] + iy, i0[2] + iz};
+                    if (n[0] < 0 || n[1] < 0 || n[2] < 0 || n[0] > g.nodes[0] - 1 || n[1] > g.nodes[1] - 1 ||
+                        n[2] > g.nodes[2] - 1)
+                        continue;
+                    const double d[3] = {x[0] - xc[0][ix + 1], x[1] - xc[1][iy + 1], x[2] - xc[2][iz + 1]};
+                    const double dist = c_len3(d);
+                    if (dist < R) {
+                        const double q = dist / R;
+                        const unsigned long long iv =
+                            (unsigned long long)__double2ll_rn((1.0 - 3.0 * q * q + 2.0 * q * q * q) * sck);
+                        if (w.lds) {
+                            const int ws = c_wslot(w, n);
+                            atomicOr(&s_f[ws], CF_KACTIVE);
+                            atomicAdd(&s_k[ws], iv);
+                        } else {
+                            const long long I = n[0] + (long long)g.nodes[0] * (n[1] + (long long)g.nodes[1] * n[2]);
+                            atomicOr(&a.flags[I], CF_KACTIVE);
+                            atomicAdd((unsigned long long *)&a.acc[off_k() * a.nn + I], iv);
+                        }
+                    }
                 }
-            }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < w.nodes; t += 256) {
+        const uint32_t f = s_f[t];
+        if (!f) continue;
+        const long long I = c_wnode(g, w, t);
+        atomicOr(&a.flags[I], f);
+        const unsigned long long v = s_k[t];
+        if (v) atomicAdd((unsigned long long *)&a.acc[off_k() * a.nn + I], v);
+    }
 }
 
 // ---- C4: kernel gradient by central differences (…_MP_Contact.cpp:159-191).
@@ -292,34 +451,66 @@ __global__ void __launch_bounds__(256) k_c_kgrad(CArgs<real> a)
 }
 
 // ---- C5: point kernel and its gradient; nearest point and surface gradient,
-// pass 1 (minimum distance, maximum |grad k|) (…_MP_Contact.cpp:199-252)
+// pass 1 (minimum distance, maximum |grad k|) (…_MP_Contact.cpp:199-252),
+// through an LDS window of the block's nodes
 template <typename real>
 __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
 {
+    __shared__ int sbox[6];
+    __shared__ unsigned long long s_near[kCWin], s_sg[kCWin];
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= a.n_all) return;
     const GridParams &g = a.g;
-    double x[3];
-    c_pos(a, j, x);
+    double x[3] = {0.0, 0.0, 0.0};
     long long idx[8];
     int ijk[8][3];
-    if (!c_adjacent(g, x, idx, ijk)) return;
-    const double usk = c_unscale(a, 4);
-    double kp = 0.0, kg[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < 8; ++k) {
-        double xi[3], N, G[3];
-        c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
-        c_bases(g, x, xi, N, G);
-        kp += N * ((double)a.acc[idx[k]] * usk);
-        for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + idx[k]];
-        const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
-        atomicMin(&a.near_d[idx[k]], (unsigned long long)__double_as_longlong(c_len3(d)));
+    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
+    bool valid = j < a.n_all;
+    if (valid) {
+        c_pos(a, j, x);
+        valid = c_adjacent(g, x, idx, ijk);
     }
-    a.kp[j] = kp;
-    for (int q = 0; q < 3; ++q) a.kg[q * a.n_all + j] = kg[q];
-    if (c_surf(a, j)) {
-        const double lk = c_len3(kg);
-        for (int k = 0; k < 8; ++k) c_umax(&a.sg_d[idx[k]], lk);
+    const CWin w = c_window(valid, ijk[0], sbox);
+    for (int t = threadIdx.x; t < w.nodes; t += 256) {
+        s_near[t] = 0x7FF0000000000000ull;   // +inf
+        s_sg[t] = 0ull;
+    }
+    __syncthreads();
+    if (valid) {
+        const double usk = c_unscale(a, 4);
+        double kp = 0.0, kg[3] = {0.0, 0.0, 0.0};
+        unsigned long long dk[8];
+        CAx A;
+        c_axes(g, x, ijk[0], A);
+        for (int k = 0; k < 8; ++k) {
+            double xi[3], N, G[3];
+            c_corner(g, A, k, xi, N, G);
+            kp += N * ((double)a.acc[idx[k]] * usk);
+            for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + idx[k]];
+            const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
+            dk[k] = (unsigned long long)__double_as_longlong(c_len3(d));
+            if (w.lds) atomicMin(&s_near[c_wslot(w, ijk[k])], dk[k]);
+            else atomicMin(&a.near_d[idx[k]], dk[k]);
+        }
+        a.kp[j] = kp;
+        for (int q = 0; q < 3; ++q) a.kg[q * a.n_all + j] = kg[q];
+        if (c_surf(a, j)) {
+            const double lk = c_len3(kg);
+            if (lk > 0.0) {
+                const unsigned long long lb = (unsigned long long)__double_as_longlong(lk);
+                for (int k = 0; k < 8; ++k) {
+                    if (w.lds) atomicMax(&s_sg[c_wslot(w, ijk[k])], lb);
+                    else atomicMax(&a.sg_d[idx[k]], lb);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < w.nodes; t += 256) {
+        const unsigned long long dn = s_near[t], sg = s_sg[t];
+        if (dn == 0x7FF0000000000000ull && sg == 0ull) continue;
+        const long long I = c_wnode(g, w, t);
+        if (dn != 0x7FF0000000000000ull) atomicMin(&a.near_d[I], dn);
+        if (sg) atomicMax(&a.sg_d[I], sg);
     }
 }
 
@@ -339,9 +530,11 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
     const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
     const bool surf = c_surf(a, j) != 0;
     const unsigned long long lk = (unsigned long long)__double_as_longlong(c_len3(kg));
+    CAx A;
+    c_axes(g, x, ijk[0], A);
     for (int k = 0; k < 8; ++k) {
-        double xi[3];
-        c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
+        double xi[3], N, G[3];
+        c_corner(g, A, k, xi, N, G);
         const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
         if ((unsigned long long)__double_as_longlong(c_len3(d)) == a.near_d[idx[k]]) atomicMin(&a.near_i[idx[k]], id);
         if (surf && lk != 0ull && lk == a.sg_d[idx[k]]) atomicMin(&a.sg_i[idx[k]], id);
@@ -400,57 +593,99 @@ __global__ void __launch_bounds__(256) k_c_detect(CArgs<real> a)
 }
 
 // ---- C6: split P2G (…_MP_Contact.cpp:288-385); MASS=true: mass and grad m,
-// MASS=false: momentum (/ field mass, sqrt(3) gate) and force
+// MASS=false: momentum (/ field mass, sqrt(3) gate) and force.  Window slot
+// s of a node: MASS: field * 4 + {m, mg.xyz}; else field * 6 + {mv.xyz, f.xyz}.
+template <bool MASS>
+__device__ __forceinline__ long long c_p2g_off(int s)
+{
+    if (MASS) {
+        const int f = s >> 2, q = s & 3;
+        return q == 0 ? off_m(f) : off_mg(f, q - 1);
+    }
+    const int f = s / 6, q = s % 6;
+    return q < 3 ? off_mv(f, q) : off_f(f, q - 3);
+}
 template <typename real, bool MASS>
 __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
 {
+    constexpr int NF = MASS ? 8 : 12;
+    __shared__ int sbox[6];
+    __shared__ unsigned long long lds[kCWin * NF];
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= a.n_all) return;
     const GridParams &g = a.g;
-    double x[3];
-    c_pos(a, j, x);
+    double x[3] = {0.0, 0.0, 0.0};
     long long idx[8];
     int ijk[8][3];
-    if (!c_adjacent(g, x, idx, ijk)) return;
-    const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
-    double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
-    if (j >= a.n_def) {
-        const int d = j - a.n_def;
-        m = a.dc_m[d];
-        vol = a.dc_vol[d];
-        for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
-        for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
-    } else {
-        m = (double)a.p.mass[j];
-        double Fd[9];
-        for (int q = 0; q < 9; ++q) Fd[q] = (double)a.p.F[q][j];
-        vol = (double)(real)(det3(Fd) * (double)a.p.vol0[j]);
-        for (int q = 0; q < 3; ++q) {
-            v[q] = (double)a.p.v[q][j];
-            if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
-        }
-        for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
+    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
+    bool valid = j < a.n_all;
+    if (valid) {
+        c_pos(a, j, x);
+        valid = c_adjacent(g, x, idx, ijk);
     }
-    const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
-    const double um = c_unscale(a, 0);
-    for (int k = 0; k < 8; ++k) {
-        const long long I = idx[k];
-        double xi[3], N, G[3];
-        c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
-        c_bases(g, x, xi, N, G);
-        const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
-        const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
-        if (MASS) {
-            c_add(&a.acc[off_m(f) * a.nn + I], N * m, sc0);
-            for (int q = 0; q < 3; ++q) c_add(&a.acc[off_mg(f, q) * a.nn + I], G[q] * m, sc1);
+    const CWin w = c_window(valid, ijk[0], sbox);
+    for (int t = threadIdx.x; t < w.nodes * NF; t += 256) lds[t] = 0ull;
+    __syncthreads();
+    if (valid) {
+        const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
+        double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
+        if (j >= a.n_def) {
+            const int d = j - a.n_def;
+            m = a.dc_m[d];
+            vol = a.dc_vol[d];
+            for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
+            for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
         } else {
-            const double mI = (double)a.acc[off_m(f) * a.nn + I] * um;
-            if (c_lenm3(mI) > g.m_min)
-                for (int q = 0; q < 3; ++q) c_add(&a.acc[off_mv(f, q) * a.nn + I], N * (m * v[q]) / mI, sc0);
-            const double fr[3] = {-vol * (G[0] * s[0] + G[1] * s[3] + G[2] * s[5]) + N * fe[0],
-                                  -vol * (G[1] * s[1] + G[0] * s[3] + G[2] * s[4]) + N * fe[1],
-                                  -vol * (G[2] * s[2] + G[0] * s[5] + G[1] * s[4]) + N * fe[2]};
-            for (int q = 0; q < 3; ++q) c_add(&a.acc[off_f(f, q) * a.nn + I], fr[q], sc1);
+            m = (double)a.p.mass[j];
+            double Fd[9];
+            for (int q = 0; q < 9; ++q) Fd[q] = (double)a.p.F[q][j];
+            vol = (double)(real)(det3(Fd) * (double)a.p.vol0[j]);
+            for (int q = 0; q < 3; ++q) {
+                v[q] = (double)a.p.v[q][j];
+                if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
+            }
+            for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
+        }
+        const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
+        const double um = c_unscale(a, 0);
+        CAx A;
+        c_axes(g, x, ijk[0], A);
+        for (int k = 0; k < 8; ++k) {
+            const long long I = idx[k];
+            double xi[3], N, G[3];
+            c_corner(g, A, k, xi, N, G);
+            const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
+            const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
+            const int ws = w.lds ? c_wslot(w, ijk[k]) : 0;
+            long long iv[6];
+            int base, cnt = 0;
+            if (MASS) {
+                base = 4 * f;
+                iv[0] = __double2ll_rn((N * m) * sc0);
+                for (int q = 0; q < 3; ++q) iv[1 + q] = __double2ll_rn((G[q] * m) * sc1);
+                cnt = 4;
+            } else {
+                base = 6 * f;
+                const double mI = (double)a.acc[off_m(f) * a.nn + I] * um;
+                const bool gate = c_lenm3(mI) > g.m_min;
+                for (int q = 0; q < 3; ++q) iv[q] = gate ? __double2ll_rn((N * (m * v[q]) / mI) * sc0) : 0ll;
+                const double fr[3] = {-vol * (G[0] * s[0] + G[1] * s[3] + G[2] * s[5]) + N * fe[0],
+                                      -vol * (G[1] * s[1] + G[0] * s[3] + G[2] * s[4]) + N * fe[1],
+                                      -vol * (G[2] * s[2] + G[0] * s[5] + G[1] * s[4]) + N * fe[2]};
+                for (int q = 0; q < 3; ++q) iv[3 + q] = __double2ll_rn(fr[q] * sc1);
+                cnt = 6;
+            }
+            for (int q = 0; q < cnt; ++q) {
+                if (w.lds) atomicAdd(&lds[(base + q) * w.nodes + ws], (unsigned long long)iv[q]);
+                else atomicAdd((unsigned long long *)&a.acc[c_p2g_off<MASS>(base + q) * a.nn + I], (unsigned long long)iv[q]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < w.nodes; t += 256) {
+        const long long I = c_wnode(g, w, t);
+        for (int q = 0; q < NF; ++q) {
+            const unsigned long long v = lds[q * w.nodes + t];
+            if (v) atomicAdd((unsigned long long *)&a.acc[c_p2g_off<MASS>(q) * a.nn + I], v);
         }
     }
 }
@@ -599,11 +834,12 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
             const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
             double v[3], xn[3] = {x[0], x[1], x[2]}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             for (int q = 0; q < 3; ++q) v[q] = (double)P.v[q][j];
+            CAx A;
+            c_axes(g, x, ijk[0], A);
             for (int k = 0; k < 8; ++k) {
                 const long long I = idx[k];
                 double xi[3], N, G[3];
-                c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
-                c_bases(g, x, xi, N, G);
+                c_corner(g, A, k, xi, N, G);
                 const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
                 const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
                 const double m = a.gm[f * a.nn + I];

@@ -595,6 +595,56 @@ def config4_worlds(n=256, seed=20240607, h=0.0002, thickness=0.001, platen_speed
     return worlds
 
 
+def config3_world(h=2.0e-3 / 21, n_holes=8, pitch=2.0e-3, radius_frac=0.4, thickness=2.0e-3, mu=0.5e6,
+                  kappa=5.0e6, rho=1000.0, platen_speed=0.5, steps=10000, cfl=0.3, margin_cells=2):
+    """a swiss-cheese block, ``n_holes`` x ``n_holes``
+    circular holes of radius ``radius_frac`` x ``pitch`` through the
+    ``thickness``, Neo-Hookean (shear modulus ``mu``, bulk modulus ``kappa``:
+    E = 9 kappa mu / (3 kappa + mu), nu = (3 kappa - 2 mu) / (2 (3 kappa + mu))),
+    points at 2 x 2 x 2 per cell (about 2.35M at the default h = 2 mm / 21),
+    on the fixed bottom node plane, compressed by a separate point platen
+    moving down at ``platen_speed`` (stored velocity, as the contact step moves
+    platens).  Points within one point spacing of a hole or of the outer
+    boundary, and the platen's lower layer, are flagged as surface for the
+    contact step.  Run it on the contact path (friction 0.2 in the benchmark)."""
+    off = 0.5 * h
+    W = n_holes * pitch
+    extent = (W + 2 * margin_cells * h, W + (2 + margin_cells) * h, thickness)
+    L, cells = _grid_for(extent, (h, h, h))
+    L = (L[0], L[1], thickness)
+    cells = (cells[0], cells[1], max(1, int(round(thickness / h))))
+    x0 = 0.5 * (L[0] - W)
+    xs = x0 + off * (0.5 + np.arange(int(round(W / off))))
+    ys = off * (0.5 + np.arange(int(round(W / off))))
+    zs = off * (0.5 + np.arange(int(round(thickness / off))))
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    R = radius_frac * pitch
+    ix = np.clip(((pts[:, 0] - x0) // pitch).astype(int), 0, n_holes - 1)
+    iy = np.clip((pts[:, 1] // pitch).astype(int), 0, n_holes - 1)
+    d = np.hypot(pts[:, 0] - (x0 + pitch * (ix + 0.5)), pts[:, 1] - pitch * (iy + 0.5))
+    keep = d > R
+    body, d = pts[keep], d[keep]
+    edge = np.minimum.reduce([body[:, 0] - x0, x0 + W - body[:, 0], body[:, 1], W - body[:, 1]])
+    surf_body = ((d - R < off) | (edge < off)).astype(np.uint8)
+    E = 9.0 * kappa * mu / (3.0 * kappa + mu)
+    nu = (3.0 * kappa - 2.0 * mu) / (2.0 * (3.0 * kappa + mu))
+    vol = off ** 3
+    mat = _material_arrays(len(body), E, nu, 0.0, 0.0, 0.0, NEOHOOKEAN, rho, vol)
+    top = create_domain_cuboid((0.5 * L[0], W + 0.5 * h, 0.5 * thickness), (W, 2.0 * off, thickness), off)
+    plat = _material_arrays(len(top), 210.0e9, 0.33, 200.0e6, 0.0, 0.0, ELASTIC, 7800.0, vol)
+    dt = cfl * h / p_wave_speed(E, nu, rho)
+    sc = _assemble(L, cells, reference_bcs(L, cells), [(body, mat, False), (top, plat, True)],
+                   dt_max=dt, time_end=steps * dt, damping=0.0, mass_min=0.001 * rho * vol,
+                   timeline_t=np.array([0.0, 1.0e3]),
+                   timeline_v=np.array([[0.0, -platen_speed, 0.0], [0.0, -platen_speed, 0.0]]), neohookean=1)
+    sc.v[sc.dc == 1] = (0.0, -platen_speed, 0.0)
+    sc.surface = np.concatenate([surf_body, (top[:, 1] < W + 0.5 * h).astype(np.uint8)])
+    sc.info = dict(name="config3", h=h, dt=dt, E=E, nu=nu, holes=n_holes, pitch=pitch, radius=R,
+                   platen_speed=platen_speed, steps=steps)
+    return sc
+
+
 def log_uniform_moduli(n, seed, lo=0.5e9, hi=2.0e9):
     """``n`` Young's moduli log-uniform in [lo, hi] from SplitMix64(seed)."""
     rnd = splitmix64(seed)
@@ -705,6 +755,7 @@ def worlds():
         "translation": translation_world,
         "config1": config1_world,
         "config2": config2_world,
+        "config3": config3_world,
         "config4": lambda **kw: batch_worlds(config4_worlds(**kw)),
         "config5": config5_world,
     }
