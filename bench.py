@@ -120,6 +120,44 @@ def cpu_baseline(sc, seconds=15.0):
                 cpu_model=_cpu_model(), nproc=os.cpu_count())
 
 
+def field_errors(m, sc, steps=100):
+    """Relative L2 error of every reported point field after ``steps`` steps,
+    for the FP32-state and the FP64 GPU runs, against the FP64 oracle (the
+    CPU restatement of the reference step, all host threads) from the same
+    initial state: displacement, velocity, F - I, stress, plastic strain,
+    alpha.  100 steps is where the platen ramp ends; earlier, strains are
+    ~1e-8 and below the resolution of FP32 state."""
+    from oracle.oracle import Oracle
+    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)))
+    orc = Oracle(sc, nthreads=threads, strain_form=1)
+    for _ in range(steps):
+        orc.step(sc.dt_max)
+        orc.w.time += sc.dt_max
+    o = orc.points()
+    x0 = sc.x
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64).ravel()
+        b = np.asarray(b, np.float64).ravel()
+        d = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a))
+
+    out = {"steps": steps, "vs": "FP64 oracle (CPU restatement of the reference step, L dt strain form)",
+           "oracle_threads": threads}
+    keep = sc.dc == 0
+    for name, prec in (("fp32", m.FP32), ("fp64", m.FP64)):
+        eng = m.PhysicsEngine(sc, precision=prec, track_strain=False, strain_form=1)
+        eng.run_steps(steps)
+        g = eng.getMaterialPoints()
+        eng.close()
+        out[name] = {"disp": rel((g["x"] - x0)[keep], (o["x"] - x0)[keep]), "v": rel(g["v"][keep], o["v"][keep]),
+                     "F": rel(g["F"][keep] - np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0]),
+                              o["F"][keep] - np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0])),
+                     "stress": rel(g["stress"][keep], o["stress"][keep]),
+                     "eps_p": rel(g["eps_p"][keep], o["eps_p"][keep]), "alpha": rel(g["alpha"][keep], o["alpha"][keep])}
+    return out
+
+
 def batched_run(m, steps, warmup):
     """Config 4: 256 independent cells (SplitMix64 seed 20240607) packed into
     one grid, FP32, one launch per step for all cells; aggregate
@@ -311,6 +349,7 @@ def main():
         out["adjoint_over_forward"] = c5["adjoint_over_forward"]
         out["adjoint"] = c5
     if not args.no_cpu:
+        out["field_errors"] = field_errors(m, sc)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(kBlock) k_readback(GridParams g, PArrays<real>
     }
     if (mask & 2u) for (int a = 0; a < 3; ++a) o.v[3 * (size_t)id + a] = (double)p.v[a][j];
     double Fd[9];
-    for (int a = 0; a < 9; ++a) Fd[a] = (double)p.F[a][j];
+    for (int a = 0; a < 9; ++a) Fd[a] = fd(p.F[a][j], a);
     if (mask & 4u) for (int a = 0; a < 9; ++a) o.F[9 * (size_t)id + a] = Fd[a];
     if (mask & 8u) for (int a = 0; a < 6; ++a) o.s[6 * (size_t)id + a] = (double)p.s[a][j];
     if ((mask & 16u) && g.track_strain) for (int a = 0; a < 6; ++a) o.eps[6 * (size_t)id + a] = (double)p.eps[a][j];

@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(256) k_c_bounds(CArgs<real> a)
         } else {
             m = (double)a.p.mass[j];
             double Fd[9];
-            for (int q = 0; q < 9; ++q) Fd[q] = (double)a.p.F[q][j];
+            for (int q = 0; q < 9; ++q) Fd[q] = fd(a.p.F[q][j], q);
             vol = det3(Fd) * (double)a.p.vol0[j];
             for (int q = 0; q < 3; ++q) {
                 v[q] = (double)a.p.v[q][j];
@@ -637,7 +637,7 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
         } else {
             m = (double)a.p.mass[j];
             double Fd[9];
-            for (int q = 0; q < 9; ++q) Fd[q] = (double)a.p.F[q][j];
+            for (int q = 0; q < 9; ++q) Fd[q] = fd(a.p.F[q][j], q);
             vol = (double)(real)(det3(Fd) * (double)a.p.vol0[j]);
             for (int q = 0; q < 3; ++q) {
                 v[q] = (double)a.p.v[q][j];
@@ -857,10 +857,11 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
             // F += L F dt, V = det(F) V0 (…_MP.cpp:299-302), then the material update
             real F[9], Lr[9], s[6];
             for (int q = 0; q < 9; ++q) { F[q] = P.F[q][j]; Lr[q] = (real)L[q]; }
-            real T[9];
+            real T[9];   // FP32 state stores D = F - I: D += (L + L D) dt
             for (int c = 0; c < 3; ++c)
                 for (int r = 0; r < 3; ++r)
-                    T[3 * c + r] = Lr[r] * F[3 * c] + Lr[3 + r] * F[3 * c + 1] + Lr[6 + r] * F[3 * c + 2];
+                    T[3 * c + r] = (FP32 ? Lr[3 * c + r] : real(0)) +
+                                   (Lr[r] * F[3 * c] + Lr[3 + r] * F[3 * c + 1] + Lr[6 + r] * F[3 * c + 2]);
             for (int q = 0; q < 9; ++q) F[q] += T[q] * (real)dt;
             for (int q = 0; q < 9; ++q) P.F[q][j] = F[q];
             for (int q = 0; q < 6; ++q) s[q] = P.s[q][j];

@@ -93,6 +93,21 @@ __device__ __forceinline__ T &elt(T *base, uint32_t i)
     return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T));
 }
 
+// Deformation gradient storage.  FP64 state keeps F; FP32 state keeps
+// D = F - I, so that per-step increments (L F dt ~ 1e-8) are not rounded
+// away against the 1.0 on the diagonal.  fd() returns component q of F in
+// FP64 from the stored value; fstore() converts F to the stored form.
+template <typename real>
+__device__ __host__ __forceinline__ double fd(real stored, int q)
+{
+    return (double)stored + ((sizeof(real) == 4 && (q % 4) == 0) ? 1.0 : 0.0);
+}
+template <typename real>
+__device__ __host__ __forceinline__ real fstore(double F, int q)
+{
+    return (real)(F - ((sizeof(real) == 4 && (q % 4) == 0) ? 1.0 : 0.0));
+}
+
 template <typename real>
 struct NodeRec {
     real d[8];

@@ -1254,7 +1254,10 @@ static int upload_t(mpm_handle *h, const mpm_points *pts)
         if (h->has_fext) RCHK(put(P.fext[a], [&](int i) { return pts->fext[3 * i + a]; }));
     }
     for (int a = 0; a < 9; ++a)
-        RCHK(put(P.F[a], [&](int i) { return pts->F ? pts->F[9 * i + a] : ((a % 4) == 0 ? 1.0 : 0.0); }));
+        RCHK(put(P.F[a], [&](int i) {   // stored form: F - I in FP32 state
+            const double F = pts->F ? pts->F[9 * i + a] : ((a % 4) == 0 ? 1.0 : 0.0);
+            return (double)fstore<real>(F, a);
+        }));
     for (int a = 0; a < 6; ++a) {
         RCHK(put(P.s[a], [&](int i) { return pts->stress ? pts->stress[6 * i + a] : 0.0; }));
         RCHK(put(P.ep[a], [&](int i) { return pts->eps_p ? pts->eps_p[6 * i + a] : 0.0; }));

@@ -385,6 +385,7 @@ struct KpArgs {
 
 // Strain increment, total strain, constitutive update and stress write of
 // point i (…_MP.cpp:304-353), shared by the plain and the contact step.
+// F is the stored deformation gradient (F - I in FP32 state, see fd()).
 template <typename real>
 __device__ __forceinline__ void material_update(const GridParams &g, const PArrays<real> &P, const Material *mats,
                                                 uint32_t i, const real L[9], real dt, const real F[9], real s[6])
@@ -439,7 +440,7 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
         }
     } else if (type == 1) {
         double Fd[9], sd[6], ds[6];
-        for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
+        for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
         for (int q = 0; q < 6; ++q) sd[q] = (double)s[q];
         neohookean_inc(E, mt.nu, sd, Fd, ds);
         for (int q = 0; q < 6; ++q) s[q] = (real)(sd[q] + ds[q]);
@@ -779,20 +780,21 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                                 for (int r = 0; r < 3; ++r) L[3 * cc + r] += wp[r] * G[cc];
                         }
                     }
-                    // F += (L F) dt  (…_MP.cpp:299)
+                    // F += (L F) dt  (…_MP.cpp:299); FP32 state: D += (L + L D) dt
                     real T[9];
 #pragma unroll
                     for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
                         for (int r = 0; r < 3; ++r)
-                            T[3 * cc + r] = L[r] * F[3 * cc] + L[3 + r] * F[3 * cc + 1] + L[6 + r] * F[3 * cc + 2];
+                            T[3 * cc + r] = (FP32 ? L[3 * cc + r] : real(0)) +
+                                            (L[r] * F[3 * cc] + L[3 + r] * F[3 * cc + 1] + L[6 + r] * F[3 * cc + 2]);
 #pragma unroll
                     for (int q = 0; q < 9; ++q) F[q] += T[q] * dt;
                     // volume from the new F (…_MP.cpp:301-302), FP64
                     {
                         double Fd[9];
 #pragma unroll
-                        for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
+                        for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
                         V = (real)(det3(Fd) * (double)V0);
                     }
                     // write back the kinematic state before the material update
@@ -809,7 +811,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             } else {
                 double Fd[9];
 #pragma unroll
-                for (int q = 0; q < 9; ++q) Fd[q] = (double)F[q];
+                for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
                 V = (real)(det3(Fd) * (double)V0);
             }
             // ---- FP64 checks of the new state (PhysicsEngine.cpp:73-79)
