@@ -9,6 +9,8 @@ reference reads the same:
 * ``getTime_Current / getTime_End / getTime_Increment / getTime_Runtime`` (:57-60)
 * ``getCount_MaterialPoint / getCount_GridPoint`` (:62-63)
 * ``reportConsole(description)`` -> the console/log line (PhysicsEngine.cpp:33-120)
+* ``getScript_MaterialPoints / getScript_GridPoints`` -> the factories' text dumps
+  (MaterialPoint_Factory_getScript.cpp:4-28, GridPoint_Factory.h:69-100)
 
 All compute runs in the HIP library; this module only marshals arrays.
 """
@@ -16,7 +18,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import _abi
+from . import _abi, dump
 from ._abi import MpmConfig, MpmPoints, check, lib
 
 FP64 = _abi.MPM_FP64
@@ -221,6 +223,24 @@ class PhysicsEngine:
     def getCount_GridPoint(self):
         nn = self.scene.nodes
         return nn[0] * nn[1] * nn[2]
+
+    # ------------------------------------------------------------ text dumps
+    def getScript_MaterialPoints(self):
+        """MaterialPoint_Factory::getScript (MaterialPoint_Factory_getScript.cpp:4-28)
+        over every point at the current state, header first."""
+        return dump.material_points_script(self.getMaterialPoints(_abi.MPM_F_X)["x"], self.scene.mass)
+
+    def getScript_GridPoints(self, active_only=True):
+        """GridPoint_Factory::getScript (GridPoint_Factory.h:69-100) over the
+        grid nodes (those with mass only, by default), header first; node mass
+        is the P2G mass of the points' current positions."""
+        return dump.grid_points_script(self.scene, self.getMaterialPoints(_abi.MPM_F_X)["x"], active_only)
+
+    def saveLatex(self, strPrefix, strPostfix=".txt"):
+        """The points' getScript table written to ``strPrefix + strPostfix``
+        (MaterialPoint_Factory::saveLatex, declared at MaterialPoint_Factory.h:41
+        and never defined by the reference); returns the path."""
+        return dump.save_latex(self.getMaterialPoints(_abi.MPM_F_X)["x"], self.scene.mass, strPrefix, strPostfix)
 
 
 def sort_keys(keys, device=0):

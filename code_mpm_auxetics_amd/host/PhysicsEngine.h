@@ -13,6 +13,7 @@
 #pragma once
 
 #include <array>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -51,6 +52,19 @@ class MaterialPoint {
     double d_Hardening_Isotropic_C0 = 0.0;
     double d_Hardening_Isotropic_C1 = 0.0;
 };
+
+// MaterialPoint_Factory::getScript (MaterialPoint_Factory_getScript.cpp:4-28):
+// the header line for nullptr, else "ID mass position_x position_y position_z",
+// tab-separated with a trailing tab; doubles as Script(d, 8) writes them
+// (Definitions.h:43-58: scientific, with a sign, 8 digits after the point).
+inline std::string getScript(const MaterialPoint *p)
+{
+    if (p == nullptr) return "ID\tmass\t\t\tposition_x\t\tposition_y\t\tposition_z\t\t\n";
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%u\t%+.8e\t%+.8e\t%+.8e\t%+.8e\t\n", p->i_ID, p->d_Mass, p->d3_Position[0],
+                  p->d3_Position[1], p->d3_Position[2]);
+    return buf;
+}
 
 class PhysicsEngine {
   public:

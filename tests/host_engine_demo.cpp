@@ -2,8 +2,9 @@
 // way the reference's game loop drives PhysicsEngine
 // (GraphicsEngine_gameLoop.cpp:26-31): a small elastic block in free flight,
 // runSimulation_Classic_SinglePass_MP(10 * dt) until finished, then prints
-// the mean position and velocity.  Exit code 3 and "no device" when no GPU is
-// visible (the constructor throws MPM_E_NO_DEVICE).
+// the mean position and velocity, then the getScript header and first row.
+// Exit code 3 and "no device" when no GPU is visible (the constructor throws
+// MPM_E_NO_DEVICE).
 #include <cstdio>
 
 #include "../code_mpm_auxetics_amd/host/PhysicsEngine.h"
@@ -47,6 +48,7 @@ int main()
             }
         std::printf("%d %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", (int)mp.size(), eng.getTime_Current(), mx[0], mx[1],
                     mx[2], mv[0], mv[1], mv[2]);
+        std::printf("%s%s", mpm_host::getScript(nullptr).c_str(), mpm_host::getScript(mp[0]).c_str());
         return 0;
     } catch (const mpm_host::Error &e) {
         std::printf("%s\n", e.what());

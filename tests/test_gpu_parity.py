@@ -257,6 +257,12 @@ def test_cpp_host_mirror_free_flight(mpm, tmp_path):
     assert n == 256 and t >= 40e-6 * (1 - 1e-12) and t <= 50e-6 * (1 + 1e-12)
     assert np.allclose(mv, v, rtol=0, atol=1e-12)
     assert np.allclose(mx, x0 + v * t, rtol=0, atol=1e-12)
+    # MaterialPoint_Factory::getScript of the first point (ID 0, mass, position)
+    lines = r.stdout.splitlines(keepends=True)
+    assert lines[1] == "ID\tmass\t\t\tposition_x\t\tposition_y\t\tposition_z\t\t\n"
+    row = lines[2].split("\t")
+    assert row[0] == "0" and row[-1] == "\n" and len(row) == 6
+    assert float(row[1]) == float("%+.8e" % (1000.0 * 0.125e-9))
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
