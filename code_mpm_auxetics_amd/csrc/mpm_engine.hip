@@ -81,6 +81,8 @@ struct mpm_handle {
     // bricks
     uint32_t *occ_list = nullptr, *occ_start = nullptr, *occ_count = nullptr;
     int32_t *n_occ_d = nullptr, *brick_slot = nullptr;
+    uint32_t *slot_order = nullptr;          // launch order of the brick blocks
+    std::vector<uint32_t> slot_order_h, occ_count_h;
     int n_occ = 0, occ_cap = 0;
     void *arena[2] = {nullptr, nullptr};
     int4 *binfo[2] = {nullptr, nullptr};    // per brick (dense): {slot, packed touched node box}
@@ -401,6 +403,17 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     k_occ_count<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->occ_list, h->occ_start, n_occ, n,
                                                               h->occ_count, h->brick_slot);
     HIPCHK(h, hipGetLastError());
+    // launch order of the brick blocks: most point chunks first, ties in slot
+    // order (a longest-first schedule; every slot still sums in slot order)
+    h->occ_count_h.resize(n_occ);
+    h->slot_order_h.resize(n_occ);
+    HIPCHK(h, hipMemcpyAsync(h->occ_count_h.data(), h->occ_count, (size_t)n_occ * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (uint32_t k = 0; k < n_occ; ++k) h->slot_order_h[k] = k;
+    std::stable_sort(h->slot_order_h.begin(), h->slot_order_h.end(), [h](uint32_t x, uint32_t y) {
+        return cdiv(h->occ_count_h[x], kPBlock) > cdiv(h->occ_count_h[y], kPBlock);
+    });
+    HIPCHK(h, hipMemcpyAsync(h->slot_order, h->slot_order_h.data(), (size_t)n_occ * 4, hipMemcpyHostToDevice, h->stream));
     // spread each brick's points over the lanes (permutation inside bricks)
     uint32_t *perm2 = (perm == h->vals[0]) ? h->vals[1] : h->vals[0];
     k_spread<<<nbk, kBlock, 0, h->stream>>>(perm, h->heads, h->headpos, h->occ_start, h->occ_count, n, perm2);
@@ -466,6 +479,7 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.occ_start = h->occ_start;
     a.occ_count = h->occ_count;
     a.n_occ = h->n_occ_d;
+    a.slot_order = h->slot_order;
     a.occ_cap = h->n_occ;    // brick blocks first, platen blocks after them
     a.mats = h->mats;
     a.steps = h->steps_d;
@@ -997,7 +1011,8 @@ static void destroy_all(mpm_handle *h)
     void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v[0], (void **)&h->dc_v[1], (void **)&h->dc_m, &h->zero_rec,
                      (void **)&h->fixed, (void **)&h->marks[0], (void **)&h->marks[1], (void **)&h->occ_list,
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d, (void **)&h->brick_slot,
-                     (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats, (void **)&h->steps_d,
+                     (void **)&h->slot_order, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
+                     (void **)&h->steps_d,
                      (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->hist_d, (void **)&h->grid3_d,
                      (void **)&h->fy_slot, (void **)&h->fy_off, (void **)&h->fy_val, (void **)&h->ghist_d,
                      (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
@@ -1130,7 +1145,7 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     }
     const size_t nb = (size_t)h->nbricks;
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
-        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) ||
+        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) || (rc = dalloc(h, &h->slot_order, nb)) ||
         (rc = dalloc(h, &h->binfo[0], nb)) || (rc = dalloc(h, &h->binfo[1], nb)) ||
         (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)) || (rc = dalloc(h, &h->grid3_d, 3)))
         return fail(rc);

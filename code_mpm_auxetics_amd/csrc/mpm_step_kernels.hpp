@@ -351,6 +351,29 @@ __device__ __forceinline__ uint32_t wave_umax_dpp(uint32_t v)
     const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
     return max(max(r0, r1), max(r2, r3));
 }
+// Box bounds over the wave through wave_umax_dpp: lo in [0, 2^20], hi >= -1.
+__device__ __forceinline__ int wave_imin_dpp(int v) { return (1 << 20) - (int)wave_umax_dpp((uint32_t)((1 << 20) - v)); }
+__device__ __forceinline__ int wave_imax_dpp(int v) { return (int)wave_umax_dpp((uint32_t)(v + 1)) - 1; }
+
+// Sum of a double over the wave: inclusive DPP row scans of both 32-bit
+// halves (lanes without a source read +0.0), then the four row totals added
+// as (r0 + r1) + (r2 + r3).  A fixed order, so the result is reproducible;
+// wave-uniform.  Call with every lane of the wave active.
+__device__ __forceinline__ double lane_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFFll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v)
+{
+    v += dpp_d<MPM_ROW_SHR(1)>(v);
+    v += dpp_d<MPM_ROW_SHR(2)>(v);
+    v += dpp_d<MPM_ROW_SHR(4)>(v);
+    v += dpp_d<MPM_ROW_SHR(8)>(v);
+    return (lane_d(v, 15) + lane_d(v, 31)) + (lane_d(v, 47) + lane_d(v, 63));
+}
 
 constexpr int kUnsetExp = -100000;
 constexpr int kMinExp = -90;   // contributions below 2^-90 of a unit are dropped
@@ -368,6 +391,7 @@ struct KpArgs {
     const unsigned long long *marks_in;   // platen override tags of step n
     unsigned long long *marks_out;        // ... of step n+1
     const uint32_t *occ_list, *occ_start, *occ_count;
+    const uint32_t *slot_order;       // brick block b works on slot slot_order[b]
     const int32_t *n_occ;
     int32_t occ_cap;
     const Material *mats;
@@ -540,8 +564,10 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         return;
     }
 
-    const int slot = blockIdx.x;
-    if (slot >= *a.n_occ) return;
+    // brick blocks are launched longest first (most point chunks), which
+    // shortens the tail of the launch; results do not depend on the order
+    if ((int)blockIdx.x >= *a.n_occ) return;
+    const int slot = (int)a.slot_order[blockIdx.x];
     const uint32_t brick = a.occ_list[slot];
     const int bc[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
     const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
@@ -570,61 +596,97 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         }
         const int nwin = wext[0] * wext[1] * wext[2];
         const real dtg = (real)st.dt, m_min = (real)g.m_min, damp = (real)g.damping;
+        // Pass 1: for each window node, the sum of the covering arenas'
+        // records (at most 8 candidates, ascending brick order, uncovered ones
+        // read a zero record), its fixity and override tag, stashed in LDS:
+        // sums in the (not yet used) arena, {tag, fixity | counted} in the
+        // node's window slot.  Branch-free, kU nodes per thread at a time, so
+        // that every global load of an iteration is in flight at once.  Each
+        // thread later reads back only the slots it wrote: no barrier.
+        real *s_sum = reinterpret_cast<real *>(s_arena);
+        constexpr int kU = FP32 ? 2 : 1;
+        for (int r0 = tid; r0 < nwin; r0 += kU * kPBlock) {
+            real sum[kU][kQ];
+            unsigned long long mk[kU];
+            uint32_t fw[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int r = (r0 + u * kPBlock < nwin) ? r0 + u * kPBlock : r0;
+                const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
+                const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
+                // per axis the arenas covering this node: own and the neighbour
+                // below (l <= 2) or above (l >= S)
+                int lower[3];
+                bool two[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const bool lo_n = l[q] <= 2 && bc[q] > 0, hi_n = l[q] >= g.S[q] && bc[q] + 1 < g.nb[q];
+                    lower[q] = lo_n ? -1 : 0;
+                    two[q] = lo_n || hi_n;
+                }
+                const real *src[8];
+                int first = -1;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int e[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+                    int dd[3];
+                    bool ok = true;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        dd[q] = lower[q] + e[q];
+                        ok &= !(e[q] && !two[q]);
+                    }
+                    const int4 bi = s_bi[(dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1)];
+                    const int pk[3] = {bi.y, bi.z, bi.w};
+                    int lp[3], lo[3], ext[3];
+                    ok &= bi.x >= 0;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        lp[q] = l[q] - dd[q] * g.S[q];   // node in that arena's window
+                        lo[q] = pk[q] & 0xFFFF;
+                        const int hi = pk[q] >> 16;
+                        ext[q] = hi - lo[q] + 1;
+                        ok &= lp[q] >= lo[q] && lp[q] <= hi;
+                    }
+                    const int rr = (lp[0] - lo[0]) + ext[0] * ((lp[1] - lo[1]) + ext[1] * (lp[2] - lo[2]));
+                    src[c] = ok ? a.arena_in + (size_t)bi.x * 8 * AN + 8 * (size_t)rr : a.zero_rec;
+                    first = (ok && first < 0) ? c : first;
+                }
+                const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
+                // counted once in the grid sums: by the lowest covering brick
+                fw[u] = (uint32_t)a.fixed[nid] | ((first == cown) ? 0x100u : 0u);
+                mk[u] = a.marks_in[nid];
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) sum[u][q] = real(0);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    real rec[8];
+                    load8(src[c], rec);
+#pragma unroll
+                    for (int q = 0; q < kQ; ++q) sum[u][q] += rec[q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int r = r0 + u * kPBlock;
+                if (r < nwin) {
+#pragma unroll
+                    for (int q = 0; q < kQ; ++q) s_sum[kQ * r + q] = sum[u][q];
+                    unsigned long long *wt = reinterpret_cast<unsigned long long *>(s_node + 6 * r);
+                    wt[0] = mk[u];
+                    wt[1] = fw[u];
+                }
+            }
+        }
+        // Pass 2: the grid update of each window node (…_MP.cpp:209-265)
         double gs[3] = {0.0, 0.0, 0.0};   // owned nodes: mass, reaction, active count
         for (int r = tid; r < nwin; r += kPBlock) {
-            const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
-            const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
-            const uint8_t fx = a.fixed[nid];
-            const unsigned long long mk = a.marks_in[nid];
-            // per axis the arenas covering this node: own and the neighbour below
-            // (l <= 2) or above (l >= S); candidates in ascending brick order
-            int lower[3];
-            bool two[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const bool lo_n = l[q] <= 2 && bc[q] > 0, hi_n = l[q] >= g.S[q] && bc[q] + 1 < g.nb[q];
-                lower[q] = lo_n ? -1 : 0;
-                two[q] = lo_n || hi_n;
-            }
-            const real *src[8];
-            int first = -1;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int e[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
-                int dd[3];
-                bool ok = true;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    dd[q] = lower[q] + e[q];
-                    ok &= !(e[q] && !two[q]);
-                }
-                const int4 bi = s_bi[(dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1)];
-                const int pk[3] = {bi.y, bi.z, bi.w};
-                int lp[3], lo[3], ext[3];
-                ok &= bi.x >= 0;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    lp[q] = l[q] - dd[q] * g.S[q];   // node in that arena's window
-                    lo[q] = pk[q] & 0xFFFF;
-                    const int hi = pk[q] >> 16;
-                    ext[q] = hi - lo[q] + 1;
-                    ok &= lp[q] >= lo[q] && lp[q] <= hi;
-                }
-                const int rr = (lp[0] - lo[0]) + ext[0] * ((lp[1] - lo[1]) + ext[1] * (lp[2] - lo[2]));
-                src[c] = ok ? a.arena_in + (size_t)bi.x * 8 * AN + 8 * (size_t)rr : a.zero_rec;
-                if (ok && first < 0) first = c;
-            }
-            const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
+            const unsigned long long *wt = reinterpret_cast<const unsigned long long *>(s_node + 6 * r);
+            const unsigned long long mk = wt[0];
+            const uint32_t fw = (uint32_t)wt[1], fx = fw & 0xFFu;
             real sum[kQ];
 #pragma unroll
-            for (int q = 0; q < kQ; ++q) sum[q] = real(0);
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                real rec[8];
-                load8(src[c], rec);
-#pragma unroll
-                for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
-            }
+            for (int q = 0; q < kQ; ++q) sum[q] = s_sum[kQ * r + q];
             const real mn = sum[0];
             real vn[3], fn[3] = {sum[4], sum[5], sum[6]};
 #pragma unroll
@@ -655,16 +717,20 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                 wn[q] = vn[q];
                 wn[3 + q] = (fabs(mn) > m_min) ? fn[q] / mn : real(0);
             }
-            if (first == cown) {   // counted once: by the lowest covering brick
+            if (fw & 0x100u) {
                 gs[0] += (double)mn;
                 gs[1] += react;
                 gs[2] += (mn > real(0)) ? 1.0 : 0.0;
-                if (a.fy_slot && (fx & 2)) a.fy_val[a.fy_slot[nid]] = react;
+                if (a.fy_slot && (fx & 2)) {
+                    const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
+                    const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
+                    a.fy_val[a.fy_slot[nid]] = react;
+                }
             }
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double w = wave_sum(gs[k]);
+            const double w = wave_sum_dpp(gs[k]);
             if (lane == 0) s_red[wv][k] = w;
         }
         __syncthreads();
@@ -952,7 +1018,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     // ---- touched node box of the brick
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const int lo = wave_imin(bmin[q]), hi = wave_imax(bmax[q]);
+        const int lo = wave_imin_dpp(bmin[q]), hi = wave_imax_dpp(bmax[q]);
         if (lane == 0) { s_box[wv][2 * q] = lo; s_box[wv][2 * q + 1] = hi; }
     }
     __syncthreads();
@@ -992,7 +1058,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     // ---- fixed-order block sums of the checks, flags
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        const double wsum = wave_sum(red[k]);
+        const double wsum = wave_sum_dpp(red[k]);
         if (lane == 0) s_red[wv][k] = wsum;
     }
     int fl = flags;
