@@ -600,83 +600,70 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         // records (at most 8 candidates, ascending brick order, uncovered ones
         // read a zero record), its fixity and override tag, stashed in LDS:
         // sums in the (not yet used) arena, {tag, fixity | counted} in the
-        // node's window slot.  Branch-free, kU nodes per thread at a time, so
-        // that every global load of an iteration is in flight at once.  Each
-        // thread later reads back only the slots it wrote: no barrier.
+        // node's window slot.  Branch-free, so that all loads of a node are in
+        // flight at once.  Each thread later reads back only the slots it
+        // wrote: no barrier.
         real *s_sum = reinterpret_cast<real *>(s_arena);
-        constexpr int kU = FP32 ? 2 : 1;
-        for (int r0 = tid; r0 < nwin; r0 += kU * kPBlock) {
-            real sum[kU][kQ];
-            unsigned long long mk[kU];
-            uint32_t fw[kU];
+        for (int r = tid; r < nwin; r += kPBlock) {
+            const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
+            const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
+            // per axis the arenas covering this node: own and the neighbour
+            // below (l <= 2) or above (l >= S)
+            int lower[3];
+            bool two[3];
 #pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int r = (r0 + u * kPBlock < nwin) ? r0 + u * kPBlock : r0;
-                const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
-                const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
-                // per axis the arenas covering this node: own and the neighbour
-                // below (l <= 2) or above (l >= S)
-                int lower[3];
-                bool two[3];
+            for (int q = 0; q < 3; ++q) {
+                const bool lo_n = l[q] <= 2 && bc[q] > 0, hi_n = l[q] >= g.S[q] && bc[q] + 1 < g.nb[q];
+                lower[q] = lo_n ? -1 : 0;
+                two[q] = lo_n || hi_n;
+            }
+            const real *src[8];
+            int first = -1;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int e[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+                int dd[3];
+                bool ok = true;
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const bool lo_n = l[q] <= 2 && bc[q] > 0, hi_n = l[q] >= g.S[q] && bc[q] + 1 < g.nb[q];
-                    lower[q] = lo_n ? -1 : 0;
-                    two[q] = lo_n || hi_n;
+                    dd[q] = lower[q] + e[q];
+                    ok &= !(e[q] && !two[q]);
                 }
-                const real *src[8];
-                int first = -1;
+                const int4 bi = s_bi[(dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1)];
+                const int pk[3] = {bi.y, bi.z, bi.w};
+                int lp[3], lo[3], ext[3];
+                ok &= bi.x >= 0;
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int e[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
-                    int dd[3];
-                    bool ok = true;
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        dd[q] = lower[q] + e[q];
-                        ok &= !(e[q] && !two[q]);
-                    }
-                    const int4 bi = s_bi[(dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1)];
-                    const int pk[3] = {bi.y, bi.z, bi.w};
-                    int lp[3], lo[3], ext[3];
-                    ok &= bi.x >= 0;
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        lp[q] = l[q] - dd[q] * g.S[q];   // node in that arena's window
-                        lo[q] = pk[q] & 0xFFFF;
-                        const int hi = pk[q] >> 16;
-                        ext[q] = hi - lo[q] + 1;
-                        ok &= lp[q] >= lo[q] && lp[q] <= hi;
-                    }
-                    const int rr = (lp[0] - lo[0]) + ext[0] * ((lp[1] - lo[1]) + ext[1] * (lp[2] - lo[2]));
-                    src[c] = ok ? a.arena_in + (size_t)bi.x * 8 * AN + 8 * (size_t)rr : a.zero_rec;
-                    first = (ok && first < 0) ? c : first;
+                for (int q = 0; q < 3; ++q) {
+                    lp[q] = l[q] - dd[q] * g.S[q];   // node in that arena's window
+                    lo[q] = pk[q] & 0xFFFF;
+                    const int hi = pk[q] >> 16;
+                    ext[q] = hi - lo[q] + 1;
+                    ok &= lp[q] >= lo[q] && lp[q] <= hi;
                 }
-                const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
-                // counted once in the grid sums: by the lowest covering brick
-                fw[u] = (uint32_t)a.fixed[nid] | ((first == cown) ? 0x100u : 0u);
-                mk[u] = a.marks_in[nid];
+                const int rr = (lp[0] - lo[0]) + ext[0] * ((lp[1] - lo[1]) + ext[1] * (lp[2] - lo[2]));
+                src[c] = ok ? a.arena_in + (size_t)bi.x * 8 * AN + 8 * (size_t)rr : a.zero_rec;
+                first = (ok && first < 0) ? c : first;
+            }
+            const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
+            // counted once in the grid sums: by the lowest covering brick
+            const uint32_t fw = (uint32_t)a.fixed[nid] | ((first == cown) ? 0x100u : 0u);
+            const unsigned long long mk = a.marks_in[nid];
+            real sum[kQ];
 #pragma unroll
-                for (int q = 0; q < kQ; ++q) sum[u][q] = real(0);
+            for (int q = 0; q < kQ; ++q) sum[q] = real(0);
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    real rec[8];
-                    load8(src[c], rec);
+            for (int c = 0; c < 8; ++c) {
+                real rec[8];
+                load8(src[c], rec);
 #pragma unroll
-                    for (int q = 0; q < kQ; ++q) sum[u][q] += rec[q];
-                }
+                for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
             }
 #pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int r = r0 + u * kPBlock;
-                if (r < nwin) {
-#pragma unroll
-                    for (int q = 0; q < kQ; ++q) s_sum[kQ * r + q] = sum[u][q];
-                    unsigned long long *wt = reinterpret_cast<unsigned long long *>(s_node + 6 * r);
-                    wt[0] = mk[u];
-                    wt[1] = fw[u];
-                }
-            }
+            for (int q = 0; q < kQ; ++q) s_sum[kQ * r + q] = sum[q];
+            unsigned long long *wt = reinterpret_cast<unsigned long long *>(s_node + 6 * r);
+            wt[0] = mk;
+            wt[1] = fw;
         }
         // Pass 2: the grid update of each window node (…_MP.cpp:209-265)
         double gs[3] = {0.0, 0.0, 0.0};   // owned nodes: mass, reaction, active count
