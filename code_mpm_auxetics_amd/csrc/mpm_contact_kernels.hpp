@@ -94,12 +94,18 @@ __device__ __forceinline__ double c_len3(const double *v) { return sqrt(v[0] * v
 __device__ __forceinline__ double c_dot3(const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
 __device__ __forceinline__ double c_lenm3(double m) { return sqrt(m * m + m * m + m * m); }
 
+// FAST (FP32 state): the contact helpers multiply by 1/h and place nodes at
+// i h instead of dividing; FP64 state keeps the reference's expressions.
+template <typename real>
+constexpr bool kCFast = sizeof(real) == 4;
+
 // GridPoint_Factory.h:40-42 node position (i*L)/n
+template <bool FAST>
 __device__ __forceinline__ void c_node_pos(const GridParams &g, int ix, int iy, int iz, double xi[3])
 {
-    xi[0] = (double)ix * g.L[0] / (double)g.cells[0];
-    xi[1] = (double)iy * g.L[1] / (double)g.cells[1];
-    xi[2] = (double)iz * g.L[2] / (double)g.cells[2];
+    const int ii[3] = {ix, iy, iz};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) xi[q] = FAST ? (double)ii[q] * g.h[q] : (double)ii[q] * g.L[q] / (double)g.cells[q];
 }
 
 // linear-hat N and grad N of point x at node xi (GridPoint_Mediator.h:64-88)
@@ -121,13 +127,14 @@ __device__ __forceinline__ void c_bases(const GridParams &g, const double x[3], 
 }
 
 // the 8 adjacent nodes of x in reference order (x-outer, z-inner); false if out of grid
+template <bool FAST>
 __device__ __forceinline__ bool c_adjacent(const GridParams &g, const double x[3], long long idx[8], int ijk[8][3])
 {
     int i0[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         if (!(x[q] >= 0.0)) return false;
-        i0[q] = (int)(x[q] / g.h[q]);
+        i0[q] = (int)(FAST ? x[q] * g.ih[q] : x[q] / g.h[q]);
         if (!(i0[q] < g.cells[q])) return false;
     }
 #pragma unroll
@@ -147,15 +154,16 @@ __device__ __forceinline__ bool c_adjacent(const GridParams &g, const double x[3
 struct CAx {
     double xi[3][2], s[3][2], sg[3][2];
 };
+template <bool FAST>
 __device__ __forceinline__ void c_axes(const GridParams &g, const double x[3], const int i0[3], CAx &A)
 {
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const double xi = (double)(i0[q] + c) * g.L[q] / (double)g.cells[q];
+            const double xi = FAST ? (double)(i0[q] + c) * g.h[q] : (double)(i0[q] + c) * g.L[q] / (double)g.cells[q];
             const double d = x[q] - xi;
-            double v = 1.0 - fabs(d) / g.h[q];
+            double v = 1.0 - (FAST ? fabs(d) * g.ih[q] : fabs(d) / g.h[q]);
             if (v < 0.0 || v > 1.0) v = 0.0;
             A.xi[q][c] = xi;
             A.s[q][c] = v;
@@ -163,6 +171,7 @@ __device__ __forceinline__ void c_axes(const GridParams &g, const double x[3], c
         }
 }
 // node position, N and grad N of corner k (x-outer, z-inner), as c_bases
+template <bool FAST>
 __device__ __forceinline__ void c_corner(const GridParams &g, const CAx &A, int k, double xi[3], double &N, double G[3])
 {
     const int c[3] = {(k >> 2) & 1, (k >> 1) & 1, k & 1};
@@ -174,9 +183,15 @@ __device__ __forceinline__ void c_corner(const GridParams &g, const CAx &A, int 
         sg[q] = A.sg[q][c[q]];
     }
     N = s[0] * s[1] * s[2];
-    G[0] = -s[1] * s[2] * sg[0] / g.h[0];
-    G[1] = -s[0] * s[2] * sg[1] / g.h[1];
-    G[2] = -s[0] * s[1] * sg[2] / g.h[2];
+    if (FAST) {
+        G[0] = -s[1] * s[2] * sg[0] * g.ih[0];
+        G[1] = -s[0] * s[2] * sg[1] * g.ih[1];
+        G[2] = -s[0] * s[1] * sg[2] * g.ih[2];
+    } else {
+        G[0] = -s[1] * s[2] * sg[0] / g.h[0];
+        G[1] = -s[0] * s[2] * sg[1] / g.h[1];
+        G[2] = -s[0] * s[1] * sg[2] / g.h[2];
+    }
 }
 
 // fixed-point scale of bound slot q: contributions |c| <= bound < 2^e are
@@ -360,7 +375,7 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
     bool valid = j < a.n_all;
     if (valid) {
         c_pos(a, j, x);
-        valid = c_adjacent(g, x, idx, ijk);
+        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
         if (!valid) atomicOr(a.gflags, FLAG_OUT_OF_GRID);
     }
     const CWin w = c_window(valid, ijk[0], sbox, 1, 2, kCWinK);
@@ -380,7 +395,8 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
         // node coordinates (i L) / n of the 4 candidate planes per axis
         double xc[3][4];
         for (int q = 0; q < 3; ++q)
-            for (int t = 0; t < 4; ++t) xc[q][t] = (double)(i0[q] + t - 1) * g.L[q] / (double)g.cells[q];
+            for (int t = 0; t < 4; ++t)
+                xc[q][t] = kCFast<real> ? (double)(i0[q] + t - 1) * g.h[q] : (double)(i0[q] + t - 1) * g.L[q] / (double)g.cells[q];
         // a node can only be within R if it is within R on every axis (a
         // slightly inclusive prefilter; the exact test below decides)
         bool near_ax[3][4];
@@ -397,7 +413,7 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
                     const double d[3] = {x[0] - xc[0][ix + 1], x[1] - xc[1][iy + 1], x[2] - xc[2][iz + 1]};
                     const double dist = c_len3(d);
                     if (dist < R) {
-                        const double q = dist / R;
+                        const double q = kCFast<real> ? dist * g.ih[0] : dist / R;
                         const unsigned long long iv =
                             (unsigned long long)__double2ll_rn((1.0 - 3.0 * q * q + 2.0 * q * q * q) * sck);
                         if (w.lds) {
@@ -467,7 +483,7 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
     bool valid = j < a.n_all;
     if (valid) {
         c_pos(a, j, x);
-        valid = c_adjacent(g, x, idx, ijk);
+        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
     }
     const CWin w = c_window(valid, ijk[0], sbox);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
@@ -480,10 +496,10 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
         double kp = 0.0, kg[3] = {0.0, 0.0, 0.0};
         unsigned long long dk[8];
         CAx A;
-        c_axes(g, x, ijk[0], A);
+        c_axes<kCFast<real>>(g, x, ijk[0], A);
         for (int k = 0; k < 8; ++k) {
             double xi[3], N, G[3];
-            c_corner(g, A, k, xi, N, G);
+            c_corner<kCFast<real>>(g, A, k, xi, N, G);
             kp += N * ((double)a.acc[idx[k]] * usk);
             for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + idx[k]];
             const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
@@ -525,16 +541,16 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
     c_pos(a, j, x);
     long long idx[8];
     int ijk[8][3];
-    if (!c_adjacent(g, x, idx, ijk)) return;
+    if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) return;
     const uint32_t id = c_id(a, j);
     const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
     const bool surf = c_surf(a, j) != 0;
     const unsigned long long lk = (unsigned long long)__double_as_longlong(c_len3(kg));
     CAx A;
-    c_axes(g, x, ijk[0], A);
+    c_axes<kCFast<real>>(g, x, ijk[0], A);
     for (int k = 0; k < 8; ++k) {
         double xi[3], N, G[3];
-        c_corner(g, A, k, xi, N, G);
+        c_corner<kCFast<real>>(g, A, k, xi, N, G);
         const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
         if ((unsigned long long)__double_as_longlong(c_len3(d)) == a.near_d[idx[k]]) atomicMin(&a.near_i[idx[k]], id);
         if (surf && lk != 0ull && lk == a.sg_d[idx[k]]) atomicMin(&a.sg_i[idx[k]], id);
@@ -554,7 +570,7 @@ __global__ void __launch_bounds__(256) k_c_point3(CArgs<real> a)
     c_pos(a, j, x);
     long long idx[8];
     int ijk[8][3];
-    if (!c_adjacent(g, x, idx, ijk)) return;
+    if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) return;
     const uint32_t id = c_id(a, j);
     double x0[3];
     c_pos(a, a.def_id0[0] ? a.n_def + (int)a.def_id0[1] : (int)a.def_id0[1], x0);
@@ -564,7 +580,7 @@ __global__ void __launch_bounds__(256) k_c_point3(CArgs<real> a)
             for (int q = 0; q < 3; ++q) a.gkg[q * a.nn + I] = a.kg[q * a.n_all + j];
         if (id == a.near_i[I]) {
             double xi[3];
-            c_node_pos(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
+            c_node_pos<kCFast<real>>(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
             const double d0[3] = {x0[0] - xi[0], x0[1] - xi[1], x0[2] - xi[2]};
             const bool zero_wins = (unsigned long long)__double_as_longlong(c_len3(d0)) <= a.near_d[I];
             a.near_surf[I] = zero_wins ? a.surf0 : c_surf(a, j);
@@ -582,7 +598,7 @@ __global__ void __launch_bounds__(256) k_c_detect(CArgs<real> a)
     c_pos(a, j, x);
     long long idx[8];
     int ijk[8][3];
-    if (!c_adjacent(a.g, x, idx, ijk)) return;
+    if (!c_adjacent<kCFast<real>>(a.g, x, idx, ijk)) return;
     const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
     for (int k = 0; k < 8; ++k) {
         const long long I = idx[k];
@@ -620,7 +636,7 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     bool valid = j < a.n_all;
     if (valid) {
         c_pos(a, j, x);
-        valid = c_adjacent(g, x, idx, ijk);
+        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
     }
     const CWin w = c_window(valid, ijk[0], sbox);
     for (int t = threadIdx.x; t < w.nodes * NF; t += 256) lds[t] = 0ull;
@@ -648,11 +664,11 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
         const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
         const double um = c_unscale(a, 0);
         CAx A;
-        c_axes(g, x, ijk[0], A);
+        c_axes<kCFast<real>>(g, x, ijk[0], A);
         for (int k = 0; k < 8; ++k) {
             const long long I = idx[k];
             double xi[3], N, G[3];
-            c_corner(g, A, k, xi, N, G);
+            c_corner<kCFast<real>>(g, A, k, xi, N, G);
             const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
             const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
             const int ws = w.lds ? c_wslot(w, ijk[k]) : 0;
@@ -667,7 +683,12 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
                 base = 6 * f;
                 const double mI = (double)a.acc[off_m(f) * a.nn + I] * um;
                 const bool gate = c_lenm3(mI) > g.m_min;
-                for (int q = 0; q < 3; ++q) iv[q] = gate ? __double2ll_rn((N * (m * v[q]) / mI) * sc0) : 0ll;
+                if (kCFast<real>) {
+                    const double rmI = gate ? 1.0 / mI : 0.0;
+                    for (int q = 0; q < 3; ++q) iv[q] = gate ? __double2ll_rn((N * (m * v[q]) * rmI) * sc0) : 0ll;
+                } else {
+                    for (int q = 0; q < 3; ++q) iv[q] = gate ? __double2ll_rn((N * (m * v[q]) / mI) * sc0) : 0ll;
+                }
                 const double fr[3] = {-vol * (G[0] * s[0] + G[1] * s[3] + G[2] * s[5]) + N * fe[0],
                                       -vol * (G[1] * s[1] + G[0] * s[3] + G[2] * s[4]) + N * fe[1],
                                       -vol * (G[2] * s[2] + G[0] * s[5] + G[1] * s[4]) + N * fe[2]};
@@ -699,7 +720,7 @@ __global__ void __launch_bounds__(256) k_c_dcmarks(CArgs<real> a, int step)
     double x[3] = {a.dc_x[3 * d], a.dc_x[3 * d + 1], a.dc_x[3 * d + 2]};
     long long idx[8];
     int ijk[8][3];
-    if (!c_adjacent(a.g, x, idx, ijk)) return;
+    if (!c_adjacent<kCFast<real>>(a.g, x, idx, ijk)) return;
     const unsigned long long tag = ((unsigned long long)(a.steps[step].epoch + 1u) << 32) | (unsigned)d;
     for (int k = 0; k < 8; ++k) atomicMax(&a.cmarks[idx[k]], tag);
 }
@@ -797,9 +818,9 @@ __global__ void __launch_bounds__(256) k_c_grid(CArgs<real> a, int step)
         rm += mm[0];
         ra += 1.0;
     }
-    rm = wave_sum(rm);
-    rr = wave_sum(rr);
-    ra = wave_sum(ra);
+    rm = wave_sum_dpp(rm);
+    rr = wave_sum_dpp(rr);
+    ra = wave_sum_dpp(ra);
     const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) { s_r[wv][0] = rm; s_r[wv][1] = rr; s_r[wv][2] = ra; }
     __syncthreads();
@@ -828,18 +849,18 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
         c_pos(a, j, x);
         long long idx[8];
         int ijk[8][3];
-        if (!c_adjacent(g, x, idx, ijk)) {
+        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) {
             flags |= FLAG_OUT_OF_GRID;
         } else {
             const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
             double v[3], xn[3] = {x[0], x[1], x[2]}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             for (int q = 0; q < 3; ++q) v[q] = (double)P.v[q][j];
             CAx A;
-            c_axes(g, x, ijk[0], A);
+            c_axes<kCFast<real>>(g, x, ijk[0], A);
             for (int k = 0; k < 8; ++k) {
                 const long long I = idx[k];
                 double xi[3], N, G[3];
-                c_corner(g, A, k, xi, N, G);
+                c_corner<kCFast<real>>(g, A, k, xi, N, G);
                 const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
                 const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
                 const double m = a.gm[f * a.nn + I];
@@ -848,8 +869,14 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
                     vI[q] = a.gv[(3 * f + q) * a.nn + I];
                     fI[q] = a.gf[(3 * f + q) * a.nn + I];
                 }
-                if (c_lenm3(m) > g.m_min)
-                    for (int q = 0; q < 3; ++q) v[q] += N * (fI[q] / m) * dt;
+                if (c_lenm3(m) > g.m_min) {
+                    if (FP32) {
+                        const double rm = 1.0 / m;
+                        for (int q = 0; q < 3; ++q) v[q] += N * (fI[q] * rm) * dt;
+                    } else {
+                        for (int q = 0; q < 3; ++q) v[q] += N * (fI[q] / m) * dt;
+                    }
+                }
                 for (int q = 0; q < 3; ++q) xn[q] += N * vI[q] * dt;
                 for (int c = 0; c < 3; ++c)
                     for (int r = 0; r < 3; ++r) L[3 * c + r] += vI[r] * G[c];
@@ -870,7 +897,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
                 const uint32_t b = P.pbrick[j];
                 const int bc[3] = {(int)(b % g.nb[0]), (int)((b / g.nb[0]) % g.nb[1]), (int)(b / (g.nb[0] * g.nb[1]))};
                 for (int q = 0; q < 3; ++q) {
-                    const double u = xn[q] / g.h[q] - (double)(bc[q] * g.S[q]);
+                    const double u = xn[q] * g.ih[q] - (double)(bc[q] * g.S[q]);
                     P.xq[q][j] = (int32_t)rint(u * kQOne);
                     if (u < -0.5 || u >= (double)g.S[q] + 0.5) flags |= FLAG_REBIN;
                 }
@@ -898,7 +925,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
         }
     }
     for (int k = 0; k < 5; ++k) {
-        const double w = wave_sum(red[k]);
+        const double w = wave_sum_dpp(red[k]);
         if (lane == 0) s_r[wv][k] = w;
     }
     int fl = flags;

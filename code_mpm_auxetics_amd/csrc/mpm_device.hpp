@@ -46,6 +46,7 @@ struct GridParams {
     int cbits;     // bits for the cell-in-brick index in sort keys
     double m_min, damping;
     int strain_form, neohookean, track_strain, has_fext, has_pointE;
+    double ih[3];  // 1/h in FP64 (FP32-state contact arithmetic multiplies by it)
 };
 
 // Per-step record filled on the host (time, dt, platen velocity, epoch).

@@ -1088,6 +1088,7 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         g.h[a] = cfg->length[a] / (double)cfg->cells[a];
         g.hf[a] = (float)g.h[a];
         g.ihf[a] = (float)(1.0 / g.h[a]);
+        g.ih[a] = 1.0 / g.h[a];
         int S = cfg->brick[a] > 0 ? cfg->brick[a] : (a == 2 ? 10 : 4);
         S = std::min(S, std::max(1, cfg->cells[a]));
         // an arena window reaches one neighbour per side only when S >= 3
