@@ -40,8 +40,12 @@ EXPORTED = [
     "mpm_checks", "mpm_reaction_history", "mpm_get_time", "mpm_last_step_ms",
     "mpm_last_error", "mpm_destroy", "mpm_sort_keys", "mpm_kernel_times", "mpm_group_reaction_history",
     "mpm_diff_create", "mpm_diff_forward", "mpm_diff_backward", "mpm_diff_last_error", "mpm_diff_destroy",
-    "mpm_report", "mpm_set_log",
+    "mpm_report", "mpm_set_log", "mpm_constitutive",
 ]
+
+# constitutive laws of mpm_constitutive (MaterialPoint.h:8-16 type codes; 16: calculateState_J2)
+MPM_LAW_ELASTIC, MPM_LAW_NEOHOOKEAN, MPM_LAW_VISCOELASTIC, MPM_LAW_PLASTIC, MPM_LAW_VONMISES, MPM_LAW_IDEALGAS = range(6)
+MPM_LAW_J2_STATE = 16
 
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
@@ -90,6 +94,15 @@ class MpmPoints(C.Structure):
     ]
 
 
+class MpmCrBatch(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("E", _dp), ("nu", _dp), ("yield_", _dp), ("alpha", _dp), ("C0", _dp), ("C1", _dp),
+        ("stress", _dp), ("strain_inc", _dp), ("F", _dp),
+        ("stress_inc", _dp), ("plastic_inc", _dp), ("alpha_inc", _dp), ("j2", _dp),
+    ]
+
+
 _lib = None
 
 
@@ -121,6 +134,7 @@ def lib():
     L.mpm_kernel_times.argtypes = [C.c_void_p, _dp]
     L.mpm_report.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32, _i32p]
     L.mpm_set_log.argtypes = [C.c_void_p, C.c_char_p]
+    L.mpm_constitutive.argtypes = [C.c_int32, C.POINTER(MpmCrBatch), C.c_int32]
     L.mpm_diff_create.argtypes = [C.POINTER(MpmConfig), C.POINTER(MpmPoints), C.c_int32, C.c_int32,
                                   C.POINTER(C.c_void_p)]
     L.mpm_diff_forward.argtypes = [C.c_void_p, _dp, _dp, _dp]

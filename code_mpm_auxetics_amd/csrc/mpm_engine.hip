@@ -1582,4 +1582,73 @@ int mpm_sort_keys(const uint32_t *keys_host, int32_t n, uint32_t *sorted_host, u
     return rc;
 }
 
+int mpm_constitutive(int32_t law, const mpm_cr_batch *b, int32_t device)
+{
+    if (!b || b->n < 0) { g_create_error = "bad constitutive batch"; return MPM_E_ARG; }
+    if (law == MPM_LAW_VISCOELASTIC || law == MPM_LAW_IDEALGAS) {
+        g_create_error = "the ViscoElastic and IdealGas laws are not dispatched by the reference step and not provided";
+        return MPM_E_ARG;
+    }
+    if (law != MPM_LAW_ELASTIC && law != MPM_LAW_NEOHOOKEAN && law != MPM_LAW_PLASTIC && law != MPM_LAW_VONMISES &&
+        law != MPM_LAW_J2_STATE) {
+        g_create_error = "unknown constitutive law";
+        return MPM_E_ARG;
+    }
+    const size_t n = (size_t)b->n;
+    if (n == 0) return MPM_OK;
+    // the inputs and outputs each law reads and writes
+    const bool j2 = law == MPM_LAW_J2_STATE;
+    bool ok = b->stress || law == MPM_LAW_ELASTIC;
+    if (!j2) {
+        ok &= b->E && b->nu && b->stress_inc && b->plastic_inc && b->alpha_inc;
+        if (law != MPM_LAW_NEOHOOKEAN) ok &= b->strain_inc != nullptr;
+        if (law == MPM_LAW_NEOHOOKEAN) ok &= b->F != nullptr;
+        if (law == MPM_LAW_PLASTIC || law == MPM_LAW_VONMISES) ok &= b->yield != nullptr;
+        if (law == MPM_LAW_VONMISES) ok &= b->alpha && b->C0 && b->C1;
+    } else {
+        ok &= b->j2 != nullptr;
+    }
+    if (!ok) { g_create_error = "constitutive batch: missing input or output array"; return MPM_E_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return MPM_E_NO_DEVICE; }
+    if (hipSetDevice(device) != hipSuccess) { g_create_error = "hipSetDevice"; return MPM_E_HIP; }
+    // one device block: the inputs given, then the outputs of this law
+    struct Part { const double *src; double *dst; size_t len; };
+    const Part parts[] = {{b->E, nullptr, n}, {b->nu, nullptr, n}, {b->yield, nullptr, n}, {b->alpha, nullptr, n},
+                          {b->C0, nullptr, n}, {b->C1, nullptr, n}, {b->stress, nullptr, 6 * n},
+                          {b->strain_inc, nullptr, 6 * n}, {b->F, nullptr, 9 * n},
+                          {nullptr, j2 ? nullptr : b->stress_inc, 6 * n}, {nullptr, j2 ? nullptr : b->plastic_inc, 6 * n},
+                          {nullptr, j2 ? nullptr : b->alpha_inc, n}, {nullptr, j2 ? b->j2 : nullptr, n}};
+    constexpr int NP = sizeof(parts) / sizeof(parts[0]);
+    size_t off[NP], total = 0;
+    for (int k = 0; k < NP; ++k) {
+        off[k] = total;
+        if (parts[k].src || parts[k].dst) total += parts[k].len;
+    }
+    double *dev = nullptr;
+    if (hipMalloc((void **)&dev, total * sizeof(double)) != hipSuccess) { g_create_error = "hipMalloc"; return MPM_E_HIP; }
+    int rc = MPM_OK;
+    double *dp[NP];
+    for (int k = 0; k < NP; ++k) {
+        dp[k] = (parts[k].src || parts[k].dst) ? dev + off[k] : nullptr;
+        if (parts[k].src && hipMemcpy(dp[k], parts[k].src, parts[k].len * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            rc = MPM_E_HIP;
+    }
+    if (rc == MPM_OK) {
+        CrBatch cb{};
+        cb.n = b->n;
+        cb.E = dp[0]; cb.nu = dp[1]; cb.sy = dp[2]; cb.alpha = dp[3]; cb.C0 = dp[4]; cb.C1 = dp[5];
+        cb.s = dp[6]; cb.de = dp[7]; cb.F = dp[8];
+        cb.ds = dp[9]; cb.dep = dp[10]; cb.da = dp[11]; cb.j2 = dp[12];
+        k_constitutive<<<cdiv(n, 256), 256, 0, 0>>>(law, cb);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = MPM_E_HIP;
+    }
+    for (int k = 0; rc == MPM_OK && k < NP; ++k)
+        if (parts[k].dst && hipMemcpy(parts[k].dst, dp[k], parts[k].len * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = MPM_E_HIP;
+    (void)hipFree(dev);
+    if (rc != MPM_OK) g_create_error = "constitutive batch: HIP error";
+    return rc;
+}
+
 }  // extern "C"

@@ -101,6 +101,32 @@ __device__ __forceinline__ bool radial_return(double E, double nu, double radius
     return true;
 }
 
+// …PerfectlyPlastic.cpp:4-72: k = sigma_y / sqrt(3).  True on the plastic branch.
+__device__ __forceinline__ bool perfectly_plastic_inc(double E, double nu, double sy, const double s[6],
+                                                      const double de[6], double ds[6], double dep[6])
+{
+    return radial_return(E, nu, sy / sqrt(3.0), s, de, ds, dep);
+}
+
+// …VonMisesHardening.cpp:4-86: radius k + alpha / sqrt(2); on the plastic
+// branch the equivalent plastic strain sqrt(2/3 sum dep_i^2) (engineering
+// shear, no factor 2) drives d alpha = C0 (sqrt(2/3) C1 - alpha) eps_bar when
+// positive.  True on the plastic branch.
+__device__ __forceinline__ bool vonmises_inc(double E, double nu, double sy, double al, double C0, double C1,
+                                             const double s[6], const double de[6], double ds[6], double dep[6],
+                                             double &da)
+{
+    da = 0.0;
+    const double k0 = sy / sqrt(3.0);
+    if (!radial_return(E, nu, k0 + al / sqrt(2.0), s, de, ds, dep)) return false;
+    double eff = 0.0;
+    for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
+    eff *= 2.0 / 3.0;
+    eff = sqrt(eff);
+    if ((sqrt(2.0 / 3.0) * C1 - al) > 0.0) da = C0 * (sqrt(2.0 / 3.0) * C1 - al) * eff;
+    return true;
+}
+
 // The same radial return for FP32 state: FP64 arithmetic, elastic constants
 // precomputed (C = E/(1+nu)/(1-2nu), 1/E, 1/G: reciprocals instead of the
 // reference's divisions).  Returns true on the plastic branch; s is updated in
@@ -509,6 +535,48 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) elt(P.s[q], i) = s[q];
+}
+
+// ---------------------------------------------------- batched constitutive
+// The constitutive functions of the step applied to n independent inputs
+// (ConstitutiveRelation.h:16-34): law 0 Elastic, 1 NeoHookean, 3
+// PerfectlyPlastic, 4 VonMisesHardening (the MaterialPoint.h:8-16 type codes),
+// 16 calculateState_J2 of the stress.  FP64, the reference's evaluation order.
+struct CrBatch {
+    int32_t n;
+    const double *E, *nu, *sy, *alpha, *C0, *C1;
+    const double *s, *de, *F;    // 6n, 6n, 9n (column-major)
+    double *ds, *dep, *da, *j2;  // 6n, 6n, n, n
+};
+__global__ void __launch_bounds__(256) k_constitutive(int law, CrBatch b)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= b.n) return;
+    double s[6], de[6], ds[6] = {0, 0, 0, 0, 0, 0}, dep[6] = {0, 0, 0, 0, 0, 0}, da = 0.0;
+    for (int q = 0; q < 6; ++q) {
+        s[q] = b.s ? b.s[6 * i + q] : 0.0;
+        de[q] = b.de ? b.de[6 * i + q] : 0.0;
+    }
+    if (law == 0) {
+        elastic_inc<double>(b.E[i], b.nu[i], de, ds);
+    } else if (law == 1) {
+        double F[9];
+        for (int q = 0; q < 9; ++q) F[q] = b.F[9 * i + q];
+        neohookean_inc(b.E[i], b.nu[i], s, F, ds);
+    } else if (law == 3) {
+        perfectly_plastic_inc(b.E[i], b.nu[i], b.sy[i], s, de, ds, dep);
+    } else if (law == 4) {
+        vonmises_inc(b.E[i], b.nu[i], b.sy[i], b.alpha[i], b.C0[i], b.C1[i], s, de, ds, dep, da);
+    }
+    if (law == 16) {
+        b.j2[i] = j2_of(s);
+        return;
+    }
+    for (int q = 0; q < 6; ++q) {
+        b.ds[6 * i + q] = ds[q];
+        b.dep[6 * i + q] = dep[q];
+    }
+    b.da[i] = da;
 }
 
 // Fixed-point fraction bits.  FP32 state: each contribution is converted to a

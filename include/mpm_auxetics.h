@@ -17,7 +17,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 2
+#define MPM_ABI_VERSION 3
 
 /* ---- status codes (the reference has none; it exits via fatalError,
  *      Errors.cpp:5-12) ---------------------------------------------------- */
@@ -223,6 +223,43 @@ void mpm_destroy(mpm_handle *h);
  *      the reference keeps points unsorted (PhysicsEngine.h:78). ----------- */
 int mpm_sort_keys(const uint32_t *keys_host, int32_t n, uint32_t *sorted_host,
                   uint32_t *perm_host, int32_t device);
+
+/* ---- constitutive increments of a batch (replaces the per-point
+ *      ConstitutiveRelation object of …_MP.cpp:322-353; interface of
+ *      ConstitutiveRelation.h:16-34).  The step's own device functions, FP64,
+ *      the reference's evaluation order, applied to n independent inputs:
+ *        law MPM_LAW_ELASTIC           calculateIncrement_Elastic(E, nu, de)
+ *                                      (…Elastic.cpp:4-15)
+ *        law MPM_LAW_NEOHOOKEAN        calculateIncrement_NeoHookean_6D(E, nu, s, F)
+ *                                      (…NeoHookean.cpp:4-22)
+ *        law MPM_LAW_PLASTIC           calculateIncrement_PerfectlyPlastic_6D(E, nu, sy, s, de)
+ *                                      (…PerfectlyPlastic.cpp:4-72)
+ *        law MPM_LAW_VONMISES          calculateIncrement_VonMisesHardening_6D(E, nu, sy,
+ *                                      alpha, C0, C1, s, de) (…VonMisesHardening.cpp:4-86)
+ *        law MPM_LAW_J2_STATE          calculateState_J2(s) (…calculateState_J2.cpp:4-26)
+ *      Inputs the law does not use may be NULL.  Outputs: stress_inc and
+ *      plastic_inc (6 per input, Voigt xx yy zz xy yz zx), alpha_inc (1), or j2
+ *      (1) for MPM_LAW_J2_STATE.  F is column-major (F[3*col+row]).  The
+ *      ViscoElastic and IdealGas laws (…ViscoElastic.cpp, …IdealGas.cpp) are
+ *      never dispatched by the reference step and return MPM_E_ARG. ------ */
+#define MPM_LAW_ELASTIC 0
+#define MPM_LAW_NEOHOOKEAN 1
+#define MPM_LAW_VISCOELASTIC 2
+#define MPM_LAW_PLASTIC 3
+#define MPM_LAW_VONMISES 4
+#define MPM_LAW_IDEALGAS 5
+#define MPM_LAW_J2_STATE 16
+
+typedef struct mpm_cr_batch {
+    int32_t n;
+    const double *E, *nu, *yield, *alpha, *C0, *C1;   /* n each */
+    const double *stress, *strain_inc;                /* 6n each */
+    const double *F;                                  /* 9n */
+    double *stress_inc, *plastic_inc;                 /* 6n each */
+    double *alpha_inc, *j2;                           /* n each */
+} mpm_cr_batch;
+
+int mpm_constitutive(int32_t law, const mpm_cr_batch *b, int32_t device);
 
 /* ---- differentiable run (no reference counterpart: the reference has no
  *      gradients).  n_steps plain steps of cfg->dt_max in FP64 for elastic
