@@ -40,7 +40,7 @@ EXPORTED = [
     "mpm_checks", "mpm_reaction_history", "mpm_get_time", "mpm_last_step_ms",
     "mpm_last_error", "mpm_destroy", "mpm_sort_keys", "mpm_kernel_times", "mpm_group_reaction_history",
     "mpm_diff_create", "mpm_diff_forward", "mpm_diff_backward", "mpm_diff_last_error", "mpm_diff_destroy",
-    "mpm_report", "mpm_set_log", "mpm_constitutive",
+    "mpm_report", "mpm_set_log", "mpm_constitutive", "mpm_step_async", "mpm_step_poll", "mpm_step_wait",
 ]
 
 # constitutive laws of mpm_constitutive (MaterialPoint.h:8-16 type codes; 16: calculateState_J2)
@@ -120,6 +120,9 @@ def lib():
     L.mpm_create.argtypes = [C.POINTER(MpmConfig), C.POINTER(C.c_void_p)]
     L.mpm_upload_points.argtypes = [C.c_void_p, C.POINTER(MpmPoints)]
     L.mpm_step.argtypes = [C.c_void_p, C.c_double, C.c_int32, _i32p]
+    L.mpm_step_async.argtypes = [C.c_void_p, C.c_double, C.c_int32]
+    L.mpm_step_poll.argtypes = [C.c_void_p, _i32p]
+    L.mpm_step_wait.argtypes = [C.c_void_p, _i32p]
     L.mpm_readback.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(MpmPoints)]
     L.mpm_checks.argtypes = [C.c_void_p, _dp]
     L.mpm_reaction_history.argtypes = [C.c_void_p, _dp, C.c_int32, _i32p]

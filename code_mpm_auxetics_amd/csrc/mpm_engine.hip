@@ -12,11 +12,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -39,6 +41,13 @@ struct Buf {
 }  // namespace
 
 struct mpm_handle {
+    // asynchronous step (mpm_step_async): 0 idle, 1 running, 2 finished, not
+    // yet collected by mpm_step_wait.  While not idle the worker thread owns
+    // the handle and every other call returns MPM_E_STATE.
+    std::atomic<int> async_state{0};
+    std::thread worker;
+    int async_rc = 0;
+    int32_t async_finished = 0;
     mpm_config cfg{};
     std::vector<double> tl_t, tl_v;
     std::vector<uint8_t> fixed_host;
@@ -979,8 +988,17 @@ extern "C" {
 
 int mpm_abi_version(void) { return MPM_ABI_VERSION; }
 
+// every call on a handle with an asynchronous step in flight is refused
+#define BUSYCHK(h)                                                                  \
+    do {                                                                            \
+        if ((h) && (h)->async_state.load(std::memory_order_acquire) != 0) return MPM_E_STATE; \
+    } while (0)
+
+static const char *kBusy = "an asynchronous step is in flight on this handle; call mpm_step_wait first";
+
 const char *mpm_last_error(const mpm_handle *h)
 {
+    if (h && h->async_state.load(std::memory_order_acquire) != 0) return kBusy;
     return h ? h->err.c_str() : g_create_error.c_str();
 }
 
@@ -1038,6 +1056,7 @@ static void destroy_all(mpm_handle *h)
 void mpm_destroy(mpm_handle *h)
 {
     if (!h) return;
+    if (h->worker.joinable()) h->worker.join();
     destroy_all(h);
     delete h;
 }
@@ -1374,6 +1393,7 @@ extern "C" {
 
 int mpm_upload_points(mpm_handle *h, const mpm_points *pts)
 {
+    BUSYCHK(h);
     if (!h || !pts || pts->n < 0 || !pts->x || !pts->mass) { if (h) h->err = "null points"; return MPM_E_ARG; }
     if (h->uploaded) { h->err = "points already uploaded"; return MPM_E_STATE; }
     if (!pts->vol0 && !pts->vol) { h->err = "vol0 or vol required"; return MPM_E_ARG; }
@@ -1384,14 +1404,71 @@ int mpm_upload_points(mpm_handle *h, const mpm_points *pts)
     return rc;
 }
 
-int mpm_step(mpm_handle *h, double dt_total, int32_t path, int32_t *finished)
+static int step_args(mpm_handle *h, double dt_total, int32_t path)
 {
-    if (!h) return MPM_E_ARG;
     if (!h->uploaded) { h->err = "mpm_step before mpm_upload_points"; return MPM_E_STATE; }
     if (path != MPM_PATH_PLAIN && path != MPM_PATH_CONTACT) { h->err = "unknown step path"; return MPM_E_ARG; }
     if (!(dt_total >= 0.0)) { h->err = "dt_total must be >= 0"; return MPM_E_ARG; }
+    return MPM_OK;
+}
+
+static int step_run(mpm_handle *h, double dt_total, int32_t path, int32_t *finished)
+{
     HIPCHK(h, hipSetDevice(h->device));
     return h->fp32 ? step_impl<float>(h, dt_total, path, finished) : step_impl<double>(h, dt_total, path, finished);
+}
+
+int mpm_step(mpm_handle *h, double dt_total, int32_t path, int32_t *finished)
+{
+    if (!h) return MPM_E_ARG;
+    BUSYCHK(h);
+    RCHK(step_args(h, dt_total, path));
+    return step_run(h, dt_total, path, finished);
+}
+
+int mpm_step_async(mpm_handle *h, double dt_total, int32_t path)
+{
+    if (!h) return MPM_E_ARG;
+    BUSYCHK(h);
+    RCHK(step_args(h, dt_total, path));
+    if (h->worker.joinable()) h->worker.join();
+    h->async_rc = MPM_OK;
+    h->async_finished = 0;
+    h->async_state.store(1, std::memory_order_release);
+    try {
+        h->worker = std::thread([h, dt_total, path]() {
+            int32_t fin = 0;
+            const int rc = step_run(h, dt_total, path, &fin);
+            h->async_rc = rc;
+            h->async_finished = fin;
+            h->async_state.store(2, std::memory_order_release);
+        });
+    } catch (...) {
+        h->async_state.store(0, std::memory_order_release);
+        h->err = "could not start the step thread";
+        return MPM_E_STATE;
+    }
+    return MPM_OK;
+}
+
+int mpm_step_poll(mpm_handle *h, int32_t *done)
+{
+    if (!h || !done) return MPM_E_ARG;
+    const int st = h->async_state.load(std::memory_order_acquire);
+    if (st == 0) { h->err = "no asynchronous step on this handle"; return MPM_E_STATE; }
+    *done = st == 2 ? 1 : 0;
+    return MPM_OK;
+}
+
+int mpm_step_wait(mpm_handle *h, int32_t *finished)
+{
+    if (!h) return MPM_E_ARG;
+    if (h->async_state.load(std::memory_order_acquire) == 0) { h->err = "no asynchronous step on this handle"; return MPM_E_STATE; }
+    h->worker.join();
+    const int rc = h->async_rc;
+    if (finished) *finished = h->async_finished;
+    h->async_state.store(0, std::memory_order_release);
+    return rc;
 }
 
 }  // extern "C"
@@ -1460,6 +1537,7 @@ extern "C" {
 
 int mpm_readback(mpm_handle *h, uint32_t field_mask, mpm_points *out)
 {
+    BUSYCHK(h);
     if (!h || !out) return MPM_E_ARG;
     if (!h->uploaded) { h->err = "readback before upload"; return MPM_E_STATE; }
     if (out->n != h->n_total) { h->err = "readback point count mismatch"; return MPM_E_ARG; }
@@ -1470,6 +1548,7 @@ int mpm_readback(mpm_handle *h, uint32_t field_mask, mpm_points *out)
 
 int mpm_checks(mpm_handle *h, double out[12])
 {
+    BUSYCHK(h);
     if (!h || !out) return MPM_E_ARG;
     if (!h->uploaded) { h->err = "checks before upload"; return MPM_E_STATE; }
     HIPCHK(h, hipSetDevice(h->device));
@@ -1478,6 +1557,7 @@ int mpm_checks(mpm_handle *h, double out[12])
 
 int mpm_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t *n)
 {
+    BUSYCHK(h);
     if (!h || (!out && cap > 0) || !n) return MPM_E_ARG;
     const int m = std::min<int>((int)h->hist.size(), cap);
     for (int k = 0; k < m; ++k) out[k] = h->hist[k];
@@ -1487,6 +1567,7 @@ int mpm_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t *n)
 
 int mpm_kernel_times(mpm_handle *h, double out[6])
 {
+    BUSYCHK(h);
     if (!h || !out) return MPM_E_ARG;
     for (int k = 0; k < 6; ++k) out[k] = h->kt[k];
     return MPM_OK;
@@ -1494,6 +1575,7 @@ int mpm_kernel_times(mpm_handle *h, double out[6])
 
 int mpm_group_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t *n)
 {
+    BUSYCHK(h);
     if (!h || (!out && cap > 0) || !n) return MPM_E_ARG;
     if (h->n_groups == 0) { *n = 0; return MPM_OK; }
     const int steps = (int)(h->ghist.size() / h->n_groups);
@@ -1505,6 +1587,7 @@ int mpm_group_reaction_history(mpm_handle *h, double *out, int32_t cap, int32_t 
 
 int mpm_get_time(mpm_handle *h, double *time, double *time_end, double *dt_max)
 {
+    BUSYCHK(h);
     if (!h) return MPM_E_ARG;
     if (time) *time = h->time;
     if (time_end) *time_end = h->cfg.time_end;
@@ -1514,6 +1597,7 @@ int mpm_get_time(mpm_handle *h, double *time, double *time_end, double *dt_max)
 
 int mpm_last_step_ms(mpm_handle *h, double *ms)
 {
+    BUSYCHK(h);
     if (!h || !ms) return MPM_E_ARG;
     *ms = h->last_ms;
     return MPM_OK;
@@ -1521,6 +1605,7 @@ int mpm_last_step_ms(mpm_handle *h, double *ms)
 
 int mpm_set_log(mpm_handle *h, const char *path)
 {
+    BUSYCHK(h);
     if (!h) return MPM_E_ARG;
     h->log_path = path ? path : "";
     return MPM_OK;
@@ -1528,6 +1613,7 @@ int mpm_set_log(mpm_handle *h, const char *path)
 
 int mpm_report(mpm_handle *h, const char *description, char *buf, int32_t cap, int32_t *len)
 {
+    BUSYCHK(h);
     if (!h || (buf && cap < 1)) return MPM_E_ARG;
     if (h->n_total == 0) {
         h->err = "report before upload";

@@ -134,6 +134,25 @@ class PhysicsEngine:
         check(lib().mpm_step(self._h, float(dTimeIncrement_Total), _abi.MPM_PATH_CONTACT, C.byref(fin)), self._h)
         return int(fin.value)
 
+    def step_async(self, dTimeIncrement_Total, path=_abi.MPM_PATH_PLAIN):
+        """Start the same advance as runSimulation_Classic_SinglePass_MP(_Contact)
+        on the handle's worker thread and return at once; collect it with
+        step_wait().  Other calls on the engine raise (MPM_E_STATE) meanwhile."""
+        check(lib().mpm_step_async(self._h, float(dTimeIncrement_Total), path), self._h)
+
+    def step_poll(self):
+        """True once the asynchronous advance has finished."""
+        done = C.c_int32(0)
+        check(lib().mpm_step_poll(self._h, C.byref(done)), self._h)
+        return bool(done.value)
+
+    def step_wait(self):
+        """Wait for the asynchronous advance; returns what the blocking call
+        would have returned (1 when d_Time >= d_TimeEnd, else 0)."""
+        fin = C.c_int32(0)
+        check(lib().mpm_step_wait(self._h, C.byref(fin)), self._h)
+        return int(fin.value)
+
     def run_steps(self, nsteps, path=_abi.MPM_PATH_PLAIN):
         """One call advancing dt_total = nsteps * dt_max, sub-stepped as the
         reference loop does (…_MP.cpp:18-27)."""

@@ -17,7 +17,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 3
+#define MPM_ABI_VERSION 4
 
 /* ---- status codes (the reference has none; it exits via fatalError,
  *      Errors.cpp:5-12) ---------------------------------------------------- */
@@ -148,6 +148,19 @@ int mpm_upload_points(mpm_handle *h, const mpm_points *pts);
  * else 0 (…_MP.cpp:385-393).  Blocks until the steps completed.  The path may
  * change from call to call (…_MP_Contact.cpp:4-676 for MPM_PATH_CONTACT). */
 int mpm_step(mpm_handle *h, double dt_total, int32_t path, int32_t *finished);
+
+/* The same advance without blocking (the threading model of the step's
+ * caller, GraphicsEngine_gameLoop.cpp:7-36, alternates step and draw on one
+ * thread; this lets a caller draw the previous readback while the GPU steps).
+ * mpm_step_async checks its arguments, starts the steps on a worker thread of
+ * the handle and returns.  Until mpm_step_wait collects the result, every
+ * other call on the handle returns MPM_E_STATE (mpm_last_error says why);
+ * mpm_step_poll sets *done to 1 once the steps have finished.  mpm_step_wait
+ * blocks until then and returns the status mpm_step would have returned, with
+ * *finished as in mpm_step.  mpm_destroy waits for a step in flight. */
+int mpm_step_async(mpm_handle *h, double dt_total, int32_t path);
+int mpm_step_poll(mpm_handle *h, int32_t *done);
+int mpm_step_wait(mpm_handle *h, int32_t *finished);
 
 /* Copy point fields (mask of MPM_F_*) back to host arrays in the original
  * point order.  Replaces PhysicsEngine::getMaterialPoints (PhysicsEngine.h:64),

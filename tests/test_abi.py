@@ -216,3 +216,11 @@ def test_cpp_constitutive_mirror_compiles(tmp_path):
     subprocess.run([gxx, "-std=c++17", "-I", ROOT, str(src), "-o", str(exe), "-L", libdir, "-lmpm_auxetics",
                     f"-Wl,-rpath,{libdir}"], check=True)
     assert subprocess.run([str(exe)]).returncode == 0
+
+
+def test_async_step_argument_errors_without_device():
+    lib = A.lib()
+    done = C.c_int32()
+    assert lib.mpm_step_async(None, C.c_double(1e-6), 0) == A.MPM_E_ARG
+    assert lib.mpm_step_poll(None, C.byref(done)) == A.MPM_E_ARG
+    assert lib.mpm_step_wait(None, C.byref(done)) == A.MPM_E_ARG

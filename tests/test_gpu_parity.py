@@ -308,3 +308,35 @@ def test_autotune_picks_a_working_launch_shape(mpm, oracle_mod):
     g, o = eng.getMaterialPoints(), orc.points()
     assert rel_l2(g["x"] - x0, o["x"] - x0) <= 1e-9
     eng.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_async_step_matches_blocking_step(mpm, prec):
+    """mpm_step_async + mpm_step_wait advance exactly as mpm_step does (bit for
+    bit), refuse every other call on the handle while the steps are in flight,
+    and mpm_destroy waits for a step still in flight."""
+    p = mpm.FP64 if prec == "fp64" else mpm.FP32
+    sc = mpm.scene.small_world(material="j2", speed=2.0)
+    dt = 30 * sc.dt_max
+    with mpm.PhysicsEngine(sc, precision=p) as ref:
+        fins = [ref.runSimulation_Classic_SinglePass_MP(dt) for _ in range(3)]
+        want = ref.getMaterialPoints()
+    with mpm.PhysicsEngine(sc, precision=p) as eng:
+        got_fins = []
+        for _ in range(3):
+            eng.step_async(dt)
+            with pytest.raises(mpm.MpmError) as e:
+                eng.checks()          # refused while in flight (or until collected)
+            assert e.value.code == mpm._abi.MPM_E_STATE
+            while not eng.step_poll():
+                pass
+            got_fins.append(eng.step_wait())
+        got = eng.getMaterialPoints()
+        with pytest.raises(mpm.MpmError):
+            eng.step_wait()           # nothing in flight
+    assert got_fins == fins
+    for k in ("x", "v", "F", "stress", "eps_p", "alpha"):
+        assert np.array_equal(got[k], want[k]), k
+    eng = mpm.PhysicsEngine(sc, precision=p)
+    eng.step_async(dt)
+    eng.close()                       # joins the worker
