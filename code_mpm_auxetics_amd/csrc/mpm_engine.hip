@@ -1398,6 +1398,22 @@ int mpm_upload_points(mpm_handle *h, const mpm_points *pts)
     if (h->uploaded) { h->err = "points already uploaded"; return MPM_E_STATE; }
     if (!pts->vol0 && !pts->vol) { h->err = "vol0 or vol required"; return MPM_E_ARG; }
     if ((uint32_t)pts->n > kMaxPoints) { h->err = "more than 2^28 points"; return MPM_E_ARG; }
+    {   // every given real input must be finite: nothing non-finite reaches a kernel
+        const size_t n = (size_t)pts->n;
+        const struct { const double *p; size_t w; const char *name; } in[] = {
+            {pts->x, 3, "x"}, {pts->v, 3, "v"}, {pts->fext, 3, "fext"}, {pts->F, 9, "F"}, {pts->strain, 6, "strain"},
+            {pts->stress, 6, "stress"}, {pts->eps_p, 6, "eps_p"}, {pts->alpha, 1, "alpha"}, {pts->mass, 1, "mass"},
+            {pts->vol, 1, "vol"}, {pts->vol0, 1, "vol0"}, {pts->E, 1, "E"}, {pts->nu, 1, "nu"}, {pts->sy, 1, "sy"},
+            {pts->C0, 1, "C0"}, {pts->C1, 1, "C1"}};
+        for (const auto &f : in) {
+            if (!f.p) continue;
+            for (size_t k = 0; k < n * f.w; ++k)
+                if (!std::isfinite(f.p[k])) {
+                    h->err = std::string("non-finite ") + f.name + " of point " + std::to_string(k / f.w);
+                    return MPM_E_ARG;
+                }
+        }
+    }
     HIPCHK(h, hipSetDevice(h->device));
     int rc = h->fp32 ? upload_t<float>(h, pts) : upload_t<double>(h, pts);
     if (rc == MPM_OK) h->uploaded = true;
