@@ -91,6 +91,7 @@ struct mpm_handle {
     uint32_t *occ_list = nullptr, *occ_start = nullptr, *occ_count = nullptr;
     int32_t *n_occ_d = nullptr, *brick_slot = nullptr;
     uint32_t *slot_order = nullptr;          // launch order of the brick blocks
+    uint8_t *slot_mat = nullptr;             // per slot: shared material index or 0xFF
     std::vector<uint32_t> slot_order_h, occ_count_h;
     int n_occ = 0, occ_cap = 0;
     void *arena[2] = {nullptr, nullptr};
@@ -448,6 +449,10 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     HIPCHK(h, hipGetLastError());
     h->cur = 1 - h->cur;
     h->n_occ = (int)n_occ;
+    if (n_occ > 0) {
+        k_slot_mat<<<n_occ, 256, 0, h->stream>>>(parr<real>(h, h->cur).mat, h->occ_start, h->occ_count, h->slot_mat);
+        HIPCHK(h, hipGetLastError());
+    }
     // brick records of both parities: stale slots must read -1; the rebuild
     // below writes this parity's records of the occupied bricks
     for (int b = 0; b < 2; ++b)
@@ -489,6 +494,7 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.occ_count = h->occ_count;
     a.n_occ = h->n_occ_d;
     a.slot_order = h->slot_order;
+    a.slot_mat = h->slot_mat;
     a.occ_cap = h->n_occ;    // brick blocks first, platen blocks after them
     a.mats = h->mats;
     a.steps = h->steps_d;
@@ -1029,7 +1035,7 @@ static void destroy_all(mpm_handle *h)
     void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v[0], (void **)&h->dc_v[1], (void **)&h->dc_m, &h->zero_rec,
                      (void **)&h->fixed, (void **)&h->marks[0], (void **)&h->marks[1], (void **)&h->occ_list,
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d, (void **)&h->brick_slot,
-                     (void **)&h->slot_order, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
+                     (void **)&h->slot_order, (void **)&h->slot_mat, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
                      (void **)&h->steps_d,
                      (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->hist_d, (void **)&h->grid3_d,
                      (void **)&h->fy_slot, (void **)&h->fy_off, (void **)&h->fy_val, (void **)&h->ghist_d,
@@ -1165,7 +1171,7 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     }
     const size_t nb = (size_t)h->nbricks;
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
-        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) || (rc = dalloc(h, &h->slot_order, nb)) ||
+        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) || (rc = dalloc(h, &h->slot_order, nb)) || (rc = dalloc(h, &h->slot_mat, nb)) ||
         (rc = dalloc(h, &h->binfo[0], nb)) || (rc = dalloc(h, &h->binfo[1], nb)) ||
         (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)) || (rc = dalloc(h, &h->grid3_d, 3)))
         return fail(rc);

@@ -418,6 +418,7 @@ struct KpArgs {
     unsigned long long *marks_out;        // ... of step n+1
     const uint32_t *occ_list, *occ_start, *occ_count;
     const uint32_t *slot_order;       // brick block b works on slot slot_order[b]
+    const uint8_t *slot_mat;          // per slot: the material of all its points, or 0xFF if mixed
     const int32_t *n_occ;
     int32_t occ_cap;
     const Material *mats;
@@ -438,7 +439,8 @@ struct KpArgs {
 // F is the stored deformation gradient (F - I in FP32 state, see fd()).
 template <typename real>
 __device__ __forceinline__ void material_update(const GridParams &g, const PArrays<real> &P, const Material *mats,
-                                                uint32_t i, const real L[9], real dt, const real F[9], real s[6])
+                                                uint32_t i, const real L[9], real dt, const real F[9], real s[6],
+                                                int umat = -1)
 {
     constexpr bool FP32 = sizeof(real) == 4;
     // strain increment (…_MP.cpp:304-312); FP32 state uses L dt
@@ -459,7 +461,17 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
 #pragma unroll
         for (int q = 0; q < 6; ++q) elt(P.eps[q], i) += de[q];
     // constitutive dispatch (…_MP.cpp:322-353)
-    const Material &mt = mats[elt(P.mat, i)];
+    // FP32 state: umat >= 0 means every point of the block has that material,
+    // read with one block-uniform load instead of the point's index and then
+    // its entry.  FP64 state reads the point's entry.
+    Material mtv{};
+    if constexpr (FP32) {
+        if (umat >= 0)
+            mtv = mats[umat];
+        else
+            mtv = mats[elt(P.mat, i)];
+    }
+    const Material &mt = FP32 ? mtv : mats[elt(P.mat, i)];
     int type = mt.type;
     if (type == 1 && !g.neohookean) type = 3;
     if (type == 2 || type == 5) type = 3;
@@ -535,6 +547,31 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) elt(P.s[q], i) = s[q];
+}
+
+// Per occupied brick: the material index shared by all its points, or 0xFF
+// if they differ (material indices are below 255).
+__global__ void __launch_bounds__(256) k_slot_mat(const uint8_t *mat, const uint32_t *occ_start,
+                                                  const uint32_t *occ_count, uint8_t *slot_mat)
+{
+    __shared__ int smin, smax;
+    const int s = blockIdx.x;
+    if (threadIdx.x == 0) {
+        smin = 255;
+        smax = 0;
+    }
+    __syncthreads();
+    const uint32_t st = occ_start[s], cnt = occ_count[s];
+    int mn = 255, mx = 0;
+    for (uint32_t t = threadIdx.x; t < cnt; t += 256) {
+        const int v = mat[st + t];
+        mn = min(mn, v);
+        mx = max(mx, v);
+    }
+    atomicMin(&smin, mn);
+    atomicMax(&smax, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) slot_mat[s] = (smin == smax) ? (uint8_t)smin : (uint8_t)0xFF;
 }
 
 // ---------------------------------------------------- batched constitutive
@@ -640,6 +677,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     const int bc[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
     const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
     const uint32_t start = a.occ_start[slot], count = a.occ_count[slot];
+    const int umat = (FP32 && a.slot_mat[slot] != 0xFF) ? (int)a.slot_mat[slot] : -1;
     const int A0 = g.A[0], A01 = g.A[0] * g.A[1], AN = g.arenaN;
 
     // ---- phase A: grid update of step n over the node window = this brick's
@@ -927,7 +965,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                     }
 #pragma unroll
                     for (int q = 0; q < 9; ++q) elt(P.F[q], i) = F[q];
-                    material_update<real>(g, P, a.mats, i, L, dt, F, s);
+                    material_update<real>(g, P, a.mats, i, L, dt, F, s, umat);
                 }
             } else {
                 double Fd[9];
