@@ -440,7 +440,7 @@ struct KpArgs {
 template <typename real>
 __device__ __forceinline__ void material_update(const GridParams &g, const PArrays<real> &P, const Material *mats,
                                                 uint32_t i, const real L[9], real dt, const real F[9], real s[6],
-                                                int umat = -1)
+                                                int umat = -1, bool have_alpha = false, real alpha_pre = real(0))
 {
     constexpr bool FP32 = sizeof(real) == 4;
     // strain increment (…_MP.cpp:304-312); FP32 state uses L dt
@@ -509,7 +509,8 @@ __device__ __forceinline__ void material_update(const GridParams &g, const PArra
     } else if constexpr (FP32) {
         // FP64 island for FP32 state
         double dep[6];
-        const double al = (type == 4) ? (double)elt(P.alpha, i) : 0.0;
+        // have_alpha: the caller loaded alpha with the point's other state
+        const double al = (type == 4) ? (double)(have_alpha ? alpha_pre : elt(P.alpha, i)) : 0.0;
         const double radius = mt.k0 + al * 0.70710678118654752440;   // al / sqrt(2)
         if (radial_return_fp32(mt.nu, C, ie, ig, radius, s, de, dep)) {
             if (type == 4) {
@@ -678,6 +679,8 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
     const uint32_t start = a.occ_start[slot], count = a.occ_count[slot];
     const int umat = (FP32 && a.slot_mat[slot] != 0xFF) ? (int)a.slot_mat[slot] : -1;
+    // FP32: load alpha with the point state when the brick may hold VonMises points
+    const bool pre_al = FP32 && (umat < 0 || a.mats[umat].type == 4);
     const int A0 = g.A[0], A01 = g.A[0] * g.A[1], AN = g.arenaN;
 
     // ---- phase A: grid update of step n over the node window = this brick's
@@ -876,6 +879,8 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             for (int q = 0; q < 6; ++q) s[q] = elt(P.s[q], i);
             m = elt(P.mass, i);
             const real V0 = elt(P.vol0, i);
+            real al_pre = real(0);
+            if (pre_al) al_pre = elt(P.alpha, i);
             real V = real(0);
 
             if (G2P) {
@@ -965,7 +970,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                     }
 #pragma unroll
                     for (int q = 0; q < 9; ++q) elt(P.F[q], i) = F[q];
-                    material_update<real>(g, P, a.mats, i, L, dt, F, s, umat);
+                    material_update<real>(g, P, a.mats, i, L, dt, F, s, umat, pre_al, al_pre);
                 }
             } else {
                 double Fd[9];
