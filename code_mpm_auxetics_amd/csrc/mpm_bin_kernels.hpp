@@ -374,17 +374,17 @@ __global__ void __launch_bounds__(256) k_sum3(const double *part, const int32_t 
     if (t < 3) out3[t] = s[t][0];
 }
 
-// FP64 check sums (mpm_checks): point partials (count *n_pp_d or n_pp) in a
-// fixed tree, the last grid update's totals (grid3: mass, reaction, active
-// nodes) and the platen points.
+// FP64 check sums (mpm_checks): point partials (count mul * *n_pp_d, or
+// n_pp) in a fixed tree, the last grid update's totals (grid3: mass, reaction,
+// active nodes) and the platen points.
 __global__ void __launch_bounds__(256) k_finalize(const double *kp_part, const int32_t *n_pp_d, int n_pp,
                                                  const double *grid3, const double *dc_v, const double *dc_m,
-                                                 int n_dc, double *out)
+                                                 int n_dc, double *out, int mul = 1)
 {
     __shared__ double s[5][256];
     const int t = threadIdx.x;
     double r[5] = {0, 0, 0, 0, 0};
-    const int npp = n_pp_d ? *n_pp_d : n_pp;
+    const int npp = n_pp_d ? mul * *n_pp_d : n_pp;
     for (int i = t; i < npp; i += 256)
         for (int k = 0; k < 5; ++k) r[k] += kp_part[5 * i + k];
     for (int j = t; j < n_dc; j += 256) {

@@ -331,8 +331,8 @@ static int ensure_slots(mpm_handle *h, int n_occ)
         h->arena_cap_slots = (size_t)want_occ;
         for (int b = 0; b < 2; ++b)
             RCHK(dev_alloc(h, &h->arena[b], h->arena_cap_slots * 8 * h->g.arenaN * h->rsize));
-        RCHK(dalloc(h, &h->kp_part, h->arena_cap_slots * 5));
-        HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * 8, h->stream));
+        RCHK(dalloc(h, &h->kp_part, h->arena_cap_slots * 5 * (kPBlock / 64)));
+        HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * (kPBlock / 64) * 8, h->stream));
         RCHK(dalloc(h, &h->gpart, (size_t)h->steps_cap * h->arena_cap_slots * 3));
         h->occ_cap = want_occ;
         h->layout++;
@@ -975,7 +975,7 @@ static int checks_impl(mpm_handle *h, double out[12])
                                            h->dc_m, h->n_dc, h->checks_d);
     else
         k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, 0, h->grid3_d, h->dc_v[h->parity], h->dc_m,
-                                           h->n_dc, h->checks_d);
+                                           h->n_dc, h->checks_d, kPBlock / 64);
     HIPCHK(h, hipGetLastError());
     double r[8] = {0};
     HIPCHK(h, hipMemcpyAsync(r, h->checks_d, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));

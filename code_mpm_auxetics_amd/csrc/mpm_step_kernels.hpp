@@ -424,7 +424,7 @@ struct KpArgs {
     const Material *mats;
     const StepRec *steps;
     int32_t *flags;
-    double *kp_part;      // per slot: 5 doubles (sum m|v|^2, m v.xyz, nonfinite)
+    double *kp_part;      // per slot and wave: 5 doubles (sum m|v|^2, m v.xyz, nonfinite)
     double *gpart;        // per slot: 3 doubles of this step's grid update (mass, reaction, active nodes)
     int32_t n_dc;
     const int32_t *fy_slot;             // batched worlds: node -> y-fixed slot or -1 (null: off)
@@ -1153,21 +1153,19 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             store8(out + 8 * (size_t)r, rec);
         }
     }
-    // ---- fixed-order block sums of the checks, flags
+    // ---- check sums: one fixed-order partial per wave, written by the wave
+    // itself (no block barrier at the end); flags: OR over the wave, only when
+    // a lane has one
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const double wsum = wave_sum_dpp(red[k]);
-        if (lane == 0) s_red[wv][k] = wsum;
+        if (lane == 0) a.kp_part[5 * ((size_t)slot * (kPBlock / 64) + wv) + k] = wsum;
     }
-    int fl = flags;
+    if (__ballot(flags != 0)) {
+        int fl = flags;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
-    if (lane == 0 && fl) atomicOr(a.flags, fl);
-    __syncthreads();
-    if (tid < 5) {
-        double r = 0.0;
-        for (int w = 0; w < kPBlock / 64; ++w) r += s_red[w][tid];
-        a.kp_part[5 * slot + tid] = r;
+        for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
+        if (lane == 0) atomicOr(a.flags, fl);
     }
 }
 
