@@ -1111,9 +1111,9 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             }
         }
     }
-    __syncthreads();
 
-    // ---- touched node box of the brick
+    // ---- touched node box of the brick: wave bounds first, so that one
+    // barrier covers both them and the arena adds
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const int lo = wave_imin_dpp(bmin[q]), hi = wave_imax_dpp(bmax[q]);
