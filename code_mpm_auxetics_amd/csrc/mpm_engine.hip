@@ -83,7 +83,6 @@ struct mpm_handle {
     std::vector<double> dc_saved_store;
 
     // grid: no node arrays -- each step's grid update lives in LDS
-    void *zero_rec = nullptr;               // 8 zero reals (uncovered arena candidates)
     uint8_t *fixed = nullptr;
     unsigned long long *marks[2] = {nullptr, nullptr};   // platen override tags, ping-pong
 
@@ -328,9 +327,19 @@ static int ensure_slots(mpm_handle *h, int n_occ)
         for (int b = 0; b < 2; ++b) dev_free(h->arena[b]);
         dev_free((void *&)h->kp_part);
         dev_free((void *&)h->gpart);
+        // the step kernel addresses arena records with 32-bit element
+        // offsets; one record of 8 zeros after the last slot is what an
+        // uncovered candidate node reads
+        const size_t elems = (size_t)want_occ * 8 * h->g.arenaN;
+        if (elems + 8 > 0xFFFFFFFFull) {
+            h->err = "brick arenas exceed 2^32 elements (use larger bricks)";
+            return MPM_E_ARG;
+        }
         h->arena_cap_slots = (size_t)want_occ;
-        for (int b = 0; b < 2; ++b)
-            RCHK(dev_alloc(h, &h->arena[b], h->arena_cap_slots * 8 * h->g.arenaN * h->rsize));
+        for (int b = 0; b < 2; ++b) {
+            RCHK(dev_alloc(h, &h->arena[b], (elems + 8) * h->rsize));
+            HIPCHK(h, hipMemsetAsync((char *)h->arena[b] + elems * h->rsize, 0, 8 * h->rsize, h->stream));
+        }
         RCHK(dalloc(h, &h->kp_part, h->arena_cap_slots * 5 * (kPBlock / 64)));
         HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * (kPBlock / 64) * 8, h->stream));
         RCHK(dalloc(h, &h->gpart, (size_t)h->steps_cap * h->arena_cap_slots * 3));
@@ -485,7 +494,7 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.binfo_in = h->binfo[p];
     a.arena_out = (real *)h->arena[g2p ? (p ^ 1) : p];
     a.binfo_out = h->binfo[g2p ? (p ^ 1) : p];
-    a.zero_rec = (const real *)h->zero_rec;
+    a.zero_off = (uint32_t)(h->arena_cap_slots * 8 * h->g.arenaN);
     a.fixed = h->fixed;
     a.marks_in = h->marks[p];
     a.marks_out = h->marks[p ^ 1];
@@ -1032,7 +1041,7 @@ static void destroy_all(mpm_handle *h)
                        (void **)&h->dc_id};
         for (void **q : cp) dev_free(*q);
     }
-    void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v[0], (void **)&h->dc_v[1], (void **)&h->dc_m, &h->zero_rec,
+    void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v[0], (void **)&h->dc_v[1], (void **)&h->dc_m,
                      (void **)&h->fixed, (void **)&h->marks[0], (void **)&h->marks[1], (void **)&h->occ_list,
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d, (void **)&h->brick_slot,
                      (void **)&h->slot_order, (void **)&h->slot_mat, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
@@ -1154,8 +1163,6 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     h->nnodes = (long long)g.nodes[0] * g.nodes[1] * g.nodes[2];
 
     int rc;
-    if ((rc = dev_alloc(h, &h->zero_rec, 64))) return fail(rc);
-    if (hipMemset(h->zero_rec, 0, 64) != hipSuccess) { h->err = "hipMemset failed"; return fail(MPM_E_HIP); }
     if ((rc = dalloc(h, &h->fixed, (size_t)h->nnodes))) return fail(rc);
     for (int b = 0; b < 2; ++b) {
         if ((rc = dalloc(h, &h->marks[b], (size_t)h->nnodes))) return fail(rc);

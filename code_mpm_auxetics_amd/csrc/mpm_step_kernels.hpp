@@ -412,7 +412,7 @@ struct KpArgs {
     const int4 *binfo_in;       // step n: per brick {slot or -1, packed touched box}
     real *arena_out;            // step n+1
     int4 *binfo_out;            // step n+1
-    const real *zero_rec;       // 8 zero reals: the record of an uncovered candidate
+    uint32_t zero_off;          // element offset in arena_in of a record of 8 zeros (after the last slot)
     const uint8_t *fixed;       // node fixity bits
     const unsigned long long *marks_in;   // platen override tags of step n
     unsigned long long *marks_out;        // ... of step n+1
@@ -633,7 +633,9 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     extern __shared__ long long s_arena[];   // node-major: kQ int64 per arena node, then
                                              // (G2P) the node window: 6 reals {v, a} per node
     __shared__ uint32_t s_max[8];            // running max bound bits per field
-    __shared__ int4 s_bi[27];                // neighbour brick records (dz, dy, dx ascending)
+    __shared__ int4 s_nb[27][2];             // neighbour arenas (dz, dy, dx ascending): {element
+                                             // base, window low corner in this brick's node
+                                             // coordinates}, {window extent, ext x * ext y}
     __shared__ int s_box[kPBlock / 64][6];
     __shared__ double s_red[kPBlock / 64][5];
 
@@ -689,69 +691,62 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     int wlo[3] = {0, 0, 0}, wext[3] = {0, 0, 0};
     if (G2P) {
         if (tid < 27) {
-            const int x = bc[0] + tid % 3 - 1, y = bc[1] + (tid / 3) % 3 - 1, z = bc[2] + tid / 9 - 1;
+            const int d[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
+            const int x = bc[0] + d[0], y = bc[1] + d[1], z = bc[2] + d[2];
             int4 bi = make_int4(-1, 0, 0, 0);
             if (x >= 0 && y >= 0 && z >= 0 && x < g.nb[0] && y < g.nb[1] && z < g.nb[2])
                 bi = a.binfo_in[x + (size_t)g.nb[0] * (y + (size_t)g.nb[1] * z)];
-            s_bi[tid] = bi;
+            // a missing neighbour gets an empty window (extent 0)
+            const int pk[3] = {bi.y, bi.z, bi.w};
+            int lo[3], ext[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                lo[q] = (pk[q] & 0xFFFF) + d[q] * g.S[q];
+                ext[q] = (bi.x >= 0) ? (pk[q] >> 16) - (pk[q] & 0xFFFF) + 1 : 0;
+            }
+            s_nb[tid][0] = make_int4((int)((uint32_t)max(bi.x, 0) * 8u * (uint32_t)AN), lo[0], lo[1], lo[2]);
+            s_nb[tid][1] = make_int4(ext[0], ext[1], ext[2], ext[0] * ext[1]);
         }
         __syncthreads();
-        const int4 own = s_bi[13];
-        const int opk[3] = {own.y, own.z, own.w};
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            wlo[q] = opk[q] & 0xFFFF;
-            wext[q] = (own.x >= 0) ? (opk[q] >> 16) - wlo[q] + 1 : 0;
-        }
+        const int4 own0 = s_nb[13][0], own1 = s_nb[13][1];
+        wlo[0] = own0.y; wlo[1] = own0.z; wlo[2] = own0.w;
+        wext[0] = own1.x; wext[1] = own1.y; wext[2] = own1.z;
         const int nwin = wext[0] * wext[1] * wext[2];
+        // window index -> node: quotients through a float reciprocal, exact
+        // for r < 2^20 (the fraction of (r + 1/2) / e stays 1/(2e) away from
+        // an integer, far above the float error)
+        const float rx = 1.0f / (float)max(wext[0], 1), rxy = 1.0f / (float)max(own1.w, 1);
         const real dtg = (real)st.dt, m_min = (real)g.m_min, damp = (real)g.damping;
         // Pass 1: for each window node, the sum of the covering arenas'
         // records (at most 8 candidates, ascending brick order, uncovered ones
-        // read a zero record), its fixity and override tag, stashed in LDS:
+        // read the zero record), its fixity and override tag, stashed in LDS:
         // sums in the (not yet used) arena, {tag, fixity | counted} in the
         // node's window slot.  Branch-free, so that all loads of a node are in
         // flight at once.  Each thread later reads back only the slots it
         // wrote: no barrier.
         real *s_sum = reinterpret_cast<real *>(s_arena);
         for (int r = tid; r < nwin; r += kPBlock) {
-            const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
+            const int lz = (int)(((float)r + 0.5f) * rxy), rem = r - lz * own1.w;
+            const int ly = (int)(((float)rem + 0.5f) * rx), lx = rem - ly * wext[0];
+            const int l[3] = {wlo[0] + lx, wlo[1] + ly, wlo[2] + lz};
             const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
             // per axis the arenas covering this node: own and the neighbour
-            // below (l <= 2) or above (l >= S)
+            // below (l <= 2) or above (l >= S); the candidates are the 2 x 2 x 2
+            // neighbours from `lower` up, and a candidate counts when the node
+            // lies in its window (brick sizes >= 3 keep the two sides apart)
             int lower[3];
-            bool two[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const bool lo_n = l[q] <= 2 && bc[q] > 0, hi_n = l[q] >= g.S[q] && bc[q] + 1 < g.nb[q];
-                lower[q] = lo_n ? -1 : 0;
-                two[q] = lo_n || hi_n;
-            }
-            const real *src[8];
+            for (int q = 0; q < 3; ++q) lower[q] = (l[q] <= 2 && bc[q] > 0) ? -1 : 0;
+            const int nb0 = (lower[2] + 1) * 9 + (lower[1] + 1) * 3 + (lower[0] + 1);
+            uint32_t eo[8];
             int first = -1;
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const int e[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
-                int dd[3];
-                bool ok = true;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    dd[q] = lower[q] + e[q];
-                    ok &= !(e[q] && !two[q]);
-                }
-                const int4 bi = s_bi[(dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1)];
-                const int pk[3] = {bi.y, bi.z, bi.w};
-                int lp[3], lo[3], ext[3];
-                ok &= bi.x >= 0;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    lp[q] = l[q] - dd[q] * g.S[q];   // node in that arena's window
-                    lo[q] = pk[q] & 0xFFFF;
-                    const int hi = pk[q] >> 16;
-                    ext[q] = hi - lo[q] + 1;
-                    ok &= lp[q] >= lo[q] && lp[q] <= hi;
-                }
-                const int rr = (lp[0] - lo[0]) + ext[0] * ((lp[1] - lo[1]) + ext[1] * (lp[2] - lo[2]));
-                src[c] = ok ? a.arena_in + (size_t)bi.x * 8 * AN + 8 * (size_t)rr : a.zero_rec;
+                const int4 w0 = s_nb[nb0 + (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1)][0];
+                const int4 w1 = s_nb[nb0 + (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1)][1];
+                const uint32_t t0 = (uint32_t)(l[0] - w0.y), t1 = (uint32_t)(l[1] - w0.z), t2 = (uint32_t)(l[2] - w0.w);
+                const bool ok = t0 < (uint32_t)w1.x && t1 < (uint32_t)w1.y && t2 < (uint32_t)w1.z;
+                eo[c] = ok ? (uint32_t)w0.x + 8u * (t0 + (uint32_t)w1.x * t1 + (uint32_t)w1.w * t2) : a.zero_off;
                 first = (ok && first < 0) ? c : first;
             }
             const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
@@ -764,7 +759,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 real rec[8];
-                load8(src[c], rec);
+                load8(a.arena_in + eo[c], rec);
 #pragma unroll
                 for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
             }
@@ -795,30 +790,37 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
 #pragma unroll
                 for (int q = 0; q < 3; ++q) fn[q] -= sc * (vn[q] * inv);
             }
+            // f / m after damping: the velocity update and the stored
+            // acceleration share it; a force zeroed below gives 0 / m = 0
+            real ac[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) ac[q] = fn[q] / mn;
             if (mn > m_min)
 #pragma unroll
-                for (int q = 0; q < 3; ++q) vn[q] += fn[q] / mn * dtg;
+                for (int q = 0; q < 3; ++q) vn[q] += ac[q] * dtg;
             double react = 0.0;
-            if (fx & 1) { vn[0] = real(0); fn[0] = real(0); }
-            if (fx & 2) { vn[1] = real(0); react = (double)fn[1]; fn[1] = real(0); }
-            if (fx & 4) { vn[2] = real(0); fn[2] = real(0); }
+            if (fx & 1) { vn[0] = real(0); ac[0] = real(0); }
+            if (fx & 2) { vn[1] = real(0); react = (double)fn[1]; ac[1] = real(0); }
+            if (fx & 4) { vn[2] = real(0); ac[2] = real(0); }
             if ((uint32_t)(mk >> 32) == st.epoch + 1u) {   // platen override (…_MP.cpp:245-265)
                 const int j = (int)(mk & 0xFFFFFFFFull);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) { vn[q] = (real)a.dc_v_in[3 * j + q]; fn[q] = real(0); }
+                for (int q = 0; q < 3; ++q) { vn[q] = (real)a.dc_v_in[3 * j + q]; ac[q] = real(0); }
             }
             real *wn = s_node + 6 * r;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 wn[q] = vn[q];
-                wn[3 + q] = (fabs(mn) > m_min) ? fn[q] / mn : real(0);
+                wn[3 + q] = (fabs(mn) > m_min) ? ac[q] : real(0);
             }
             if (fw & 0x100u) {
                 gs[0] += (double)mn;
                 gs[1] += react;
                 gs[2] += (mn > real(0)) ? 1.0 : 0.0;
                 if (a.fy_slot && (fx & 2)) {
-                    const int l[3] = {wlo[0] + r % wext[0], wlo[1] + (r / wext[0]) % wext[1], wlo[2] + r / (wext[0] * wext[1])};
+                    const int lz = (int)(((float)r + 0.5f) * rxy), rem = r - lz * own1.w;
+                    const int ly = (int)(((float)rem + 0.5f) * rx), lx = rem - ly * wext[0];
+                    const int l[3] = {wlo[0] + lx, wlo[1] + ly, wlo[2] + lz};
                     const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
                     a.fy_val[a.fy_slot[nid]] = react;
                 }
@@ -1143,8 +1145,12 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         double scl[kQ];
 #pragma unroll
         for (int q = 0; q < kQ; ++q) scl[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, ecur[q] - FRAC);
+        // record index -> node as in the grid update (float quotients, exact here)
+        const float rbx = 1.0f / (float)bx, rbxy = 1.0f / (float)(bx * by);
         for (int r = tid; r < nbox; r += kPBlock) {
-            const int lx = box[0] + r % bx, ly = box[2] + (r / bx) % by, lz = box[4] + r / (bx * by);
+            const int qz = (int)(((float)r + 0.5f) * rbxy), rem = r - qz * (bx * by);
+            const int qy = (int)(((float)rem + 0.5f) * rbx);
+            const int lx = box[0] + rem - qy * bx, ly = box[2] + qy, lz = box[4] + qz;
             const long long *src = s_arena + kQ * (lx + A0 * ly + A01 * lz);
             real rec[8];
 #pragma unroll
