@@ -221,13 +221,11 @@ __global__ void k_occ(const uint32_t *skeys, const uint32_t *head, const uint32_
     occ_start[pos[j]] = j;
 }
 
-__global__ void k_occ_count(const uint32_t *occ_list, const uint32_t *occ_start, uint32_t n_occ,
-                            uint32_t n, uint32_t *occ_count, int32_t *brick_slot)
+__global__ void k_occ_count(const uint32_t *occ_start, uint32_t n_occ, uint32_t n, uint32_t *occ_count)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_occ) return;
     occ_count[s] = ((s + 1 < n_occ) ? occ_start[s + 1] : n) - occ_start[s];
-    brick_slot[occ_list[s]] = (int32_t)s;
 }
 
 // Spread the points of each brick over the lanes of a wave.  Sorted position

@@ -88,7 +88,7 @@ struct mpm_handle {
 
     // bricks
     uint32_t *occ_list = nullptr, *occ_start = nullptr, *occ_count = nullptr;
-    int32_t *n_occ_d = nullptr, *brick_slot = nullptr;
+    int32_t *n_occ_d = nullptr;
     uint32_t *slot_order = nullptr;          // launch order of the brick blocks
     uint8_t *slot_mat = nullptr;             // per slot: shared material index or 0xFF
     std::vector<uint32_t> slot_order_h, occ_count_h;
@@ -418,9 +418,7 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     RCHK(ensure_slots(h, (int)n_occ));
     k_occ<<<nbk, kBlock, 0, h->stream>>>(sk, h->heads, h->headpos, n, h->g.cbits, h->occ_list, h->occ_start);
     HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipMemsetAsync(h->brick_slot, 0xFF, (size_t)h->nbricks * 4, h->stream));
-    k_occ_count<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->occ_list, h->occ_start, n_occ, n,
-                                                              h->occ_count, h->brick_slot);
+    k_occ_count<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->occ_start, n_occ, n, h->occ_count);
     HIPCHK(h, hipGetLastError());
     // launch order of the brick blocks: most point chunks first, ties in slot
     // order (a longest-first schedule; every slot still sums in slot order)
@@ -1043,7 +1041,7 @@ static void destroy_all(mpm_handle *h)
     }
     void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v[0], (void **)&h->dc_v[1], (void **)&h->dc_m,
                      (void **)&h->fixed, (void **)&h->marks[0], (void **)&h->marks[1], (void **)&h->occ_list,
-                     (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d, (void **)&h->brick_slot,
+                     (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d,
                      (void **)&h->slot_order, (void **)&h->slot_mat, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
                      (void **)&h->steps_d,
                      (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->hist_d, (void **)&h->grid3_d,
@@ -1178,7 +1176,7 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     }
     const size_t nb = (size_t)h->nbricks;
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
-        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->brick_slot, nb)) || (rc = dalloc(h, &h->slot_order, nb)) || (rc = dalloc(h, &h->slot_mat, nb)) ||
+        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->slot_order, nb)) || (rc = dalloc(h, &h->slot_mat, nb)) ||
         (rc = dalloc(h, &h->binfo[0], nb)) || (rc = dalloc(h, &h->binfo[1], nb)) ||
         (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)) || (rc = dalloc(h, &h->grid3_d, 3)))
         return fail(rc);
