@@ -137,7 +137,10 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out);
  * v_MarkedMaterialPoints_Displacement_Control vectors that initializeWorld_*
  * fills (PhysicsEngine.h:78-80, …PolygonCell.cpp:137-191).  Points with dc!=0
  * are displacement-controlled (platens), in index order.  At most 2^28 points
- * (MPM_E_ARG beyond); a non-finite real input is MPM_E_ARG as well. */
+ * (MPM_E_ARG beyond); a non-finite real input is MPM_E_ARG as well.  The
+ * brick arenas (8 (Sx+3)(Sy+3)(Sz+3) reals per occupied brick, for 1.25x the
+ * occupied bricks) must stay below 2^32 elements: beyond, this call or the
+ * mpm_step whose rebin needs more returns MPM_E_ARG. */
 int mpm_upload_points(mpm_handle *h, const mpm_points *pts);
 
 /* Advance by dt_total in steps of at most dt_max.  Replaces
