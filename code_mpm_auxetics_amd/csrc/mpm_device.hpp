@@ -147,7 +147,7 @@ __device__ __forceinline__ double sgn(double a) { return (double)((0.0 < a) - (a
 __device__ __forceinline__ float sgn(float a) { return (float)((0.0f < a) - (a < 0.0f)); }
 
 // glm determinant of a column-major 3x3 (…_MP.cpp:301), always in FP64.
-__device__ __forceinline__ double det3(const double m[9])
+__device__ __host__ __forceinline__ double det3(const double m[9])
 {
     return +m[0] * (m[4] * m[8] - m[7] * m[5])
            - m[3] * (m[1] * m[8] - m[7] * m[2])

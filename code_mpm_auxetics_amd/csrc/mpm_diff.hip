@@ -1,10 +1,13 @@
 // mpm_diff.hip -- differentiable FP64 run of the plain step and its adjoint.
 //
 // The forward is the plain step of PhysicsEngine_runSimulation_Classic_-
-// SinglePass_MP.cpp:41-368 for elastic points (…Elastic.cpp:4-15) with the
-// strain increment L dt, on a dense node grid: P2G (…:106-202), grid update
-// with damping, boundary conditions and the platen override (…:209-265), G2P
-// with F, volume and stress update (…:271-319).  The backward pass is the
+// SinglePass_MP.cpp:41-368 with the strain increment L dt, on a dense node
+// grid: P2G (…:106-202), grid update with damping, boundary conditions and the
+// platen override (…:209-265), G2P with F, volume and the constitutive update
+// (…:271-353) for elastic (…Elastic.cpp:4-15), NeoHookean (…NeoHookean.cpp:
+// 4-22, when dispatched), perfectly plastic (…PerfectlyPlastic.cpp:4-72) and
+// J2-hardening points (…VonMisesHardening.cpp:4-86; the hardening variable
+// alpha is part of the state).  The backward pass is the
 // hand-written adjoint of each phase in reverse order; the forward keeps
 // checkpoints every C steps and the backward recomputes one segment at a time.
 //
@@ -18,12 +21,14 @@
 // 1e-9 m on positions of 1e-2 m keep full FP64 precision; the run's output
 // is u.
 //
-// Adjoint of one step (state s = x, u, v, F, sigma; parameters E; N and grad N
-// evaluated at the step's starting positions; piecewise-constant choices --
-// cells, mass gates, boundary masks, platen nodes -- have zero derivative):
+// Adjoint of one step (state s = x, u, v, F, sigma, alpha; parameters E; N and
+// grad N evaluated at the step's starting positions; piecewise-constant
+// choices -- cells, mass gates, boundary masks, platen nodes, each point's
+// yield and hardening decision -- have zero derivative):
 //   G2P      x' = x + dt sum_I N vt_I (u' likewise), v' = v + dt sum_I N a_I,
 //            L = sum_I vt_I (x) grad N,  F' = F + dt L F,
-//            sigma' = sigma + C(E) eps(L dt)
+//            sigma' = sigma + C(E) eps(L dt) (elastic), P(F') (NeoHookean),
+//            or the radial return of the trial stress (plastic laws)
 //   grid     vh = P/m, vt = mask(vh + f/m dt) (or the platen velocity),
 //            a = mask(f)/m (0 on platen nodes)
 //   P2G      m_I = sum N m, P_I = sum N m v, f_I = -sum V sigma.grad N,
@@ -58,8 +63,10 @@ struct DArgs {
     int n, n_dc;
     double dt, damping;
     // point state (SoA, component-major: x[a * n + p])
-    double *x, *u, *v, *F, *s;
+    double *x, *u, *v, *F, *s, *al;   // al: hardening variable alpha (n)
     const double *m, *V0, *E, *nu, *fext;
+    const uint8_t *law;              // per point: LAW_* after the reference's dispatch
+    const double *sy, *C0, *C1;      // yield stress, hardening constants
     const uint8_t *fixed;
     double *xdc, *vdc;             // platen positions / velocities (3 * n_dc)
     double vtl[3];                 // TimeLine velocity of this step (platen move)
@@ -75,7 +82,7 @@ struct DArgs {
     double *nsc;                   // 5 x nn fixed-point scales of the node fields
     int32_t *err;
     // adjoint
-    double *xb, *vb, *Fb, *sb, *Eb;
+    double *xb, *vb, *Fb, *sb, *Eb, *alb;
     double *scratch;               // 12 per point: L-bar (9), dt x-bar' (3)
     long long *aacc;               // 6 x nn: vt-bar xyz | a-bar xyz
     double *nmb, *nPb, *nfb;
@@ -160,7 +167,7 @@ __device__ __forceinline__ void matmul(const double A[9], const double B[9], dou
         for (int r = 0; r < 3; ++r) C[3 * c + r] = A[r] * B[3 * c] + A[3 + r] * B[3 * c + 1] + A[6 + r] * B[3 * c + 2];
 }
 // elastic stress increment of eps (engineering shear): Hooke (…Elastic.cpp:4-15)
-__device__ __forceinline__ void hooke(double E, double nu, const double de[6], double ds[6])
+__device__ __host__ __forceinline__ void hooke(double E, double nu, const double de[6], double ds[6])
 {
     const double C = E / (1.0 + nu) / (1.0 - 2.0 * nu);
     ds[0] = C * ((1.0 - nu) * de[0] + nu * de[1] + nu * de[2]);
@@ -180,6 +187,204 @@ __device__ __forceinline__ void eps_of(const double L[9], double dt, double de[6
     de[4] = L[5] * dt + L[7] * dt;
     de[5] = L[6] * dt + L[2] * dt;
 }
+// Material laws of the run, after the reference's dispatch (…_MP.cpp:332-339):
+// elastic, NeoHookean (when dispatched), perfectly plastic, J2 with
+// hardening.  Forward in the order of the plain step's FP64 functions.
+enum { LAW_ELASTIC = 0, LAW_NEOHOOKEAN = 1, LAW_PLASTIC = 3, LAW_VONMISES = 4 };
+
+// J2 of a Voigt stress (…calculateState_J2.cpp:4-26)
+__device__ __host__ __forceinline__ double j2v(const double s[6])
+{
+    double j2 = 0.0;
+    j2 += 1.0 / 6.0 * (s[0] - s[1]) * (s[0] - s[1]);
+    j2 += 1.0 / 6.0 * (s[1] - s[2]) * (s[1] - s[2]);
+    j2 += 1.0 / 6.0 * (s[2] - s[0]) * (s[2] - s[0]);
+    j2 += s[3] * s[3];
+    j2 += s[4] * s[4];
+    j2 += s[5] * s[5];
+    return j2;
+}
+
+// Radial return of …PerfectlyPlastic.cpp:4-72 (k = sigma_y / sqrt 3) and
+// …VonMisesHardening.cpp:4-86 (radius k + alpha / sqrt 2; on the plastic
+// branch d alpha = C0 (sqrt(2/3) C1 - alpha) eps_bar when positive, eps_bar =
+// sqrt(2/3 sum dep_i^2)), with every intermediate kept for the adjoint.
+struct RR {
+    double del[6], st[6], p, dev[6], j2, radius, sc, ds[6], dee[6], dep[6], eff, hs, da;
+    bool plastic, harden;
+};
+__device__ __host__ __forceinline__ void rr_eval(double E, double nu, double sy, double C0, double C1, bool vm,
+                                        const double s[6], double al, const double de[6], RR &r)
+{
+    hooke(E, nu, de, r.del);
+    for (int i = 0; i < 6; ++i) r.st[i] = s[i] + r.del[i];
+    r.p = 1.0 / 3.0 * (r.st[0] + r.st[1] + r.st[2]);
+    for (int i = 0; i < 6; ++i) r.dev[i] = r.st[i];
+    r.dev[0] -= r.p;
+    r.dev[1] -= r.p;
+    r.dev[2] -= r.p;
+    r.j2 = j2v(r.st);
+    const double k0 = sy / sqrt(3.0);
+    r.radius = vm ? k0 + al / sqrt(2.0) : k0;
+    r.plastic = !(r.j2 <= r.radius * r.radius);
+    r.harden = false;
+    r.sc = 1.0;
+    r.eff = r.hs = r.da = 0.0;
+    for (int i = 0; i < 6; ++i) r.dee[i] = r.dep[i] = 0.0;
+    if (!r.plastic) {
+        for (int i = 0; i < 6; ++i) r.ds[i] = r.del[i];
+        return;
+    }
+    r.sc = r.radius / sqrt(r.j2);
+    for (int i = 0; i < 6; ++i) r.ds[i] = r.dev[i] * r.sc - s[i];
+    r.ds[0] += r.p;
+    r.ds[1] += r.p;
+    r.ds[2] += r.p;
+    const double G = 0.5 * E / (1.0 + nu);
+    r.dee[0] = r.ds[0] / E - r.ds[1] * nu / E - r.ds[2] * nu / E;
+    r.dee[1] = r.ds[1] / E - r.ds[0] * nu / E - r.ds[2] * nu / E;
+    r.dee[2] = r.ds[2] / E - r.ds[0] * nu / E - r.ds[1] * nu / E;
+    r.dee[3] = r.ds[3] / G;
+    r.dee[4] = r.ds[4] / G;
+    r.dee[5] = r.ds[5] / G;
+    for (int i = 0; i < 6; ++i) r.dep[i] = de[i] - r.dee[i];
+    if (vm) {
+        double eff = 0.0;
+        for (int q = 0; q < 6; ++q) eff += r.dep[q] * r.dep[q];
+        eff *= 2.0 / 3.0;
+        r.eff = sqrt(eff);
+        r.hs = sqrt(2.0 / 3.0) * C1 - al;
+        r.harden = r.hs > 0.0;
+        if (r.harden) r.da = C0 * r.hs * r.eff;
+    }
+}
+
+// Adjoint of rr_eval with its branches held (s' = s + ds, alpha' = alpha + da).
+// In: sb = d/ds', ab = d/dalpha'.  Out: sbo = d/ds, abo = d/dalpha, eb = d/dde
+// (engineering shear), gE = d/dE.
+__device__ __host__ __forceinline__ void rr_adjoint(double E, double nu, double C0, bool vm, const double de[6], const RR &r,
+                                           const double sb[6], double ab, double sbo[6], double &abo, double eb[6],
+                                           double &gE)
+{
+    for (int i = 0; i < 6; ++i) { sbo[i] = sb[i]; eb[i] = 0.0; }
+    abo = ab;
+    gE = 0.0;
+    double delb[6];
+    if (!r.plastic) {
+        for (int i = 0; i < 6; ++i) delb[i] = sb[i];
+    } else {
+        // alpha' = alpha + C0 hs eff, hs = sqrt(2/3) C1 - alpha
+        double effb = 0.0;
+        if (vm && r.harden) {
+            abo -= ab * C0 * r.eff;
+            effb = ab * C0 * r.hs;
+        }
+        // eff = sqrt(2/3 sum dep^2)
+        double depb[6];
+        for (int i = 0; i < 6; ++i) depb[i] = (r.eff > 0.0) ? effb * (2.0 / 3.0) * r.dep[i] / r.eff : 0.0;
+        // dep = de - dee(ds), dee = compliance(E) ds
+        double dsb[6];
+        for (int i = 0; i < 6; ++i) {
+            eb[i] += depb[i];
+            gE += depb[i] * r.dee[i] / E;
+        }
+        const double G = 0.5 * E / (1.0 + nu);
+        dsb[0] = sb[0] - (depb[0] - nu * depb[1] - nu * depb[2]) / E;
+        dsb[1] = sb[1] - (depb[1] - nu * depb[0] - nu * depb[2]) / E;
+        dsb[2] = sb[2] - (depb[2] - nu * depb[0] - nu * depb[1]) / E;
+        for (int i = 3; i < 6; ++i) dsb[i] = sb[i] - depb[i] / G;
+        // ds = dev sc - s + p (normal components)
+        double devb[6], scb = 0.0, pb = dsb[0] + dsb[1] + dsb[2];
+        for (int i = 0; i < 6; ++i) {
+            sbo[i] -= dsb[i];
+            devb[i] = r.sc * dsb[i];
+            scb += dsb[i] * r.dev[i];
+        }
+        // sc = radius / sqrt(j2), radius = k0 (+ alpha / sqrt 2)
+        if (vm) abo += scb / sqrt(r.j2) / sqrt(2.0);
+        const double j2b = -0.5 * scb * r.sc / r.j2;
+        // j2(st): d/dst_i = dev_i (normal), 2 st_i (shear); dev = st - p, p = tr st / 3
+        double stb[6];
+        for (int i = 0; i < 3; ++i) {
+            stb[i] = j2b * r.dev[i] + devb[i];
+            pb -= devb[i];
+        }
+        for (int i = 3; i < 6; ++i) stb[i] = j2b * 2.0 * r.st[i] + devb[i];
+        for (int i = 0; i < 3; ++i) stb[i] += pb / 3.0;
+        // st = s + del
+        for (int i = 0; i < 6; ++i) {
+            sbo[i] += stb[i];
+            delb[i] = stb[i];
+        }
+    }
+    // del = C(E) de
+    double t[6];
+    hooke(E, nu, delb, t);
+    for (int i = 0; i < 6; ++i) {
+        eb[i] += t[i];
+        gE += delb[i] * r.del[i] / E;
+    }
+}
+
+// NeoHookean total stress (…NeoHookean.cpp:4-22) of the updated F; the
+// increment P - s added to s gives P (rounded)
+__device__ __host__ __forceinline__ void nh_stress(double E, double nu, const double F[9], double P[9])
+{
+    const double mu = 0.5 * E / (1.0 + nu);
+    const double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double J = det3(F);
+    const double a = lam * log(J) / J;
+    const double b = mu / J;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            const double fft = F[r] * F[c] + F[3 + r] * F[3 + c] + F[6 + r] * F[6 + c];
+            const double id = (c == r) ? 1.0 : 0.0;
+            P[3 * c + r] = a * id + b * (fft - id);
+        }
+}
+// cofactor matrix, cof[3c + r] = d det / d F[3c + r] (column-major F)
+__device__ __host__ __forceinline__ void cofactor(const double F[9], double cof[9])
+{
+    cof[0] = F[4] * F[8] - F[5] * F[7];
+    cof[1] = F[5] * F[6] - F[3] * F[8];
+    cof[2] = F[3] * F[7] - F[4] * F[6];
+    cof[3] = F[2] * F[7] - F[1] * F[8];
+    cof[4] = F[0] * F[8] - F[2] * F[6];
+    cof[5] = F[1] * F[6] - F[0] * F[7];
+    cof[6] = F[1] * F[5] - F[2] * F[4];
+    cof[7] = F[2] * F[3] - F[0] * F[5];
+    cof[8] = F[0] * F[4] - F[1] * F[3];
+}
+// Adjoint of s' = P(F) (Voigt xx yy zz xy yz zx of P): Fb += dP/dF^T sb,
+// gE = d/dE (P is linear in E).  sigma = lam ln J / J I + mu / J (F F^T - I).
+__device__ __host__ __forceinline__ void nh_adjoint(double E, double nu, const double F[9], const double sb[6], double Fb[9],
+                                           double &gE)
+{
+    const double mu = 0.5 * E / (1.0 + nu);
+    const double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double J = det3(F), lJ = log(J);
+    const double b = mu / J;
+    // symmetric adjoint of the matrix entries: the shear value stands for both
+    const double S[9] = {sb[0], 0.5 * sb[3], 0.5 * sb[5], 0.5 * sb[3], sb[1], 0.5 * sb[4], 0.5 * sb[5], 0.5 * sb[4], sb[2]};
+    double P[9];
+    nh_stress(E, nu, F, P);
+    gE = (sb[0] * P[0] + sb[1] * P[4] + sb[2] * P[8] + sb[3] * P[1] + sb[4] * P[5] + sb[5] * P[6]) / E;
+    double bb = 0.0;   // d/db = S : (F F^T - I)
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            const double fft = F[r] * F[c] + F[3 + r] * F[3 + c] + F[6 + r] * F[6 + c];
+            bb += S[3 * c + r] * (fft - ((c == r) ? 1.0 : 0.0));
+        }
+    const double ab = S[0] + S[4] + S[8];   // d/da = tr S
+    const double Jb = ab * lam * (1.0 - lJ) / (J * J) - bb * mu / (J * J);
+    double cof[9];
+    cofactor(F, cof);
+    // F F^T: d/dF = 2 S F (S symmetric), times b
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            Fb[3 * c + r] += 2.0 * b * (S[r] * F[3 * c] + S[3 + r] * F[3 * c + 1] + S[6 + r] * F[3 * c + 2]) + Jb * cof[3 * c + r];
+}
+
 // f contribution of stress s (Voigt) and gradient G: -V (sigma . G)
 __device__ __forceinline__ void sigma_dot(const double s[6], const double G[3], double o[3])
 {
@@ -457,21 +662,41 @@ __global__ __launch_bounds__(256) void kd_g2p(DArgs a)
             for (int c = 0; c < 3; ++c) L[3 * c + q] += vt * G[c];
         }
     }
-    double F[9], T[9], s[6], de[6], ds[6];
+    double F[9], T[9], Fn[9], s[6], de[6], ds[6];
     for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
     matmul(L, F, T);
-    for (int q = 0; q < 9; ++q) a.F[q * n + p] = F[q] + T[q] * a.dt;
+    for (int q = 0; q < 9; ++q) {
+        Fn[q] = F[q] + T[q] * a.dt;
+        a.F[q * n + p] = Fn[q];
+    }
     for (int q = 0; q < 3; ++q) {
         a.x[q * n + p] = x[q] + dx[q] * a.dt;
         a.u[q * n + p] += dx[q] * a.dt;
         a.v[q * n + p] += dv[q] * a.dt;
     }
     eps_of(L, a.dt, de);
-    hooke(a.E[p], a.nu[p], de, ds);
-    for (int q = 0; q < 6; ++q) {
-        s[q] = a.s[q * n + p];
-        a.s[q * n + p] = s[q] + ds[q];
+    for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
+    // constitutive update (…_MP.cpp:322-353)
+    const int law = a.law[p];
+    if (law == LAW_ELASTIC) {
+        hooke(a.E[p], a.nu[p], de, ds);
+    } else if (law == LAW_NEOHOOKEAN) {
+        double P[9];
+        nh_stress(a.E[p], a.nu[p], Fn, P);
+        ds[0] = P[0] - s[0];
+        ds[1] = P[4] - s[1];
+        ds[2] = P[8] - s[2];
+        ds[3] = P[1] - s[3];
+        ds[4] = P[5] - s[4];
+        ds[5] = P[6] - s[5];
+    } else {
+        RR r;
+        const double al = a.al[p];
+        rr_eval(a.E[p], a.nu[p], a.sy[p], a.C0[p], a.C1[p], law == LAW_VONMISES, s, al, de, r);
+        for (int q = 0; q < 6; ++q) ds[q] = r.ds[q];
+        if (law == LAW_VONMISES && r.plastic) a.al[p] = al + r.da;
     }
+    for (int q = 0; q < 6; ++q) a.s[q * n + p] = s[q] + ds[q];
 }
 
 // platen motion at the end of the step (…_MP.cpp:360-368)
@@ -514,15 +739,39 @@ __global__ __launch_bounds__(256) void kd_a_g2p(DArgs a)
     for (int q = 0; q < 9; ++q) { F[q] = a.F[q * n + p]; Fb[q] = a.Fb[q * n + p]; }
     for (int q = 0; q < 6; ++q) sbv[q] = a.sb[q * n + p];
     for (int q = 0; q < 3; ++q) { xb[q] = a.xb[q * n + p]; vb[q] = a.vb[q * n + p]; }
-    // sigma' = sigma + C(E) eps(L dt): E-bar, eps-bar = C^T sigma-bar'
+    // the constitutive update: E-bar, eps-bar, and sigma-bar, alpha-bar before
+    // the update (elastic: sigma-bar is unchanged)
     const double E = a.E[p], nu = a.nu[p];
-    double de[6], ds[6], eb[6];
+    double de[6], eb[6], gE = 0.0;
     eps_of(L, dt, de);
-    hooke(E, nu, de, ds);
-    double gE = 0.0;
-    for (int q = 0; q < 6; ++q) gE += sbv[q] * ds[q] / E;
+    const int law = a.law[p];
+    if (law == LAW_ELASTIC) {
+        // sigma' = sigma + C(E) eps(L dt): eps-bar = C^T sigma-bar'
+        double ds[6];
+        hooke(E, nu, de, ds);
+        for (int q = 0; q < 6; ++q) gE += sbv[q] * ds[q] / E;
+        hooke(E, nu, sbv, eb);   // the normal block is symmetric; shear: C (1/2 - nu)
+    } else if (law == LAW_NEOHOOKEAN) {
+        // sigma' = P(F'), F' = F + dt L F: F'-bar gains dP/dF'^T sigma-bar';
+        // sigma' does not depend on sigma or on eps
+        double T[9], Fp[9];
+        matmul(L, F, T);
+        for (int q = 0; q < 9; ++q) Fp[q] = F[q] + T[q] * dt;
+        nh_adjoint(E, nu, Fp, sbv, Fb, gE);
+        for (int q = 0; q < 6; ++q) {
+            eb[q] = 0.0;
+            a.sb[q * n + p] = 0.0;
+        }
+    } else {
+        double s[6], sbo[6], abo;
+        for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
+        RR r;
+        rr_eval(E, nu, a.sy[p], a.C0[p], a.C1[p], law == LAW_VONMISES, s, a.al[p], de, r);
+        rr_adjoint(E, nu, a.C0[p], law == LAW_VONMISES, de, r, sbv, a.alb[p], sbo, abo, eb, gE);
+        for (int q = 0; q < 6; ++q) a.sb[q * n + p] = sbo[q];
+        a.alb[p] = abo;
+    }
     a.Eb[p] += gE;
-    hooke(E, nu, sbv, eb);   // the normal block is symmetric; shear: C (1/2 - nu)
     // L-bar from eps (engineering shear) and from F' = F + dt L F
     double Lb[9];
     for (int c = 0; c < 3; ++c)
@@ -789,8 +1038,10 @@ struct mpm_diff {
     std::vector<double> tl_t, tl_v, times;   // times[k]: start time of step k
     std::vector<int> def_ids, dc_ids;
     // device; the state arrays x, u, v, F, s, xdc, vdc are slices of st
-    double *st = nullptr, *x = nullptr, *u = nullptr, *v = nullptr, *F = nullptr, *s = nullptr;
+    double *st = nullptr, *x = nullptr, *u = nullptr, *v = nullptr, *F = nullptr, *s = nullptr, *al = nullptr;
     double *m = nullptr, *V0 = nullptr, *E = nullptr, *nu = nullptr, *fext = nullptr;
+    uint8_t *law = nullptr;
+    double *sy = nullptr, *C0 = nullptr, *C1 = nullptr, *al0 = nullptr, *alb = nullptr;
     uint8_t *fixed = nullptr;
     double *xdc = nullptr, *vdc = nullptr;
     double *x0dc = nullptr, *v0dc = nullptr;   // platen state at step 0
@@ -812,8 +1063,8 @@ struct mpm_diff {
     double *seg = nullptr;     // one segment: ckpt x state
     bool have_forward = false;
     // doubles of one saved state: x, u, v (3 n each), F (9 n), sigma (6 n),
-    // platen x and v (3 n_dc each)
-    size_t state() const { return 24 * (size_t)n + 6 * (size_t)n_dc; }
+    // alpha (n), platen x and v (3 n_dc each)
+    size_t state() const { return 25 * (size_t)n + 6 * (size_t)n_dc; }
 };
 
 namespace {
@@ -846,8 +1097,9 @@ DArgs args_of(mpm_diff *d, int step, int slot)
     a.dt = d->dt;
     a.damping = d->damping;
     a.fext = d->fext;
-    a.x = d->x; a.u = d->u; a.v = d->v; a.F = d->F; a.s = d->s;
+    a.x = d->x; a.u = d->u; a.v = d->v; a.F = d->F; a.s = d->s; a.al = d->al;
     a.m = d->m; a.V0 = d->V0; a.E = d->E; a.nu = d->nu;
+    a.law = d->law; a.sy = d->sy; a.C0 = d->C0; a.C1 = d->C1;
     a.fixed = d->fixed;
     a.xdc = d->xdc; a.vdc = d->vdc;
     // TimeLine::getVelocity at the step's start time (TimeLine.h:33-46)
@@ -867,7 +1119,7 @@ DArgs args_of(mpm_diff *d, int step, int slot)
     a.cnt = (unsigned *)(nd + 13 * nn);
     a.nflag = (uint8_t *)(a.cnt + nn);
     a.nb = d->nb; a.nsc = d->nsc; a.err = d->errd;
-    a.xb = d->xb; a.vb = d->vb; a.Fb = d->Fb; a.sb = d->sb; a.Eb = d->Eb; a.scratch = d->scratch;
+    a.xb = d->xb; a.vb = d->vb; a.Fb = d->Fb; a.sb = d->sb; a.Eb = d->Eb; a.alb = d->alb; a.scratch = d->scratch;
     a.aacc = d->aacc; a.nmb = d->nmb; a.nPb = d->nPb; a.nfb = d->nfb;
     return a;
 }
@@ -901,7 +1153,7 @@ int forward_step(mpm_diff *d, int step, int slot, bool advance)
     return MPM_OK;
 }
 
-// copy the state (x, u, v, F, sigma, platen x and v: one contiguous block)
+// copy the state (x, u, v, F, sigma, alpha, platen x and v: one contiguous block)
 // to buf (save) or from it (load), stream-ordered
 int move_state(mpm_diff *d, double *buf, bool save)
 {
@@ -929,7 +1181,7 @@ void free_all(mpm_diff *d)
 {
     void *ptrs[] = {d->st, d->m, d->V0, d->E, d->nu, d->fext, d->fixed, d->x0dc, d->v0dc, d->v0, d->F0, d->marks, d->nb, d->nsc, d->acc, d->aacc, d->nodebuf, d->items,
                     d->nmb, d->nPb, d->nfb, d->errd, d->xb, d->vb, d->Fb, d->sb, d->Eb,
-                    d->scratch, d->ck, d->seg};
+                    d->scratch, d->ck, d->seg, d->law, d->sy, d->C0, d->C1, d->al0, d->alb};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -987,11 +1239,22 @@ int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_step
         d->err = "differentiable run: no deformable points";
         return fail(MPM_E_ARG);
     }
-    for (int i : d->def_ids)
-        if (pts->type && pts->type[i] != MPM_ELASTIC) {
-            d->err = "differentiable run: elastic points only";
+    // laws after the reference's dispatch (…_MP.cpp:332-339): NeoHookean only
+    // when cfg->neohookean, ViscoElastic and IdealGas fall to PerfectlyPlastic
+    std::vector<uint8_t> hlaw(d->n);
+    for (int k = 0; k < d->n; ++k) {
+        const int t = pts->type ? pts->type[d->def_ids[k]] : MPM_ELASTIC;
+        if (t < MPM_ELASTIC || t > MPM_GASS) {
+            d->err = "differentiable run: unknown material type";
             return fail(MPM_E_ARG);
         }
+        hlaw[k] = (t == MPM_ELASTIC) ? LAW_ELASTIC : (t == MPM_NEOHOOKEAN && cfg->neohookean) ? LAW_NEOHOOKEAN
+                  : (t == MPM_VONMISESHARDENING) ? LAW_VONMISES : LAW_PLASTIC;
+        if ((hlaw[k] == LAW_PLASTIC && !pts->sy) || (hlaw[k] == LAW_VONMISES && (!pts->sy || !pts->C0 || !pts->C1))) {
+            d->err = "differentiable run: plastic points need sy (and C0, C1 for hardening)";
+            return fail(MPM_E_ARG);
+        }
+    }
     // device order: by brick of the starting cell, then Morton order of the
     // cell inside it (ties: input order); scatter work items of at most 256
     // points of one brick
@@ -1030,7 +1293,9 @@ int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_step
         (rc = dalloc_(d, &d->items, items.size())) || (rc = dalloc_(d, &d->nmb, nn)) || (rc = dalloc_(d, &d->nPb, 3 * nn)) || (rc = dalloc_(d, &d->nfb, 3 * nn)) ||
         (rc = dalloc_(d, &d->errd, 1)) || (rc = dalloc_(d, &d->xb, 3 * n)) ||
         (rc = dalloc_(d, &d->vb, 3 * n)) || (rc = dalloc_(d, &d->Fb, 9 * n)) || (rc = dalloc_(d, &d->sb, 6 * n)) ||
-        (rc = dalloc_(d, &d->Eb, n)) || (rc = dalloc_(d, &d->scratch, 12 * n)) ||
+        (rc = dalloc_(d, &d->Eb, n)) || (rc = dalloc_(d, &d->scratch, 12 * n)) || (rc = dalloc_(d, &d->law, n)) ||
+        (rc = dalloc_(d, &d->sy, n)) || (rc = dalloc_(d, &d->C0, n)) || (rc = dalloc_(d, &d->C1, n)) ||
+        (rc = dalloc_(d, &d->al0, n)) || (rc = dalloc_(d, &d->alb, n)) ||
         (rc = dalloc_(d, &d->ck, (size_t)d->n_ckpt * state)) || (rc = dalloc_(d, &d->seg, (size_t)d->ckpt * state)))
         return fail(rc);
     d->x = d->st;
@@ -1038,7 +1303,8 @@ int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_step
     d->v = d->u + 3 * n;
     d->F = d->v + 3 * n;
     d->s = d->F + 9 * n;
-    d->xdc = d->s + 6 * n;
+    d->al = d->s + 6 * n;
+    d->xdc = d->al + n;
     d->vdc = d->xdc + 3 * nd;
     // static point data, component-major (blocking copies from pageable memory)
     std::vector<double> tmp(9 * n);
@@ -1049,6 +1315,13 @@ int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_step
     if (up(d->V0, n) != hipSuccess) return fail(MPM_E_HIP);
     for (size_t k = 0; k < n; ++k) tmp[k] = pts->nu[d->def_ids[k]];
     if (up(d->nu, n) != hipSuccess) return fail(MPM_E_HIP);
+    const double *pp[4] = {pts->sy, pts->C0, pts->C1, pts->alpha};
+    double *dp[4] = {d->sy, d->C0, d->C1, d->al0};
+    for (int f = 0; f < 4; ++f) {
+        for (size_t k = 0; k < n; ++k) tmp[k] = pp[f] ? pp[f][d->def_ids[k]] : 0.0;
+        if (up(dp[f], n) != hipSuccess) return fail(MPM_E_HIP);
+    }
+    if (hipMemcpy(d->law, hlaw.data(), n, hipMemcpyHostToDevice) != hipSuccess) return fail(MPM_E_HIP);
     for (size_t k = 0; k < n; ++k)
         for (int a = 0; a < 3; ++a) tmp[a * n + k] = pts->v ? pts->v[3 * d->def_ids[k] + a] : 0.0;
     if (up(d->v0, 3 * n) != hipSuccess) return fail(MPM_E_HIP);
@@ -1096,6 +1369,7 @@ int mpm_diff_forward(mpm_diff *d, const double *E, const double *x0, double *u_f
     DCHK(hipMemcpyAsync(d->v, d->v0, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, d->stream));
     DCHK(hipMemcpyAsync(d->F, d->F0, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, d->stream));
     DCHK(hipMemsetAsync(d->s, 0, sizeof(double) * 6 * n, d->stream));
+    DCHK(hipMemcpyAsync(d->al, d->al0, sizeof(double) * n, hipMemcpyDeviceToDevice, d->stream));
     if (nd) {
         DCHK(hipMemcpyAsync(d->xdc, d->x0dc, sizeof(double) * 3 * nd, hipMemcpyDeviceToDevice, d->stream));
         DCHK(hipMemcpyAsync(d->vdc, d->v0dc, sizeof(double) * 3 * nd, hipMemcpyDeviceToDevice, d->stream));
@@ -1142,6 +1416,7 @@ int mpm_diff_backward(mpm_diff *d, const double *g_ufinal, double *g_E, double *
     DCHK(hipMemsetAsync(d->Fb, 0, sizeof(double) * 9 * n, d->stream));
     DCHK(hipMemsetAsync(d->sb, 0, sizeof(double) * 6 * n, d->stream));
     DCHK(hipMemsetAsync(d->Eb, 0, sizeof(double) * n, d->stream));
+    DCHK(hipMemsetAsync(d->alb, 0, sizeof(double) * n, d->stream));
     for (int c = d->n_ckpt - 1; c >= 0; --c) {
         const int k0 = c * d->ckpt, k1 = std::min(d->steps, k0 + d->ckpt);
         // replay the segment from its checkpoint, keeping every step's state

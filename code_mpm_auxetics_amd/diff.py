@@ -3,10 +3,13 @@ displacements with respect to the per-point Young's modulus and the initial
 positions.
 
 The forward and the adjoint run in the HIP library (``mpm_diff_*`` in
-``include/mpm_auxetics.h``): FP64 elastic steps of
+``include/mpm_auxetics.h``): FP64 steps of
 PhysicsEngine_runSimulation_Classic_SinglePass_MP.cpp:41-368 (strain increment
-L dt, damping, TimeLine platen) with checkpoints every ``checkpoint`` steps,
-and the hand-written adjoint of P2G, grid update and G2P replayed from them.
+L dt, damping, TimeLine platen) for elastic, NeoHookean (when the scene
+dispatches it), perfectly plastic and hardening J2 points, with checkpoints
+every ``checkpoint`` steps, and the hand-written adjoint of P2G, grid update,
+G2P and the constitutive updates replayed from them.  The yield decision of
+each point and step is held fixed, as the other discrete choices are.
 The run carries the displacement u = x - x0 as state and returns it, so that
 displacements many orders below the positions keep full FP64 precision.
 ``rollout`` exposes the pair as a ``torch.autograd.Function``; the effective
@@ -56,6 +59,7 @@ class DiffRun:
         cfg.mass_min = scene.mass_min
         cfg.precision = _abi.MPM_FP64
         cfg.strain_form = 1
+        cfg.neohookean = int(getattr(scene, "neohookean", 0))
         cfg.device = int(device)
         self._keep = {}
         tl_t = np.ascontiguousarray(scene.timeline_t, np.float64)
@@ -77,6 +81,9 @@ class DiffRun:
                                         ("vol", scene.vol0, C.c_double, np.float64),
                                         ("E", scene.E, C.c_double, np.float64),
                                         ("nu", scene.nu, C.c_double, np.float64),
+                                        ("sy", scene.sy, C.c_double, np.float64),
+                                        ("C0", scene.C0, C.c_double, np.float64),
+                                        ("C1", scene.C1, C.c_double, np.float64),
                                         ("type", scene.type, C.c_int32, np.int32),
                                         ("dc", scene.dc, C.c_uint8, np.uint8)):
             a = np.ascontiguousarray(arr, dtype)
@@ -85,7 +92,7 @@ class DiffRun:
         h = C.c_void_p()
         code = lib().mpm_diff_create(C.byref(cfg), C.byref(pts), self.n_steps, self.checkpoint, C.byref(h))
         if code != _abi.MPM_OK:
-            raise _abi.MpmError(code, "mpm_diff_create failed (elastic points only; device present?)")
+            raise _abi.MpmError(code, "mpm_diff_create failed (invalid scene or no device)")
         self._h = h
         self.n = scene.n
 

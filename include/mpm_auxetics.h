@@ -278,9 +278,11 @@ typedef struct mpm_cr_batch {
 int mpm_constitutive(int32_t law, const mpm_cr_batch *b, int32_t device);
 
 /* ---- differentiable run (no reference counterpart: the reference has no
- *      gradients).  n_steps plain steps of cfg->dt_max in FP64 for elastic
- *      points (…_MP.cpp:106-319 with …Elastic.cpp:4-15, strain increment
- *      L dt, damping and TimeLine platen as the reference), on a dense node
+ *      gradients).  n_steps plain steps of cfg->dt_max in FP64
+ *      (…_MP.cpp:106-353, strain increment L dt, damping and TimeLine
+ *      platen as the reference; elastic, NeoHookean when cfg->neohookean,
+ *      perfectly plastic and J2-hardening points, each point's yield
+ *      decision held fixed in the adjoint), on a dense node
  *      grid with deterministic fixed-point node sums; the backward pass is the
  *      adjoint of every phase, replayed from checkpoints every `checkpoint`
  *      steps.  Parameters are the per-point Young's modulus and the initial
@@ -290,9 +292,11 @@ int mpm_constitutive(int32_t law, const mpm_cr_batch *b, int32_t device);
  *      as in the reference and get zero gradients). ------------------------ */
 typedef struct mpm_diff mpm_diff;
 
-/* Copies the scene: masses, vol0 (or vol), nu, initial v, fext, type (must be
- * MPM_ELASTIC for non-platen points), dc flags, platen positions; node
- * fixity, damping and TimeLine from cfg.  MPM_E_ARG for other materials. */
+/* Copies the scene: masses, vol0 (or vol), nu, sy, C0, C1, initial alpha
+ * (0 if null), initial v, fext, type, dc flags, platen positions; node
+ * fixity, damping, TimeLine and the NeoHookean dispatch from cfg.  Types map
+ * as the reference's dispatch does (…_MP.cpp:332-339).  MPM_E_ARG for an
+ * unknown type, or for plastic points without sy (or C0, C1 for hardening). */
 int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_steps,
                     int32_t checkpoint, mpm_diff **out);
 

@@ -2,7 +2,8 @@
 torch.autograd.Function): forward parity with the FP64 oracle, adjoint
 gradients against central finite differences of the GPU forward and of the
 FP64 CPU oracle, bit-identical gradients for every checkpoint interval, and
-the error paths.
+the error paths; for elastic, J2-hardening, perfectly plastic and NeoHookean
+points.
 
 Finite differences use eps = 1e-4 relative: E(1 +/- eps r) for the moduli and
 x0 +/- eps h r for the positions (h the cell size), r standard normal.  The
@@ -20,6 +21,9 @@ from oracle.oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 EPS = 1.0e-4
+# plastic worlds: the adjoint holds each point's yield decision; a smaller
+# step moves fewer points across the yield surface inside the difference
+EPS_PL = 1.0e-6
 
 
 def small_scene(n_steps=30, seed=7):
@@ -30,8 +34,8 @@ def small_scene(n_steps=30, seed=7):
     return sc
 
 
-def oracle_final_u(sc, E, x0, n_steps):
-    """Final displacements of the FP64 oracle (strain increment L dt)."""
+def oracle_points(sc, E, x0, n_steps):
+    """Final point state of the FP64 oracle (strain increment L dt)."""
     s = sc.copy()
     s.E = np.asarray(E, float).copy()
     s.x = np.asarray(x0, float).copy()
@@ -39,7 +43,12 @@ def oracle_final_u(sc, E, x0, n_steps):
     for _ in range(n_steps):
         o.step(s.dt_max)
         o.w.time += s.dt_max
-    return o.points()["x"] - s.x
+    return o.points()
+
+
+def oracle_final_u(sc, E, x0, n_steps):
+    """Final displacements of the FP64 oracle (strain increment L dt)."""
+    return oracle_points(sc, E, x0, n_steps)["x"] - np.asarray(x0, float)
 
 
 def adjoint_grads(run, sc, sets):
@@ -149,9 +158,13 @@ def test_zero_loss_gradient_and_error_paths():
         g[sc.dc == 1] = 1.0
         gE, gx = run.backward(g)
         assert not gE.any() and not gx.any()
-    j2 = scene.small_world(material="j2", steps=5)
+    bad = small_scene(n_steps=5)
+    bad.type = bad.type.copy()
+    bad.type[np.flatnonzero(bad.dc == 0)[0]] = 9   # no such material type
     with pytest.raises(_abi.MpmError):
-        diff.DiffRun(j2, 5)
+        diff.DiffRun(bad, 5)
+    with pytest.raises(_abi.MpmError):
+        diff.DiffRun(sc, 5, checkpoint=0)
 
 
 def test_config5_gradient_on_64_directions():
@@ -175,3 +188,99 @@ def test_config5_gradient_on_64_directions():
             worst = max(worst, abs(fd - ad) / abs(ad))
         print(f"config5: worst relative |fd - adjoint| over 64 directions {worst:.3e}", flush=True)
         assert worst <= 1e-4
+
+
+LAWS = ["j2", "plastic", "neohookean"]
+
+
+def law_scene(material, n_steps=30):
+    sc = scene.small_world(material=material, steps=n_steps, speed=2.0)
+    if material == "plastic":   # E = 1 GPa: a yield stress of 1 MPa is reached within the run
+        sc.sy = np.where(sc.dc == 0, 1.0e6, sc.sy)
+    return sc
+
+
+@pytest.mark.parametrize("material", LAWS)
+def test_forward_matches_oracle_other_laws(material):
+    n_steps = 30
+    sc = law_scene(material, n_steps)
+    with diff.DiffRun(sc, n_steps, checkpoint=7) as run:
+        u_gpu = run.forward(sc.E, sc.x)
+    o = oracle_points(sc, sc.E, sc.x, n_steps)
+    u_or = o["x"] - sc.x
+    disp = np.abs(u_or).max()
+    assert disp > 1e-7
+    np.testing.assert_allclose(u_gpu, u_or, rtol=0, atol=1e-9 * disp)
+    if material != "neohookean":
+        yielded = int((np.abs(o["eps_p"][sc.dc == 0]).max(axis=1) > 0).sum())
+        print(f"{material}: {yielded} points yielded", flush=True)
+        assert yielded >= 20
+
+
+@pytest.mark.parametrize("material", LAWS)
+def test_gradient_matches_central_fd_other_laws(material):
+    n_steps = 30
+    sc = law_scene(material, n_steps)
+    sets = diff.boundary_sets(sc)
+    eps = EPS if material == "neohookean" else EPS_PL
+    with diff.DiffRun(sc, n_steps, checkpoint=7) as run:
+        loss0, gE, gx = adjoint_grads(run, sc, sets)
+
+        def loss(E, x0):
+            return float(diff.poisson_loss(x0, run.forward(E, x0), sets))
+
+        assert abs(loss(sc.E, sc.x) - loss0) <= 1e-14 * abs(loss0)
+        for dE, dx in directions(sc, 6, 13):
+            fd_E = (loss(sc.E + eps * dE, sc.x) - loss(sc.E - eps * dE, sc.x)) / (2 * eps)
+            fd_x = (loss(sc.E, sc.x + eps * dx) - loss(sc.E, sc.x - eps * dx)) / (2 * eps)
+            ad_E, ad_x = float(gE @ dE), float((gx * dx).sum())
+            print(f"{material} E: fd {fd_E:.9e} adjoint {ad_E:.9e}   x0: fd {fd_x:.9e} adjoint {ad_x:.9e}", flush=True)
+            assert abs(fd_E - ad_E) <= 1e-4 * abs(ad_E) + 1e-12
+            assert abs(fd_x - ad_x) <= 1e-4 * abs(ad_x) + 1e-12
+
+
+def test_j2_gradient_matches_fp64_cpu_reference():
+    """J2 hardening: the modulus gradient of 8 points that yield and 6
+    position components against central differences of the FP64 oracle.  The
+    oracle's displacements are x - x0, rounded at the positions' ulp (about
+    1e-12 of u here), so its differences take the 1e-4 step, as for elastic
+    points; one point's perturbation moves few yield decisions."""
+    n_steps = 20
+    sc = law_scene("j2", n_steps)
+    sets = diff.boundary_sets(sc)
+    with diff.DiffRun(sc, n_steps, checkpoint=6) as run:
+        _, gE, gx = adjoint_grads(run, sc, sets)
+
+    def loss(E, x0):
+        return float(diff.poisson_loss(x0, oracle_final_u(sc, E, x0, n_steps), sets))
+
+    o = oracle_points(sc, sc.E, sc.x, n_steps)
+    plastic = np.flatnonzero((sc.dc == 0) & (np.abs(o["eps_p"]).max(axis=1) > 0))
+    assert len(plastic) >= 8
+    rng = np.random.default_rng(9)
+    scale = np.abs(gE).max()
+    for i in rng.choice(plastic, 8, replace=False):
+        d = np.zeros(sc.n)
+        d[i] = EPS * sc.E[i]
+        ref = (loss(sc.E + d, sc.x) - loss(sc.E - d, sc.x)) / (2 * d[i])
+        print(f"point {i}: dL/dE adjoint {gE[i]:.9e} oracle FD {ref:.9e}", flush=True)
+        assert abs(gE[i] - ref) <= 1e-4 * scale, (i, gE[i], ref)
+    for i in rng.choice(plastic, 2, replace=False):
+        for a in range(3):
+            d = np.zeros((sc.n, 3))
+            d[i, a] = EPS * sc.h[0]
+            ref = (loss(sc.E, sc.x + d) - loss(sc.E, sc.x - d)) / (2 * d[i, a])
+            assert abs(gx[i, a] - ref) <= 1e-4 * np.abs(gx).max(), (i, a, gx[i, a], ref)
+
+
+def test_j2_gradient_is_independent_of_the_checkpoint_interval():
+    sc = law_scene("j2", 24)
+    sets = diff.boundary_sets(sc)
+    out = []
+    for ck in (1, 5, 24):
+        with diff.DiffRun(sc, 24, checkpoint=ck) as run:
+            out.append(adjoint_grads(run, sc, sets))
+    for o in out[1:]:
+        assert o[0] == out[0][0]
+        assert np.array_equal(o[1], out[0][1])
+        assert np.array_equal(o[2], out[0][2])
