@@ -637,7 +637,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                                              // base, window low corner in this brick's node
                                              // coordinates}, {window extent, ext x * ext y}
     __shared__ int s_box[kPBlock / 64][6];
-    __shared__ double s_red[kPBlock / 64][5];
+    __shared__ double s_red[kPBlock / 64][3];
 
     constexpr bool FP32 = sizeof(real) == 4;
     const GridParams &g = a.g;
@@ -689,6 +689,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     // touched box of the last P2G (…_MP.cpp:209-265)
     real *s_node = reinterpret_cast<real *>(s_arena + kQ * AN);
     int wlo[3] = {0, 0, 0}, wext[3] = {0, 0, 0};
+    int stash = 0;   // reals of the arena the grid update uses as its stash
     if (G2P) {
         if (tid < 27) {
             const int d[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
@@ -725,6 +726,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         // flight at once.  Each thread later reads back only the slots it
         // wrote: no barrier.
         real *s_sum = reinterpret_cast<real *>(s_arena);
+        stash = kQ * nwin;
         for (int r = tid; r < nwin; r += kPBlock) {
             const int lz = (int)(((float)r + 0.5f) * rxy), rem = r - lz * own1.w;
             const int ly = (int)(((float)rem + 0.5f) * rx), lx = rem - ly * wext[0];
@@ -775,9 +777,16 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             const unsigned long long *wt = reinterpret_cast<const unsigned long long *>(s_node + 6 * r);
             const unsigned long long mk = wt[0];
             const uint32_t fw = (uint32_t)wt[1], fx = fw & 0xFFu;
+            // each stash slot is read, then zeroed for the P2G arena, by the
+            // thread that wrote it
             real sum[kQ];
 #pragma unroll
-            for (int q = 0; q < kQ; ++q) sum[q] = s_sum[kQ * r + q];
+            for (int q = 0; q < kQ; ++q) {
+                sum[q] = s_sum[kQ * r + q];
+                s_sum[kQ * r + q] = real(0);
+            }
+            // the rest of the last 8-byte arena word that the stash ends in
+            if (FP32 && r == nwin - 1 && (stash & 1)) s_sum[stash] = real(0);
             const real mn = sum[0];
             real vn[3], fn[3] = {sum[4], sum[5], sum[6]};
 #pragma unroll
@@ -826,20 +835,19 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                 }
             }
         }
+        // wave sums to LDS; the block's sum is taken after the barrier that
+        // precedes the point chunks (no barrier of its own)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const double w = wave_sum_dpp(gs[k]);
             if (lane == 0) s_red[wv][k] = w;
         }
-        __syncthreads();
-        if (tid < 3) {
-            double t = 0.0;
-            for (int w = 0; w < kPBlock / 64; ++w) t += s_red[w][tid];
-            a.gpart[3 * (size_t)slot + tid] = t;
-        }
     }
 
-    for (int i = tid; i < kQ * AN; i += kPBlock) s_arena[i] = 0;
+    // the arena past the stash (the stash zeroes itself): no barrier between
+    // the grid update and this clear
+    const int clear0 = (stash * (int)sizeof(real) + 7) / 8;
+    for (int i = clear0 + tid; i < kQ * AN; i += kPBlock) s_arena[i] = 0;
     if (tid < 8) s_max[tid] = 0u;
     int H = 6;   // a node takes at most one contribution per point: count < 2^(H-1)
     while ((1u << (H - 1)) <= count) ++H;
@@ -855,6 +863,11 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     const float qscale[3] = {(float)(dtd * (double)g.ihf[0] * kQOne), (float)(dtd * (double)g.ihf[1] * kQOne),
                              (float)(dtd * (double)g.ihf[2] * kQOne)};
     __syncthreads();
+    if (G2P && tid < 3) {
+        double t = 0.0;
+        for (int w = 0; w < kPBlock / 64; ++w) t += s_red[w][tid];
+        a.gpart[3 * (size_t)slot + tid] = t;
+    }
 
     for (uint32_t c0 = 0; c0 < count; c0 += kPBlock) {
         const bool active = (c0 + tid) < count;
