@@ -420,16 +420,30 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     HIPCHK(h, hipGetLastError());
     k_occ_count<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->occ_start, n_occ, n, h->occ_count);
     HIPCHK(h, hipGetLastError());
-    // launch order of the brick blocks: most point chunks first, ties in slot
-    // order (a longest-first schedule; every slot still sums in slot order)
+    // launch order of the brick blocks.  Workgroups go to the 8 XCDs round
+    // robin by block index, so block 8k + x takes the k-th slot of group x,
+    // the x-th run of consecutive slots (bricks in space order): a brick and
+    // most of its neighbours then run on one XCD every step, and the arena
+    // records a block reads were written through that XCD's L2.  Inside a
+    // group: most point chunks first, ties in slot order (a longest-first
+    // schedule).  Every slot still sums in slot order.
+    constexpr uint32_t kXcd = 8;
     h->occ_count_h.resize(n_occ);
-    h->slot_order_h.resize(n_occ);
+    h->slot_order_h.clear();
     HIPCHK(h, hipMemcpyAsync(h->occ_count_h.data(), h->occ_count, (size_t)n_occ * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    for (uint32_t k = 0; k < n_occ; ++k) h->slot_order_h[k] = k;
-    std::stable_sort(h->slot_order_h.begin(), h->slot_order_h.end(), [h](uint32_t x, uint32_t y) {
-        return cdiv(h->occ_count_h[x], kPBlock) > cdiv(h->occ_count_h[y], kPBlock);
-    });
+    {
+        const uint32_t per = cdiv(n_occ, kXcd);
+        std::vector<std::vector<uint32_t>> grp(kXcd);
+        for (uint32_t k = 0; k < n_occ; ++k) grp[k / per].push_back(k);
+        for (auto &gl : grp)
+            std::stable_sort(gl.begin(), gl.end(), [h](uint32_t x, uint32_t y) {
+                return cdiv(h->occ_count_h[x], kPBlock) > cdiv(h->occ_count_h[y], kPBlock);
+            });
+        for (uint32_t k = 0; k < per; ++k)
+            for (uint32_t x = 0; x < kXcd; ++x)
+                if (k < grp[x].size()) h->slot_order_h.push_back(grp[x][k]);
+    }
     HIPCHK(h, hipMemcpyAsync(h->slot_order, h->slot_order_h.data(), (size_t)n_occ * 4, hipMemcpyHostToDevice, h->stream));
     // spread each brick's points over the lanes (permutation inside bricks)
     uint32_t *perm2 = (perm == h->vals[0]) ? h->vals[1] : h->vals[0];
