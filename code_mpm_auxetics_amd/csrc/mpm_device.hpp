@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mpm_laws.hpp"   // det3 and the FP64 constitutive laws
+
 namespace mpm {
 
 constexpr int kBlock = 256;          // threads per particle workgroup
@@ -24,6 +26,10 @@ constexpr int kQ = 7;                // node fields: m, p.xyz, f.xyz
 constexpr uint32_t kNoBrick = 0xFFFFFFFFu;
 constexpr int kQFrac = 26;                       // fractional bits of FP32 positions
 constexpr double kQOne = 67108864.0;             // 2^26
+// FP32 state: largest brick edge in cells.  Positions are Q6.26 cells from
+// the brick origin (|u| < 32); a point lies within [-1, S + 1] of the origin
+// (the arena window) plus one step's motion before the drift check flags it.
+constexpr int kMaxBrickFp32 = 28;
 
 enum : int {
     FLAG_REBIN = 1,        // a point drifted past half a cell outside its brick
@@ -145,13 +151,5 @@ __device__ __forceinline__ double dpp_r(double v) { return dpp_d<CTRL>(v); }
 
 __device__ __forceinline__ double sgn(double a) { return (double)((0.0 < a) - (a < 0.0)); }
 __device__ __forceinline__ float sgn(float a) { return (float)((0.0f < a) - (a < 0.0f)); }
-
-// glm determinant of a column-major 3x3 (…_MP.cpp:301), always in FP64.
-__device__ __host__ __forceinline__ double det3(const double m[9])
-{
-    return +m[0] * (m[4] * m[8] - m[7] * m[5])
-           - m[3] * (m[1] * m[8] - m[7] * m[2])
-           + m[6] * (m[1] * m[5] - m[4] * m[2]);
-}
 
 }  // namespace mpm

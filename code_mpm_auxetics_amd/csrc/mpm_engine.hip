@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -32,6 +33,16 @@ using namespace mpm;
 namespace {
 
 thread_local std::string g_create_error;
+
+// mpm_constitutive: staging buffer and stream per device, kept until the
+// process ends
+struct CrStage {
+    double *buf = nullptr;
+    size_t cap = 0;   // doubles
+    hipStream_t stream = nullptr;
+};
+std::mutex g_cr_mu;
+std::map<int, CrStage> g_cr_stage;
 
 struct Buf {
     void *p = nullptr;
@@ -93,6 +104,7 @@ struct mpm_handle {
     uint8_t *slot_mat = nullptr;             // per slot: shared material index or 0xFF
     std::vector<uint32_t> slot_order_h, occ_count_h;
     int n_occ = 0, occ_cap = 0;
+    int n_xcd = 8;                           // XCDs the dispatcher deals workgroups to (MI355X: 8)
     void *arena[2] = {nullptr, nullptr};
     int4 *binfo[2] = {nullptr, nullptr};    // per brick (dense): {slot, packed touched node box}
     size_t arena_cap_slots = 0;
@@ -420,27 +432,35 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     HIPCHK(h, hipGetLastError());
     k_occ_count<<<cdiv(n_occ, kBlock), kBlock, 0, h->stream>>>(h->occ_start, n_occ, n, h->occ_count);
     HIPCHK(h, hipGetLastError());
-    // launch order of the brick blocks.  Workgroups go to the 8 XCDs round
-    // robin by block index, so block 8k + x takes the k-th slot of group x,
+    // launch order of the brick blocks.  Workgroups go to the XCDs round
+    // robin by block index, so block X k + x takes the k-th slot of group x,
     // the x-th run of consecutive slots (bricks in space order): a brick and
     // most of its neighbours then run on one XCD every step, and the arena
-    // records a block reads were written through that XCD's L2.  Inside a
-    // group: most point chunks first, ties in slot order (a longest-first
-    // schedule).  Every slot still sums in slot order.
-    constexpr uint32_t kXcd = 8;
+    // records a block reads were written through that XCD's L2.  Group sizes
+    // differ by at most one (the first n_occ % X groups take one more), so
+    // every round of X blocks but the last visits each group once and the
+    // mapping holds to the end.  Inside a group: most point chunks first,
+    // ties in slot order (a longest-first schedule).  Every slot still sums
+    // in slot order.
+    const uint32_t kXcd = (uint32_t)h->n_xcd;
     h->occ_count_h.resize(n_occ);
     h->slot_order_h.clear();
     HIPCHK(h, hipMemcpyAsync(h->occ_count_h.data(), h->occ_count, (size_t)n_occ * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     {
-        const uint32_t per = cdiv(n_occ, kXcd);
+        const uint32_t q = n_occ / kXcd, r = n_occ % kXcd;
         std::vector<std::vector<uint32_t>> grp(kXcd);
-        for (uint32_t k = 0; k < n_occ; ++k) grp[k / per].push_back(k);
+        uint32_t k0 = 0;
+        for (uint32_t x = 0; x < kXcd; ++x) {
+            const uint32_t sz = q + (x < r ? 1u : 0u);
+            for (uint32_t k = 0; k < sz; ++k) grp[x].push_back(k0 + k);
+            k0 += sz;
+        }
         for (auto &gl : grp)
             std::stable_sort(gl.begin(), gl.end(), [h](uint32_t x, uint32_t y) {
                 return cdiv(h->occ_count_h[x], kPBlock) > cdiv(h->occ_count_h[y], kPBlock);
             });
-        for (uint32_t k = 0; k < per; ++k)
+        for (uint32_t k = 0; k <= q; ++k)
             for (uint32_t x = 0; x < kXcd; ++x)
                 if (k < grp[x].size()) h->slot_order_h.push_back(grp[x][k]);
     }
@@ -1100,6 +1120,17 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         g_create_error = "bad timeline";
         return MPM_E_ARG;
     }
+    // FP32 state holds positions as Q6.26 cells relative to the brick origin
+    // (|u| < 32); a point stays within [-1, S + 1] cells of it plus one
+    // step's motion before the drift check flags it
+    for (int a = 0; a < 3; ++a) {
+        const int S = std::min(cfg->brick[a] > 0 ? cfg->brick[a] : (a == 2 ? 10 : 4), std::max(1, cfg->cells[a]));
+        if (cfg->precision == MPM_FP32 && S > kMaxBrickFp32) {
+            g_create_error = "FP32 state: brick size must be at most " + std::to_string(kMaxBrickFp32) +
+                             " cells per axis (Q6.26 positions relative to the brick origin)";
+            return MPM_E_ARG;
+        }
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return MPM_E_NO_DEVICE; }
     if (cfg->device < 0 || cfg->device >= ndev) { g_create_error = "bad device ordinal"; return MPM_E_ARG; }
@@ -1161,6 +1192,12 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
             e2 = hipFuncSetAttribute((const void *)k_particle<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
         }
         if (e1 != hipSuccess || e2 != hipSuccess) { h->err = "hipFuncSetAttribute (dynamic LDS) failed"; return fail(MPM_E_HIP); }
+    }
+    {
+        // the XCDs workgroups are dealt to round robin (brick launch order)
+        int nx = 0;
+        if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, h->device) == hipSuccess && nx >= 1 && nx <= 64)
+            h->n_xcd = nx;
     }
     const int ncb = g.S[0] * g.S[1] * g.S[2];
     g.cbits = 0;
@@ -1716,64 +1753,55 @@ int mpm_constitutive(int32_t law, const mpm_cr_batch *b, int32_t device)
         g_create_error = "the ViscoElastic and IdealGas laws are not dispatched by the reference step and not provided";
         return MPM_E_ARG;
     }
-    if (law != MPM_LAW_ELASTIC && law != MPM_LAW_NEOHOOKEAN && law != MPM_LAW_PLASTIC && law != MPM_LAW_VONMISES &&
-        law != MPM_LAW_J2_STATE) {
+    if (!cr_law_known(law)) {
         g_create_error = "unknown constitutive law";
         return MPM_E_ARG;
     }
-    const size_t n = (size_t)b->n;
-    if (n == 0) return MPM_OK;
-    // the inputs and outputs each law reads and writes
-    const bool j2 = law == MPM_LAW_J2_STATE;
-    bool ok = b->stress || law == MPM_LAW_ELASTIC;
-    if (!j2) {
-        ok &= b->E && b->nu && b->stress_inc && b->plastic_inc && b->alpha_inc;
-        if (law != MPM_LAW_NEOHOOKEAN) ok &= b->strain_inc != nullptr;
-        if (law == MPM_LAW_NEOHOOKEAN) ok &= b->F != nullptr;
-        if (law == MPM_LAW_PLASTIC || law == MPM_LAW_VONMISES) ok &= b->yield != nullptr;
-        if (law == MPM_LAW_VONMISES) ok &= b->alpha && b->C0 && b->C1;
-    } else {
-        ok &= b->j2 != nullptr;
-    }
-    if (!ok) { g_create_error = "constitutive batch: missing input or output array"; return MPM_E_ARG; }
+    if (b->n == 0) return MPM_OK;
+    CrPart parts[kCrParts];
+    const char *why = nullptr;
+    const long long total = cr_layout(law, b, parts, &why);
+    if (total < 0) { g_create_error = why; return MPM_E_ARG; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return MPM_E_NO_DEVICE; }
+    if (device < 0 || device >= ndev) { g_create_error = "bad device ordinal"; return MPM_E_ARG; }
+    // one staging buffer and stream per device, kept for the process and
+    // grown when a batch needs more; calls are serialised per device
+    std::lock_guard<std::mutex> lk(g_cr_mu);
+    CrStage &cs = g_cr_stage[device];
     if (hipSetDevice(device) != hipSuccess) { g_create_error = "hipSetDevice"; return MPM_E_HIP; }
-    // one device block: the inputs given, then the outputs of this law
-    struct Part { const double *src; double *dst; size_t len; };
-    const Part parts[] = {{b->E, nullptr, n}, {b->nu, nullptr, n}, {b->yield, nullptr, n}, {b->alpha, nullptr, n},
-                          {b->C0, nullptr, n}, {b->C1, nullptr, n}, {b->stress, nullptr, 6 * n},
-                          {b->strain_inc, nullptr, 6 * n}, {b->F, nullptr, 9 * n},
-                          {nullptr, j2 ? nullptr : b->stress_inc, 6 * n}, {nullptr, j2 ? nullptr : b->plastic_inc, 6 * n},
-                          {nullptr, j2 ? nullptr : b->alpha_inc, n}, {nullptr, j2 ? b->j2 : nullptr, n}};
-    constexpr int NP = sizeof(parts) / sizeof(parts[0]);
-    size_t off[NP], total = 0;
-    for (int k = 0; k < NP; ++k) {
-        off[k] = total;
-        if (parts[k].src || parts[k].dst) total += parts[k].len;
+    if (!cs.stream && hipStreamCreateWithFlags(&cs.stream, hipStreamNonBlocking) != hipSuccess) {
+        cs.stream = nullptr;
+        g_create_error = "hipStreamCreate";
+        return MPM_E_HIP;
     }
-    double *dev = nullptr;
-    if (hipMalloc((void **)&dev, total * sizeof(double)) != hipSuccess) { g_create_error = "hipMalloc"; return MPM_E_HIP; }
+    if ((size_t)total > cs.cap) {
+        if (cs.buf) (void)hipFree(cs.buf);
+        cs.buf = nullptr;
+        cs.cap = 0;
+        const size_t want = std::max<size_t>((size_t)total, 1u << 16);
+        if (hipMalloc((void **)&cs.buf, want * sizeof(double)) != hipSuccess) {
+            cs.buf = nullptr;
+            g_create_error = "hipMalloc (constitutive staging)";
+            return MPM_E_NOMEM;
+        }
+        cs.cap = want;
+    }
     int rc = MPM_OK;
-    double *dp[NP];
-    for (int k = 0; k < NP; ++k) {
-        dp[k] = (parts[k].src || parts[k].dst) ? dev + off[k] : nullptr;
-        if (parts[k].src && hipMemcpy(dp[k], parts[k].src, parts[k].len * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    for (int k = 0; k < kCrParts && rc == MPM_OK; ++k)
+        if (parts[k].src && hipMemcpyAsync(cs.buf + parts[k].off, parts[k].src, parts[k].len * sizeof(double),
+                                           hipMemcpyHostToDevice, cs.stream) != hipSuccess)
             rc = MPM_E_HIP;
-    }
     if (rc == MPM_OK) {
-        CrBatch cb{};
-        cb.n = b->n;
-        cb.E = dp[0]; cb.nu = dp[1]; cb.sy = dp[2]; cb.alpha = dp[3]; cb.C0 = dp[4]; cb.C1 = dp[5];
-        cb.s = dp[6]; cb.de = dp[7]; cb.F = dp[8];
-        cb.ds = dp[9]; cb.dep = dp[10]; cb.da = dp[11]; cb.j2 = dp[12];
-        k_constitutive<<<cdiv(n, 256), 256, 0, 0>>>(law, cb);
-        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = MPM_E_HIP;
+        const CrBatch cb = cr_views(b->n, cs.buf, parts);
+        k_constitutive<<<cdiv((size_t)b->n, 256), 256, 0, cs.stream>>>(law, cb);
+        if (hipGetLastError() != hipSuccess) rc = MPM_E_HIP;
     }
-    for (int k = 0; rc == MPM_OK && k < NP; ++k)
-        if (parts[k].dst && hipMemcpy(parts[k].dst, dp[k], parts[k].len * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    for (int k = 0; k < kCrParts && rc == MPM_OK; ++k)
+        if (parts[k].dst && hipMemcpyAsync(parts[k].dst, cs.buf + parts[k].off, parts[k].len * sizeof(double),
+                                           hipMemcpyDeviceToHost, cs.stream) != hipSuccess)
             rc = MPM_E_HIP;
-    (void)hipFree(dev);
+    if (hipStreamSynchronize(cs.stream) != hipSuccess) rc = MPM_E_HIP;
     if (rc != MPM_OK) g_create_error = "constitutive batch: HIP error";
     return rc;
 }

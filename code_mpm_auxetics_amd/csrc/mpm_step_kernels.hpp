@@ -38,94 +38,8 @@
 namespace mpm {
 
 // ------------------------------------------------------------- constitutive
-// ConstitutiveRelation_calculateIncrement_Elastic.cpp:4-15
-template <typename T>
-__device__ __forceinline__ void elastic_inc(T E, T nu, const T de[6], T ds[6])
-{
-    T C = E / (T(1) + nu) / (T(1) - T(2) * nu);
-    ds[0] = C * ((T(1) - nu) * de[0] + nu * de[1] + nu * de[2]);
-    ds[1] = C * ((T(1) - nu) * de[1] + nu * de[0] + nu * de[2]);
-    ds[2] = C * ((T(1) - nu) * de[2] + nu * de[0] + nu * de[1]);
-    ds[3] = C * ((T(0.5) - nu) * de[3]);
-    ds[4] = C * ((T(0.5) - nu) * de[4]);
-    ds[5] = C * ((T(0.5) - nu) * de[5]);
-}
-
-// ConstitutiveRelation_calculateState_J2.cpp:4-26
-__device__ __forceinline__ double j2_of(const double s[6])
-{
-    double j2 = 0.0;
-    j2 += 1.0 / 6.0 * (s[0] - s[1]) * (s[0] - s[1]);
-    j2 += 1.0 / 6.0 * (s[1] - s[2]) * (s[1] - s[2]);
-    j2 += 1.0 / 6.0 * (s[2] - s[0]) * (s[2] - s[0]);
-    j2 += s[3] * s[3];
-    j2 += s[4] * s[4];
-    j2 += s[5] * s[5];
-    return j2;
-}
-
-// Radial return shared by …PerfectlyPlastic.cpp:4-72 and
-// …VonMisesHardening.cpp:4-86; always evaluated in FP64.
-__device__ __forceinline__ bool radial_return(double E, double nu, double radius,
-                                              const double s[6], const double de[6],
-                                              double ds[6], double dep[6])
-{
-    double st[6], dev[6];
-    elastic_inc<double>(E, nu, de, ds);
-    for (int i = 0; i < 6; ++i) st[i] = s[i] + ds[i];
-    double p = 1.0 / 3.0 * (st[0] + st[1] + st[2]);
-    for (int i = 0; i < 6; ++i) dev[i] = st[i];
-    dev[0] -= p;
-    dev[1] -= p;
-    dev[2] -= p;
-    double j2 = j2_of(st);
-    if (j2 <= radius * radius) {
-        for (int i = 0; i < 6; ++i) dep[i] = 0.0;
-        return false;
-    }
-    double sc = radius / sqrt(j2);
-    for (int i = 0; i < 6; ++i) dev[i] *= sc;
-    for (int i = 0; i < 6; ++i) ds[i] = dev[i] - s[i];
-    ds[0] += p;
-    ds[1] += p;
-    ds[2] += p;
-    double G = 0.5 * E / (1.0 + nu);
-    double dee[6];
-    dee[0] = ds[0] / E - ds[1] * nu / E - ds[2] * nu / E;
-    dee[1] = ds[1] / E - ds[0] * nu / E - ds[2] * nu / E;
-    dee[2] = ds[2] / E - ds[0] * nu / E - ds[1] * nu / E;
-    dee[3] = ds[3] / G;
-    dee[4] = ds[4] / G;
-    dee[5] = ds[5] / G;
-    for (int i = 0; i < 6; ++i) dep[i] = de[i] - dee[i];
-    return true;
-}
-
-// …PerfectlyPlastic.cpp:4-72: k = sigma_y / sqrt(3).  True on the plastic branch.
-__device__ __forceinline__ bool perfectly_plastic_inc(double E, double nu, double sy, const double s[6],
-                                                      const double de[6], double ds[6], double dep[6])
-{
-    return radial_return(E, nu, sy / sqrt(3.0), s, de, ds, dep);
-}
-
-// …VonMisesHardening.cpp:4-86: radius k + alpha / sqrt(2); on the plastic
-// branch the equivalent plastic strain sqrt(2/3 sum dep_i^2) (engineering
-// shear, no factor 2) drives d alpha = C0 (sqrt(2/3) C1 - alpha) eps_bar when
-// positive.  True on the plastic branch.
-__device__ __forceinline__ bool vonmises_inc(double E, double nu, double sy, double al, double C0, double C1,
-                                             const double s[6], const double de[6], double ds[6], double dep[6],
-                                             double &da)
-{
-    da = 0.0;
-    const double k0 = sy / sqrt(3.0);
-    if (!radial_return(E, nu, k0 + al / sqrt(2.0), s, de, ds, dep)) return false;
-    double eff = 0.0;
-    for (int q = 0; q < 6; ++q) eff += dep[q] * dep[q];
-    eff *= 2.0 / 3.0;
-    eff = sqrt(eff);
-    if ((sqrt(2.0 / 3.0) * C1 - al) > 0.0) da = C0 * (sqrt(2.0 / 3.0) * C1 - al) * eff;
-    return true;
-}
+// The FP64 laws (elastic_inc, j2_of, radial_return, perfectly_plastic_inc,
+// vonmises_inc, neohookean_inc) are in mpm_laws.hpp.
 
 // The same radial return for FP32 state: FP64 arithmetic, elastic constants
 // precomputed (C = E/(1+nu)/(1-2nu), 1/E, 1/G: reciprocals instead of the
@@ -163,32 +77,6 @@ __device__ __forceinline__ bool radial_return_fp32(double nu, double C, double i
     dep[4] = de[4] - ds[4] * ig;
     dep[5] = de[5] - ds[5] * ig;
     return true;
-}
-
-// ConstitutiveRelation_calculateIncrement_NeoHookean.cpp:4-22, FP64.
-__device__ __forceinline__ void neohookean_inc(double E, double nu, const double s[6],
-                                               const double F[9], double ds[6])
-{
-    double mu = 0.5 * E / (1.0 + nu);
-    double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
-    double J = det3(F);
-    double a = lam * log(J) / J;
-    double b = mu / J;
-    double P[9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            double fft = F[r] * F[c] + F[3 + r] * F[3 + c] + F[6 + r] * F[6 + c];
-            double id = (c == r) ? 1.0 : 0.0;
-            P[3 * c + r] = a * id + b * (fft - id);
-        }
-    ds[0] = P[0] - s[0];
-    ds[1] = P[4] - s[1];
-    ds[2] = P[8] - s[2];
-    ds[3] = P[1] - s[3];
-    ds[4] = P[5] - s[4];
-    ds[5] = P[6] - s[5];
 }
 
 // ------------------------------------------------------------------ locate
@@ -576,45 +464,12 @@ __global__ void __launch_bounds__(256) k_slot_mat(const uint8_t *mat, const uint
 }
 
 // ---------------------------------------------------- batched constitutive
-// The constitutive functions of the step applied to n independent inputs
-// (ConstitutiveRelation.h:16-34): law 0 Elastic, 1 NeoHookean, 3
-// PerfectlyPlastic, 4 VonMisesHardening (the MaterialPoint.h:8-16 type codes),
-// 16 calculateState_J2 of the stress.  FP64, the reference's evaluation order.
-struct CrBatch {
-    int32_t n;
-    const double *E, *nu, *sy, *alpha, *C0, *C1;
-    const double *s, *de, *F;    // 6n, 6n, 9n (column-major)
-    double *ds, *dep, *da, *j2;  // 6n, 6n, n, n
-};
+// One thread per input of an mpm_constitutive batch (body: cr_apply).
 __global__ void __launch_bounds__(256) k_constitutive(int law, CrBatch b)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= b.n) return;
-    double s[6], de[6], ds[6] = {0, 0, 0, 0, 0, 0}, dep[6] = {0, 0, 0, 0, 0, 0}, da = 0.0;
-    for (int q = 0; q < 6; ++q) {
-        s[q] = b.s ? b.s[6 * i + q] : 0.0;
-        de[q] = b.de ? b.de[6 * i + q] : 0.0;
-    }
-    if (law == 0) {
-        elastic_inc<double>(b.E[i], b.nu[i], de, ds);
-    } else if (law == 1) {
-        double F[9];
-        for (int q = 0; q < 9; ++q) F[q] = b.F[9 * i + q];
-        neohookean_inc(b.E[i], b.nu[i], s, F, ds);
-    } else if (law == 3) {
-        perfectly_plastic_inc(b.E[i], b.nu[i], b.sy[i], s, de, ds, dep);
-    } else if (law == 4) {
-        vonmises_inc(b.E[i], b.nu[i], b.sy[i], b.alpha[i], b.C0[i], b.C1[i], s, de, ds, dep, da);
-    }
-    if (law == 16) {
-        b.j2[i] = j2_of(s);
-        return;
-    }
-    for (int q = 0; q < 6; ++q) {
-        b.ds[6 * i + q] = ds[q];
-        b.dep[6 * i + q] = dep[q];
-    }
-    b.da[i] = da;
+    cr_apply(law, b, i);
 }
 
 // Fixed-point fraction bits.  FP32 state: each contribution is converted to a
@@ -632,7 +487,8 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
 {
     extern __shared__ long long s_arena[];   // node-major: kQ int64 per arena node, then
                                              // (G2P) the node window: 6 reals {v, a} per node
-    __shared__ uint32_t s_max[8];            // running max bound bits per field
+    __shared__ uint32_t s_max[3][8];         // running max bound bits per field, one buffer per
+                                             // chunk index mod 3 (see the chunk loop)
     __shared__ int4 s_nb[27][2];             // neighbour arenas (dz, dy, dx ascending): {element
                                              // base, window low corner in this brick's node
                                              // coordinates}, {window extent, ext x * ext y}
@@ -848,11 +704,14 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     // the grid update and this clear
     const int clear0 = (stash * (int)sizeof(real) + 7) / 8;
     for (int i = clear0 + tid; i < kQ * AN; i += kPBlock) s_arena[i] = 0;
-    if (tid < 8) s_max[tid] = 0u;
+    if (tid < 24) s_max[tid >> 3][tid & 7] = 0u;
     int H = 6;   // a node takes at most one contribution per point: count < 2^(H-1)
     while ((1u << (H - 1)) <= count) ++H;
     const int FRAC = FP32 ? kFrac32 : 61 - H;
     int ecur[kQ];                        // block-uniform field exponents (value < 2^e)
+    uint32_t run_max[kQ];                // block-uniform running max of the bound bits
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) run_max[q] = 0u;
 #pragma unroll
     for (int q = 0; q < kQ; ++q) ecur[q] = kUnsetExp;
     double red[5] = {0, 0, 0, 0, 0};     // this lane's running check sums (chunk order)
@@ -1044,19 +903,30 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             }
         }
 
-        // ---- block-wide fixed-point exponents: running max of the bounds
+        // ---- block-wide fixed-point exponents: running max of the bounds.
+        // Chunk c writes and reads buffer c % 3.  After the barrier below a
+        // wave may run ahead into chunk c + 1 and write its maxima while a
+        // slower wave still reads chunk c's: the buffers differ, and no wave
+        // reaches chunk c + 2 (buffer c + 2 = c - 1 mod 3) before every wave
+        // has passed chunk c + 1's barrier, long after its chunk-c read.  Each
+        // wave adds the running max of the chunks before (block-uniform), so
+        // the buffer holds the running max; wave 0 clears buffer c + 2 after
+        // this chunk's barrier, when chunk c - 1's reads of it are complete.
+        const int mbuf = (int)((c0 / kPBlock) % 3u);
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
-            const uint32_t w = wave_umax_dpp(bq[q]);
-            if (lane == 0 && w) atomicMax(&s_max[q], w);
+            const uint32_t w = max(wave_umax_dpp(bq[q]), run_max[q]);
+            if (lane == 0 && w) atomicMax(&s_max[mbuf][q], w);
         }
         __syncthreads();
+        if (tid < 8) s_max[mbuf == 0 ? 2 : mbuf - 1][tid] = 0u;
         {
             bool any = false;
             int sh[kQ];
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {
-                const uint32_t mb = s_max[q];
+                const uint32_t mb = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_max[mbuf][q]);
+                run_max[q] = mb;
                 // positive float bits: biased exponent + 1 bounds the value
                 const int e = (mb == 0u) ? kUnsetExp : max((int)(mb >> 23) - 126, kMinExp);
                 const int en = __builtin_amdgcn_readfirstlane(e);

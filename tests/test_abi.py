@@ -134,6 +134,29 @@ def test_create_rejects_bad_arguments_before_touching_a_device():
     assert lib.mpm_create(C.byref(_config(n_timeline=2)), C.byref(h)) == A.MPM_E_ARG
 
 
+def test_create_rejects_fp32_bricks_beyond_the_fixed_point_range():
+    """FP32 state keeps positions as Q6.26 cells from the brick origin (int32:
+    |u| < 32 cells).  A brick edge above 28 cells would let a drifting point
+    overflow it, so mpm_create refuses such bricks in FP32 (before any device
+    call) and accepts them in FP64, which stores absolute positions."""
+    lib = A.lib()
+    h = C.c_void_p()
+    big = _config(precision=A.MPM_FP32)
+    big.cells[2] = 64
+    big.brick[0], big.brick[1], big.brick[2] = 3, 3, 32
+    assert lib.mpm_create(C.byref(big), C.byref(h)) == A.MPM_E_ARG
+    assert b"at most 28 cells" in lib.mpm_last_error(None)
+    assert not h.value
+    for prec, edge in ((A.MPM_FP32, 28), (A.MPM_FP64, 32)):
+        cfg = _config(precision=prec)
+        cfg.cells[2] = 64
+        cfg.brick[0], cfg.brick[1], cfg.brick[2] = 3, 3, edge
+        rc = lib.mpm_create(C.byref(cfg), C.byref(h))
+        assert rc in (A.MPM_OK, A.MPM_E_NO_DEVICE), lib.mpm_last_error(None)
+        if rc == A.MPM_OK:
+            lib.mpm_destroy(h)
+
+
 def test_create_without_a_device_reports_no_device():
     import torch
     if torch.cuda.is_available():

@@ -9,11 +9,11 @@ device may change.  Tolerances are relative to each output's magnitude."""
 import numpy as np
 import pytest
 
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(
-    reason="held back: the GPU test run that included these cases ended with a device memory-fault report; no "
-           "faulting access has been found in the batched constitutive kernel, and it stays off the device until one is")]
+import cr_inputs as ci
 
-N = 2000
+pytestmark = pytest.mark.gpu
+
+N = ci.N
 
 
 @pytest.fixture(scope="module")
@@ -30,14 +30,7 @@ def _close(a, b, rtol):
     return np.all(np.abs(a - b) <= rtol * scale)
 
 
-def _inputs(seed, n=N):
-    rng = np.random.default_rng(seed)
-    E = 10 ** rng.uniform(8.5, 11.5, n)
-    nu = rng.uniform(0.05, 0.45, n)
-    sy = 10 ** rng.uniform(7.5, 9.0, n)
-    s = rng.normal(size=(n, 6)) * (sy[:, None] * rng.uniform(0.01, 0.25, n)[:, None])
-    de = rng.normal(size=(n, 6)) * (sy / E)[:, None] * rng.uniform(0.0, 1.5, n)[:, None]
-    return E, nu, sy, s, de
+_inputs = ci.inputs
 
 
 def test_elastic(cr, oracle_mod):
@@ -82,10 +75,7 @@ def test_perfectly_plastic(cr, oracle_mod):
 
 def test_vonmises_hardening(cr, oracle_mod):
     E, nu, sy, s, de = _inputs(4)
-    rng = np.random.default_rng(5)
-    alpha = rng.uniform(0.0, 0.3, N) * sy
-    C1 = 10 ** rng.uniform(8, 15, N)
-    C0 = 7.0e8 / C1 * rng.uniform(0.5, 2.0, N)
+    alpha, C0, C1 = ci.hardening(5, sy)
     radius = sy / np.sqrt(3.0) + alpha / np.sqrt(2.0)
     margin = _trial_margin(oracle_mod, E, nu, s, de, radius)
     keep = np.abs(margin) > 1e-9
@@ -104,11 +94,7 @@ def test_vonmises_hardening(cr, oracle_mod):
 
 
 def test_neohookean(cr, oracle_mod):
-    rng = np.random.default_rng(6)
-    E = 10 ** rng.uniform(6, 10, N)
-    nu = rng.uniform(0.05, 0.45, N)
-    s = rng.normal(size=(N, 6)) * 1e6
-    Fm = np.eye(3)[None] + rng.normal(size=(N, 3, 3)) * 0.1   # [row][col]
+    E, nu, s, Fm = ci.neohookean_inputs(6)
     got = cr.increments("neohookean", E=E, nu=nu, stress=s, F=Fm)["stress_inc"]
     for i in range(N):
         want = oracle_mod.neohookean(E[i], nu[i], s[i], Fm[i].T.reshape(9))   # column-major

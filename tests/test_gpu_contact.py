@@ -90,11 +90,7 @@ def test_contact_blocks_invariants(mpm, oracle_mod):
     assert rel_l2(vc, vo) <= 1e-3
 
 
-HELD = ("held back: the GPU test run that last included this case ended with a device memory-fault "
-        "report; no faulting access has been found in the code it runs, and it stays off the device until one is")
-
-
-@pytest.mark.parametrize("material", ["elastic", pytest.param("j2", marks=pytest.mark.skip(reason=HELD))])
+@pytest.mark.parametrize("material", ["elastic", "j2"])
 def test_contact_world_with_platens_fp64(mpm, oracle_mod, material):
     """Platen points take part in the contact P2G and override both fields;
     the reaction history is the platen's (…_MP_Contact.cpp:494-518)."""
@@ -104,16 +100,16 @@ def test_contact_world_with_platens_fp64(mpm, oracle_mod, material):
     top = (sc.dc == 1) & (sc.x[:, 1] > sc.x[:, 1].mean())
     sc.v[top] = (0.0, -2.0, 0.0)
     x0 = sc.x.copy()
-    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64)
     orc = oracle_mod.Oracle(sc)
-    eng.run_steps(60, path=mpm.PATH_CONTACT)
     orc.run_steps(60, 1)
-    _cmp(eng.getMaterialPoints(), orc.points(), x0, 1e-9)
-    hg, ho = eng.reaction_history(), orc.reaction
+    with mpm.PhysicsEngine(sc, precision=mpm.FP64) as eng:
+        eng.run_steps(60, path=mpm.PATH_CONTACT)
+        pts, hg = eng.getMaterialPoints(), eng.reaction_history()
+    _cmp(pts, orc.points(), x0, 1e-9)
+    ho = orc.reaction
     assert len(hg) == len(ho)
     assert np.abs(hg - ho).max() <= 1e-8 * np.abs(ho).max() + 1e-20
     assert np.abs(ho).max() > 0
-    eng.close()
 
 
 def test_contact_fp32_bound(mpm, oracle_mod):
