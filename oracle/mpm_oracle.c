@@ -18,15 +18,29 @@
  *                                 alpha increment closed form, reduces to
  *                                 perfectly plastic when C0 == 0.
  *   or_neohookean               : F = I gives 0; uniaxial stretch closed form.
+ *   or_j2                       : uniaxial sigma^2/3, pure shear tau^2,
+ *                                 hydrostatic 0, 1/2 dev:dev of the tensor
+ *                                 (numpy) and rotation invariance.
  *   or_det3                     : numpy.linalg.det on random matrices.
  *   or_timeline_velocity        : interpolation closed form, outside = 0.
  *   or_step_plain               : mass / momentum invariants, free flight,
  *                                 uniform-stress patch, 1-point hand step,
- *                                 committed golden fixtures.
+ *                                 committed golden fixtures; phase 5 damping
+ *                                 (one-point closed form with and without
+ *                                 velocity), the y reaction into Force_Temp
+ *                                 (bottom-node share of the weights), phase 6
+ *                                 platen override (point sharing a platen
+ *                                 point's cell moves with the platen, keeps
+ *                                 v and F).
  *   or_step_contact             : with no surface points it reduces to a
  *                                 single-field step with sqrt(3) mass gates
  *                                 (checked against or_step_plain); contact
- *                                 correction conserves node momentum.
+ *                                 correction conserves node momentum; kernel
+ *                                 density special values (w(0) = 1, w(1) = 0,
+ *                                 w(1/2) = 1/2, cell centre w(sqrt(3)/2));
+ *                                 kernel gradient exact on a linear field,
+ *                                 zero on a constant one, and the x - 1 row
+ *                                 wrap at ix = 0.
  *   glm rounding order (determinant, outerProduct, mat*mat): parity unpinned
  *   beyond this restatement -- the reference cannot be compiled here (glm is
  *   absent), so no reference build exists to compare ULPs against.

@@ -355,3 +355,203 @@ def test_contact_correction_conserves_momentum_and_resists_approach(O):
     closing_c = vc[fa, 0].mean() - vc[fb, 0].mean()
     closing_p = vp[fa, 0].mean() - vp[fb, 0].mean()
     assert closing_c < closing_p
+
+
+# ------------------------------------------------- pins of single phases
+def _one_scene(x, v=None, stress_free=True, cells=(8, 8, 8), h=0.001, m=2.0e-6, V0=1.0e-9, dt=1.0e-6,
+               fext=None, damping=0.0, fixed=None, dc=None, timeline=None):
+    x = np.atleast_2d(np.asarray(x, float))
+    n = len(x)
+    nn = np.prod([c + 1 for c in cells])
+    kw = {}
+    if timeline is not None:
+        kw = dict(timeline_t=np.asarray(timeline[0], float), timeline_v=np.asarray(timeline[1], float))
+    return scene.Scene(length=tuple(c * h for c in cells), cells=cells,
+                       fixed=np.zeros(nn, np.uint8) if fixed is None else fixed, x=x, mass=np.full(n, m),
+                       vol0=np.full(n, V0), E=np.full(n, 1e7), nu=np.full(n, 0.3), sy=np.zeros(n), C0=np.zeros(n),
+                       C1=np.zeros(n), type=np.zeros(n, np.int32),
+                       dc=np.zeros(n, np.uint8) if dc is None else np.asarray(dc, np.uint8),
+                       v=np.zeros((n, 3)) if v is None else np.atleast_2d(np.asarray(v, float)).copy(),
+                       fext=None if fext is None else np.atleast_2d(np.asarray(fext, float)).copy(),
+                       dt_max=dt, time_end=1.0, damping=damping, mass_min=0.0, **kw)
+
+
+def _j2_tensor(s):
+    """J2 = 1/2 dev(sigma):dev(sigma) of the Voigt stress (tensor shear)."""
+    S = np.array([[s[0], s[3], s[5]], [s[3], s[1], s[4]], [s[5], s[4], s[2]]])
+    D = S - np.trace(S) / 3.0 * np.eye(3)
+    return 0.5 * float(np.sum(D * D))
+
+
+def test_j2_closed_forms_and_tensor_invariant(O):
+    """calculateState_J2 (ConstitutiveRelation_calculateState_J2.cpp:4-26):
+    uniaxial sigma gives sigma^2/3, pure shear tau gives tau^2 on every shear
+    component, a hydrostatic state gives 0, and random stresses give
+    1/2 dev:dev of the tensor, which is invariant under rotation."""
+    for sig in (1.0, -250e6, 3.7e4):
+        for ax in range(3):
+            s = np.zeros(6)
+            s[ax] = sig
+            assert O.j2(s) == pytest.approx(sig * sig / 3.0, rel=1e-15)
+    for tau in (1.0, 80e6):
+        for ax in range(3, 6):
+            s = np.zeros(6)
+            s[ax] = tau
+            assert O.j2(s) == pytest.approx(tau * tau, rel=1e-15)
+    assert O.j2(np.array([5e7, 5e7, 5e7, 0, 0, 0])) == 0.0
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        s = rng.normal(size=6) * 1e8
+        assert O.j2(s) == pytest.approx(_j2_tensor(s), rel=1e-12)
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        S = np.array([[s[0], s[3], s[5]], [s[3], s[1], s[4]], [s[5], s[4], s[2]]])
+        R = Q @ S @ Q.T
+        sr = np.array([R[0, 0], R[1, 1], R[2, 2], R[0, 1], R[1, 2], R[2, 0]])
+        assert O.j2(sr) == pytest.approx(O.j2(s), rel=1e-11)
+
+
+def test_damping_one_point_closed_form(O):
+    """One stress-free point with velocity v and external force f: every node
+    gets v_I = v and f_I = N_I f; damping (…_MP.cpp:213-218) removes
+    c_d |f_I| v/|v|, and G2P (ΣN = 1) gives
+    v' = v + dt/m (f - c_d |f| v/|v|), x' = x + v dt + dt^2/m (f - c_d |f| v/|v|).
+    With v = 0 the damping switch (|v_I| > 1e-9) is off."""
+    h, m, dt, cd = 0.001, 2.0e-6, 1.0e-6, 0.37
+    x = np.array([3.37 * h, 4.71 * h, 2.23 * h])
+    v = np.array([0.4, -0.3, 0.2])
+    f = np.array([1.5e-3, 2.0e-3, -0.7e-3])
+    for vel in (v, np.zeros(3)):
+        orc = O.Oracle(_one_scene(x, v=vel, fext=f, damping=cd, m=m, dt=dt))
+        orc.step(dt)
+        nv = np.linalg.norm(vel)
+        acc = (f - (cd * np.linalg.norm(f) * vel / nv if nv > 0 else 0.0)) / m
+        p = orc.points()
+        assert np.allclose(p["v"][0], vel + dt * acc, rtol=1e-12, atol=1e-15)
+        assert np.allclose(p["x"][0], x + vel * dt + dt * dt * acc, rtol=0, atol=1e-18)
+    # the damping term is visible at this size
+    assert cd * np.linalg.norm(f) / m * dt > 1e-4
+
+
+def test_reaction_of_fixed_bottom_nodes_closed_form(O):
+    """Points in the first cell layer under external forces f_p = -m_p g, on
+    a grid whose y = 0 node plane is fixed in y: the reaction (the y force
+    the fixed nodes give up into Force_Temp, …_MP.cpp:227-232) of the first
+    step is the share of the weights the bottom nodes carry,
+    -g Σ m_p (1 - y_p / h).  Points in the second layer add nothing."""
+    h, m, g = 0.001, 2.0e-6, 9.81
+    cells = (8, 8, 8)
+    rng = np.random.default_rng(4)
+    x = np.column_stack([rng.uniform(2.1, 5.9, 12) * h, rng.uniform(0.05, 0.95, 12) * h, rng.uniform(2.1, 5.9, 12) * h])
+    x2 = x.copy()
+    x2[:4, 1] += h   # second layer
+    nn = np.prod([c + 1 for c in cells])
+    fixed = np.zeros(nn, np.uint8)
+    iy = (np.arange(nn) // (cells[0] + 1)) % (cells[1] + 1)
+    fixed[iy == 0] |= 2   # FIX_Y
+    for X in (x, x2):
+        masses = rng.uniform(0.5, 2.0, len(X)) * m
+        sc = _one_scene(X, fext=np.column_stack([np.zeros(len(X)), -masses * g, np.zeros(len(X))]), fixed=fixed, m=m)
+        sc.mass = masses
+        orc = O.Oracle(sc)
+        orc.run_steps(1)
+        first = X[:, 1] < h
+        want = -g * float(np.sum(masses[first] * (1.0 - X[first, 1] / h)))
+        assert orc.reaction[0] == pytest.approx(want, rel=1e-12)
+
+
+def test_platen_override_closed_form(O):
+    """A stressed, moving point sharing its cell with a displacement-controlled
+    point: the platen override (…_MP.cpp:245-265) sets all 8 nodes to the
+    platen velocity with zero force, so the point keeps its velocity and
+    F (L = v_platen (x) Σ grad N = 0) and moves by v_platen dt; the platen
+    point moves by its TimeLine velocity (…_MP.cpp:360-368)."""
+    h, dt = 0.001, 1.0e-6
+    xp = np.array([3.37 * h, 4.71 * h, 2.23 * h])
+    xd = np.array([3.61 * h, 4.12 * h, 2.88 * h])
+    vp = np.array([0.4, -0.3, 0.2])
+    vd = np.array([0.0, -1.5, 0.25])
+    sc = _one_scene(np.vstack([xp, xd]), v=np.vstack([vp, vd]), dc=[0, 1], dt=dt,
+                    timeline=([0.0, 1.0], [vd, vd]))
+    orc = O.Oracle(sc)
+    orc.a["stress"][0] = np.array([3.0e6, -1.0e6, 2.0e6, 4.0e5, -7.0e5, 9.0e5])
+    orc.run_steps(1)
+    p = orc.points()
+    assert np.array_equal(p["v"][0], vp)
+    assert np.allclose(p["x"][0], xp + vd * dt, rtol=0, atol=4e-18)
+    assert np.allclose(p["F"][0], np.eye(3).ravel(), rtol=0, atol=1e-15)
+    assert np.allclose(p["x"][1], xd + vd * dt, rtol=0, atol=4e-18)
+    # without the platen point the same point is accelerated by its stress
+    orc2 = O.Oracle(_one_scene(xp, v=vp, dt=dt))
+    orc2.a["stress"][0] = np.array([3.0e6, -1.0e6, 2.0e6, 4.0e5, -7.0e5, 9.0e5])
+    orc2.run_steps(1)
+    assert np.abs(orc2.points()["F"][0] - np.eye(3).ravel()).max() > 1e-9
+
+
+def _kernel_grid(O, pts, cells=(8, 8, 8), h=0.001):
+    sc = _one_scene(np.asarray(pts, float), cells=cells, h=h)
+    orc = O.Oracle(sc)
+    orc.step(sc.dt_max, 1)
+    return orc.a, np.array(sc.nodes)
+
+
+def test_kernel_density_special_values(O):
+    """Kernel density (…_MP_Contact.cpp:121-153), R = h, w(q) = 1 - 3q^2 + 2q^3:
+    a point on a node gives that node 1 and its face neighbours (q = 1)
+    nothing; a point midway along an edge gives both ends w(1/2) = 1/2; a point
+    at a cell centre gives each corner w(sqrt(3)/2); nodes beyond R get
+    nothing.  Sums over points add."""
+    h = 0.001
+    nv = (9, 9, 9)
+    lin = lambda i, j, k: i + nv[0] * (j + nv[1] * k)
+    a, _ = _kernel_grid(O, [[3 * h, 4 * h, 2 * h]])
+    assert a["gk"][lin(3, 4, 2)] == pytest.approx(1.0, abs=1e-12)
+    assert abs(a["gk"][lin(4, 4, 2)]) < 1e-12 and abs(a["gk"][lin(3, 5, 2)]) < 1e-12
+    assert np.count_nonzero(a["gk"] > 1e-12) == 1
+    a, _ = _kernel_grid(O, [[3.5 * h, 4 * h, 2 * h]])
+    assert a["gk"][lin(3, 4, 2)] == pytest.approx(0.5, rel=1e-12)
+    assert a["gk"][lin(4, 4, 2)] == pytest.approx(0.5, rel=1e-12)
+    assert np.count_nonzero(a["gk"] > 1e-12) == 2
+    q = math.sqrt(3) / 2
+    wc = 1 - 3 * q * q + 2 * q ** 3
+    a, _ = _kernel_grid(O, [[3.5 * h, 4.5 * h, 2.5 * h]])
+    corners = [lin(3 + i, 4 + j, 2 + k) for i in range(2) for j in range(2) for k in range(2)]
+    assert np.allclose(a["gk"][corners], wc, rtol=1e-12)
+    assert np.count_nonzero(a["gk"] > 1e-12) == 8
+    a, _ = _kernel_grid(O, [[3.5 * h, 4 * h, 2 * h], [3 * h, 4 * h, 2 * h]])
+    assert a["gk"][lin(3, 4, 2)] == pytest.approx(1.5, rel=1e-12)
+
+
+def test_kernel_gradient_central_difference_and_row_wrap(O):
+    """Kernel gradient (…_MP_Contact.cpp:159-191): nodes with k >= 1 take the
+    central difference of their neighbours; a constant field has zero
+    gradient (and |grad k| < 1/h is zeroed).  A field that rises linearly
+    along x -- points on the nodes of a line, weighted by how many sit on
+    each -- gives (k[i+1] - k[i-1]) / 2h exactly.  GridPoint_Factory::getIndex
+    range-checks only the linear index (GridPoint_Factory.h:27-31), so at
+    ix = 0 the x - 1 neighbour is the last node of the previous row: a point
+    placed there changes the gradient at (0, iy, iz)."""
+    h = 0.001
+    nv = (9, 9, 9)
+    lin = lambda i, j, k: i + nv[0] * (j + nv[1] * k)
+    # k = 1, 2, 3, 4, 5 at x = 2..6 on the line (y, z) = (4, 4)
+    pts = [[i * h, 4 * h, 4 * h] for i in range(2, 7) for _ in range(i - 1)]
+    a, _ = _kernel_grid(O, pts)
+    for i in range(3, 6):
+        assert a["gk"][lin(i, 4, 4)] == pytest.approx(i - 1, rel=1e-12)
+        assert a["gkgrad"][lin(i, 4, 4)][0] == pytest.approx(1.0 / h, rel=1e-12)
+        assert a["gkgrad"][lin(i, 4, 4)][1] == 0.0 and a["gkgrad"][lin(i, 4, 4)][2] == 0.0
+    # a constant field: one point on each node of a 3 x 3 x 3 patch; the centre
+    # node has k = 1 and equal neighbours
+    pts = [[i * h, j * h, k * h] for i in range(3, 6) for j in range(3, 6) for k in range(3, 6)]
+    a, _ = _kernel_grid(O, pts)
+    assert a["gk"][lin(4, 4, 4)] == pytest.approx(1.0, rel=1e-12)
+    assert np.array_equal(a["gkgrad"][lin(4, 4, 4)], np.zeros(3))
+    # the ix = 0 wrap
+    base = [[0.0, 4 * h, 4 * h], [0.0, 4 * h, 4 * h]]   # k = 2 at (0, 4, 4)
+    a0, _ = _kernel_grid(O, base)
+    g0 = a0["gkgrad"][lin(0, 4, 4)].copy()
+    far = [[8 * h * (1 - 1e-12), 3 * h, 4 * h]] * 4     # k ~ 4 at (8, 3, 4), the previous row's last node
+    a1, _ = _kernel_grid(O, base + far)
+    assert a1["gk"][lin(8, 3, 4)] > 3.9
+    g1 = a1["gkgrad"][lin(0, 4, 4)]
+    assert g1[0] == pytest.approx(g0[0] - a1["gk"][lin(8, 3, 4)] / (2 * h), rel=1e-9)
