@@ -95,7 +95,35 @@ def load(name):
     return sc, path, steps, sf, out
 
 
+def config1_scene():
+    """Config 1 as the benchmark states it: one 10 x 10 x 1 mm polygon cell,
+    0.8 mm walls, h = 0.125 mm, elastic 1 GPa, platen at 0.1 m/s."""
+    return scene.config1_world(platen="literal", steps=2000)
+
+
+CONFIG1_STRIDE = 97   # every 97th deformable point of the final state is kept
+
+
+def save_config1(steps=2000, threads=8):
+    """The config-1 output run through the oracle: the per-step platen
+    reaction (reportConsole's Force_y, PhysicsEngine.cpp:65-71) of all
+    ``steps`` steps, and the final positions, velocities and stresses of a
+    fixed subsample of the points.  The scene is rebuilt by config1_scene();
+    its size and coordinate sums are stored to detect a changed builder."""
+    sc = config1_scene()
+    orc = Oracle(sc, nthreads=threads)
+    orc.run_steps(steps, 0)
+    p = orc.points()
+    idx = np.nonzero(sc.dc == 0)[0][::CONFIG1_STRIDE]
+    np.savez_compressed(os.path.join(HERE, "config1_output.npz"), steps=np.int64(steps), idx=idx.astype(np.int64),
+                        n=np.int64(sc.n), x_sum=np.float64(sc.x.sum()), mass_sum=np.float64(sc.mass.sum()),
+                        reaction=np.asarray(orc.reaction), x=p["x"][idx], v=p["v"][idx], stress=p["stress"][idx],
+                        report=orc.report())
+
+
 if __name__ == "__main__":
     for name, (make, path, steps, sf) in CASES.items():
         save(name, make(), path, steps, sf)
         print("wrote", name)
+    save_config1()
+    print("wrote config1_output")
