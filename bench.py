@@ -158,6 +158,85 @@ def field_errors(m, sc, steps=100):
     return out
 
 
+def plastic_window(m, steps, pre=1500, sy_scale=0.06, speed=1.0):
+    """The config-2 scene with its yield stress scaled by ``sy_scale`` (250 MPa
+    -> 15 MPa; everything else unchanged), so that the J2 return map takes its
+    FP64 plastic branch inside the stable window of the loading (the 250 MPa
+    scene stays elastic until the reference update diverges, near step
+    4000).  ``pre`` untimed steps reach the yielding state, then ``steps``
+    steps are timed in FP32 and FP64.  Reports the fraction of points whose
+    plastic strain changed during the window and the FP32 fields against the
+    FP64 run from the same start."""
+    sc = build_scene(0.21, speed)
+    d = sc.dc == 0
+    sc.sy = sc.sy.copy()
+    sc.sy[d] *= sy_scale
+    out = {"workload": "config2_polygon_mesh_10x10_j2_plastic_window", "sy_pa": float(sc.sy[d][0]),
+           "pre_steps": pre, "steps": steps, "n_points": int(d.sum())}
+    state = {}
+    F = m._abi
+    for name, prec in (("fp32", m.FP32), ("fp64", m.FP64)):
+        with m.PhysicsEngine(sc, precision=prec, track_strain=False) as eng:
+            eng.run_steps(pre)
+            ep0 = eng.getMaterialPoints(F.MPM_F_EPSP)["eps_p"][d]
+            t0 = time.perf_counter()
+            eng.run_steps(steps)
+            t1 = time.perf_counter()
+            g = eng.getMaterialPoints(F.MPM_F_X | F.MPM_F_STRESS | F.MPM_F_EPSP | F.MPM_F_ALPHA)
+        changed = np.any(g["eps_p"][d] != ep0, axis=1)
+        out[name] = {"ms_per_step": round((t1 - t0) * 1e3 / steps, 5),
+                     "value": round(int(d.sum()) * steps / (t1 - t0), 1),
+                     "yield_fraction": round(float(changed.mean()), 4)}
+        state[name] = g
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64).ravel()
+        b = np.asarray(b, np.float64).ravel()
+        n = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a))
+    a, b = state["fp32"], state["fp64"]
+    out["fp32_vs_fp64"] = {"disp": rel((a["x"] - sc.x)[d], (b["x"] - sc.x)[d]), "stress": rel(a["stress"][d], b["stress"][d]),
+                           "eps_p": rel(a["eps_p"][d], b["eps_p"][d]), "alpha": rel(a["alpha"][d], b["alpha"][d])}
+    return out
+
+
+def config1_run(m, out_dir=None):
+    """Config 1 as stated: one 10 x 10 x 1 mm polygon cell, 0.8 mm walls,
+    h = 0.125 mm, elastic, platen at 0.1 m/s, 2000 FP64 steps.  Writes the
+    per-step platen force and the final positions and stresses (the config's
+    outputs) and compares them with the oracle's run frozen in
+    tests/golden/config1_output.npz."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_golden
+    sc = make_golden.config1_scene()
+    z = np.load(os.path.join(ROOT, "tests", "golden", "config1_output.npz"))
+    steps = int(z["steps"])
+    with m.PhysicsEngine(sc, precision=m.FP64, track_strain=False) as eng:
+        t0 = time.perf_counter()
+        eng.runSimulation_Classic_SinglePass_MP(steps * sc.dt_max)
+        t1 = time.perf_counter()
+        force = eng.reaction_history()
+        g = eng.getMaterialPoints(m._abi.MPM_F_X | m._abi.MPM_F_STRESS | m._abi.MPM_F_V)
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        np.savetxt(os.path.join(out_dir, "config1_platen_force.csv"),
+                   np.column_stack([np.arange(1, len(force) + 1) * sc.dt_max, force]), delimiter=",",
+                   header="time_s,force_y_N", comments="")
+        np.savez_compressed(os.path.join(out_dir, "config1_final_points.npz"), x=g["x"], stress=g["stress"],
+                            dc=sc.dc)
+    idx = z["idx"]
+
+    def rel(a, b):
+        return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+    return {"workload": "config1_polygon_cell_elastic_fp64", "n_points": sc.n_deformable, "steps": len(force),
+            "ms_per_step": round((t1 - t0) * 1e3 / len(force), 5), "dtype": "f64",
+            "value": round(sc.n_deformable * len(force) / (t1 - t0), 1), "unit": "particle-steps/s",
+            "final_platen_force_N": float(force[-1]),
+            "vs_oracle_fixture": {"force": rel(force, z["reaction"]), "disp": rel(g["x"][idx] - sc.x[idx], z["x"] - sc.x[idx]),
+                                  "stress": rel(g["stress"][idx], z["stress"])},
+            "outputs": out_dir}
+
+
 def batched_run(m, steps, warmup):
     """Config 4: 256 independent cells (SplitMix64 seed 20240607) packed into
     one grid, FP32, one launch per step for all cells; aggregate
@@ -271,10 +350,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="config2", choices=["config2", "config3", "config4", "config5"],
+    ap.add_argument("--workload", default="config2", choices=["config1", "config2", "config3", "config4", "config5"],
                     help="config4: the 256-cell batch (~8M points) in one launch per step; config5: the "
                          "differentiable run alone; each prints its own line")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the config-5 adjoint timing in the main line")
+    ap.add_argument("--no-plastic", action="store_true", help="skip the plastic-window leg in the main line")
+    ap.add_argument("--out-dir", default=None, help="config1: directory for the force history and final points")
     ap.add_argument("--tune", action="store_true",
                     help="pick brick size and batch length with the autotuner before timing")
     ap.add_argument("--counters", action="store_true",
@@ -286,6 +367,9 @@ def main():
         sys.exit(2)
     import code_mpm_auxetics_amd as m
 
+    if args.workload == "config1":
+        print(json.dumps(config1_run(m, args.out_dir)), flush=True)
+        return
     if args.workload == "config4":
         batched_run(m, args.steps, args.warmup)
         return
@@ -344,6 +428,8 @@ def main():
         "active_grid_nodes": int(nodes),
         "roofline": roof,
     }
+    if not args.no_plastic:
+        out["plastic_window"] = plastic_window(m, args.steps)
     if not args.no_adjoint:
         c5 = config5_run()
         out["adjoint_over_forward"] = c5["adjoint_over_forward"]

@@ -327,6 +327,41 @@ __global__ void __launch_bounds__(256) k_step_sums(const double *gpart, int occ_
     if (t == 0) hist[blockIdx.x] = s[0];
 }
 
+// The per-step reactions of a batch and the last step's grid totals in one
+// launch: blocks 0..nsteps-1 are k_step_sums, block nsteps is k_sum3 of the
+// last step (the same fixed trees, so the same bits).
+__global__ void __launch_bounds__(256) k_batch_sums(const double *gpart, int occ_cap, const int32_t *n_occ,
+                                                   int nsteps, double *hist, double *grid3)
+{
+    __shared__ double s[3][256];
+    const int t = threadIdx.x, no = *n_occ;
+    if ((int)blockIdx.x < nsteps) {
+        const double *gp = gpart + (size_t)blockIdx.x * occ_cap * 3;
+        double r = 0.0;
+        for (int i = t; i < no; i += 256) r += gp[3 * i + 1];
+        s[0][t] = r;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) s[0][t] += s[0][t + w];
+            __syncthreads();
+        }
+        if (t == 0) hist[blockIdx.x] = s[0][0];
+        return;
+    }
+    const double *part = gpart + (size_t)(nsteps - 1) * occ_cap * 3;
+    double r[3] = {0, 0, 0};
+    for (int i = t; i < no; i += 256)
+        for (int k = 0; k < 3; ++k) r[k] += part[3 * i + k];
+    for (int k = 0; k < 3; ++k) s[k][t] = r[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 3; ++k) s[k][t] += s[k][t + w];
+        __syncthreads();
+    }
+    if (t < 3) grid3[t] = s[t][0];
+}
+
 // Per-group reaction of a batch: block (step k, group g) sums the group's
 // y-fixed slots [off[g], off[g+1]) of step k in a fixed tree.
 __global__ void __launch_bounds__(256) k_group_sums(const double *fy_val, int n_fy, const int32_t *off, int n_groups,

@@ -55,13 +55,17 @@ struct GridParams {
     double ih[3];  // 1/h in FP64 (FP32-state contact arithmetic multiplies by it)
 };
 
-// Per-step record filled on the host (time, dt, platen velocity, epoch).
+// Per-step record filled on the host (time, dt, platen velocity, epoch, and
+// the ping-pong parity the step reads: a step reads buffers [par] and writes
+// [par ^ 1], so one captured batch serves either starting parity).
 struct StepRec {
     double dt;
     double t;
     double vtl[3];
     int32_t has_tl;
     uint32_t epoch;
+    int32_t par;
+    int32_t pad;
 };
 
 // Material table entry (reference per-point fields E, nu, sigma_y, C0, C1 and

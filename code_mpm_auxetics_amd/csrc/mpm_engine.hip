@@ -65,6 +65,7 @@ struct mpm_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t bev[2] = {nullptr, nullptr};   // end of the batch using host buffer set 0 / 1
     std::string err;
     GridParams g{};
     bool fp32 = false;
@@ -114,20 +115,28 @@ struct mpm_handle {
     int n_mats = 0;
     int has_pointE = 0, has_fext = 0;
 
+    // pinned host buffers in two sets (steps_cap entries each), one per batch
+    // in flight: step records, reaction history, flags, group history
     StepRec *steps_d = nullptr, *steps_h = nullptr;
     int steps_cap = 0;
+    // results of a batch in one device block (out_d): the flags word, the
+    // per-step reaction history (steps_cap), the per-group history
+    // (steps_cap x n_groups); flags_d, hist_d and ghist_d point into it, and
+    // one copy brings a batch's results to out_h (pinned, two sets)
+    double *out_d = nullptr, *out_h = nullptr;
+    size_t out_stride = 0;                  // doubles per set
     int32_t *flags_d = nullptr, *flags_h = nullptr;
     double *kp_part = nullptr;
     double *gpart = nullptr;                // [steps_cap][occ_cap][3] grid-update partials
     size_t kp_cap = 0;
-    double *hist_d = nullptr, *hist_h = nullptr;
+    double *hist_d = nullptr;
     std::vector<double> hist;
     double *checks_d = nullptr;
     double *grid3_d = nullptr;              // last grid update: mass, reaction, active nodes
     // batched worlds: reaction per group (world)
     int n_groups = 0, n_fy = 0;
     int32_t *fy_slot = nullptr, *fy_off = nullptr;   // node -> y-fixed slot; CSR offsets per group
-    double *fy_val = nullptr, *ghist_d = nullptr, *ghist_h = nullptr;
+    double *fy_val = nullptr, *ghist_d = nullptr;
     std::vector<double> ghist;              // [step][group] of the last mpm_step call
 
     // contact step (allocated at its first use)
@@ -144,8 +153,8 @@ struct mpm_handle {
         bool kernel_valid = false;          // kp/kg hold the last contact step's point kernel
     } ct;
     int last_path = 0;                      // path of the last step (checks read its partials)
-    // hipGraphs of batches of plain steps, keyed by (steps, starting parity)
-    // and valid for one brick layout (bumped at every rebin / reallocation)
+    // hipGraphs of batches of plain steps, keyed by their step count and
+    // valid for one brick layout (bumped at every rebin / reallocation)
     struct GraphEntry {
         int steps, parity;
         hipGraph_t graph;
@@ -520,16 +529,17 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     KpArgs<real> a{};
     a.g = h->g;
     a.p = parr<real>(h, h->cur);
-    const int p = h->parity;
-    // a step reads parity p and writes p^1; a rebuild (P2G only) writes p
-    a.arena_in = (const real *)h->arena[p];
-    a.binfo_in = h->binfo[p];
-    a.arena_out = (real *)h->arena[g2p ? (p ^ 1) : p];
-    a.binfo_out = h->binfo[g2p ? (p ^ 1) : p];
+    // a step reads the parity of its step record and writes the other one;
+    // a rebuild (P2G only) writes the current parity
+    for (int b = 0; b < 2; ++b) {
+        a.arena[b] = (real *)h->arena[b];
+        a.binfo[b] = h->binfo[b];
+        a.marks[b] = h->marks[b];
+        a.dc_v[b] = h->dc_v[b];
+    }
+    a.rebuild_par = h->parity;
     a.zero_off = (uint32_t)(h->arena_cap_slots * 8 * h->g.arenaN);
     a.fixed = h->fixed;
-    a.marks_in = h->marks[p];
-    a.marks_out = h->marks[p ^ 1];
     a.occ_list = h->occ_list;
     a.occ_start = h->occ_start;
     a.occ_count = h->occ_count;
@@ -544,8 +554,6 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.gpart = h->gpart + (size_t)step * h->occ_cap * 3;
     a.n_dc = h->n_dc;
     a.dc_x = h->dc_x;
-    a.dc_v_in = h->dc_v[p];
-    a.dc_v_out = h->dc_v[p ^ 1];
     a.fy_slot = (g2p && h->n_fy > 0) ? h->fy_slot : nullptr;
     a.fy_val = h->fy_val ? h->fy_val + (size_t)step * h->n_fy : nullptr;
     const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long) + (g2p ? (size_t)6 * h->g.arenaN * h->rsize : 0);
@@ -789,20 +797,19 @@ static int group_pre(mpm_handle *h, int nsteps)
     HIPCHK(h, hipGetLastError());
     return MPM_OK;
 }
-// ... and reduce them per (step, group) in a fixed order, then copy back
+// ... and reduce them per (step, group) in a fixed order into ghist_d (the
+// caller copies them back)
 static int group_post(mpm_handle *h, int nsteps)
 {
     if (h->n_groups == 0) return MPM_OK;
     if (h->n_fy > 0) {
         k_group_sums<<<(unsigned)(nsteps * h->n_groups), 256, 0, h->stream>>>(h->fy_val, h->n_fy, h->fy_off, h->n_groups,
                                                                             h->ghist_d);
-        HIPCHK(h, hipGetLastError());
     } else {
         const size_t n = (size_t)nsteps * h->n_groups;
         k_fill_f64<<<cdiv(n, 256), 256, 0, h->stream>>>(h->ghist_d, n, 0.0);
     }
-    HIPCHK(h, hipMemcpyAsync(h->ghist_h, h->ghist_d, sizeof(double) * nsteps * h->n_groups, hipMemcpyDeviceToHost,
-                             h->stream));
+    HIPCHK(h, hipGetLastError());
     return MPM_OK;
 }
 
@@ -815,57 +822,97 @@ static void drop_graphs(mpm_handle *h)
     h->graphs.clear();
 }
 
-// A batch of plain steps as one hipGraph: step records in, the fused step
-// kernels, the per-step reaction sums, history and flags back.  Captured
-// for a (steps, starting parity, brick layout) and replayed while those hold;
-// the host fills steps_h (pinned) before every replay.  No memset nodes.
+// The kernels of a batch of nsteps plain steps, starting at step record 0:
+// the fused step kernels, the per-step reaction sums, the last step's grid
+// totals and the per-group reactions.  Parities come from the step records,
+// so the same sequence serves every starting parity.
 template <typename real>
-static int graph_batch(mpm_handle *h, int nsteps)
+static int batch_kernels(mpm_handle *h, int nsteps, std::vector<ProfMark> *marks)
+{
+    RCHK(group_pre(h, nsteps));
+    for (int k = 0; k < nsteps; ++k) {
+        const int pe = marks ? prof_begin(h, *marks, 0) : -1;
+        RCHK(launch_particle<real>(h, true, k));
+        prof_end(h, pe);
+    }
+    k_batch_sums<<<(unsigned)nsteps + 1, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, nsteps, h->hist_d,
+                                                              h->grid3_d);
+    HIPCHK(h, hipGetLastError());
+    RCHK(group_post(h, nsteps));
+    return MPM_OK;
+}
+
+// Bytes of a batch's result block that the host reads: the flags word, the
+// reaction history and (batched worlds) the per-group history.
+static size_t out_bytes(const mpm_handle *h, int nsteps)
+{
+    return sizeof(double) * (h->n_groups > 0 ? 1 + (size_t)h->steps_cap + (size_t)nsteps * h->n_groups
+                                             : 1 + (size_t)nsteps);
+}
+
+// A whole batch: step records in from host set `slot`, the kernels, the
+// result block out to host set `slot`.
+template <typename real>
+static int batch_ops(mpm_handle *h, int nsteps, int slot, std::vector<ProfMark> *marks)
+{
+    HIPCHK(h, hipMemcpyAsync(h->steps_d, h->steps_h + (size_t)slot * h->steps_cap, sizeof(StepRec) * nsteps,
+                             hipMemcpyHostToDevice, h->stream));
+    RCHK(batch_kernels<real>(h, nsteps, marks));
+    HIPCHK(h, hipMemcpyAsync(h->out_h + (size_t)slot * h->out_stride, h->out_d, out_bytes(h, nsteps),
+                             hipMemcpyDeviceToHost, h->stream));
+    return MPM_OK;
+}
+
+// The hipGraph of batch_ops(nsteps, slot) for the current brick layout,
+// captured on first use.  No memset nodes.  upload and every plain-path rebin
+// capture the full batch length for both host sets at once, so that no
+// capture falls inside a run of steps.
+template <typename real>
+static int batch_graph(mpm_handle *h, int nsteps, int slot, hipGraphExec_t *out)
 {
     if (h->graph_layout != h->layout) {
         drop_graphs(h);
         h->graph_layout = h->layout;
     }
-    hipGraphExec_t exec = nullptr;
     for (const auto &e : h->graphs)
-        if (e.steps == nsteps && e.parity == h->parity) exec = e.exec;
-    if (!exec) {
-        const int p0 = h->parity;
-        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        int rc = MPM_OK;
-        if (hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * nsteps, hipMemcpyHostToDevice, h->stream) != hipSuccess)
-            rc = MPM_E_HIP;
-        if (rc == MPM_OK && group_pre(h, nsteps) != MPM_OK) rc = MPM_E_HIP;
-        for (int k = 0; k < nsteps && rc == MPM_OK; ++k) {
-            rc = launch_particle<real>(h, true, k);
-            h->parity ^= 1;
-        }
-        if (rc == MPM_OK) {
-            k_step_sums<<<(unsigned)nsteps, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, h->hist_d);
-            k_sum3<<<1, 256, 0, h->stream>>>(h->gpart + (size_t)(nsteps - 1) * h->occ_cap * 3, h->n_occ_d, 0, h->grid3_d);
-            if (group_post(h, nsteps) != MPM_OK) rc = MPM_E_HIP;
-            if (hipGetLastError() != hipSuccess ||
-                hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * nsteps, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-                hipMemcpyAsync(h->flags_h, h->flags_d, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
-                rc = MPM_E_HIP;
-        }
-        h->parity = p0;
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(h->stream, &g);
-        if (rc != MPM_OK || ec != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            h->err = "hipGraph capture failed";
-            return MPM_E_HIP;
-        }
-        if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) {
-            (void)hipGraphDestroy(g);
-            h->err = "hipGraphInstantiate failed";
-            return MPM_E_HIP;
-        }
-        if (h->graphs.size() >= 4) drop_graphs(h);
-        h->graphs.push_back({nsteps, p0, g, exec});
+        if (e.steps == nsteps && e.parity == slot) { *out = e.exec; return MPM_OK; }
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_ops<real>(h, nsteps, slot, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    if (rc != MPM_OK || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        h->err = "hipGraph capture failed";
+        return MPM_E_HIP;
     }
-    HIPCHK(h, hipGraphLaunch(exec, h->stream));
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        h->err = "hipGraphInstantiate failed";
+        return MPM_E_HIP;
+    }
+    if (h->graphs.size() >= 4) drop_graphs(h);
+    h->graphs.push_back({nsteps, slot, g, exec});
+    *out = exec;
+    return MPM_OK;
+}
+
+// steps of a full plain-path batch (the drift check interval)
+static int batch_len(const mpm_handle *h)
+{
+    return std::max(1, std::min(h->cfg.drift_check > 0 ? h->cfg.drift_check : 8, h->steps_cap));
+}
+
+static bool graphs_on(const mpm_handle *h) { return h->cfg.launch == MPM_LAUNCH_GRAPH && !h->cfg.profile; }
+
+// capture the full-length batch for the current layout now (outside any run
+// of steps)
+static int prepare_graph(mpm_handle *h)
+{
+    if (!graphs_on(h) || h->n_occ == 0) return MPM_OK;
+    hipGraphExec_t e = nullptr;
+    for (int slot = 0; slot < 2; ++slot)
+        RCHK(h->fp32 ? batch_graph<float>(h, batch_len(h), slot, &e) : batch_graph<double>(h, batch_len(h), slot, &e));
     return MPM_OK;
 }
 
@@ -885,89 +932,139 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
             HIPCHK(h, hipGetLastError());
         }
         h->plain_stale = false;
+        RCHK(prepare_graph(h));
     }
     h->hist.clear();
     h->ghist.clear();
     for (double &v : h->kt) v = 0.0;
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-    double acc = 0.0;
-    int in_batch = 0;
-    auto flush = [&](bool force_check) -> int {
-        if (in_batch == 0) return MPM_OK;
-        // launch the batch: steps records already in steps_h[0..in_batch)
-        std::vector<ProfMark> marks;
-        bool graphed = false;   // the graph already copied history and flags back
-        if (path == MPM_PATH_CONTACT || !(h->cfg.launch == MPM_LAUNCH_GRAPH && !h->cfg.profile && h->n_occ > 0))
-            HIPCHK(h, hipMemcpyAsync(h->steps_d, h->steps_h, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
-        if (path == MPM_PATH_CONTACT) {
-            for (int k = 0; k < in_batch; ++k) RCHK(launch_contact<real>(h, k));
-            {
-                CArgs<real> ca = contact_args<real>(h);
-                if (ca.n_all > 0)
-                    k_c_kernel_out<real><<<cdiv(ca.n_all, 256), 256, 0, h->stream>>>(ca, h->ct.kp_id, h->ct.kg_id, h->n_total);
+    // Plain-path batches are pipelined: batch i+1 is enqueued before the
+    // host reads batch i's flags, so the device never waits for the host
+    // between batches.  Step records, history and flags of the two batches
+    // use two sets of pinned host buffers (slot), and a rebin that batch i
+    // asks for runs after batch i+1.  The contact path and profiling runs
+    // consume every batch before the next.
+    const bool pipeline = path == MPM_PATH_PLAIN && !h->cfg.profile;
+    const int K = batch_len(h);
+    const size_t cap = (size_t)h->steps_cap;
+    struct Batch {
+        bool on = false;
+        int n = 0, slot = 0;
+    };
+    Batch pend;   // enqueued, results not yet read
+    int slot = 0, in_batch = 0;
+    std::vector<ProfMark> marks;
+    auto recs = [&](int sl) { return h->steps_h + (size_t)sl * cap; };
+
+    // enqueue the in_batch steps whose records are in recs(slot)
+    auto launch = [&]() -> int {
+        StepRec *r = recs(slot);
+        for (int k = 0; k < in_batch; ++k) r[k].par = (h->parity + k) & 1;
+        if (path == MPM_PATH_PLAIN && h->n_occ > 0) {
+            if (graphs_on(h) && in_batch == K) {
+                hipGraphExec_t exec = nullptr;
+                RCHK(batch_graph<real>(h, in_batch, slot, &exec));
+                HIPCHK(h, hipGraphLaunch(exec, h->stream));
+            } else {
+                RCHK(batch_ops<real>(h, in_batch, slot, h->cfg.profile ? &marks : nullptr));
             }
+            h->last_path = 0;
+        } else if (path == MPM_PATH_CONTACT) {
+            HIPCHK(h, hipMemcpyAsync(h->steps_d, r, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
+            for (int k = 0; k < in_batch; ++k) RCHK(launch_contact<real>(h, k));
+            CArgs<real> ca = contact_args<real>(h);
+            if (ca.n_all > 0)
+                k_c_kernel_out<real><<<cdiv(ca.n_all, 256), 256, 0, h->stream>>>(ca, h->ct.kp_id, h->ct.kg_id, h->n_total);
             // per-step platen reaction of the batch
             k_c_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->ct.cpart, h->ct.nblk_n, h->hist_d);
             k_sum3<<<1, 256, 0, h->stream>>>(h->ct.cpart + (size_t)(in_batch - 1) * h->ct.nblk_n * 3, nullptr,
                                             h->ct.nblk_n, h->grid3_d);
             HIPCHK(h, hipGetLastError());
+            if (h->n_groups > 0)
+                k_fill_f64<<<cdiv((size_t)in_batch * h->n_groups, 256), 256, 0, h->stream>>>(
+                    h->ghist_d, (size_t)in_batch * h->n_groups, 0.0);
+            HIPCHK(h, hipMemcpyAsync(h->out_h + (size_t)slot * h->out_stride, h->out_d, out_bytes(h, in_batch),
+                                     hipMemcpyDeviceToHost, h->stream));
             h->plain_stale = true;
             h->last_path = 1;
-        } else if (h->cfg.launch == MPM_LAUNCH_GRAPH && !h->cfg.profile && h->n_occ > 0) {
-            RCHK(graph_batch<real>(h, in_batch));
-            h->parity ^= (in_batch & 1);
-            h->last_path = 0;
-            graphed = true;
         } else {
-            RCHK(group_pre(h, in_batch));
-            for (int k = 0; k < in_batch; ++k) {
-                const int pe = prof_begin(h, marks, 0);
-                RCHK(launch_particle<real>(h, true, k));
-                prof_end(h, pe);
-                h->parity ^= 1;
-            }
+            // no deformable points: platen blocks only, no sums
+            HIPCHK(h, hipMemcpyAsync(h->steps_d, r, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
+            for (int k = 0; k < in_batch; ++k) RCHK(launch_particle<real>(h, true, k));
+            k_fill_f64<<<cdiv(in_batch, 256), 256, 0, h->stream>>>(h->hist_d, (size_t)in_batch, 0.0);
+            k_fill_f64<<<1, 256, 0, h->stream>>>(h->grid3_d, 3, 0.0);
+            HIPCHK(h, hipGetLastError());
+            if (h->n_groups > 0)
+                k_fill_f64<<<cdiv((size_t)in_batch * h->n_groups, 256), 256, 0, h->stream>>>(
+                    h->ghist_d, (size_t)in_batch * h->n_groups, 0.0);
+            HIPCHK(h, hipMemcpyAsync(h->out_h + (size_t)slot * h->out_stride, h->out_d, out_bytes(h, in_batch),
+                                     hipMemcpyDeviceToHost, h->stream));
             h->last_path = 0;
-            if (h->n_occ > 0) {   // per-step y reaction of the batch, last step's grid totals
-                k_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, h->hist_d);
-                k_sum3<<<1, 256, 0, h->stream>>>(h->gpart + (size_t)(in_batch - 1) * h->occ_cap * 3, h->n_occ_d, 0,
-                                                h->grid3_d);
-                HIPCHK(h, hipGetLastError());
-                RCHK(group_post(h, in_batch));
-            } else {
-                HIPCHK(h, hipMemsetAsync(h->hist_d, 0, sizeof(double) * in_batch, h->stream));
-                HIPCHK(h, hipMemsetAsync(h->grid3_d, 0, 24, h->stream));
-            }
         }
-        if (!graphed)
-            HIPCHK(h, hipMemcpyAsync(h->hist_h, h->hist_d, sizeof(double) * in_batch, hipMemcpyDeviceToHost, h->stream));
-        bool need = false;
-        RCHK(check_flags(h, &need, graphed));   // synchronises
-        for (int k = 0; k < in_batch; ++k) h->hist.push_back(h->hist_h[k]);
-        if (h->n_groups > 0) {
-            const bool have = path != MPM_PATH_CONTACT && h->n_occ > 0;
-            for (int k = 0; k < in_batch; ++k)
-                for (int gr = 0; gr < h->n_groups; ++gr)
-                    h->ghist.push_back(have ? h->ghist_h[(size_t)k * h->n_groups + gr] : 0.0);
-        }
-        if (h->cfg.profile) prof_collect(h, marks);
+        HIPCHK(h, hipEventRecord(h->bev[slot], h->stream));
+        h->parity ^= (in_batch & 1);
+        pend.on = true;
+        pend.n = in_batch;
+        pend.slot = slot;
+        slot ^= 1;
         in_batch = 0;
+        return MPM_OK;
+    };
+    // wait for a launched batch and read its results; *need is set when it
+    // asks for a rebin
+    auto collect = [&](Batch &b, bool *need) -> int {
+        if (!b.on) return MPM_OK;
+        b.on = false;
+        HIPCHK(h, hipEventSynchronize(h->bev[b.slot]));
+        const double *o = h->out_h + (size_t)b.slot * h->out_stride;
+        const int f = *reinterpret_cast<const int32_t *>(o);
+        for (int k = 0; k < b.n; ++k) h->hist.push_back(o[1 + k]);
+        if (h->n_groups > 0) {
+            const double *gh = o + 1 + cap;
+            for (int k = 0; k < b.n; ++k)
+                for (int gr = 0; gr < h->n_groups; ++gr) h->ghist.push_back(gh[(size_t)k * h->n_groups + gr]);
+        }
+        if (f & (FLAG_OUT_OF_GRID | FLAG_ARENA | FLAG_NAN)) {
+            (void)hipStreamSynchronize(h->stream);
+            if (f & FLAG_NAN) { h->err = "non-finite point state"; return MPM_E_NAN; }
+            if (f & FLAG_OUT_OF_GRID) { h->err = "a material point left the grid box"; return MPM_E_OUT_OF_GRID; }
+            h->err = "a point moved more than one cell beyond its brick between drift checks "
+                     "(lower mpm_config.drift_check)";
+            return MPM_E_ARENA;
+        }
+        *need = *need || (f & FLAG_REBIN) != 0;
+        return MPM_OK;
+    };
+    // launch the steps gathered so far; read the previous batch (or, with
+    // drain, every batch); rebin when asked
+    auto flush = [&](bool drain) -> int {
+        Batch prev = pend;
+        pend.on = false;
+        if (in_batch > 0) RCHK(launch());
+        else pend = prev, prev.on = false;
+        bool need = false;
+        RCHK(collect(prev, &need));
+        if (need || drain || !pipeline) RCHK(collect(pend, &need));
+        if (h->cfg.profile) prof_collect(h, marks);
         if (need) {
-            int pe = prof_begin(h, marks, 2);
+            HIPCHK(h, hipMemsetAsync(h->flags_d, 0, 4, h->stream));
+            const int pe = prof_begin(h, marks, 2);
             RCHK(rebin(h, false));
             prof_end(h, pe);
             HIPCHK(h, hipStreamSynchronize(h->stream));
             if (h->cfg.profile) prof_collect(h, marks);
             if (path == MPM_PATH_CONTACT) RCHK(locate_point0<real>(h));
+            else RCHK(prepare_graph(h));
         }
-        (void)force_check;
         return MPM_OK;
     };
-    const int K = std::max(1, std::min(h->cfg.drift_check > 0 ? h->cfg.drift_check : 8, h->steps_cap));
+    int rc = MPM_OK;
+    double acc = 0.0;
     while (acc < dt_total) {
         double dt = h->cfg.dt_max;
         if (h->cfg.dt_max > (dt_total - acc)) dt = dt_total - acc;
         acc += dt;
-        StepRec &r = h->steps_h[in_batch];
+        StepRec &r = recs(slot)[in_batch];
         r.dt = dt;
         r.t = h->time;
         r.has_tl = h->tl_t.empty() ? 0 : 1;
@@ -981,7 +1078,10 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
             h->console_last = h->time;
             report = false;
         }
-        if (in_batch == K || report) RCHK(flush(true));
+        if (in_batch == K || report) {
+            rc = flush(report);
+            if (rc != MPM_OK) break;
+        }
         if (report) {
             h->console_last = h->time;
             float ms_now = 0.f;
@@ -993,7 +1093,11 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
         h->time += dt;
         h->cycle++;
     }
-    RCHK(flush(true));
+    if (rc == MPM_OK) rc = flush(true);
+    if (rc != MPM_OK) {
+        (void)hipStreamSynchronize(h->stream);
+        return rc;
+    }
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     HIPCHK(h, hipEventSynchronize(h->ev1));
     float ms = 0.f;
@@ -1078,8 +1182,8 @@ static void destroy_all(mpm_handle *h)
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d,
                      (void **)&h->slot_order, (void **)&h->slot_mat, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
                      (void **)&h->steps_d,
-                     (void **)&h->flags_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->hist_d, (void **)&h->grid3_d,
-                     (void **)&h->fy_slot, (void **)&h->fy_off, (void **)&h->fy_val, (void **)&h->ghist_d,
+                     (void **)&h->out_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->grid3_d,
+                     (void **)&h->fy_slot, (void **)&h->fy_off, (void **)&h->fy_val,
                      (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
                      (void **)&h->headpos, (void **)&h->gdesc};
     for (void **p : ptrs) dev_free(*p);
@@ -1087,16 +1191,18 @@ static void destroy_all(mpm_handle *h)
     h->scan_scratch.clear();
     if (h->steps_h) (void)hipHostFree(h->steps_h);
     if (h->flags_h) (void)hipHostFree(h->flags_h);
-    if (h->hist_h) (void)hipHostFree(h->hist_h);
-    if (h->ghist_h) (void)hipHostFree(h->ghist_h);
-    h->ghist_h = nullptr;
+    if (h->out_h) (void)hipHostFree(h->out_h);
+    h->out_h = nullptr;
+    h->flags_d = nullptr;
+    h->hist_d = h->ghist_d = nullptr;
     h->steps_h = nullptr;
     h->flags_h = nullptr;
-    h->hist_h = nullptr;
     for (hipEvent_t e : h->pev) (void)hipEventDestroy(e);
     h->pev.clear();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    for (hipEvent_t &e : h->bev)
+        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -1155,7 +1261,12 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     };
     if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return fail(MPM_E_HIP); }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { h->err = "stream"; return fail(MPM_E_HIP); }
-    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) { h->err = "event"; return fail(MPM_E_HIP); }
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&h->bev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->bev[1], hipEventDisableTiming) != hipSuccess) {
+        h->err = "event";
+        return fail(MPM_E_HIP);
+    }
 
     GridParams &g = h->g;
     for (int a = 0; a < 3; ++a) {
@@ -1229,16 +1340,14 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
         (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->slot_order, nb)) || (rc = dalloc(h, &h->slot_mat, nb)) ||
         (rc = dalloc(h, &h->binfo[0], nb)) || (rc = dalloc(h, &h->binfo[1], nb)) ||
-        (rc = dalloc(h, &h->flags_d, 1)) || (rc = dalloc(h, &h->checks_d, 16)) || (rc = dalloc(h, &h->grid3_d, 3)))
+        (rc = dalloc(h, &h->checks_d, 16)) || (rc = dalloc(h, &h->grid3_d, 3)))
         return fail(rc);
     (void)hipMemsetAsync(h->n_occ_d, 0, 4, h->stream);
-    (void)hipMemsetAsync(h->flags_d, 0, 4, h->stream);
     (void)hipMemsetAsync(h->grid3_d, 0, 24, h->stream);
     h->steps_cap = 64;
-    if ((rc = dalloc(h, &h->steps_d, h->steps_cap)) || (rc = dalloc(h, &h->hist_d, h->steps_cap))) return fail(rc);
-    if (hipHostMalloc((void **)&h->steps_h, sizeof(StepRec) * h->steps_cap, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&h->flags_h, 4, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&h->hist_h, sizeof(double) * h->steps_cap, hipHostMallocDefault) != hipSuccess) {
+    if ((rc = dalloc(h, &h->steps_d, h->steps_cap))) return fail(rc);
+    if (hipHostMalloc((void **)&h->steps_h, 2 * sizeof(StepRec) * h->steps_cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&h->flags_h, 16, hipHostMallocDefault) != hipSuccess) {
         h->err = "hipHostMalloc";
         return fail(MPM_E_NOMEM);
     }
@@ -1257,16 +1366,25 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         h->n_groups = ng;
         h->n_fy = cnt[ng];
         if ((rc = dalloc(h, &h->fy_slot, (size_t)h->nnodes)) || (rc = dalloc(h, &h->fy_off, (size_t)ng + 1)) ||
-            (rc = dalloc(h, &h->fy_val, (size_t)h->steps_cap * std::max(h->n_fy, 1))) ||
-            (rc = dalloc(h, &h->ghist_d, (size_t)h->steps_cap * ng)))
+            (rc = dalloc(h, &h->fy_val, (size_t)h->steps_cap * std::max(h->n_fy, 1))))
             return fail(rc);
         if (hipMemcpy(h->fy_slot, slot.data(), sizeof(int32_t) * h->nnodes, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(h->fy_off, cnt.data(), sizeof(int32_t) * (ng + 1), hipMemcpyHostToDevice) != hipSuccess ||
-            hipHostMalloc((void **)&h->ghist_h, sizeof(double) * h->steps_cap * ng, hipHostMallocDefault) != hipSuccess) {
+            hipMemcpy(h->fy_off, cnt.data(), sizeof(int32_t) * (ng + 1), hipMemcpyHostToDevice) != hipSuccess) {
             h->err = "group tables";
             return fail(MPM_E_HIP);
         }
     }
+    // the result block of a batch and its two host sets
+    h->out_stride = 1 + (size_t)h->steps_cap * (1 + (size_t)h->n_groups);
+    if ((rc = dalloc(h, &h->out_d, h->out_stride))) return fail(rc);
+    if (hipHostMalloc((void **)&h->out_h, 2 * sizeof(double) * h->out_stride, hipHostMallocDefault) != hipSuccess) {
+        h->err = "hipHostMalloc";
+        return fail(MPM_E_NOMEM);
+    }
+    h->flags_d = reinterpret_cast<int32_t *>(h->out_d);
+    h->hist_d = h->out_d + 1;
+    h->ghist_d = h->n_groups > 0 ? h->out_d + 1 + h->steps_cap : nullptr;
+    (void)hipMemsetAsync(h->out_d, 0, sizeof(double) * h->out_stride, h->stream);
     if (hipStreamSynchronize(h->stream) != hipSuccess) { h->err = "init sync"; return fail(MPM_E_HIP); }
     *out = h;
     return MPM_OK;
@@ -1448,6 +1566,8 @@ static int upload_t(mpm_handle *h, const mpm_points *pts)
     HIPCHK(h, hipMemsetAsync(h->flags_d, 0, 4, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->rebins = 0;
+    RCHK(prepare_graph(h));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return MPM_OK;
 }
 

@@ -296,14 +296,16 @@ template <typename real>
 struct KpArgs {
     GridParams g;
     PArrays<real> p;
-    const real *arena_in;       // step n: P2G sums per slot (box-compact 8-real records)
-    const int4 *binfo_in;       // step n: per brick {slot or -1, packed touched box}
-    real *arena_out;            // step n+1
-    int4 *binfo_out;            // step n+1
-    uint32_t zero_off;          // element offset in arena_in of a record of 8 zeros (after the last slot)
+    // ping-pong pairs, indexed by parity: a step reads [par] (step n) and
+    // writes [par ^ 1] (step n+1), par from its step record; the arena
+    // rebuild (G2P = false) writes [rebuild_par]
+    real *arena[2];             // P2G sums per slot (box-compact 8-real records)
+    int4 *binfo[2];             // per brick {slot or -1, packed touched box}
+    unsigned long long *marks[2];         // platen override tags
+    double *dc_v[2];                      // platen velocities
+    int32_t rebuild_par;
+    uint32_t zero_off;          // element offset in an arena of a record of 8 zeros (after the last slot)
     const uint8_t *fixed;       // node fixity bits
-    const unsigned long long *marks_in;   // platen override tags of step n
-    unsigned long long *marks_out;        // ... of step n+1
     const uint32_t *occ_list, *occ_start, *occ_count;
     const uint32_t *slot_order;       // brick block b works on slot slot_order[b]
     const uint8_t *slot_mat;          // per slot: the material of all its points, or 0xFF if mixed
@@ -318,8 +320,6 @@ struct KpArgs {
     const int32_t *fy_slot;             // batched worlds: node -> y-fixed slot or -1 (null: off)
     double *fy_val;                     // this step's reaction per y-fixed slot
     double *dc_x;                       // 3 per platen point, FP64
-    const double *dc_v_in;              // platen velocities of step n
-    double *dc_v_out;                   // ... after this step
 };
 
 // Strain increment, total strain, constitutive update and stress write of
@@ -501,6 +501,16 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     const StepRec st = a.steps[step];
     const double dtd = st.dt;
     const real dt = (real)dtd;
+    // this step's ping-pong parity (a rebuild writes the current parity)
+    const int par = G2P ? (st.par & 1) : (a.rebuild_par & 1);
+    const real *const arena_in = a.arena[par];
+    const int4 *const binfo_in = a.binfo[par];
+    real *const arena_out = a.arena[G2P ? par ^ 1 : par];
+    int4 *const binfo_out = a.binfo[G2P ? par ^ 1 : par];
+    const unsigned long long *const marks_in = a.marks[par];
+    unsigned long long *const marks_out = a.marks[par ^ 1];
+    const double *const dc_v_in = a.dc_v[par];
+    double *const dc_v_out = a.dc_v[par ^ 1];
 
     // ---- platen (displacement-controlled) blocks: phase 8 + override marks
     if ((int)blockIdx.x >= a.occ_cap) {
@@ -510,8 +520,8 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         double *x = &a.dc_x[3 * j], v[3];
         // …_MP.cpp:360-368: velocity from the TimeLine at the step's start time
         for (int q = 0; q < 3; ++q) {
-            v[q] = st.has_tl ? st.vtl[q] : a.dc_v_in[3 * j + q];
-            a.dc_v_out[3 * j + q] = v[q];
+            v[q] = st.has_tl ? st.vtl[q] : dc_v_in[3 * j + q];
+            dc_v_out[3 * j + q] = v[q];
             x[q] += v[q] * dtd;
         }
         // override marks for step n+1 (…_MP.cpp:245-265): the highest platen
@@ -523,7 +533,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         for (int k = 0; k < 8; ++k) {
             const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
             if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= g.nodes[0] || n1 >= g.nodes[1] || n2 >= g.nodes[2]) continue;
-            atomicMax(&a.marks_out[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
+            atomicMax(&marks_out[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
         }
         return;
     }
@@ -552,7 +562,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             const int x = bc[0] + d[0], y = bc[1] + d[1], z = bc[2] + d[2];
             int4 bi = make_int4(-1, 0, 0, 0);
             if (x >= 0 && y >= 0 && z >= 0 && x < g.nb[0] && y < g.nb[1] && z < g.nb[2])
-                bi = a.binfo_in[x + (size_t)g.nb[0] * (y + (size_t)g.nb[1] * z)];
+                bi = binfo_in[x + (size_t)g.nb[0] * (y + (size_t)g.nb[1] * z)];
             // a missing neighbour gets an empty window (extent 0)
             const int pk[3] = {bi.y, bi.z, bi.w};
             int lo[3], ext[3];
@@ -610,14 +620,14 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
             // counted once in the grid sums: by the lowest covering brick
             const uint32_t fw = (uint32_t)a.fixed[nid] | ((first == cown) ? 0x100u : 0u);
-            const unsigned long long mk = a.marks_in[nid];
+            const unsigned long long mk = marks_in[nid];
             real sum[kQ];
 #pragma unroll
             for (int q = 0; q < kQ; ++q) sum[q] = real(0);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 real rec[8];
-                load8(a.arena_in + eo[c], rec);
+                load8(arena_in + eo[c], rec);
 #pragma unroll
                 for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
             }
@@ -670,7 +680,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             if ((uint32_t)(mk >> 32) == st.epoch + 1u) {   // platen override (…_MP.cpp:245-265)
                 const int j = (int)(mk & 0xFFFFFFFFull);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) { vn[q] = (real)a.dc_v_in[3 * j + q]; ac[q] = real(0); }
+                for (int q = 0; q < 3; ++q) { vn[q] = (real)dc_v_in[3 * j + q]; ac[q] = real(0); }
             }
             real *wn = s_node + 6 * r;
 #pragma unroll
@@ -1015,7 +1025,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     }
     if (tid == 0) {
         const bool empty = box[1] < box[0] || box[3] < box[2] || box[5] < box[4];
-        a.binfo_out[brick] = make_int4(empty ? -1 : slot, box[0] | (box[1] << 16), box[2] | (box[3] << 16),
+        binfo_out[brick] = make_int4(empty ? -1 : slot, box[0] | (box[1] << 16), box[2] | (box[3] << 16),
                                    box[4] | (box[5] << 16));
     }
 
@@ -1024,7 +1034,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     if (box[1] >= box[0]) {
         const int bx = box[1] - box[0] + 1, by = box[3] - box[2] + 1, bz = box[5] - box[4] + 1;
         const int nbox = bx * by * bz;
-        real *out = a.arena_out + (size_t)slot * 8 * AN;
+        real *out = arena_out + (size_t)slot * 8 * AN;
         double scl[kQ];
 #pragma unroll
         for (int q = 0; q < kQ; ++q) scl[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, ecur[q] - FRAC);

@@ -157,6 +157,28 @@ def test_fp32_error_vs_fp64_oracle(mpm, oracle_mod, material, speed, steps):
     eng.close()
 
 
+def test_fp32_plastic_fields_vs_fp64_oracle(mpm, oracle_mod):
+    """FP32 state with the FP64 return map, in a window where most points
+    yield: the plastic strain and hardening variable follow the FP64 oracle
+    (L dt strain form), besides displacement and stress."""
+    sc = mpm.scene.small_world(material="j2", speed=2.0, nx=2, ny=1)
+    d = sc.dc == 0
+    x0 = sc.x.copy()
+    with mpm.PhysicsEngine(sc, precision=mpm.FP32) as eng:
+        eng.run_steps(400)
+        g = eng.getMaterialPoints()
+    orc = oracle_mod.Oracle(sc, strain_form=1)
+    orc.run_steps(400)
+    o = orc.points()
+    yielded = np.any(o["eps_p"][d] != 0.0, axis=1).mean()
+    e = {k: rel_l2(g[k][d], o[k][d]) for k in ("eps_p", "alpha", "stress")}
+    e["disp"] = rel_l2((g["x"] - x0)[d], (o["x"] - x0)[d])
+    print(f"yielded {yielded:.3f}: " + " ".join(f"{k} {v:.3e}" for k, v in e.items()), flush=True)
+    assert yielded > 0.3
+    assert e["disp"] <= 1e-4 and e["stress"] <= 1e-4
+    assert e["eps_p"] <= 1e-3 and e["alpha"] <= 1e-3
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_per_point_modulus(mpm, oracle_mod, prec):
     """More distinct materials than the 255-entry table holds: Young's modulus
