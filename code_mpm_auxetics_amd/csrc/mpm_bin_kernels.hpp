@@ -327,12 +327,22 @@ __global__ void __launch_bounds__(256) k_step_sums(const double *gpart, int occ_
     if (t == 0) hist[blockIdx.x] = s[0];
 }
 
+// The step records of a batch, read from the host's pinned record set
+// (mapped host memory) into device memory; one block.
+__global__ void __launch_bounds__(64) k_load_steps(const uint32_t *src, uint32_t *dst, int nwords)
+{
+    for (int i = threadIdx.x; i < nwords; i += 64) dst[i] = src[i];
+}
+
 // The per-step reactions of a batch and the last step's grid totals in one
 // launch: blocks 0..nsteps-1 are k_step_sums, block nsteps is k_sum3 of the
-// last step (the same fixed trees, so the same bits).
+// last step (the same fixed trees, so the same bits).  The reactions and the
+// flags word go straight to the host's pinned result set `out` (mapped host
+// memory): out[0] flags, out[1 + k] the reaction of step k.
 __global__ void __launch_bounds__(256) k_batch_sums(const double *gpart, int occ_cap, const int32_t *n_occ,
-                                                   int nsteps, double *hist, double *grid3)
+                                                   int nsteps, double *out, double *grid3, const int32_t *flags)
 {
+    double *const hist = out + 1;
     __shared__ double s[3][256];
     const int t = threadIdx.x, no = *n_occ;
     if ((int)blockIdx.x < nsteps) {
@@ -360,6 +370,11 @@ __global__ void __launch_bounds__(256) k_batch_sums(const double *gpart, int occ
         __syncthreads();
     }
     if (t < 3) grid3[t] = s[t][0];
+    if (t == 0) {
+        int32_t *o = reinterpret_cast<int32_t *>(out);
+        o[0] = *flags;
+        o[1] = 0;
+    }
 }
 
 // Per-group reaction of a batch: block (step k, group g) sums the group's

@@ -67,6 +67,7 @@ struct StepRec {
     int32_t par;
     int32_t pad;
 };
+static_assert(sizeof(StepRec) % 4 == 0, "step records are copied as 32-bit words");
 
 // Material table entry (reference per-point fields E, nu, sigma_y, C0, C1 and
 // type; MaterialPoint.h:28-56), deduplicated on upload.

@@ -125,6 +125,10 @@ struct mpm_handle {
     // one copy brings a batch's results to out_h (pinned, two sets)
     double *out_d = nullptr, *out_h = nullptr;
     size_t out_stride = 0;                  // doubles per set
+    // device addresses of the pinned sets: the plain path's kernels read the
+    // step records and write the results there directly (no copies)
+    double *out_hd = nullptr;
+    StepRec *steps_hd = nullptr;
     int32_t *flags_d = nullptr, *flags_h = nullptr;
     double *kp_part = nullptr;
     double *gpart = nullptr;                // [steps_cap][occ_cap][3] grid-update partials
@@ -797,17 +801,17 @@ static int group_pre(mpm_handle *h, int nsteps)
     HIPCHK(h, hipGetLastError());
     return MPM_OK;
 }
-// ... and reduce them per (step, group) in a fixed order into ghist_d (the
-// caller copies them back)
-static int group_post(mpm_handle *h, int nsteps)
+// ... and reduce them per (step, group) in a fixed order into dst (the
+// group part of a result set)
+static int group_post(mpm_handle *h, int nsteps, double *dst)
 {
     if (h->n_groups == 0) return MPM_OK;
     if (h->n_fy > 0) {
         k_group_sums<<<(unsigned)(nsteps * h->n_groups), 256, 0, h->stream>>>(h->fy_val, h->n_fy, h->fy_off, h->n_groups,
-                                                                            h->ghist_d);
+                                                                            dst);
     } else {
         const size_t n = (size_t)nsteps * h->n_groups;
-        k_fill_f64<<<cdiv(n, 256), 256, 0, h->stream>>>(h->ghist_d, n, 0.0);
+        k_fill_f64<<<cdiv(n, 256), 256, 0, h->stream>>>(dst, n, 0.0);
     }
     HIPCHK(h, hipGetLastError());
     return MPM_OK;
@@ -822,23 +826,30 @@ static void drop_graphs(mpm_handle *h)
     h->graphs.clear();
 }
 
-// The kernels of a batch of nsteps plain steps, starting at step record 0:
-// the fused step kernels, the per-step reaction sums, the last step's grid
-// totals and the per-group reactions.  Parities come from the step records,
-// so the same sequence serves every starting parity.
+// A batch of nsteps plain steps with the step records of host set `slot`
+// and its results written to result set `slot`: the records in, the fused
+// step kernels, the per-step reaction sums with the flags word, the last
+// step's grid totals and the per-group reactions.  Kernels only, with the
+// pinned sets accessed in place.  Parities come from the step records, so
+// the same sequence serves every starting parity.
 template <typename real>
-static int batch_kernels(mpm_handle *h, int nsteps, std::vector<ProfMark> *marks)
+static int batch_ops(mpm_handle *h, int nsteps, int slot, std::vector<ProfMark> *marks)
 {
+    double *out = h->out_hd + (size_t)slot * h->out_stride;
+    k_load_steps<<<1, 64, 0, h->stream>>>(reinterpret_cast<const uint32_t *>(h->steps_hd + (size_t)slot * h->steps_cap),
+                                         reinterpret_cast<uint32_t *>(h->steps_d),
+                                         (int)(nsteps * sizeof(StepRec) / 4));
+    HIPCHK(h, hipGetLastError());
     RCHK(group_pre(h, nsteps));
     for (int k = 0; k < nsteps; ++k) {
         const int pe = marks ? prof_begin(h, *marks, 0) : -1;
         RCHK(launch_particle<real>(h, true, k));
         prof_end(h, pe);
     }
-    k_batch_sums<<<(unsigned)nsteps + 1, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, nsteps, h->hist_d,
-                                                              h->grid3_d);
+    k_batch_sums<<<(unsigned)nsteps + 1, 256, 0, h->stream>>>(h->gpart, h->occ_cap, h->n_occ_d, nsteps, out,
+                                                              h->grid3_d, h->flags_d);
     HIPCHK(h, hipGetLastError());
-    RCHK(group_post(h, nsteps));
+    RCHK(group_post(h, nsteps, out + 1 + h->steps_cap));
     return MPM_OK;
 }
 
@@ -848,19 +859,6 @@ static size_t out_bytes(const mpm_handle *h, int nsteps)
 {
     return sizeof(double) * (h->n_groups > 0 ? 1 + (size_t)h->steps_cap + (size_t)nsteps * h->n_groups
                                              : 1 + (size_t)nsteps);
-}
-
-// A whole batch: step records in from host set `slot`, the kernels, the
-// result block out to host set `slot`.
-template <typename real>
-static int batch_ops(mpm_handle *h, int nsteps, int slot, std::vector<ProfMark> *marks)
-{
-    HIPCHK(h, hipMemcpyAsync(h->steps_d, h->steps_h + (size_t)slot * h->steps_cap, sizeof(StepRec) * nsteps,
-                             hipMemcpyHostToDevice, h->stream));
-    RCHK(batch_kernels<real>(h, nsteps, marks));
-    HIPCHK(h, hipMemcpyAsync(h->out_h + (size_t)slot * h->out_stride, h->out_d, out_bytes(h, nsteps),
-                             hipMemcpyDeviceToHost, h->stream));
-    return MPM_OK;
 }
 
 // The hipGraph of batch_ops(nsteps, slot) for the current brick layout,
@@ -1346,7 +1344,9 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     (void)hipMemsetAsync(h->grid3_d, 0, 24, h->stream);
     h->steps_cap = 64;
     if ((rc = dalloc(h, &h->steps_d, h->steps_cap))) return fail(rc);
-    if (hipHostMalloc((void **)&h->steps_h, 2 * sizeof(StepRec) * h->steps_cap, hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc((void **)&h->steps_h, 2 * sizeof(StepRec) * h->steps_cap,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&h->steps_hd, h->steps_h, 0) != hipSuccess ||
         hipHostMalloc((void **)&h->flags_h, 16, hipHostMallocDefault) != hipSuccess) {
         h->err = "hipHostMalloc";
         return fail(MPM_E_NOMEM);
@@ -1377,7 +1377,9 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     // the result block of a batch and its two host sets
     h->out_stride = 1 + (size_t)h->steps_cap * (1 + (size_t)h->n_groups);
     if ((rc = dalloc(h, &h->out_d, h->out_stride))) return fail(rc);
-    if (hipHostMalloc((void **)&h->out_h, 2 * sizeof(double) * h->out_stride, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&h->out_h, 2 * sizeof(double) * h->out_stride,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&h->out_hd, h->out_h, 0) != hipSuccess) {
         h->err = "hipHostMalloc";
         return fail(MPM_E_NOMEM);
     }
