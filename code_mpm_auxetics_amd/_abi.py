@@ -33,6 +33,8 @@ MPM_FIX_X, MPM_FIX_Y, MPM_FIX_Z = 1, 2, 4
 MPM_F_X, MPM_F_V, MPM_F_F, MPM_F_STRESS, MPM_F_STRAIN, MPM_F_EPSP, MPM_F_ALPHA, MPM_F_VOL, MPM_F_KERNEL = (
     1 << k for k in range(9))
 MPM_F_ALL = 0x1FF
+MPM_F_J2 = 1 << 9
+MPM_F_CELL = 1 << 10
 
 # every symbol the header declares (checked by tests/test_abi.py)
 EXPORTED = [
@@ -41,6 +43,7 @@ EXPORTED = [
     "mpm_last_error", "mpm_destroy", "mpm_sort_keys", "mpm_kernel_times", "mpm_group_reaction_history",
     "mpm_diff_create", "mpm_diff_forward", "mpm_diff_backward", "mpm_diff_last_error", "mpm_diff_destroy",
     "mpm_report", "mpm_set_log", "mpm_constitutive", "mpm_step_async", "mpm_step_poll", "mpm_step_wait",
+    "mpm_grid_nodes",
 ]
 
 # constitutive laws of mpm_constitutive (MaterialPoint.h:8-16 type codes; 16: calculateState_J2)
@@ -91,6 +94,16 @@ class MpmPoints(C.Structure):
         ("E", _dp), ("nu", _dp), ("sy", _dp), ("C0", _dp), ("C1", _dp),
         ("type", _i32p), ("dc", _u8p), ("surface", _u8p),
         ("kernel", _dp), ("kernel_grad", _dp),
+        ("j2", _dp), ("cell", _i32p), ("order", _u32p),
+    ]
+
+
+class MpmNodes(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("active", _u8p), ("fixed", _u8p),
+        ("x", _dp), ("mass", _dp), ("v", _dp), ("f", _dp), ("f_temp", _dp),
+        ("kernel", _dp), ("kernel_grad", _dp),
     ]
 
 
@@ -124,6 +137,7 @@ def lib():
     L.mpm_step_poll.argtypes = [C.c_void_p, _i32p]
     L.mpm_step_wait.argtypes = [C.c_void_p, _i32p]
     L.mpm_readback.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(MpmPoints)]
+    L.mpm_grid_nodes.argtypes = [C.c_void_p, C.POINTER(MpmNodes)]
     L.mpm_checks.argtypes = [C.c_void_p, _dp]
     L.mpm_reaction_history.argtypes = [C.c_void_p, _dp, C.c_int32, _i32p]
     L.mpm_group_reaction_history.argtypes = [C.c_void_p, _dp, C.c_int32, _i32p]

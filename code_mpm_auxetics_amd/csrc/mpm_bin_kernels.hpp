@@ -274,6 +274,9 @@ __global__ void k_find_id(const uint32_t *ids, uint32_t n, uint32_t id, uint32_t
 // ------------------------------------------------------------ readback
 struct ReadbackOut {
     double *x, *v, *F, *s, *eps, *ep, *alpha, *vol;
+    double *j2;         // 2 per point: J2 of the stress, J2 of the plastic strain
+    int32_t *cell;      // 3 per point: the binning cell
+    uint32_t *order;    // 1 per point: position in the binned order
 };
 
 template <typename real>
@@ -306,6 +309,30 @@ __global__ void __launch_bounds__(kBlock) k_readback(GridParams g, PArrays<real>
     if (mask & 32u) for (int a = 0; a < 6; ++a) o.ep[6 * (size_t)id + a] = (double)p.ep[a][j];
     if (mask & 64u) o.alpha[id] = (double)p.alpha[j];
     if (mask & 128u) o.vol[id] = det3(Fd) * (double)p.vol0[j];
+    if (mask & 512u) {   // calculateState_J2 (…calculateState_J2.cpp:4-26) of stress and plastic strain
+        double sd[6], ed[6];
+        for (int a = 0; a < 6; ++a) {
+            sd[a] = (double)p.s[a][j];
+            ed[a] = (double)p.ep[a][j];
+        }
+        o.j2[2 * (size_t)id] = j2_of(sd);
+        o.j2[2 * (size_t)id + 1] = j2_of(ed);
+    }
+    if (mask & 1024u) {  // the cell of k_rebin_keys, from the same state
+        for (int a = 0; a < 3; ++a) {
+            int c;
+            if (sizeof(real) == 4) {
+                const uint32_t pb = p.pbrick[j];
+                const int bc = (a == 0) ? (int)(pb % g.nb[0]) : (a == 1) ? (int)((pb / g.nb[0]) % g.nb[1])
+                                                                       : (int)(pb / (g.nb[0] * g.nb[1]));
+                c = bc * g.S[a] + (p.xq[a][j] >> kQFrac);
+            } else {
+                c = (int)((double)p.x[a][j] / g.h[a]);
+            }
+            o.cell[3 * (size_t)id + a] = c;
+        }
+        o.order[id] = j;
+    }
 }
 
 // Reaction of each step of a batch (PhysicsEngine.cpp:65-71): block k sums the

@@ -978,4 +978,23 @@ __global__ void __launch_bounds__(256) k_c_step_sums(const double *cpart, int nb
     if (t == 0) hist[blockIdx.x] = s[0];
 }
 
+// Node readback after a contact step (mpm_grid_nodes): the step's first
+// field (mass, velocity, force of …_MP_Contact.cpp:392-518), its active flag,
+// and the kernel grid (density and central-difference gradient,
+// …_MP_Contact.cpp:121-191).  One thread per node.
+template <typename real>
+__global__ void __launch_bounds__(256) k_c_nodes(CArgs<real> a, NodeOut o)
+{
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I >= a.nn) return;
+    o.active[I] = (a.flags[I] & CF_ACTIVE) ? 1 : 0;
+    o.mass[I] = a.gm[I];
+    for (int q = 0; q < 3; ++q) {
+        o.v[3 * I + q] = a.gv[q * a.nn + I];
+        o.f[3 * I + q] = a.gf[q * a.nn + I];
+        o.kg[3 * I + q] = a.gkgrad[q * a.nn + I];
+    }
+    o.k[I] = (double)a.acc[off_k() * a.nn + I] * c_unscale(a, 4);
+}
+
 }  // namespace mpm

@@ -1326,8 +1326,10 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         if ((rc = dalloc(h, &h->marks[b], (size_t)h->nnodes))) return fail(rc);
         (void)hipMemsetAsync(h->marks[b], 0, (size_t)h->nnodes * 8, h->stream);
     }
+    h->fixed_host.assign((size_t)h->nnodes, 0);
+    if (cfg->node_fixed) h->fixed_host.assign(cfg->node_fixed, cfg->node_fixed + h->nnodes);
     if (cfg->node_fixed) {
-        if (hipMemcpyAsync(h->fixed, cfg->node_fixed, (size_t)h->nnodes, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+        if (hipMemcpyAsync(h->fixed, h->fixed_host.data(), (size_t)h->nnodes, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
             h->err = "copy node_fixed";
             return fail(MPM_E_HIP);
         }
@@ -1677,16 +1679,27 @@ template <typename real>
 static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
 {
     const int n = h->n_total;
-    double *st[8] = {nullptr};
-    const int widths[8] = {3, 3, 9, 6, 6, 6, 1, 1};
-    double *dst[8] = {out->x, out->v, out->F, out->stress, out->strain, out->eps_p, out->alpha, out->vol};
-    for (int k = 0; k < 8; ++k) {
-        if (!(mask & (1u << k)) || !dst[k]) continue;
+    constexpr int NF = 9;   // x v F stress strain eps_p alpha vol j2 (doubles)
+    double *st[NF] = {nullptr};
+    const int widths[NF] = {3, 3, 9, 6, 6, 6, 1, 1, 2};
+    const uint32_t bits[NF] = {MPM_F_X, MPM_F_V, MPM_F_F, MPM_F_STRESS, MPM_F_STRAIN, MPM_F_EPSP,
+                               MPM_F_ALPHA, MPM_F_VOL, MPM_F_J2};
+    double *dst[NF] = {out->x, out->v, out->F, out->stress, out->strain, out->eps_p, out->alpha, out->vol, out->j2};
+    for (int k = 0; k < NF; ++k) {
+        if (!(mask & bits[k]) || !dst[k]) continue;
         RCHK(dalloc(h, &st[k], (size_t)widths[k] * std::max(n, 1)));
     }
+    int32_t *cell_d = nullptr;
+    uint32_t *order_d = nullptr;
+    const bool want_cell = (mask & MPM_F_CELL) && out->cell && out->order;
+    if (want_cell) {
+        RCHK(dalloc(h, &cell_d, 3 * (size_t)std::max(n, 1)));
+        RCHK(dalloc(h, &order_d, (size_t)std::max(n, 1)));
+    }
     uint32_t m = 0;
-    for (int k = 0; k < 8; ++k) if (st[k]) m |= (1u << k);
-    ReadbackOut o{st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7]};
+    for (int k = 0; k < NF; ++k) if (st[k]) m |= bits[k];
+    if (want_cell) m |= MPM_F_CELL;
+    ReadbackOut o{st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7], st[8], cell_d, order_d};
     if (h->n_def > 0) {
         k_readback<real><<<cdiv(h->n_def, kBlock), kBlock, 0, h->stream>>>(h->g, parr<real>(h, h->cur), (uint32_t)h->n_def, o, m);
         HIPCHK(h, hipGetLastError());
@@ -1696,7 +1709,7 @@ static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
         HIPCHK(h, hipMemcpyAsync(hx.data(), h->dc_x, sizeof(double) * 3 * h->n_dc, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipMemcpyAsync(hv.data(), h->dc_v[h->parity], sizeof(double) * 3 * h->n_dc, hipMemcpyDeviceToHost, h->stream));
     }
-    for (int k = 0; k < 8; ++k)
+    for (int k = 0; k < NF; ++k)
         if (st[k] && h->n_def > 0) {
             // copy whole array; platen entries are overwritten below
             std::vector<double> buf((size_t)widths[k] * n);
@@ -1705,6 +1718,17 @@ static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
             for (int i : h->def_id_host)
                 for (int w = 0; w < widths[k]; ++w) dst[k][(size_t)widths[k] * i + w] = buf[(size_t)widths[k] * i + w];
         }
+    if (want_cell && h->n_def > 0) {
+        std::vector<int32_t> cb(3 * (size_t)n);
+        std::vector<uint32_t> ob((size_t)n);
+        HIPCHK(h, hipMemcpyAsync(cb.data(), cell_d, sizeof(int32_t) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(ob.data(), order_d, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int i : h->def_id_host) {
+            for (int a = 0; a < 3; ++a) out->cell[3 * (size_t)i + a] = cb[3 * (size_t)i + a];
+            out->order[i] = ob[i];
+        }
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     for (int j = 0; j < h->n_dc; ++j) {
         const int i = h->dc_id_host[j];
@@ -1717,8 +1741,18 @@ static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
         if (st[5]) for (int a = 0; a < 6; ++a) out->eps_p[6 * i + a] = sv[21 + a];
         if (st[6]) out->alpha[i] = sv[27];
         if (st[7]) out->vol[i] = sv[28];
+        if (st[8]) {   // platen points keep their initial stress and plastic strain
+            out->j2[2 * (size_t)i] = j2_of(sv + 9);
+            out->j2[2 * (size_t)i + 1] = j2_of(sv + 21);
+        }
+        if (want_cell) {   // platens are not binned: the cell of trunc(x / h), no order
+            for (int a = 0; a < 3; ++a) out->cell[3 * (size_t)i + a] = (int32_t)(hx[3 * j + a] / h->g.h[a]);
+            out->order[i] = 0xFFFFFFFFu;
+        }
     }
-    for (int k = 0; k < 8; ++k) dev_free((void *&)st[k]);
+    for (int k = 0; k < NF; ++k) dev_free((void *&)st[k]);
+    dev_free((void *&)cell_d);
+    dev_free((void *&)order_d);
     // d_Kernel / d3_Kernel_Gradient of the last contact step (zero before one)
     if (mask & MPM_F_KERNEL) {
         if (out->kernel) {
@@ -1733,7 +1767,78 @@ static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
     return MPM_OK;
 }
 
+// mpm_grid_nodes: node fields into dense device arrays, then to the caller
+template <typename real>
+static int grid_nodes_t(mpm_handle *h, mpm_nodes *out)
+{
+    const size_t nn = (size_t)h->nnodes;
+    uint8_t *act = nullptr;
+    double *buf = nullptr;   // mass 1, v 3, f 3, ft 3, k 1, kg 3 per node
+    RCHK(dalloc(h, &act, nn));
+    if (int rc = dalloc(h, &buf, 14 * nn)) { dev_free((void *&)act); return rc; }
+    NodeOut o{act, buf, buf + nn, buf + 4 * nn, buf + 7 * nn, buf + 10 * nn, buf + 11 * nn};
+    int rc = MPM_OK;
+    auto run = [&]() -> int {
+        HIPCHK(h, hipMemsetAsync(act, 0, nn, h->stream));
+        HIPCHK(h, hipMemsetAsync(buf, 0, 14 * nn * sizeof(double), h->stream));
+        if (h->last_path == 1 && h->ct.ready) {
+            k_c_nodes<real><<<cdiv(nn, 256), 256, 0, h->stream>>>(contact_args<real>(h), o);
+            HIPCHK(h, hipGetLastError());
+        } else if (h->n_occ > 0) {
+            // the next step's record: current time, dt_max, epoch = step count
+            StepRec st{};
+            st.dt = h->cfg.dt_max;
+            st.t = h->time;
+            st.epoch = (uint32_t)h->cycle;
+            st.par = h->parity;
+            k_grid_state<real><<<(unsigned)h->n_occ, 256, 0, h->stream>>>(
+                h->g, (const real *)h->arena[h->parity], h->binfo[h->parity],
+                (uint32_t)(h->arena_cap_slots * 8 * h->g.arenaN), h->fixed, h->marks[h->parity], h->dc_v[h->parity],
+                h->occ_list, h->n_occ_d, st, o);
+            HIPCHK(h, hipGetLastError());
+        }
+        std::vector<double> hb(14 * nn);
+        std::vector<uint8_t> ha(nn);
+        HIPCHK(h, hipMemcpyAsync(hb.data(), buf, 14 * nn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(ha.data(), act, nn, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        auto put = [&](double *dst, size_t off, int w) {
+            if (dst) std::memcpy(dst, hb.data() + off, sizeof(double) * w * nn);
+        };
+        put(out->mass, 0, 1);
+        put(out->v, nn, 3);
+        put(out->f, 4 * nn, 3);
+        put(out->f_temp, 7 * nn, 3);
+        put(out->kernel, 10 * nn, 1);
+        put(out->kernel_grad, 11 * nn, 3);
+        if (out->active) std::memcpy(out->active, ha.data(), nn);
+        if (out->fixed) std::memcpy(out->fixed, h->fixed_host.data(), nn);
+        if (out->x) {   // GridPoint_Factory.h:40-42: (i L) / n per axis
+            const int nx = h->g.nodes[0], ny = h->g.nodes[1];
+            for (size_t I = 0; I < nn; ++I) {
+                const int i[3] = {(int)(I % nx), (int)((I / nx) % ny), (int)(I / ((size_t)nx * ny))};
+                for (int a = 0; a < 3; ++a) out->x[3 * I + a] = ((double)i[a] * h->g.L[a]) / (double)h->g.cells[a];
+            }
+        }
+        return MPM_OK;
+    };
+    rc = run();
+    dev_free((void *&)act);
+    dev_free((void *&)buf);
+    return rc;
+}
+
 extern "C" {
+
+int mpm_grid_nodes(mpm_handle *h, mpm_nodes *out)
+{
+    BUSYCHK(h);
+    if (!h || !out) return MPM_E_ARG;
+    if (!h->uploaded) { h->err = "grid readback before upload"; return MPM_E_STATE; }
+    if (out->n != h->nnodes) { h->err = "grid readback node count mismatch"; return MPM_E_ARG; }
+    HIPCHK(h, hipSetDevice(h->device));
+    return h->fp32 ? grid_nodes_t<float>(h, out) : grid_nodes_t<double>(h, out);
+}
 
 int mpm_readback(mpm_handle *h, uint32_t field_mask, mpm_points *out)
 {

@@ -1069,3 +1069,122 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
 }
 
 }  // namespace mpm
+
+namespace mpm {
+
+// ------------------------------------------------------------ grid readback
+// Dense node arrays (x fastest) written by k_grid_state / k_c_nodes.
+struct NodeOut {
+    uint8_t *active;
+    double *mass, *v, *f, *ft;     // 1, 3, 3, 3 per node
+    double *k, *kg;                // contact kernel grid: 1, 3 per node (may be null)
+};
+
+// The grid of the next plain step from the current arenas (the P2G sums of
+// the current points, written by the last step or the rebuild): for every
+// node of every occupied brick's window that the brick owns (it is the
+// lowest covering brick, as in the step's grid sums), the covering arenas'
+// sums in ascending brick order and the grid update of k_particle's phase A
+// (…_MP.cpp:209-265) with step record `st`.  The candidate order and the
+// arithmetic are those of phase A, so the values are the ones the next step
+// computes.  Output: mass, velocity after the update, force after damping,
+// boundary conditions and the platen override, Force_Temp.y (the reaction
+// of y-fixed nodes).  One block per occupied slot.
+template <typename real>
+__global__ void __launch_bounds__(256) k_grid_state(GridParams g, const real *arena, const int4 *binfo,
+                                                    uint32_t zero_off, const uint8_t *fixed,
+                                                    const unsigned long long *marks, const double *dc_v,
+                                                    const uint32_t *occ_list, const int32_t *n_occ, StepRec st,
+                                                    NodeOut o)
+{
+    const int slot = blockIdx.x;
+    if (slot >= *n_occ) return;
+    const uint32_t brick = occ_list[slot];
+    const int bc[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
+    const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
+    const int AN = g.arenaN;
+    // the 27 neighbour windows (dz, dy, dx ascending), as phase A stages them
+    __shared__ int4 s_nb[27][2];
+    if (threadIdx.x < 27) {
+        const int t = threadIdx.x;
+        const int d[3] = {t % 3 - 1, (t / 3) % 3 - 1, t / 9 - 1};
+        const int x = bc[0] + d[0], y = bc[1] + d[1], z = bc[2] + d[2];
+        int4 bi = make_int4(-1, 0, 0, 0);
+        if (x >= 0 && y >= 0 && z >= 0 && x < g.nb[0] && y < g.nb[1] && z < g.nb[2])
+            bi = binfo[x + (size_t)g.nb[0] * (y + (size_t)g.nb[1] * z)];
+        const int pk[3] = {bi.y, bi.z, bi.w};
+        int lo[3], ext[3];
+        for (int q = 0; q < 3; ++q) {
+            lo[q] = (pk[q] & 0xFFFF) + d[q] * g.S[q];
+            ext[q] = (bi.x >= 0) ? (pk[q] >> 16) - (pk[q] & 0xFFFF) + 1 : 0;
+        }
+        s_nb[t][0] = make_int4((int)((uint32_t)max(bi.x, 0) * 8u * (uint32_t)AN), lo[0], lo[1], lo[2]);
+        s_nb[t][1] = make_int4(ext[0], ext[1], ext[2], ext[0] * ext[1]);
+    }
+    __syncthreads();
+    const int4 own0 = s_nb[13][0], own1 = s_nb[13][1];
+    const int wlo[3] = {own0.y, own0.z, own0.w}, wext[3] = {own1.x, own1.y, own1.z};
+    const int nwin = wext[0] * wext[1] * wext[2];
+    const real dtg = (real)st.dt, m_min = (real)g.m_min, damp = (real)g.damping;
+    for (int r = threadIdx.x; r < nwin; r += 256) {
+        const int lz = r / own1.w, rem = r - lz * own1.w;
+        const int ly = rem / wext[0], lx = rem - ly * wext[0];
+        const int l[3] = {wlo[0] + lx, wlo[1] + ly, wlo[2] + lz};
+        int lower[3];
+        for (int q = 0; q < 3; ++q) lower[q] = (l[q] <= 2 && bc[q] > 0) ? -1 : 0;
+        const int nb0 = (lower[2] + 1) * 9 + (lower[1] + 1) * 3 + (lower[0] + 1);
+        uint32_t eo[8];
+        int first = -1;
+        for (int c = 0; c < 8; ++c) {
+            const int4 w0 = s_nb[nb0 + (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1)][0];
+            const int4 w1 = s_nb[nb0 + (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1)][1];
+            const uint32_t t0 = (uint32_t)(l[0] - w0.y), t1 = (uint32_t)(l[1] - w0.z), t2 = (uint32_t)(l[2] - w0.w);
+            const bool ok = t0 < (uint32_t)w1.x && t1 < (uint32_t)w1.y && t2 < (uint32_t)w1.z;
+            eo[c] = ok ? (uint32_t)w0.x + 8u * (t0 + (uint32_t)w1.x * t1 + (uint32_t)w1.w * t2) : zero_off;
+            first = (ok && first < 0) ? c : first;
+        }
+        const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
+        if (first != cown) continue;   // another brick owns this node
+        const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
+        real sum[kQ];
+        for (int q = 0; q < kQ; ++q) sum[q] = real(0);
+        for (int c = 0; c < 8; ++c) {
+            real rec[8];
+            load8(arena + eo[c], rec);
+            for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
+        }
+        const uint32_t fx = fixed[nid];
+        const unsigned long long mk = marks[nid];
+        const real mn = sum[0];
+        real vn[3], fn[3] = {sum[4], sum[5], sum[6]};
+        for (int q = 0; q < 3; ++q) vn[q] = (mn > m_min) ? sum[1 + q] / mn : real(0);
+        const real vv = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+        if (sqrt(vv) > real(1.0e-9)) {   // damping (…_MP.cpp:209-222)
+            const real lf = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+            const real inv = real(1) / sqrt(vv);
+            const real sc = damp * lf;
+            for (int q = 0; q < 3; ++q) fn[q] -= sc * (vn[q] * inv);
+        }
+        real ac[3];
+        for (int q = 0; q < 3; ++q) ac[q] = fn[q] / mn;
+        if (mn > m_min)
+            for (int q = 0; q < 3; ++q) vn[q] += ac[q] * dtg;
+        double react = 0.0;
+        if (fx & 1) { vn[0] = real(0); fn[0] = real(0); }
+        if (fx & 2) { vn[1] = real(0); react = (double)fn[1]; fn[1] = real(0); }
+        if (fx & 4) { vn[2] = real(0); fn[2] = real(0); }
+        if ((uint32_t)(mk >> 32) == st.epoch + 1u) {   // platen override (…_MP.cpp:245-265)
+            const int j = (int)(mk & 0xFFFFFFFFull);
+            for (int q = 0; q < 3; ++q) { vn[q] = (real)dc_v[3 * j + q]; fn[q] = real(0); }
+        }
+        o.active[nid] = mn > real(0) ? 1 : 0;
+        o.mass[nid] = (double)mn;
+        for (int q = 0; q < 3; ++q) {
+            o.v[3 * nid + q] = (double)vn[q];
+            o.f[3 * nid + q] = (double)fn[q];
+        }
+        o.ft[3 * nid + 1] = react;
+    }
+}
+
+}  // namespace mpm

@@ -87,12 +87,15 @@ def node_masses(length, cells, x, mass, dc):
     return out
 
 
-def grid_points_script(sc, x=None, active_only=False):
-    """Header plus one row per node of the scene's grid (x fastest).  ``x``
-    is the points' current positions (default: the scene's initial ones);
-    ``active_only`` keeps the nodes with mass only."""
+def grid_points_script(sc, x=None, active_only=False, mass=None):
+    """Header plus one row per node of the scene's grid (x fastest).  Node
+    masses are ``mass`` when given (a solver's grid readback), else the P2G
+    mass of points at ``x`` (default: the scene's initial positions) -- the
+    host-side factory view before any solver exists; ``active_only`` keeps
+    the nodes with mass only."""
     pos = _scene.node_positions(sc.length, sc.cells)
-    m = node_masses(sc.length, sc.cells, sc.x if x is None else x, sc.mass, sc.dc)
+    m = np.asarray(mass, np.float64) if mass is not None else node_masses(sc.length, sc.cells,
+                                                                           sc.x if x is None else x, sc.mass, sc.dc)
     fx = np.asarray(sc.fixed).reshape(-1)
     rows = [GP_HEADER]
     for i in (np.flatnonzero(m > 0.0) if active_only else range(len(pos))):

@@ -19,7 +19,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi, dump
-from ._abi import MpmConfig, MpmPoints, check, lib
+from ._abi import MpmConfig, MpmNodes, MpmPoints, check, lib
 
 FP64 = _abi.MPM_FP64
 FP32 = _abi.MPM_FP32
@@ -162,6 +162,10 @@ class PhysicsEngine:
 
     # ------------------------------------------------------------- readback
     def getMaterialPoints(self, mask=_abi.MPM_F_ALL):
+        """The MaterialPoint fields of ``mask`` in the original point order.
+        With MPM_F_J2: ``j2`` (n, 2), calculateState_J2 of the stress and of
+        the plastic strain; with MPM_F_CELL: ``cell`` (n, 3) the binning cell
+        and ``order`` (n,) the position in the binned device order."""
         n = self.scene.n
         if not self.track_strain:
             mask &= ~_abi.MPM_F_STRAIN
@@ -170,12 +174,42 @@ class PhysicsEngine:
         if mask & _abi.MPM_F_KERNEL:
             out["kernel"] = np.zeros(n)
             out["kernel_grad"] = np.zeros((n, 3))
+        if mask & _abi.MPM_F_J2:
+            out["j2"] = np.zeros((n, 2))
         pts = MpmPoints()
         pts.n = n
         for k, a in out.items():
             setattr(pts, k, _ptr(a))
-        check(lib().mpm_readback(self._h, mask & _abi.MPM_F_ALL, C.byref(pts)), self._h)
+        if mask & _abi.MPM_F_CELL:
+            out["cell"] = np.zeros((n, 3), np.int32)
+            out["order"] = np.zeros(n, np.uint32)
+            pts.cell = _ptr(out["cell"], C.c_int32)
+            pts.order = _ptr(out["order"], C.c_uint32)
+        keep = _abi.MPM_F_ALL | _abi.MPM_F_J2 | _abi.MPM_F_CELL
+        check(lib().mpm_readback(self._h, mask & keep, C.byref(pts)), self._h)
         return out
+
+    def getGridPoints(self):
+        """GridPoint fields of every node (PhysicsEngine.h:65, GridPoint.h:12-28),
+        x fastest: ``active``, ``fixed`` (FIX_* bits), ``x``, ``mass``, ``v``,
+        ``f``, ``f_temp`` (see mpm_grid_nodes for which step they describe)."""
+        nn = self.getCount_GridPoint()
+        out = dict(active=np.zeros(nn, np.uint8), fixed=np.zeros(nn, np.uint8), x=np.zeros((nn, 3)),
+                   mass=np.zeros(nn), v=np.zeros((nn, 3)), f=np.zeros((nn, 3)), f_temp=np.zeros((nn, 3)),
+                   kernel=np.zeros(nn), kernel_grad=np.zeros((nn, 3)))
+        nd = MpmNodes()
+        nd.n = nn
+        for k, a in out.items():
+            setattr(nd, k, _ptr(a, C.c_uint8) if a.dtype == np.uint8 else _ptr(a))
+        check(lib().mpm_grid_nodes(self._h, C.byref(nd)), self._h)
+        return out
+
+    def getGridPoints_Kernel(self):
+        """The kernel grid of the contact step (PhysicsEngine.h:66): node
+        positions with ``kernel`` (d_Kernel) and ``kernel_grad``
+        (d3_Kernel_Gradient) of the last contact step (zeros before one)."""
+        g = self.getGridPoints()
+        return {k: g[k] for k in ("x", "kernel", "kernel_grad", "active", "fixed")}
 
     def checks(self):
         out = np.zeros(12)
@@ -252,8 +286,8 @@ class PhysicsEngine:
     def getScript_GridPoints(self, active_only=True):
         """GridPoint_Factory::getScript (GridPoint_Factory.h:69-100) over the
         grid nodes (those with mass only, by default), header first; node mass
-        is the P2G mass of the points' current positions."""
-        return dump.grid_points_script(self.scene, self.getMaterialPoints(_abi.MPM_F_X)["x"], active_only)
+        from mpm_grid_nodes (the P2G mass of the points' current positions)."""
+        return dump.grid_points_script(self.scene, active_only=active_only, mass=self.getGridPoints()["mass"])
 
     def saveLatex(self, strPrefix, strPostfix=".txt"):
         """The points' getScript table written to ``strPrefix + strPostfix``

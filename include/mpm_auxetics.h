@@ -17,7 +17,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 4
+#define MPM_ABI_VERSION 5
 
 /* ---- status codes (the reference has none; it exits via fatalError,
  *      Errors.cpp:5-12) ---------------------------------------------------- */
@@ -61,11 +61,19 @@ enum mpm_material {
 /* node fixity bits: GridPoint::b3_Fixed (GridPoint.h:18) */
 enum { MPM_FIX_X = 1, MPM_FIX_Y = 2, MPM_FIX_Z = 4 };
 
-/* readback field mask bits */
+/* readback field mask bits.  MPM_F_ALL covers the MaterialPoint fields;
+ * MPM_F_J2 and MPM_F_CELL are derived quantities outside it. */
 enum {
     MPM_F_X = 1u << 0, MPM_F_V = 1u << 1, MPM_F_F = 1u << 2, MPM_F_STRESS = 1u << 3,
     MPM_F_STRAIN = 1u << 4, MPM_F_EPSP = 1u << 5, MPM_F_ALPHA = 1u << 6,
-    MPM_F_VOL = 1u << 7, MPM_F_KERNEL = 1u << 8, MPM_F_ALL = 0x1FFu
+    MPM_F_VOL = 1u << 7, MPM_F_KERNEL = 1u << 8, MPM_F_ALL = 0x1FFu,
+    MPM_F_J2 = 1u << 9,      /* j2: calculateState_J2 of the stress and of the
+                                plastic strain, 2 per point (the quantities
+                                GraphicsEngine_drawGame.cpp:164,178,252 colour by) */
+    MPM_F_CELL = 1u << 10    /* cell (3 int32 per point): the grid cell the step
+                                bins the point into; order (1 uint32 per point):
+                                its position in the binned device order
+                                (0xFFFFFFFF for platen points) */
 };
 
 /* Solver configuration.  Replaces the protected members that
@@ -121,7 +129,27 @@ typedef struct mpm_points {
     uint8_t *dc;               /* b_DisplacementControl                         */
     uint8_t *surface;          /* b_Surface (contact path)                      */
     double *kernel, *kernel_grad;  /* d_Kernel, d3_Kernel_Gradient (readback)   */
+    double *j2;                /* MPM_F_J2 readback: 2 per point               */
+    int32_t *cell;             /* MPM_F_CELL readback: 3 per point             */
+    uint32_t *order;           /* MPM_F_CELL readback: 1 per point             */
 } mpm_points;
+
+/* Grid nodes, caller-allocated, (cells+1) product entries in the reference's
+ * linear order ix + iy nx + iz nx ny (GridPoint_Factory.h:23-32).  Mirrors the
+ * GridPoint fields (GridPoint.h:12-28).  Pointers a caller does not need may
+ * be NULL. */
+typedef struct mpm_nodes {
+    int64_t n;
+    uint8_t *active;           /* b_Active                                      */
+    uint8_t *fixed;            /* b3_Fixed as MPM_FIX_* bits                    */
+    double *x;                 /* d3_Position, 3 per node                       */
+    double *mass;              /* d_Mass                                        */
+    double *v;                 /* d3_Velocity, 3 per node                       */
+    double *f;                 /* d3_Force, 3 per node                          */
+    double *f_temp;            /* d3_Force_Temp, 3 per node                     */
+    double *kernel;            /* kernel grid d_Kernel (contact path)           */
+    double *kernel_grad;       /* kernel grid d3_Kernel_Gradient, 3 per node    */
+} mpm_nodes;
 
 typedef struct mpm_handle mpm_handle;
 
@@ -171,6 +199,21 @@ int mpm_step_wait(mpm_handle *h, int32_t *finished);
  * kernel / kernel_grad with d_Kernel / d3_Kernel_Gradient of the last contact
  * step (…_MP_Contact.cpp:199-219), zeros before one. */
 int mpm_readback(mpm_handle *h, uint32_t field_mask, mpm_points *out);
+
+/* Grid node state.  Replaces PhysicsEngine::getGridPoints and
+ * getGridPoints_Kernel (PhysicsEngine.h:65-66), whose callers read the live
+ * GridPoint objects (GraphicsEngine_drawGame.cpp:98,302,373,454).  The fused
+ * plain step keeps no grid between steps, so after a plain step (or upload)
+ * the nodes are the grid of the next step: the P2G of the current points
+ * (…_MP.cpp:41-202) and the grid update at the current time with dt_max
+ * (…_MP.cpp:209-265) -- velocity after the update, force after damping and
+ * boundary conditions, Force_Temp.y the reaction of y-fixed nodes; kernel
+ * fields zero.  After a contact step they are that step's main (first) field
+ * (…_MP_Contact.cpp:392-518) with its kernel grid (…_MP_Contact.cpp:121-191),
+ * Force_Temp zero (the platen reaction is in mpm_reaction_history).
+ * out->n must equal the node count.  Positions are (i L)/n per axis
+ * (GridPoint_Factory.h:40-42). */
+int mpm_grid_nodes(mpm_handle *h, mpm_nodes *out);
 
 /* FP64 global checks, computed on the device.  Replaces the sums of
  * PhysicsEngine::reportConsole (PhysicsEngine.cpp:33-97):

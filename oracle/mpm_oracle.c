@@ -357,7 +357,8 @@ static void grid_zero_active(or_world *w)
 }
 
 /* PhysicsEngine::runSimulation_Classic_SinglePass_MP, one step (…_MP.cpp:31-370) */
-int or_step_plain(or_world *w, double dt)
+/* phases 1-6 of the plain step: the grid of a step from the current points */
+static int plain_grid(or_world *w, double dt)
 {
     long long nn = nnodes_of(w);
     int T = w->nthreads > 1 ? w->nthreads : 1;
@@ -465,6 +466,20 @@ int or_step_plain(or_world *w, double dt)
             for (int a = 0; a < 3; ++a) { w->gv[3 * I + a] = w->v[3 * p + a]; w->gf[3 * I + a] = 0.0; }
         }
     }
+    (void)nn;
+    return OR_OK;
+}
+
+int or_grid_plain(or_world *w, double dt)
+{
+    return plain_grid(w, dt);
+}
+
+int or_step_plain(or_world *w, double dt)
+{
+    int T = w->nthreads > 1 ? w->nthreads : 1;
+    int rc = plain_grid(w, dt);
+    if (rc) return rc;
     /* phase 7: G2P + F + volume + strain + constitutive (…_MP.cpp:271-354) */
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(T) schedule(static) if (T > 1)

@@ -113,6 +113,10 @@ void or_timeline_velocity(int n, const double *t, const double *v, double time,
 /* full steps -------------------------------------------------------------- */
 /* one step of runSimulation_Classic_SinglePass_MP, without the time advance */
 int or_step_plain(or_world *w, double dt);
+/* phases 1-6 of that step only (…_MP.cpp:41-265): the grid arrays (gm, gv,
+ * gf, gft, gactive) then hold the grid of a step taken from the current
+ * points; the points are not changed */
+int or_grid_plain(or_world *w, double dt);
 /* one step of runSimulation_Classic_SinglePass_MP_Contact, without time advance */
 int or_step_contact(or_world *w, double dt);
 /* the sub-stepping loop of runSimulation_Classic_SinglePass_MP(_Contact);

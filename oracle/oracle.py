@@ -56,6 +56,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.or_step_plain.argtypes = [C.POINTER(OrWorld), C.c_double]
         L.or_step_contact.argtypes = [C.POINTER(OrWorld), C.c_double]
+        L.or_grid_plain.argtypes = [C.POINTER(OrWorld), C.c_double]
         L.or_run.argtypes = [C.POINTER(OrWorld), C.c_double, C.c_int, _dp, C.c_int, _ip, _ip]
         L.or_report.argtypes = [C.POINTER(OrWorld), _dp]
         L.or_bases.argtypes = [_dp, _dp, _dp, _dp, _dp]
@@ -227,6 +228,16 @@ class Oracle:
         rc = (lib().or_step_contact if path else lib().or_step_plain)(C.byref(self.w), dt)
         if rc != 0:
             raise RuntimeError(f"oracle error {rc}")
+
+    def grid_state(self, dt=None):
+        """Phases 1-6 of a plain step from the current points (or_grid_plain):
+        the grid of that step, points unchanged.  Returns node arrays."""
+        rc = lib().or_grid_plain(C.byref(self.w), self.scene.dt_max if dt is None else dt)
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}")
+        a = self.a
+        return dict(active=a["gactive"].copy(), mass=a["gm"].copy(), v=a["gv"].copy(), f=a["gf"].copy(),
+                    f_temp=a["gft"].copy())
 
     def report(self):
         out = np.zeros(8)

@@ -70,6 +70,31 @@ def test_struct_layout_matches_c(tmp_path):
     assert list(map(int, lines[2].split())) == [getattr(A.MpmPoints, f).offset for f in fields_pts]
 
 
+def test_nodes_layout_matches_c(tmp_path):
+    """sizeof / offsetof of mpm_nodes from a C compile of the header."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    fields = [f[0] for f in A.MpmNodes._fields_]
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpm_auxetics.h"', "int main(void){",
+           'printf("%zu\n", sizeof(mpm_nodes));']
+    src += [f'printf("%zu ", offsetof(mpm_nodes, {f}));' for f in fields]
+    src.append('printf("\n"); return 0; }')
+    c = tmp_path / "nodes.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "nodes"
+    subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    assert int(lines[0]) == C.sizeof(A.MpmNodes)
+    assert list(map(int, lines[1].split())) == [getattr(A.MpmNodes, f).offset for f in fields]
+
+
+def test_grid_nodes_argument_errors_without_device():
+    lib = A.lib()
+    nd = A.MpmNodes()
+    assert lib.mpm_grid_nodes(None, C.byref(nd)) == A.MPM_E_ARG
+
+
 def test_constitutive_batch_layout_matches_c(tmp_path):
     """sizeof / offsetof of mpm_cr_batch (the ctypes field ``yield_`` is the
     C field ``yield``) from a C compile of the header."""
