@@ -77,9 +77,9 @@ def test_nodes_layout_matches_c(tmp_path):
         pytest.skip("gcc not available")
     fields = [f[0] for f in A.MpmNodes._fields_]
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpm_auxetics.h"', "int main(void){",
-           'printf("%zu\n", sizeof(mpm_nodes));']
+           'printf("%zu\\n", sizeof(mpm_nodes));']
     src += [f'printf("%zu ", offsetof(mpm_nodes, {f}));' for f in fields]
-    src.append('printf("\n"); return 0; }')
+    src.append('printf("\\n"); return 0; }')
     c = tmp_path / "nodes.c"
     c.write_text("\n".join(src))
     exe = tmp_path / "nodes"
