@@ -43,7 +43,7 @@ EXPORTED = [
     "mpm_last_error", "mpm_destroy", "mpm_sort_keys", "mpm_kernel_times", "mpm_group_reaction_history",
     "mpm_diff_create", "mpm_diff_forward", "mpm_diff_backward", "mpm_diff_last_error", "mpm_diff_destroy",
     "mpm_report", "mpm_set_log", "mpm_constitutive", "mpm_step_async", "mpm_step_poll", "mpm_step_wait",
-    "mpm_grid_nodes",
+    "mpm_grid_nodes", "mpm_constitutive_noalloc",
 ]
 
 # constitutive laws of mpm_constitutive (MaterialPoint.h:8-16 type codes; 16: calculateState_J2)
@@ -152,6 +152,7 @@ def lib():
     L.mpm_report.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32, _i32p]
     L.mpm_set_log.argtypes = [C.c_void_p, C.c_char_p]
     L.mpm_constitutive.argtypes = [C.c_int32, C.POINTER(MpmCrBatch), C.c_int32]
+    L.mpm_constitutive_noalloc.argtypes = [C.c_int32, C.POINTER(MpmCrBatch), C.c_int32]
     L.mpm_diff_create.argtypes = [C.POINTER(MpmConfig), C.POINTER(MpmPoints), C.c_int32, C.c_int32,
                                   C.POINTER(C.c_void_p)]
     L.mpm_diff_forward.argtypes = [C.c_void_p, _dp, _dp, _dp]

@@ -2034,3 +2034,71 @@ int mpm_constitutive(int32_t law, const mpm_cr_batch *b, int32_t device)
 }
 
 }  // extern "C"
+// ---- mpm_constitutive_noalloc: the same batch through a fixed device array.
+// `scratch` is a file-scope device array (one copy per GPU the library runs
+// on, held until the process ends).  In host code its name is only a
+// stand-in: kernels and copies use the address hipGetSymbolAddress returns,
+// looked up once per device with that device current.
+constexpr size_t kNoAllocInputs = 8192;                 // inputs per call
+constexpr size_t kNoAllocDoubles = 41 * kNoAllocInputs;  // 41 = the most doubles one input uses
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mu;
+double *g_na_ptr[kMaxDevices] = {nullptr};
+}  // namespace
+
+extern "C" int mpm_constitutive_noalloc(int32_t law, const mpm_cr_batch *b, int32_t device)
+{
+    if (!b || b->n < 0) { g_create_error = "bad constitutive batch"; return MPM_E_ARG; }
+    if (law == MPM_LAW_VISCOELASTIC || law == MPM_LAW_IDEALGAS) {
+        g_create_error = "the ViscoElastic and IdealGas laws are not dispatched by the reference step and not provided";
+        return MPM_E_ARG;
+    }
+    if (!cr_law_known(law)) { g_create_error = "unknown constitutive law"; return MPM_E_ARG; }
+    if ((size_t)b->n > kNoAllocInputs) {
+        g_create_error = "mpm_constitutive_noalloc: at most " + std::to_string(kNoAllocInputs) +
+                         " inputs per call (use mpm_constitutive for larger batches)";
+        return MPM_E_ARG;
+    }
+    if (b->n == 0) return MPM_OK;
+    CrPart parts[kCrParts];
+    const char *why = nullptr;
+    const long long total = cr_layout(law, b, parts, &why);
+    if (total < 0) { g_create_error = why; return MPM_E_ARG; }
+    if ((size_t)total > kNoAllocDoubles) { g_create_error = "mpm_constitutive_noalloc: batch exceeds the scratch array"; return MPM_E_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return MPM_E_NO_DEVICE; }
+    if (device < 0 || device >= ndev || device >= kMaxDevices) { g_create_error = "bad device ordinal"; return MPM_E_ARG; }
+    // calls on one device share its array: serialise them in this process
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (hipSetDevice(device) != hipSuccess) { g_create_error = "hipSetDevice"; return MPM_E_HIP; }
+    if (!g_na_ptr[device]) {
+        double *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_create_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return MPM_E_HIP;
+        }
+        g_na_ptr[device] = p;
+    }
+    double *const buf = g_na_ptr[device];
+    int rc = MPM_OK;
+    for (int k = 0; k < kCrParts && rc == MPM_OK; ++k)
+        if (parts[k].src && hipMemcpy(buf + parts[k].off, parts[k].src, parts[k].len * sizeof(double),
+                                      hipMemcpyHostToDevice) != hipSuccess)
+            rc = MPM_E_HIP;
+    if (rc == MPM_OK) {
+        const CrBatch cb = cr_views(b->n, buf, parts);
+        k_constitutive<<<cdiv((size_t)b->n, 256), 256, 0, 0>>>(law, cb);
+        if (hipGetLastError() != hipSuccess) rc = MPM_E_HIP;
+    }
+    for (int k = 0; k < kCrParts && rc == MPM_OK; ++k)
+        if (parts[k].dst && hipMemcpy(parts[k].dst, buf + parts[k].off, parts[k].len * sizeof(double),
+                                      hipMemcpyDeviceToHost) != hipSuccess)
+            rc = MPM_E_HIP;
+    if (hipDeviceSynchronize() != hipSuccess) rc = MPM_E_HIP;
+    if (rc != MPM_OK) g_create_error = "constitutive batch: HIP error";
+    return rc;
+}

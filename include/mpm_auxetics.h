@@ -320,6 +320,17 @@ typedef struct mpm_cr_batch {
 
 int mpm_constitutive(int32_t law, const mpm_cr_batch *b, int32_t device);
 
+/* mpm_constitutive without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The batch is staged
+ * in a fixed device array of the library (41 x 8192 doubles, 2.6 MiB, one
+ * per GPU, held until the process ends) instead of a buffer the library
+ * allocates; the results equal mpm_constitutive's bit for bit.  At most 8192
+ * inputs per call (MPM_E_ARG beyond).  Calls on one GPU share the array:
+ * this process serialises its own calls, and any other user of the same
+ * library copy on that GPU at the same time races with them. */
+int mpm_constitutive_noalloc(int32_t law, const mpm_cr_batch *b, int32_t device);
+
 /* ---- differentiable run (no reference counterpart: the reference has no
  *      gradients).  n_steps plain steps of cfg->dt_max in FP64
  *      (…_MP.cpp:106-353, strain increment L dt, damping and TimeLine
