@@ -35,12 +35,16 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+NOALLOC_MAX = 8192   # inputs mpm_constitutive_noalloc takes per call
+
+
 def increments(law, E=None, nu=None, stress=None, strain_inc=None, F=None, yield_stress=None, alpha=None,
-               C0=None, C1=None, device=0):
+               C0=None, C1=None, device=0, noalloc=False):
     """Apply one law to n inputs on the GPU.  ``law`` is a name of ``LAWS``
     or an ``MPM_LAW_*`` code; scalars broadcast.  Returns a dict with
     ``stress_inc`` (n, 6), ``plastic_inc`` (n, 6), ``alpha_inc`` (n,), or
-    ``j2`` (n,) for the J2 state."""
+    ``j2`` (n,) for the J2 state.  ``noalloc`` uses mpm_constitutive_noalloc
+    (no device allocation, at most NOALLOC_MAX inputs; same results)."""
     code = LAWS[law] if isinstance(law, str) else int(law)
     sizes = [np.asarray(v).reshape(-1, k).shape[0] for v, k in ((stress, 6), (strain_inc, 6), (F, 9))
              if v is not None and np.asarray(v).ndim > 1]
@@ -73,13 +77,16 @@ def increments(law, E=None, nu=None, stress=None, strain_inc=None, F=None, yield
         out["stress_inc"], out["plastic_inc"], out["alpha_inc"] = np.zeros((n, 6)), np.zeros((n, 6)), np.zeros(n)
         b.stress_inc, b.plastic_inc, b.alpha_inc = (_ptr(out["stress_inc"]), _ptr(out["plastic_inc"]),
                                                     _ptr(out["alpha_inc"]))
-    check(lib().mpm_constitutive(code, C.byref(b), device), None)
+    entry = lib().mpm_constitutive_noalloc if noalloc else lib().mpm_constitutive
+    check(entry(code, C.byref(b), device), None)
     return out
 
 
 class ConstitutiveRelation:
     """ConstitutiveRelation.h:16-34: one call per point, results in the
-    output members (ConstitutiveRelation.h:21-24)."""
+    output members (ConstitutiveRelation.h:21-24).  Each call evaluates one
+    input through mpm_constitutive_noalloc, so per-point use allocates no
+    device memory."""
 
     def __init__(self, device=0):
         self.device = device
@@ -95,16 +102,16 @@ class ConstitutiveRelation:
 
     def calculateIncrement_Elastic(self, dE, dNu, d6StrainIncrement):
         self._store(increments("elastic", E=dE, nu=dNu, strain_inc=np.reshape(d6StrainIncrement, (1, 6)),
-                               device=self.device))
+                               device=self.device, noalloc=True))
 
     def calculateIncrement_NeoHookean_6D(self, dE, dNu, d6Stress_Current, d33DeformationGradient):
         self._store(increments("neohookean", E=dE, nu=dNu, stress=np.reshape(d6Stress_Current, (1, 6)),
-                               F=np.reshape(d33DeformationGradient, (1, 3, 3)), device=self.device))
+                               F=np.reshape(d33DeformationGradient, (1, 3, 3)), device=self.device, noalloc=True))
 
     def calculateIncrement_PerfectlyPlastic_6D(self, dE, dNu, dYield, d6StressCurrent, d6StrainIncrement):
         self._store(increments("plastic", E=dE, nu=dNu, yield_stress=dYield,
                                stress=np.reshape(d6StressCurrent, (1, 6)),
-                               strain_inc=np.reshape(d6StrainIncrement, (1, 6)), device=self.device))
+                               strain_inc=np.reshape(d6StrainIncrement, (1, 6)), device=self.device, noalloc=True))
 
     def calculateIncrement_VonMisesHardening_6D(self, dE, dNu, dYield, dBackstress_Isotropic,
                                                 dHardening_Isotropic_C0, dHardening_Isotropic_C1,
@@ -112,10 +119,11 @@ class ConstitutiveRelation:
         self._store(increments("vonmises", E=dE, nu=dNu, yield_stress=dYield, alpha=dBackstress_Isotropic,
                                C0=dHardening_Isotropic_C0, C1=dHardening_Isotropic_C1,
                                stress=np.reshape(d6StressCurrent, (1, 6)),
-                               strain_inc=np.reshape(d6StrainIncrement, (1, 6)), device=self.device))
+                               strain_inc=np.reshape(d6StrainIncrement, (1, 6)), device=self.device, noalloc=True))
 
     def calculateState_J2(self, d6State):
-        self.d_J2 = float(increments("j2", stress=np.reshape(d6State, (1, 6)), device=self.device)["j2"][0])
+        self.d_J2 = float(increments("j2", stress=np.reshape(d6State, (1, 6)), device=self.device,
+                                     noalloc=True)["j2"][0])
 
     def calculateIncrement_ViscoElastic_6D(self, dE, dNu, dEta, d6Stress_Current, d6Strain_New, d6Strain_Rate_New):
         raise NotImplementedError("ViscoElastic is never dispatched by the reference step (…_MP.cpp:332-339)")

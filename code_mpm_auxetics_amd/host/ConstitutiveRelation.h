@@ -4,8 +4,9 @@
 //
 // Method names, arguments and output members are the reference's; each call
 // evaluates the step's own device function for one input (FP64, the
-// reference's evaluation order).  Batches go through mpm_constitutive
-// directly.  glm is not used: the deformation gradient is a column-major
+// reference's evaluation order) through mpm_constitutive_noalloc, so the
+// per-point use of the reference allocates no device memory.  Batches go
+// through mpm_constitutive directly.  glm is not used: the deformation gradient is a column-major
 // std::array<double, 9> (m[col][row] at [3*col+row], as glm stores it).
 // The ViscoElastic and IdealGas laws are never dispatched by the reference
 // step (…_MP.cpp:332-339): their methods keep the signatures and throw.
@@ -71,7 +72,7 @@ class ConstitutiveRelation {
         b.n = 1;
         b.stress = d6State;
         b.j2 = &d_J2;
-        check(mpm_constitutive(MPM_LAW_J2_STATE, &b, device_));
+        check(mpm_constitutive_noalloc(MPM_LAW_J2_STATE, &b, device_));
     }
     void calculateIncrement_ViscoElastic_6D(double, double, double, const double[6], const double[6], const double[6])
     {
@@ -96,10 +97,10 @@ class ConstitutiveRelation {
         b.alpha_inc = &dBackstress_IsotropicIncrement;
         return b;
     }
-    void run(int32_t law, const mpm_cr_batch &b) { check(mpm_constitutive(law, &b, device_)); }
+    void run(int32_t law, const mpm_cr_batch &b) { check(mpm_constitutive_noalloc(law, &b, device_)); }
     static void check(int rc)
     {
-        if (rc != MPM_OK) throw std::runtime_error("mpm_constitutive: " + std::string(mpm_last_error(nullptr)));
+        if (rc != MPM_OK) throw std::runtime_error("mpm_constitutive_noalloc: " + std::string(mpm_last_error(nullptr)));
     }
 };
 
