@@ -239,29 +239,36 @@ def config1_run(m, out_dir=None):
 
 def batched_run(m, steps, warmup):
     """Config 4: 256 independent cells (SplitMix64 seed 20240607) packed into
-    one grid, FP32, one launch per step for all cells; aggregate
-    particle-steps/s and the per-cell effective Poisson's ratio."""
+    one grid, FP32, one launch per step for all cells (and, timed beside it,
+    the persistent launch: a resident grid of blocks pulling every cell's
+    bricks from per-XCD work queues); aggregate particle-steps/s and the
+    per-cell effective Poisson's ratio."""
     from code_mpm_auxetics_amd import scene
     ws = scene.config4_worlds()
     sc = scene.batch_worlds(ws)
     n = sc.n_deformable
-    eng = m.PhysicsEngine(sc, precision=m.FP32, track_strain=False)
-    eng.run_steps(warmup)
-    t0 = time.perf_counter()
-    eng.run_steps(steps)
-    t1 = time.perf_counter()
-    x1 = eng.getMaterialPoints(m._abi.MPM_F_X)["x"]
-    eng.close()
+    times = {}
+    for persistent in (True, False):
+        with m.PhysicsEngine(sc, precision=m.FP32, track_strain=False, persistent=persistent) as eng:
+            eng.run_steps(warmup)
+            t0 = time.perf_counter()
+            eng.run_steps(steps)
+            t1 = time.perf_counter()
+            x1 = eng.getMaterialPoints(m._abi.MPM_F_X)["x"]
+        times[persistent] = t1 - t0
     nus = []
     for i in range(len(ws)):
         keep = (sc.info["world"] == i) & (sc.dc == 0)
         nus.append(scene.effective_poisson(sc.x, x1, keep))
     nus = np.array(nus)
     poly = np.array([w.info["topology"] == "polygon" for w in ws])
+    t = times[False]
     print(json.dumps({"workload": "config4_batched_256_cells", "n_points": n, "cells": len(ws),
                       "grid_cells": list(sc.cells), "steps": steps, "warmup": warmup,
-                      "value": round(n * steps / (t1 - t0), 1), "unit": "particle-steps/s",
-                      "ms_per_step": round((t1 - t0) * 1e3 / steps, 4), "dtype": "f32",
+                      "value": round(n * steps / t, 1), "unit": "particle-steps/s",
+                      "ms_per_step": round(t * 1e3 / steps, 4), "dtype": "f32",
+                      "persistent_value": round(n * steps / times[True], 1),
+                      "persistent_ms_per_step": round(times[True] * 1e3 / steps, 4),
                       "poisson_median_polygon": round(float(np.median(nus[poly])), 4),
                       "poisson_median_swisscheese": round(float(np.median(nus[~poly])), 4),
                       "poisson_min": round(float(nus.min()), 4), "poisson_max": round(float(nus.max()), 4),

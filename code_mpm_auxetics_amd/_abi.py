@@ -83,6 +83,7 @@ class MpmConfig(C.Structure):
         ("launch", C.c_int32),
         ("node_group", C.POINTER(C.c_uint16)),
         ("n_groups", C.c_int32),
+        ("persistent", C.c_int32),
     ]
 
 

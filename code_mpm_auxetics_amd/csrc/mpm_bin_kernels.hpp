@@ -423,6 +423,13 @@ __global__ void __launch_bounds__(256) k_group_sums(const double *fy_val, int n_
     if (t == 0) ghist[(size_t)k * n_groups + gr] = s[0];
 }
 
+// fill a u32 array (graph-safe replacement of a memset)
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // fill a double array (graph-safe replacement of a memset)
 __global__ void k_fill_f64(double *p, size_t n, double v)
 {

@@ -106,6 +106,9 @@ struct mpm_handle {
     std::vector<uint32_t> slot_order_h, occ_count_h;
     int n_occ = 0, occ_cap = 0;
     int n_xcd = 8;                           // XCDs the dispatcher deals workgroups to (MI355X: 8)
+    int n_cu = 0;                            // compute units of the device
+    uint32_t *queue = nullptr;               // persistent launch: per-step work counters
+    int persist_blocks = 0;                  // persistent launch: resident brick blocks
     void *arena[2] = {nullptr, nullptr};
     int4 *binfo[2] = {nullptr, nullptr};    // per brick (dense): {slot, packed touched node box}
     size_t arena_cap_slots = 0;
@@ -551,6 +554,10 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.slot_order = h->slot_order;
     a.slot_mat = h->slot_mat;
     a.occ_cap = h->n_occ;    // brick blocks first, platen blocks after them
+    const bool persist = g2p && h->cfg.persistent && h->persist_blocks > 0;
+    a.n_xcd = h->n_xcd;
+    a.queue = h->queue;
+    if (persist) a.occ_cap = h->persist_blocks;
     a.mats = h->mats;
     a.steps = h->steps_d;
     a.flags = h->flags_d;
@@ -561,10 +568,12 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.fy_slot = (g2p && h->n_fy > 0) ? h->fy_slot : nullptr;
     a.fy_val = h->fy_val ? h->fy_val + (size_t)step * h->n_fy : nullptr;
     const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long) + (g2p ? (size_t)6 * h->g.arenaN * h->rsize : 0);
-    unsigned grid = (unsigned)h->n_occ;
+    unsigned grid = (unsigned)a.occ_cap;
     if (g2p) grid += cdiv(h->n_dc, kPBlock);
     if (grid == 0) return MPM_OK;
-    if (g2p)
+    if (persist)
+        k_particle<real, true, true><<<grid, kPBlock, lds, h->stream>>>(a, step);
+    else if (g2p)
         k_particle<real, true><<<grid, kPBlock, lds, h->stream>>>(a, step);
     else
         k_particle<real, false><<<grid, kPBlock, lds, h->stream>>>(a, step);
@@ -840,6 +849,10 @@ static int batch_ops(mpm_handle *h, int nsteps, int slot, std::vector<ProfMark> 
                                          reinterpret_cast<uint32_t *>(h->steps_d),
                                          (int)(nsteps * sizeof(StepRec) / 4));
     HIPCHK(h, hipGetLastError());
+    if (h->cfg.persistent && h->persist_blocks > 0) {
+        k_fill_u32<<<cdiv((size_t)nsteps * kQueueStride, 256), 256, 0, h->stream>>>(h->queue, (size_t)nsteps * kQueueStride, 0u);
+        HIPCHK(h, hipGetLastError());
+    }
     RCHK(group_pre(h, nsteps));
     for (int k = 0; k < nsteps; ++k) {
         const int pe = marks ? prof_begin(h, *marks, 0) : -1;
@@ -1180,7 +1193,7 @@ static void destroy_all(mpm_handle *h)
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d,
                      (void **)&h->slot_order, (void **)&h->slot_mat, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
                      (void **)&h->steps_d,
-                     (void **)&h->out_d, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->grid3_d,
+                     (void **)&h->out_d, (void **)&h->queue, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->grid3_d,
                      (void **)&h->fy_slot, (void **)&h->fy_off, (void **)&h->fy_val,
                      (void **)&h->checks_d, (void **)&h->radix_hist, (void **)&h->radix_off, (void **)&h->heads,
                      (void **)&h->headpos, (void **)&h->gdesc};
@@ -1296,17 +1309,23 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
         if (h->fp32) {
             e1 = hipFuncSetAttribute((const void *)k_particle<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
             e2 = hipFuncSetAttribute((const void *)k_particle<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
+            if (e1 == hipSuccess)
+                e1 = hipFuncSetAttribute((const void *)k_particle<float, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
         } else {
             e1 = hipFuncSetAttribute((const void *)k_particle<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
             e2 = hipFuncSetAttribute((const void *)k_particle<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
+            if (e1 == hipSuccess)
+                e1 = hipFuncSetAttribute((const void *)k_particle<double, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
         }
         if (e1 != hipSuccess || e2 != hipSuccess) { h->err = "hipFuncSetAttribute (dynamic LDS) failed"; return fail(MPM_E_HIP); }
     }
     {
         // the XCDs workgroups are dealt to round robin (brick launch order)
         int nx = 0;
-        if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, h->device) == hipSuccess && nx >= 1 && nx <= 64)
+        if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, h->device) == hipSuccess && nx >= 1 &&
+            nx <= kQueueStride)
             h->n_xcd = nx;
+        (void)hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device);
     }
     const int ncb = g.S[0] * g.S[1] * g.S[2];
     g.cbits = 0;
@@ -1345,7 +1364,18 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     (void)hipMemsetAsync(h->n_occ_d, 0, 4, h->stream);
     (void)hipMemsetAsync(h->grid3_d, 0, 24, h->stream);
     h->steps_cap = 64;
-    if ((rc = dalloc(h, &h->steps_d, h->steps_cap))) return fail(rc);
+    if ((rc = dalloc(h, &h->steps_d, h->steps_cap)) || (rc = dalloc(h, &h->queue, (size_t)h->steps_cap * kQueueStride)))
+        return fail(rc);
+    if (cfg->persistent) {
+        // resident brick blocks: blocks per CU at the step kernel's LDS, a
+        // multiple of the XCD count
+        const size_t need = (size_t)kQ * g.arenaN * 8 + (size_t)6 * g.arenaN * h->rsize;
+        int per_cu = 0;
+        const hipError_t e = h->fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_particle<float, true, true>, kPBlock, need)
+                                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_particle<double, true, true>, kPBlock, need);
+        if (e != hipSuccess || per_cu < 1 || h->n_cu < 1) { h->err = "occupancy query for the persistent launch failed"; return fail(MPM_E_HIP); }
+        h->persist_blocks = std::max(h->n_xcd, (per_cu * h->n_cu) / h->n_xcd * h->n_xcd);
+    }
     if (hipHostMalloc((void **)&h->steps_h, 2 * sizeof(StepRec) * h->steps_cap,
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void **)&h->steps_hd, h->steps_h, 0) != hipSuccess ||

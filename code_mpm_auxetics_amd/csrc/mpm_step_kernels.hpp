@@ -320,7 +320,12 @@ struct KpArgs {
     const int32_t *fy_slot;             // batched worlds: node -> y-fixed slot or -1 (null: off)
     double *fy_val;                     // this step's reaction per y-fixed slot
     double *dc_x;                       // 3 per platen point, FP64
+    // persistent launch: XCD count, per-step work counters (kQueueStride
+    // per step, zeroed before the batch)
+    int32_t n_xcd;
+    uint32_t *queue;
 };
+constexpr int kQueueStride = 64;   // work counters per step (>= XCDs)
 
 // Strain increment, total strain, constitutive update and stress write of
 // point i (…_MP.cpp:304-353), shared by the plain and the contact step.
@@ -482,7 +487,7 @@ constexpr int kPBlock = MPM_KP_BLOCK;
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
 
 // One brick per workgroup.  Dynamic LDS: long long arena[7 * arenaN].
-template <typename real, bool G2P>
+template <typename real, bool G2P, bool PERSIST = false>
 __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : MPM_KP_WAVES64) k_particle(KpArgs<real> a, int step)
 {
     extern __shared__ long long s_arena[];   // node-major: kQ int64 per arena node, then
@@ -538,533 +543,559 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         return;
     }
 
-    // brick blocks are launched longest first (most point chunks), which
-    // shortens the tail of the launch; results do not depend on the order
-    if ((int)blockIdx.x >= *a.n_occ) return;
-    const int slot = (int)a.slot_order[blockIdx.x];
-    const uint32_t brick = a.occ_list[slot];
-    const int bc[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
-    const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
-    const uint32_t start = a.occ_start[slot], count = a.occ_count[slot];
-    const int umat = (FP32 && a.slot_mat[slot] != 0xFF) ? (int)a.slot_mat[slot] : -1;
-    // FP32: load alpha with the point state when the brick may hold VonMises points
-    const bool pre_al = FP32 && (umat < 0 || a.mats[umat].type == 4);
-    const int A0 = g.A[0], A01 = g.A[0] * g.A[1], AN = g.arenaN;
-
-    // ---- phase A: grid update of step n over the node window = this brick's
-    // touched box of the last P2G (…_MP.cpp:209-265)
-    real *s_node = reinterpret_cast<real *>(s_arena + kQ * AN);
-    int wlo[3] = {0, 0, 0}, wext[3] = {0, 0, 0};
-    int stash = 0;   // reals of the arena the grid update uses as its stash
-    if (G2P) {
-        if (tid < 27) {
-            const int d[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
-            const int x = bc[0] + d[0], y = bc[1] + d[1], z = bc[2] + d[2];
-            int4 bi = make_int4(-1, 0, 0, 0);
-            if (x >= 0 && y >= 0 && z >= 0 && x < g.nb[0] && y < g.nb[1] && z < g.nb[2])
-                bi = binfo_in[x + (size_t)g.nb[0] * (y + (size_t)g.nb[1] * z)];
-            // a missing neighbour gets an empty window (extent 0)
-            const int pk[3] = {bi.y, bi.z, bi.w};
-            int lo[3], ext[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                lo[q] = (pk[q] & 0xFFFF) + d[q] * g.S[q];
-                ext[q] = (bi.x >= 0) ? (pk[q] >> 16) - (pk[q] & 0xFFFF) + 1 : 0;
-            }
-            s_nb[tid][0] = make_int4((int)((uint32_t)max(bi.x, 0) * 8u * (uint32_t)AN), lo[0], lo[1], lo[2]);
-            s_nb[tid][1] = make_int4(ext[0], ext[1], ext[2], ext[0] * ext[1]);
+    // Brick work.  One launch per step: block b works on slot slot_order[b]
+    // (brick blocks are launched longest first, most point chunks, which
+    // shortens the tail of the launch).  Persistent launch (PERSIST): a
+    // fixed grid of resident blocks; block b pulls the next item of work
+    // queue b mod X (X = XCDs: group x holds slot_order[X i + x], the bricks
+    // the one-launch order puts on XCD x, longest first) from a per-step
+    // counter, so each group's bricks still run on one XCD.  A slot's work is
+    // the same whichever block takes it: results do not depend on the order.
+    __shared__ int s_item;
+    for (int it = 0;; ++it) {
+        int item;
+        if constexpr (!PERSIST) {
+            if ((int)blockIdx.x >= *a.n_occ) return;
+            item = (int)blockIdx.x;
+        } else {
+            const int X = a.n_xcd, x = (int)blockIdx.x % X;
+            if (tid == 0) s_item = (int)atomicAdd(&a.queue[(size_t)step * kQueueStride + x], 1u);
+            __syncthreads();
+            const int i = s_item;   // rewritten only after the barrier that ends this item
+            const int no = *a.n_occ, nx = no / X + ((x < no % X) ? 1 : 0);
+            if (i >= nx) return;
+            item = X * i + x;
         }
-        __syncthreads();
-        const int4 own0 = s_nb[13][0], own1 = s_nb[13][1];
-        wlo[0] = own0.y; wlo[1] = own0.z; wlo[2] = own0.w;
-        wext[0] = own1.x; wext[1] = own1.y; wext[2] = own1.z;
-        const int nwin = wext[0] * wext[1] * wext[2];
-        // window index -> node: quotients through a float reciprocal, exact
-        // for r < 2^20 (the fraction of (r + 1/2) / e stays 1/(2e) away from
-        // an integer, far above the float error)
-        const float rx = 1.0f / (float)max(wext[0], 1), rxy = 1.0f / (float)max(own1.w, 1);
-        const real dtg = (real)st.dt, m_min = (real)g.m_min, damp = (real)g.damping;
-        // Pass 1: for each window node, the sum of the covering arenas'
-        // records (at most 8 candidates, ascending brick order, uncovered ones
-        // read the zero record), its fixity and override tag, stashed in LDS:
-        // sums in the (not yet used) arena, {tag, fixity | counted} in the
-        // node's window slot.  Branch-free, so that all loads of a node are in
-        // flight at once.  Each thread later reads back only the slots it
-        // wrote: no barrier.
-        real *s_sum = reinterpret_cast<real *>(s_arena);
-        stash = kQ * nwin;
-        for (int r = tid; r < nwin; r += kPBlock) {
-            const int lz = (int)(((float)r + 0.5f) * rxy), rem = r - lz * own1.w;
-            const int ly = (int)(((float)rem + 0.5f) * rx), lx = rem - ly * wext[0];
-            const int l[3] = {wlo[0] + lx, wlo[1] + ly, wlo[2] + lz};
-            const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
-            // per axis the arenas covering this node: own and the neighbour
-            // below (l <= 2) or above (l >= S); the candidates are the 2 x 2 x 2
-            // neighbours from `lower` up, and a candidate counts when the node
-            // lies in its window (brick sizes >= 3 keep the two sides apart)
-            int lower[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) lower[q] = (l[q] <= 2 && bc[q] > 0) ? -1 : 0;
-            const int nb0 = (lower[2] + 1) * 9 + (lower[1] + 1) * 3 + (lower[0] + 1);
-            uint32_t eo[8];
-            int first = -1;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int4 w0 = s_nb[nb0 + (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1)][0];
-                const int4 w1 = s_nb[nb0 + (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1)][1];
-                const uint32_t t0 = (uint32_t)(l[0] - w0.y), t1 = (uint32_t)(l[1] - w0.z), t2 = (uint32_t)(l[2] - w0.w);
-                const bool ok = t0 < (uint32_t)w1.x && t1 < (uint32_t)w1.y && t2 < (uint32_t)w1.z;
-                eo[c] = ok ? (uint32_t)w0.x + 8u * (t0 + (uint32_t)w1.x * t1 + (uint32_t)w1.w * t2) : a.zero_off;
-                first = (ok && first < 0) ? c : first;
-            }
-            const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
-            // counted once in the grid sums: by the lowest covering brick
-            const uint32_t fw = (uint32_t)a.fixed[nid] | ((first == cown) ? 0x100u : 0u);
-            const unsigned long long mk = marks_in[nid];
-            real sum[kQ];
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) sum[q] = real(0);
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                real rec[8];
-                load8(arena_in + eo[c], rec);
-#pragma unroll
-                for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
-            }
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) s_sum[kQ * r + q] = sum[q];
-            unsigned long long *wt = reinterpret_cast<unsigned long long *>(s_node + 6 * r);
-            wt[0] = mk;
-            wt[1] = fw;
-        }
-        // Pass 2: the grid update of each window node (…_MP.cpp:209-265)
-        double gs[3] = {0.0, 0.0, 0.0};   // owned nodes: mass, reaction, active count
-        for (int r = tid; r < nwin; r += kPBlock) {
-            const unsigned long long *wt = reinterpret_cast<const unsigned long long *>(s_node + 6 * r);
-            const unsigned long long mk = wt[0];
-            const uint32_t fw = (uint32_t)wt[1], fx = fw & 0xFFu;
-            // each stash slot is read, then zeroed for the P2G arena, by the
-            // thread that wrote it
-            real sum[kQ];
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                sum[q] = s_sum[kQ * r + q];
-                s_sum[kQ * r + q] = real(0);
-            }
-            // the rest of the last 8-byte arena word that the stash ends in
-            if (FP32 && r == nwin - 1 && (stash & 1)) s_sum[stash] = real(0);
-            const real mn = sum[0];
-            real vn[3], fn[3] = {sum[4], sum[5], sum[6]};
-#pragma unroll
-            for (int q = 0; q < 3; ++q) vn[q] = (mn > m_min) ? sum[1 + q] / mn : real(0);
-            const real vv = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
-            if (sqrt(vv) > real(1.0e-9)) {   // damping (…_MP.cpp:209-222)
-                const real lf = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
-                const real inv = real(1) / sqrt(vv);
-                const real sc = damp * lf;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) fn[q] -= sc * (vn[q] * inv);
-            }
-            // f / m after damping: the velocity update and the stored
-            // acceleration share it; a force zeroed below gives 0 / m = 0
-            real ac[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) ac[q] = fn[q] / mn;
-            if (mn > m_min)
-#pragma unroll
-                for (int q = 0; q < 3; ++q) vn[q] += ac[q] * dtg;
-            double react = 0.0;
-            if (fx & 1) { vn[0] = real(0); ac[0] = real(0); }
-            if (fx & 2) { vn[1] = real(0); react = (double)fn[1]; ac[1] = real(0); }
-            if (fx & 4) { vn[2] = real(0); ac[2] = real(0); }
-            if ((uint32_t)(mk >> 32) == st.epoch + 1u) {   // platen override (…_MP.cpp:245-265)
-                const int j = (int)(mk & 0xFFFFFFFFull);
-#pragma unroll
-                for (int q = 0; q < 3; ++q) { vn[q] = (real)dc_v_in[3 * j + q]; ac[q] = real(0); }
-            }
-            real *wn = s_node + 6 * r;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                wn[q] = vn[q];
-                wn[3 + q] = (fabs(mn) > m_min) ? ac[q] : real(0);
-            }
-            if (fw & 0x100u) {
-                gs[0] += (double)mn;
-                gs[1] += react;
-                gs[2] += (mn > real(0)) ? 1.0 : 0.0;
-                if (a.fy_slot && (fx & 2)) {
-                    const int lz = (int)(((float)r + 0.5f) * rxy), rem = r - lz * own1.w;
-                    const int ly = (int)(((float)rem + 0.5f) * rx), lx = rem - ly * wext[0];
-                    const int l[3] = {wlo[0] + lx, wlo[1] + ly, wlo[2] + lz};
-                    const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
-                    a.fy_val[a.fy_slot[nid]] = react;
-                }
-            }
-        }
-        // wave sums to LDS; the block's sum is taken after the barrier that
-        // precedes the point chunks (no barrier of its own)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double w = wave_sum_dpp(gs[k]);
-            if (lane == 0) s_red[wv][k] = w;
-        }
-    }
+        const int slot = (int)a.slot_order[item];
+        const uint32_t brick = a.occ_list[slot];
+        const int bc[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
+        const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
+        const uint32_t start = a.occ_start[slot], count = a.occ_count[slot];
+        const int umat = (FP32 && a.slot_mat[slot] != 0xFF) ? (int)a.slot_mat[slot] : -1;
+        // FP32: load alpha with the point state when the brick may hold VonMises points
+        const bool pre_al = FP32 && (umat < 0 || a.mats[umat].type == 4);
+        const int A0 = g.A[0], A01 = g.A[0] * g.A[1], AN = g.arenaN;
 
-    // the arena past the stash (the stash zeroes itself): no barrier between
-    // the grid update and this clear
-    const int clear0 = (stash * (int)sizeof(real) + 7) / 8;
-    for (int i = clear0 + tid; i < kQ * AN; i += kPBlock) s_arena[i] = 0;
-    if (tid < 24) s_max[tid >> 3][tid & 7] = 0u;
-    int H = 6;   // a node takes at most one contribution per point: count < 2^(H-1)
-    while ((1u << (H - 1)) <= count) ++H;
-    const int FRAC = FP32 ? kFrac32 : 61 - H;
-    int ecur[kQ];                        // block-uniform field exponents (value < 2^e)
-    uint32_t run_max[kQ];                // block-uniform running max of the bound bits
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) run_max[q] = 0u;
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) ecur[q] = kUnsetExp;
-    double red[5] = {0, 0, 0, 0, 0};     // this lane's running check sums (chunk order)
-    int flags = 0;
-    int bmin[3] = {1 << 20, 1 << 20, 1 << 20}, bmax[3] = {-1, -1, -1};
-    const PArrays<real> &P = a.p;
-    // FP32 position increment scale: dt / h * 2^26 per axis
-    const float qscale[3] = {(float)(dtd * (double)g.ihf[0] * kQOne), (float)(dtd * (double)g.ihf[1] * kQOne),
-                             (float)(dtd * (double)g.ihf[2] * kQOne)};
-    __syncthreads();
-    if (G2P && tid < 3) {
-        double t = 0.0;
-        for (int w = 0; w < kPBlock / 64; ++w) t += s_red[w][tid];
-        a.gpart[3 * (size_t)slot + tid] = t;
-    }
-
-    for (uint32_t c0 = 0; c0 < count; c0 += kPBlock) {
-        const bool active = (c0 + tid) < count;
-        const uint32_t i = start + c0 + tid;
-        int key = -1;
-        Axis<real> bx, by, bz;
-        real m = real(0), mv[3] = {real(0), real(0), real(0)}, vs[6], fe[3] = {real(0), real(0), real(0)};
-        uint32_t bq[kQ] = {0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int q = 0; q < 6; ++q) vs[q] = real(0);
-
-        if (active) {
-            real x[3], v[3], F[9], s[6];
-            int32_t xq[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (FP32) xq[q] = elt(P.xq[q], i);
-                else x[q] = elt(P.x[q], i);
-                v[q] = elt(P.v[q], i);
-            }
-#pragma unroll
-            for (int q = 0; q < 9; ++q) F[q] = elt(P.F[q], i);
-#pragma unroll
-            for (int q = 0; q < 6; ++q) s[q] = elt(P.s[q], i);
-            m = elt(P.mass, i);
-            const real V0 = elt(P.vol0, i);
-            real al_pre = real(0);
-            if (pre_al) al_pre = elt(P.alpha, i);
-            real V = real(0);
-
-            if (G2P) {
-                // ---- G2P of step n (…_MP.cpp:279-297)
-                axes_of(x, xq, g, O, bx, by, bz);
-                const bool oob = bx.cell < 0 || bx.cell >= g.cells[0] || by.cell < 0 || by.cell >= g.cells[1] ||
-                                 bz.cell < 0 || bz.cell >= g.cells[2];
-                // the 8 corner nodes in the window (the P2G of step n put them there)
-                const int wl[3] = {bx.cell - (O[0] - 1) - wlo[0], by.cell - (O[1] - 1) - wlo[1], bz.cell - (O[2] - 1) - wlo[2]};
-                const bool outw = wl[0] < 0 || wl[0] + 1 >= wext[0] || wl[1] < 0 || wl[1] + 1 >= wext[1] ||
-                                  wl[2] < 0 || wl[2] + 1 >= wext[2];
-                if (oob || outw) {
-                    flags |= oob ? FLAG_OUT_OF_GRID : FLAG_ARENA;
-                } else {
-                    const real *w0 = s_node + 6 * (wl[0] + wext[0] * (wl[1] + wext[1] * wl[2]));
-                    const int sy_ = wext[0], sz_ = wext[0] * wext[1];
-                    real L[9];
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) L[q] = real(0);
-                    real dx[3] = {real(0), real(0), real(0)};
-                    if constexpr (FP32) {
-                        // 8 corners fully unrolled; node record {v.xyz a.x | a.y a.z m -}
-                        const Hat32 w = hat32(bx, by, bz);
-                        float av[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            const int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
-                            const float N = w.wxy[2 * ax + ay] * w.wz[az];
-                            const float G[3] = {w.gx[ax] * w.wyz[2 * ay + az], w.gy[ay] * w.wxz[2 * ax + az],
-                                                w.gz[az] * w.wxy[2 * ax + ay]};
-                            const real *wp = w0 + 6 * (ax + sy_ * ay + sz_ * az);
-                            const float vn[3] = {wp[0], wp[1], wp[2]}, an[3] = {wp[3], wp[4], wp[5]};
-#pragma unroll
-                            for (int q = 0; q < 3; ++q) {
-                                av[q] += N * an[q];
-                                dx[q] += N * vn[q];
-                            }
-#pragma unroll
-                            for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                                for (int r = 0; r < 3; ++r) L[3 * cc + r] += vn[r] * G[cc];
-                        }
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) {
-                            v[q] += av[q] * dt;
-                            xq[q] += __float2int_rn((float)dx[q] * qscale[q]);
-                        }
-                    } else {
-#pragma unroll 2
-                        for (int k = 0; k < 8; ++k) {
-                            real N, G[3];
-                            corner(bx, by, bz, k, N, G);
-                            const real *wp = w0 + 6 * (((k >> 2) & 1) + sy_ * ((k >> 1) & 1) + sz_ * (k & 1));
-#pragma unroll
-                            for (int q = 0; q < 3; ++q) v[q] += N * wp[3 + q] * dt;
-#pragma unroll
-                            for (int q = 0; q < 3; ++q) x[q] += N * wp[q] * dt;
-#pragma unroll
-                            for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                                for (int r = 0; r < 3; ++r) L[3 * cc + r] += wp[r] * G[cc];
-                        }
-                    }
-                    // F += (L F) dt  (…_MP.cpp:299); FP32 state: D += (L + L D) dt
-                    real T[9];
-#pragma unroll
-                    for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                        for (int r = 0; r < 3; ++r)
-                            T[3 * cc + r] = (FP32 ? L[3 * cc + r] : real(0)) +
-                                            (L[r] * F[3 * cc] + L[3 + r] * F[3 * cc + 1] + L[6 + r] * F[3 * cc + 2]);
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) F[q] += T[q] * dt;
-                    // volume from the new F (…_MP.cpp:301-302), FP64
-                    {
-                        double Fd[9];
-#pragma unroll
-                        for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
-                        V = (real)(det3(Fd) * (double)V0);
-                    }
-                    // write back the kinematic state before the material update
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        if (FP32) elt(P.xq[q], i) = xq[q];
-                        else elt(P.x[q], i) = x[q];
-                        elt(P.v[q], i) = v[q];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) elt(P.F[q], i) = F[q];
-                    material_update<real>(g, P, a.mats, i, L, dt, F, s, umat, pre_al, al_pre);
-                }
-            } else {
-                double Fd[9];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
-                V = (real)(det3(Fd) * (double)V0);
-            }
-            // ---- FP64 checks of the new state (PhysicsEngine.cpp:73-79)
-            {
-                const double md = (double)m, vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
-                const double vv = vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2];
-                red[0] += md * vv;
-                red[1] += md * vd[0];
-                red[2] += md * vd[1];
-                red[3] += md * vd[2];
-                const real ssum = s[0] + s[1] + s[2] + s[3] + s[4] + s[5];
-                const bool fin = isfinite(vv) && isfinite(ssum) && (FP32 || isfinite(x[0] + x[1] + x[2]));
-                if (!fin) { red[4] += 1.0; flags |= FLAG_NAN; }
-            }
-            // ---- P2G inputs of step n+1 (…_MP.cpp:106-185)
-            axes_of(x, xq, g, O, bx, by, bz);
-            const int cl[3] = {bx.cell, by.cell, bz.cell};
-            const real cu[3] = {bx.u, by.u, bz.u};
-            int lc[3];
-            bool ok = true;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                lc[q] = cl[q] - (O[q] - 1);
-                if (cl[q] < 0 || cl[q] >= g.cells[q]) { flags |= FLAG_OUT_OF_GRID; ok = false; }
-                if (lc[q] < 0 || lc[q] > g.S[q] + 1) { flags |= FLAG_ARENA; ok = false; }
-                if (cu[q] < real(-0.5) || cu[q] >= (real)g.S[q] + real(0.5)) flags |= FLAG_REBIN;
-            }
-            if (ok) {
-                key = lc[0] + A0 * lc[1] + A01 * lc[2];
-#pragma unroll
+        // ---- phase A: grid update of step n over the node window = this brick's
+        // touched box of the last P2G (…_MP.cpp:209-265)
+        real *s_node = reinterpret_cast<real *>(s_arena + kQ * AN);
+        int wlo[3] = {0, 0, 0}, wext[3] = {0, 0, 0};
+        int stash = 0;   // reals of the arena the grid update uses as its stash
+        if (G2P) {
+            if (tid < 27) {
+                const int d[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
+                const int x = bc[0] + d[0], y = bc[1] + d[1], z = bc[2] + d[2];
+                int4 bi = make_int4(-1, 0, 0, 0);
+                if (x >= 0 && y >= 0 && z >= 0 && x < g.nb[0] && y < g.nb[1] && z < g.nb[2])
+                    bi = binfo_in[x + (size_t)g.nb[0] * (y + (size_t)g.nb[1] * z)];
+                // a missing neighbour gets an empty window (extent 0)
+                const int pk[3] = {bi.y, bi.z, bi.w};
+                int lo[3], ext[3];
+    #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    bmin[q] = min(bmin[q], lc[q]);
-                    bmax[q] = max(bmax[q], lc[q] + 1);
+                    lo[q] = (pk[q] & 0xFFFF) + d[q] * g.S[q];
+                    ext[q] = (bi.x >= 0) ? (pk[q] >> 16) - (pk[q] & 0xFFFF) + 1 : 0;
                 }
-                if (g.has_fext)
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) fe[q] = elt(P.fext[q], i);
-#pragma unroll
-                for (int q = 0; q < 3; ++q) mv[q] = m * v[q];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) vs[q] = V * s[q];
-                // bounds of |contribution| per field (N <= 1, |dN/dx_j| <= 1/h_j)
-                const float ih0 = (float)bx.ih, ih1 = (float)by.ih, ih2 = (float)bz.ih;
-                bq[0] = fbits(fabsf((float)m) * 1.0001f);
-                bq[1] = fbits(fabsf((float)mv[0]) * 1.0001f);
-                bq[2] = fbits(fabsf((float)mv[1]) * 1.0001f);
-                bq[3] = fbits(fabsf((float)mv[2]) * 1.0001f);
-                bq[4] = fbits((fabsf((float)vs[0]) * ih0 + fabsf((float)vs[3]) * ih1 + fabsf((float)vs[5]) * ih2 + fabsf((float)fe[0])) * 1.0001f);
-                bq[5] = fbits((fabsf((float)vs[3]) * ih0 + fabsf((float)vs[1]) * ih1 + fabsf((float)vs[4]) * ih2 + fabsf((float)fe[1])) * 1.0001f);
-                bq[6] = fbits((fabsf((float)vs[5]) * ih0 + fabsf((float)vs[4]) * ih1 + fabsf((float)vs[2]) * ih2 + fabsf((float)fe[2])) * 1.0001f);
+                s_nb[tid][0] = make_int4((int)((uint32_t)max(bi.x, 0) * 8u * (uint32_t)AN), lo[0], lo[1], lo[2]);
+                s_nb[tid][1] = make_int4(ext[0], ext[1], ext[2], ext[0] * ext[1]);
+            }
+            __syncthreads();
+            const int4 own0 = s_nb[13][0], own1 = s_nb[13][1];
+            wlo[0] = own0.y; wlo[1] = own0.z; wlo[2] = own0.w;
+            wext[0] = own1.x; wext[1] = own1.y; wext[2] = own1.z;
+            const int nwin = wext[0] * wext[1] * wext[2];
+            // window index -> node: quotients through a float reciprocal, exact
+            // for r < 2^20 (the fraction of (r + 1/2) / e stays 1/(2e) away from
+            // an integer, far above the float error)
+            const float rx = 1.0f / (float)max(wext[0], 1), rxy = 1.0f / (float)max(own1.w, 1);
+            const real dtg = (real)st.dt, m_min = (real)g.m_min, damp = (real)g.damping;
+            // Pass 1: for each window node, the sum of the covering arenas'
+            // records (at most 8 candidates, ascending brick order, uncovered ones
+            // read the zero record), its fixity and override tag, stashed in LDS:
+            // sums in the (not yet used) arena, {tag, fixity | counted} in the
+            // node's window slot.  Branch-free, so that all loads of a node are in
+            // flight at once.  Each thread later reads back only the slots it
+            // wrote: no barrier.
+            real *s_sum = reinterpret_cast<real *>(s_arena);
+            stash = kQ * nwin;
+            for (int r = tid; r < nwin; r += kPBlock) {
+                const int lz = (int)(((float)r + 0.5f) * rxy), rem = r - lz * own1.w;
+                const int ly = (int)(((float)rem + 0.5f) * rx), lx = rem - ly * wext[0];
+                const int l[3] = {wlo[0] + lx, wlo[1] + ly, wlo[2] + lz};
+                const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
+                // per axis the arenas covering this node: own and the neighbour
+                // below (l <= 2) or above (l >= S); the candidates are the 2 x 2 x 2
+                // neighbours from `lower` up, and a candidate counts when the node
+                // lies in its window (brick sizes >= 3 keep the two sides apart)
+                int lower[3];
+    #pragma unroll
+                for (int q = 0; q < 3; ++q) lower[q] = (l[q] <= 2 && bc[q] > 0) ? -1 : 0;
+                const int nb0 = (lower[2] + 1) * 9 + (lower[1] + 1) * 3 + (lower[0] + 1);
+                uint32_t eo[8];
+                int first = -1;
+    #pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int4 w0 = s_nb[nb0 + (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1)][0];
+                    const int4 w1 = s_nb[nb0 + (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1)][1];
+                    const uint32_t t0 = (uint32_t)(l[0] - w0.y), t1 = (uint32_t)(l[1] - w0.z), t2 = (uint32_t)(l[2] - w0.w);
+                    const bool ok = t0 < (uint32_t)w1.x && t1 < (uint32_t)w1.y && t2 < (uint32_t)w1.z;
+                    eo[c] = ok ? (uint32_t)w0.x + 8u * (t0 + (uint32_t)w1.x * t1 + (uint32_t)w1.w * t2) : a.zero_off;
+                    first = (ok && first < 0) ? c : first;
+                }
+                const int cown = (lower[0] < 0 ? 1 : 0) + (lower[1] < 0 ? 2 : 0) + (lower[2] < 0 ? 4 : 0);
+                // counted once in the grid sums: by the lowest covering brick
+                const uint32_t fw = (uint32_t)a.fixed[nid] | ((first == cown) ? 0x100u : 0u);
+                const unsigned long long mk = marks_in[nid];
+                real sum[kQ];
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) sum[q] = real(0);
+    #pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    real rec[8];
+                    load8(arena_in + eo[c], rec);
+    #pragma unroll
+                    for (int q = 0; q < kQ; ++q) sum[q] += rec[q];
+                }
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) s_sum[kQ * r + q] = sum[q];
+                unsigned long long *wt = reinterpret_cast<unsigned long long *>(s_node + 6 * r);
+                wt[0] = mk;
+                wt[1] = fw;
+            }
+            // Pass 2: the grid update of each window node (…_MP.cpp:209-265)
+            double gs[3] = {0.0, 0.0, 0.0};   // owned nodes: mass, reaction, active count
+            for (int r = tid; r < nwin; r += kPBlock) {
+                const unsigned long long *wt = reinterpret_cast<const unsigned long long *>(s_node + 6 * r);
+                const unsigned long long mk = wt[0];
+                const uint32_t fw = (uint32_t)wt[1], fx = fw & 0xFFu;
+                // each stash slot is read, then zeroed for the P2G arena, by the
+                // thread that wrote it
+                real sum[kQ];
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) {
+                    sum[q] = s_sum[kQ * r + q];
+                    s_sum[kQ * r + q] = real(0);
+                }
+                // the rest of the last 8-byte arena word that the stash ends in
+                if (FP32 && r == nwin - 1 && (stash & 1)) s_sum[stash] = real(0);
+                const real mn = sum[0];
+                real vn[3], fn[3] = {sum[4], sum[5], sum[6]};
+    #pragma unroll
+                for (int q = 0; q < 3; ++q) vn[q] = (mn > m_min) ? sum[1 + q] / mn : real(0);
+                const real vv = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+                if (sqrt(vv) > real(1.0e-9)) {   // damping (…_MP.cpp:209-222)
+                    const real lf = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+                    const real inv = real(1) / sqrt(vv);
+                    const real sc = damp * lf;
+    #pragma unroll
+                    for (int q = 0; q < 3; ++q) fn[q] -= sc * (vn[q] * inv);
+                }
+                // f / m after damping: the velocity update and the stored
+                // acceleration share it; a force zeroed below gives 0 / m = 0
+                real ac[3];
+    #pragma unroll
+                for (int q = 0; q < 3; ++q) ac[q] = fn[q] / mn;
+                if (mn > m_min)
+    #pragma unroll
+                    for (int q = 0; q < 3; ++q) vn[q] += ac[q] * dtg;
+                double react = 0.0;
+                if (fx & 1) { vn[0] = real(0); ac[0] = real(0); }
+                if (fx & 2) { vn[1] = real(0); react = (double)fn[1]; ac[1] = real(0); }
+                if (fx & 4) { vn[2] = real(0); ac[2] = real(0); }
+                if ((uint32_t)(mk >> 32) == st.epoch + 1u) {   // platen override (…_MP.cpp:245-265)
+                    const int j = (int)(mk & 0xFFFFFFFFull);
+    #pragma unroll
+                    for (int q = 0; q < 3; ++q) { vn[q] = (real)dc_v_in[3 * j + q]; ac[q] = real(0); }
+                }
+                real *wn = s_node + 6 * r;
+    #pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    wn[q] = vn[q];
+                    wn[3 + q] = (fabs(mn) > m_min) ? ac[q] : real(0);
+                }
+                if (fw & 0x100u) {
+                    gs[0] += (double)mn;
+                    gs[1] += react;
+                    gs[2] += (mn > real(0)) ? 1.0 : 0.0;
+                    if (a.fy_slot && (fx & 2)) {
+                        const int lz = (int)(((float)r + 0.5f) * rxy), rem = r - lz * own1.w;
+                        const int ly = (int)(((float)rem + 0.5f) * rx), lx = rem - ly * wext[0];
+                        const int l[3] = {wlo[0] + lx, wlo[1] + ly, wlo[2] + lz};
+                        const long long nid = (O[0] - 1 + l[0]) + (long long)g.nodes[0] * ((O[1] - 1 + l[1]) + (long long)g.nodes[1] * (O[2] - 1 + l[2]));
+                        a.fy_val[a.fy_slot[nid]] = react;
+                    }
+                }
+            }
+            // wave sums to LDS; the block's sum is taken after the barrier that
+            // precedes the point chunks (no barrier of its own)
+    #pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double w = wave_sum_dpp(gs[k]);
+                if (lane == 0) s_red[wv][k] = w;
             }
         }
 
-        // ---- block-wide fixed-point exponents: running max of the bounds.
-        // Chunk c writes and reads buffer c % 3.  After the barrier below a
-        // wave may run ahead into chunk c + 1 and write its maxima while a
-        // slower wave still reads chunk c's: the buffers differ, and no wave
-        // reaches chunk c + 2 (buffer c + 2 = c - 1 mod 3) before every wave
-        // has passed chunk c + 1's barrier, long after its chunk-c read.  Each
-        // wave adds the running max of the chunks before (block-uniform), so
-        // the buffer holds the running max; wave 0 clears buffer c + 2 after
-        // this chunk's barrier, when chunk c - 1's reads of it are complete.
-        const int mbuf = (int)((c0 / kPBlock) % 3u);
-#pragma unroll
-        for (int q = 0; q < kQ; ++q) {
-            const uint32_t w = max(wave_umax_dpp(bq[q]), run_max[q]);
-            if (lane == 0 && w) atomicMax(&s_max[mbuf][q], w);
+        // the arena past the stash (the stash zeroes itself): no barrier between
+        // the grid update and this clear
+        const int clear0 = (stash * (int)sizeof(real) + 7) / 8;
+        for (int i = clear0 + tid; i < kQ * AN; i += kPBlock) s_arena[i] = 0;
+        if (tid < 24) s_max[tid >> 3][tid & 7] = 0u;
+        int H = 6;   // a node takes at most one contribution per point: count < 2^(H-1)
+        while ((1u << (H - 1)) <= count) ++H;
+        const int FRAC = FP32 ? kFrac32 : 61 - H;
+        int ecur[kQ];                        // block-uniform field exponents (value < 2^e)
+        uint32_t run_max[kQ];                // block-uniform running max of the bound bits
+    #pragma unroll
+        for (int q = 0; q < kQ; ++q) run_max[q] = 0u;
+    #pragma unroll
+        for (int q = 0; q < kQ; ++q) ecur[q] = kUnsetExp;
+        double red[5] = {0, 0, 0, 0, 0};     // this lane's running check sums (chunk order)
+        int flags = 0;
+        int bmin[3] = {1 << 20, 1 << 20, 1 << 20}, bmax[3] = {-1, -1, -1};
+        const PArrays<real> &P = a.p;
+        // FP32 position increment scale: dt / h * 2^26 per axis
+        const float qscale[3] = {(float)(dtd * (double)g.ihf[0] * kQOne), (float)(dtd * (double)g.ihf[1] * kQOne),
+                                 (float)(dtd * (double)g.ihf[2] * kQOne)};
+        __syncthreads();
+        if (G2P && tid < 3) {
+            double t = 0.0;
+            for (int w = 0; w < kPBlock / 64; ++w) t += s_red[w][tid];
+            a.gpart[3 * (size_t)slot + tid] = t;
+        }
+
+        for (uint32_t c0 = 0; c0 < count; c0 += kPBlock) {
+            const bool active = (c0 + tid) < count;
+            const uint32_t i = start + c0 + tid;
+            int key = -1;
+            Axis<real> bx, by, bz;
+            real m = real(0), mv[3] = {real(0), real(0), real(0)}, vs[6], fe[3] = {real(0), real(0), real(0)};
+            uint32_t bq[kQ] = {0, 0, 0, 0, 0, 0, 0};
+    #pragma unroll
+            for (int q = 0; q < 6; ++q) vs[q] = real(0);
+
+            if (active) {
+                real x[3], v[3], F[9], s[6];
+                int32_t xq[3];
+    #pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (FP32) xq[q] = elt(P.xq[q], i);
+                    else x[q] = elt(P.x[q], i);
+                    v[q] = elt(P.v[q], i);
+                }
+    #pragma unroll
+                for (int q = 0; q < 9; ++q) F[q] = elt(P.F[q], i);
+    #pragma unroll
+                for (int q = 0; q < 6; ++q) s[q] = elt(P.s[q], i);
+                m = elt(P.mass, i);
+                const real V0 = elt(P.vol0, i);
+                real al_pre = real(0);
+                if (pre_al) al_pre = elt(P.alpha, i);
+                real V = real(0);
+
+                if (G2P) {
+                    // ---- G2P of step n (…_MP.cpp:279-297)
+                    axes_of(x, xq, g, O, bx, by, bz);
+                    const bool oob = bx.cell < 0 || bx.cell >= g.cells[0] || by.cell < 0 || by.cell >= g.cells[1] ||
+                                     bz.cell < 0 || bz.cell >= g.cells[2];
+                    // the 8 corner nodes in the window (the P2G of step n put them there)
+                    const int wl[3] = {bx.cell - (O[0] - 1) - wlo[0], by.cell - (O[1] - 1) - wlo[1], bz.cell - (O[2] - 1) - wlo[2]};
+                    const bool outw = wl[0] < 0 || wl[0] + 1 >= wext[0] || wl[1] < 0 || wl[1] + 1 >= wext[1] ||
+                                      wl[2] < 0 || wl[2] + 1 >= wext[2];
+                    if (oob || outw) {
+                        flags |= oob ? FLAG_OUT_OF_GRID : FLAG_ARENA;
+                    } else {
+                        const real *w0 = s_node + 6 * (wl[0] + wext[0] * (wl[1] + wext[1] * wl[2]));
+                        const int sy_ = wext[0], sz_ = wext[0] * wext[1];
+                        real L[9];
+    #pragma unroll
+                        for (int q = 0; q < 9; ++q) L[q] = real(0);
+                        real dx[3] = {real(0), real(0), real(0)};
+                        if constexpr (FP32) {
+                            // 8 corners fully unrolled; node record {v.xyz a.x | a.y a.z m -}
+                            const Hat32 w = hat32(bx, by, bz);
+                            float av[3] = {0.f, 0.f, 0.f};
+    #pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                const int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
+                                const float N = w.wxy[2 * ax + ay] * w.wz[az];
+                                const float G[3] = {w.gx[ax] * w.wyz[2 * ay + az], w.gy[ay] * w.wxz[2 * ax + az],
+                                                    w.gz[az] * w.wxy[2 * ax + ay]};
+                                const real *wp = w0 + 6 * (ax + sy_ * ay + sz_ * az);
+                                const float vn[3] = {wp[0], wp[1], wp[2]}, an[3] = {wp[3], wp[4], wp[5]};
+    #pragma unroll
+                                for (int q = 0; q < 3; ++q) {
+                                    av[q] += N * an[q];
+                                    dx[q] += N * vn[q];
+                                }
+    #pragma unroll
+                                for (int cc = 0; cc < 3; ++cc)
+    #pragma unroll
+                                    for (int r = 0; r < 3; ++r) L[3 * cc + r] += vn[r] * G[cc];
+                            }
+    #pragma unroll
+                            for (int q = 0; q < 3; ++q) {
+                                v[q] += av[q] * dt;
+                                xq[q] += __float2int_rn((float)dx[q] * qscale[q]);
+                            }
+                        } else {
+    #pragma unroll 2
+                            for (int k = 0; k < 8; ++k) {
+                                real N, G[3];
+                                corner(bx, by, bz, k, N, G);
+                                const real *wp = w0 + 6 * (((k >> 2) & 1) + sy_ * ((k >> 1) & 1) + sz_ * (k & 1));
+    #pragma unroll
+                                for (int q = 0; q < 3; ++q) v[q] += N * wp[3 + q] * dt;
+    #pragma unroll
+                                for (int q = 0; q < 3; ++q) x[q] += N * wp[q] * dt;
+    #pragma unroll
+                                for (int cc = 0; cc < 3; ++cc)
+    #pragma unroll
+                                    for (int r = 0; r < 3; ++r) L[3 * cc + r] += wp[r] * G[cc];
+                            }
+                        }
+                        // F += (L F) dt  (…_MP.cpp:299); FP32 state: D += (L + L D) dt
+                        real T[9];
+    #pragma unroll
+                        for (int cc = 0; cc < 3; ++cc)
+    #pragma unroll
+                            for (int r = 0; r < 3; ++r)
+                                T[3 * cc + r] = (FP32 ? L[3 * cc + r] : real(0)) +
+                                                (L[r] * F[3 * cc] + L[3 + r] * F[3 * cc + 1] + L[6 + r] * F[3 * cc + 2]);
+    #pragma unroll
+                        for (int q = 0; q < 9; ++q) F[q] += T[q] * dt;
+                        // volume from the new F (…_MP.cpp:301-302), FP64
+                        {
+                            double Fd[9];
+    #pragma unroll
+                            for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
+                            V = (real)(det3(Fd) * (double)V0);
+                        }
+                        // write back the kinematic state before the material update
+    #pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            if (FP32) elt(P.xq[q], i) = xq[q];
+                            else elt(P.x[q], i) = x[q];
+                            elt(P.v[q], i) = v[q];
+                        }
+    #pragma unroll
+                        for (int q = 0; q < 9; ++q) elt(P.F[q], i) = F[q];
+                        material_update<real>(g, P, a.mats, i, L, dt, F, s, umat, pre_al, al_pre);
+                    }
+                } else {
+                    double Fd[9];
+    #pragma unroll
+                    for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
+                    V = (real)(det3(Fd) * (double)V0);
+                }
+                // ---- FP64 checks of the new state (PhysicsEngine.cpp:73-79)
+                {
+                    const double md = (double)m, vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
+                    const double vv = vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2];
+                    red[0] += md * vv;
+                    red[1] += md * vd[0];
+                    red[2] += md * vd[1];
+                    red[3] += md * vd[2];
+                    const real ssum = s[0] + s[1] + s[2] + s[3] + s[4] + s[5];
+                    const bool fin = isfinite(vv) && isfinite(ssum) && (FP32 || isfinite(x[0] + x[1] + x[2]));
+                    if (!fin) { red[4] += 1.0; flags |= FLAG_NAN; }
+                }
+                // ---- P2G inputs of step n+1 (…_MP.cpp:106-185)
+                axes_of(x, xq, g, O, bx, by, bz);
+                const int cl[3] = {bx.cell, by.cell, bz.cell};
+                const real cu[3] = {bx.u, by.u, bz.u};
+                int lc[3];
+                bool ok = true;
+    #pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    lc[q] = cl[q] - (O[q] - 1);
+                    if (cl[q] < 0 || cl[q] >= g.cells[q]) { flags |= FLAG_OUT_OF_GRID; ok = false; }
+                    if (lc[q] < 0 || lc[q] > g.S[q] + 1) { flags |= FLAG_ARENA; ok = false; }
+                    if (cu[q] < real(-0.5) || cu[q] >= (real)g.S[q] + real(0.5)) flags |= FLAG_REBIN;
+                }
+                if (ok) {
+                    key = lc[0] + A0 * lc[1] + A01 * lc[2];
+    #pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        bmin[q] = min(bmin[q], lc[q]);
+                        bmax[q] = max(bmax[q], lc[q] + 1);
+                    }
+                    if (g.has_fext)
+    #pragma unroll
+                        for (int q = 0; q < 3; ++q) fe[q] = elt(P.fext[q], i);
+    #pragma unroll
+                    for (int q = 0; q < 3; ++q) mv[q] = m * v[q];
+    #pragma unroll
+                    for (int q = 0; q < 6; ++q) vs[q] = V * s[q];
+                    // bounds of |contribution| per field (N <= 1, |dN/dx_j| <= 1/h_j)
+                    const float ih0 = (float)bx.ih, ih1 = (float)by.ih, ih2 = (float)bz.ih;
+                    bq[0] = fbits(fabsf((float)m) * 1.0001f);
+                    bq[1] = fbits(fabsf((float)mv[0]) * 1.0001f);
+                    bq[2] = fbits(fabsf((float)mv[1]) * 1.0001f);
+                    bq[3] = fbits(fabsf((float)mv[2]) * 1.0001f);
+                    bq[4] = fbits((fabsf((float)vs[0]) * ih0 + fabsf((float)vs[3]) * ih1 + fabsf((float)vs[5]) * ih2 + fabsf((float)fe[0])) * 1.0001f);
+                    bq[5] = fbits((fabsf((float)vs[3]) * ih0 + fabsf((float)vs[1]) * ih1 + fabsf((float)vs[4]) * ih2 + fabsf((float)fe[1])) * 1.0001f);
+                    bq[6] = fbits((fabsf((float)vs[5]) * ih0 + fabsf((float)vs[4]) * ih1 + fabsf((float)vs[2]) * ih2 + fabsf((float)fe[2])) * 1.0001f);
+                }
+            }
+
+            // ---- block-wide fixed-point exponents: running max of the bounds.
+            // Chunk c writes and reads buffer c % 3.  After the barrier below a
+            // wave may run ahead into chunk c + 1 and write its maxima while a
+            // slower wave still reads chunk c's: the buffers differ, and no wave
+            // reaches chunk c + 2 (buffer c + 2 = c - 1 mod 3) before every wave
+            // has passed chunk c + 1's barrier, long after its chunk-c read.  Each
+            // wave adds the running max of the chunks before (block-uniform), so
+            // the buffer holds the running max; wave 0 clears buffer c + 2 after
+            // this chunk's barrier, when chunk c - 1's reads of it are complete.
+            const int mbuf = (int)((c0 / kPBlock) % 3u);
+    #pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const uint32_t w = max(wave_umax_dpp(bq[q]), run_max[q]);
+                if (lane == 0 && w) atomicMax(&s_max[mbuf][q], w);
+            }
+            __syncthreads();
+            if (tid < 8) s_max[mbuf == 0 ? 2 : mbuf - 1][tid] = 0u;
+            {
+                bool any = false;
+                int sh[kQ];
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) {
+                    const uint32_t mb = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_max[mbuf][q]);
+                    run_max[q] = mb;
+                    // positive float bits: biased exponent + 1 bounds the value
+                    const int e = (mb == 0u) ? kUnsetExp : max((int)(mb >> 23) - 126, kMinExp);
+                    const int en = __builtin_amdgcn_readfirstlane(e);
+                    sh[q] = (ecur[q] != kUnsetExp && en > ecur[q]) ? en - ecur[q] : 0;
+                    any |= sh[q] != 0;
+                    ecur[q] = en;
+                }
+                if (any) {   // the exponent grew: rescale what the arena holds
+                    for (int idx = tid; idx < kQ * AN; idx += kPBlock) {
+                        int shq = 0;
+    #pragma unroll
+                        for (int q = 0; q < kQ; ++q) shq = (idx % kQ == q) ? sh[q] : shq;
+                        if (shq) s_arena[idx] = (shq >= 63) ? (s_arena[idx] >> 63) : (s_arena[idx] >> shq);
+                    }
+                    __syncthreads();
+                }
+            }
+
+            // ---- P2G: every lane adds its 8 x 7 contributions (fixed point)
+            if (key >= 0) {
+                real sc[kQ];
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) {
+                    if constexpr (FP32)
+                        sc[q] = (ecur[q] == kUnsetExp) ? 0.f : __int_as_float((127 + FRAC - ecur[q]) << 23);
+                    else
+                        sc[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, FRAC - ecur[q]);
+                }
+                // scaled per-point factors (power-of-two scaling is exact)
+                const real ms = m * sc[0], mvs[3] = {mv[0] * sc[1], mv[1] * sc[2], mv[2] * sc[3]};
+                const real f4[4] = {-vs[0] * sc[4], -vs[3] * sc[4], -vs[5] * sc[4], fe[0] * sc[4]};
+                const real f5[4] = {-vs[3] * sc[5], -vs[1] * sc[5], -vs[4] * sc[5], fe[1] * sc[5]};
+                const real f6[4] = {-vs[5] * sc[6], -vs[4] * sc[6], -vs[2] * sc[6], fe[2] * sc[6]};
+                long long *rowp[4];   // arena records of the (ay, az) corner rows
+    #pragma unroll
+                for (int r = 0; r < 4; ++r) rowp[r] = s_arena + kQ * (key + A0 * (r >> 1) + A01 * (r & 1));
+                Hat32 w;
+                if constexpr (FP32) w = hat32(bx, by, bz);
+    #pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
+                    real N, G[3];
+                    if constexpr (FP32) {
+                        N = w.wxy[2 * ax + ay] * w.wz[az];
+                        G[0] = w.gx[ax] * w.wyz[2 * ay + az];
+                        G[1] = w.gy[ay] * w.wxz[2 * ax + az];
+                        G[2] = w.gz[az] * w.wxy[2 * ax + ay];
+                    } else {
+                        corner(bx, by, bz, k, N, G);
+                    }
+                    real c[kQ];
+                    c[0] = N * ms;
+                    c[1] = N * mvs[0];
+                    c[2] = N * mvs[1];
+                    c[3] = N * mvs[2];
+                    c[4] = (G[0] * f4[0] + G[1] * f4[1] + G[2] * f4[2]) + N * f4[3];
+                    c[5] = (G[1] * f5[1] + G[0] * f5[0] + G[2] * f5[2]) + N * f5[3];
+                    c[6] = (G[2] * f6[2] + G[0] * f6[0] + G[1] * f6[1]) + N * f6[3];
+                    unsigned long long *rec = reinterpret_cast<unsigned long long *>(rowp[2 * ay + az] + kQ * ax);
+    #pragma unroll
+                    for (int q = 0; q < kQ; ++q) {
+                        long long iv;
+                        if constexpr (FP32) iv = (long long)__float2int_rn((float)c[q]);
+                        else iv = __double2ll_rn((double)c[q]);
+                        atomicAdd(rec + q, (unsigned long long)iv);
+                    }
+                }
+            }
+        }
+
+        // ---- touched node box of the brick: wave bounds first, so that one
+        // barrier covers both them and the arena adds
+    #pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int lo = wave_imin_dpp(bmin[q]), hi = wave_imax_dpp(bmax[q]);
+            if (lane == 0) { s_box[wv][2 * q] = lo; s_box[wv][2 * q + 1] = hi; }
         }
         __syncthreads();
-        if (tid < 8) s_max[mbuf == 0 ? 2 : mbuf - 1][tid] = 0u;
-        {
-            bool any = false;
-            int sh[kQ];
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                const uint32_t mb = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_max[mbuf][q]);
-                run_max[q] = mb;
-                // positive float bits: biased exponent + 1 bounds the value
-                const int e = (mb == 0u) ? kUnsetExp : max((int)(mb >> 23) - 126, kMinExp);
-                const int en = __builtin_amdgcn_readfirstlane(e);
-                sh[q] = (ecur[q] != kUnsetExp && en > ecur[q]) ? en - ecur[q] : 0;
-                any |= sh[q] != 0;
-                ecur[q] = en;
-            }
-            if (any) {   // the exponent grew: rescale what the arena holds
-                for (int idx = tid; idx < kQ * AN; idx += kPBlock) {
-                    int shq = 0;
-#pragma unroll
-                    for (int q = 0; q < kQ; ++q) shq = (idx % kQ == q) ? sh[q] : shq;
-                    if (shq) s_arena[idx] = (shq >= 63) ? (s_arena[idx] >> 63) : (s_arena[idx] >> shq);
-                }
-                __syncthreads();
-            }
+        int box[6];
+    #pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            int lo = 1 << 20, hi = -1;
+            for (int w = 0; w < kPBlock / 64; ++w) { lo = min(lo, s_box[w][2 * q]); hi = max(hi, s_box[w][2 * q + 1]); }
+            box[2 * q] = lo;
+            box[2 * q + 1] = hi;
+        }
+        if (tid == 0) {
+            const bool empty = box[1] < box[0] || box[3] < box[2] || box[5] < box[4];
+            binfo_out[brick] = make_int4(empty ? -1 : slot, box[0] | (box[1] << 16), box[2] | (box[3] << 16),
+                                       box[4] | (box[5] << 16));
         }
 
-        // ---- P2G: every lane adds its 8 x 7 contributions (fixed point)
-        if (key >= 0) {
-            real sc[kQ];
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                if constexpr (FP32)
-                    sc[q] = (ecur[q] == kUnsetExp) ? 0.f : __int_as_float((127 + FRAC - ecur[q]) << 23);
-                else
-                    sc[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, FRAC - ecur[q]);
-            }
-            // scaled per-point factors (power-of-two scaling is exact)
-            const real ms = m * sc[0], mvs[3] = {mv[0] * sc[1], mv[1] * sc[2], mv[2] * sc[3]};
-            const real f4[4] = {-vs[0] * sc[4], -vs[3] * sc[4], -vs[5] * sc[4], fe[0] * sc[4]};
-            const real f5[4] = {-vs[3] * sc[5], -vs[1] * sc[5], -vs[4] * sc[5], fe[1] * sc[5]};
-            const real f6[4] = {-vs[5] * sc[6], -vs[4] * sc[6], -vs[2] * sc[6], fe[2] * sc[6]};
-            long long *rowp[4];   // arena records of the (ay, az) corner rows
-#pragma unroll
-            for (int r = 0; r < 4; ++r) rowp[r] = s_arena + kQ * (key + A0 * (r >> 1) + A01 * (r & 1));
-            Hat32 w;
-            if constexpr (FP32) w = hat32(bx, by, bz);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
-                real N, G[3];
-                if constexpr (FP32) {
-                    N = w.wxy[2 * ax + ay] * w.wz[az];
-                    G[0] = w.gx[ax] * w.wyz[2 * ay + az];
-                    G[1] = w.gy[ay] * w.wxz[2 * ax + az];
-                    G[2] = w.gz[az] * w.wxy[2 * ax + ay];
-                } else {
-                    corner(bx, by, bz, k, N, G);
-                }
-                real c[kQ];
-                c[0] = N * ms;
-                c[1] = N * mvs[0];
-                c[2] = N * mvs[1];
-                c[3] = N * mvs[2];
-                c[4] = (G[0] * f4[0] + G[1] * f4[1] + G[2] * f4[2]) + N * f4[3];
-                c[5] = (G[1] * f5[1] + G[0] * f5[0] + G[2] * f5[2]) + N * f5[3];
-                c[6] = (G[2] * f6[2] + G[0] * f6[0] + G[1] * f6[1]) + N * f6[3];
-                unsigned long long *rec = reinterpret_cast<unsigned long long *>(rowp[2 * ay + az] + kQ * ax);
-#pragma unroll
-                for (int q = 0; q < kQ; ++q) {
-                    long long iv;
-                    if constexpr (FP32) iv = (long long)__float2int_rn((float)c[q]);
-                    else iv = __double2ll_rn((double)c[q]);
-                    atomicAdd(rec + q, (unsigned long long)iv);
-                }
+        // ---- arena out (fixed point -> real) over the touched box only, as
+        // box-compact node records of 8 reals {m, p.xyz, f.xyz, 0}
+        if (box[1] >= box[0]) {
+            const int bx = box[1] - box[0] + 1, by = box[3] - box[2] + 1, bz = box[5] - box[4] + 1;
+            const int nbox = bx * by * bz;
+            real *out = arena_out + (size_t)slot * 8 * AN;
+            double scl[kQ];
+    #pragma unroll
+            for (int q = 0; q < kQ; ++q) scl[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, ecur[q] - FRAC);
+            // record index -> node as in the grid update (float quotients, exact here)
+            const float rbx = 1.0f / (float)bx, rbxy = 1.0f / (float)(bx * by);
+            for (int r = tid; r < nbox; r += kPBlock) {
+                const int qz = (int)(((float)r + 0.5f) * rbxy), rem = r - qz * (bx * by);
+                const int qy = (int)(((float)rem + 0.5f) * rbx);
+                const int lx = box[0] + rem - qy * bx, ly = box[2] + qy, lz = box[4] + qz;
+                const long long *src = s_arena + kQ * (lx + A0 * ly + A01 * lz);
+                real rec[8];
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) rec[q] = (real)((double)src[q] * scl[q]);
+                rec[7] = real(0);
+                store8(out + 8 * (size_t)r, rec);
             }
         }
-    }
-
-    // ---- touched node box of the brick: wave bounds first, so that one
-    // barrier covers both them and the arena adds
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int lo = wave_imin_dpp(bmin[q]), hi = wave_imax_dpp(bmax[q]);
-        if (lane == 0) { s_box[wv][2 * q] = lo; s_box[wv][2 * q + 1] = hi; }
-    }
-    __syncthreads();
-    int box[6];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        int lo = 1 << 20, hi = -1;
-        for (int w = 0; w < kPBlock / 64; ++w) { lo = min(lo, s_box[w][2 * q]); hi = max(hi, s_box[w][2 * q + 1]); }
-        box[2 * q] = lo;
-        box[2 * q + 1] = hi;
-    }
-    if (tid == 0) {
-        const bool empty = box[1] < box[0] || box[3] < box[2] || box[5] < box[4];
-        binfo_out[brick] = make_int4(empty ? -1 : slot, box[0] | (box[1] << 16), box[2] | (box[3] << 16),
-                                   box[4] | (box[5] << 16));
-    }
-
-    // ---- arena out (fixed point -> real) over the touched box only, as
-    // box-compact node records of 8 reals {m, p.xyz, f.xyz, 0}
-    if (box[1] >= box[0]) {
-        const int bx = box[1] - box[0] + 1, by = box[3] - box[2] + 1, bz = box[5] - box[4] + 1;
-        const int nbox = bx * by * bz;
-        real *out = arena_out + (size_t)slot * 8 * AN;
-        double scl[kQ];
-#pragma unroll
-        for (int q = 0; q < kQ; ++q) scl[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, ecur[q] - FRAC);
-        // record index -> node as in the grid update (float quotients, exact here)
-        const float rbx = 1.0f / (float)bx, rbxy = 1.0f / (float)(bx * by);
-        for (int r = tid; r < nbox; r += kPBlock) {
-            const int qz = (int)(((float)r + 0.5f) * rbxy), rem = r - qz * (bx * by);
-            const int qy = (int)(((float)rem + 0.5f) * rbx);
-            const int lx = box[0] + rem - qy * bx, ly = box[2] + qy, lz = box[4] + qz;
-            const long long *src = s_arena + kQ * (lx + A0 * ly + A01 * lz);
-            real rec[8];
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) rec[q] = (real)((double)src[q] * scl[q]);
-            rec[7] = real(0);
-            store8(out + 8 * (size_t)r, rec);
+        // ---- check sums: one fixed-order partial per wave, written by the wave
+        // itself (no block barrier at the end); flags: OR over the wave, only when
+        // a lane has one
+    #pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double wsum = wave_sum_dpp(red[k]);
+            if (lane == 0) a.kp_part[5 * ((size_t)slot * (kPBlock / 64) + wv) + k] = wsum;
         }
-    }
-    // ---- check sums: one fixed-order partial per wave, written by the wave
-    // itself (no block barrier at the end); flags: OR over the wave, only when
-    // a lane has one
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const double wsum = wave_sum_dpp(red[k]);
-        if (lane == 0) a.kp_part[5 * ((size_t)slot * (kPBlock / 64) + wv) + k] = wsum;
-    }
-    if (__ballot(flags != 0)) {
-        int fl = flags;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
-        if (lane == 0) atomicOr(a.flags, fl);
+        if (__ballot(flags != 0)) {
+            int fl = flags;
+    #pragma unroll
+            for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
+            if (lane == 0) atomicOr(a.flags, fl);
+        }
+        if constexpr (!PERSIST) {
+            return;
+        } else {
+            __syncthreads();   // the next item reuses the LDS
+        }
     }
 }
 

@@ -38,7 +38,7 @@ class PhysicsEngine:
 
     def __init__(self, scene, precision=FP64, device=0, brick=(0, 0, 0), drift_check=8,
                  track_strain=True, strain_form=0, verbose=False, friction=0.0, profile=False,
-                 graph=True):
+                 graph=True, persistent=False):
         self.scene = scene
         L = lib()
         cfg = MpmConfig()
@@ -69,6 +69,7 @@ class PhysicsEngine:
         cfg.verbose = 1 if verbose else 0
         cfg.profile = 1 if profile else 0
         cfg.launch = _abi.MPM_LAUNCH_GRAPH if graph else _abi.MPM_LAUNCH_DIRECT
+        cfg.persistent = 1 if persistent else 0
         if scene.node_group is not None and scene.n_groups > 0:
             self._group = np.ascontiguousarray(scene.node_group, np.uint16)
             cfg.node_group = self._group.ctypes.data_as(C.POINTER(C.c_uint16))

@@ -114,6 +114,11 @@ typedef struct mpm_config {
                                   NULL; copied at mpm_create.  Groups must not
                                   share nodes (keep 2 empty cells between them) */
     int32_t n_groups;          /* number of groups when node_group != NULL      */
+    int32_t persistent;        /* 1: plain steps as a persistent launch -- a
+                                  resident grid of blocks pulls bricks from
+                                  per-XCD work queues (batched worlds: all
+                                  cells' bricks in one queue set); 0: one
+                                  block per brick.  Results are identical.   */
 } mpm_config;
 
 /* Host point arrays, FP64, caller-allocated, original point order.  Mirrors
