@@ -497,6 +497,8 @@ int or_step_plain(or_world *w, double dt)
             if (fabs(m) > w->m_min)
                 for (int a = 0; a < 3; ++a) v[a] += N * (fI[a] / m) * dt;
             for (int a = 0; a < 3; ++a) x[a] += N * vI[a] * dt;
+            if (w->u)
+                for (int a = 0; a < 3; ++a) w->u[3 * p + a] += N * vI[a] * dt;
             for (int c = 0; c < 3; ++c)
                 for (int r = 0; r < 3; ++r) L[3 * c + r] += vI[r] * G[c];
         }
@@ -509,6 +511,8 @@ int or_step_plain(or_world *w, double dt)
         int p = w->dc_list[j];
         if (w->n_tl != 0) or_timeline_velocity(w->n_tl, w->tl_t, w->tl_v, w->time, &w->v[3 * p]);
         for (int a = 0; a < 3; ++a) w->x[3 * p + a] += w->v[3 * p + a] * dt;
+        if (w->u)
+            for (int a = 0; a < 3; ++a) w->u[3 * p + a] += w->v[3 * p + a] * dt;
     }
     return OR_OK;
 }
@@ -756,6 +760,8 @@ int or_step_contact(or_world *w, double dt)
             if (len_m3(m) > w->m_min)
                 for (int a = 0; a < 3; ++a) v[a] += N * (fI[a] / m) * dt;
             for (int a = 0; a < 3; ++a) x[a] += N * vI[a] * dt;
+            if (w->u)
+                for (int a = 0; a < 3; ++a) w->u[3 * p + a] += N * vI[a] * dt;
             for (int c = 0; c < 3; ++c)
                 for (int r = 0; r < 3; ++r) L[3 * c + r] += vI[r] * G[c];
         }
@@ -767,6 +773,8 @@ int or_step_contact(or_world *w, double dt)
     for (int j = 0; j < w->n_dc; ++j) {
         int p = w->dc_list[j];
         for (int a = 0; a < 3; ++a) w->x[3 * p + a] += w->v[3 * p + a] * dt;
+        if (w->u)
+            for (int a = 0; a < 3; ++a) w->u[3 * p + a] += w->v[3 * p + a] * dt;
     }
     return OR_OK;
 }

@@ -84,6 +84,10 @@ typedef struct or_world {
     double *thread_grid;           /* nthreads * nnodes * 7 doubles (m, v3, f3) */
     int strain_form;               /* 0: reference ((I+L dt)-I); 1: L dt      */
     int neohookean_dispatch;       /* 0: reference (falls to PerfectlyPlastic) */
+    /* displacement u = x - x0 carried as its own state (3 per point, may be
+     * NULL): the same increments as x, so that small displacements keep the
+     * digits a difference of large positions loses */
+    double *u;
 } or_world;
 
 /* shape function and neighbours ------------------------------------------ */

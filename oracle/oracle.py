@@ -37,7 +37,7 @@ class OrWorld(C.Structure):
         ("time", C.c_double), ("time_end", C.c_double), ("cycle", C.c_longlong),
         ("n_tl", C.c_int), ("tl_t", _dp), ("tl_v", _dp),
         ("nthreads", C.c_int), ("thread_grid", _dp),
-        ("strain_form", C.c_int), ("neohookean_dispatch", C.c_int),
+        ("strain_form", C.c_int), ("neohookean_dispatch", C.c_int), ("u", _dp),
     ]
 
 
@@ -207,6 +207,8 @@ class Oracle:
         w.nthreads = nthreads
         w.strain_form = strain_form
         w.neohookean_dispatch = int(s.neohookean)
+        a["u"] = np.zeros((n, 3))
+        w.u = _p(a["u"])
         self.w = w
         self.reaction = []
 
@@ -247,4 +249,4 @@ class Oracle:
     def points(self):
         a = self.a
         return dict(x=a["x"], v=a["v"], F=a["F"], stress=a["stress"], strain=a["strain"], eps_p=a["eps_p"],
-                    alpha=a["alpha"], vol=a["vol"])
+                    alpha=a["alpha"], vol=a["vol"], u=a["u"])

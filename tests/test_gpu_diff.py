@@ -190,6 +190,38 @@ def test_config5_gradient_on_64_directions():
         assert worst <= 1e-4
 
 
+def test_config5_gradient_vs_fp64_cpu_oracle():
+    """The full config-5 run (about 160k points, 400 steps, checkpoints every
+    20): adjoint directional derivatives against central differences of the
+    FP64 CPU oracle on 2 random directions.  The oracle carries the
+    displacement as its own state, as the differentiable run does, so the
+    loss keeps the digits that x_final - x0 would lose."""
+    import os
+    sc = scene.config5_world()
+    sets = diff.boundary_sets(sc)
+    n_steps = 400
+    with diff.DiffRun(sc, n_steps, checkpoint=20) as run:
+        _, gE, gx = adjoint_grads(run, sc, sets)
+    threads = max(1, min(16, os.cpu_count() or 1))
+
+    def loss(E, x0):
+        s = sc.copy()
+        s.E = np.asarray(E, float).copy()
+        s.x = np.asarray(x0, float).copy()
+        o = Oracle(s, strain_form=1, nthreads=threads)
+        for _ in range(n_steps):
+            o.step(s.dt_max)
+            o.w.time += s.dt_max
+        return float(diff.poisson_loss(s.x, o.points()["u"], sets))
+
+    for k, (dE, dx) in enumerate(directions(sc, 2, 29)):
+        fd = (loss(sc.E + EPS * dE, sc.x + EPS * dx) - loss(sc.E - EPS * dE, sc.x - EPS * dx)) / (2 * EPS)
+        ad = float(gE @ dE) + float((gx * dx).sum())
+        print(f"config5 direction {k}: adjoint {ad:.9e} oracle FD {fd:.9e} rel {abs(fd - ad) / abs(ad):.3e}",
+              flush=True)
+        assert abs(fd - ad) <= 1e-4 * abs(ad)
+
+
 LAWS = ["j2", "plastic", "neohookean"]
 
 

@@ -185,6 +185,8 @@ def test_free_flight(O):
     p = orc.points()
     assert np.allclose(p["x"], sc.x + sc.v * sc.dt_max, rtol=0, atol=1e-17)
     assert np.allclose(p["v"], sc.v, rtol=0, atol=1e-15)
+    # the displacement state takes the same increments: u = v dt exactly to rounding
+    assert np.allclose(p["u"], sc.v * sc.dt_max, rtol=1e-13, atol=0)
     assert np.abs(p["stress"]).max() < 1e-6
 
 
