@@ -20,6 +20,7 @@
 namespace mpm {
 
 constexpr uint32_t CF_ACTIVE = 1, CF_KACTIVE = 2, CF_CPOS = 4, CF_CNEG = 8;
+constexpr uint32_t CF_GATE0 = 16;   // CF_GATE0 << f: field f's node mass passes the m_min gate
 constexpr int kCB = 6;   // bounds: m, m/h, |v|, |force|, 1 (kernel), spare
 
 template <typename real>
@@ -42,8 +43,10 @@ struct CArgs {
     double *gm, *gv, *gf;         // 2 x nn, 6 x nn, 6 x nn (field-major: field, component)
     uint32_t *flags;              // CF_* per node
     unsigned long long *near_d, *sg_d;
-    uint32_t *near_i, *sg_i;
+    unsigned long long *near_i, *sg_i;   // (point id << 32) | current index of the winner
     uint8_t *near_surf;
+    uint8_t *fmask;               // per point (current order, platens after deformable): field of each corner
+    double *ga;                   // 6 x nn: gated node acceleration f / m per field (field-major)
     unsigned long long *cmarks;   // platen tags of this step
     unsigned long long *bounds;   // kCB positive doubles as bits
     double *kp, *kg;              // per point (current order, platens after deformable): k, grad k (3 x n_all)
@@ -240,7 +243,10 @@ struct CWin {
     int lo[3], dim[3], nodes;
     bool lds;
 };
-// nodes i0 - below .. i0 + above of every valid point's cell i0, at most cap
+// nodes i0 - below .. i0 + above of every valid point's cell i0, at most cap.
+// The cell bounds are reduced over each wave first (DPP), so one lane per
+// wave updates the block's box: per-lane atomics on one LDS word would be
+// serialized lane by lane.  Call with every thread of the block.
 __device__ __forceinline__ CWin c_window(bool valid, const int i0[3], int *sbox, int below = 0, int above = 1,
                                          int cap = kCWin)
 {
@@ -248,11 +254,17 @@ __device__ __forceinline__ CWin c_window(bool valid, const int i0[3], int *sbox,
         sbox[threadIdx.x] = 0x7FFFFFFF;
         sbox[3 + threadIdx.x] = -1;
     }
+    int lo[3], hi[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = wave_imin_dpp(valid ? i0[q] : (1 << 20));
+        hi[q] = wave_imax_dpp(valid ? i0[q] : -1);
+    }
     __syncthreads();
-    if (valid)
+    if ((threadIdx.x & 63) == 0 && hi[0] >= 0)
         for (int q = 0; q < 3; ++q) {
-            atomicMin(&sbox[q], i0[q]);
-            atomicMax(&sbox[3 + q], i0[q]);
+            atomicMin(&sbox[q], lo[q]);
+            atomicMax(&sbox[3 + q], hi[q]);
         }
     __syncthreads();
     CWin w{};
@@ -295,51 +307,51 @@ __global__ void __launch_bounds__(256) k_c_clear(CArgs<real> a)
         for (int q = 0; q < 3; ++q) a.gkgrad[q * a.nn + I] = a.gkg[q * a.nn + I] = 0.0;
         a.flags[I] = 0u;
         a.near_d[I] = 0x7FF0000000000000ull;   // +inf
-        a.near_i[I] = 0xFFFFFFFFu;
+        a.near_i[I] = ~0ull;
         a.sg_d[I] = 0ull;
-        a.sg_i[I] = 0xFFFFFFFFu;
+        a.sg_i[I] = ~0ull;
         a.near_surf[I] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x < kCB) a.bounds[threadIdx.x] = 0ull;
+    // bound slot 4 is the kernel density's (contributions <= 1)
+    if (blockIdx.x == 0 && threadIdx.x < kCB)
+        a.bounds[threadIdx.x] = threadIdx.x == 4 ? (unsigned long long)__double_as_longlong(1.0) : 0ull;
 }
 
-// ---- bounds of the contributions of this step (wave maxima, one atomic per
-// wave and bound) ----------------------------------------------------------------
+// ---- bounds of the contributions of this step: per point, then block maxima
+// (one atomic per block and bound, only when the block beats the current one)
 template <typename real>
-__global__ void __launch_bounds__(256) k_c_bounds(CArgs<real> a)
+__device__ __forceinline__ void c_point_bounds(const CArgs<real> &a, int j, double b[4])
 {
-    const int j = blockIdx.x * 256 + threadIdx.x;
     const GridParams &g = a.g;
-    double b[4] = {0.0, 0.0, 0.0, 0.0};
-    if (j < a.n_all) {
-        const double ihm = fmax(1.0 / g.h[0], fmax(1.0 / g.h[1], 1.0 / g.h[2]));
-        double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
-        if (j >= a.n_def) {
-            const int d = j - a.n_def;
-            m = a.dc_m[d];
-            vol = a.dc_vol[d];
-            for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
-            for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
-        } else {
-            m = (double)a.p.mass[j];
-            double Fd[9];
-            for (int q = 0; q < 9; ++q) Fd[q] = fd(a.p.F[q][j], q);
-            vol = det3(Fd) * (double)a.p.vol0[j];
-            for (int q = 0; q < 3; ++q) {
-                v[q] = (double)a.p.v[q][j];
-                if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
-            }
-            for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
+    const double ihm = fmax(1.0 / g.h[0], fmax(1.0 / g.h[1], 1.0 / g.h[2]));
+    double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
+    if (j >= a.n_def) {
+        const int d = j - a.n_def;
+        m = a.dc_m[d];
+        vol = a.dc_vol[d];
+        for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
+        for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
+    } else {
+        m = (double)a.p.mass[j];
+        double Fd[9];
+        for (int q = 0; q < 9; ++q) Fd[q] = fd(a.p.F[q][j], q);
+        vol = det3(Fd) * (double)a.p.vol0[j];
+        for (int q = 0; q < 3; ++q) {
+            v[q] = (double)a.p.v[q][j];
+            if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
         }
-        double smax = 0.0;
-        for (int q = 0; q < 6; ++q) smax = fmax(smax, fabs(s[q]));
-        b[0] = fabs(m);
-        b[1] = fabs(m) * ihm;
-        b[2] = fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2])));
-        b[3] = fabs(vol) * 3.0 * smax * ihm + fmax(fabs(fe[0]), fmax(fabs(fe[1]), fabs(fe[2])));
+        for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
     }
-    // block maxima; the global slot is only written when the block beats it
-    __shared__ double s_b[4][4];
+    double smax = 0.0;
+    for (int q = 0; q < 6; ++q) smax = fmax(smax, fabs(s[q]));
+    b[0] = fabs(m);
+    b[1] = fabs(m) * ihm;
+    b[2] = fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2])));
+    b[3] = fabs(vol) * 3.0 * smax * ihm + fmax(fabs(fe[0]), fmax(fabs(fe[1]), fabs(fe[2])));
+}
+template <typename real>
+__device__ __forceinline__ void c_bounds_flush(const CArgs<real> &a, const double b[4], double (*s_b)[4])
+{
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const double w = wave_dmax(b[q]);
@@ -353,12 +365,14 @@ __global__ void __launch_bounds__(256) k_c_bounds(CArgs<real> a)
         if (w > 0.0 && wb > __hip_atomic_load(&a.bounds[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(&a.bounds[q], wb);
     }
-    if (j == 0) a.bounds[4] = (unsigned long long)__double_as_longlong(1.0);
 }
 
-// ---- C2 + C3: active nodes, kernel on the grid (…_MP_Contact.cpp:94-153).
-// The block's kernel sums and node flags go through an LDS window of the
-// nodes i0 - 1 .. i0 + 2 of its points' cells.
+// ---- C2 + C3: active nodes, kernel on the grid (…_MP_Contact.cpp:94-153),
+// and the bounds of the step's P2G contributions.  The block's kernel sums
+// and node flags go through an LDS window of the nodes i0 - 1 .. i0 + 2 of
+// its points' cells.  Per axis only the node planes within R of the point
+// (a slightly inclusive prefilter) are visited, in ascending order as the
+// reference's -1..2 loops; the exact distance test decides.
 constexpr int kCWinK = 1024;
 template <typename real>
 __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
@@ -366,6 +380,7 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
     __shared__ int sbox[6];
     __shared__ unsigned long long s_k[kCWinK];
     __shared__ uint32_t s_f[kCWinK];
+    __shared__ double s_b[4][4];
     const int j = blockIdx.x * 256 + threadIdx.x;
     const GridParams &g = a.g;
     double x[3] = {0.0, 0.0, 0.0};
@@ -373,11 +388,14 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
     int ijk[8][3];
     ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
     bool valid = j < a.n_all;
+    double b[4] = {0.0, 0.0, 0.0, 0.0};
     if (valid) {
+        c_point_bounds(a, j, b);
         c_pos(a, j, x);
         valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
         if (!valid) atomicOr(a.gflags, FLAG_OUT_OF_GRID);
     }
+    c_bounds_flush(a, b, s_b);
     const CWin w = c_window(valid, ijk[0], sbox, 1, 2, kCWinK);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         s_k[t] = 0ull;
@@ -392,41 +410,46 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
         const double R = fabs(1.0 * g.h[0]);
         const double sck = c_scale(a, 4);
         const int *i0 = ijk[0];
-        // node coordinates (i L) / n of the 4 candidate planes per axis
-        double xc[3][4];
-        for (int q = 0; q < 3; ++q)
-            for (int t = 0; t < 4; ++t)
-                xc[q][t] = kCFast<real> ? (double)(i0[q] + t - 1) * g.h[q] : (double)(i0[q] + t - 1) * g.L[q] / (double)g.cells[q];
-        // a node can only be within R if it is within R on every axis (a
-        // slightly inclusive prefilter; the exact test below decides)
-        bool near_ax[3][4];
-        for (int q = 0; q < 3; ++q)
-            for (int t = 0; t < 4; ++t) near_ax[q][t] = fabs(x[q] - xc[q][t]) < R * (1.0 + 1.0e-12);
-        for (int ix = -1; ix < 3; ++ix)
-            for (int iy = -1; iy < 3; ++iy)
-                for (int iz = -1; iz < 3; ++iz) {
-                    if (!(near_ax[0][ix + 1] && near_ax[1][iy + 1] && near_ax[2][iz + 1])) continue;
-                    const int n[3] = {i0[0] + ix, i0[1] + iy, i0[2] + iz};
-                    if (n[0] < 0 || n[1] < 0 || n[2] < 0 || n[0] > g.nodes[0] - 1 || n[1] > g.nodes[1] - 1 ||
-                        n[2] > g.nodes[2] - 1)
-                        continue;
-                    const double d[3] = {x[0] - xc[0][ix + 1], x[1] - xc[1][iy + 1], x[2] - xc[2][iz + 1]};
+        // per axis, the candidate planes i0 - 1 + t (t = 0..3) inside the grid
+        // and within R on that axis, as bits
+        uint32_t near[3];
+        for (int q = 0; q < 3; ++q) {
+            near[q] = 0u;
+            for (int t = 0; t < 4; ++t) {
+                const int n = i0[q] + t - 1;
+                const double xc = kCFast<real> ? (double)n * g.h[q] : (double)n * g.L[q] / (double)g.cells[q];
+                if (n >= 0 && n <= g.nodes[q] - 1 && fabs(x[q] - xc) < R * (1.0 + 1.0e-12)) near[q] |= 1u << t;
+            }
+        }
+        for (uint32_t mx = near[0]; mx; mx &= mx - 1) {
+            const int n0 = i0[0] + __ffs(mx) - 2;
+            const double d0 = x[0] - (kCFast<real> ? (double)n0 * g.h[0] : (double)n0 * g.L[0] / (double)g.cells[0]);
+            for (uint32_t my = near[1]; my; my &= my - 1) {
+                const int n1 = i0[1] + __ffs(my) - 2;
+                const double d1 = x[1] - (kCFast<real> ? (double)n1 * g.h[1] : (double)n1 * g.L[1] / (double)g.cells[1]);
+                for (uint32_t mz = near[2]; mz; mz &= mz - 1) {
+                    const int n2 = i0[2] + __ffs(mz) - 2;
+                    const double d2 = x[2] - (kCFast<real> ? (double)n2 * g.h[2] : (double)n2 * g.L[2] / (double)g.cells[2]);
+                    const double d[3] = {d0, d1, d2};
                     const double dist = c_len3(d);
                     if (dist < R) {
                         const double q = kCFast<real> ? dist * g.ih[0] : dist / R;
                         const unsigned long long iv =
                             (unsigned long long)__double2ll_rn((1.0 - 3.0 * q * q + 2.0 * q * q * q) * sck);
+                        const int n[3] = {n0, n1, n2};
                         if (w.lds) {
                             const int ws = c_wslot(w, n);
                             atomicOr(&s_f[ws], CF_KACTIVE);
                             atomicAdd(&s_k[ws], iv);
                         } else {
-                            const long long I = n[0] + (long long)g.nodes[0] * (n[1] + (long long)g.nodes[1] * n[2]);
+                            const long long I = n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2);
                             atomicOr(&a.flags[I], CF_KACTIVE);
                             atomicAdd((unsigned long long *)&a.acc[off_k() * a.nn + I], iv);
                         }
                     }
                 }
+            }
+        }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
@@ -468,12 +491,15 @@ __global__ void __launch_bounds__(256) k_c_kgrad(CArgs<real> a)
 
 // ---- C5: point kernel and its gradient; nearest point and surface gradient,
 // pass 1 (minimum distance, maximum |grad k|) (…_MP_Contact.cpp:199-252),
-// through an LDS window of the block's nodes
+// through an LDS window of the block's nodes.  The window's kernel values and
+// gradients are staged in LDS first (one load per node instead of one per
+// point corner).
 template <typename real>
 __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
 {
     __shared__ int sbox[6];
     __shared__ unsigned long long s_near[kCWin], s_sg[kCWin];
+    __shared__ double s_kv[4][kCWin];
     const int j = blockIdx.x * 256 + threadIdx.x;
     const GridParams &g = a.g;
     double x[3] = {0.0, 0.0, 0.0};
@@ -486,13 +512,16 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
         valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
     }
     const CWin w = c_window(valid, ijk[0], sbox);
+    const double usk = c_unscale(a, 4);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         s_near[t] = 0x7FF0000000000000ull;   // +inf
         s_sg[t] = 0ull;
+        const long long I = c_wnode(g, w, t);
+        s_kv[0][t] = (double)a.acc[I] * usk;
+        for (int q = 0; q < 3; ++q) s_kv[1 + q][t] = a.gkgrad[q * a.nn + I];
     }
     __syncthreads();
     if (valid) {
-        const double usk = c_unscale(a, 4);
         double kp = 0.0, kg[3] = {0.0, 0.0, 0.0};
         unsigned long long dk[8];
         CAx A;
@@ -500,8 +529,14 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
         for (int k = 0; k < 8; ++k) {
             double xi[3], N, G[3];
             c_corner<kCFast<real>>(g, A, k, xi, N, G);
-            kp += N * ((double)a.acc[idx[k]] * usk);
-            for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + idx[k]];
+            if (w.lds) {
+                const int ws = c_wslot(w, ijk[k]);
+                kp += N * s_kv[0][ws];
+                for (int q = 0; q < 3; ++q) kg[q] += N * s_kv[1 + q][ws];
+            } else {
+                kp += N * ((double)a.acc[idx[k]] * usk);
+                for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + idx[k]];
+            }
             const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
             dk[k] = (unsigned long long)__double_as_longlong(c_len3(d));
             if (w.lds) atomicMin(&s_near[c_wslot(w, ijk[k])], dk[k]);
@@ -530,87 +565,99 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
     }
 }
 
-// pass 2: the smallest index among the points at the minimum
+// pass 2: the smallest point id among the points at the minimum distance
+// (largest |grad k|), kept with the winner's current index as the 64-bit key
+// (id << 32) | index; the window's minima are staged in LDS
 template <typename real>
 __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
 {
+    __shared__ int sbox[6];
+    __shared__ unsigned long long s_nd[kCWin], s_sd[kCWin], s_ni[kCWin], s_si[kCWin];
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= a.n_all) return;
     const GridParams &g = a.g;
-    double x[3];
-    c_pos(a, j, x);
+    double x[3] = {0.0, 0.0, 0.0};
     long long idx[8];
     int ijk[8][3];
-    if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) return;
-    const uint32_t id = c_id(a, j);
-    const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
-    const bool surf = c_surf(a, j) != 0;
-    const unsigned long long lk = (unsigned long long)__double_as_longlong(c_len3(kg));
-    CAx A;
-    c_axes<kCFast<real>>(g, x, ijk[0], A);
-    for (int k = 0; k < 8; ++k) {
-        double xi[3], N, G[3];
-        c_corner<kCFast<real>>(g, A, k, xi, N, G);
-        const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
-        if ((unsigned long long)__double_as_longlong(c_len3(d)) == a.near_d[idx[k]]) atomicMin(&a.near_i[idx[k]], id);
-        if (surf && lk != 0ull && lk == a.sg_d[idx[k]]) atomicMin(&a.sg_i[idx[k]], id);
+    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
+    bool valid = j < a.n_all;
+    if (valid) {
+        c_pos(a, j, x);
+        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
+    }
+    const CWin w = c_window(valid, ijk[0], sbox);
+    for (int t = threadIdx.x; t < w.nodes; t += 256) {
+        const long long I = c_wnode(g, w, t);
+        s_nd[t] = a.near_d[I];
+        s_sd[t] = a.sg_d[I];
+        s_ni[t] = ~0ull;
+        s_si[t] = ~0ull;
+    }
+    __syncthreads();
+    if (valid) {
+        const unsigned long long key = ((unsigned long long)c_id(a, j) << 32) | (uint32_t)j;
+        const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
+        const bool surf = c_surf(a, j) != 0;
+        const unsigned long long lk = (unsigned long long)__double_as_longlong(c_len3(kg));
+        CAx A;
+        c_axes<kCFast<real>>(g, x, ijk[0], A);
+        for (int k = 0; k < 8; ++k) {
+            double xi[3], N, G[3];
+            c_corner<kCFast<real>>(g, A, k, xi, N, G);
+            const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
+            const unsigned long long db = (unsigned long long)__double_as_longlong(c_len3(d));
+            if (w.lds) {
+                const int ws = c_wslot(w, ijk[k]);
+                if (db == s_nd[ws]) atomicMin(&s_ni[ws], key);
+                if (surf && lk != 0ull && lk == s_sd[ws]) atomicMin(&s_si[ws], key);
+            } else {
+                if (db == a.near_d[idx[k]]) atomicMin(&a.near_i[idx[k]], key);
+                if (surf && lk != 0ull && lk == a.sg_d[idx[k]]) atomicMin(&a.sg_i[idx[k]], key);
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < w.nodes; t += 256) {
+        const unsigned long long ni = s_ni[t], si = s_si[t];
+        if (ni == ~0ull && si == ~0ull) continue;
+        const long long I = c_wnode(g, w, t);
+        if (ni != ~0ull) atomicMin(&a.near_i[I], ni);
+        if (si != ~0ull) atomicMin(&a.sg_i[I], si);
     }
 }
 
-// pass 3: the winners publish their surface flag / kernel gradient.  The
+// pass 3, per node: the winners' surface flag and kernel gradient.  The
 // reference starts every node's nearest point at point 0, so point 0 wins
 // where it is at least as close as the nearest touching point.
 template <typename real>
-__global__ void __launch_bounds__(256) k_c_point3(CArgs<real> a)
+__global__ void __launch_bounds__(256) k_c_publish(CArgs<real> a)
 {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= a.n_all) return;
-    const GridParams &g = a.g;
-    double x[3];
-    c_pos(a, j, x);
-    long long idx[8];
-    int ijk[8][3];
-    if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) return;
-    const uint32_t id = c_id(a, j);
-    double x0[3];
-    c_pos(a, a.def_id0[0] ? a.n_def + (int)a.def_id0[1] : (int)a.def_id0[1], x0);
-    for (int k = 0; k < 8; ++k) {
-        const long long I = idx[k];
-        if (id == a.sg_i[I])
-            for (int q = 0; q < 3; ++q) a.gkg[q * a.nn + I] = a.kg[q * a.n_all + j];
-        if (id == a.near_i[I]) {
-            double xi[3];
-            c_node_pos<kCFast<real>>(g, ijk[k][0], ijk[k][1], ijk[k][2], xi);
-            const double d0[3] = {x0[0] - xi[0], x0[1] - xi[1], x0[2] - xi[2]};
-            const bool zero_wins = (unsigned long long)__double_as_longlong(c_len3(d0)) <= a.near_d[I];
-            a.near_surf[I] = zero_wins ? a.surf0 : c_surf(a, j);
-        }
+    const long long I = blockIdx.x * 256ll + threadIdx.x;
+    if (I >= a.nn) return;
+    const unsigned long long ni = a.near_i[I], si = a.sg_i[I];
+    if (si != ~0ull) {
+        const int j = (int)(uint32_t)si;
+        for (int q = 0; q < 3; ++q) a.gkg[q * a.nn + I] = a.kg[q * a.n_all + j];
+    }
+    if (ni != ~0ull) {
+        const GridParams &g = a.g;
+        const int j = (int)(uint32_t)ni;
+        const long long r = I / g.nodes[0];
+        double xi[3], x0[3];
+        c_node_pos<kCFast<real>>(g, (int)(I % g.nodes[0]), (int)(r % g.nodes[1]), (int)(r / g.nodes[1]), xi);
+        c_pos(a, a.def_id0[0] ? a.n_def + (int)a.def_id0[1] : (int)a.def_id0[1], x0);
+        const double d0[3] = {x0[0] - xi[0], x0[1] - xi[1], x0[2] - xi[2]};
+        const bool zero_wins = (unsigned long long)__double_as_longlong(c_len3(d0)) <= a.near_d[I];
+        a.near_surf[I] = zero_wins ? a.surf0 : c_surf(a, j);
     }
 }
 
-// ---- C6: contact detection (…_MP_Contact.cpp:258-282) ---------------------------
-template <typename real>
-__global__ void __launch_bounds__(256) k_c_detect(CArgs<real> a)
-{
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= a.n_all || !c_surf(a, j)) return;
-    double x[3];
-    c_pos(a, j, x);
-    long long idx[8];
-    int ijk[8][3];
-    if (!c_adjacent<kCFast<real>>(a.g, x, idx, ijk)) return;
-    const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
-    for (int k = 0; k < 8; ++k) {
-        const long long I = idx[k];
-        if (!a.near_surf[I]) continue;
-        const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
-        atomicOr(&a.flags[I], c_dot3(gk, kg) >= 0.0 ? CF_CPOS : CF_CNEG);
-    }
-}
-
-// ---- C6: split P2G (…_MP_Contact.cpp:288-385); MASS=true: mass and grad m,
-// MASS=false: momentum (/ field mass, sqrt(3) gate) and force.  Window slot
-// s of a node: MASS: field * 4 + {m, mg.xyz}; else field * 6 + {mv.xyz, f.xyz}.
+// ---- C6: contact detection (…_MP_Contact.cpp:258-282) and split P2G
+// (…_MP_Contact.cpp:288-385).  MASS=true: mass and grad m; the field of each
+// point corner (sign of grad k_I . grad k_p) is stored as a bit mask per
+// point, and surface points flag the nodes whose nearest point is a surface
+// point (CF_CPOS / CF_CNEG, the same sign).  MASS=false: momentum (/ field
+// mass, sqrt(3) gate) and force, fields from the mask.  Window slot s of a
+// node: MASS: field * 4 + {m, mg.xyz}; else field * 6 + {mv.xyz, f.xyz}.
 template <bool MASS>
 __device__ __forceinline__ long long c_p2g_off(int s)
 {
@@ -625,8 +672,11 @@ template <typename real, bool MASS>
 __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
 {
     constexpr int NF = MASS ? 8 : 12;
+    constexpr int NS = MASS ? kCWin : 1;   // staged surface gradients and flags (MASS only)
     __shared__ int sbox[6];
     __shared__ unsigned long long lds[kCWin * NF];
+    __shared__ double s_gk[3][NS];
+    __shared__ uint32_t s_ns[NS];   // MASS: bit 0 near_surf (staged); bits CF_CPOS | CF_CNEG (detection)
     const int j = blockIdx.x * 256 + threadIdx.x;
     const GridParams &g = a.g;
     double x[3] = {0.0, 0.0, 0.0};
@@ -640,6 +690,13 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     }
     const CWin w = c_window(valid, ijk[0], sbox);
     for (int t = threadIdx.x; t < w.nodes * NF; t += 256) lds[t] = 0ull;
+    if constexpr (MASS) {
+        for (int t = threadIdx.x; t < w.nodes; t += 256) {
+            const long long I = c_wnode(g, w, t);
+            for (int q = 0; q < 3; ++q) s_gk[q][t] = a.gkg[q * a.nn + I];
+            s_ns[t] = a.near_surf[I] ? 1u : 0u;
+        }
+    }
     __syncthreads();
     if (valid) {
         const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
@@ -663,15 +720,37 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
         }
         const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
         const double um = c_unscale(a, 0);
+        const bool surf = MASS && c_surf(a, j) != 0;
+        const uint32_t fm = MASS ? 0u : (uint32_t)a.fmask[j];
+        uint32_t mask = 0u;
         CAx A;
         c_axes<kCFast<real>>(g, x, ijk[0], A);
         for (int k = 0; k < 8; ++k) {
             const long long I = idx[k];
             double xi[3], N, G[3];
             c_corner<kCFast<real>>(g, A, k, xi, N, G);
-            const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
-            const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
             const int ws = w.lds ? c_wslot(w, ijk[k]) : 0;
+            int f;
+            if constexpr (MASS) {
+                double gk[3];
+                bool ns;
+                if (w.lds) {
+                    for (int q = 0; q < 3; ++q) gk[q] = s_gk[q][ws];
+                    ns = (s_ns[ws] & 1u) != 0;
+                } else {
+                    for (int q = 0; q < 3; ++q) gk[q] = a.gkg[q * a.nn + I];
+                    ns = a.near_surf[I] != 0;
+                }
+                f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
+                mask |= (uint32_t)f << k;
+                if (surf && ns) {
+                    const uint32_t cf = f == 0 ? CF_CPOS : CF_CNEG;
+                    if (w.lds) atomicOr(&s_ns[ws], cf);
+                    else atomicOr(&a.flags[I], cf);
+                }
+            } else {
+                f = (int)((fm >> k) & 1u);
+            }
             long long iv[6];
             int base, cnt = 0;
             if (MASS) {
@@ -700,6 +779,7 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
                 else atomicAdd((unsigned long long *)&a.acc[c_p2g_off<MASS>(base + q) * a.nn + I], (unsigned long long)iv[q]);
             }
         }
+        if constexpr (MASS) a.fmask[j] = (uint8_t)mask;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
@@ -707,6 +787,10 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
         for (int q = 0; q < NF; ++q) {
             const unsigned long long v = lds[q * w.nodes + t];
             if (v) atomicAdd((unsigned long long *)&a.acc[c_p2g_off<MASS>(q) * a.nn + I], v);
+        }
+        if constexpr (MASS) {
+            const uint32_t cf = s_ns[t] & (CF_CPOS | CF_CNEG);
+            if (cf) atomicOr(&a.flags[I], cf);
         }
     }
 }
@@ -733,7 +817,8 @@ __global__ void __launch_bounds__(256) k_c_grid(CArgs<real> a, int step)
     __shared__ double s_r[4][3];
     const long long I = blockIdx.x * 256ll + threadIdx.x;
     double rm = 0.0, rr = 0.0, ra = 0.0;
-    if (I < a.nn && (a.flags[I] & CF_ACTIVE)) {
+    const uint32_t flI = I < a.nn ? a.flags[I] : 0u;
+    if (flI & CF_ACTIVE) {
         const GridParams &g = a.g;
         const StepRec st = a.steps[step];
         const double dt = st.dt;
@@ -764,7 +849,7 @@ __global__ void __launch_bounds__(256) k_c_grid(CArgs<real> a, int step)
         for (int s = 0; s < 2; ++s)
             for (int q = 0; q < 3; ++q)
                 if (fx & (1u << q)) { v[s][q] = 0.0; f[s][q] = 0.0; }
-        if (a.flags[I] & CF_CNEG) {
+        if (flI & CF_CNEG) {
             const double m = mm[0], mn = mm[1];
             double vcm[3], nrm[3], fc[3] = {0, 0, 0}, fcn[3] = {0, 0, 0};
             for (int q = 0; q < 3; ++q) vcm[q] = (m * v[0][q] + mn * v[1][q]) / (m + mn);
@@ -810,11 +895,20 @@ __global__ void __launch_bounds__(256) k_c_grid(CArgs<real> a, int step)
                 f[0][q] = f[1][q] = 0.0;
             }
         }
-        for (int s = 0; s < 2; ++s)
+        // G2P's node acceleration per field, f / m where the field mass
+        // passes the gate (…_MP_Contact.cpp:560-566), with the gate as a flag
+        uint32_t gates = 0u;
+        for (int s = 0; s < 2; ++s) {
+            const bool gt = c_lenm3(mm[s]) > g.m_min;
+            if (gt) gates |= CF_GATE0 << s;
+            const double rms = 1.0 / mm[s];
             for (int q = 0; q < 3; ++q) {
                 a.gv[(3 * s + q) * a.nn + I] = v[s][q];
                 a.gf[(3 * s + q) * a.nn + I] = f[s][q];
+                a.ga[(3 * s + q) * a.nn + I] = gt ? (kCFast<real> ? f[s][q] * rms : f[s][q] / mm[s]) : 0.0;
             }
+        }
+        if (gates) a.flags[I] = flI | gates;
         rm += mm[0];
         ra += 1.0;
     }
@@ -852,7 +946,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
         if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) {
             flags |= FLAG_OUT_OF_GRID;
         } else {
-            const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
+            const uint32_t fm = a.fmask[j];
             double v[3], xn[3] = {x[0], x[1], x[2]}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             for (int q = 0; q < 3; ++q) v[q] = (double)P.v[q][j];
             CAx A;
@@ -861,22 +955,15 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
                 const long long I = idx[k];
                 double xi[3], N, G[3];
                 c_corner<kCFast<real>>(g, A, k, xi, N, G);
-                const double gk[3] = {a.gkg[I], a.gkg[a.nn + I], a.gkg[2 * a.nn + I]};
-                const int f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
-                const double m = a.gm[f * a.nn + I];
-                double vI[3], fI[3];
+                const int f = (int)((fm >> k) & 1u);
+                const uint32_t fl = a.flags[I];
+                double vI[3], aI[3];
                 for (int q = 0; q < 3; ++q) {
                     vI[q] = a.gv[(3 * f + q) * a.nn + I];
-                    fI[q] = a.gf[(3 * f + q) * a.nn + I];
+                    aI[q] = a.ga[(3 * f + q) * a.nn + I];
                 }
-                if (c_lenm3(m) > g.m_min) {
-                    if (FP32) {
-                        const double rm = 1.0 / m;
-                        for (int q = 0; q < 3; ++q) v[q] += N * (fI[q] * rm) * dt;
-                    } else {
-                        for (int q = 0; q < 3; ++q) v[q] += N * (fI[q] / m) * dt;
-                    }
-                }
+                if (fl & (CF_GATE0 << f))
+                    for (int q = 0; q < 3; ++q) v[q] += N * aI[q] * dt;
                 for (int q = 0; q < 3; ++q) xn[q] += N * vI[q] * dt;
                 for (int c = 0; c < 3; ++c)
                     for (int r = 0; r < 3; ++r) L[3 * c + r] += vI[r] * G[c];

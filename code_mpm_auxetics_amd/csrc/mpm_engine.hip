@@ -150,10 +150,11 @@ struct mpm_handle {
     struct Contact {
         bool ready = false;
         long long *acc = nullptr;
-        double *gkgrad = nullptr, *gkg = nullptr, *gm = nullptr, *gv = nullptr, *gf = nullptr;
-        uint32_t *flags = nullptr, *near_i = nullptr, *sg_i = nullptr, *id0 = nullptr;
+        double *gkgrad = nullptr, *gkg = nullptr, *gm = nullptr, *gv = nullptr, *gf = nullptr, *ga = nullptr;
+        uint32_t *flags = nullptr, *id0 = nullptr;
         unsigned long long *near_d = nullptr, *sg_d = nullptr, *cmarks = nullptr, *bounds = nullptr;
-        uint8_t *near_surf = nullptr;
+        unsigned long long *near_i = nullptr, *sg_i = nullptr;
+        uint8_t *near_surf = nullptr, *fmask = nullptr;
         double *kp = nullptr, *kg = nullptr, *cpart = nullptr, *ppart = nullptr;
         double *kp_id = nullptr, *kg_id = nullptr;   // by original point index
         int nblk_n = 0, nblk_p = 0;
@@ -596,6 +597,8 @@ static int ensure_contact(mpm_handle *h)
     RCHK(dalloc(h, &c.gm, 2 * nn));
     RCHK(dalloc(h, &c.gv, 6 * nn));
     RCHK(dalloc(h, &c.gf, 6 * nn));
+    RCHK(dalloc(h, &c.ga, 6 * nn));
+    RCHK(dalloc(h, &c.fmask, n_all));
     RCHK(dalloc(h, &c.flags, nn));
     RCHK(dalloc(h, &c.near_i, nn));
     RCHK(dalloc(h, &c.sg_i, nn));
@@ -653,6 +656,8 @@ static CArgs<real> contact_args(mpm_handle *h)
     a.near_i = c.near_i;
     a.sg_i = c.sg_i;
     a.near_surf = c.near_surf;
+    a.fmask = c.fmask;
+    a.ga = c.ga;
     a.cmarks = c.cmarks;
     a.bounds = c.bounds;
     a.kp = c.kp;
@@ -678,17 +683,13 @@ static int launch_contact(mpm_handle *h, int k)
     const unsigned gp = cdiv((size_t)std::max(a.n_all, 1), 256), gn = (unsigned)c.nblk_n;
     hipStream_t st = h->stream;
     k_c_clear<real><<<gn, 256, 0, st>>>(a);
-    if (a.n_all > 0) {
-        k_c_bounds<real><<<gp, 256, 0, st>>>(a);
-        k_c_kernel<real><<<gp, 256, 0, st>>>(a);
-    }
+    if (a.n_all > 0) k_c_kernel<real><<<gp, 256, 0, st>>>(a);   // kernel density, active nodes, bounds
     k_c_kgrad<real><<<gn, 256, 0, st>>>(a);
     if (a.n_all > 0) {
         k_c_point1<real><<<gp, 256, 0, st>>>(a);
         k_c_point2<real><<<gp, 256, 0, st>>>(a);
-        k_c_point3<real><<<gp, 256, 0, st>>>(a);
-        k_c_detect<real><<<gp, 256, 0, st>>>(a);
-        k_c_p2g<real, true><<<gp, 256, 0, st>>>(a);
+        k_c_publish<real><<<gn, 256, 0, st>>>(a);
+        k_c_p2g<real, true><<<gp, 256, 0, st>>>(a);    // with detection and the field mask
         k_c_p2g<real, false><<<gp, 256, 0, st>>>(a);
     }
     if (a.n_dc > 0) k_c_dcmarks<real><<<cdiv(a.n_dc, 256), 256, 0, st>>>(a, k);
@@ -1183,7 +1184,7 @@ static void destroy_all(mpm_handle *h)
                        (void **)&c.gf, (void **)&c.flags, (void **)&c.near_i, (void **)&c.sg_i, (void **)&c.id0,
                        (void **)&c.near_d, (void **)&c.sg_d, (void **)&c.cmarks, (void **)&c.bounds,
                        (void **)&c.near_surf, (void **)&c.kp, (void **)&c.kg, (void **)&c.cpart, (void **)&c.ppart,
-                       (void **)&c.kp_id, (void **)&c.kg_id,
+                       (void **)&c.kp_id, (void **)&c.kg_id, (void **)&c.ga, (void **)&c.fmask,
                        (void **)&h->dc_vol, (void **)&h->dc_s, (void **)&h->dc_fext, (void **)&h->dc_surf,
                        (void **)&h->dc_id};
         for (void **q : cp) dev_free(*q);
@@ -1231,6 +1232,10 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     *out = nullptr;
     for (int a = 0; a < 3; ++a)
         if (!(cfg->length[a] > 0.0) || cfg->cells[a] < 1) { g_create_error = "grid length and cells must be positive"; return MPM_E_ARG; }
+    // cell indices enter the wave reductions of the contact windows, which
+    // take values in [0, 2^20]
+    for (int a = 0; a < 3; ++a)
+        if (cfg->cells[a] > (1 << 20) - 2) { g_create_error = "at most 2^20 - 2 cells per axis"; return MPM_E_ARG; }
     if (!(cfg->dt_max > 0.0)) { g_create_error = "dt_max must be positive"; return MPM_E_ARG; }
     if (cfg->precision != MPM_FP64 && cfg->precision != MPM_FP32) { g_create_error = "unknown precision"; return MPM_E_ARG; }
     if (cfg->n_timeline < 0 || (cfg->n_timeline > 0 && (!cfg->timeline_t || !cfg->timeline_v))) {

@@ -157,6 +157,12 @@ def test_create_rejects_bad_arguments_before_touching_a_device():
     bad.cells[1] = 0
     assert lib.mpm_create(C.byref(bad), C.byref(h)) == A.MPM_E_ARG
     assert lib.mpm_create(C.byref(_config(n_timeline=2)), C.byref(h)) == A.MPM_E_ARG
+    # cell indices enter 2^20-bounded wave reductions of the contact step
+    huge = _config()
+    huge.cells[0] = (1 << 20) - 1
+    assert lib.mpm_create(C.byref(huge), C.byref(h)) == A.MPM_E_ARG
+    assert b"2^20" in lib.mpm_last_error(None)
+    assert not h.value
 
 
 def test_create_rejects_fp32_bricks_beyond_the_fixed_point_range():
