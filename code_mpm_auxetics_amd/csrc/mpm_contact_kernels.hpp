@@ -217,6 +217,21 @@ __device__ __forceinline__ double c_unscale(const CArgs<real> &a, int q)
     frexp(b * 1.0001, &e);
     return ldexp(1.0, e - 62 + a.H);
 }
+// round-to-nearest-even of a scaled contribution to int64.  With H >= 11
+// headroom bits every scaled contribution is below 2^51 in magnitude, and
+// adding 1.5 * 2^52 leaves rint(x) in the low mantissa bits (one add and one
+// integer subtract instead of the general conversion).  The add must not be
+// fused with the product that feeds it.
+template <bool SMALL>
+__device__ __forceinline__ long long c_toll(double x)
+{
+#pragma clang fp contract(off)
+    if constexpr (SMALL) {
+        const double y = x + 6755399441055744.0;   // 1.5 * 2^52
+        return __double_as_longlong(y) - 0x4338000000000000ll;
+    }
+    return __double2ll_rn(x);
+}
 __device__ __forceinline__ void c_add(long long *p, double v, double sc)
 {
     atomicAdd((unsigned long long *)p, (unsigned long long)__double2ll_rn(v * sc));
@@ -374,7 +389,7 @@ __device__ __forceinline__ void c_bounds_flush(const CArgs<real> &a, const doubl
 // (a slightly inclusive prefilter) are visited, in ascending order as the
 // reference's -1..2 loops; the exact distance test decides.
 constexpr int kCWinK = 1024;
-template <typename real>
+template <typename real, bool SMALL>
 __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
 {
     __shared__ int sbox[6];
@@ -435,7 +450,7 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
                     if (dist < R) {
                         const double q = kCFast<real> ? dist * g.ih[0] : dist / R;
                         const unsigned long long iv =
-                            (unsigned long long)__double2ll_rn((1.0 - 3.0 * q * q + 2.0 * q * q * q) * sck);
+                            (unsigned long long)c_toll<SMALL>((1.0 - 3.0 * q * q + 2.0 * q * q * q) * sck);
                         const int n[3] = {n0, n1, n2};
                         if (w.lds) {
                             const int ws = c_wslot(w, n);
@@ -584,6 +599,15 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
         c_pos(a, j, x);
         valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
     }
+    // the point's own inputs first: their latency overlaps the window set-up
+    unsigned long long key = 0ull, lk = 0ull;
+    bool surf = false;
+    if (valid) {
+        key = ((unsigned long long)c_id(a, j) << 32) | (uint32_t)j;
+        const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
+        surf = c_surf(a, j) != 0;
+        lk = (unsigned long long)__double_as_longlong(c_len3(kg));
+    }
     const CWin w = c_window(valid, ijk[0], sbox);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         const long long I = c_wnode(g, w, t);
@@ -594,10 +618,6 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
     }
     __syncthreads();
     if (valid) {
-        const unsigned long long key = ((unsigned long long)c_id(a, j) << 32) | (uint32_t)j;
-        const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
-        const bool surf = c_surf(a, j) != 0;
-        const unsigned long long lk = (unsigned long long)__double_as_longlong(c_len3(kg));
         CAx A;
         c_axes<kCFast<real>>(g, x, ijk[0], A);
         for (int k = 0; k < 8; ++k) {
@@ -668,7 +688,7 @@ __device__ __forceinline__ long long c_p2g_off(int s)
     const int f = s / 6, q = s % 6;
     return q < 3 ? off_mv(f, q) : off_f(f, q - 3);
 }
-template <typename real, bool MASS>
+template <typename real, bool MASS, bool SMALL>
 __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
 {
     constexpr int NF = MASS ? 8 : 12;
@@ -688,6 +708,39 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
         c_pos(a, j, x);
         valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
     }
+    // the point's own inputs are loaded before the window is set up, so
+    // their latency overlaps the window's barriers and staging.  MASS needs
+    // the mass only; momentum and force need v, V = det F V0, stress, f_ext.
+    double kg[3] = {0.0, 0.0, 0.0}, m = 0.0, v[3] = {0.0, 0.0, 0.0}, vol = 0.0, s[6] = {0, 0, 0, 0, 0, 0},
+           fe[3] = {0, 0, 0};
+    bool surf = false;
+    uint32_t fm = 0u;
+    if (valid) {
+        for (int q = 0; q < 3; ++q) kg[q] = a.kg[q * a.n_all + j];
+        if (j >= a.n_def) {
+            const int d = j - a.n_def;
+            m = a.dc_m[d];
+            if constexpr (!MASS) {
+                vol = a.dc_vol[d];
+                for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
+                for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
+            }
+        } else {
+            m = (double)a.p.mass[j];
+            if constexpr (!MASS) {
+                double Fd[9];
+                for (int q = 0; q < 9; ++q) Fd[q] = fd(a.p.F[q][j], q);
+                vol = (double)(real)(det3(Fd) * (double)a.p.vol0[j]);
+                for (int q = 0; q < 3; ++q) {
+                    v[q] = (double)a.p.v[q][j];
+                    if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
+                }
+                for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
+            }
+        }
+        if constexpr (MASS) surf = c_surf(a, j) != 0;
+        else fm = a.fmask[j];
+    }
     const CWin w = c_window(valid, ijk[0], sbox);
     for (int t = threadIdx.x; t < w.nodes * NF; t += 256) lds[t] = 0ull;
     if constexpr (MASS) {
@@ -699,29 +752,8 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     }
     __syncthreads();
     if (valid) {
-        const double kg[3] = {a.kg[j], a.kg[a.n_all + j], a.kg[2 * a.n_all + j]};
-        double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
-        if (j >= a.n_def) {
-            const int d = j - a.n_def;
-            m = a.dc_m[d];
-            vol = a.dc_vol[d];
-            for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
-            for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
-        } else {
-            m = (double)a.p.mass[j];
-            double Fd[9];
-            for (int q = 0; q < 9; ++q) Fd[q] = fd(a.p.F[q][j], q);
-            vol = (double)(real)(det3(Fd) * (double)a.p.vol0[j]);
-            for (int q = 0; q < 3; ++q) {
-                v[q] = (double)a.p.v[q][j];
-                if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
-            }
-            for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
-        }
         const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
         const double um = c_unscale(a, 0);
-        const bool surf = MASS && c_surf(a, j) != 0;
-        const uint32_t fm = MASS ? 0u : (uint32_t)a.fmask[j];
         uint32_t mask = 0u;
         CAx A;
         c_axes<kCFast<real>>(g, x, ijk[0], A);
@@ -731,9 +763,9 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
             c_corner<kCFast<real>>(g, A, k, xi, N, G);
             const int ws = w.lds ? c_wslot(w, ijk[k]) : 0;
             int f;
+            bool ns = false;
             if constexpr (MASS) {
                 double gk[3];
-                bool ns;
                 if (w.lds) {
                     for (int q = 0; q < 3; ++q) gk[q] = s_gk[q][ws];
                     ns = (s_ns[ws] & 1u) != 0;
@@ -755,23 +787,26 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
             int base, cnt = 0;
             if (MASS) {
                 base = 4 * f;
-                iv[0] = __double2ll_rn((N * m) * sc0);
-                for (int q = 0; q < 3; ++q) iv[1 + q] = __double2ll_rn((G[q] * m) * sc1);
-                cnt = 4;
+                iv[0] = c_toll<SMALL>((N * m) * sc0);
+                for (int q = 0; q < 3; ++q) iv[1 + q] = c_toll<SMALL>((G[q] * m) * sc1);
+                // grad m is read only by the contact correction, at nodes
+                // flagged CF_CNEG, which requires a surface nearest point:
+                // elsewhere its sums are never used and are not formed
+                cnt = ns ? 4 : 1;
             } else {
                 base = 6 * f;
                 const double mI = (double)a.acc[off_m(f) * a.nn + I] * um;
                 const bool gate = c_lenm3(mI) > g.m_min;
                 if (kCFast<real>) {
                     const double rmI = gate ? 1.0 / mI : 0.0;
-                    for (int q = 0; q < 3; ++q) iv[q] = gate ? __double2ll_rn((N * (m * v[q]) * rmI) * sc0) : 0ll;
+                    for (int q = 0; q < 3; ++q) iv[q] = gate ? c_toll<SMALL>((N * (m * v[q]) * rmI) * sc0) : 0ll;
                 } else {
-                    for (int q = 0; q < 3; ++q) iv[q] = gate ? __double2ll_rn((N * (m * v[q]) / mI) * sc0) : 0ll;
+                    for (int q = 0; q < 3; ++q) iv[q] = gate ? c_toll<SMALL>((N * (m * v[q]) / mI) * sc0) : 0ll;
                 }
                 const double fr[3] = {-vol * (G[0] * s[0] + G[1] * s[3] + G[2] * s[5]) + N * fe[0],
                                       -vol * (G[1] * s[1] + G[0] * s[3] + G[2] * s[4]) + N * fe[1],
                                       -vol * (G[2] * s[2] + G[0] * s[5] + G[1] * s[4]) + N * fe[2]};
-                for (int q = 0; q < 3; ++q) iv[3 + q] = __double2ll_rn(fr[q] * sc1);
+                for (int q = 0; q < 3; ++q) iv[3 + q] = c_toll<SMALL>(fr[q] * sc1);
                 cnt = 6;
             }
             for (int q = 0; q < cnt; ++q) {
@@ -930,25 +965,46 @@ template <typename real>
 __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
 {
     __shared__ double s_r[4][5];
+    __shared__ int sbox[6];
+    __shared__ double s_n[12][kCWin];   // per window node: v.xyz of field f at 3f, a.xyz at 6 + 3f
+    __shared__ uint32_t s_fl[kCWin];
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const GridParams &g = a.g;
     double red[5] = {0, 0, 0, 0, 0};
     int flags = 0;
+    double x[3] = {0.0, 0.0, 0.0};
+    long long idx[8];
+    int ijk[8][3];
+    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
+    bool valid = false;
+    uint32_t fm = 0u;
+    double v[3] = {0.0, 0.0, 0.0};
     if (j < a.n_def) {
+        c_pos(a, j, x);
+        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
+        if (!valid) flags |= FLAG_OUT_OF_GRID;
+        // loaded before the window set-up: the latency overlaps it
+        fm = a.fmask[j];
+        for (int q = 0; q < 3; ++q) v[q] = (double)a.p.v[q][j];
+    }
+    // the block's node window (velocity, gated acceleration, flags) staged in LDS
+    const CWin w = c_window(valid, ijk[0], sbox);
+    for (int t = threadIdx.x; t < w.nodes; t += 256) {
+        const long long I = c_wnode(g, w, t);
+        s_fl[t] = a.flags[I];
+        for (int q = 0; q < 6; ++q) {
+            s_n[q][t] = a.gv[q * a.nn + I];
+            s_n[6 + q][t] = a.ga[q * a.nn + I];
+        }
+    }
+    __syncthreads();
+    if (valid) {
         constexpr bool FP32 = sizeof(real) == 4;
-        const GridParams &g = a.g;
         const PArrays<real> &P = a.p;
         const double dt = a.steps[step].dt;
-        double x[3];
-        c_pos(a, j, x);
-        long long idx[8];
-        int ijk[8][3];
-        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) {
-            flags |= FLAG_OUT_OF_GRID;
-        } else {
-            const uint32_t fm = a.fmask[j];
-            double v[3], xn[3] = {x[0], x[1], x[2]}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int q = 0; q < 3; ++q) v[q] = (double)P.v[q][j];
+        {
+            double xn[3] = {x[0], x[1], x[2]}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             CAx A;
             c_axes<kCFast<real>>(g, x, ijk[0], A);
             for (int k = 0; k < 8; ++k) {
@@ -956,11 +1012,21 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
                 double xi[3], N, G[3];
                 c_corner<kCFast<real>>(g, A, k, xi, N, G);
                 const int f = (int)((fm >> k) & 1u);
-                const uint32_t fl = a.flags[I];
+                uint32_t fl;
                 double vI[3], aI[3];
-                for (int q = 0; q < 3; ++q) {
-                    vI[q] = a.gv[(3 * f + q) * a.nn + I];
-                    aI[q] = a.ga[(3 * f + q) * a.nn + I];
+                if (w.lds) {
+                    const int ws = c_wslot(w, ijk[k]);
+                    fl = s_fl[ws];
+                    for (int q = 0; q < 3; ++q) {
+                        vI[q] = s_n[3 * f + q][ws];
+                        aI[q] = s_n[6 + 3 * f + q][ws];
+                    }
+                } else {
+                    fl = a.flags[I];
+                    for (int q = 0; q < 3; ++q) {
+                        vI[q] = a.gv[(3 * f + q) * a.nn + I];
+                        aI[q] = a.ga[(3 * f + q) * a.nn + I];
+                    }
                 }
                 if (fl & (CF_GATE0 << f))
                     for (int q = 0; q < 3; ++q) v[q] += N * aI[q] * dt;
@@ -1012,8 +1078,8 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
         }
     }
     for (int k = 0; k < 5; ++k) {
-        const double w = wave_sum_dpp(red[k]);
-        if (lane == 0) s_r[wv][k] = w;
+        const double wsum = wave_sum_dpp(red[k]);
+        if (lane == 0) s_r[wv][k] = wsum;
     }
     int fl = flags;
     for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
@@ -1021,7 +1087,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
     __syncthreads();
     if (threadIdx.x < 5) {
         double r = 0.0;
-        for (int w = 0; w < 4; ++w) r += s_r[w][threadIdx.x];
+        for (int q = 0; q < 4; ++q) r += s_r[q][threadIdx.x];
         a.ppart[5 * (size_t)blockIdx.x + threadIdx.x] = r;
     }
 }

@@ -683,14 +683,25 @@ static int launch_contact(mpm_handle *h, int k)
     const unsigned gp = cdiv((size_t)std::max(a.n_all, 1), 256), gn = (unsigned)c.nblk_n;
     hipStream_t st = h->stream;
     k_c_clear<real><<<gn, 256, 0, st>>>(a);
-    if (a.n_all > 0) k_c_kernel<real><<<gp, 256, 0, st>>>(a);   // kernel density, active nodes, bounds
+    // H >= 11 headroom bits keep every scaled contribution below 2^51 (the
+    // short round-to-int64 of c_toll)
+    const bool small = a.H >= 11;
+    if (a.n_all > 0) {   // kernel density, active nodes, bounds
+        if (small) k_c_kernel<real, true><<<gp, 256, 0, st>>>(a);
+        else k_c_kernel<real, false><<<gp, 256, 0, st>>>(a);
+    }
     k_c_kgrad<real><<<gn, 256, 0, st>>>(a);
     if (a.n_all > 0) {
         k_c_point1<real><<<gp, 256, 0, st>>>(a);
         k_c_point2<real><<<gp, 256, 0, st>>>(a);
         k_c_publish<real><<<gn, 256, 0, st>>>(a);
-        k_c_p2g<real, true><<<gp, 256, 0, st>>>(a);    // with detection and the field mask
-        k_c_p2g<real, false><<<gp, 256, 0, st>>>(a);
+        if (small) {   // mass with detection and the field mask, then momentum and force
+            k_c_p2g<real, true, true><<<gp, 256, 0, st>>>(a);
+            k_c_p2g<real, false, true><<<gp, 256, 0, st>>>(a);
+        } else {
+            k_c_p2g<real, true, false><<<gp, 256, 0, st>>>(a);
+            k_c_p2g<real, false, false><<<gp, 256, 0, st>>>(a);
+        }
     }
     if (a.n_dc > 0) k_c_dcmarks<real><<<cdiv(a.n_dc, 256), 256, 0, st>>>(a, k);
     k_c_grid<real><<<gn, 256, 0, st>>>(a, k);
