@@ -55,6 +55,8 @@ struct CArgs {
     int32_t *gflags;
     double *cpart;                // per node block: mass, reaction, active nodes
     double *ppart;                // per point block: sum m|v|^2, m v.xyz, nonfinite
+    const uint32_t *slot_order, *occ_start, *occ_count;   // brick blocks: slot per block, point range per slot
+    int32_t n_bb;                 // brick blocks (occupied bricks); platen blocks of 256 points follow
     int32_t H;                    // headroom bits of the fixed-point sums
     double mu;                    // Coulomb coefficient of the contact correction (0: frictionless)
 };
@@ -258,12 +260,13 @@ struct CWin {
     int lo[3], dim[3], nodes;
     bool lds;
 };
-// nodes i0 - below .. i0 + above of every valid point's cell i0, at most cap.
-// The cell bounds are reduced over each wave first (DPP), so one lane per
+// nodes lo - below .. hi + above over the cells [lo, hi] of the block's
+// points (per thread: the cells of its points, any = it has one), at most
+// cap.  The bounds are reduced over each wave first (DPP), so one lane per
 // wave updates the block's box: per-lane atomics on one LDS word would be
 // serialized lane by lane.  Call with every thread of the block.
-__device__ __forceinline__ CWin c_window(bool valid, const int i0[3], int *sbox, int below = 0, int above = 1,
-                                         int cap = kCWin)
+__device__ __forceinline__ CWin c_window_box(bool any, const int lo_in[3], const int hi_in[3], int *sbox, int below,
+                                             int above, int cap)
 {
     if (threadIdx.x < 3) {
         sbox[threadIdx.x] = 0x7FFFFFFF;
@@ -272,8 +275,8 @@ __device__ __forceinline__ CWin c_window(bool valid, const int i0[3], int *sbox,
     int lo[3], hi[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        lo[q] = wave_imin_dpp(valid ? i0[q] : (1 << 20));
-        hi[q] = wave_imax_dpp(valid ? i0[q] : -1);
+        lo[q] = wave_imin_dpp(any ? lo_in[q] : (1 << 20));
+        hi[q] = wave_imax_dpp(any ? hi_in[q] : -1);
     }
     __syncthreads();
     if ((threadIdx.x & 63) == 0 && hi[0] >= 0)
@@ -293,6 +296,49 @@ __device__ __forceinline__ CWin c_window(bool valid, const int i0[3], int *sbox,
     w.lds = nodes <= cap;
     w.nodes = w.lds ? (int)nodes : 0;
     return w;
+}
+
+// The points of a block: brick blocks (the first n_bb, in the plain step's
+// launch order, longest first) take all points of one occupied brick, which
+// are contiguous in the binned order; platen blocks take 256 platen points
+// each (indices n_def.. after the deformable points).
+struct CRange {
+    int j0, n;
+};
+template <typename real>
+__device__ __forceinline__ CRange c_range(const CArgs<real> &a)
+{
+    const int b = (int)blockIdx.x;
+    if (b < a.n_bb) {
+        const uint32_t sl = a.slot_order[b];
+        return CRange{(int)a.occ_start[sl], (int)a.occ_count[sl]};
+    }
+    const int j0 = a.n_def + (b - a.n_bb) * 256;
+    return CRange{j0, min(256, a.n_all - j0)};
+}
+// the cell range of this thread's points of the block (first loop of a pass)
+template <typename real>
+__device__ __forceinline__ bool c_range_box(const CArgs<real> &a, const CRange &r, int lo[3], int hi[3])
+{
+    bool any = false;
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = 1 << 20;
+        hi[q] = -1;
+    }
+#pragma clang loop unroll(disable)
+    for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
+        double x[3];
+        long long idx[8];
+        int ijk[8][3];
+        c_pos(a, j, x);
+        if (!c_adjacent<kCFast<real>>(a.g, x, idx, ijk)) continue;
+        any = true;
+        for (int q = 0; q < 3; ++q) {
+            lo[q] = min(lo[q], ijk[0][q]);
+            hi[q] = max(hi[q], ijk[0][q]);
+        }
+    }
+    return any;
 }
 __device__ __forceinline__ int c_wslot(const CWin &w, const int n[3])
 {
@@ -396,34 +442,56 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
     __shared__ unsigned long long s_k[kCWinK];
     __shared__ uint32_t s_f[kCWinK];
     __shared__ double s_b[4][4];
-    const int j = blockIdx.x * 256 + threadIdx.x;
     const GridParams &g = a.g;
-    double x[3] = {0.0, 0.0, 0.0};
-    long long idx[8];
-    int ijk[8][3];
-    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
-    bool valid = j < a.n_all;
+    const CRange r = c_range(a);
+    // first loop: the bounds of the P2G contributions, the box of cells
     double b[4] = {0.0, 0.0, 0.0, 0.0};
-    if (valid) {
-        c_point_bounds(a, j, b);
-        c_pos(a, j, x);
-        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
-        if (!valid) atomicOr(a.gflags, FLAG_OUT_OF_GRID);
+    int lo[3], hi[3];
+    bool any = false;
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = 1 << 20;
+        hi[q] = -1;
     }
+    int oog = 0;
+#pragma clang loop unroll(disable)
+    for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
+        double bp[4], x[3];
+        c_point_bounds(a, j, bp);
+        for (int q = 0; q < 4; ++q) b[q] = fmax(b[q], bp[q]);
+        long long idx[8];
+        int ijk[8][3];
+        c_pos(a, j, x);
+        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) {
+            oog = 1;
+            continue;
+        }
+        any = true;
+        for (int q = 0; q < 3; ++q) {
+            lo[q] = min(lo[q], ijk[0][q]);
+            hi[q] = max(hi[q], ijk[0][q]);
+        }
+    }
+    if (oog) atomicOr(a.gflags, FLAG_OUT_OF_GRID);
     c_bounds_flush(a, b, s_b);
-    const CWin w = c_window(valid, ijk[0], sbox, 1, 2, kCWinK);
+    const CWin w = c_window_box(any, lo, hi, sbox, 1, 2, kCWinK);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         s_k[t] = 0ull;
         s_f[t] = 0u;
     }
     __syncthreads();
-    if (valid) {
+    const double R = fabs(1.0 * g.h[0]);
+    const double sck = c_scale(a, 4);
+#pragma clang loop unroll(disable)
+    for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
+        double x[3];
+        long long idx[8];
+        int ijk[8][3];
+        c_pos(a, j, x);
+        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) continue;
         for (int k = 0; k < 8; ++k) {
             if (w.lds) atomicOr(&s_f[c_wslot(w, ijk[k])], CF_ACTIVE);
             else atomicOr(&a.flags[idx[k]], CF_ACTIVE);
         }
-        const double R = fabs(1.0 * g.h[0]);
-        const double sck = c_scale(a, 4);
         const int *i0 = ijk[0];
         // per axis, the candidate planes i0 - 1 + t (t = 0..3) inside the grid
         // and within R on that axis, as bits
@@ -515,18 +583,11 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
     __shared__ int sbox[6];
     __shared__ unsigned long long s_near[kCWin], s_sg[kCWin];
     __shared__ double s_kv[4][kCWin];
-    const int j = blockIdx.x * 256 + threadIdx.x;
     const GridParams &g = a.g;
-    double x[3] = {0.0, 0.0, 0.0};
-    long long idx[8];
-    int ijk[8][3];
-    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
-    bool valid = j < a.n_all;
-    if (valid) {
-        c_pos(a, j, x);
-        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
-    }
-    const CWin w = c_window(valid, ijk[0], sbox);
+    const CRange r = c_range(a);
+    int lo[3], hi[3];
+    const bool any = c_range_box(a, r, lo, hi);
+    const CWin w = c_window_box(any, lo, hi, sbox, 0, 1, kCWin);
     const double usk = c_unscale(a, 4);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         s_near[t] = 0x7FF0000000000000ull;   // +inf
@@ -536,30 +597,36 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
         for (int q = 0; q < 3; ++q) s_kv[1 + q][t] = a.gkgrad[q * a.nn + I];
     }
     __syncthreads();
-    if (valid) {
+#pragma clang loop unroll(disable)
+    for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
+        double x[3];
+        long long idx[8];
+        int ijk[8][3];
+        c_pos(a, j, x);
+        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) continue;
+        const bool surf = c_surf(a, j) != 0;
         double kp = 0.0, kg[3] = {0.0, 0.0, 0.0};
-        unsigned long long dk[8];
         CAx A;
         c_axes<kCFast<real>>(g, x, ijk[0], A);
         for (int k = 0; k < 8; ++k) {
             double xi[3], N, G[3];
             c_corner<kCFast<real>>(g, A, k, xi, N, G);
+            const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
+            const unsigned long long dk = (unsigned long long)__double_as_longlong(c_len3(d));
             if (w.lds) {
                 const int ws = c_wslot(w, ijk[k]);
                 kp += N * s_kv[0][ws];
                 for (int q = 0; q < 3; ++q) kg[q] += N * s_kv[1 + q][ws];
+                atomicMin(&s_near[ws], dk);
             } else {
                 kp += N * ((double)a.acc[idx[k]] * usk);
                 for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + idx[k]];
+                atomicMin(&a.near_d[idx[k]], dk);
             }
-            const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
-            dk[k] = (unsigned long long)__double_as_longlong(c_len3(d));
-            if (w.lds) atomicMin(&s_near[c_wslot(w, ijk[k])], dk[k]);
-            else atomicMin(&a.near_d[idx[k]], dk[k]);
         }
         a.kp[j] = kp;
         for (int q = 0; q < 3; ++q) a.kg[q * a.n_all + j] = kg[q];
-        if (c_surf(a, j)) {
+        if (surf) {
             const double lk = c_len3(kg);
             if (lk > 0.0) {
                 const unsigned long long lb = (unsigned long long)__double_as_longlong(lk);
@@ -608,7 +675,7 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
         surf = c_surf(a, j) != 0;
         lk = (unsigned long long)__double_as_longlong(c_len3(kg));
     }
-    const CWin w = c_window(valid, ijk[0], sbox);
+    const CWin w = c_window_box(valid, ijk[0], ijk[0], sbox, 0, 1, kCWin);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         const long long I = c_wnode(g, w, t);
         s_nd[t] = a.near_d[I];
@@ -697,26 +764,35 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     __shared__ unsigned long long lds[kCWin * NF];
     __shared__ double s_gk[3][NS];
     __shared__ uint32_t s_ns[NS];   // MASS: bit 0 near_surf (staged); bits CF_CPOS | CF_CNEG (detection)
-    const int j = blockIdx.x * 256 + threadIdx.x;
     const GridParams &g = a.g;
-    double x[3] = {0.0, 0.0, 0.0};
-    long long idx[8];
-    int ijk[8][3];
-    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
-    bool valid = j < a.n_all;
-    if (valid) {
-        c_pos(a, j, x);
-        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
+    const CRange r = c_range(a);
+    int lo[3], hi[3];
+    const bool any = c_range_box(a, r, lo, hi);
+    const CWin w = c_window_box(any, lo, hi, sbox, 0, 1, kCWin);
+    for (int t = threadIdx.x; t < w.nodes * NF; t += 256) lds[t] = 0ull;
+    if constexpr (MASS) {
+        for (int t = threadIdx.x; t < w.nodes; t += 256) {
+            const long long I = c_wnode(g, w, t);
+            for (int q = 0; q < 3; ++q) s_gk[q][t] = a.gkg[q * a.nn + I];
+            s_ns[t] = a.near_surf[I] ? 1u : 0u;
+        }
     }
-    // the point's own inputs are loaded before the window is set up, so
-    // their latency overlaps the window's barriers and staging.  MASS needs
-    // the mass only; momentum and force need v, V = det F V0, stress, f_ext.
-    double kg[3] = {0.0, 0.0, 0.0}, m = 0.0, v[3] = {0.0, 0.0, 0.0}, vol = 0.0, s[6] = {0, 0, 0, 0, 0, 0},
-           fe[3] = {0, 0, 0};
-    bool surf = false;
-    uint32_t fm = 0u;
-    if (valid) {
-        for (int q = 0; q < 3; ++q) kg[q] = a.kg[q * a.n_all + j];
+    __syncthreads();
+    const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
+    const double um = c_unscale(a, 0);
+#pragma clang loop unroll(disable)
+    for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
+        double x[3];
+        long long idx[8];
+        int ijk[8][3];
+        c_pos(a, j, x);
+        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) continue;
+        // MASS needs the mass only; momentum and force need v, V = det F V0,
+        // stress and f_ext
+        double kg[3] = {0.0, 0.0, 0.0}, m, v[3] = {0.0, 0.0, 0.0}, vol = 0.0, s[6] = {0, 0, 0, 0, 0, 0},
+               fe[3] = {0, 0, 0};
+        if constexpr (MASS)
+            for (int q = 0; q < 3; ++q) kg[q] = a.kg[q * a.n_all + j];
         if (j >= a.n_def) {
             const int d = j - a.n_def;
             m = a.dc_m[d];
@@ -738,22 +814,8 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
                 for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
             }
         }
-        if constexpr (MASS) surf = c_surf(a, j) != 0;
-        else fm = a.fmask[j];
-    }
-    const CWin w = c_window(valid, ijk[0], sbox);
-    for (int t = threadIdx.x; t < w.nodes * NF; t += 256) lds[t] = 0ull;
-    if constexpr (MASS) {
-        for (int t = threadIdx.x; t < w.nodes; t += 256) {
-            const long long I = c_wnode(g, w, t);
-            for (int q = 0; q < 3; ++q) s_gk[q][t] = a.gkg[q * a.nn + I];
-            s_ns[t] = a.near_surf[I] ? 1u : 0u;
-        }
-    }
-    __syncthreads();
-    if (valid) {
-        const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
-        const double um = c_unscale(a, 0);
+        const bool surf = MASS && c_surf(a, j) != 0;
+        const uint32_t fm = MASS ? 0u : (uint32_t)a.fmask[j];
         uint32_t mask = 0u;
         CAx A;
         c_axes<kCFast<real>>(g, x, ijk[0], A);
@@ -989,7 +1051,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
         for (int q = 0; q < 3; ++q) v[q] = (double)a.p.v[q][j];
     }
     // the block's node window (velocity, gated acceleration, flags) staged in LDS
-    const CWin w = c_window(valid, ijk[0], sbox);
+    const CWin w = c_window_box(valid, ijk[0], ijk[0], sbox, 0, 1, kCWin);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         const long long I = c_wnode(g, w, t);
         s_fl[t] = a.flags[I];

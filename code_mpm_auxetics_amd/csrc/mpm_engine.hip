@@ -157,7 +157,8 @@ struct mpm_handle {
         uint8_t *near_surf = nullptr, *fmask = nullptr;
         double *kp = nullptr, *kg = nullptr, *cpart = nullptr, *ppart = nullptr;
         double *kp_id = nullptr, *kg_id = nullptr;   // by original point index
-        int nblk_n = 0, nblk_p = 0;
+        int nblk_n = 0, nblk_p = 0;         // node blocks; capacity of the point-sum partials
+        int n_ppart = 0;                    // point-sum partials the last contact G2P wrote
         bool kernel_valid = false;          // kp/kg hold the last contact step's point kernel
     } ct;
     int last_path = 0;                      // path of the last step (checks read its partials)
@@ -666,6 +667,10 @@ static CArgs<real> contact_args(mpm_handle *h)
     a.steps = h->steps_d;
     a.gflags = h->flags_d;
     a.ppart = c.ppart;
+    a.slot_order = h->slot_order;
+    a.occ_start = h->occ_start;
+    a.occ_count = h->occ_count;
+    a.n_bb = h->n_def > 0 ? h->n_occ : 0;
     int H = 2;
     while ((1ll << (H - 2)) <= (long long)a.n_all) ++H;   // 2^(H-2) > points touching a node
     a.H = H;
@@ -680,20 +685,22 @@ static int launch_contact(mpm_handle *h, int k)
     auto &c = h->ct;
     CArgs<real> a = contact_args<real>(h);
     a.cpart = c.cpart + (size_t)k * c.nblk_n * 3;
-    const unsigned gp = cdiv((size_t)std::max(a.n_all, 1), 256), gn = (unsigned)c.nblk_n;
+    // point passes: one block per occupied brick (all its points), then
+    // blocks of 256 platen points; G2P: blocks of 256 deformable points
+    const unsigned gp = (unsigned)a.n_bb + cdiv((size_t)a.n_dc, 256), gn = (unsigned)c.nblk_n;
     hipStream_t st = h->stream;
     k_c_clear<real><<<gn, 256, 0, st>>>(a);
     // H >= 11 headroom bits keep every scaled contribution below 2^51 (the
     // short round-to-int64 of c_toll)
     const bool small = a.H >= 11;
-    if (a.n_all > 0) {   // kernel density, active nodes, bounds
+    if (gp > 0) {   // kernel density, active nodes, bounds
         if (small) k_c_kernel<real, true><<<gp, 256, 0, st>>>(a);
         else k_c_kernel<real, false><<<gp, 256, 0, st>>>(a);
     }
     k_c_kgrad<real><<<gn, 256, 0, st>>>(a);
-    if (a.n_all > 0) {
+    if (gp > 0) {
         k_c_point1<real><<<gp, 256, 0, st>>>(a);
-        k_c_point2<real><<<gp, 256, 0, st>>>(a);
+        k_c_point2<real><<<cdiv((size_t)a.n_all, 256), 256, 0, st>>>(a);   // blocks of 256 points
         k_c_publish<real><<<gn, 256, 0, st>>>(a);
         if (small) {   // mass with detection and the field mask, then momentum and force
             k_c_p2g<real, true, true><<<gp, 256, 0, st>>>(a);
@@ -706,6 +713,7 @@ static int launch_contact(mpm_handle *h, int k)
     if (a.n_dc > 0) k_c_dcmarks<real><<<cdiv(a.n_dc, 256), 256, 0, st>>>(a, k);
     k_c_grid<real><<<gn, 256, 0, st>>>(a, k);
     if (a.n_def > 0) k_c_g2p<real><<<(unsigned)c.nblk_p, 256, 0, st>>>(a, k);
+    c.n_ppart = a.n_def > 0 ? c.nblk_p : 0;
     if (a.n_dc > 0) k_c_dcmove<real><<<cdiv(a.n_dc, 256), 256, 0, st>>>(a, k);
     HIPCHK(h, hipGetLastError());
     c.kernel_valid = true;
@@ -1139,7 +1147,7 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
 static int checks_impl(mpm_handle *h, double out[12])
 {
     if (h->last_path == 1)
-        k_finalize<<<1, 256, 0, h->stream>>>(h->ct.ppart, nullptr, h->ct.nblk_p, h->grid3_d, h->dc_v[h->parity],
+        k_finalize<<<1, 256, 0, h->stream>>>(h->ct.ppart, nullptr, h->ct.n_ppart, h->grid3_d, h->dc_v[h->parity],
                                            h->dc_m, h->n_dc, h->checks_d);
     else
         k_finalize<<<1, 256, 0, h->stream>>>(h->kp_part, h->n_occ_d, 0, h->grid3_d, h->dc_v[h->parity], h->dc_m,
