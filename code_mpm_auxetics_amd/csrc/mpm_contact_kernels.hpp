@@ -755,6 +755,170 @@ __device__ __forceinline__ long long c_p2g_off(int s)
     const int f = s / 6, q = s % 6;
     return q < 3 ? off_mv(f, q) : off_f(f, q - 3);
 }
+// The inputs of one point for the split P2G.  MASS needs the mass, the point
+// kernel gradient and the surface flag; momentum and force need v,
+// V = det F V0, stress, f_ext and the field mask.
+struct CP2gIn {
+    double x[3], m, kg[3], v[3], vol, s[6], fe[3];
+    bool surf;
+    uint32_t fm;
+};
+// Raw stored values of a deformable point, loaded one iteration ahead of
+// their use (the loads of the next point overlap the current point's work).
+template <typename real>
+struct CP2gRaw {
+    int32_t xq[3];   // FP32 state
+    uint32_t pb;     // FP32 state
+    real x[3];       // FP64 state
+    real m, v0, F[9], v[3], s[6], fe[3];
+    double kg[3];
+    uint8_t surf, fm;
+};
+template <typename real, bool MASS>
+__device__ __forceinline__ void c_p2g_raw(const CArgs<real> &a, int j, CP2gRaw<real> &r)
+{
+    const PArrays<real> &P = a.p;
+    if constexpr (sizeof(real) == 4) {
+        for (int q = 0; q < 3; ++q) r.xq[q] = P.xq[q][j];
+        r.pb = P.pbrick[j];
+    } else {
+        for (int q = 0; q < 3; ++q) r.x[q] = P.x[q][j];
+    }
+    r.m = P.mass[j];
+    if constexpr (MASS) {
+        for (int q = 0; q < 3; ++q) r.kg[q] = a.kg[q * a.n_all + j];
+        r.surf = a.surf[j];
+    } else {
+        r.v0 = P.vol0[j];
+        for (int q = 0; q < 9; ++q) r.F[q] = P.F[q][j];
+        for (int q = 0; q < 3; ++q) {
+            r.v[q] = P.v[q][j];
+            r.fe[q] = a.g.has_fext ? P.fext[q][j] : real(0);
+        }
+        for (int q = 0; q < 6; ++q) r.s[q] = P.s[q][j];
+        r.fm = a.fmask[j];
+    }
+}
+// the same quantities as c_pos and the P2G's point loads, from the raw values
+template <typename real, bool MASS>
+__device__ __forceinline__ void c_p2g_in_raw(const CArgs<real> &a, const CP2gRaw<real> &r, CP2gIn &in)
+{
+    const GridParams &g = a.g;
+    if constexpr (sizeof(real) == 4) {
+        const uint32_t b = r.pb;
+        const int bc[3] = {(int)(b % g.nb[0]), (int)((b / g.nb[0]) % g.nb[1]), (int)(b / (g.nb[0] * g.nb[1]))};
+        for (int q = 0; q < 3; ++q) in.x[q] = ((double)(bc[q] * g.S[q]) + (double)r.xq[q] * (1.0 / kQOne)) * g.h[q];
+    } else {
+        for (int q = 0; q < 3; ++q) in.x[q] = r.x[q];
+    }
+    in.m = (double)r.m;
+    if constexpr (MASS) {
+        for (int q = 0; q < 3; ++q) in.kg[q] = r.kg[q];
+        in.surf = r.surf != 0;
+    } else {
+        double Fd[9];
+        for (int q = 0; q < 9; ++q) Fd[q] = fd(r.F[q], q);
+        in.vol = (double)(real)(det3(Fd) * (double)r.v0);
+        for (int q = 0; q < 3; ++q) {
+            in.v[q] = (double)r.v[q];
+            in.fe[q] = g.has_fext ? (double)r.fe[q] : 0.0;
+        }
+        for (int q = 0; q < 6; ++q) in.s[q] = (double)r.s[q];
+        in.fm = r.fm;
+    }
+}
+// a platen point's inputs (FP64 arrays)
+template <typename real, bool MASS>
+__device__ __forceinline__ void c_p2g_in_dc(const CArgs<real> &a, int j, CP2gIn &in)
+{
+    const int d = j - a.n_def;
+    for (int q = 0; q < 3; ++q) in.x[q] = a.dc_x[3 * d + q];
+    in.m = a.dc_m[d];
+    if constexpr (MASS) {
+        for (int q = 0; q < 3; ++q) in.kg[q] = a.kg[q * a.n_all + j];
+        in.surf = a.dc_surf[d] != 0;
+    } else {
+        in.vol = a.dc_vol[d];
+        for (int q = 0; q < 3; ++q) { in.v[q] = a.dc_v[3 * d + q]; in.fe[q] = a.dc_fext[3 * d + q]; }
+        for (int q = 0; q < 6; ++q) in.s[q] = a.dc_s[6 * d + q];
+        in.fm = a.fmask[j];
+    }
+}
+// one point's split P2G into the block's window (or global memory when the
+// window did not fit)
+template <typename real, bool MASS, bool SMALL>
+__device__ __forceinline__ void c_p2g_point(const CArgs<real> &a, const CWin &w, unsigned long long *lds,
+                                            const double *s_gk, uint32_t *s_ns, int j, const CP2gIn &in,
+                                            double sc0, double sc1, double um)
+{
+    const GridParams &g = a.g;
+    long long idx[8];
+    int ijk[8][3];
+    if (!c_adjacent<kCFast<real>>(g, in.x, idx, ijk)) return;
+    uint32_t mask = 0u;
+    CAx A;
+    c_axes<kCFast<real>>(g, in.x, ijk[0], A);
+    for (int k = 0; k < 8; ++k) {
+        const long long I = idx[k];
+        double xi[3], N, G[3];
+        c_corner<kCFast<real>>(g, A, k, xi, N, G);
+        const int ws = w.lds ? c_wslot(w, ijk[k]) : 0;
+        int f;
+        bool ns = false;
+        if constexpr (MASS) {
+            double gk[3];
+            if (w.lds) {
+                for (int q = 0; q < 3; ++q) gk[q] = s_gk[q * kCWin + ws];
+                ns = (s_ns[ws] & 1u) != 0;
+            } else {
+                for (int q = 0; q < 3; ++q) gk[q] = a.gkg[q * a.nn + I];
+                ns = a.near_surf[I] != 0;
+            }
+            f = (c_dot3(gk, in.kg) >= 0.0) ? 0 : 1;
+            mask |= (uint32_t)f << k;
+            if (in.surf && ns) {
+                const uint32_t cf = f == 0 ? CF_CPOS : CF_CNEG;
+                if (w.lds) atomicOr(&s_ns[ws], cf);
+                else atomicOr(&a.flags[I], cf);
+            }
+        } else {
+            f = (int)((in.fm >> k) & 1u);
+        }
+        long long iv[6];
+        int base, cnt = 0;
+        if (MASS) {
+            base = 4 * f;
+            iv[0] = c_toll<SMALL>((N * in.m) * sc0);
+            for (int q = 0; q < 3; ++q) iv[1 + q] = c_toll<SMALL>((G[q] * in.m) * sc1);
+            // grad m is read only by the contact correction, at nodes
+            // flagged CF_CNEG, which requires a surface nearest point:
+            // elsewhere its sums are never used and are not formed
+            cnt = ns ? 4 : 1;
+        } else {
+            base = 6 * f;
+            const double mI = (double)a.acc[off_m(f) * a.nn + I] * um;
+            const bool gate = c_lenm3(mI) > g.m_min;
+            if (kCFast<real>) {
+                const double rmI = gate ? 1.0 / mI : 0.0;
+                for (int q = 0; q < 3; ++q) iv[q] = gate ? c_toll<SMALL>((N * (in.m * in.v[q]) * rmI) * sc0) : 0ll;
+            } else {
+                for (int q = 0; q < 3; ++q) iv[q] = gate ? c_toll<SMALL>((N * (in.m * in.v[q]) / mI) * sc0) : 0ll;
+            }
+            const double *sv = in.s;
+            const double fr[3] = {-in.vol * (G[0] * sv[0] + G[1] * sv[3] + G[2] * sv[5]) + N * in.fe[0],
+                                  -in.vol * (G[1] * sv[1] + G[0] * sv[3] + G[2] * sv[4]) + N * in.fe[1],
+                                  -in.vol * (G[2] * sv[2] + G[0] * sv[5] + G[1] * sv[4]) + N * in.fe[2]};
+            for (int q = 0; q < 3; ++q) iv[3 + q] = c_toll<SMALL>(fr[q] * sc1);
+            cnt = 6;
+        }
+        for (int q = 0; q < cnt; ++q) {
+            if (w.lds) atomicAdd(&lds[(base + q) * w.nodes + ws], (unsigned long long)iv[q]);
+            else atomicAdd((unsigned long long *)&a.acc[c_p2g_off<MASS>(base + q) * a.nn + I], (unsigned long long)iv[q]);
+        }
+    }
+    if constexpr (MASS) a.fmask[j] = (uint8_t)mask;
+}
+
 template <typename real, bool MASS, bool SMALL>
 __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
 {
@@ -762,7 +926,7 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     constexpr int NS = MASS ? kCWin : 1;   // staged surface gradients and flags (MASS only)
     __shared__ int sbox[6];
     __shared__ unsigned long long lds[kCWin * NF];
-    __shared__ double s_gk[3][NS];
+    __shared__ double s_gk[3 * NS];   // MASS: staged surface gradients, component q at q * kCWin
     __shared__ uint32_t s_ns[NS];   // MASS: bit 0 near_surf (staged); bits CF_CPOS | CF_CNEG (detection)
     const GridParams &g = a.g;
     const CRange r = c_range(a);
@@ -773,110 +937,34 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     if constexpr (MASS) {
         for (int t = threadIdx.x; t < w.nodes; t += 256) {
             const long long I = c_wnode(g, w, t);
-            for (int q = 0; q < 3; ++q) s_gk[q][t] = a.gkg[q * a.nn + I];
+            for (int q = 0; q < 3; ++q) s_gk[q * kCWin + t] = a.gkg[q * a.nn + I];
             s_ns[t] = a.near_surf[I] ? 1u : 0u;
         }
     }
     __syncthreads();
     const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
     const double um = c_unscale(a, 0);
+    const int end = r.j0 + r.n;
+    if ((int)blockIdx.x < a.n_bb) {
+        // deformable points of one brick, the next point's inputs in flight
+        int j = r.j0 + (int)threadIdx.x;
+        CP2gRaw<real> cur, nxt;
+        if (j < end) c_p2g_raw<real, MASS>(a, j, cur);
 #pragma clang loop unroll(disable)
-    for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
-        double x[3];
-        long long idx[8];
-        int ijk[8][3];
-        c_pos(a, j, x);
-        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) continue;
-        // MASS needs the mass only; momentum and force need v, V = det F V0,
-        // stress and f_ext
-        double kg[3] = {0.0, 0.0, 0.0}, m, v[3] = {0.0, 0.0, 0.0}, vol = 0.0, s[6] = {0, 0, 0, 0, 0, 0},
-               fe[3] = {0, 0, 0};
-        if constexpr (MASS)
-            for (int q = 0; q < 3; ++q) kg[q] = a.kg[q * a.n_all + j];
-        if (j >= a.n_def) {
-            const int d = j - a.n_def;
-            m = a.dc_m[d];
-            if constexpr (!MASS) {
-                vol = a.dc_vol[d];
-                for (int q = 0; q < 3; ++q) { v[q] = a.dc_v[3 * d + q]; fe[q] = a.dc_fext[3 * d + q]; }
-                for (int q = 0; q < 6; ++q) s[q] = a.dc_s[6 * d + q];
-            }
-        } else {
-            m = (double)a.p.mass[j];
-            if constexpr (!MASS) {
-                double Fd[9];
-                for (int q = 0; q < 9; ++q) Fd[q] = fd(a.p.F[q][j], q);
-                vol = (double)(real)(det3(Fd) * (double)a.p.vol0[j]);
-                for (int q = 0; q < 3; ++q) {
-                    v[q] = (double)a.p.v[q][j];
-                    if (g.has_fext) fe[q] = (double)a.p.fext[q][j];
-                }
-                for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
-            }
+        for (; j < end; j += 256) {
+            if (j + 256 < end) c_p2g_raw<real, MASS>(a, j + 256, nxt);
+            CP2gIn in;
+            c_p2g_in_raw<real, MASS>(a, cur, in);
+            c_p2g_point<real, MASS, SMALL>(a, w, lds, s_gk, s_ns, j, in, sc0, sc1, um);
+            cur = nxt;
         }
-        const bool surf = MASS && c_surf(a, j) != 0;
-        const uint32_t fm = MASS ? 0u : (uint32_t)a.fmask[j];
-        uint32_t mask = 0u;
-        CAx A;
-        c_axes<kCFast<real>>(g, x, ijk[0], A);
-        for (int k = 0; k < 8; ++k) {
-            const long long I = idx[k];
-            double xi[3], N, G[3];
-            c_corner<kCFast<real>>(g, A, k, xi, N, G);
-            const int ws = w.lds ? c_wslot(w, ijk[k]) : 0;
-            int f;
-            bool ns = false;
-            if constexpr (MASS) {
-                double gk[3];
-                if (w.lds) {
-                    for (int q = 0; q < 3; ++q) gk[q] = s_gk[q][ws];
-                    ns = (s_ns[ws] & 1u) != 0;
-                } else {
-                    for (int q = 0; q < 3; ++q) gk[q] = a.gkg[q * a.nn + I];
-                    ns = a.near_surf[I] != 0;
-                }
-                f = (c_dot3(gk, kg) >= 0.0) ? 0 : 1;
-                mask |= (uint32_t)f << k;
-                if (surf && ns) {
-                    const uint32_t cf = f == 0 ? CF_CPOS : CF_CNEG;
-                    if (w.lds) atomicOr(&s_ns[ws], cf);
-                    else atomicOr(&a.flags[I], cf);
-                }
-            } else {
-                f = (int)((fm >> k) & 1u);
-            }
-            long long iv[6];
-            int base, cnt = 0;
-            if (MASS) {
-                base = 4 * f;
-                iv[0] = c_toll<SMALL>((N * m) * sc0);
-                for (int q = 0; q < 3; ++q) iv[1 + q] = c_toll<SMALL>((G[q] * m) * sc1);
-                // grad m is read only by the contact correction, at nodes
-                // flagged CF_CNEG, which requires a surface nearest point:
-                // elsewhere its sums are never used and are not formed
-                cnt = ns ? 4 : 1;
-            } else {
-                base = 6 * f;
-                const double mI = (double)a.acc[off_m(f) * a.nn + I] * um;
-                const bool gate = c_lenm3(mI) > g.m_min;
-                if (kCFast<real>) {
-                    const double rmI = gate ? 1.0 / mI : 0.0;
-                    for (int q = 0; q < 3; ++q) iv[q] = gate ? c_toll<SMALL>((N * (m * v[q]) * rmI) * sc0) : 0ll;
-                } else {
-                    for (int q = 0; q < 3; ++q) iv[q] = gate ? c_toll<SMALL>((N * (m * v[q]) / mI) * sc0) : 0ll;
-                }
-                const double fr[3] = {-vol * (G[0] * s[0] + G[1] * s[3] + G[2] * s[5]) + N * fe[0],
-                                      -vol * (G[1] * s[1] + G[0] * s[3] + G[2] * s[4]) + N * fe[1],
-                                      -vol * (G[2] * s[2] + G[0] * s[5] + G[1] * s[4]) + N * fe[2]};
-                for (int q = 0; q < 3; ++q) iv[3 + q] = c_toll<SMALL>(fr[q] * sc1);
-                cnt = 6;
-            }
-            for (int q = 0; q < cnt; ++q) {
-                if (w.lds) atomicAdd(&lds[(base + q) * w.nodes + ws], (unsigned long long)iv[q]);
-                else atomicAdd((unsigned long long *)&a.acc[c_p2g_off<MASS>(base + q) * a.nn + I], (unsigned long long)iv[q]);
-            }
+    } else {
+#pragma clang loop unroll(disable)
+        for (int j = r.j0 + (int)threadIdx.x; j < end; j += 256) {
+            CP2gIn in;
+            c_p2g_in_dc<real, MASS>(a, j, in);
+            c_p2g_point<real, MASS, SMALL>(a, w, lds, s_gk, s_ns, j, in, sc0, sc1, um);
         }
-        if constexpr (MASS) a.fmask[j] = (uint8_t)mask;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
