@@ -57,6 +57,7 @@ struct CArgs {
     double *ppart;                // per point block: sum m|v|^2, m v.xyz, nonfinite
     const uint32_t *slot_order, *occ_start, *occ_count;   // brick blocks: slot per block, point range per slot
     int32_t n_bb;                 // brick blocks (occupied bricks); platen blocks of 256 points follow
+    int32_t n_xcd;                // XCDs the dispatcher deals workgroups to, round robin by block index
     int32_t H;                    // headroom bits of the fixed-point sums
     double mu;                    // Coulomb coefficient of the contact correction (0: frictionless)
 };
@@ -296,6 +297,16 @@ __device__ __forceinline__ CWin c_window_box(bool any, const int lo_in[3], const
     w.lds = nodes <= cap;
     w.nodes = w.lds ? (int)nodes : 0;
     return w;
+}
+
+// Chunk of 256 consecutive points taken by this block in a launch of nblk
+// chunk blocks: block b runs on XCD b mod X, so XCD x takes the x-th run of
+// consecutive chunks, in order.  The chunks of one brick then share an L2
+// (each stages the brick's node box).  A bijection of [0, nblk).
+__device__ __forceinline__ int c_chunk(int nblk, int X)
+{
+    const int b = (int)blockIdx.x, x = b % X, k = b / X, q = nblk / X, r = nblk % X;
+    return x * q + min(x, r) + k;
 }
 
 // The points of a block: brick blocks (the first n_bb, in the plain step's
@@ -655,7 +666,7 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
 {
     __shared__ int sbox[6];
     __shared__ unsigned long long s_nd[kCWin], s_sd[kCWin], s_ni[kCWin], s_si[kCWin];
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int j = c_chunk((int)gridDim.x, a.n_xcd) * 256 + threadIdx.x;
     const GridParams &g = a.g;
     double x[3] = {0.0, 0.0, 0.0};
     long long idx[8];
@@ -1118,7 +1129,8 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
     __shared__ int sbox[6];
     __shared__ double s_n[12][kCWin];   // per window node: v.xyz of field f at 3f, a.xyz at 6 + 3f
     __shared__ uint32_t s_fl[kCWin];
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int chunk = c_chunk((int)gridDim.x, a.n_xcd);
+    const int j = chunk * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const GridParams &g = a.g;
     double red[5] = {0, 0, 0, 0, 0};
@@ -1238,7 +1250,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
     if (threadIdx.x < 5) {
         double r = 0.0;
         for (int q = 0; q < 4; ++q) r += s_r[q][threadIdx.x];
-        a.ppart[5 * (size_t)blockIdx.x + threadIdx.x] = r;
+        a.ppart[5 * (size_t)chunk + threadIdx.x] = r;
     }
 }
 

@@ -671,6 +671,7 @@ static CArgs<real> contact_args(mpm_handle *h)
     a.occ_start = h->occ_start;
     a.occ_count = h->occ_count;
     a.n_bb = h->n_def > 0 ? h->n_occ : 0;
+    a.n_xcd = std::max(h->n_xcd, 1);
     int H = 2;
     while ((1ll << (H - 2)) <= (long long)a.n_all) ++H;   // 2^(H-2) > points touching a node
     a.H = H;
