@@ -1302,11 +1302,14 @@ __global__ void __launch_bounds__(256) k_c_nodes(CArgs<real> a, NodeOut o)
 {
     const long long I = blockIdx.x * 256ll + threadIdx.x;
     if (I >= a.nn) return;
-    o.active[I] = (a.flags[I] & CF_ACTIVE) ? 1 : 0;
-    o.mass[I] = a.gm[I];
+    // the step writes node fields of active nodes only; the others read as
+    // the reference's reset node, zero (…_MP_Contact.cpp:41-67)
+    const bool act = (a.flags[I] & CF_ACTIVE) != 0;
+    o.active[I] = act ? 1 : 0;
+    o.mass[I] = act ? a.gm[I] : 0.0;
     for (int q = 0; q < 3; ++q) {
-        o.v[3 * I + q] = a.gv[q * a.nn + I];
-        o.f[3 * I + q] = a.gf[q * a.nn + I];
+        o.v[3 * I + q] = act ? a.gv[q * a.nn + I] : 0.0;
+        o.f[3 * I + q] = act ? a.gf[q * a.nn + I] : 0.0;
         o.kg[3 * I + q] = a.gkgrad[q * a.nn + I];
     }
     o.k[I] = (double)a.acc[off_k() * a.nn + I] * c_unscale(a, 4);

@@ -184,3 +184,8 @@ def test_grid_nodes_contact_vs_oracle(mpm, oracle_mod):
     assert rel_l2(gk["kernel"], a["gk"]) <= 1e-12
     assert rel_l2(gk["kernel_grad"], a["gkgrad"]) <= 1e-9
     assert np.abs(a["gkgrad"]).max() > 0
+    # nodes the last step did not touch read as reset nodes (…_MP_Contact.cpp:41-67)
+    idle = gg["active"] == 0
+    assert idle.sum() > 0 and not act[idle].any()
+    for k in ("mass", "v", "f"):
+        assert not gg[k][idle].any(), k
