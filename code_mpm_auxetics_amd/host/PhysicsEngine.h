@@ -53,6 +53,21 @@ class MaterialPoint {
     double d_Hardening_Isotropic_C1 = 0.0;
 };
 
+// GridPoint.h:6-33 fields, filled by getGridPoints() (mpm_grid_nodes); the
+// kernel grid (d_Kernel, d3_Kernel_Gradient) of the contact step is carried
+// too.  b3_Fixed holds the fixity per axis.
+class GridPoint {
+  public:
+    bool b_Active = false;
+    std::array<int, 3> i3_Index{};
+    std::array<bool, 3> b3_Fixed{};
+    std::array<double, 3> d3_Position{};
+    double d_Mass = 0.0;
+    std::array<double, 3> d3_Velocity{}, d3_Force{}, d3_Force_Temp{};
+    double d_Kernel = 0.0;
+    std::array<double, 3> d3_Kernel_Gradient{};
+};
+
 // MaterialPoint_Factory::getScript (MaterialPoint_Factory_getScript.cpp:4-28):
 // the header line for nullptr, else "ID mass position_x position_y position_z",
 // tab-separated with a trailing tab; doubles as Script(d, 8) writes them
@@ -75,6 +90,7 @@ class PhysicsEngine {
         mpm_handle *h = nullptr;
         check(mpm_create(&cfg, &h), nullptr);
         h_.reset(h);
+        for (int a = 0; a < 3; ++a) cells_[a] = cfg.cells[a];
         t_end_ = cfg.time_end;
         track_strain_ = cfg.track_strain != 0;
     }
@@ -148,6 +164,40 @@ class PhysicsEngine {
         return out;
     }
 
+    // PhysicsEngine.h:65-66: the grid nodes (x fastest, GridPoint_Factory.h:23-32).
+    // On the plain path the step keeps no node grid, so these are the next
+    // step's nodes (the P2G of the current points and the grid update); after
+    // a contact step, that step's first field and its kernel grid.
+    std::vector<GridPoint *> getGridPoints()
+    {
+        refreshNodes();
+        std::vector<GridPoint *> out;
+        out.reserve(nodes_.size());
+        for (GridPoint &g : nodes_) out.push_back(&g);
+        return out;
+    }
+    std::vector<GridPoint *> getGridPoints_Kernel() { return getGridPoints(); }
+
+    // ConstitutiveRelation::calculateState_J2 (ConstitutiveRelation_calculateState_J2.cpp:4-26)
+    // of every point's stress and plastic strain, evaluated on the device
+    // (MPM_F_J2), in point order: what the drawing colours by
+    // (GraphicsEngine_drawGame.cpp:164,178,252)
+    void getState_J2(std::vector<double> &j2_stress, std::vector<double> &j2_plastic)
+    {
+        const size_t n = mirror_.size();
+        std::vector<double> j2(2 * n);
+        mpm_points P{};
+        P.n = (int32_t)n;
+        P.j2 = j2.data();
+        check(mpm_readback(h_.get(), MPM_F_J2, &P), h_.get());
+        j2_stress.resize(n);
+        j2_plastic.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            j2_stress[i] = j2[2 * i];
+            j2_plastic[i] = j2[2 * i + 1];
+        }
+    }
+
     // reportConsole sums (PhysicsEngine.cpp:33-97): grid mass, reaction
     // (Force_y), sum m|v|^2, momentum, non-finite count, time, cycle, ...
     std::array<double, 12> checks() const
@@ -178,7 +228,9 @@ class PhysicsEngine {
     };
     std::unique_ptr<mpm_handle, Del> h_;
     std::vector<MaterialPoint> mirror_;
+    std::vector<GridPoint> nodes_;
     bool stale_ = true;
+    std::array<int, 3> cells_{};
     double t_end_ = 0.0;
     bool track_strain_ = false;
 
@@ -192,6 +244,34 @@ class PhysicsEngine {
         check(mpm_step(h_.get(), dt_total, path, &fin), h_.get());
         stale_ = true;
         return fin;
+    }
+    void refreshNodes()
+    {
+        const size_t nx = (size_t)cells_[0] + 1, ny = (size_t)cells_[1] + 1, nz = (size_t)cells_[2] + 1;
+        const size_t nn = nx * ny * nz;
+        std::vector<uint8_t> act(nn), fix(nn);
+        std::vector<double> x(3 * nn), m(nn), v(3 * nn), f(3 * nn), ft(3 * nn), k(nn), kg(3 * nn);
+        mpm_nodes N{};
+        N.n = (int64_t)nn;
+        N.active = act.data(); N.fixed = fix.data(); N.x = x.data(); N.mass = m.data(); N.v = v.data();
+        N.f = f.data(); N.f_temp = ft.data(); N.kernel = k.data(); N.kernel_grad = kg.data();
+        check(mpm_grid_nodes(h_.get(), &N), h_.get());
+        nodes_.resize(nn);
+        for (size_t I = 0; I < nn; ++I) {
+            GridPoint &g = nodes_[I];
+            g.b_Active = act[I] != 0;
+            g.i3_Index = {(int)(I % nx), (int)((I / nx) % ny), (int)(I / (nx * ny))};
+            for (int a = 0; a < 3; ++a) {
+                g.b3_Fixed[a] = ((fix[I] >> a) & 1u) != 0;
+                g.d3_Position[a] = x[3 * I + a];
+                g.d3_Velocity[a] = v[3 * I + a];
+                g.d3_Force[a] = f[3 * I + a];
+                g.d3_Force_Temp[a] = ft[3 * I + a];
+                g.d3_Kernel_Gradient[a] = kg[3 * I + a];
+            }
+            g.d_Mass = m[I];
+            g.d_Kernel = k[I];
+        }
     }
     void refresh()
     {

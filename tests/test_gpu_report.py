@@ -89,6 +89,32 @@ def test_driver_writes_description_log_and_snapshots(tmp_path):
     assert len(snaps) == 2 and all(re.fullmatch(r"Snapshot_\d{6}\.bmp", s) for s in snaps)
 
 
+def test_driver_writes_the_three_canvases(tmp_path):
+    """--canvas ALL: per snapshot the MAIN view and the two J2 views of the
+    device J2 readback.  The elastic world's stress view runs from blue to
+    the pure red of its largest J2; with no plastic strain its plastic view
+    is blue points on the black background."""
+    spec = tmp_path / "w.json"
+    spec.write_text(json.dumps({"world": "small", "set": {"console_interval": 1.0}}))
+    rc = run.main(["--world", str(spec), "--snapshot-dir", str(tmp_path / "bmp"), "--snapshot-interval", "0",
+                   "--frames", "2", "--frame-steps", "40", "--canvas", "ALL"])
+    assert rc == 0
+    snaps = sorted(p.name for p in (tmp_path / "bmp").glob("Snapshot_*.bmp"))
+    assert len(snaps) == 3
+
+    def pixels(suffix):
+        d = (tmp_path / "bmp" / [s for s in snaps if s.endswith(suffix)][0]).read_bytes()
+        w, h = np.frombuffer(d[18:26], "<i4")
+        stride = (3 * w + 3) // 4 * 4
+        rows = np.frombuffer(d[54:], np.uint8).reshape(h, stride)[:, :3 * w]
+        return rows.reshape(-1, 3)[:, ::-1]   # RGB
+
+    ps = pixels("_J2_STRESS.bmp")
+    assert ((ps == (255, 0, 0)).all(axis=1)).any() and (ps[:, 2] > 0).any()
+    pp = pixels("_J2_PLASTICSTRAIN.bmp")
+    assert (pp[:, 0] == 0).all() and ((pp == (0, 0, 255)).all(axis=1)).any()
+
+
 def test_text_dumps_of_the_live_state(tmp_path):
     sc = scene.small_world(steps=40)
     with PhysicsEngine(sc, precision=FP64) as eng:

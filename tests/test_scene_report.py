@@ -107,3 +107,40 @@ def test_render_colors_platens_white():
     assert img.shape[2] == 3
     cols = {tuple(c) for c in img.reshape(-1, 3)}
     assert (255, 255, 255) in cols and (255, 0, 0) in cols
+
+
+def test_canvas_colors_follow_drawgame():
+    """drawGame's canvases (GraphicsEngine_drawGame.cpp:73-83, 161-172, 245-257):
+    MAIN by flags; J2 canvases blend blue to red by J2 over the largest J2
+    (stress) or over yield stress / Young's modulus of point 0 (plastic strain)."""
+    sc = scene.small_world(material="j2")
+    main = run.canvas_colors("MAIN", sc)
+    assert (main[sc.dc != 0] == (255, 255, 255)).all()
+    assert (main[(sc.dc == 0) & (sc.surface == 0)] == (255, 0, 0)).all()
+    j2 = np.zeros((sc.n, 2))
+    j2[:, 0] = np.linspace(0.0, 4.0, sc.n)
+    j2[:, 1] = np.linspace(0.0, 2.0, sc.n) * sc.sy[0] / sc.E[0]
+    cs = run.canvas_colors("J2_STRESS", sc, j2)
+    assert tuple(cs[0]) == (0, 0, 255) and tuple(cs[-1]) == (255, 0, 0)
+    mid = np.argmin(np.abs(j2[:, 0] - 2.0))
+    w = np.float32(j2[mid, 0]) / np.float32(4.0)
+    assert tuple(cs[mid]) == (int(np.rint(w * 255)), 0, int(np.rint((1 - w) * 255)))
+    cp = run.canvas_colors("J2_PLASTICSTRAIN", sc, j2)
+    assert tuple(cp[0]) == (0, 0, 255)
+    # beyond the yield strain the blend leaves [0, 1]; the framebuffer clamps it
+    assert tuple(cp[-1]) == (255, 0, 0) and (cp[j2[:, 1] > sc.sy[0] / sc.E[0]][:, 2] == 0).all()
+    # an all-zero stress field: J2 max falls back to 1e-12, every point blue
+    assert (run.canvas_colors("J2_STRESS", sc, np.zeros((sc.n, 2))) == (0, 0, 255)).all()
+    # elastic point 0 (yield stress 0): no plastic scale, every point blue
+    el = scene.small_world()
+    assert el.sy[0] == 0.0
+    assert (run.canvas_colors("J2_PLASTICSTRAIN", el, np.zeros((el.n, 2))) == (0, 0, 255)).all()
+
+
+def test_render_draws_fixed_nodes_gray():
+    sc = scene.small_world()
+    assert (np.asarray(sc.fixed) != 0).any()
+    img = run.render(sc, sc.x, sc.vol0, size=200, fixed_nodes=True)
+    assert ((img == (128, 128, 128)).all(axis=2)).any()
+    plain = run.render(sc, sc.x, sc.vol0, size=200)
+    assert not ((plain == (128, 128, 128)).all(axis=2)).any()
