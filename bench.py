@@ -299,32 +299,37 @@ def config3_run(m, steps, warmup):
 
 def config5_run(reps=3):
     """Config 5: the differentiable run on the 4x4 polygon mesh (about 160k
-    points, per-point E log-uniform in [0.5, 2] GPa), 400 FP64 steps with
-    checkpoints every 20.  Times the forward alone and forward + adjoint
-    through the torch.autograd.Function with the effective-Poisson loss
-    (best of ``reps``, wall clock around each call, state on the GPU)."""
+    points, per-point E log-uniform in [0.5, 2] GPa), 400 steps with
+    checkpoints every 20, FP32 state as the config states (FP64 arithmetic
+    inside each step and in the adjoint) and, beside it, FP64 state.  Times
+    the forward alone and forward + adjoint through the
+    torch.autograd.Function with the effective-Poisson loss (best of
+    ``reps``, wall clock around each call, state on the GPU)."""
     import torch
+    import code_mpm_auxetics_amd as m
     from code_mpm_auxetics_amd import diff, scene
     sc = scene.config5_world()
     sets = diff.boundary_sets(sc)
-    with diff.DiffRun(sc, 400, checkpoint=20) as run:
-        run.forward(sc.E, sc.x)
-        t_f, t_fb = [], []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            u = run.forward(sc.E, sc.x)
-            t_f.append(time.perf_counter() - t0)
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            E = torch.tensor(sc.E, requires_grad=True)
-            x0 = torch.tensor(sc.x, requires_grad=True)
-            loss = diff.poisson_loss(x0, diff.rollout(run, E, x0), sets)
-            loss.backward()
-            t_fb.append(time.perf_counter() - t0)
-    f, fb = min(t_f), min(t_fb)
+    res = {}
+    for name, prec in (("fp32", m.FP32), ("fp64", m.FP64)):
+        with diff.DiffRun(sc, 400, checkpoint=20, precision=prec) as run:
+            run.forward(sc.E, sc.x)
+            t_f, t_fb = [], []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                u = run.forward(sc.E, sc.x)
+                t_f.append(time.perf_counter() - t0)
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                E = torch.tensor(sc.E, requires_grad=True)
+                x0 = torch.tensor(sc.x, requires_grad=True)
+                loss = diff.poisson_loss(x0, diff.rollout(run, E, x0), sets)
+                loss.backward()
+                t_fb.append(time.perf_counter() - t0)
+        res[name] = dict(f=min(t_f), fb=min(t_fb), loss=float(loss.detach()),
+                         nu=float(diff.effective_poisson(sc.x, u, sets)))
     # the same 400 steps on the product step (no checkpoints or node slots
     # kept), FP64 and FP32 state, L dt strain increment as in the run above
-    import code_mpm_auxetics_amd as m
     prod = {}
     for name, prec in (("fp64", m.FP64), ("fp32", m.FP32)):
         best = None
@@ -335,14 +340,19 @@ def config5_run(reps=3):
                 el = time.perf_counter() - t0
             best = el if best is None else min(best, el)
         prod[name] = best
+    r32, r64 = res["fp32"], res["fp64"]
     return {"workload": "config5_polygon_mesh_4x4_elastic_adjoint", "n_points": sc.n_deformable, "steps": 400,
+            "checkpoint": 20, "dtype": "f32", "state": "fp32 (u, v, F - I, sigma, alpha); fp64 step and adjoint arithmetic",
+            "forward_ms": round(r32["f"] * 1e3, 2), "forward_adjoint_ms": round(r32["fb"] * 1e3, 2),
+            "adjoint_over_forward": round(r32["fb"] / r32["f"], 3),
+            "forward_particle_steps_per_s": round(sc.n_deformable * 400 / r32["f"], 1),
+            "loss": r32["loss"], "nu_eff": r32["nu"],
+            "fp64_state": {"forward_ms": round(r64["f"] * 1e3, 2), "forward_adjoint_ms": round(r64["fb"] * 1e3, 2),
+                           "adjoint_over_forward": round(r64["fb"] / r64["f"], 3), "loss": r64["loss"]},
             "product_forward_fp64_ms": round(prod["fp64"] * 1e3, 2),
             "product_forward_fp32_ms": round(prod["fp32"] * 1e3, 2),
-            "forward_adjoint_over_product_fp64_forward": round(fb / prod["fp64"], 3),
-            "checkpoint": 20, "dtype": "f64", "forward_ms": round(f * 1e3, 2),
-            "forward_adjoint_ms": round(fb * 1e3, 2), "adjoint_over_forward": round(fb / f, 3),
-            "forward_particle_steps_per_s": round(sc.n_deformable * 400 / f, 1),
-            "loss": float(loss.detach()), "nu_eff": float(diff.effective_poisson(sc.x, u, sets))}
+            "forward_adjoint_over_product_fp32_forward": round(r32["fb"] / prod["fp32"], 3),
+            "forward_adjoint_over_product_fp64_forward": round(r64["fb"] / prod["fp64"], 3)}
 
 
 def counters_run(m, sc, warmup, steps):

@@ -1,4 +1,5 @@
-// mpm_diff.hip -- differentiable FP64 run of the plain step and its adjoint.
+// mpm_diff.hip -- differentiable run of the plain step and its adjoint, with
+// FP64 or FP32 point state (FP64 arithmetic in every step and the adjoint).
 //
 // The forward is the plain step of PhysicsEngine_runSimulation_Classic_-
 // SinglePass_MP.cpp:41-368 with the strain increment L dt, on a dense node
@@ -19,7 +20,10 @@
 // are not quantised against the loaded region's forces).  Each point also
 // carries its displacement u = x - x0 as state, so that displacements of
 // 1e-9 m on positions of 1e-2 m keep full FP64 precision; the run's output
-// is u.
+// is u.  FP32 state stores u, v, F - I, sigma and alpha in float (positions
+// x0 + u, x0 FP64 and fixed for the run) and rounds them once per step; the
+// adjoint replays the same rounded states and takes the rounding as the
+// identity.
 //
 // Adjoint of one step (state s = x, u, v, F, sigma, alpha; parameters E; N and
 // grad N evaluated at the step's starting positions; piecewise-constant
@@ -64,6 +68,12 @@ struct DArgs {
     double dt, damping;
     // point state (SoA, component-major: x[a * n + p])
     double *x, *u, *v, *F, *s, *al;   // al: hardening variable alpha (n)
+    // FP32 state (fp32 != 0): u, v, D = F - I, sigma, alpha in float, the
+    // positions x0 + u (x0 FP64, fixed for the run); the FP64 pointers above
+    // are then unused.  Arithmetic stays FP64.
+    int fp32;
+    const double *x0;
+    float *u32, *v32, *D32, *s32, *al32;
     const double *m, *V0, *E, *nu, *fext;
     const uint8_t *law;              // per point: LAW_* after the reference's dispatch
     const double *sy, *C0, *C1;      // yield stress, hardening constants
@@ -460,6 +470,33 @@ __device__ __forceinline__ long long win_node(const DGrid &g, const Win &w, int 
     return (w.lo[0] + jx) + (long long)g.nodes[0] * ((w.lo[1] + jy) + (long long)g.nodes[1] * (w.lo[2] + jz));
 }
 __device__ __forceinline__ unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
+
+// ----------------------------------------------------- point state access
+// FP64 state reads and writes the arrays as they are.  FP32 state widens
+// every read to FP64 (positions x0 + u, F = I + D) and rounds each write
+// once; the adjoint reads the same rounded states, so replays match the
+// forward bit for bit and the rounding is taken as the identity.
+__device__ __forceinline__ void ld_x(const DArgs &a, int p, double x[3])
+{
+    const int n = a.n;
+    if (a.fp32)
+        for (int q = 0; q < 3; ++q) x[q] = a.x0[q * n + p] + (double)a.u32[q * n + p];
+    else
+        for (int q = 0; q < 3; ++q) x[q] = a.x[q * n + p];
+}
+__device__ __forceinline__ double ld_v(const DArgs &a, int q, int p)
+{
+    return a.fp32 ? (double)a.v32[q * a.n + p] : a.v[q * a.n + p];
+}
+__device__ __forceinline__ double ld_F(const DArgs &a, int q, int p)
+{
+    return a.fp32 ? (double)a.D32[q * a.n + p] + ((q % 4 == 0) ? 1.0 : 0.0) : a.F[q * a.n + p];
+}
+__device__ __forceinline__ double ld_s(const DArgs &a, int q, int p)
+{
+    return a.fp32 ? (double)a.s32[q * a.n + p] : a.s[q * a.n + p];
+}
+__device__ __forceinline__ double ld_al(const DArgs &a, int p) { return a.fp32 ? (double)a.al32[p] : a.al[p]; }
 __device__ __forceinline__ long long fixp(double v, double sc) { return __double2ll_rn(v * sc); }
 
 // --------------------------------------------------------------- forward
@@ -497,7 +534,7 @@ __global__ __launch_bounds__(256) void kd_p2g(DArgs a)
     double x[3] = {0.0, 0.0, 0.0};
     int i0[3] = {0, 0, 0};
     if (valid) {
-        for (int q = 0; q < 3; ++q) x[q] = a.x[q * n + p];
+        ld_x(a, p, x);
         valid = cell_of(a.g, x, i0);
         if (!valid && BOUNDS) atomicOr(a.err, 1);
     }
@@ -508,10 +545,10 @@ __global__ __launch_bounds__(256) void kd_p2g(DArgs a)
         Bases B;
         bases(a.g, x, i0, B);
         double F[9], s[6], v[3], fe[3];
-        for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
-        for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
+        for (int q = 0; q < 9; ++q) F[q] = ld_F(a, q, p);
+        for (int q = 0; q < 6; ++q) s[q] = ld_s(a, q, p);
         for (int q = 0; q < 3; ++q) {
-            v[q] = a.v[q * n + p];
+            v[q] = ld_v(a, q, p);
             fe[q] = a.fext[q * n + p];
         }
         const double m = a.m[p], V = det3(F) * a.V0[p];
@@ -645,7 +682,8 @@ __global__ __launch_bounds__(256) void kd_g2p(DArgs a)
     if (p >= a.n) return;
     const int n = a.n;
     const long long nn = a.g.nn;
-    double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
+    double x[3];
+    ld_x(a, p, x);
     int i0[3];
     if (!cell_of(a.g, x, i0)) return;
     Bases B;
@@ -663,19 +701,25 @@ __global__ __launch_bounds__(256) void kd_g2p(DArgs a)
         }
     }
     double F[9], T[9], Fn[9], s[6], de[6], ds[6];
-    for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
+    for (int q = 0; q < 9; ++q) F[q] = ld_F(a, q, p);
     matmul(L, F, T);
     for (int q = 0; q < 9; ++q) {
         Fn[q] = F[q] + T[q] * a.dt;
-        a.F[q * n + p] = Fn[q];
+        if (a.fp32) a.D32[q * n + p] = (float)((double)a.D32[q * n + p] + T[q] * a.dt);
+        else a.F[q * n + p] = Fn[q];
     }
     for (int q = 0; q < 3; ++q) {
-        a.x[q * n + p] = x[q] + dx[q] * a.dt;
-        a.u[q * n + p] += dx[q] * a.dt;
-        a.v[q * n + p] += dv[q] * a.dt;
+        if (a.fp32) {
+            a.u32[q * n + p] = (float)((double)a.u32[q * n + p] + dx[q] * a.dt);
+            a.v32[q * n + p] = (float)((double)a.v32[q * n + p] + dv[q] * a.dt);
+        } else {
+            a.x[q * n + p] = x[q] + dx[q] * a.dt;
+            a.u[q * n + p] += dx[q] * a.dt;
+            a.v[q * n + p] += dv[q] * a.dt;
+        }
     }
     eps_of(L, a.dt, de);
-    for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
+    for (int q = 0; q < 6; ++q) s[q] = ld_s(a, q, p);
     // constitutive update (…_MP.cpp:322-353)
     const int law = a.law[p];
     if (law == LAW_ELASTIC) {
@@ -691,12 +735,18 @@ __global__ __launch_bounds__(256) void kd_g2p(DArgs a)
         ds[5] = P[6] - s[5];
     } else {
         RR r;
-        const double al = a.al[p];
+        const double al = ld_al(a, p);
         rr_eval(a.E[p], a.nu[p], a.sy[p], a.C0[p], a.C1[p], law == LAW_VONMISES, s, al, de, r);
         for (int q = 0; q < 6; ++q) ds[q] = r.ds[q];
-        if (law == LAW_VONMISES && r.plastic) a.al[p] = al + r.da;
+        if (law == LAW_VONMISES && r.plastic) {
+            if (a.fp32) a.al32[p] = (float)(al + r.da);
+            else a.al[p] = al + r.da;
+        }
     }
-    for (int q = 0; q < 6; ++q) a.s[q * n + p] = s[q] + ds[q];
+    for (int q = 0; q < 6; ++q) {
+        if (a.fp32) a.s32[q * n + p] = (float)(s[q] + ds[q]);
+        else a.s[q * n + p] = s[q] + ds[q];
+    }
 }
 
 // platen motion at the end of the step (…_MP.cpp:360-368)
@@ -708,6 +758,21 @@ __global__ __launch_bounds__(256) void kd_dc(DArgs a)
         if (a.has_tl) a.vdc[3 * j + q] = a.vtl[q];
         a.xdc[3 * j + q] += a.vdc[3 * j + q] * a.dt;
     }
+}
+
+// FP32 state at step 0: u = 0, v = v0, D = 0, sigma = 0, alpha = alpha0
+__global__ __launch_bounds__(256) void kd_init32(DArgs a, const double *v0, const double *al0)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.n) return;
+    const int n = a.n;
+    for (int q = 0; q < 3; ++q) {
+        a.u32[q * n + p] = 0.0f;
+        a.v32[q * n + p] = (float)v0[q * n + p];
+    }
+    for (int q = 0; q < 9; ++q) a.D32[q * n + p] = 0.0f;
+    for (int q = 0; q < 6; ++q) a.s32[q * n + p] = 0.0f;
+    a.al32[p] = (float)al0[p];
 }
 
 // --------------------------------------------------------------- adjoint
@@ -722,7 +787,8 @@ __global__ __launch_bounds__(256) void kd_a_g2p(DArgs a)
     const int n = a.n;
     const long long nn = a.g.nn;
     const double dt = a.dt;
-    double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
+    double x[3];
+    ld_x(a, p, x);
     int i0[3];
     if (!cell_of(a.g, x, i0)) return;
     Bases B;
@@ -736,7 +802,7 @@ __global__ __launch_bounds__(256) void kd_a_g2p(DArgs a)
             for (int c = 0; c < 3; ++c) L[3 * c + q] += a.nvt[q * nn + I] * G[c];
     }
     double F[9], Fb[9], sbv[6], xb[3], vb[3];
-    for (int q = 0; q < 9; ++q) { F[q] = a.F[q * n + p]; Fb[q] = a.Fb[q * n + p]; }
+    for (int q = 0; q < 9; ++q) { F[q] = ld_F(a, q, p); Fb[q] = a.Fb[q * n + p]; }
     for (int q = 0; q < 6; ++q) sbv[q] = a.sb[q * n + p];
     for (int q = 0; q < 3; ++q) { xb[q] = a.xb[q * n + p]; vb[q] = a.vb[q * n + p]; }
     // the constitutive update: E-bar, eps-bar, and sigma-bar, alpha-bar before
@@ -764,9 +830,9 @@ __global__ __launch_bounds__(256) void kd_a_g2p(DArgs a)
         }
     } else {
         double s[6], sbo[6], abo;
-        for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
+        for (int q = 0; q < 6; ++q) s[q] = ld_s(a, q, p);
         RR r;
-        rr_eval(E, nu, a.sy[p], a.C0[p], a.C1[p], law == LAW_VONMISES, s, a.al[p], de, r);
+        rr_eval(E, nu, a.sy[p], a.C0[p], a.C1[p], law == LAW_VONMISES, s, ld_al(a, p), de, r);
         rr_adjoint(E, nu, a.C0[p], law == LAW_VONMISES, de, r, sbv, a.alb[p], sbo, abo, eb, gE);
         for (int q = 0; q < 6; ++q) a.sb[q * n + p] = sbo[q];
         a.alb[p] = abo;
@@ -833,7 +899,7 @@ __global__ __launch_bounds__(256) void kd_a_scatter(DArgs a)
     double x[3] = {0.0, 0.0, 0.0};
     int i0[3] = {0, 0, 0};
     if (valid) {
-        for (int q = 0; q < 3; ++q) x[q] = a.x[q * n + p];
+        ld_x(a, p, x);
         valid = cell_of(a.g, x, i0);
     }
     const Win w = block_window(valid, i0, sbox);
@@ -967,16 +1033,17 @@ __global__ __launch_bounds__(256) void kd_a_p2g(DArgs a)
     if (p >= a.n) return;
     const int n = a.n;
     const long long nn = a.g.nn;
-    double x[3] = {a.x[p], a.x[n + p], a.x[2 * n + p]};
+    double x[3];
+    ld_x(a, p, x);
     int i0[3];
     if (!cell_of(a.g, x, i0)) return;
     Bases B;
     bases(a.g, x, i0, B);
     double F[9], s[6], v[3], fe[3];
-    for (int q = 0; q < 9; ++q) F[q] = a.F[q * n + p];
-    for (int q = 0; q < 6; ++q) s[q] = a.s[q * n + p];
+    for (int q = 0; q < 9; ++q) F[q] = ld_F(a, q, p);
+    for (int q = 0; q < 6; ++q) s[q] = ld_s(a, q, p);
     for (int q = 0; q < 3; ++q) {
-        v[q] = a.v[q * n + p];
+        v[q] = ld_v(a, q, p);
         fe[q] = a.fext[q * n + p];
     }
     const double m = a.m[p], det = det3(F), V = det * a.V0[p];
@@ -1038,7 +1105,11 @@ struct mpm_diff {
     std::vector<double> tl_t, tl_v, times;   // times[k]: start time of step k
     std::vector<int> def_ids, dc_ids;
     // device; the state arrays x, u, v, F, s, xdc, vdc are slices of st
+    // (FP32 state: u32, v32, D32, s32, al32, xdc, vdc; x0d holds x0)
+    bool fp32 = false;
     double *st = nullptr, *x = nullptr, *u = nullptr, *v = nullptr, *F = nullptr, *s = nullptr, *al = nullptr;
+    float *u32 = nullptr, *v32 = nullptr, *D32 = nullptr, *s32 = nullptr, *al32 = nullptr;
+    double *x0d = nullptr;
     double *m = nullptr, *V0 = nullptr, *E = nullptr, *nu = nullptr, *fext = nullptr;
     uint8_t *law = nullptr;
     double *sy = nullptr, *C0 = nullptr, *C1 = nullptr, *al0 = nullptr, *alb = nullptr;
@@ -1063,8 +1134,9 @@ struct mpm_diff {
     double *seg = nullptr;     // one segment: ckpt x state
     bool have_forward = false;
     // doubles of one saved state: x, u, v (3 n each), F (9 n), sigma (6 n),
-    // alpha (n), platen x and v (3 n_dc each)
-    size_t state() const { return 25 * (size_t)n + 6 * (size_t)n_dc; }
+    // alpha (n), platen x and v (3 n_dc each); FP32 state: 22 floats per
+    // point (u, v, D, sigma, alpha) = 11 doubles, platen x and v in FP64
+    size_t state() const { return (fp32 ? 11 : 25) * (size_t)n + 6 * (size_t)n_dc; }
 };
 
 namespace {
@@ -1098,6 +1170,9 @@ DArgs args_of(mpm_diff *d, int step, int slot)
     a.damping = d->damping;
     a.fext = d->fext;
     a.x = d->x; a.u = d->u; a.v = d->v; a.F = d->F; a.s = d->s; a.al = d->al;
+    a.fp32 = d->fp32 ? 1 : 0;
+    a.x0 = d->x0d;
+    a.u32 = d->u32; a.v32 = d->v32; a.D32 = d->D32; a.s32 = d->s32; a.al32 = d->al32;
     a.m = d->m; a.V0 = d->V0; a.E = d->E; a.nu = d->nu;
     a.law = d->law; a.sy = d->sy; a.C0 = d->C0; a.C1 = d->C1;
     a.fixed = d->fixed;
@@ -1179,7 +1254,7 @@ uint64_t morton3(uint32_t x, uint32_t y, uint32_t z)
 
 void free_all(mpm_diff *d)
 {
-    void *ptrs[] = {d->st, d->m, d->V0, d->E, d->nu, d->fext, d->fixed, d->x0dc, d->v0dc, d->v0, d->F0, d->marks, d->nb, d->nsc, d->acc, d->aacc, d->nodebuf, d->items,
+    void *ptrs[] = {d->st, d->x0d, d->m, d->V0, d->E, d->nu, d->fext, d->fixed, d->x0dc, d->v0dc, d->v0, d->F0, d->marks, d->nb, d->nsc, d->acc, d->aacc, d->nodebuf, d->items,
                     d->nmb, d->nPb, d->nfb, d->errd, d->xb, d->vb, d->Fb, d->sb, d->Eb,
                     d->scratch, d->ck, d->seg, d->law, d->sy, d->C0, d->C1, d->al0, d->alb};
     for (void *p : ptrs)
@@ -1200,8 +1275,10 @@ int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_step
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MPM_E_NO_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev || hipSetDevice(cfg->device) != hipSuccess) return MPM_E_ARG;
+    if (cfg->precision != MPM_FP64 && cfg->precision != MPM_FP32) return MPM_E_ARG;
     mpm_diff *d = new mpm_diff();
     d->device = cfg->device;
+    d->fp32 = cfg->precision == MPM_FP32;
     auto fail = [&](int code) {
         free_all(d);
         delete d;
@@ -1296,15 +1373,25 @@ int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_step
         (rc = dalloc_(d, &d->Eb, n)) || (rc = dalloc_(d, &d->scratch, 12 * n)) || (rc = dalloc_(d, &d->law, n)) ||
         (rc = dalloc_(d, &d->sy, n)) || (rc = dalloc_(d, &d->C0, n)) || (rc = dalloc_(d, &d->C1, n)) ||
         (rc = dalloc_(d, &d->al0, n)) || (rc = dalloc_(d, &d->alb, n)) ||
-        (rc = dalloc_(d, &d->ck, (size_t)d->n_ckpt * state)) || (rc = dalloc_(d, &d->seg, (size_t)d->ckpt * state)))
+        (rc = dalloc_(d, &d->ck, (size_t)d->n_ckpt * state)) || (rc = dalloc_(d, &d->seg, (size_t)d->ckpt * state)) ||
+        (d->fp32 && (rc = dalloc_(d, &d->x0d, 3 * n))))
         return fail(rc);
-    d->x = d->st;
-    d->u = d->x + 3 * n;
-    d->v = d->u + 3 * n;
-    d->F = d->v + 3 * n;
-    d->s = d->F + 9 * n;
-    d->al = d->s + 6 * n;
-    d->xdc = d->al + n;
+    if (d->fp32) {
+        d->u32 = reinterpret_cast<float *>(d->st);
+        d->v32 = d->u32 + 3 * n;
+        d->D32 = d->v32 + 3 * n;
+        d->s32 = d->D32 + 9 * n;
+        d->al32 = d->s32 + 6 * n;
+        d->xdc = d->st + 11 * n;
+    } else {
+        d->x = d->st;
+        d->u = d->x + 3 * n;
+        d->v = d->u + 3 * n;
+        d->F = d->v + 3 * n;
+        d->s = d->F + 9 * n;
+        d->al = d->s + 6 * n;
+        d->xdc = d->al + n;
+    }
     d->vdc = d->xdc + 3 * nd;
     // static point data, component-major (blocking copies from pageable memory)
     std::vector<double> tmp(9 * n);
@@ -1364,12 +1451,19 @@ int mpm_diff_forward(mpm_diff *d, const double *E, const double *x0, double *u_f
         for (int a = 0; a < 3; ++a) hx[a * n + k] = x0[3 * d->def_ids[k] + a];
     }
     DCHK(hipMemcpy(d->E, hE.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    DCHK(hipMemcpy(d->x, hx.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-    DCHK(hipMemsetAsync(d->u, 0, sizeof(double) * 3 * n, d->stream));
-    DCHK(hipMemcpyAsync(d->v, d->v0, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, d->stream));
-    DCHK(hipMemcpyAsync(d->F, d->F0, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, d->stream));
-    DCHK(hipMemsetAsync(d->s, 0, sizeof(double) * 6 * n, d->stream));
-    DCHK(hipMemcpyAsync(d->al, d->al0, sizeof(double) * n, hipMemcpyDeviceToDevice, d->stream));
+    if (d->fp32) {
+        // u = 0, v = v0, D = 0, sigma = 0, alpha = alpha0, rounded once
+        DCHK(hipMemcpy(d->x0d, hx.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        kd_init32<<<gsz(n), 256, 0, d->stream>>>(args_of(d, 0, 0), d->v0, d->al0);
+        DCHK(hipGetLastError());
+    } else {
+        DCHK(hipMemcpy(d->x, hx.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        DCHK(hipMemsetAsync(d->u, 0, sizeof(double) * 3 * n, d->stream));
+        DCHK(hipMemcpyAsync(d->v, d->v0, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, d->stream));
+        DCHK(hipMemcpyAsync(d->F, d->F0, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, d->stream));
+        DCHK(hipMemsetAsync(d->s, 0, sizeof(double) * 6 * n, d->stream));
+        DCHK(hipMemcpyAsync(d->al, d->al0, sizeof(double) * n, hipMemcpyDeviceToDevice, d->stream));
+    }
     if (nd) {
         DCHK(hipMemcpyAsync(d->xdc, d->x0dc, sizeof(double) * 3 * nd, hipMemcpyDeviceToDevice, d->stream));
         DCHK(hipMemcpyAsync(d->vdc, d->v0dc, sizeof(double) * 3 * nd, hipMemcpyDeviceToDevice, d->stream));
@@ -1383,13 +1477,17 @@ int mpm_diff_forward(mpm_diff *d, const double *E, const double *x0, double *u_f
     int32_t err = 0;
     std::vector<double> pdc(3 * nd), p0(3 * nd);
     DCHK(hipMemcpyAsync(&err, d->errd, 4, hipMemcpyDeviceToHost, d->stream));
-    DCHK(hipMemcpyAsync(hx.data(), d->u, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, d->stream));
+    std::vector<float> hu32(d->fp32 ? 3 * n : 0);
+    if (d->fp32) DCHK(hipMemcpyAsync(hu32.data(), d->u32, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, d->stream));
+    else DCHK(hipMemcpyAsync(hx.data(), d->u, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, d->stream));
     if (nd) {
         DCHK(hipMemcpyAsync(pdc.data(), d->xdc, sizeof(double) * 3 * nd, hipMemcpyDeviceToHost, d->stream));
         DCHK(hipMemcpyAsync(p0.data(), d->x0dc, sizeof(double) * 3 * nd, hipMemcpyDeviceToHost, d->stream));
     }
     DCHK(hipStreamSynchronize(d->stream));
     if (err) return dfail(d, "a material point left the grid box", MPM_E_OUT_OF_GRID);
+    if (d->fp32)
+        for (size_t k = 0; k < 3 * n; ++k) hx[k] = (double)hu32[k];
     for (size_t k = 0; k < n; ++k)
         for (int a = 0; a < 3; ++a) u_final[3 * d->def_ids[k] + a] = hx[a * n + k];
     for (size_t j = 0; j < nd; ++j)   // platen displacements (no parameters)

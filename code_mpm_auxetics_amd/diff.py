@@ -42,10 +42,15 @@ class DiffRun:
     """One differentiable run of ``n_steps`` steps of ``scene.dt_max`` on one
     GPU.  ``forward(E, x0)`` returns the final displacements (n, 3);
     ``backward(g)`` returns (dL/dE, dL/dx0) for dL/du_final = g of the last
-    forward."""
+    forward.  ``precision``: FP64 state, or FP32 state (u, v, F - I, sigma,
+    alpha stored in float, positions x0 + u, every step's arithmetic and the
+    adjoint in FP64, the rounding of the stored state taken as the identity)."""
 
-    def __init__(self, scene, n_steps, checkpoint=20, device=0):
+    def __init__(self, scene, n_steps, checkpoint=20, device=0, precision=_abi.MPM_FP64):
+        if precision not in (_abi.MPM_FP64, _abi.MPM_FP32):
+            raise ValueError("precision must be FP64 or FP32")
         self.scene = scene
+        self.precision = int(precision)
         self.n_steps = int(n_steps)
         self.checkpoint = int(checkpoint)
         cfg = MpmConfig()
@@ -57,7 +62,7 @@ class DiffRun:
         cfg.time_end = scene.time_end
         cfg.damping = scene.damping
         cfg.mass_min = scene.mass_min
-        cfg.precision = _abi.MPM_FP64
+        cfg.precision = self.precision
         cfg.strain_form = 1
         cfg.neohookean = int(getattr(scene, "neohookean", 0))
         cfg.device = int(device)

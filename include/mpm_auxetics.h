@@ -337,7 +337,7 @@ int mpm_constitutive(int32_t law, const mpm_cr_batch *b, int32_t device);
 int mpm_constitutive_noalloc(int32_t law, const mpm_cr_batch *b, int32_t device);
 
 /* ---- differentiable run (no reference counterpart: the reference has no
- *      gradients).  n_steps plain steps of cfg->dt_max in FP64
+ *      gradients).  n_steps plain steps of cfg->dt_max, FP64 arithmetic
  *      (…_MP.cpp:106-353, strain increment L dt, damping and TimeLine
  *      platen as the reference; elastic, NeoHookean when cfg->neohookean,
  *      perfectly plastic and J2-hardening points, each point's yield
@@ -346,7 +346,10 @@ int mpm_constitutive_noalloc(int32_t law, const mpm_cr_batch *b, int32_t device)
  *      adjoint of every phase, replayed from checkpoints every `checkpoint`
  *      steps.  Parameters are the per-point Young's modulus and the initial
  *      point positions; the output is the final displacement u = x - x0,
- *      carried as its own state so that it keeps full FP64 precision.  Host
+ *      carried as its own state so that it keeps full FP64 precision.
+ *      cfg->precision: MPM_FP64 state, or MPM_FP32 state (u, v, F - I,
+ *      sigma, alpha in float, rounded once per step, positions x0 + u;
+ *      the adjoint takes the rounding as the identity).  Host
  *      arrays are in the original point order of `pts` (platen points move
  *      as in the reference and get zero gradients). ------------------------ */
 typedef struct mpm_diff mpm_diff;
@@ -355,7 +358,8 @@ typedef struct mpm_diff mpm_diff;
  * (0 if null), initial v, fext, type, dc flags, platen positions; node
  * fixity, damping, TimeLine and the NeoHookean dispatch from cfg.  Types map
  * as the reference's dispatch does (…_MP.cpp:332-339).  MPM_E_ARG for an
- * unknown type, or for plastic points without sy (or C0, C1 for hardening). */
+ * unknown type, for plastic points without sy (or C0, C1 for hardening), or
+ * for a precision other than MPM_FP64 and MPM_FP32. */
 int mpm_diff_create(const mpm_config *cfg, const mpm_points *pts, int32_t n_steps,
                     int32_t checkpoint, mpm_diff **out);
 
