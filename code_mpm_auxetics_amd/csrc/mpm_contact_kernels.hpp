@@ -21,7 +21,7 @@ namespace mpm {
 
 constexpr uint32_t CF_ACTIVE = 1, CF_KACTIVE = 2, CF_CPOS = 4, CF_CNEG = 8;
 constexpr uint32_t CF_GATE0 = 16;   // CF_GATE0 << f: field f's node mass passes the m_min gate
-constexpr int kCB = 6;   // bounds: m, m/h, |v|, |force|, 1 (kernel), spare
+constexpr int kCB = 6;   // bounds: m, m/h, m|v|, |force|, 1 (kernel), spare
 
 template <typename real>
 struct CArgs {
@@ -418,7 +418,7 @@ __device__ __forceinline__ void c_point_bounds(const CArgs<real> &a, int j, doub
     for (int q = 0; q < 6; ++q) smax = fmax(smax, fabs(s[q]));
     b[0] = fabs(m);
     b[1] = fabs(m) * ihm;
-    b[2] = fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2])));
+    b[2] = fabs(m) * fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2])));
     b[3] = fabs(vol) * 3.0 * smax * ihm + fmax(fabs(fe[0]), fmax(fabs(fe[1]), fabs(fe[2])));
 }
 template <typename real>
@@ -753,8 +753,9 @@ __global__ void __launch_bounds__(256) k_c_publish(CArgs<real> a)
 // (…_MP_Contact.cpp:288-385).  MASS=true: mass and grad m; the field of each
 // point corner (sign of grad k_I . grad k_p) is stored as a bit mask per
 // point, and surface points flag the nodes whose nearest point is a surface
-// point (CF_CPOS / CF_CNEG, the same sign).  MASS=false: momentum (/ field
-// mass, sqrt(3) gate) and force, fields from the mask.  Window slot s of a
+// point (CF_CPOS / CF_CNEG, the same sign).  MASS=false: momentum N m v
+// (divided by the field mass, sqrt(3) gate, in the grid update) and force,
+// fields from the mask.  Window slot s of a
 // node: MASS: field * 4 + {m, mg.xyz}; else field * 6 + {mv.xyz, f.xyz}.
 template <bool MASS>
 __device__ __forceinline__ long long c_p2g_off(int s)
@@ -860,7 +861,7 @@ __device__ __forceinline__ void c_p2g_in_dc(const CArgs<real> &a, int j, CP2gIn 
 template <typename real, bool MASS, bool SMALL>
 __device__ __forceinline__ void c_p2g_point(const CArgs<real> &a, const CWin &w, unsigned long long *lds,
                                             const double *s_gk, uint32_t *s_ns, int j, const CP2gIn &in,
-                                            double sc0, double sc1, double um)
+                                            double sc0, double sc1)
 {
     const GridParams &g = a.g;
     long long idx[8];
@@ -907,14 +908,11 @@ __device__ __forceinline__ void c_p2g_point(const CArgs<real> &a, const CWin &w,
             cnt = ns ? 4 : 1;
         } else {
             base = 6 * f;
-            const double mI = (double)a.acc[off_m(f) * a.nn + I] * um;
-            const bool gate = c_lenm3(mI) > g.m_min;
-            if (kCFast<real>) {
-                const double rmI = gate ? 1.0 / mI : 0.0;
-                for (int q = 0; q < 3; ++q) iv[q] = gate ? c_toll<SMALL>((N * (in.m * in.v[q]) * rmI) * sc0) : 0ll;
-            } else {
-                for (int q = 0; q < 3; ++q) iv[q] = gate ? c_toll<SMALL>((N * (in.m * in.v[q]) / mI) * sc0) : 0ll;
-            }
+            // momentum N m v; the grid update divides the sum by the field's
+            // node mass where it passes the gate, which equals the
+            // reference's per-contribution N m v / m_I (…_MP_Contact.cpp:
+            // 355-360) summed over the node's points
+            for (int q = 0; q < 3; ++q) iv[q] = c_toll<SMALL>((N * (in.m * in.v[q])) * sc0);
             const double *sv = in.s;
             const double fr[3] = {-in.vol * (G[0] * sv[0] + G[1] * sv[3] + G[2] * sv[5]) + N * in.fe[0],
                                   -in.vol * (G[1] * sv[1] + G[0] * sv[3] + G[2] * sv[4]) + N * in.fe[1],
@@ -954,7 +952,6 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     }
     __syncthreads();
     const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
-    const double um = c_unscale(a, 0);
     const int end = r.j0 + r.n;
     if ((int)blockIdx.x < a.n_bb) {
         // deformable points of one brick, the next point's inputs in flight
@@ -966,7 +963,7 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
             if (j + 256 < end) c_p2g_raw<real, MASS>(a, j + 256, nxt);
             CP2gIn in;
             c_p2g_in_raw<real, MASS>(a, cur, in);
-            c_p2g_point<real, MASS, SMALL>(a, w, lds, s_gk, s_ns, j, in, sc0, sc1, um);
+            c_p2g_point<real, MASS, SMALL>(a, w, lds, s_gk, s_ns, j, in, sc0, sc1);
             cur = nxt;
         }
     } else {
@@ -974,7 +971,7 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
         for (int j = r.j0 + (int)threadIdx.x; j < end; j += 256) {
             CP2gIn in;
             c_p2g_in_dc<real, MASS>(a, j, in);
-            c_p2g_point<real, MASS, SMALL>(a, w, lds, s_gk, s_ns, j, in, sc0, sc1, um);
+            c_p2g_point<real, MASS, SMALL>(a, w, lds, s_gk, s_ns, j, in, sc0, sc1);
         }
     }
     __syncthreads();
@@ -1022,8 +1019,12 @@ __global__ void __launch_bounds__(256) k_c_grid(CArgs<real> a, int step)
         double mm[2], v[2][3], f[2][3], mg[2][3];
         for (int fl = 0; fl < 2; ++fl) {
             mm[fl] = (double)a.acc[off_m(fl) * a.nn + I] * um;
+            // field velocity: momentum over mass where the mass passes the gate
+            const bool gate = c_lenm3(mm[fl]) > g.m_min;
+            const double rm = (kCFast<real> && gate) ? 1.0 / mm[fl] : 0.0;
             for (int q = 0; q < 3; ++q) {
-                v[fl][q] = (double)a.acc[off_mv(fl, q) * a.nn + I] * umv;
+                const double pq = (double)a.acc[off_mv(fl, q) * a.nn + I] * umv;
+                v[fl][q] = !gate ? 0.0 : kCFast<real> ? pq * rm : pq / mm[fl];
                 f[fl][q] = (double)a.acc[off_f(fl, q) * a.nn + I] * uf;
                 mg[fl][q] = (double)a.acc[off_mg(fl, q) * a.nn + I] * umg;
             }
