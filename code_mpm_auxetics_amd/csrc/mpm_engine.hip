@@ -555,11 +555,13 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.n_occ = h->n_occ_d;
     a.slot_order = h->slot_order;
     a.slot_mat = h->slot_mat;
-    a.occ_cap = h->n_occ;    // brick blocks first, platen blocks after them
+    // platen blocks first (a multiple of the XCD count), then the brick blocks
     const bool persist = g2p && h->cfg.persistent && h->persist_blocks > 0;
+    const int xcd = std::max(h->n_xcd, 1);
+    a.n_pblk = g2p ? (int)(cdiv(cdiv(h->n_dc, kPBlock), xcd) * xcd) : 0;
     a.n_xcd = h->n_xcd;
     a.queue = h->queue;
-    if (persist) a.occ_cap = h->persist_blocks;
+    const int brick_blocks = persist ? h->persist_blocks : h->n_occ;
     a.mats = h->mats;
     a.steps = h->steps_d;
     a.flags = h->flags_d;
@@ -570,8 +572,8 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.fy_slot = (g2p && h->n_fy > 0) ? h->fy_slot : nullptr;
     a.fy_val = h->fy_val ? h->fy_val + (size_t)step * h->n_fy : nullptr;
     const size_t lds = (size_t)kQ * h->g.arenaN * sizeof(long long) + (g2p ? (size_t)6 * h->g.arenaN * h->rsize : 0);
-    unsigned grid = (unsigned)a.occ_cap;
-    if (g2p) grid += cdiv(h->n_dc, kPBlock);
+    if (brick_blocks == 0 && h->n_dc == 0) return MPM_OK;
+    const unsigned grid = (unsigned)(a.n_pblk + brick_blocks);
     if (grid == 0) return MPM_OK;
     if (persist)
         k_particle<real, true, true><<<grid, kPBlock, lds, h->stream>>>(a, step);

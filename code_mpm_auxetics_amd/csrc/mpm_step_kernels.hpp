@@ -48,6 +48,30 @@ namespace mpm {
 __device__ __forceinline__ bool radial_return_fp32(double nu, double C, double ie, double ig, double radius,
                                                    float s[6], const float de[6], double dep[6])
 {
+    // FP32 elastic trial first.  Where its J2 lies inside the yield surface by
+    // more than the FP32 evaluation error (bounded by 2^-17 of the squared
+    // trial norm: the component differences of J2 carry absolute errors of a
+    // few ulp of the largest component), the FP64 test below would find the
+    // point elastic too; the FP32 trial is then the update.  Otherwise the
+    // trial, the yield test and the return map run in FP64.
+    {
+        const float Cf = (float)C, nf = (float)nu, a0 = 1.0f - nf, a1 = 0.5f - nf;
+        float t[6];
+        t[0] = s[0] + Cf * (a0 * de[0] + nf * de[1] + nf * de[2]);
+        t[1] = s[1] + Cf * (a0 * de[1] + nf * de[0] + nf * de[2]);
+        t[2] = s[2] + Cf * (a0 * de[2] + nf * de[0] + nf * de[1]);
+        t[3] = s[3] + Cf * (a1 * de[3]);
+        t[4] = s[4] + Cf * (a1 * de[4]);
+        t[5] = s[5] + Cf * (a1 * de[5]);
+        const float d01 = t[0] - t[1], d12 = t[1] - t[2], d20 = t[2] - t[0];
+        const float j2 = (1.0f / 6.0f) * (d01 * d01 + d12 * d12 + d20 * d20) + t[3] * t[3] + t[4] * t[4] + t[5] * t[5];
+        const float nrm = t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + t[3] * t[3] + t[4] * t[4] + t[5] * t[5];
+        if (j2 + 0x1p-17f * nrm <= (float)(radius * radius) * (1.0f - 0x1p-20f)) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s[i] = t[i];
+            return false;
+        }
+    }
     double st[6];
     st[0] = (double)s[0] + C * ((1.0 - nu) * de[0] + nu * de[1] + nu * de[2]);
     st[1] = (double)s[1] + C * ((1.0 - nu) * de[1] + nu * de[0] + nu * de[2]);
@@ -289,6 +313,83 @@ __device__ __forceinline__ double wave_sum_dpp(double v)
     return (lane_d(v, 15) + lane_d(v, 31)) + (lane_d(v, 47) + lane_d(v, 63));
 }
 
+// One point's state as the step reads it: position (FP64 absolute or FP32
+// Q6.26 fixed point), velocity, F (F - I in FP32 state), stress, mass,
+// initial volume and, when the caller asks for it, alpha.
+template <typename real>
+struct PointState {
+    real x[3];
+    int32_t xq[3];
+    real v[3], F[9], s[6], m, V0, al;
+};
+template <typename real>
+__device__ __forceinline__ PointState<real> load_point(const PArrays<real> &P, uint32_t i, bool with_alpha)
+{
+    PointState<real> t;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if constexpr (sizeof(real) == 4) {
+            t.xq[q] = elt(P.xq[q], i);
+            t.x[q] = real(0);
+        } else {
+            t.x[q] = elt(P.x[q], i);
+            t.xq[q] = 0;
+        }
+        t.v[q] = elt(P.v[q], i);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) t.F[q] = elt(P.F[q], i);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t.s[q] = elt(P.s[q], i);
+    t.m = elt(P.mass, i);
+    t.V0 = elt(P.vol0, i);
+    t.al = with_alpha ? elt(P.alpha, i) : real(0);
+    return t;
+}
+
+// Node velocity, damped force and f / m from a node's P2G sums {m, p.xyz,
+// f.xyz} (…_MP.cpp:209-222), shared by the step's grid update and the grid
+// readback.  FP32 state: one hardware reciprocal of the mass (1 ulp) and a
+// reciprocal square root instead of IEEE divisions.  A zero mass gives
+// non-finite f / m, which the callers gate by the mass.
+template <typename real>
+__device__ __forceinline__ void node_damp(const real sum[kQ], real m_min, real damp, real vn[3], real fn[3],
+                                          real ac[3])
+{
+    const real mn = sum[0];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) fn[q] = sum[4 + q];
+    if constexpr (sizeof(real) == 4) {
+        const float imn = __builtin_amdgcn_rcpf(mn);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) vn[q] = (mn > m_min) ? sum[1 + q] * imn : 0.0f;
+        const float vv = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+        if (vv > 1.0e-18f) {   // |v| > 1e-9
+            const float lf = __builtin_amdgcn_sqrtf(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+            const float sc = damp * lf * __builtin_amdgcn_rsqf(vv);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) fn[q] -= sc * vn[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ac[q] = fn[q] * imn;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) vn[q] = (mn > m_min) ? sum[1 + q] / mn : real(0);
+        const real vv = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+        if (sqrt(vv) > real(1.0e-9)) {
+            const real lf = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+            const real inv = real(1) / sqrt(vv);
+            const real sc = damp * lf;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) fn[q] -= sc * (vn[q] * inv);
+        }
+        // f / m after damping: the velocity update and the stored
+        // acceleration share it; a force zeroed later gives 0 / m = 0
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ac[q] = fn[q] / mn;
+    }
+}
+
 constexpr int kUnsetExp = -100000;
 constexpr int kMinExp = -90;   // contributions below 2^-90 of a unit are dropped
 
@@ -310,7 +411,7 @@ struct KpArgs {
     const uint32_t *slot_order;       // brick block b works on slot slot_order[b]
     const uint8_t *slot_mat;          // per slot: the material of all its points, or 0xFF if mixed
     const int32_t *n_occ;
-    int32_t occ_cap;
+    int32_t n_pblk;             // platen blocks at the front of the grid
     const Material *mats;
     const StepRec *steps;
     int32_t *flags;
@@ -486,6 +587,15 @@ constexpr int kPBlock = MPM_KP_BLOCK;
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
 
+// Nearest integer (ties up: floor(x + 1/2)) in one instruction, for the FP32
+// fixed-point terms (|x| < 2^31).
+__device__ __forceinline__ int round_i32(float x)
+{
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // One brick per workgroup.  Dynamic LDS: long long arena[7 * arenaN].
 template <typename real, bool G2P, bool PERSIST = false>
 __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : MPM_KP_WAVES64) k_particle(KpArgs<real> a, int step)
@@ -517,28 +627,84 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     const double *const dc_v_in = a.dc_v[par];
     double *const dc_v_out = a.dc_v[par ^ 1];
 
-    // ---- platen (displacement-controlled) blocks: phase 8 + override marks
-    if ((int)blockIdx.x >= a.occ_cap) {
+    // ---- platen (displacement-controlled) blocks: phase 8 + override marks.
+    // Platen blocks come first in the grid (n_pblk of them, a multiple of the
+    // XCD count); brick block bb = blockIdx - n_pblk keeps its XCD.
+    const int bb = (int)blockIdx.x - a.n_pblk;
+    if (bb < 0) {
         if (!G2P) return;
-        const int j = ((int)blockIdx.x - a.occ_cap) * kPBlock + tid;
-        if (j >= a.n_dc) return;
-        double *x = &a.dc_x[3 * j], v[3];
-        // …_MP.cpp:360-368: velocity from the TimeLine at the step's start time
-        for (int q = 0; q < 3; ++q) {
-            v[q] = st.has_tl ? st.vtl[q] : dc_v_in[3 * j + q];
-            dc_v_out[3 * j + q] = v[q];
-            x[q] += v[q] * dtd;
+        const int j = (int)blockIdx.x * kPBlock + tid;
+        const bool on = j < a.n_dc;
+        int i0[3] = {0, 0, 0};
+        unsigned long long tag = 0;
+        if (on) {
+            double *x = &a.dc_x[3 * j], v[3];
+            // …_MP.cpp:360-368: velocity from the TimeLine at the step's start time
+            for (int q = 0; q < 3; ++q) {
+                v[q] = st.has_tl ? st.vtl[q] : dc_v_in[3 * j + q];
+                dc_v_out[3 * j + q] = v[q];
+                x[q] += v[q] * dtd;
+            }
+            // override marks for step n+1 (…_MP.cpp:245-265): the highest
+            // platen index touching a node wins, as in the reference's
+            // sequential loop.  The tag carries epoch (n+1)+1 so that 0 means
+            // "no mark".
+            for (int q = 0; q < 3; ++q) i0[q] = (int)(x[q] / g.h[q]);
+            tag = ((unsigned long long)(st.epoch + 2u) << 32) | (unsigned)j;
         }
-        // override marks for step n+1 (…_MP.cpp:245-265): the highest platen
-        // index touching a node wins, as in the reference's sequential loop.
-        // The tag carries epoch (n+1)+1 so that 0 means "no mark".
-        int i0[3];
-        for (int q = 0; q < 3; ++q) i0[q] = (int)(x[q] / g.h[q]);
-        const unsigned long long tag = ((unsigned long long)(st.epoch + 2u) << 32) | (unsigned)j;
-        for (int k = 0; k < 8; ++k) {
-            const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
-            if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= g.nodes[0] || n1 >= g.nodes[1] || n2 >= g.nodes[2]) continue;
-            atomicMax(&marks_out[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
+        // Many platen points share a node, and same-address global atomics
+        // serialize at the L2.  The block first takes the maximum per node in
+        // LDS over the bounding box of its points' base nodes (platen points
+        // are stored in a spatially compact order), then issues one global
+        // atomicMax per marked node.  A box larger than the LDS arena falls
+        // back to one global atomic per point and corner.  Either way each
+        // node ends with the maximum tag.
+        __shared__ int s_pbox[kPBlock / 64][6];
+        int lo[3], hi[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int c = min(max(i0[q], -1), g.nodes[q]);
+            lo[q] = wave_imin(on ? c : (1 << 20));
+            hi[q] = wave_imax(on ? c + 1 : -(1 << 20));
+        }
+        if (lane == 0)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) { s_pbox[wv][2 * q] = lo[q]; s_pbox[wv][2 * q + 1] = hi[q]; }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            for (int w = 0; w < kPBlock / 64; ++w) {
+                lo[q] = min(lo[q], s_pbox[w][2 * q]);
+                hi[q] = max(hi[q], s_pbox[w][2 * q + 1]);
+            }
+        }
+        if (hi[0] < lo[0]) return;   // no point in this block (uniform)
+        const int e0 = hi[0] - lo[0] + 1, e1 = hi[1] - lo[1] + 1, e2 = hi[2] - lo[2] + 1;
+        const long long nbox = (long long)e0 * e1 * e2;
+        unsigned long long *s_mk = reinterpret_cast<unsigned long long *>(s_arena);
+        const bool in_lds = nbox <= (long long)kQ * g.arenaN;   // uniform
+        if (in_lds) {
+            for (int r = tid; r < (int)nbox; r += kPBlock) s_mk[r] = 0ull;
+            __syncthreads();
+        }
+        if (on)
+            for (int k = 0; k < 8; ++k) {
+                const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
+                if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= g.nodes[0] || n1 >= g.nodes[1] || n2 >= g.nodes[2]) continue;
+                if (in_lds)
+                    atomicMax(&s_mk[(n0 - lo[0]) + e0 * ((n1 - lo[1]) + e1 * (n2 - lo[2]))], tag);
+                else
+                    atomicMax(&marks_out[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
+            }
+        if (in_lds) {
+            __syncthreads();
+            for (int r = tid; r < (int)nbox; r += kPBlock) {
+                const unsigned long long t = s_mk[r];
+                if (!t) continue;
+                const int n2 = lo[2] + r / (e0 * e1), rem = r % (e0 * e1);
+                const int n1 = lo[1] + rem / e0, n0 = lo[0] + rem % e0;
+                atomicMax(&marks_out[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], t);
+            }
         }
         return;
     }
@@ -555,10 +721,10 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     for (int it = 0;; ++it) {
         int item;
         if constexpr (!PERSIST) {
-            if ((int)blockIdx.x >= *a.n_occ) return;
-            item = (int)blockIdx.x;
+            if (bb >= *a.n_occ) return;
+            item = bb;
         } else {
-            const int X = a.n_xcd, x = (int)blockIdx.x % X;
+            const int X = a.n_xcd, x = bb % X;
             if (tid == 0) s_item = (int)atomicAdd(&a.queue[(size_t)step * kQueueStride + x], 1u);
             __syncthreads();
             const int i = s_item;   // rewritten only after the barrier that ends this item
@@ -572,6 +738,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
         const uint32_t start = a.occ_start[slot], count = a.occ_count[slot];
         const int umat = (FP32 && a.slot_mat[slot] != 0xFF) ? (int)a.slot_mat[slot] : -1;
+        // the shared material, loaded once per brick (block-uniform)
         // FP32: load alpha with the point state when the brick may hold VonMises points
         const bool pre_al = FP32 && (umat < 0 || a.mats[umat].type == 4);
         const int A0 = g.A[0], A01 = g.A[0] * g.A[1], AN = g.arenaN;
@@ -679,22 +846,8 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                 // the rest of the last 8-byte arena word that the stash ends in
                 if (FP32 && r == nwin - 1 && (stash & 1)) s_sum[stash] = real(0);
                 const real mn = sum[0];
-                real vn[3], fn[3] = {sum[4], sum[5], sum[6]};
-    #pragma unroll
-                for (int q = 0; q < 3; ++q) vn[q] = (mn > m_min) ? sum[1 + q] / mn : real(0);
-                const real vv = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
-                if (sqrt(vv) > real(1.0e-9)) {   // damping (…_MP.cpp:209-222)
-                    const real lf = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
-                    const real inv = real(1) / sqrt(vv);
-                    const real sc = damp * lf;
-    #pragma unroll
-                    for (int q = 0; q < 3; ++q) fn[q] -= sc * (vn[q] * inv);
-                }
-                // f / m after damping: the velocity update and the stored
-                // acceleration share it; a force zeroed below gives 0 / m = 0
-                real ac[3];
-    #pragma unroll
-                for (int q = 0; q < 3; ++q) ac[q] = fn[q] / mn;
+                real vn[3], fn[3], ac[3];
+                node_damp<real>(sum, m_min, damp, vn, fn, ac);
                 if (mn > m_min)
     #pragma unroll
                     for (int q = 0; q < 3; ++q) vn[q] += ac[q] * dtg;
@@ -774,22 +927,22 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             for (int q = 0; q < 6; ++q) vs[q] = real(0);
 
             if (active) {
+                const PointState<real> pt = load_point(P, i, pre_al);
                 real x[3], v[3], F[9], s[6];
                 int32_t xq[3];
     #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    if (FP32) xq[q] = elt(P.xq[q], i);
-                    else x[q] = elt(P.x[q], i);
-                    v[q] = elt(P.v[q], i);
+                    x[q] = pt.x[q];
+                    xq[q] = pt.xq[q];
+                    v[q] = pt.v[q];
                 }
     #pragma unroll
-                for (int q = 0; q < 9; ++q) F[q] = elt(P.F[q], i);
+                for (int q = 0; q < 9; ++q) F[q] = pt.F[q];
     #pragma unroll
-                for (int q = 0; q < 6; ++q) s[q] = elt(P.s[q], i);
-                m = elt(P.mass, i);
-                const real V0 = elt(P.vol0, i);
-                real al_pre = real(0);
-                if (pre_al) al_pre = elt(P.alpha, i);
+                for (int q = 0; q < 6; ++q) s[q] = pt.s[q];
+                m = pt.m;
+                const real V0 = pt.V0;
+                const real al_pre = pt.al;
                 real V = real(0);
 
                 if (G2P) {
@@ -1024,7 +1177,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     #pragma unroll
                     for (int q = 0; q < kQ; ++q) {
                         long long iv;
-                        if constexpr (FP32) iv = (long long)__float2int_rn((float)c[q]);
+                        if constexpr (FP32) iv = (long long)round_i32((float)c[q]);
                         else iv = __double2ll_rn((double)c[q]);
                         atomicAdd(rec + q, (unsigned long long)iv);
                     }
@@ -1187,17 +1340,8 @@ __global__ void __launch_bounds__(256) k_grid_state(GridParams g, const real *ar
         const uint32_t fx = fixed[nid];
         const unsigned long long mk = marks[nid];
         const real mn = sum[0];
-        real vn[3], fn[3] = {sum[4], sum[5], sum[6]};
-        for (int q = 0; q < 3; ++q) vn[q] = (mn > m_min) ? sum[1 + q] / mn : real(0);
-        const real vv = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
-        if (sqrt(vv) > real(1.0e-9)) {   // damping (…_MP.cpp:209-222)
-            const real lf = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
-            const real inv = real(1) / sqrt(vv);
-            const real sc = damp * lf;
-            for (int q = 0; q < 3; ++q) fn[q] -= sc * (vn[q] * inv);
-        }
-        real ac[3];
-        for (int q = 0; q < 3; ++q) ac[q] = fn[q] / mn;
+        real vn[3], fn[3], ac[3];
+        node_damp<real>(sum, m_min, damp, vn, fn, ac);   // damping (…_MP.cpp:209-222)
         if (mn > m_min)
             for (int q = 0; q < 3; ++q) vn[q] += ac[q] * dtg;
         double react = 0.0;
