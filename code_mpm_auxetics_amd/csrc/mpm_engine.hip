@@ -110,6 +110,7 @@ struct mpm_handle {
     uint32_t *queue = nullptr;               // persistent launch: per-step work counters
     int persist_blocks = 0;                  // persistent launch: resident brick blocks
     void *arena[2] = {nullptr, nullptr};
+    int8_t *exps[2] = {nullptr, nullptr};   // FP32: per slot, the fixed-point exponents of its arena (8 B)
     int4 *binfo[2] = {nullptr, nullptr};    // per brick (dense): {slot, packed touched node box}
     size_t arena_cap_slots = 0;
     int parity = 0;
@@ -354,7 +355,10 @@ static int ensure_slots(mpm_handle *h, int n_occ)
 {
     const int want_occ = std::max(1, n_occ + n_occ / 4 + 16);
     if ((size_t)n_occ > h->arena_cap_slots || h->arena[0] == nullptr) {
-        for (int b = 0; b < 2; ++b) dev_free(h->arena[b]);
+        for (int b = 0; b < 2; ++b) {
+            dev_free(h->arena[b]);
+            dev_free((void *&)h->exps[b]);
+        }
         dev_free((void *&)h->kp_part);
         dev_free((void *&)h->gpart);
         // the step kernel addresses arena records with 32-bit element
@@ -369,6 +373,9 @@ static int ensure_slots(mpm_handle *h, int n_occ)
         for (int b = 0; b < 2; ++b) {
             RCHK(dev_alloc(h, &h->arena[b], (elems + 8) * h->rsize));
             HIPCHK(h, hipMemsetAsync((char *)h->arena[b] + elems * h->rsize, 0, 8 * h->rsize, h->stream));
+            // every arena rebuild writes the exponents of its slots before a step reads them
+            RCHK(dalloc(h, &h->exps[b], (size_t)want_occ * 8));
+            HIPCHK(h, hipMemsetAsync(h->exps[b], 0x80, (size_t)want_occ * 8, h->stream));
         }
         RCHK(dalloc(h, &h->kp_part, h->arena_cap_slots * 5 * (kPBlock / 64)));
         HIPCHK(h, hipMemsetAsync(h->kp_part, 0, h->arena_cap_slots * 5 * (kPBlock / 64) * 8, h->stream));
@@ -542,6 +549,7 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     // a rebuild (P2G only) writes the current parity
     for (int b = 0; b < 2; ++b) {
         a.arena[b] = (real *)h->arena[b];
+        a.exps[b] = h->exps[b];
         a.binfo[b] = h->binfo[b];
         a.marks[b] = h->marks[b];
         a.dc_v[b] = h->dc_v[b];
@@ -1196,6 +1204,7 @@ static void destroy_all(mpm_handle *h)
         for (auto &f : h->pfields[b]) dev_free(f.p);
         h->pfields[b].clear();
         dev_free(h->arena[b]);
+        dev_free((void *&)h->exps[b]);
         dev_free((void *&)h->keys[b]);
         dev_free((void *&)h->vals[b]);
     }

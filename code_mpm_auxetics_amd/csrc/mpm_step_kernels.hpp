@@ -401,6 +401,8 @@ struct KpArgs {
     // writes [par ^ 1] (step n+1), par from its step record; the arena
     // rebuild (G2P = false) writes [rebuild_par]
     real *arena[2];             // P2G sums per slot (box-compact 8-real records)
+    int8_t *exps[2];            // FP32 state: per slot, 8 bytes: each field's exponent of its largest
+                                // contribution bound in the step that wrote the arena (kExpNone: all zero)
     int4 *binfo[2];             // per brick {slot or -1, packed touched box}
     unsigned long long *marks[2];         // platen override tags
     double *dc_v[2];                      // platen velocities
@@ -583,6 +585,8 @@ __global__ void __launch_bounds__(256) k_constitutive(int law, CrBatch b)
 // is < 2^26, then added to the 64-bit arena.  FP64 state: 64-bit conversion
 // with FRAC = 61 - H, count < 2^(H-1), so every node sum stays below 2^60.
 constexpr int kFrac32 = 26;
+constexpr int kFracApri = 27;   // FP32 state, a priori exponents (4 bits of headroom in 32 bits)
+constexpr int kExpNone = -128;  // stored exponent of a field whose contributions were all zero
 constexpr int kPBlock = MPM_KP_BLOCK;
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
@@ -608,6 +612,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                                              // base, window low corner in this brick's node
                                              // coordinates}, {window extent, ext x * ext y}
     __shared__ int s_box[kPBlock / 64][6];
+    __shared__ uint32_t s_rmax[kPBlock / 64][kQ];   // a priori: wave maxima of the bounds
     __shared__ double s_red[kPBlock / 64][3];
 
     constexpr bool FP32 = sizeof(real) == 4;
@@ -895,13 +900,35 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         if (tid < 24) s_max[tid >> 3][tid & 7] = 0u;
         int H = 6;   // a node takes at most one contribution per point: count < 2^(H-1)
         while ((1u << (H - 1)) <= count) ++H;
-        const int FRAC = FP32 ? kFrac32 : 61 - H;
+        // Fixed-point exponents.  Agreement (FP64 state, the arena rebuild):
+        // per chunk, a block-wide running maximum of the contribution bounds
+        // sets each field's exponent, and the arena is rescaled when it grows.
+        // A priori (FP32 state, a step): each field's exponent is the brick's
+        // largest bound of the previous step plus one, with 27 fraction bits,
+        // so that contributions up to 2^4 times that bound still convert
+        // exactly to 32 bits; no per-chunk agreement.  The block checks its
+        // bounds after the last chunk and, when one outgrew the exponent (or
+        // fell 8x below it, or the field was zero before), redoes its P2G
+        // from the new state with agreement.
+        constexpr bool APRI = FP32 && G2P;
+        int FRAC = FP32 ? (APRI ? kFracApri : kFrac32) : 61 - H;
         int ecur[kQ];                        // block-uniform field exponents (value < 2^e)
         uint32_t run_max[kQ];                // block-uniform running max of the bound bits
+        uint32_t rmax[kQ];                   // a priori: this lane's running max of the bound bits
     #pragma unroll
-        for (int q = 0; q < kQ; ++q) run_max[q] = 0u;
+        for (int q = 0; q < kQ; ++q) {
+            run_max[q] = 0u;
+            rmax[q] = 0u;
+            ecur[q] = kUnsetExp;
+        }
+        if constexpr (APRI) {
+            const unsigned long long ep = *reinterpret_cast<const unsigned long long *>(a.exps[par] + 8 * (size_t)slot);
     #pragma unroll
-        for (int q = 0; q < kQ; ++q) ecur[q] = kUnsetExp;
+            for (int q = 0; q < kQ; ++q) {
+                const int e = (int)(int8_t)(uint8_t)(ep >> (8 * q));
+                ecur[q] = (e == kExpNone) ? kUnsetExp : e + 1;
+            }
+        }
         double red[5] = {0, 0, 0, 0, 0};     // this lane's running check sums (chunk order)
         int flags = 0;
         int bmin[3] = {1 << 20, 1 << 20, 1 << 20}, bmax[3] = {-1, -1, -1};
@@ -909,6 +936,146 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         // FP32 position increment scale: dt / h * 2^26 per axis
         const float qscale[3] = {(float)(dtd * (double)g.ihf[0] * kQOne), (float)(dtd * (double)g.ihf[1] * kQOne),
                                  (float)(dtd * (double)g.ihf[2] * kQOne)};
+
+        // P2G inputs of step n+1 for one point (…_MP.cpp:106-185): arena key
+        // (or -1), bases, m v, V sigma, f_ext and the per-field bounds of
+        // |contribution| (N <= 1, |dN/dx_j| <= 1/h_j); updates the flags and
+        // the touched box.
+        auto p2g_inputs = [&](const real x[3], const int32_t xq[3], real m, const real v[3], real V, const real s[6],
+                              uint32_t i, int &key, Axis<real> &bx, Axis<real> &by, Axis<real> &bz, real mv[3],
+                              real vs[6], real fe[3], uint32_t bq[kQ]) {
+            axes_of(x, xq, g, O, bx, by, bz);
+            const int cl[3] = {bx.cell, by.cell, bz.cell};
+            const real cu[3] = {bx.u, by.u, bz.u};
+            int lc[3];
+            bool ok = true;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                lc[q] = cl[q] - (O[q] - 1);
+                if (cl[q] < 0 || cl[q] >= g.cells[q]) { flags |= FLAG_OUT_OF_GRID; ok = false; }
+                if (lc[q] < 0 || lc[q] > g.S[q] + 1) { flags |= FLAG_ARENA; ok = false; }
+                if (cu[q] < real(-0.5) || cu[q] >= (real)g.S[q] + real(0.5)) flags |= FLAG_REBIN;
+            }
+            if (ok) {
+                key = lc[0] + A0 * lc[1] + A01 * lc[2];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    bmin[q] = min(bmin[q], lc[q]);
+                    bmax[q] = max(bmax[q], lc[q] + 1);
+                }
+                if (g.has_fext)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) fe[q] = elt(P.fext[q], i);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) mv[q] = m * v[q];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) vs[q] = V * s[q];
+                // bounds of |contribution| per field (N <= 1, |dN/dx_j| <= 1/h_j)
+                const float ih0 = (float)bx.ih, ih1 = (float)by.ih, ih2 = (float)bz.ih;
+                bq[0] = fbits(fabsf((float)m) * 1.0001f);
+                bq[1] = fbits(fabsf((float)mv[0]) * 1.0001f);
+                bq[2] = fbits(fabsf((float)mv[1]) * 1.0001f);
+                bq[3] = fbits(fabsf((float)mv[2]) * 1.0001f);
+                bq[4] = fbits((fabsf((float)vs[0]) * ih0 + fabsf((float)vs[3]) * ih1 + fabsf((float)vs[5]) * ih2 + fabsf((float)fe[0])) * 1.0001f);
+                bq[5] = fbits((fabsf((float)vs[3]) * ih0 + fabsf((float)vs[1]) * ih1 + fabsf((float)vs[4]) * ih2 + fabsf((float)fe[1])) * 1.0001f);
+                bq[6] = fbits((fabsf((float)vs[5]) * ih0 + fabsf((float)vs[4]) * ih1 + fabsf((float)vs[2]) * ih2 + fabsf((float)fe[2])) * 1.0001f);
+            }
+        };
+        // Block-wide exponent agreement of one chunk (see above).
+        auto agree = [&](uint32_t c0, const uint32_t bq[kQ]) {
+            // ---- block-wide fixed-point exponents: running max of the bounds.
+            // Chunk c writes and reads buffer c % 3.  After the barrier below a
+            // wave may run ahead into chunk c + 1 and write its maxima while a
+            // slower wave still reads chunk c's: the buffers differ, and no wave
+            // reaches chunk c + 2 (buffer c + 2 = c - 1 mod 3) before every wave
+            // has passed chunk c + 1's barrier, long after its chunk-c read.  Each
+            // wave adds the running max of the chunks before (block-uniform), so
+            // the buffer holds the running max; wave 0 clears buffer c + 2 after
+            // this chunk's barrier, when chunk c - 1's reads of it are complete.
+            const int mbuf = (int)((c0 / kPBlock) % 3u);
+    #pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const uint32_t w = max(wave_umax_dpp(bq[q]), run_max[q]);
+                if (lane == 0 && w) atomicMax(&s_max[mbuf][q], w);
+            }
+            __syncthreads();
+            if (tid < 8) s_max[mbuf == 0 ? 2 : mbuf - 1][tid] = 0u;
+            {
+                bool any = false;
+                int sh[kQ];
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) {
+                    const uint32_t mb = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_max[mbuf][q]);
+                    run_max[q] = mb;
+                    // positive float bits: biased exponent + 1 bounds the value
+                    const int e = (mb == 0u) ? kUnsetExp : max((int)(mb >> 23) - 126, kMinExp);
+                    const int en = __builtin_amdgcn_readfirstlane(e);
+                    sh[q] = (ecur[q] != kUnsetExp && en > ecur[q]) ? en - ecur[q] : 0;
+                    any |= sh[q] != 0;
+                    ecur[q] = en;
+                }
+                if (any) {   // the exponent grew: rescale what the arena holds
+                    for (int idx = tid; idx < kQ * AN; idx += kPBlock) {
+                        int shq = 0;
+    #pragma unroll
+                        for (int q = 0; q < kQ; ++q) shq = (idx % kQ == q) ? sh[q] : shq;
+                        if (shq) s_arena[idx] = (shq >= 63) ? (s_arena[idx] >> 63) : (s_arena[idx] >> shq);
+                    }
+                    __syncthreads();
+                }
+            }
+        };
+        // P2G adds of one point: its 8 x 7 contributions in fixed point.
+        auto p2g_adds = [&](int key, const Axis<real> &bx, const Axis<real> &by, const Axis<real> &bz, real m,
+                            const real mv[3], const real vs[6], const real fe[3]) {
+                real sc[kQ];
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) {
+                    if constexpr (FP32)
+                        sc[q] = (ecur[q] == kUnsetExp) ? 0.f : __int_as_float((127 + FRAC - ecur[q]) << 23);
+                    else
+                        sc[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, FRAC - ecur[q]);
+                }
+                // scaled per-point factors (power-of-two scaling is exact)
+                const real ms = m * sc[0], mvs[3] = {mv[0] * sc[1], mv[1] * sc[2], mv[2] * sc[3]};
+                const real f4[4] = {-vs[0] * sc[4], -vs[3] * sc[4], -vs[5] * sc[4], fe[0] * sc[4]};
+                const real f5[4] = {-vs[3] * sc[5], -vs[1] * sc[5], -vs[4] * sc[5], fe[1] * sc[5]};
+                const real f6[4] = {-vs[5] * sc[6], -vs[4] * sc[6], -vs[2] * sc[6], fe[2] * sc[6]};
+                long long *rowp[4];   // arena records of the (ay, az) corner rows
+    #pragma unroll
+                for (int r = 0; r < 4; ++r) rowp[r] = s_arena + kQ * (key + A0 * (r >> 1) + A01 * (r & 1));
+                Hat32 w;
+                if constexpr (FP32) w = hat32(bx, by, bz);
+    #pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
+                    real N, G[3];
+                    if constexpr (FP32) {
+                        N = w.wxy[2 * ax + ay] * w.wz[az];
+                        G[0] = w.gx[ax] * w.wyz[2 * ay + az];
+                        G[1] = w.gy[ay] * w.wxz[2 * ax + az];
+                        G[2] = w.gz[az] * w.wxy[2 * ax + ay];
+                    } else {
+                        corner(bx, by, bz, k, N, G);
+                    }
+                    real c[kQ];
+                    c[0] = N * ms;
+                    c[1] = N * mvs[0];
+                    c[2] = N * mvs[1];
+                    c[3] = N * mvs[2];
+                    c[4] = (G[0] * f4[0] + G[1] * f4[1] + G[2] * f4[2]) + N * f4[3];
+                    c[5] = (G[1] * f5[1] + G[0] * f5[0] + G[2] * f5[2]) + N * f5[3];
+                    c[6] = (G[2] * f6[2] + G[0] * f6[0] + G[1] * f6[1]) + N * f6[3];
+                    unsigned long long *rec = reinterpret_cast<unsigned long long *>(rowp[2 * ay + az] + kQ * ax);
+    #pragma unroll
+                    for (int q = 0; q < kQ; ++q) {
+                        long long iv;
+                        if constexpr (FP32) iv = (long long)round_i32((float)c[q]);
+                        else iv = __double2ll_rn((double)c[q]);
+                        atomicAdd(rec + q, (unsigned long long)iv);
+                    }
+                }
+        };
         __syncthreads();
         if (G2P && tid < 3) {
             double t = 0.0;
@@ -1052,147 +1219,96 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                     const bool fin = isfinite(vv) && isfinite(ssum) && (FP32 || isfinite(x[0] + x[1] + x[2]));
                     if (!fin) { red[4] += 1.0; flags |= FLAG_NAN; }
                 }
-                // ---- P2G inputs of step n+1 (…_MP.cpp:106-185)
-                axes_of(x, xq, g, O, bx, by, bz);
-                const int cl[3] = {bx.cell, by.cell, bz.cell};
-                const real cu[3] = {bx.u, by.u, bz.u};
-                int lc[3];
-                bool ok = true;
-    #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    lc[q] = cl[q] - (O[q] - 1);
-                    if (cl[q] < 0 || cl[q] >= g.cells[q]) { flags |= FLAG_OUT_OF_GRID; ok = false; }
-                    if (lc[q] < 0 || lc[q] > g.S[q] + 1) { flags |= FLAG_ARENA; ok = false; }
-                    if (cu[q] < real(-0.5) || cu[q] >= (real)g.S[q] + real(0.5)) flags |= FLAG_REBIN;
-                }
-                if (ok) {
-                    key = lc[0] + A0 * lc[1] + A01 * lc[2];
-    #pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        bmin[q] = min(bmin[q], lc[q]);
-                        bmax[q] = max(bmax[q], lc[q] + 1);
-                    }
-                    if (g.has_fext)
-    #pragma unroll
-                        for (int q = 0; q < 3; ++q) fe[q] = elt(P.fext[q], i);
-    #pragma unroll
-                    for (int q = 0; q < 3; ++q) mv[q] = m * v[q];
-    #pragma unroll
-                    for (int q = 0; q < 6; ++q) vs[q] = V * s[q];
-                    // bounds of |contribution| per field (N <= 1, |dN/dx_j| <= 1/h_j)
-                    const float ih0 = (float)bx.ih, ih1 = (float)by.ih, ih2 = (float)bz.ih;
-                    bq[0] = fbits(fabsf((float)m) * 1.0001f);
-                    bq[1] = fbits(fabsf((float)mv[0]) * 1.0001f);
-                    bq[2] = fbits(fabsf((float)mv[1]) * 1.0001f);
-                    bq[3] = fbits(fabsf((float)mv[2]) * 1.0001f);
-                    bq[4] = fbits((fabsf((float)vs[0]) * ih0 + fabsf((float)vs[3]) * ih1 + fabsf((float)vs[5]) * ih2 + fabsf((float)fe[0])) * 1.0001f);
-                    bq[5] = fbits((fabsf((float)vs[3]) * ih0 + fabsf((float)vs[1]) * ih1 + fabsf((float)vs[4]) * ih2 + fabsf((float)fe[1])) * 1.0001f);
-                    bq[6] = fbits((fabsf((float)vs[5]) * ih0 + fabsf((float)vs[4]) * ih1 + fabsf((float)vs[2]) * ih2 + fabsf((float)fe[2])) * 1.0001f);
-                }
+                p2g_inputs(x, xq, m, v, V, s, i, key, bx, by, bz, mv, vs, fe, bq);
             }
 
-            // ---- block-wide fixed-point exponents: running max of the bounds.
-            // Chunk c writes and reads buffer c % 3.  After the barrier below a
-            // wave may run ahead into chunk c + 1 and write its maxima while a
-            // slower wave still reads chunk c's: the buffers differ, and no wave
-            // reaches chunk c + 2 (buffer c + 2 = c - 1 mod 3) before every wave
-            // has passed chunk c + 1's barrier, long after its chunk-c read.  Each
-            // wave adds the running max of the chunks before (block-uniform), so
-            // the buffer holds the running max; wave 0 clears buffer c + 2 after
-            // this chunk's barrier, when chunk c - 1's reads of it are complete.
-            const int mbuf = (int)((c0 / kPBlock) % 3u);
+            if constexpr (APRI) {
     #pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                const uint32_t w = max(wave_umax_dpp(bq[q]), run_max[q]);
-                if (lane == 0 && w) atomicMax(&s_max[mbuf][q], w);
-            }
-            __syncthreads();
-            if (tid < 8) s_max[mbuf == 0 ? 2 : mbuf - 1][tid] = 0u;
-            {
-                bool any = false;
-                int sh[kQ];
-    #pragma unroll
-                for (int q = 0; q < kQ; ++q) {
-                    const uint32_t mb = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_max[mbuf][q]);
-                    run_max[q] = mb;
-                    // positive float bits: biased exponent + 1 bounds the value
-                    const int e = (mb == 0u) ? kUnsetExp : max((int)(mb >> 23) - 126, kMinExp);
-                    const int en = __builtin_amdgcn_readfirstlane(e);
-                    sh[q] = (ecur[q] != kUnsetExp && en > ecur[q]) ? en - ecur[q] : 0;
-                    any |= sh[q] != 0;
-                    ecur[q] = en;
-                }
-                if (any) {   // the exponent grew: rescale what the arena holds
-                    for (int idx = tid; idx < kQ * AN; idx += kPBlock) {
-                        int shq = 0;
-    #pragma unroll
-                        for (int q = 0; q < kQ; ++q) shq = (idx % kQ == q) ? sh[q] : shq;
-                        if (shq) s_arena[idx] = (shq >= 63) ? (s_arena[idx] >> 63) : (s_arena[idx] >> shq);
-                    }
-                    __syncthreads();
-                }
+                for (int q = 0; q < kQ; ++q) rmax[q] = max(rmax[q], bq[q]);
+            } else {
+                agree(c0, bq);
             }
 
             // ---- P2G: every lane adds its 8 x 7 contributions (fixed point)
-            if (key >= 0) {
-                real sc[kQ];
-    #pragma unroll
-                for (int q = 0; q < kQ; ++q) {
-                    if constexpr (FP32)
-                        sc[q] = (ecur[q] == kUnsetExp) ? 0.f : __int_as_float((127 + FRAC - ecur[q]) << 23);
-                    else
-                        sc[q] = (ecur[q] == kUnsetExp) ? 0.0 : ldexp(1.0, FRAC - ecur[q]);
-                }
-                // scaled per-point factors (power-of-two scaling is exact)
-                const real ms = m * sc[0], mvs[3] = {mv[0] * sc[1], mv[1] * sc[2], mv[2] * sc[3]};
-                const real f4[4] = {-vs[0] * sc[4], -vs[3] * sc[4], -vs[5] * sc[4], fe[0] * sc[4]};
-                const real f5[4] = {-vs[3] * sc[5], -vs[1] * sc[5], -vs[4] * sc[5], fe[1] * sc[5]};
-                const real f6[4] = {-vs[5] * sc[6], -vs[4] * sc[6], -vs[2] * sc[6], fe[2] * sc[6]};
-                long long *rowp[4];   // arena records of the (ay, az) corner rows
-    #pragma unroll
-                for (int r = 0; r < 4; ++r) rowp[r] = s_arena + kQ * (key + A0 * (r >> 1) + A01 * (r & 1));
-                Hat32 w;
-                if constexpr (FP32) w = hat32(bx, by, bz);
-    #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int ax = (k >> 2) & 1, ay = (k >> 1) & 1, az = k & 1;
-                    real N, G[3];
-                    if constexpr (FP32) {
-                        N = w.wxy[2 * ax + ay] * w.wz[az];
-                        G[0] = w.gx[ax] * w.wyz[2 * ay + az];
-                        G[1] = w.gy[ay] * w.wxz[2 * ax + az];
-                        G[2] = w.gz[az] * w.wxy[2 * ax + ay];
-                    } else {
-                        corner(bx, by, bz, k, N, G);
-                    }
-                    real c[kQ];
-                    c[0] = N * ms;
-                    c[1] = N * mvs[0];
-                    c[2] = N * mvs[1];
-                    c[3] = N * mvs[2];
-                    c[4] = (G[0] * f4[0] + G[1] * f4[1] + G[2] * f4[2]) + N * f4[3];
-                    c[5] = (G[1] * f5[1] + G[0] * f5[0] + G[2] * f5[2]) + N * f5[3];
-                    c[6] = (G[2] * f6[2] + G[0] * f6[0] + G[1] * f6[1]) + N * f6[3];
-                    unsigned long long *rec = reinterpret_cast<unsigned long long *>(rowp[2 * ay + az] + kQ * ax);
-    #pragma unroll
-                    for (int q = 0; q < kQ; ++q) {
-                        long long iv;
-                        if constexpr (FP32) iv = (long long)round_i32((float)c[q]);
-                        else iv = __double2ll_rn((double)c[q]);
-                        atomicAdd(rec + q, (unsigned long long)iv);
-                    }
-                }
-            }
+            if (key >= 0) p2g_adds(key, bx, by, bz, m, mv, vs, fe);
         }
 
-        // ---- touched node box of the brick: wave bounds first, so that one
-        // barrier covers both them and the arena adds
+        // ---- touched node box of the brick (and, a priori, the wave maxima
+        // of the bounds): wave values first, so that one barrier covers both
+        // them and the arena adds
     #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const int lo = wave_imin_dpp(bmin[q]), hi = wave_imax_dpp(bmax[q]);
             if (lane == 0) { s_box[wv][2 * q] = lo; s_box[wv][2 * q + 1] = hi; }
         }
+        if constexpr (APRI) {
+    #pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const uint32_t w = wave_umax_dpp(rmax[q]);
+                if (lane == 0) s_rmax[wv][q] = w;
+            }
+        }
         __syncthreads();
+        // the block's exponents of this step (the arena's record for the next)
+        int enext[kQ];
+    #pragma unroll
+        for (int q = 0; q < kQ; ++q) enext[q] = ecur[q];
+        if constexpr (APRI) {
+            bool redo = false;
+    #pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                uint32_t mb = 0u;
+                for (int w = 0; w < kPBlock / 64; ++w) mb = max(mb, s_rmax[w][q]);
+                const int e = (mb == 0u) ? kUnsetExp : max((int)(mb >> 23) - 126, kMinExp);
+                enext[q] = e;
+                if (e != kUnsetExp && (ecur[q] == kUnsetExp || e > ecur[q] + 4 || e < ecur[q] - 3)) redo = true;
+            }
+            if (redo) {
+                // P2G again from the state just written, with agreement; the
+                // flags, check sums and touched box of the pass above stand
+                for (int i = tid; i < kQ * AN; i += kPBlock) s_arena[i] = 0;
+                if (tid < 24) s_max[tid >> 3][tid & 7] = 0u;
+                FRAC = kFrac32;
+    #pragma unroll
+                for (int q = 0; q < kQ; ++q) {
+                    run_max[q] = 0u;
+                    ecur[q] = kUnsetExp;
+                }
+                const int fl0 = flags;
+                const int bsave[6] = {bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2]};
+                __syncthreads();
+                for (uint32_t c0 = 0; c0 < count; c0 += kPBlock) {
+                    const uint32_t i = start + c0 + tid;
+                    int key = -1;
+                    Axis<real> bx, by, bz;
+                    real m = real(0), mv[3] = {real(0), real(0), real(0)}, vs[6] = {real(0), real(0), real(0), real(0), real(0), real(0)};
+                    real fe[3] = {real(0), real(0), real(0)};
+                    uint32_t bq[kQ] = {0, 0, 0, 0, 0, 0, 0};
+                    if (c0 + tid < count) {
+                        const PointState<real> pt = load_point(P, i, false);
+                        double Fd[9];
+    #pragma unroll
+                        for (int q = 0; q < 9; ++q) Fd[q] = fd(pt.F[q], q);
+                        const real V = (real)(det3(Fd) * (double)pt.V0);
+                        m = pt.m;
+                        p2g_inputs(pt.x, pt.xq, pt.m, pt.v, V, pt.s, i, key, bx, by, bz, mv, vs, fe, bq);
+                    }
+                    agree(c0, bq);
+                    if (key >= 0) p2g_adds(key, bx, by, bz, m, mv, vs, fe);
+                }
+                flags = fl0;
+                bmin[0] = bsave[0]; bmin[1] = bsave[1]; bmin[2] = bsave[2];
+                bmax[0] = bsave[3]; bmax[1] = bsave[4]; bmax[2] = bsave[5];
+                __syncthreads();
+            }
+        }
+        if (tid == 0 && FP32) {
+            unsigned long long ep = 0ull;
+    #pragma unroll
+            for (int q = 0; q < kQ; ++q)
+                ep |= (unsigned long long)(uint8_t)(int8_t)(enext[q] == kUnsetExp ? kExpNone : enext[q]) << (8 * q);
+            *reinterpret_cast<unsigned long long *>(a.exps[G2P ? par ^ 1 : par] + 8 * (size_t)slot) = ep;
+        }
         int box[6];
     #pragma unroll
         for (int q = 0; q < 3; ++q) {
