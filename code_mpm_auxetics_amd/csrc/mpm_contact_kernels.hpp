@@ -992,14 +992,22 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
 template <typename real>
 __global__ void __launch_bounds__(256) k_c_dcmarks(CArgs<real> a, int step)
 {
+    // per node the largest tag, reduced per block in LDS (mark_corners_max)
+    __shared__ unsigned long long s_mk[4096];
+    __shared__ int s_pbox[4][6];
     const int d = blockIdx.x * 256 + threadIdx.x;
-    if (d >= a.n_dc) return;
-    double x[3] = {a.dc_x[3 * d], a.dc_x[3 * d + 1], a.dc_x[3 * d + 2]};
-    long long idx[8];
-    int ijk[8][3];
-    if (!c_adjacent<kCFast<real>>(a.g, x, idx, ijk)) return;
-    const unsigned long long tag = ((unsigned long long)(a.steps[step].epoch + 1u) << 32) | (unsigned)d;
-    for (int k = 0; k < 8; ++k) atomicMax(&a.cmarks[idx[k]], tag);
+    bool on = false;
+    int i0[3] = {0, 0, 0};
+    unsigned long long tag = 0ull;
+    if (d < a.n_dc) {
+        const double x[3] = {a.dc_x[3 * d], a.dc_x[3 * d + 1], a.dc_x[3 * d + 2]};
+        long long idx[8];
+        int ijk[8][3];
+        on = c_adjacent<kCFast<real>>(a.g, x, idx, ijk);
+        for (int q = 0; q < 3; ++q) i0[q] = ijk[0][q];
+        tag = ((unsigned long long)(a.steps[step].epoch + 1u) << 32) | (unsigned)d;
+    }
+    mark_corners_max(on, i0, tag, a.g, a.cmarks, s_mk, 4096, s_pbox);
 }
 
 // ---- C7 + C8 (nodes): two-field update, contact correction, platen override

@@ -390,6 +390,67 @@ __device__ __forceinline__ void node_damp(const real sum[kQ], real m_min, real d
     }
 }
 
+// Override marks of platen points (…_MP.cpp:245-265): each node touched by
+// a point (corners i0 .. i0 + 1; those outside the grid are skipped) ends
+// with the largest tag of the points touching it.  Many platen points share
+// a node, and same-address global atomics serialize at the L2, so the block
+// first takes the maximum per node in LDS (s_mk, cap entries) over the
+// bounding box of its points' base nodes (platen points are stored in a
+// spatially compact order) and then issues one global atomicMax per marked
+// node; a box larger than cap falls back to one global atomic per point and
+// corner.  Called by every thread of a 256-thread block (on: the thread has
+// a point).  s_pbox: LDS scratch of the block's wave bounds.
+__device__ __forceinline__ void mark_corners_max(bool on, const int i0[3], unsigned long long tag, const GridParams &g,
+                                                 unsigned long long *marks, unsigned long long *s_mk, int cap,
+                                                 int (*s_pbox)[6])
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int lo[3], hi[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int c = min(max(i0[q], -1), g.nodes[q]);
+        lo[q] = wave_imin(on ? c : (1 << 20));
+        hi[q] = wave_imax(on ? c + 1 : -(1 << 20));
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { s_pbox[wv][2 * q] = lo[q]; s_pbox[wv][2 * q + 1] = hi[q]; }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        for (int w = 0; w < 4; ++w) {
+            lo[q] = min(lo[q], s_pbox[w][2 * q]);
+            hi[q] = max(hi[q], s_pbox[w][2 * q + 1]);
+        }
+    if (hi[0] < lo[0]) return;   // no point in this block (uniform)
+    const int e0 = hi[0] - lo[0] + 1, e1 = hi[1] - lo[1] + 1, e2 = hi[2] - lo[2] + 1;
+    const long long nbox = (long long)e0 * e1 * e2;
+    const bool in_lds = nbox <= (long long)cap;   // uniform
+    if (in_lds) {
+        for (int r = tid; r < (int)nbox; r += 256) s_mk[r] = 0ull;
+        __syncthreads();
+    }
+    if (on)
+        for (int k = 0; k < 8; ++k) {
+            const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
+            if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= g.nodes[0] || n1 >= g.nodes[1] || n2 >= g.nodes[2]) continue;
+            if (in_lds)
+                atomicMax(&s_mk[(n0 - lo[0]) + e0 * ((n1 - lo[1]) + e1 * (n2 - lo[2]))], tag);
+            else
+                atomicMax(&marks[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
+        }
+    if (in_lds) {
+        __syncthreads();
+        for (int r = tid; r < (int)nbox; r += 256) {
+            const unsigned long long t = s_mk[r];
+            if (!t) continue;
+            const int n2 = lo[2] + r / (e0 * e1), rem = r % (e0 * e1);
+            const int n1 = lo[1] + rem / e0, n0 = lo[0] + rem % e0;
+            atomicMax(&marks[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], t);
+        }
+    }
+}
+
 constexpr int kUnsetExp = -100000;
 constexpr int kMinExp = -90;   // contributions below 2^-90 of a unit are dropped
 
@@ -657,60 +718,9 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             for (int q = 0; q < 3; ++q) i0[q] = (int)(x[q] / g.h[q]);
             tag = ((unsigned long long)(st.epoch + 2u) << 32) | (unsigned)j;
         }
-        // Many platen points share a node, and same-address global atomics
-        // serialize at the L2.  The block first takes the maximum per node in
-        // LDS over the bounding box of its points' base nodes (platen points
-        // are stored in a spatially compact order), then issues one global
-        // atomicMax per marked node.  A box larger than the LDS arena falls
-        // back to one global atomic per point and corner.  Either way each
-        // node ends with the maximum tag.
         __shared__ int s_pbox[kPBlock / 64][6];
-        int lo[3], hi[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int c = min(max(i0[q], -1), g.nodes[q]);
-            lo[q] = wave_imin(on ? c : (1 << 20));
-            hi[q] = wave_imax(on ? c + 1 : -(1 << 20));
-        }
-        if (lane == 0)
-#pragma unroll
-            for (int q = 0; q < 3; ++q) { s_pbox[wv][2 * q] = lo[q]; s_pbox[wv][2 * q + 1] = hi[q]; }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            for (int w = 0; w < kPBlock / 64; ++w) {
-                lo[q] = min(lo[q], s_pbox[w][2 * q]);
-                hi[q] = max(hi[q], s_pbox[w][2 * q + 1]);
-            }
-        }
-        if (hi[0] < lo[0]) return;   // no point in this block (uniform)
-        const int e0 = hi[0] - lo[0] + 1, e1 = hi[1] - lo[1] + 1, e2 = hi[2] - lo[2] + 1;
-        const long long nbox = (long long)e0 * e1 * e2;
-        unsigned long long *s_mk = reinterpret_cast<unsigned long long *>(s_arena);
-        const bool in_lds = nbox <= (long long)kQ * g.arenaN;   // uniform
-        if (in_lds) {
-            for (int r = tid; r < (int)nbox; r += kPBlock) s_mk[r] = 0ull;
-            __syncthreads();
-        }
-        if (on)
-            for (int k = 0; k < 8; ++k) {
-                const int n0 = i0[0] + ((k >> 2) & 1), n1 = i0[1] + ((k >> 1) & 1), n2 = i0[2] + (k & 1);
-                if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= g.nodes[0] || n1 >= g.nodes[1] || n2 >= g.nodes[2]) continue;
-                if (in_lds)
-                    atomicMax(&s_mk[(n0 - lo[0]) + e0 * ((n1 - lo[1]) + e1 * (n2 - lo[2]))], tag);
-                else
-                    atomicMax(&marks_out[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], tag);
-            }
-        if (in_lds) {
-            __syncthreads();
-            for (int r = tid; r < (int)nbox; r += kPBlock) {
-                const unsigned long long t = s_mk[r];
-                if (!t) continue;
-                const int n2 = lo[2] + r / (e0 * e1), rem = r % (e0 * e1);
-                const int n1 = lo[1] + rem / e0, n0 = lo[0] + rem % e0;
-                atomicMax(&marks_out[n0 + (long long)g.nodes[0] * (n1 + (long long)g.nodes[1] * n2)], t);
-            }
-        }
+        mark_corners_max(on, i0, tag, g, marks_out, reinterpret_cast<unsigned long long *>(s_arena), kQ * g.arenaN,
+                         s_pbox);
         return;
     }
 
