@@ -103,6 +103,7 @@ struct mpm_handle {
     int32_t *n_occ_d = nullptr;
     uint32_t *slot_order = nullptr;          // launch order of the brick blocks
     uint8_t *slot_mat = nullptr;             // per slot: shared material index or 0xFF
+    int4 *idesc = nullptr;                   // per launch-order item: 2 int4 (k_item_desc)
     std::vector<uint32_t> slot_order_h, occ_count_h;
     int n_occ = 0, occ_cap = 0;
     int n_xcd = 8;                           // XCDs the dispatcher deals workgroups to (MI355X: 8)
@@ -518,6 +519,10 @@ static int rebin_t(mpm_handle *h, bool from_abs)
     if (n_occ > 0) {
         k_slot_mat<<<n_occ, 256, 0, h->stream>>>(parr<real>(h, h->cur).mat, h->occ_start, h->occ_count, h->slot_mat);
         HIPCHK(h, hipGetLastError());
+        k_item_desc<<<cdiv(n_occ, 256), 256, 0, h->stream>>>(h->g, h->slot_order, h->occ_list, h->occ_start,
+                                                              h->occ_count, h->slot_mat, h->mats, h->fp32 ? 1 : 0,
+                                                              (int)n_occ, h->idesc);
+        HIPCHK(h, hipGetLastError());
     }
     // brick records of both parities: stale slots must read -1; the rebuild
     // below writes this parity's records of the occupied bricks
@@ -562,6 +567,7 @@ static int launch_particle(mpm_handle *h, bool g2p, int step)
     a.occ_count = h->occ_count;
     a.n_occ = h->n_occ_d;
     a.slot_order = h->slot_order;
+    a.idesc = h->idesc;
     a.slot_mat = h->slot_mat;
     // platen blocks first (a multiple of the XCD count), then the brick blocks
     const bool persist = g2p && h->cfg.persistent && h->persist_blocks > 0;
@@ -1223,7 +1229,7 @@ static void destroy_all(mpm_handle *h)
     void **ptrs[] = {(void **)&h->dc_x, (void **)&h->dc_v[0], (void **)&h->dc_v[1], (void **)&h->dc_m,
                      (void **)&h->fixed, (void **)&h->marks[0], (void **)&h->marks[1], (void **)&h->occ_list,
                      (void **)&h->occ_start, (void **)&h->occ_count, (void **)&h->n_occ_d,
-                     (void **)&h->slot_order, (void **)&h->slot_mat, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
+                     (void **)&h->slot_order, (void **)&h->slot_mat, (void **)&h->idesc, (void **)&h->binfo[0], (void **)&h->binfo[1], (void **)&h->mats,
                      (void **)&h->steps_d,
                      (void **)&h->out_d, (void **)&h->queue, (void **)&h->kp_part, (void **)&h->gpart, (void **)&h->grid3_d,
                      (void **)&h->fy_slot, (void **)&h->fy_off, (void **)&h->fy_val,
@@ -1393,7 +1399,7 @@ int mpm_create(const mpm_config *cfg, mpm_handle **out)
     }
     const size_t nb = (size_t)h->nbricks;
     if ((rc = dalloc(h, &h->occ_list, nb)) || (rc = dalloc(h, &h->occ_start, nb)) || (rc = dalloc(h, &h->occ_count, nb)) ||
-        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->slot_order, nb)) || (rc = dalloc(h, &h->slot_mat, nb)) ||
+        (rc = dalloc(h, &h->n_occ_d, 1)) || (rc = dalloc(h, &h->slot_order, nb)) || (rc = dalloc(h, &h->slot_mat, nb)) || (rc = dalloc(h, &h->idesc, 2 * nb)) ||
         (rc = dalloc(h, &h->binfo[0], nb)) || (rc = dalloc(h, &h->binfo[1], nb)) ||
         (rc = dalloc(h, &h->checks_d, 16)) || (rc = dalloc(h, &h->grid3_d, 3)))
         return fail(rc);

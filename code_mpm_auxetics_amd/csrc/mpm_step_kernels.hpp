@@ -472,6 +472,7 @@ struct KpArgs {
     const uint8_t *fixed;       // node fixity bits
     const uint32_t *occ_list, *occ_start, *occ_count;
     const uint32_t *slot_order;       // brick block b works on slot slot_order[b]
+    const int4 *idesc;                // per block b: its slot's descriptor (k_item_desc), 2 int4
     const uint8_t *slot_mat;          // per slot: the material of all its points, or 0xFF if mixed
     const int32_t *n_occ;
     int32_t n_pblk;             // platen blocks at the front of the grid
@@ -632,6 +633,26 @@ __global__ void __launch_bounds__(256) k_slot_mat(const uint8_t *mat, const uint
     if (threadIdx.x == 0) slot_mat[s] = (smin == smax) ? (uint8_t)smin : (uint8_t)0xFF;
 }
 
+// Per launch-order item of the step kernel (block b works on slot
+// slot_order[b]): {slot, brick, first point, point count} and {the brick's one
+// material (FP32 state) or -1, alpha loaded with the point state (FP32: a
+// VonMises point may be present), brick x | brick y << 16, brick z}.
+__global__ void __launch_bounds__(256) k_item_desc(GridParams g, const uint32_t *slot_order, const uint32_t *occ_list,
+                                                   const uint32_t *occ_start, const uint32_t *occ_count,
+                                                   const uint8_t *slot_mat, const Material *mats, int fp32, int n_occ,
+                                                   int4 *idesc)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_occ) return;
+    const uint32_t slot = slot_order[i], brick = occ_list[slot];
+    const int umat = (fp32 && slot_mat[slot] != 0xFF) ? (int)slot_mat[slot] : -1;
+    const int pre_al = (fp32 && (umat < 0 || mats[umat].type == 4)) ? 1 : 0;
+    const int bx = (int)(brick % (uint32_t)g.nb[0]), by = (int)((brick / (uint32_t)g.nb[0]) % (uint32_t)g.nb[1]);
+    const int bz = (int)(brick / ((uint32_t)g.nb[0] * (uint32_t)g.nb[1]));
+    idesc[2 * i] = make_int4((int)slot, (int)brick, (int)occ_start[slot], (int)occ_count[slot]);
+    idesc[2 * i + 1] = make_int4(umat, pre_al, bx | (by << 16), bz);
+}
+
 // ---------------------------------------------------- batched constitutive
 // One thread per input of an mpm_constitutive batch (body: cr_apply).
 __global__ void __launch_bounds__(256) k_constitutive(int law, CrBatch b)
@@ -747,16 +768,22 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             if (i >= nx) return;
             item = X * i + x;
         }
-        const int slot = (int)a.slot_order[item];
-        const uint32_t brick = a.occ_list[slot];
-        const int bc[3] = {(int)(brick % g.nb[0]), (int)((brick / g.nb[0]) % g.nb[1]), (int)(brick / (g.nb[0] * g.nb[1]))};
+        // the item's descriptor (k_item_desc): one uniform 32-byte load instead
+        // of a chain of dependent lookups (slot, brick, start, count, material)
+        const int4 d0 = a.idesc[2 * item], d1 = a.idesc[2 * item + 1];
+        const int slot = d0.x;
+        const uint32_t brick = (uint32_t)d0.y;
+        const int bc[3] = {d1.z & 0xFFFF, d1.z >> 16, d1.w};
         const int O[3] = {bc[0] * g.S[0], bc[1] * g.S[1], bc[2] * g.S[2]};
-        const uint32_t start = a.occ_start[slot], count = a.occ_count[slot];
-        const int umat = (FP32 && a.slot_mat[slot] != 0xFF) ? (int)a.slot_mat[slot] : -1;
-        // the shared material, loaded once per brick (block-uniform)
-        // FP32: load alpha with the point state when the brick may hold VonMises points
-        const bool pre_al = FP32 && (umat < 0 || a.mats[umat].type == 4);
+        const uint32_t start = (uint32_t)d0.z, count = (uint32_t)d0.w;
+        const int umat = d1.x;           // FP32: the brick's one material, or -1
+        const bool pre_al = d1.y != 0;   // FP32: load alpha with the point state (VonMises points may be present)
         const int A0 = g.A[0], A01 = g.A[0] * g.A[1], AN = g.arenaN;
+
+        // The first chunk's point state, loaded after the gather pass has issued
+        // (and waited for) its arena loads, so that its HBM latency overlaps the
+        // update pass, the arena clear and the barrier before the chunks.
+        PointState<real> pt0;
 
         // ---- phase A: grid update of step n over the node window = this brick's
         // touched box of the last P2G (…_MP.cpp:209-265)
@@ -844,6 +871,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                 wt[0] = mk;
                 wt[1] = fw;
             }
+            if ((uint32_t)tid < count) pt0 = load_point(a.p, start + tid, pre_al);
             // Pass 2: the grid update of each window node (…_MP.cpp:209-265)
             double gs[3] = {0.0, 0.0, 0.0};   // owned nodes: mass, reaction, active count
             for (int r = tid; r < nwin; r += kPBlock) {
@@ -1104,7 +1132,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             for (int q = 0; q < 6; ++q) vs[q] = real(0);
 
             if (active) {
-                const PointState<real> pt = load_point(P, i, pre_al);
+                const PointState<real> pt = (G2P && c0 == 0) ? pt0 : load_point(P, i, pre_al);
                 real x[3], v[3], F[9], s[6];
                 int32_t xq[3];
     #pragma unroll
