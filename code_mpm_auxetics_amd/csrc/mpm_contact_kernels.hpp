@@ -1007,7 +1007,7 @@ __global__ void __launch_bounds__(256) k_c_dcmarks(CArgs<real> a, int step)
         for (int q = 0; q < 3; ++q) i0[q] = ijk[0][q];
         tag = ((unsigned long long)(a.steps[step].epoch + 1u) << 32) | (unsigned)d;
     }
-    mark_corners_max(on, i0, tag, a.g, a.cmarks, s_mk, 4096, s_pbox);
+    mark_corners_max<256>(on, i0, tag, a.g, a.cmarks, s_mk, 4096, s_pbox);
 }
 
 // ---- C7 + C8 (nodes): two-field update, contact correction, platen override

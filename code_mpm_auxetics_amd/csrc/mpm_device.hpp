@@ -65,7 +65,8 @@ struct StepRec {
     int32_t has_tl;
     uint32_t epoch;
     int32_t par;
-    int32_t pad;
+    int32_t skip_sums;   // 1: nothing reads this step's point check sums or grid mass and node
+                         // count (a step of a batch before its last); 0: compute them
 };
 static_assert(sizeof(StepRec) % 4 == 0, "step records are copied as 32-bit words");
 

@@ -1007,7 +1007,10 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
     // enqueue the in_batch steps whose records are in recs(slot)
     auto launch = [&]() -> int {
         StepRec *r = recs(slot);
-        for (int k = 0; k < in_batch; ++k) r[k].par = (h->parity + k) & 1;
+        for (int k = 0; k < in_batch; ++k) {
+            r[k].par = (h->parity + k) & 1;
+            r[k].skip_sums = (k + 1 < in_batch) ? 1 : 0;   // sums are read after the batch's last step
+        }
         if (path == MPM_PATH_PLAIN && h->n_occ > 0) {
             if (graphs_on(h) && in_batch == K) {
                 hipGraphExec_t exec = nullptr;

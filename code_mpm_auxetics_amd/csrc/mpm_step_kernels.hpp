@@ -398,8 +398,9 @@ __device__ __forceinline__ void node_damp(const real sum[kQ], real m_min, real d
 // bounding box of its points' base nodes (platen points are stored in a
 // spatially compact order) and then issues one global atomicMax per marked
 // node; a box larger than cap falls back to one global atomic per point and
-// corner.  Called by every thread of a 256-thread block (on: the thread has
-// a point).  s_pbox: LDS scratch of the block's wave bounds.
+// corner.  Called by every thread of an NT-thread block (on: the thread has
+// a point).  s_pbox: LDS scratch of the block's wave bounds (NT / 64 rows).
+template <int NT>
 __device__ __forceinline__ void mark_corners_max(bool on, const int i0[3], unsigned long long tag, const GridParams &g,
                                                  unsigned long long *marks, unsigned long long *s_mk, int cap,
                                                  int (*s_pbox)[6])
@@ -418,7 +419,7 @@ __device__ __forceinline__ void mark_corners_max(bool on, const int i0[3], unsig
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 3; ++q)
-        for (int w = 0; w < 4; ++w) {
+        for (int w = 0; w < NT / 64; ++w) {
             lo[q] = min(lo[q], s_pbox[w][2 * q]);
             hi[q] = max(hi[q], s_pbox[w][2 * q + 1]);
         }
@@ -427,7 +428,7 @@ __device__ __forceinline__ void mark_corners_max(bool on, const int i0[3], unsig
     const long long nbox = (long long)e0 * e1 * e2;
     const bool in_lds = nbox <= (long long)cap;   // uniform
     if (in_lds) {
-        for (int r = tid; r < (int)nbox; r += 256) s_mk[r] = 0ull;
+        for (int r = tid; r < (int)nbox; r += NT) s_mk[r] = 0ull;
         __syncthreads();
     }
     if (on)
@@ -441,7 +442,7 @@ __device__ __forceinline__ void mark_corners_max(bool on, const int i0[3], unsig
         }
     if (in_lds) {
         __syncthreads();
-        for (int r = tid; r < (int)nbox; r += 256) {
+        for (int r = tid; r < (int)nbox; r += NT) {
             const unsigned long long t = s_mk[r];
             if (!t) continue;
             const int n2 = lo[2] + r / (e0 * e1), rem = r % (e0 * e1);
@@ -701,6 +702,9 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     const GridParams &g = a.g;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, rl = lane & 15;
     const StepRec st = a.steps[step];
+    // the point check sums and the grid mass and node count of this step are
+    // read (the last step of a batch, and an arena rebuild)
+    const bool sums = !G2P || st.skip_sums == 0;
     const double dtd = st.dt;
     const real dt = (real)dtd;
     // this step's ping-pong parity (a rebuild writes the current parity)
@@ -740,7 +744,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             tag = ((unsigned long long)(st.epoch + 2u) << 32) | (unsigned)j;
         }
         __shared__ int s_pbox[kPBlock / 64][6];
-        mark_corners_max(on, i0, tag, g, marks_out, reinterpret_cast<unsigned long long *>(s_arena), kQ * g.arenaN,
+        mark_corners_max<kPBlock>(on, i0, tag, g, marks_out, reinterpret_cast<unsigned long long *>(s_arena), kQ * g.arenaN,
                          s_pbox);
         return;
     }
@@ -926,6 +930,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
             // precedes the point chunks (no barrier of its own)
     #pragma unroll
             for (int k = 0; k < 3; ++k) {
+                if (k != 1 && !sums) continue;   // only the reaction is read every step
                 const double w = wave_sum_dpp(gs[k]);
                 if (lane == 0) s_red[wv][k] = w;
             }
@@ -1118,7 +1123,7 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         if (G2P && tid < 3) {
             double t = 0.0;
             for (int w = 0; w < kPBlock / 64; ++w) t += s_red[w][tid];
-            a.gpart[3 * (size_t)slot + tid] = t;
+            a.gpart[3 * (size_t)slot + tid] = (tid == 1 || sums) ? t : 0.0;
         }
 
         for (uint32_t c0 = 0; c0 < count; c0 += kPBlock) {
@@ -1245,8 +1250,13 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
                     for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
                     V = (real)(det3(Fd) * (double)V0);
                 }
-                // ---- FP64 checks of the new state (PhysicsEngine.cpp:73-79)
-                {
+                // ---- FP64 checks of the new state (PhysicsEngine.cpp:73-79), when
+                // the step's sums are read; the non-finite flag in every step
+                if (!sums) {
+                    const real sv = fabs(v[0]) + fabs(v[1]) + fabs(v[2]);
+                    const real ssum = s[0] + s[1] + s[2] + s[3] + s[4] + s[5];
+                    if (!(isfinite(sv) && isfinite(ssum) && (FP32 || isfinite(x[0] + x[1] + x[2])))) flags |= FLAG_NAN;
+                } else {
                     const double md = (double)m, vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
                     const double vv = vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2];
                     red[0] += md * vv;
@@ -1387,11 +1397,12 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
         // ---- check sums: one fixed-order partial per wave, written by the wave
         // itself (no block barrier at the end); flags: OR over the wave, only when
         // a lane has one
+        if (sums)
     #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const double wsum = wave_sum_dpp(red[k]);
-            if (lane == 0) a.kp_part[5 * ((size_t)slot * (kPBlock / 64) + wv) + k] = wsum;
-        }
+            for (int k = 0; k < 5; ++k) {
+                const double wsum = wave_sum_dpp(red[k]);
+                if (lane == 0) a.kp_part[5 * ((size_t)slot * (kPBlock / 64) + wv) + k] = wsum;
+            }
         if (__ballot(flags != 0)) {
             int fl = flags;
     #pragma unroll
