@@ -162,6 +162,7 @@ struct mpm_handle {
         int nblk_n = 0, nblk_p = 0;         // node blocks; capacity of the point-sum partials
         int n_ppart = 0;                    // point-sum partials the last contact G2P wrote
         bool kernel_valid = false;          // kp/kg hold the last contact step's point kernel
+        bool kernel_dirty = false;          // kp_id/kg_id not yet rewritten from kp/kg since that step
     } ct;
     int last_path = 0;                      // path of the last step (checks read its partials)
     // hipGraphs of batches of plain steps, keyed by their step count and
@@ -429,8 +430,26 @@ static void prof_collect(mpm_handle *h, std::vector<ProfMark> &marks)
 
 // ------------------------------------------------------------------ rebin
 template <typename real>
+static CArgs<real> contact_args(mpm_handle *h);
+// d_Kernel / d3_Kernel_Gradient of the last contact step by original point
+// index (readback order), from the per-point values in the current order
+template <typename real>
+static int kernel_out(mpm_handle *h)
+{
+    if (!h->ct.kernel_dirty) return MPM_OK;
+    h->ct.kernel_dirty = false;
+    CArgs<real> ca = contact_args<real>(h);
+    if (ca.n_all > 0)
+        k_c_kernel_out<real><<<cdiv(ca.n_all, 256), 256, 0, h->stream>>>(ca, h->ct.kp_id, h->ct.kg_id, h->n_total);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));   // readers copy kp_id / kg_id right after
+    return MPM_OK;
+}
+
+template <typename real>
 static int rebin_t(mpm_handle *h, bool from_abs)
 {
+    RCHK(kernel_out<real>(h));   // before the points are reordered
     const uint32_t n = (uint32_t)h->n_def;
     if (n == 0) {
         h->n_occ = 0;
@@ -1023,9 +1042,9 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
         } else if (path == MPM_PATH_CONTACT) {
             HIPCHK(h, hipMemcpyAsync(h->steps_d, r, sizeof(StepRec) * in_batch, hipMemcpyHostToDevice, h->stream));
             for (int k = 0; k < in_batch; ++k) RCHK(launch_contact<real>(h, k));
-            CArgs<real> ca = contact_args<real>(h);
-            if (ca.n_all > 0)
-                k_c_kernel_out<real><<<cdiv(ca.n_all, 256), 256, 0, h->stream>>>(ca, h->ct.kp_id, h->ct.kg_id, h->n_total);
+            // the point kernel by original index is written when it is read
+            // (readback) or before a rebin reorders the points
+            h->ct.kernel_dirty = true;
             // per-step platen reaction of the batch
             k_c_step_sums<<<(unsigned)in_batch, 256, 0, h->stream>>>(h->ct.cpart, h->ct.nblk_n, h->hist_d);
             k_sum3<<<1, 256, 0, h->stream>>>(h->ct.cpart + (size_t)(in_batch - 1) * h->ct.nblk_n * 3, nullptr,
@@ -1830,6 +1849,7 @@ static int readback_t(mpm_handle *h, uint32_t mask, mpm_points *out)
     dev_free((void *&)order_d);
     // d_Kernel / d3_Kernel_Gradient of the last contact step (zero before one)
     if (mask & MPM_F_KERNEL) {
+        RCHK(kernel_out<real>(h));
         if (out->kernel) {
             if (h->ct.kernel_valid) HIPCHK(h, hipMemcpy(out->kernel, h->ct.kp_id, sizeof(double) * n, hipMemcpyDeviceToHost));
             else std::fill(out->kernel, out->kernel + n, 0.0);

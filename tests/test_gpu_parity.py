@@ -362,3 +362,30 @@ def test_async_step_matches_blocking_step(mpm, prec):
     eng = mpm.PhysicsEngine(sc, precision=p)
     eng.step_async(dt)
     eng.close()                       # joins the worker
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_platen_marks_wide_platen_fp64(mpm, oracle_mod, shuffle):
+    """Platen override marks (…_MP.cpp:245-265: the highest platen index
+    touching a node wins).  The step kernel takes the maximum per node in LDS
+    over the bounding box of a block's 256 platen points when the box fits its
+    arena, else one global atomic per point and corner.  With bricks of
+    3 x 3 x 3 cells (1512 arena entries) and a platen 81 x 2 x 11 nodes wide,
+    platen points in their stored (compact) order take the LDS path and a
+    shuffled order takes the fallback; both match the oracle."""
+    sc = mpm.scene.polygon_mesh_world(4, 1, 0.004, 0.004, 0.0008, 0.002, 0.0002, "elastic", 1.0e9, 0.3, 1200.0,
+                                      platen_speed=2.0, steps=40)
+    if shuffle:
+        d = np.nonzero(sc.dc != 0)[0]
+        perm = np.arange(sc.n)
+        perm[d] = np.random.default_rng(3).permutation(d)
+        for k in ("x", "mass", "vol0", "E", "nu", "sy", "C0", "C1", "type", "dc", "v", "fext", "surface"):
+            setattr(sc, k, getattr(sc, k)[perm].copy())
+    x0 = sc.x.copy()
+    eng = mpm.PhysicsEngine(sc, precision=mpm.FP64, brick=(3, 3, 3))
+    orc = oracle_mod.Oracle(sc)
+    eng.run_steps(40)
+    orc.run_steps(40)
+    _compare(eng, orc, x0, 1e-9, 1e-9, 1e-9, deform_only=False)
+    assert len(eng.reaction_history()) == len(orc.reaction)
+    eng.close()
