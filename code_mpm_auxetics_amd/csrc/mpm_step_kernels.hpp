@@ -674,14 +674,6 @@ constexpr int kPBlock = MPM_KP_BLOCK;
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
 
-// Nearest integer (ties up: floor(x + 1/2)) in one instruction, for the FP32
-// fixed-point terms (|x| < 2^31).
-__device__ __forceinline__ int round_i32(float x)
-{
-    int r;
-    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-    return r;
-}
 
 // One brick per workgroup.  Dynamic LDS: long long arena[7 * arenaN].
 template <typename real, bool G2P, bool PERSIST = false>
@@ -1113,7 +1105,10 @@ __global__ void __launch_bounds__(kPBlock, sizeof(real) == 4 ? MPM_KP_WAVES32 : 
     #pragma unroll
                     for (int q = 0; q < kQ; ++q) {
                         long long iv;
-                        if constexpr (FP32) iv = (long long)round_i32((float)c[q]);
+                        // round to nearest even: the one-instruction floor(x + 1/2)
+                        // (v_cvt_rpi_i32_f32) rounds odd integers above 2^23 up,
+                        // since its add rounds, which biased the node sums
+                        if constexpr (FP32) iv = (long long)__float2int_rn((float)c[q]);
                         else iv = __double2ll_rn((double)c[q]);
                         atomicAdd(rec + q, (unsigned long long)iv);
                     }
