@@ -939,6 +939,12 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     __shared__ uint32_t s_ns[NS];   // MASS: bit 0 near_surf (staged); bits CF_CPOS | CF_CNEG (detection)
     const GridParams &g = a.g;
     const CRange r = c_range(a);
+    const int end = r.j0 + r.n;
+    const bool brick = (int)blockIdx.x < a.n_bb;
+    // brick blocks: the first point's inputs are loaded before the window
+    // set-up, so their latency overlaps it
+    CP2gRaw<real> cur, nxt;
+    if (brick && r.j0 + (int)threadIdx.x < end) c_p2g_raw<real, MASS>(a, r.j0 + (int)threadIdx.x, cur);
     int lo[3], hi[3];
     const bool any = c_range_box(a, r, lo, hi);
     const CWin w = c_window_box(any, lo, hi, sbox, 0, 1, kCWin);
@@ -952,12 +958,9 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     }
     __syncthreads();
     const double sc0 = c_scale(a, MASS ? 0 : 2), sc1 = c_scale(a, MASS ? 1 : 3);
-    const int end = r.j0 + r.n;
-    if ((int)blockIdx.x < a.n_bb) {
+    if (brick) {
         // deformable points of one brick, the next point's inputs in flight
         int j = r.j0 + (int)threadIdx.x;
-        CP2gRaw<real> cur, nxt;
-        if (j < end) c_p2g_raw<real, MASS>(a, j, cur);
 #pragma clang loop unroll(disable)
         for (; j < end; j += 256) {
             if (j + 256 < end) c_p2g_raw<real, MASS>(a, j + 256, nxt);
@@ -1151,13 +1154,20 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
     bool valid = false;
     uint32_t fm = 0u;
     double v[3] = {0.0, 0.0, 0.0};
+    real F[9], s[6], mass = real(0);
+    int umat = -1;
     if (j < a.n_def) {
         c_pos(a, j, x);
         valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
         if (!valid) flags |= FLAG_OUT_OF_GRID;
-        // loaded before the window set-up: the latency overlaps it
+        // the point's own state is loaded before the window set-up: the
+        // latency overlaps it
         fm = a.fmask[j];
         for (int q = 0; q < 3; ++q) v[q] = (double)a.p.v[q][j];
+        for (int q = 0; q < 9; ++q) F[q] = a.p.F[q][j];
+        for (int q = 0; q < 6; ++q) s[q] = a.p.s[q][j];
+        mass = a.p.mass[j];
+        if constexpr (sizeof(real) == 4) umat = a.p.mat[j];
     }
     // the block's node window (velocity, gated acceleration, flags) staged in LDS
     const CWin w = c_window_box(valid, ijk[0], ijk[0], sbox, 0, 1, kCWin);
@@ -1206,8 +1216,8 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
                     for (int r = 0; r < 3; ++r) L[3 * c + r] += vI[r] * G[c];
             }
             // F += L F dt, V = det(F) V0 (…_MP.cpp:299-302), then the material update
-            real F[9], Lr[9], s[6];
-            for (int q = 0; q < 9; ++q) { F[q] = P.F[q][j]; Lr[q] = (real)L[q]; }
+            real Lr[9];
+            for (int q = 0; q < 9; ++q) Lr[q] = (real)L[q];
             real T[9];   // FP32 state stores D = F - I: D += (L + L D) dt
             for (int c = 0; c < 3; ++c)
                 for (int r = 0; r < 3; ++r)
@@ -1215,7 +1225,6 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
                                    (Lr[r] * F[3 * c] + Lr[3 + r] * F[3 * c + 1] + Lr[6 + r] * F[3 * c + 2]);
             for (int q = 0; q < 9; ++q) F[q] += T[q] * (real)dt;
             for (int q = 0; q < 9; ++q) P.F[q][j] = F[q];
-            for (int q = 0; q < 6; ++q) s[q] = P.s[q][j];
             for (int q = 0; q < 3; ++q) P.v[q][j] = (real)v[q];
             if constexpr (FP32) {
                 const uint32_t b = P.pbrick[j];
@@ -1234,10 +1243,12 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
                     if (u < -0.5 || u >= (double)g.S[q] + 0.5) flags |= FLAG_REBIN;
                 }
             }
-            material_update<real>(g, P, a.mats, (uint32_t)j, Lr, (real)dt, F, s);
+            // FP32 state: the point's own material index, loaded above, as
+            // the block-uniform one (the same table entry)
+            material_update<real>(g, P, a.mats, (uint32_t)j, Lr, (real)dt, F, s, umat);
             // checks of the new state (PhysicsEngine.cpp:73-79)
-            const double md = (double)P.mass[j];
-            const double vd[3] = {(double)P.v[0][j], (double)P.v[1][j], (double)P.v[2][j]};
+            const double md = (double)mass;
+            const double vd[3] = {(double)(real)v[0], (double)(real)v[1], (double)(real)v[2]};
             const double vv = vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2];
             red[0] = md * vv;
             for (int q = 0; q < 3; ++q) red[1 + q] = md * vd[q];
