@@ -132,25 +132,31 @@ __device__ __forceinline__ void c_bases(const GridParams &g, const double x[3], 
     G[2] = -s[0] * s[1] * sg[2] / g.h[2];
 }
 
-// the 8 adjacent nodes of x in reference order (x-outer, z-inner); false if out of grid
+// the cell of x (its lowest node i0); false if out of grid
 template <bool FAST>
-__device__ __forceinline__ bool c_adjacent(const GridParams &g, const double x[3], long long idx[8], int ijk[8][3])
+__device__ __forceinline__ bool c_cell(const GridParams &g, const double x[3], int i0[3])
 {
-    int i0[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         if (!(x[q] >= 0.0)) return false;
         i0[q] = (int)(FAST ? x[q] * g.ih[q] : x[q] / g.h[q]);
         if (!(i0[q] < g.cells[q])) return false;
     }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        ijk[k][0] = i0[0] + ((k >> 2) & 1);
-        ijk[k][1] = i0[1] + ((k >> 1) & 1);
-        ijk[k][2] = i0[2] + (k & 1);
-        idx[k] = ijk[k][0] + (long long)g.nodes[0] * (ijk[k][1] + (long long)g.nodes[1] * ijk[k][2]);
-    }
     return true;
+}
+// corner k of a cell (x-outer, z-inner): offsets 0/1 along x, y, z
+__device__ __forceinline__ int c_kx(int k) { return (k >> 2) & 1; }
+__device__ __forceinline__ int c_ky(int k) { return (k >> 1) & 1; }
+__device__ __forceinline__ int c_kz(int k) { return k & 1; }
+// linear node index of the cell's lowest node
+__device__ __forceinline__ long long c_node0(const GridParams &g, const int i0[3])
+{
+    return i0[0] + (long long)g.nodes[0] * (i0[1] + (long long)g.nodes[1] * i0[2]);
+}
+// node index of corner k from the lowest node's
+__device__ __forceinline__ long long c_cnode(const GridParams &g, long long I0, int k)
+{
+    return I0 + c_kx(k) + (long long)g.nodes[0] * (c_ky(k) + (long long)g.nodes[1] * c_kz(k));
 }
 
 // Per axis, the two node planes of the cell (node index i0 and i0 + 1): the
@@ -339,14 +345,13 @@ __device__ __forceinline__ bool c_range_box(const CArgs<real> &a, const CRange &
 #pragma clang loop unroll(disable)
     for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
         double x[3];
-        long long idx[8];
-        int ijk[8][3];
+        int i0[3];
         c_pos(a, j, x);
-        if (!c_adjacent<kCFast<real>>(a.g, x, idx, ijk)) continue;
+        if (!c_cell<kCFast<real>>(a.g, x, i0)) continue;
         any = true;
         for (int q = 0; q < 3; ++q) {
-            lo[q] = min(lo[q], ijk[0][q]);
-            hi[q] = max(hi[q], ijk[0][q]);
+            lo[q] = min(lo[q], i0[q]);
+            hi[q] = max(hi[q], i0[q]);
         }
     }
     return any;
@@ -354,6 +359,11 @@ __device__ __forceinline__ bool c_range_box(const CArgs<real> &a, const CRange &
 __device__ __forceinline__ int c_wslot(const CWin &w, const int n[3])
 {
     return (n[0] - w.lo[0]) + w.dim[0] * ((n[1] - w.lo[1]) + w.dim[1] * (n[2] - w.lo[2]));
+}
+// window slot of corner k from the slot of the cell's lowest node
+__device__ __forceinline__ int c_cslot(const CWin &w, int s0, int k)
+{
+    return s0 + c_kx(k) + w.dim[0] * (c_ky(k) + w.dim[1] * c_kz(k));
 }
 __device__ __forceinline__ long long c_wnode(const GridParams &g, const CWin &w, int t)
 {
@@ -469,17 +479,16 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
         double bp[4], x[3];
         c_point_bounds(a, j, bp);
         for (int q = 0; q < 4; ++q) b[q] = fmax(b[q], bp[q]);
-        long long idx[8];
-        int ijk[8][3];
+        int i0[3];
         c_pos(a, j, x);
-        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) {
+        if (!c_cell<kCFast<real>>(g, x, i0)) {
             oog = 1;
             continue;
         }
         any = true;
         for (int q = 0; q < 3; ++q) {
-            lo[q] = min(lo[q], ijk[0][q]);
-            hi[q] = max(hi[q], ijk[0][q]);
+            lo[q] = min(lo[q], i0[q]);
+            hi[q] = max(hi[q], i0[q]);
         }
     }
     if (oog) atomicOr(a.gflags, FLAG_OUT_OF_GRID);
@@ -495,15 +504,17 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
 #pragma clang loop unroll(disable)
     for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
         double x[3];
-        long long idx[8];
-        int ijk[8][3];
+        int i0[3];
         c_pos(a, j, x);
-        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) continue;
-        for (int k = 0; k < 8; ++k) {
-            if (w.lds) atomicOr(&s_f[c_wslot(w, ijk[k])], CF_ACTIVE);
-            else atomicOr(&a.flags[idx[k]], CF_ACTIVE);
+        if (!c_cell<kCFast<real>>(g, x, i0)) continue;
+        {
+            const long long I0 = c_node0(g, i0);
+            const int s0 = w.lds ? c_wslot(w, i0) : 0;
+            for (int k = 0; k < 8; ++k) {
+                if (w.lds) atomicOr(&s_f[c_cslot(w, s0, k)], CF_ACTIVE);
+                else atomicOr(&a.flags[c_cnode(g, I0, k)], CF_ACTIVE);
+            }
         }
-        const int *i0 = ijk[0];
         // per axis, the candidate planes i0 - 1 + t (t = 0..3) inside the grid
         // and within R on that axis, as bits
         uint32_t near[3];
@@ -611,28 +622,32 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
 #pragma clang loop unroll(disable)
     for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
         double x[3];
-        long long idx[8];
-        int ijk[8][3];
+        int i0[3];
         c_pos(a, j, x);
-        if (!c_adjacent<kCFast<real>>(g, x, idx, ijk)) continue;
+        if (!c_cell<kCFast<real>>(g, x, i0)) continue;
+        // corner nodes and window slots are formed per corner from the
+        // cell's lowest node, which keeps fewer registers live
+        const long long I0 = c_node0(g, i0);
+        const int s0 = w.lds ? c_wslot(w, i0) : 0;
         const bool surf = c_surf(a, j) != 0;
         double kp = 0.0, kg[3] = {0.0, 0.0, 0.0};
         CAx A;
-        c_axes<kCFast<real>>(g, x, ijk[0], A);
+        c_axes<kCFast<real>>(g, x, i0, A);
         for (int k = 0; k < 8; ++k) {
             double xi[3], N, G[3];
             c_corner<kCFast<real>>(g, A, k, xi, N, G);
             const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
             const unsigned long long dk = (unsigned long long)__double_as_longlong(c_len3(d));
             if (w.lds) {
-                const int ws = c_wslot(w, ijk[k]);
+                const int ws = c_cslot(w, s0, k);
                 kp += N * s_kv[0][ws];
                 for (int q = 0; q < 3; ++q) kg[q] += N * s_kv[1 + q][ws];
                 atomicMin(&s_near[ws], dk);
             } else {
-                kp += N * ((double)a.acc[idx[k]] * usk);
-                for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + idx[k]];
-                atomicMin(&a.near_d[idx[k]], dk);
+                const long long I = c_cnode(g, I0, k);
+                kp += N * ((double)a.acc[I] * usk);
+                for (int q = 0; q < 3; ++q) kg[q] += N * a.gkgrad[q * a.nn + I];
+                atomicMin(&a.near_d[I], dk);
             }
         }
         a.kp[j] = kp;
@@ -642,8 +657,8 @@ __global__ void __launch_bounds__(256) k_c_point1(CArgs<real> a)
             if (lk > 0.0) {
                 const unsigned long long lb = (unsigned long long)__double_as_longlong(lk);
                 for (int k = 0; k < 8; ++k) {
-                    if (w.lds) atomicMax(&s_sg[c_wslot(w, ijk[k])], lb);
-                    else atomicMax(&a.sg_d[idx[k]], lb);
+                    if (w.lds) atomicMax(&s_sg[c_cslot(w, s0, k)], lb);
+                    else atomicMax(&a.sg_d[c_cnode(g, I0, k)], lb);
                 }
             }
         }
@@ -669,13 +684,11 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
     const int j = c_chunk((int)gridDim.x, a.n_xcd) * 256 + threadIdx.x;
     const GridParams &g = a.g;
     double x[3] = {0.0, 0.0, 0.0};
-    long long idx[8];
-    int ijk[8][3];
-    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
+    int i0[3] = {0, 0, 0};
     bool valid = j < a.n_all;
     if (valid) {
         c_pos(a, j, x);
-        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
+        valid = c_cell<kCFast<real>>(g, x, i0);
     }
     // the point's own inputs first: their latency overlaps the window set-up
     unsigned long long key = 0ull, lk = 0ull;
@@ -686,7 +699,7 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
         surf = c_surf(a, j) != 0;
         lk = (unsigned long long)__double_as_longlong(c_len3(kg));
     }
-    const CWin w = c_window_box(valid, ijk[0], ijk[0], sbox, 0, 1, kCWin);
+    const CWin w = c_window_box(valid, i0, i0, sbox, 0, 1, kCWin);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         const long long I = c_wnode(g, w, t);
         s_nd[t] = a.near_d[I];
@@ -697,19 +710,22 @@ __global__ void __launch_bounds__(256) k_c_point2(CArgs<real> a)
     __syncthreads();
     if (valid) {
         CAx A;
-        c_axes<kCFast<real>>(g, x, ijk[0], A);
+        c_axes<kCFast<real>>(g, x, i0, A);
+        const long long I0 = c_node0(g, i0);
+        const int s0 = w.lds ? c_wslot(w, i0) : 0;
         for (int k = 0; k < 8; ++k) {
             double xi[3], N, G[3];
             c_corner<kCFast<real>>(g, A, k, xi, N, G);
             const double d[3] = {x[0] - xi[0], x[1] - xi[1], x[2] - xi[2]};
             const unsigned long long db = (unsigned long long)__double_as_longlong(c_len3(d));
             if (w.lds) {
-                const int ws = c_wslot(w, ijk[k]);
+                const int ws = c_cslot(w, s0, k);
                 if (db == s_nd[ws]) atomicMin(&s_ni[ws], key);
                 if (surf && lk != 0ull && lk == s_sd[ws]) atomicMin(&s_si[ws], key);
             } else {
-                if (db == a.near_d[idx[k]]) atomicMin(&a.near_i[idx[k]], key);
-                if (surf && lk != 0ull && lk == a.sg_d[idx[k]]) atomicMin(&a.sg_i[idx[k]], key);
+                const long long I = c_cnode(g, I0, k);
+                if (db == a.near_d[I]) atomicMin(&a.near_i[I], key);
+                if (surf && lk != 0ull && lk == a.sg_d[I]) atomicMin(&a.sg_i[I], key);
             }
         }
     }
@@ -864,17 +880,18 @@ __device__ __forceinline__ void c_p2g_point(const CArgs<real> &a, const CWin &w,
                                             double sc0, double sc1)
 {
     const GridParams &g = a.g;
-    long long idx[8];
-    int ijk[8][3];
-    if (!c_adjacent<kCFast<real>>(g, in.x, idx, ijk)) return;
+    int i0[3];
+    if (!c_cell<kCFast<real>>(g, in.x, i0)) return;
     uint32_t mask = 0u;
     CAx A;
-    c_axes<kCFast<real>>(g, in.x, ijk[0], A);
+    c_axes<kCFast<real>>(g, in.x, i0, A);
+    const long long I0 = c_node0(g, i0);
+    const int s0 = w.lds ? c_wslot(w, i0) : 0;
     for (int k = 0; k < 8; ++k) {
-        const long long I = idx[k];
+        const long long I = c_cnode(g, I0, k);
         double xi[3], N, G[3];
         c_corner<kCFast<real>>(g, A, k, xi, N, G);
-        const int ws = w.lds ? c_wslot(w, ijk[k]) : 0;
+        const int ws = w.lds ? c_cslot(w, s0, k) : 0;
         int f;
         bool ns = false;
         if constexpr (MASS) {
@@ -1004,10 +1021,7 @@ __global__ void __launch_bounds__(256) k_c_dcmarks(CArgs<real> a, int step)
     unsigned long long tag = 0ull;
     if (d < a.n_dc) {
         const double x[3] = {a.dc_x[3 * d], a.dc_x[3 * d + 1], a.dc_x[3 * d + 2]};
-        long long idx[8];
-        int ijk[8][3];
-        on = c_adjacent<kCFast<real>>(a.g, x, idx, ijk);
-        for (int q = 0; q < 3; ++q) i0[q] = ijk[0][q];
+        on = c_cell<kCFast<real>>(a.g, x, i0);
         tag = ((unsigned long long)(a.steps[step].epoch + 1u) << 32) | (unsigned)d;
     }
     mark_corners_max<256>(on, i0, tag, a.g, a.cmarks, s_mk, 4096, s_pbox);
@@ -1148,9 +1162,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
     double red[5] = {0, 0, 0, 0, 0};
     int flags = 0;
     double x[3] = {0.0, 0.0, 0.0};
-    long long idx[8];
-    int ijk[8][3];
-    ijk[0][0] = ijk[0][1] = ijk[0][2] = 0;
+    int i0[3] = {0, 0, 0};
     bool valid = false;
     uint32_t fm = 0u;
     double v[3] = {0.0, 0.0, 0.0};
@@ -1158,7 +1170,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
     int umat = -1;
     if (j < a.n_def) {
         c_pos(a, j, x);
-        valid = c_adjacent<kCFast<real>>(g, x, idx, ijk);
+        valid = c_cell<kCFast<real>>(g, x, i0);
         if (!valid) flags |= FLAG_OUT_OF_GRID;
         // the point's own state is loaded before the window set-up: the
         // latency overlaps it
@@ -1170,7 +1182,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
         if constexpr (sizeof(real) == 4) umat = a.p.mat[j];
     }
     // the block's node window (velocity, gated acceleration, flags) staged in LDS
-    const CWin w = c_window_box(valid, ijk[0], ijk[0], sbox, 0, 1, kCWin);
+    const CWin w = c_window_box(valid, i0, i0, sbox, 0, 1, kCWin);
     for (int t = threadIdx.x; t < w.nodes; t += 256) {
         const long long I = c_wnode(g, w, t);
         s_fl[t] = a.flags[I];
@@ -1187,16 +1199,18 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
         {
             double xn[3] = {x[0], x[1], x[2]}, L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             CAx A;
-            c_axes<kCFast<real>>(g, x, ijk[0], A);
+            c_axes<kCFast<real>>(g, x, i0, A);
+            const long long I0 = c_node0(g, i0);
+            const int s0 = w.lds ? c_wslot(w, i0) : 0;
             for (int k = 0; k < 8; ++k) {
-                const long long I = idx[k];
+                const long long I = c_cnode(g, I0, k);
                 double xi[3], N, G[3];
                 c_corner<kCFast<real>>(g, A, k, xi, N, G);
                 const int f = (int)((fm >> k) & 1u);
                 uint32_t fl;
                 double vI[3], aI[3];
                 if (w.lds) {
-                    const int ws = c_wslot(w, ijk[k]);
+                    const int ws = c_cslot(w, s0, k);
                     fl = s_fl[ws];
                     for (int q = 0; q < 3; ++q) {
                         vI[q] = s_n[3 * f + q][ws];
