@@ -263,6 +263,10 @@ __device__ __forceinline__ double wave_dmax(double v)
 // global atomic (the sums are exact integers, so this equals adding every
 // term globally).  A box beyond the window falls back to global atomics.
 constexpr int kCWin = 512;
+// the P2G passes' window: 416 nodes keep their LDS under 40 KB, four blocks
+// per CU (a brick of 4 x 4 x 10 cells touches 275 nodes, 330 with a drifted
+// layer)
+constexpr int kCWinP = 416;
 struct CWin {
     int lo[3], dim[3], nodes;
     bool lds;
@@ -897,7 +901,7 @@ __device__ __forceinline__ void c_p2g_point(const CArgs<real> &a, const CWin &w,
         if constexpr (MASS) {
             double gk[3];
             if (w.lds) {
-                for (int q = 0; q < 3; ++q) gk[q] = s_gk[q * kCWin + ws];
+                for (int q = 0; q < 3; ++q) gk[q] = s_gk[q * kCWinP + ws];
                 ns = (s_ns[ws] & 1u) != 0;
             } else {
                 for (int q = 0; q < 3; ++q) gk[q] = a.gkg[q * a.nn + I];
@@ -949,10 +953,10 @@ template <typename real, bool MASS, bool SMALL>
 __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
 {
     constexpr int NF = MASS ? 8 : 12;
-    constexpr int NS = MASS ? kCWin : 1;   // staged surface gradients and flags (MASS only)
+    constexpr int NS = MASS ? kCWinP : 1;   // staged surface gradients and flags (MASS only)
     __shared__ int sbox[6];
-    __shared__ unsigned long long lds[kCWin * NF];
-    __shared__ double s_gk[3 * NS];   // MASS: staged surface gradients, component q at q * kCWin
+    __shared__ unsigned long long lds[kCWinP * NF];
+    __shared__ double s_gk[3 * NS];   // MASS: staged surface gradients, component q at q * kCWinP
     __shared__ uint32_t s_ns[NS];   // MASS: bit 0 near_surf (staged); bits CF_CPOS | CF_CNEG (detection)
     const GridParams &g = a.g;
     const CRange r = c_range(a);
@@ -964,12 +968,12 @@ __global__ void __launch_bounds__(256) k_c_p2g(CArgs<real> a)
     if (brick && r.j0 + (int)threadIdx.x < end) c_p2g_raw<real, MASS>(a, r.j0 + (int)threadIdx.x, cur);
     int lo[3], hi[3];
     const bool any = c_range_box(a, r, lo, hi);
-    const CWin w = c_window_box(any, lo, hi, sbox, 0, 1, kCWin);
+    const CWin w = c_window_box(any, lo, hi, sbox, 0, 1, kCWinP);
     for (int t = threadIdx.x; t < w.nodes * NF; t += 256) lds[t] = 0ull;
     if constexpr (MASS) {
         for (int t = threadIdx.x; t < w.nodes; t += 256) {
             const long long I = c_wnode(g, w, t);
-            for (int q = 0; q < 3; ++q) s_gk[q * kCWin + t] = a.gkg[q * a.nn + I];
+            for (int q = 0; q < 3; ++q) s_gk[q * kCWinP + t] = a.gkg[q * a.nn + I];
             s_ns[t] = a.near_surf[I] ? 1u : 0u;
         }
     }
