@@ -48,7 +48,9 @@ struct CArgs {
     uint8_t *fmask;               // per point (current order, platens after deformable): field of each corner
     double *ga;                   // 6 x nn: gated node acceleration f / m per field (field-major)
     unsigned long long *cmarks;   // platen tags of this step
-    unsigned long long *bounds;   // kCB positive doubles as bits
+    unsigned long long *bounds;   // kCB positive doubles as bits, this step's
+    unsigned long long *bounds_next;   // the next step's, maxed by the G2P over the new point state
+    int32_t def_bounds;           // 1: the kernel pass forms the deformable points' bounds itself
     double *kp, *kg;              // per point (current order, platens after deformable): k, grad k (3 x n_all)
     const Material *mats;
     const StepRec *steps;
@@ -398,18 +400,40 @@ __global__ void __launch_bounds__(256) k_c_clear(CArgs<real> a)
         a.sg_i[I] = ~0ull;
         a.near_surf[I] = 0;
     }
-    // bound slot 4 is the kernel density's (contributions <= 1)
-    if (blockIdx.x == 0 && threadIdx.x < kCB)
-        a.bounds[threadIdx.x] = threadIdx.x == 4 ? (unsigned long long)__double_as_longlong(1.0) : 0ull;
+    // bound slot 4 is the kernel density's (contributions <= 1).  This
+    // step's bounds already hold the deformable points' maxima when the
+    // previous step's G2P formed them; the next step's start from zero.
+    if (blockIdx.x == 0 && threadIdx.x < kCB) {
+        const unsigned long long b0 = threadIdx.x == 4 ? (unsigned long long)__double_as_longlong(1.0) : 0ull;
+        if (a.def_bounds) a.bounds[threadIdx.x] = b0;
+        a.bounds_next[threadIdx.x] = b0;
+    }
 }
 
 // ---- bounds of the contributions of this step: per point, then block maxima
 // (one atomic per block and bound, only when the block beats the current one)
+__device__ __forceinline__ double c_ihm(const GridParams &g)
+{
+    return fmax(1.0 / g.h[0], fmax(1.0 / g.h[1], 1.0 / g.h[2]));
+}
+// the four bounds of a point's contributions from its mass, velocity,
+// volume, stress and external force (the same expressions wherever they are
+// formed, unfused, so both places give the same bits)
+__device__ __forceinline__ void c_bounds_of(double ihm, double m, const double v[3], double vol, const double s[6],
+                                            const double fe[3], double b[4])
+{
+#pragma clang fp contract(off)
+    double smax = 0.0;
+    for (int q = 0; q < 6; ++q) smax = fmax(smax, fabs(s[q]));
+    b[0] = fabs(m);
+    b[1] = fabs(m) * ihm;
+    b[2] = fabs(m) * fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2])));
+    b[3] = fabs(vol) * 3.0 * smax * ihm + fmax(fabs(fe[0]), fmax(fabs(fe[1]), fabs(fe[2])));
+}
 template <typename real>
 __device__ __forceinline__ void c_point_bounds(const CArgs<real> &a, int j, double b[4])
 {
     const GridParams &g = a.g;
-    const double ihm = fmax(1.0 / g.h[0], fmax(1.0 / g.h[1], 1.0 / g.h[2]));
     double m, v[3], vol, s[6], fe[3] = {0, 0, 0};
     if (j >= a.n_def) {
         const int d = j - a.n_def;
@@ -428,12 +452,7 @@ __device__ __forceinline__ void c_point_bounds(const CArgs<real> &a, int j, doub
         }
         for (int q = 0; q < 6; ++q) s[q] = (double)a.p.s[q][j];
     }
-    double smax = 0.0;
-    for (int q = 0; q < 6; ++q) smax = fmax(smax, fabs(s[q]));
-    b[0] = fabs(m);
-    b[1] = fabs(m) * ihm;
-    b[2] = fabs(m) * fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2])));
-    b[3] = fabs(vol) * 3.0 * smax * ihm + fmax(fabs(fe[0]), fmax(fabs(fe[1]), fabs(fe[2])));
+    c_bounds_of(c_ihm(g), m, v, vol, s, fe, b);
 }
 template <typename real>
 __device__ __forceinline__ void c_bounds_flush(const CArgs<real> &a, const double b[4], double (*s_b)[4])
@@ -481,8 +500,12 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
 #pragma clang loop unroll(disable)
     for (int j = r.j0 + (int)threadIdx.x; j < r.j0 + r.n; j += 256) {
         double bp[4], x[3];
-        c_point_bounds(a, j, bp);
-        for (int q = 0; q < 4; ++q) b[q] = fmax(b[q], bp[q]);
+        // the deformable points' bounds come from the previous step's G2P
+        // unless this is the first step of a run
+        if (j >= a.n_def || a.def_bounds) {
+            c_point_bounds(a, j, bp);
+            for (int q = 0; q < 4; ++q) b[q] = fmax(b[q], bp[q]);
+        }
         int i0[3];
         c_pos(a, j, x);
         if (!c_cell<kCFast<real>>(g, x, i0)) {
@@ -1156,6 +1179,7 @@ template <typename real>
 __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
 {
     __shared__ double s_r[4][5];
+    __shared__ double s_bn[4][4];   // per wave: maxima of the next step's bounds
     __shared__ int sbox[6];
     __shared__ double s_n[12][kCWin];   // per window node: v.xyz of field f at 3f, a.xyz at 6 + 3f
     __shared__ uint32_t s_fl[kCWin];
@@ -1164,6 +1188,7 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const GridParams &g = a.g;
     double red[5] = {0, 0, 0, 0, 0};
+    double bn[4] = {0.0, 0.0, 0.0, 0.0};   // bounds of the new state's contributions in the next step
     int flags = 0;
     double x[3] = {0.0, 0.0, 0.0};
     int i0[3] = {0, 0, 0};
@@ -1275,11 +1300,26 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
                 red[4] = 1.0;
                 flags |= FLAG_NAN;
             }
+            // the next step's P2G bounds of this point, from the stored new
+            // state (the values the kernel pass would read)
+            {
+                double Fd[9], sd[6], fe[3] = {0.0, 0.0, 0.0};
+                for (int q = 0; q < 9; ++q) Fd[q] = fd(F[q], q);
+                for (int q = 0; q < 6; ++q) sd[q] = (double)s[q];
+                if (g.has_fext)
+                    for (int q = 0; q < 3; ++q) fe[q] = (double)P.fext[q][j];
+                const double vol = det3(Fd) * (double)P.vol0[j];
+                c_bounds_of(c_ihm(g), md, vd, vol, sd, fe, bn);
+            }
         }
     }
     for (int k = 0; k < 5; ++k) {
         const double wsum = wave_sum_dpp(red[k]);
         if (lane == 0) s_r[wv][k] = wsum;
+    }
+    for (int q = 0; q < 4; ++q) {
+        const double wm = wave_dmax(bn[q]);
+        if (lane == 0) s_bn[wv][q] = wm;
     }
     int fl = flags;
     for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
@@ -1289,6 +1329,13 @@ __global__ void __launch_bounds__(256) k_c_g2p(CArgs<real> a, int step)
         double r = 0.0;
         for (int q = 0; q < 4; ++q) r += s_r[q][threadIdx.x];
         a.ppart[5 * (size_t)chunk + threadIdx.x] = r;
+    }
+    if (threadIdx.x >= 64 && threadIdx.x < 68) {
+        const int q = threadIdx.x - 64;
+        const double wm = fmax(fmax(s_bn[0][q], s_bn[1][q]), fmax(s_bn[2][q], s_bn[3][q]));
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(wm);
+        if (wm > 0.0 && wb > __hip_atomic_load(&a.bounds_next[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&a.bounds_next[q], wb);
     }
 }
 

@@ -162,6 +162,8 @@ struct mpm_handle {
         int nblk_n = 0, nblk_p = 0;         // node blocks; capacity of the point-sum partials
         int n_ppart = 0;                    // point-sum partials the last contact G2P wrote
         bool kernel_valid = false;          // kp/kg hold the last contact step's point kernel
+        int bsel = 0;                       // which half of bounds is the coming step's
+        bool bnext_valid = false;           // the previous G2P formed the coming step's deformable bounds
         bool kernel_dirty = false;          // kp_id/kg_id not yet rewritten from kp/kg since that step
     } ct;
     int last_path = 0;                      // path of the last step (checks read its partials)
@@ -642,7 +644,7 @@ static int ensure_contact(mpm_handle *h)
     RCHK(dalloc(h, &c.near_d, nn));
     RCHK(dalloc(h, &c.sg_d, nn));
     RCHK(dalloc(h, &c.cmarks, nn));
-    RCHK(dalloc(h, &c.bounds, (size_t)kCB));
+    RCHK(dalloc(h, &c.bounds, (size_t)2 * kCB));   // this step's and the next step's
     RCHK(dalloc(h, &c.near_surf, nn));
     RCHK(dalloc(h, &c.kp, n_all));
     RCHK(dalloc(h, &c.kg, 3 * n_all));
@@ -695,7 +697,9 @@ static CArgs<real> contact_args(mpm_handle *h)
     a.fmask = c.fmask;
     a.ga = c.ga;
     a.cmarks = c.cmarks;
-    a.bounds = c.bounds;
+    a.bounds = c.bounds + (size_t)kCB * c.bsel;
+    a.bounds_next = c.bounds + (size_t)kCB * (c.bsel ^ 1);
+    a.def_bounds = c.bnext_valid ? 0 : 1;
     a.kp = c.kp;
     a.kg = c.kg;
     a.mats = h->mats;
@@ -753,6 +757,9 @@ static int launch_contact(mpm_handle *h, int k)
     if (a.n_dc > 0) k_c_dcmove<real><<<cdiv(a.n_dc, 256), 256, 0, st>>>(a, k);
     HIPCHK(h, hipGetLastError());
     c.kernel_valid = true;
+    // the G2P maxed the next step's deformable bounds over the new state
+    c.bsel ^= 1;
+    c.bnext_valid = a.n_def > 0;
     return MPM_OK;
 }
 
@@ -989,6 +996,9 @@ static int step_impl(mpm_handle *h, double dt_total, int path, int32_t *finished
     if (path == MPM_PATH_CONTACT) {
         RCHK(ensure_contact(h));
         RCHK(locate_point0<real>(h));
+        // the first contact step of a call forms every bound from the point
+        // state as it is now (uploads and plain steps may have changed it)
+        h->ct.bnext_valid = false;
     } else if (h->plain_stale) {
         // contact steps moved the points: re-bin, rebuild the arenas and the
         // platen tags of the current step

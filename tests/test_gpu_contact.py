@@ -152,6 +152,33 @@ def test_switching_paths(mpm, oracle_mod):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("world", ["blocks", "platens"])
+def test_carried_bounds_equal_recomputed(mpm, prec, world):
+    """The fixed-point scales of a contact step come from bounds over the
+    points' contributions.  Within one call the previous step's G2P forms
+    the deformable points' bounds from the state it writes; the first step
+    of every call forms them in the kernel-density pass from the stored
+    state.  Both must give the same bits: 25 steps in one call equal 25
+    calls of one step, field by field."""
+    if world == "blocks":
+        sc = make_golden.contact_blocks()
+    else:
+        sc = mpm.scene.small_world(material="j2", speed=2.0)
+    # a power-of-two step below both worlds' limits: 25 of them add up
+    # exactly, so the one call and the 25 calls take the same 25 steps
+    sc.dt_max = 2.0 ** -26
+    p = mpm.FP64 if prec == "fp64" else mpm.FP32
+    out = []
+    for calls in (1, 25):
+        with mpm.PhysicsEngine(sc, precision=p) as eng:
+            for _ in range(calls):
+                eng.run_steps(25 // calls, path=mpm.PATH_CONTACT)
+            out.append(eng.getMaterialPoints())
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
 def _sliding_blocks():
     """The closing blocks with a tangential relative velocity (A +y, B -y)."""
     sc = make_golden.contact_blocks()
