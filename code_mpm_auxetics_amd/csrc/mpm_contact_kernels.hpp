@@ -1071,14 +1071,19 @@ __global__ void __launch_bounds__(256) k_c_grid(CArgs<real> a, int step)
         double mm[2], v[2][3], f[2][3], mg[2][3];
         for (int fl = 0; fl < 2; ++fl) {
             mm[fl] = (double)a.acc[off_m(fl) * a.nn + I] * um;
-            // field velocity: momentum over mass where the mass passes the gate
+            // field velocity: momentum over mass where the mass passes the
+            // gate (elsewhere the momentum sums are not read); the mass
+            // gradient only where the contact correction reads it
             const bool gate = c_lenm3(mm[fl]) > g.m_min;
             const double rm = (kCFast<real> && gate) ? 1.0 / mm[fl] : 0.0;
             for (int q = 0; q < 3; ++q) {
-                const double pq = (double)a.acc[off_mv(fl, q) * a.nn + I] * umv;
-                v[fl][q] = !gate ? 0.0 : kCFast<real> ? pq * rm : pq / mm[fl];
                 f[fl][q] = (double)a.acc[off_f(fl, q) * a.nn + I] * uf;
-                mg[fl][q] = (double)a.acc[off_mg(fl, q) * a.nn + I] * umg;
+                v[fl][q] = 0.0;
+                if (gate) {
+                    const double pq = (double)a.acc[off_mv(fl, q) * a.nn + I] * umv;
+                    v[fl][q] = kCFast<real> ? pq * rm : pq / mm[fl];
+                }
+                mg[fl][q] = (flI & CF_CNEG) ? (double)a.acc[off_mg(fl, q) * a.nn + I] * umg : 0.0;
             }
         }
         const uint8_t fx = a.fixed[I];
