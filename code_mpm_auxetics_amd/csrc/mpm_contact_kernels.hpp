@@ -3,8 +3,9 @@
 //
 // One launch per phase over points or grid nodes.  Every node sum is a 64-bit
 // fixed-point integer add (global atomics), with per-field scales fixed for
-// the step by bounds taken over the points first, so the result is
-// independent of scheduling.  Order-dependent choices of the reference's
+// the step by bounds over the points' contributions, formed before the step's
+// first scatter (by the previous step's G2P, or by the kernel-density pass in
+// the first step of a call), so the result is independent of scheduling.  Order-dependent choices of the reference's
 // sequential loops are restated as order-free rules:
 //   nearest point of a node   = smallest (distance, point index), starting
 //                               from point 0 (…_MP_Contact.cpp:228-233)
@@ -473,7 +474,8 @@ __device__ __forceinline__ void c_bounds_flush(const CArgs<real> &a, const doubl
 }
 
 // ---- C2 + C3: active nodes, kernel on the grid (…_MP_Contact.cpp:94-153),
-// and the bounds of the step's P2G contributions.  The block's kernel sums
+// and the bounds of the platen points' P2G contributions (of every point's
+// in the first step of a call).  The block's kernel sums
 // and node flags go through an LDS window of the nodes i0 - 1 .. i0 + 2 of
 // its points' cells.  Per axis only the node planes within R of the point
 // (a slightly inclusive prefilter) are visited, in ascending order as the
@@ -488,7 +490,8 @@ __global__ void __launch_bounds__(256) k_c_kernel(CArgs<real> a)
     __shared__ double s_b[4][4];
     const GridParams &g = a.g;
     const CRange r = c_range(a);
-    // first loop: the bounds of the P2G contributions, the box of cells
+    // first loop: the bounds of the P2G contributions where formed here, the
+    // box of cells
     double b[4] = {0.0, 0.0, 0.0, 0.0};
     int lo[3], hi[3];
     bool any = false;
